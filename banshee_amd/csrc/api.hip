@@ -1,0 +1,356 @@
+// C ABI: context, device memory helpers, scene construction and World::Commit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <thread>
+
+#include "internal.h"
+
+namespace bsh {
+
+static thread_local std::string tl_error;
+
+void set_error(const std::string& msg) { tl_error = msg; }
+
+bsh_status fail(bsh_status code, const std::string& msg)
+{
+    tl_error = msg;
+    return code;
+}
+
+bsh_status hip_fail(hipError_t e, const char* what)
+{
+    tl_error = std::string(what) + ": " + hipGetErrorString(e);
+    return BSH_ERR_HIP;
+}
+
+CameraDev make_camera(const bsh_camera& c)
+{
+    // PerscpectiveCamera ctor, perspective_camera.cpp:5-23
+    f3 eye = mk(c.eye[0], c.eye[1], c.eye[2]);
+    f3 at = mk(c.at[0], c.at[1], c.at[2]);
+    f3 up = mk(c.up[0], c.up[1], c.up[2]);
+    f3 fwd = normalize(at - eye);
+    f3 right = cross(fwd, normalize(up));
+    f3 up2 = cross(right, fwd);
+    CameraDev d;
+    d.eye[0] = eye.x; d.eye[1] = eye.y; d.eye[2] = eye.z;
+    d.fwd[0] = fwd.x; d.fwd[1] = fwd.y; d.fwd[2] = fwd.z;
+    d.right[0] = right.x; d.right[1] = right.y; d.right[2] = right.z;
+    d.up[0] = up2.x; d.up[1] = up2.y; d.up[2] = up2.z;
+    d.znear = c.znear;
+    d.zfar = c.zfar;
+    d.dimx = 2.f * tanf(c.fovy * 0.5f) * c.znear;
+    d.dimy = d.dimx / c.aspect;
+    return d;
+}
+
+}  // namespace bsh
+
+using namespace bsh;
+
+extern "C" {
+
+const char* bsh_last_error(void) { return tl_error.c_str(); }
+
+bsh_status bsh_context_create(int device, bsh_context** out)
+{
+    if (!out) return fail(BSH_ERR_INVALID, "bsh_context_create: out is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= device || device < 0)
+        return fail(BSH_ERR_NO_DEVICE, "bsh_context_create: no HIP device " + std::to_string(device));
+    BSH_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    BSH_HIP(hipGetDeviceProperties(&prop, device));
+    std::string arch = prop.gcnArchName;
+    if (arch.rfind("gfx950", 0) != 0)
+        return fail(BSH_ERR_NO_DEVICE, "bsh_context_create: device is " + arch + ", this build targets gfx950");
+    bsh_context* c = new bsh_context();
+    c->device = device;
+    c->num_cus = prop.multiProcessorCount;
+    c->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
+    *out = c;
+    return BSH_OK;
+}
+
+bsh_status bsh_context_destroy(bsh_context* ctx)
+{
+    delete ctx;
+    return BSH_OK;
+}
+
+bsh_status bsh_device_alloc(bsh_context* ctx, size_t bytes, void** dptr)
+{
+    if (!ctx || !dptr) return fail(BSH_ERR_INVALID, "bsh_device_alloc: NULL argument");
+    BSH_HIP(hipSetDevice(ctx->device));
+    BSH_HIP(hipMalloc(dptr, std::max<size_t>(bytes, 1)));
+    return BSH_OK;
+}
+
+bsh_status bsh_device_free(bsh_context* ctx, void* dptr)
+{
+    if (!ctx) return fail(BSH_ERR_INVALID, "bsh_device_free: NULL context");
+    if (dptr) BSH_HIP(hipFree(dptr));
+    return BSH_OK;
+}
+
+bsh_status bsh_copy_to_device(bsh_context* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx) return fail(BSH_ERR_INVALID, "bsh_copy_to_device: NULL context");
+    if (bytes) BSH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return BSH_OK;
+}
+
+bsh_status bsh_copy_to_host(bsh_context* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx) return fail(BSH_ERR_INVALID, "bsh_copy_to_host: NULL context");
+    if (bytes) BSH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return BSH_OK;
+}
+
+bsh_status bsh_device_sync(bsh_context* ctx)
+{
+    if (!ctx) return fail(BSH_ERR_INVALID, "bsh_device_sync: NULL context");
+    BSH_HIP(hipDeviceSynchronize());
+    return BSH_OK;
+}
+
+bsh_status bsh_scene_create(bsh_context* ctx, const float* verts, size_t nverts, const float* normals,
+                            size_t nnormals, const int32_t* tri_vidx, const int32_t* tri_nidx,
+                            const int32_t* tri_material, size_t ntris, const bsh_bundle* bundles, size_t nbundles,
+                            bsh_scene** out)
+{
+    if (!ctx || !out) return fail(BSH_ERR_INVALID, "bsh_scene_create: NULL context or out");
+    if ((nverts && !verts) || (nnormals && !normals) || (ntris && (!tri_vidx || !tri_nidx || !tri_material)) ||
+        (nbundles && !bundles))
+        return fail(BSH_ERR_INVALID, "bsh_scene_create: NULL array with nonzero count");
+    if (ntris >= (size_t)1 << 29) return fail(BSH_ERR_UNSUPPORTED, "bsh_scene_create: more than 2^29 triangles");
+    for (size_t i = 0; i < 3 * ntris; ++i)
+    {
+        if (tri_vidx[i] < 0 || (size_t)tri_vidx[i] >= nverts)
+            return fail(BSH_ERR_INVALID, "bsh_scene_create: vertex index out of range at triangle " +
+                                             std::to_string(i / 3));
+        if (tri_nidx[i] < 0 || (size_t)tri_nidx[i] >= nnormals)
+            return fail(BSH_ERR_INVALID, "bsh_scene_create: normal index out of range at triangle " +
+                                             std::to_string(i / 3));
+    }
+    size_t covered = 0;
+    for (size_t b = 0; b < nbundles; ++b)
+    {
+        const bsh_bundle& bb = bundles[b];
+        if (bb.tri_begin != (int32_t)covered || bb.tri_count < 0)
+            return fail(BSH_ERR_INVALID, "bsh_scene_create: bundles must tile the triangle range in order");
+        covered += (size_t)bb.tri_count;
+    }
+    if (covered != ntris) return fail(BSH_ERR_INVALID, "bsh_scene_create: bundles do not cover all triangles");
+    for (size_t i = 0; i < 3 * nverts; ++i)
+        if (!std::isfinite(verts[i])) return fail(BSH_ERR_INVALID, "bsh_scene_create: non-finite vertex");
+    bsh_scene* s = new bsh_scene();
+    s->ctx = ctx;
+    s->verts.assign(verts, verts + 3 * nverts);
+    s->normals.assign(normals, normals + 3 * nnormals);
+    s->tri_v.assign(tri_vidx, tri_vidx + 3 * ntris);
+    s->tri_n.assign(tri_nidx, tri_nidx + 3 * ntris);
+    s->tri_m.assign(tri_material, tri_material + ntris);
+    s->bundles.assign(bundles, bundles + nbundles);
+    *out = s;
+    return BSH_OK;
+}
+
+bsh_status bsh_scene_set_materials(bsh_scene* s, const bsh_material* mats, size_t n)
+{
+    if (!s || (n && !mats)) return fail(BSH_ERR_INVALID, "bsh_scene_set_materials: NULL argument");
+    for (size_t i = 0; i < n; ++i)
+    {
+        int t = (int)mats[i].type;
+        if (t < 0 || t > 2 || (float)t != mats[i].type)
+            return fail(BSH_ERR_INVALID, "bsh_scene_set_materials: unknown material type at " + std::to_string(i));
+    }
+    s->mats.assign(mats, mats + n);
+    s->committed = false;
+    return BSH_OK;
+}
+
+bsh_status bsh_scene_set_lights(bsh_scene* s, const bsh_area_light* lights, size_t n)
+{
+    if (!s || (n && !lights)) return fail(BSH_ERR_INVALID, "bsh_scene_set_lights: NULL argument");
+    s->lights.assign(lights, lights + n);
+    s->committed = false;
+    return BSH_OK;
+}
+
+bsh_status bsh_scene_set_camera(bsh_scene* s, const bsh_camera* cam)
+{
+    if (!s || !cam) return fail(BSH_ERR_INVALID, "bsh_scene_set_camera: NULL argument");
+    s->cam = *cam;
+    s->has_cam = true;
+    return BSH_OK;
+}
+
+bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
+{
+    if (!s) return fail(BSH_ERR_INVALID, "bsh_scene_commit: NULL scene");
+    BSH_HIP(hipSetDevice(s->ctx->device));
+    const size_t nt = s->tri_m.size();
+    // materials referenced by triangles must exist
+    for (size_t i = 0; i < nt; ++i)
+        if (s->tri_m[i] < 0 || (size_t)s->tri_m[i] >= s->mats.size())
+            return fail(BSH_ERR_INVALID, "bsh_scene_commit: material index out of range at triangle " +
+                                             std::to_string(i));
+    // lights: validate exactly as AreaLight's ctor does (arealight.cpp:16-19)
+    std::vector<int32_t> bundle_light(s->bundles.size(), -1);
+    s->h_lights.clear();
+    for (size_t l = 0; l < s->lights.size(); ++l)
+    {
+        const bsh_area_light& L = s->lights[l];
+        if (L.bundle < 0 || (size_t)L.bundle >= s->bundles.size() || L.shape < 0 ||
+            L.shape >= s->bundles[L.bundle].tri_count)
+            return fail(BSH_ERR_INVALID, "bsh_scene_commit: light " + std::to_string(l) + " out of range");
+        int tri = s->bundles[L.bundle].tri_begin + L.shape;
+        const bsh_material& m = s->mats[s->tri_m[tri]];
+        if ((int)m.type != BSH_EMISSIVE)
+            return fail(BSH_ERR_INVALID, "AreaLight: The material passed is not an emissive one");
+        bundle_light[L.bundle] = (int32_t)l;  // bundle_.arealight_ = this (last one wins)
+        const float* V = s->verts.data();
+        const int32_t* tv = &s->tri_v[3 * tri];
+        f3 p1 = mk(V[3 * tv[0]], V[3 * tv[0] + 1], V[3 * tv[0] + 2]);
+        f3 p2 = mk(V[3 * tv[1]], V[3 * tv[1] + 1], V[3 * tv[1] + 2]);
+        f3 p3 = mk(V[3 * tv[2]], V[3 * tv[2] + 1], V[3 * tv[2] + 2]);
+        LightDev d;
+        d.tri = tri;
+        d.area = sqrtf(fabsf(sqnorm(cross(p3 - p1, p3 - p2)))) * 0.5f;  // GetShapeSurfaceArea, mesh.cpp:320
+        d.pdf = 1.f / d.area;                                              // FillSample, mesh.cpp:121
+        d.pad = 0.f;
+        d.e = make_float4(m.color[0], m.color[1], m.color[2], 0.f);
+        s->h_lights.push_back(d);
+    }
+    // BVH
+    std::vector<float> tv(9 * nt);
+    for (size_t i = 0; i < nt; ++i)
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) tv[9 * i + 3 * k + c] = s->verts[3 * s->tri_v[3 * i + k] + c];
+    BuildParams bp;
+    if (params)
+    {
+        if (params->max_leaf > 0) bp.max_leaf = params->max_leaf;
+        if (params->sah_bins > 0) bp.bins = params->sah_bins;
+        bp.threads = params->threads;
+    }
+    BuildOutput bo;
+    build_bvh(tv.data(), nt, bp, bo);
+    std::vector<float4> tris(3 * std::max<size_t>(nt, 1));
+    for (size_t k = 0; k < nt; ++k)
+    {
+        int prim = bo.tri_order[k];
+        const float* p = &tv[9 * prim];
+        f3 p1 = mk(p[0], p[1], p[2]), p2 = mk(p[3], p[4], p[5]), p3 = mk(p[6], p[7], p[8]);
+        f3 e1 = p2 - p1, e2 = p3 - p1;   // mesh.cpp:13-14
+        float pb;
+        std::memcpy(&pb, &prim, 4);
+        tris[3 * k + 0] = make_float4(p1.x, p1.y, p1.z, pb);
+        tris[3 * k + 1] = make_float4(e1.x, e1.y, e1.z, 0.f);
+        tris[3 * k + 2] = make_float4(e2.x, e2.y, e2.z, 0.f);
+    }
+    std::vector<int4> shade(2 * std::max<size_t>(nt, 1));
+    {
+        size_t b = 0;
+        for (size_t i = 0; i < nt; ++i)
+        {
+            while (b + 1 < s->bundles.size() && (int32_t)i >= s->bundles[b].tri_begin + s->bundles[b].tri_count) ++b;
+            shade[2 * i] = make_int4(s->tri_v[3 * i], s->tri_v[3 * i + 1], s->tri_v[3 * i + 2], s->tri_m[i]);
+            shade[2 * i + 1] = make_int4(s->tri_n[3 * i], s->tri_n[3 * i + 1], s->tri_n[3 * i + 2],
+                                         s->bundles.empty() ? -1 : bundle_light[b]);
+        }
+    }
+    auto pack3 = [](const std::vector<float>& v) {
+        std::vector<float4> o(std::max<size_t>(v.size() / 3, 1));
+        for (size_t i = 0; i < v.size() / 3; ++i) o[i] = make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.f);
+        return o;
+    };
+    std::vector<float4> verts4 = pack3(s->verts), norms4 = pack3(s->normals);
+    std::vector<float4> mats4(2 * std::max<size_t>(s->mats.size(), 1));
+    for (size_t i = 0; i < s->mats.size(); ++i)
+    {
+        const bsh_material& m = s->mats[i];
+        mats4[2 * i] = make_float4(m.type, m.color[0], m.color[1], m.color[2]);
+        mats4[2 * i + 1] = make_float4(m.eta, m.alpha, 0.f, 0.f);
+    }
+    BSH_HIP(s->d_nodes.alloc(bo.nodes.size()));
+    BSH_HIP(hipMemcpy(s->d_nodes.p, bo.nodes.data(), bo.nodes.size() * sizeof(GpuNode), hipMemcpyHostToDevice));
+    BSH_HIP(s->d_tris.alloc(tris.size()));
+    BSH_HIP(hipMemcpy(s->d_tris.p, tris.data(), tris.size() * sizeof(float4), hipMemcpyHostToDevice));
+    BSH_HIP(s->d_shade.alloc(shade.size()));
+    BSH_HIP(hipMemcpy(s->d_shade.p, shade.data(), shade.size() * sizeof(int4), hipMemcpyHostToDevice));
+    BSH_HIP(s->d_verts.alloc(verts4.size()));
+    BSH_HIP(hipMemcpy(s->d_verts.p, verts4.data(), verts4.size() * sizeof(float4), hipMemcpyHostToDevice));
+    BSH_HIP(s->d_normals.alloc(norms4.size()));
+    BSH_HIP(hipMemcpy(s->d_normals.p, norms4.data(), norms4.size() * sizeof(float4), hipMemcpyHostToDevice));
+    BSH_HIP(s->d_mats.alloc(mats4.size()));
+    BSH_HIP(hipMemcpy(s->d_mats.p, mats4.data(), mats4.size() * sizeof(float4), hipMemcpyHostToDevice));
+    BSH_HIP(s->d_lights.alloc(std::max<size_t>(s->h_lights.size(), 1)));
+    if (!s->h_lights.empty())
+        BSH_HIP(hipMemcpy(s->d_lights.p, s->h_lights.data(), s->h_lights.size() * sizeof(LightDev),
+                          hipMemcpyHostToDevice));
+    s->nnodes = bo.nodes.size();
+    s->max_depth = bo.max_depth;
+    s->build_ms = bo.build_ms;
+    s->committed = true;
+    return BSH_OK;
+}
+
+bsh_status bsh_scene_destroy(bsh_scene* s)
+{
+    if (s)
+    {
+        (void)hipSetDevice(s->ctx->device);
+        delete s;
+    }
+    return BSH_OK;
+}
+
+bsh_status bsh_scene_info(const bsh_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* bytes, double* build_ms,
+                          int32_t* depth)
+{
+    if (!s) return fail(BSH_ERR_INVALID, "bsh_scene_info: NULL scene");
+    if (nodes) *nodes = s->nnodes;
+    if (tris) *tris = s->tri_m.size();
+    if (bytes)
+        *bytes = s->nnodes * sizeof(GpuNode) + s->tri_m.size() * (48 + 32) + (s->verts.size() + s->normals.size()) / 3 * 16;
+    if (build_ms) *build_ms = s->build_ms;
+    if (depth) *depth = s->max_depth;
+    return BSH_OK;
+}
+
+bsh_status bsh_intersect_closest(bsh_scene* s, const bsh_ray* rays, size_t n, bsh_hit* hits, void* stream)
+{
+    return launch_intersect(s, rays, n, 0, hits, nullptr, nullptr, (hipStream_t)stream);
+}
+
+bsh_status bsh_intersect_any(bsh_scene* s, const bsh_ray* rays, size_t n, int32_t* occluded, void* stream)
+{
+    return launch_intersect(s, rays, n, 1, nullptr, occluded, nullptr, (hipStream_t)stream);
+}
+
+bsh_status bsh_intersect_count(bsh_scene* s, const bsh_ray* rays, size_t n, int any, bsh_hit* hits, uint32_t* visits,
+                               void* stream)
+{
+    return launch_intersect(s, rays, n, any ? 1 : 0, hits, nullptr, visits, (hipStream_t)stream);
+}
+
+bsh_status bsh_render(bsh_scene* s, const bsh_render_params* p, float* accum, float* samples, void* stream)
+{
+    return render_frame(s, p, accum, samples, (hipStream_t)stream);
+}
+
+bsh_status bsh_render_stats_get(const bsh_scene* s, bsh_render_stats* out)
+{
+    if (!s || !out) return fail(BSH_ERR_INVALID, "bsh_render_stats_get: NULL argument");
+    *out = s->stats;
+    return BSH_OK;
+}
+
+}  // extern "C"

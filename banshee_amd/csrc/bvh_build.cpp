@@ -1,0 +1,341 @@
+// Binned-SAH BVH2 builder for the GPU traversal layout (see bvh_build.h).
+#include "bvh_build.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <future>
+#include <limits>
+#include <thread>
+
+namespace bsh {
+namespace {
+
+struct Aabb
+{
+    float lo[3], hi[3];
+    void reset()
+    {
+        for (int k = 0; k < 3; ++k)
+        {
+            lo[k] = std::numeric_limits<float>::max();
+            hi[k] = -std::numeric_limits<float>::max();
+        }
+    }
+    void grow(const Aabb& b)
+    {
+        for (int k = 0; k < 3; ++k)
+        {
+            lo[k] = std::min(lo[k], b.lo[k]);
+            hi[k] = std::max(hi[k], b.hi[k]);
+        }
+    }
+    void grow(const float* p)
+    {
+        for (int k = 0; k < 3; ++k)
+        {
+            lo[k] = std::min(lo[k], p[k]);
+            hi[k] = std::max(hi[k], p[k]);
+        }
+    }
+    float area() const
+    {
+        float e0 = hi[0] - lo[0], e1 = hi[1] - lo[1], e2 = hi[2] - lo[2];
+        if (e0 < 0.f || e1 < 0.f || e2 < 0.f) return 0.f;
+        return 2.f * (e0 * e1 + e0 * e2 + e1 * e2);
+    }
+};
+
+struct TNode
+{
+    Aabb box;
+    int left = -1, right = -1;
+    int first = 0, count = 0;
+};
+
+int ceil_log2(int n)
+{
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+class Builder
+{
+public:
+    Builder(const float* tv, size_t n, const BuildParams& p) : tv_(tv), n_(n), p_(p)
+    {
+        if (p_.max_leaf < 1) p_.max_leaf = 1;
+        if (p_.max_leaf > 8) p_.max_leaf = 8;
+        if (p_.bins < 4) p_.bins = 4;
+        if (p_.bins > 64) p_.bins = 64;
+        threads_ = p_.threads > 0 ? p_.threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    }
+
+    void run(BuildOutput& out)
+    {
+        tb_.resize(n_);
+        cen_.resize(3 * n_);
+        idx_.resize(n_);
+        Aabb root, croot;
+        root.reset();
+        croot.reset();
+        for (size_t i = 0; i < n_; ++i)
+        {
+            Aabb b;
+            b.reset();
+            b.grow(tv_ + 9 * i);
+            b.grow(tv_ + 9 * i + 3);
+            b.grow(tv_ + 9 * i + 6);
+            tb_[i] = b;
+            for (int k = 0; k < 3; ++k) cen_[3 * i + k] = 0.5f * (b.lo[k] + b.hi[k]);
+            idx_[i] = (int)i;
+            root.grow(b);
+            croot.grow(&cen_[3 * i]);
+        }
+        nodes_.resize(std::max<size_t>(1, 2 * n_));
+        ncount_ = 0;
+        int r = -1;
+        if (n_ > 0) r = build(0, (int)n_, root, croot, 0);
+        flatten(r, root, out);
+    }
+
+private:
+    int alloc() { return ncount_.fetch_add(1); }
+
+    void bounds(int first, int count, Aabb& box, Aabb& cbox) const
+    {
+        box.reset();
+        cbox.reset();
+        for (int i = first; i < first + count; ++i)
+        {
+            box.grow(tb_[idx_[i]]);
+            cbox.grow(&cen_[3 * idx_[i]]);
+        }
+    }
+
+    int build(int first, int count, const Aabb& box, const Aabb& cbox, int depth)
+    {
+        int id = alloc();
+        TNode& nd = nodes_[id];
+        nd.box = box;
+        nd.first = first;
+        nd.count = count;
+        if (count == 1) return id;
+
+        const float ci = 1.0f, ct = 1.0f;
+        float ext[3] = {cbox.hi[0] - cbox.lo[0], cbox.hi[1] - cbox.lo[1], cbox.hi[2] - cbox.lo[2]};
+        bool degenerate = !(ext[0] > 0.f) && !(ext[1] > 0.f) && !(ext[2] > 0.f);
+        bool forced = depth + ceil_log2(count) >= kMaxStack - 2;
+
+        int mid = -1;
+        if (!degenerate && !forced)
+        {
+            const int B = p_.bins;
+            struct Bin { Aabb b; int n; };
+            Bin bins[3][64];
+            for (int a = 0; a < 3; ++a)
+                for (int i = 0; i < B; ++i) { bins[a][i].b.reset(); bins[a][i].n = 0; }
+            float scale[3];
+            for (int a = 0; a < 3; ++a) scale[a] = ext[a] > 0.f ? (float)B / ext[a] : 0.f;
+            for (int i = first; i < first + count; ++i)
+            {
+                int t = idx_[i];
+                for (int a = 0; a < 3; ++a)
+                {
+                    if (!(ext[a] > 0.f)) continue;
+                    int bi = (int)((cen_[3 * t + a] - cbox.lo[a]) * scale[a]);
+                    bi = std::min(std::max(bi, 0), B - 1);
+                    bins[a][bi].b.grow(tb_[t]);
+                    bins[a][bi].n++;
+                }
+            }
+            float best = std::numeric_limits<float>::max();
+            int best_axis = -1, best_bin = -1;
+            float inv_area = 1.f / std::max(box.area(), 1e-30f);
+            for (int a = 0; a < 3; ++a)
+            {
+                if (!(ext[a] > 0.f)) continue;
+                float rarea[64];
+                int rcnt[64];
+                Aabb acc;
+                acc.reset();
+                int c = 0;
+                for (int i = B - 1; i > 0; --i)
+                {
+                    acc.grow(bins[a][i].b);
+                    c += bins[a][i].n;
+                    rarea[i] = acc.area();
+                    rcnt[i] = c;
+                }
+                acc.reset();
+                c = 0;
+                for (int i = 0; i < B - 1; ++i)
+                {
+                    acc.grow(bins[a][i].b);
+                    c += bins[a][i].n;
+                    if (c == 0 || rcnt[i + 1] == 0) continue;
+                    float cost = ct + ci * (c * acc.area() + rcnt[i + 1] * rarea[i + 1]) * inv_area;
+                    if (cost < best)
+                    {
+                        best = cost;
+                        best_axis = a;
+                        best_bin = i;
+                    }
+                }
+            }
+            float leaf_cost = ci * (float)count;
+            if (count <= p_.max_leaf && (best_axis < 0 || leaf_cost <= best)) return id;
+            if (best_axis >= 0)
+            {
+                const int a = best_axis;
+                const float lo = cbox.lo[a], sc = scale[a];
+                const int bb = best_bin;
+                auto it = std::partition(idx_.begin() + first, idx_.begin() + first + count, [&](int t) {
+                    int bi = (int)((cen_[3 * t + a] - lo) * sc);
+                    bi = std::min(std::max(bi, 0), B - 1);
+                    return bi <= bb;
+                });
+                mid = (int)(it - idx_.begin());
+                if (mid == first || mid == first + count) mid = -1;
+            }
+        }
+        else if (count <= p_.max_leaf)
+        {
+            return id;
+        }
+        if (mid < 0)
+        {
+            // object median on the widest centroid axis (or by index when all centroids coincide)
+            mid = first + count / 2;
+            int a = 0;
+            if (ext[1] > ext[a]) a = 1;
+            if (ext[2] > ext[a]) a = 2;
+            if (!degenerate)
+                std::nth_element(idx_.begin() + first, idx_.begin() + mid, idx_.begin() + first + count,
+                                 [&](int x, int y) { return cen_[3 * x + a] < cen_[3 * y + a]; });
+        }
+        Aabb lb, lcb, rb, rcb;
+        bounds(first, mid - first, lb, lcb);
+        bounds(mid, first + count - mid, rb, rcb);
+        int l, r;
+        if (count > 65536 && active_.load() < threads_)
+        {
+            active_++;
+            auto fut = std::async(std::launch::async, [&]() { return build(first, mid - first, lb, lcb, depth + 1); });
+            r = build(mid, first + count - mid, rb, rcb, depth + 1);
+            l = fut.get();
+            active_--;
+        }
+        else
+        {
+            l = build(first, mid - first, lb, lcb, depth + 1);
+            r = build(mid, first + count - mid, rb, rcb, depth + 1);
+        }
+        nodes_[id].left = l;
+        nodes_[id].right = r;
+        return id;
+    }
+
+    static void pad(const Aabb& b, float* lo, float* hi)
+    {
+        // Outward rounding by a relative 2^-20 keeps the slab test conservative for
+        // triangles lying in a box face (axis-aligned walls).
+        for (int k = 0; k < 3; ++k)
+        {
+            float m = std::max(std::fabs(b.lo[k]), std::fabs(b.hi[k]));
+            float e = m * 9.5367431640625e-7f + 1e-30f;
+            lo[k] = b.lo[k] - e;
+            hi[k] = b.hi[k] + e;
+        }
+    }
+
+    struct ChildRef { int ref; int count; Aabb box; };
+
+    void flatten(int root, const Aabb& rootbox, BuildOutput& out)
+    {
+        out.nodes.clear();
+        out.tri_order.assign(idx_.begin(), idx_.end());
+        out.max_depth = 0;
+        ChildRef empty;
+        empty.ref = 0;
+        empty.count = -1;
+        empty.box.reset();
+        if (root < 0 || nodes_[root].left < 0)
+        {
+            // root is a leaf (or the scene is empty): wrap it in one internal node
+            out.nodes.resize(1);
+            ChildRef c0 = empty;
+            if (root >= 0) { c0.ref = nodes_[root].first; c0.count = nodes_[root].count; c0.box = rootbox; }
+            write(out.nodes[0], c0, empty);
+            return;
+        }
+        out.nodes.reserve(ncount_.load());
+        emit(root, 0, out);
+    }
+
+    int emit(int t, int depth, BuildOutput& out)
+    {
+        out.max_depth = std::max(out.max_depth, depth);
+        int me = (int)out.nodes.size();
+        out.nodes.emplace_back();
+        const TNode& nd = nodes_[t];
+        ChildRef c[2];
+        int kids[2] = {nd.left, nd.right};
+        for (int k = 0; k < 2; ++k)
+        {
+            const TNode& ch = nodes_[kids[k]];
+            c[k].box = ch.box;
+            if (ch.left < 0)
+            {
+                c[k].ref = ch.first;
+                c[k].count = ch.count;
+            }
+            else
+            {
+                c[k].ref = emit(kids[k], depth + 1, out);
+                c[k].count = 0;
+            }
+        }
+        write(out.nodes[me], c[0], c[1]);
+        return me;
+    }
+
+    static void write(GpuNode& g, const ChildRef& a, const ChildRef& b)
+    {
+        float alo[3], ahi[3], blo[3], bhi[3];
+        if (a.count < 0) { for (int k = 0; k < 3; ++k) { alo[k] = 1.f; ahi[k] = -1.f; } }
+        else pad(a.box, alo, ahi);
+        if (b.count < 0) { for (int k = 0; k < 3; ++k) { blo[k] = 1.f; bhi[k] = -1.f; } }
+        else pad(b.box, blo, bhi);
+        g.n0[0] = alo[0]; g.n0[1] = ahi[0]; g.n0[2] = alo[1]; g.n0[3] = ahi[1];
+        g.n1[0] = blo[0]; g.n1[1] = bhi[0]; g.n1[2] = blo[1]; g.n1[3] = bhi[1];
+        g.n2[0] = alo[2]; g.n2[1] = ahi[2]; g.n2[2] = blo[2]; g.n2[3] = bhi[2];
+        g.n3[0] = a.ref; g.n3[1] = b.ref; g.n3[2] = a.count; g.n3[3] = b.count;
+    }
+
+    const float* tv_;
+    size_t n_;
+    BuildParams p_;
+    int threads_;
+    std::vector<Aabb> tb_;
+    std::vector<float> cen_;
+    std::vector<int> idx_;
+    std::vector<TNode> nodes_;
+    std::atomic<int> ncount_{0};
+    std::atomic<int> active_{1};
+};
+
+}  // namespace
+
+void build_bvh(const float* tri_verts, size_t ntris, const BuildParams& p, BuildOutput& out)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    Builder b(tri_verts, ntris, p);
+    b.run(out);
+    out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace bsh

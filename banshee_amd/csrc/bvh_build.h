@@ -1,0 +1,54 @@
+// Host-side BVH construction for the GPU layout.
+//
+// Replaces Bvh::Build/BuildImpl/BuildNode/FindSahSplit (accelerator/bvh.cpp:16-355):
+// a binned-SAH build over triangle bounds, but with multi-triangle leaves, near-first
+// child ordering decided at traversal time, and a hard depth cap so the traversal
+// stack has a fixed size. The tree differs from the reference's; closest-hit results
+// do not depend on the tree except for exact ties in t (which the reference resolves
+// by test order, mesh.cpp:33).
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+#include <vector>
+
+namespace bsh {
+
+// GPU node: 4 x 16 B.
+//   n0 = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
+//   n1 = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
+//   n2 = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
+//   n3 = (c0.ref, c1.ref, c0.count, c1.count) as int32
+// count == 0: ref is an internal node index; count > 0: ref is the first triangle
+// of a leaf of `count` triangles; count < 0: empty child (never entered).
+struct GpuNode
+{
+    float n0[4], n1[4], n2[4];
+    int32_t n3[4];
+};
+static_assert(sizeof(GpuNode) == 64, "node is 64 B");
+
+// Stack entries the traversal kernel reserves per lane. The builder caps the depth of
+// internal nodes at kMaxStack - 2 so that the near-first traversal, which pushes at
+// most one far child per internal ancestor, never needs more.
+constexpr int kMaxStack = 32;
+
+struct BuildParams
+{
+    int max_leaf = 4;
+    int bins = 16;
+    int threads = 0;
+};
+
+struct BuildOutput
+{
+    std::vector<GpuNode> nodes;
+    std::vector<int32_t> tri_order;   // leaf order -> global primitive index
+    int max_depth = 0;
+    double build_ms = 0.0;
+};
+
+// tris: per triangle the three vertices, 9 floats, global primitive order.
+void build_bvh(const float* tri_verts, size_t ntris, const BuildParams& p, BuildOutput& out);
+
+}  // namespace bsh

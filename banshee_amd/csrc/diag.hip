@@ -1,0 +1,176 @@
+// Diagnostic entry points: run single device routines (BSDFs, area lights, camera,
+// keyed stream) over host arrays so tests can pin them against the reference one
+// routine at a time. Not used by the render path.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "internal.h"
+#include "shading.h"
+
+namespace bsh {
+
+__global__ void k_diag_bsdf(float4 m0, float4 m1, uint32_t n, const float* frames, const float* view, const float* u,
+                            const float* ea, const float* eb, float* out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Mat m;
+    m.type = (int)m0.x;
+    m.c = mk(m0.y, m0.z, m0.w);
+    m.eta = m1.x;
+    m.alpha = m1.y;
+    const float* f = frames + 12 * i;
+    Frame h;
+    h.p = mk(0.f, 0.f, 0.f);
+    h.n = mk(f[0], f[1], f[2]);
+    h.ng = mk(f[3], f[4], f[5]);
+    h.dpdu = mk(f[6], f[7], f[8]);
+    h.dpdv = mk(f[9], f[10], f[11]);
+    f3 wi = mk(view[3 * i], view[3 * i + 1], view[3 * i + 2]);
+    f3 wo;
+    float pdf = 0.f;
+    int type = 0;
+    f3 fs = mat_sample(m, h, u[2 * i], u[2 * i + 1], wi, wo, pdf, type);
+    f3 a = mk(ea[3 * i], ea[3 * i + 1], ea[3 * i + 2]), b = mk(eb[3 * i], eb[3 * i + 1], eb[3 * i + 2]);
+    f3 fe = mat_eval(m, h, a, b);
+    float pe = mat_pdf(m, h, a, b);
+    float* o = out + 12 * i;
+    o[0] = fs.x; o[1] = fs.y; o[2] = fs.z; o[3] = pdf;
+    o[4] = wo.x; o[5] = wo.y; o[6] = wo.z; o[7] = (float)type;
+    o[8] = fe.x; o[9] = fe.y; o[10] = fe.z; o[11] = pe;
+}
+
+__global__ void k_diag_light(SceneDev s, int light, uint32_t n, const float* p, const float* u, const float* w,
+                             float* out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const LightDev L = s.lights[light];
+    f3 pp = mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+    f3 d;
+    float pdf = 0.f;
+    f3 le = light_sample(s, L, pp, u[2 * i], u[2 * i + 1], d, pdf);
+    float gp = light_pdf(s, L, pp, mk(w[3 * i], w[3 * i + 1], w[3 * i + 2]));
+    float* o = out + 8 * i;
+    o[0] = le.x; o[1] = le.y; o[2] = le.z; o[3] = pdf;
+    o[4] = d.x; o[5] = d.y; o[6] = d.z; o[7] = gp;
+}
+
+__global__ void k_diag_camera(CameraDev c, uint32_t n, const float* smp, float* rays)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float hx = smp[2 * i] - 0.5f, hy = smp[2 * i + 1] - 0.5f;
+    float csx = hx * c.dimx, csy = hy * c.dimy;
+    f3 d = normalize(c.znear * mk(c.fwd[0], c.fwd[1], c.fwd[2]) + csx * mk(c.right[0], c.right[1], c.right[2]) +
+                     csy * mk(c.up[0], c.up[1], c.up[2]));
+    float* o = rays + 8 * i;
+    o[0] = c.eye[0]; o[1] = c.eye[1]; o[2] = c.eye[2]; o[3] = c.znear;
+    o[4] = d.x; o[5] = d.y; o[6] = d.z; o[7] = c.zfar;
+}
+
+__global__ void k_diag_keyed(uint32_t seed, uint32_t n, const uint32_t* cs, const uint32_t* stream, const uint32_t* ctr,
+                             uint32_t* out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = ks_u32(ks_base(seed, cs[i]), stream[i], ctr[i]);
+}
+
+template <class T>
+static bsh_status up(DevBuf<T>& d, const T* h, size_t n)
+{
+    BSH_HIP(d.alloc(n));
+    BSH_HIP(hipMemcpy(d.p, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return BSH_OK;
+}
+
+#define BSH_TRY(x)                        \
+    do                                    \
+    {                                     \
+        bsh_status r_ = (x);              \
+        if (r_ != BSH_OK) return r_;      \
+    } while (0)
+
+}  // namespace bsh
+
+using namespace bsh;
+
+extern "C" {
+
+bsh_status bsh_diag_bsdf(bsh_context* ctx, const bsh_material* mat, size_t n, const float* frames, const float* view,
+                         const float* u, const float* ea, const float* eb, float* out)
+{
+    if (!ctx || !mat) return fail(BSH_ERR_INVALID, "bsh_diag_bsdf: NULL argument");
+    if (n == 0) return BSH_OK;
+    BSH_HIP(hipSetDevice(ctx->device));
+    DevBuf<float> df, dv, du, da, db, dout;
+    BSH_TRY(up(df, frames, 12 * n));
+    BSH_TRY(up(dv, view, 3 * n));
+    BSH_TRY(up(du, u, 2 * n));
+    BSH_TRY(up(da, ea, 3 * n));
+    BSH_TRY(up(db, eb, 3 * n));
+    BSH_HIP(dout.alloc(12 * n));
+    float4 m0 = make_float4(mat->type, mat->color[0], mat->color[1], mat->color[2]);
+    float4 m1 = make_float4(mat->eta, mat->alpha, 0.f, 0.f);
+    hipLaunchKernelGGL(k_diag_bsdf, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, 0, m0, m1, (uint32_t)n, df.p, dv.p,
+                       du.p, da.p, db.p, dout.p);
+    BSH_HIP(hipGetLastError());
+    BSH_HIP(hipMemcpy(out, dout.p, 12 * n * sizeof(float), hipMemcpyDeviceToHost));
+    return BSH_OK;
+}
+
+bsh_status bsh_diag_light(bsh_scene* s, int light, size_t n, const float* p, const float* u, const float* w, float* out)
+{
+    if (!s) return fail(BSH_ERR_INVALID, "bsh_diag_light: NULL scene");
+    if (!s->committed) return fail(BSH_ERR_STATE, "bsh_diag_light: scene not committed");
+    if (light < 0 || (size_t)light >= s->lights.size()) return fail(BSH_ERR_INVALID, "bsh_diag_light: bad light");
+    if (n == 0) return BSH_OK;
+    BSH_HIP(hipSetDevice(s->ctx->device));
+    DevBuf<float> dp, du, dw, dout;
+    BSH_TRY(up(dp, p, 3 * n));
+    BSH_TRY(up(du, u, 2 * n));
+    BSH_TRY(up(dw, w, 3 * n));
+    BSH_HIP(dout.alloc(8 * n));
+    hipLaunchKernelGGL(k_diag_light, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, 0, s->dev(), light, (uint32_t)n,
+                       dp.p, du.p, dw.p, dout.p);
+    BSH_HIP(hipGetLastError());
+    BSH_HIP(hipMemcpy(out, dout.p, 8 * n * sizeof(float), hipMemcpyDeviceToHost));
+    return BSH_OK;
+}
+
+bsh_status bsh_diag_camera(bsh_context* ctx, const bsh_camera* cam, size_t n, const float* samples, float* rays)
+{
+    if (!ctx || !cam) return fail(BSH_ERR_INVALID, "bsh_diag_camera: NULL argument");
+    if (n == 0) return BSH_OK;
+    BSH_HIP(hipSetDevice(ctx->device));
+    DevBuf<float> ds, dr;
+    BSH_TRY(up(ds, samples, 2 * n));
+    BSH_HIP(dr.alloc(8 * n));
+    hipLaunchKernelGGL(k_diag_camera, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, 0, make_camera(*cam),
+                       (uint32_t)n, ds.p, dr.p);
+    BSH_HIP(hipGetLastError());
+    BSH_HIP(hipMemcpy(rays, dr.p, 8 * n * sizeof(float), hipMemcpyDeviceToHost));
+    return BSH_OK;
+}
+
+bsh_status bsh_diag_keyed(bsh_context* ctx, uint32_t seed, size_t n, const uint32_t* camsample, const uint32_t* stream,
+                          const uint32_t* counter, uint32_t* out)
+{
+    if (!ctx) return fail(BSH_ERR_INVALID, "bsh_diag_keyed: NULL context");
+    if (n == 0) return BSH_OK;
+    BSH_HIP(hipSetDevice(ctx->device));
+    DevBuf<uint32_t> a, b, c, o;
+    BSH_TRY(up(a, camsample, n));
+    BSH_TRY(up(b, stream, n));
+    BSH_TRY(up(c, counter, n));
+    BSH_HIP(o.alloc(n));
+    hipLaunchKernelGGL(k_diag_keyed, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, 0, seed, (uint32_t)n, a.p, b.p,
+                       c.p, o.p);
+    BSH_HIP(hipGetLastError());
+    BSH_HIP(hipMemcpy(out, o.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return BSH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+// Host-side objects behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/banshee_hip.h"
+#include "bsh_math.h"
+#include "bvh_build.h"
+#include "scene_dev.h"
+
+struct bsh_context
+{
+    int device = 0;
+    int num_cus = 0;
+    size_t lds_per_cu = 0;
+};
+
+namespace bsh {
+
+void set_error(const std::string& msg);
+bsh_status fail(bsh_status code, const std::string& msg);
+bsh_status hip_fail(hipError_t e, const char* what);
+
+#define BSH_HIP(call)                                        \
+    do                                                       \
+    {                                                        \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return bsh::hip_fail(e_, #call); \
+    } while (0)
+
+template <class T>
+struct DevBuf
+{
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count)
+    {
+        if (count <= n && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e != hipSuccess) { p = nullptr; n = 0; return e; }
+        n = count;
+        return hipSuccess;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+// Camera frame precomputed on the host exactly as the PerscpectiveCamera ctor does
+// (camera/perspective_camera.cpp:5-23).
+struct CameraDev
+{
+    float eye[3], fwd[3], right[3], up[3];
+    float znear, zfar, dimx, dimy;
+};
+CameraDev make_camera(const bsh_camera& c);
+
+// Wavefront render buffers, sized for `paths` in-flight camera samples.
+struct RenderWork
+{
+    size_t paths = 0;
+    int slots = 0;   // NEE slots per path = 2 * lights at bounce 0
+    DevBuf<float4> ray_o, ray_d, thr, thr_nee, rad, out;
+    DevBuf<int4> hit;
+    DevBuf<uint4> meta;
+    DevBuf<uint32_t> nee_mask;
+    DevBuf<float4> nee_a, nee_b, nee_wi;
+    DevBuf<int4> nee_res;
+    DevBuf<float4> qc_o[2], qc_d[2], qa_o[2], qa_d[2];   // closest / any-hit queues
+    DevBuf<int> qc_own[2], qa_own[2];
+    DevBuf<uint32_t> counters;     // per launch: count, work, (pad) x 4
+    DevBuf<unsigned long long> stats;  // closest, any
+};
+
+}  // namespace bsh
+
+struct bsh_scene
+{
+    bsh_context* ctx = nullptr;
+    std::vector<float> verts, normals;
+    std::vector<int32_t> tri_v, tri_n, tri_m;
+    std::vector<bsh_bundle> bundles;
+    std::vector<bsh_material> mats;
+    std::vector<bsh_area_light> lights;
+    bsh_camera cam{};
+    bool has_cam = false;
+    bool committed = false;
+
+    bsh::DevBuf<bsh::GpuNode> d_nodes;
+    bsh::DevBuf<float4> d_tris, d_verts, d_normals, d_mats;
+    bsh::DevBuf<int4> d_shade;
+    bsh::DevBuf<bsh::LightDev> d_lights;
+    std::vector<bsh::LightDev> h_lights;
+    uint64_t nnodes = 0;
+    int max_depth = 0;
+    double build_ms = 0.0;
+
+    bsh::RenderWork work;
+    bsh_render_stats stats{};
+
+    bsh::SceneDev dev() const
+    {
+        bsh::SceneDev s;
+        s.nodes = reinterpret_cast<const float4*>(d_nodes.p);
+        s.tris = d_tris.p;
+        s.shade = d_shade.p;
+        s.verts = d_verts.p;
+        s.normals = d_normals.p;
+        s.mats = d_mats.p;
+        s.lights = d_lights.p;
+        s.nlights = (int)lights.size();
+        s.ntris = (int)tri_v.size() / 3;
+        return s;
+    }
+};
+
+namespace bsh {
+bsh_status launch_intersect(bsh_scene* s, const bsh_ray* rays, size_t n, int any, bsh_hit* hits, int32_t* occ,
+                            uint32_t* visits, hipStream_t st);
+bsh_status render_frame(bsh_scene* s, const bsh_render_params* p, float* accum, float* samples, hipStream_t st);
+}  // namespace bsh

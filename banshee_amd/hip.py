@@ -1,0 +1,340 @@
+"""ctypes binding of libbanshee_hip.so (the C ABI in include/banshee_hip.h).
+
+The library is the only compute path: if it is missing or no gfx950 device is
+present, every call raises; nothing falls back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbanshee_hip.so")
+
+c_void_p = ctypes.c_void_p
+c_int = ctypes.c_int
+c_size_t = ctypes.c_size_t
+c_uint32 = ctypes.c_uint32
+
+
+class BshError(RuntimeError):
+    pass
+
+
+class Ray(ctypes.Structure):
+    _fields_ = [("ox", ctypes.c_float), ("oy", ctypes.c_float), ("oz", ctypes.c_float), ("tmin", ctypes.c_float),
+                ("dx", ctypes.c_float), ("dy", ctypes.c_float), ("dz", ctypes.c_float), ("tmax", ctypes.c_float)]
+
+
+class Bundle(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("tri_begin", "tri_count", "vert_begin", "vert_count", "norm_begin",
+                                              "norm_count")]
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_float), ("color", ctypes.c_float * 3), ("eta", ctypes.c_float),
+                ("alpha", ctypes.c_float), ("pad", ctypes.c_float * 2)]
+
+
+class AreaLight(ctypes.Structure):
+    _fields_ = [("bundle", ctypes.c_int32), ("shape", ctypes.c_int32)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("eye", ctypes.c_float * 3), ("at", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
+                ("znear", ctypes.c_float), ("zfar", ctypes.c_float), ("fovy", ctypes.c_float),
+                ("aspect", ctypes.c_float)]
+
+
+class BvhParams(ctypes.Structure):
+    _fields_ = [("max_leaf", ctypes.c_int32), ("sah_bins", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("crop_x", ctypes.c_int32),
+                ("crop_y", ctypes.c_int32), ("crop_w", ctypes.c_int32), ("crop_h", ctypes.c_int32),
+                ("img_grid", ctypes.c_int32), ("spp_begin", ctypes.c_int32), ("spp_count", ctypes.c_int32),
+                ("max_depth", ctypes.c_int32), ("seed", ctypes.c_uint32), ("chunk_spp", ctypes.c_int32),
+                ("bg", ctypes.c_float * 3), ("flags", ctypes.c_int32)]
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [("rays_closest", ctypes.c_uint64), ("rays_any", ctypes.c_uint64),
+                ("camera_samples", ctypes.c_uint64), ("node_visits", ctypes.c_uint64),
+                ("tri_tests", ctypes.c_uint64), ("trace_ms", ctypes.c_double), ("shade_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("trace_launches", ctypes.c_uint32),
+                ("shade_launches", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+
+
+# Every symbol include/banshee_hip.h declares.
+EXPORTS = [
+    "bsh_context_create", "bsh_context_destroy", "bsh_last_error", "bsh_device_alloc", "bsh_device_free",
+    "bsh_copy_to_device", "bsh_copy_to_host", "bsh_device_sync", "bsh_scene_create", "bsh_scene_set_materials",
+    "bsh_scene_set_lights", "bsh_scene_set_camera", "bsh_scene_commit", "bsh_scene_destroy", "bsh_scene_info",
+    "bsh_intersect_closest", "bsh_intersect_any", "bsh_intersect_count", "bsh_render", "bsh_render_stats_get",
+    "bsh_diag_bsdf", "bsh_diag_light", "bsh_diag_camera", "bsh_diag_keyed",
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BshError(f"{LIB_PATH} is not built; run `python -m banshee_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.bsh_last_error.restype = ctypes.c_char_p
+        for name in EXPORTS:
+            f = getattr(L, name)
+            if name != "bsh_last_error":
+                f.restype = c_int
+        _lib = L
+    return _lib
+
+
+def _check(code: int, what: str):
+    if code != 0:
+        msg = lib().bsh_last_error().decode()
+        raise BshError(f"{what} failed ({code}): {msg}")
+
+
+def _p(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+class DeviceArray:
+    """A device allocation owned by a Context, with numpy-shaped host copies."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (shape,)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = c_void_p()
+        _check(lib().bsh_device_alloc(ctx.h, c_size_t(self.nbytes), ctypes.byref(self.ptr)), "bsh_device_alloc")
+
+    @classmethod
+    def from_host(cls, ctx, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.nbytes == self.nbytes
+        _check(lib().bsh_copy_to_device(self.ctx.h, self.ptr, _p(a), c_size_t(self.nbytes)), "bsh_copy_to_device")
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        _check(lib().bsh_copy_to_host(self.ctx.h, _p(out), self.ptr, c_size_t(self.nbytes)), "bsh_copy_to_host")
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            lib().bsh_device_free(self.ctx.h, self.ptr)
+            self.ptr = c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = c_void_p()
+        _check(lib().bsh_context_create(c_int(device), ctypes.byref(self.h)), "bsh_context_create")
+
+    def sync(self):
+        _check(lib().bsh_device_sync(self.h), "bsh_device_sync")
+
+    def alloc(self, shape, dtype) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a) -> DeviceArray:
+        return DeviceArray.from_host(self, a)
+
+    # ---- diagnostics ----
+    def diag_bsdf(self, mat_row, frames, view, u, ea, eb):
+        m = Material()
+        m.type = float(mat_row[0])
+        for k in range(3):
+            m.color[k] = float(mat_row[1 + k])
+        m.eta = float(mat_row[4])
+        m.alpha = float(mat_row[5])
+        arrs = [np.ascontiguousarray(x, np.float32) for x in (frames, view, u, ea, eb)]
+        n = arrs[0].shape[0]
+        out = np.zeros((n, 12), np.float32)
+        _check(lib().bsh_diag_bsdf(self.h, ctypes.byref(m), c_size_t(n), *[_p(a) for a in arrs], _p(out)),
+               "bsh_diag_bsdf")
+        return out
+
+    def diag_camera(self, cam13, samples):
+        c = camera_struct(cam13)
+        s = np.ascontiguousarray(samples, np.float32)
+        out = np.zeros((s.shape[0], 8), np.float32)
+        _check(lib().bsh_diag_camera(self.h, ctypes.byref(c), c_size_t(s.shape[0]), _p(s), _p(out)),
+               "bsh_diag_camera")
+        return out
+
+    def diag_keyed(self, seed, camsample, stream, counter):
+        a = np.ascontiguousarray(camsample, np.uint32)
+        b = np.ascontiguousarray(stream, np.uint32)
+        c = np.ascontiguousarray(counter, np.uint32)
+        out = np.zeros(a.shape[0], np.uint32)
+        _check(lib().bsh_diag_keyed(self.h, c_uint32(seed), c_size_t(a.shape[0]), _p(a), _p(b), _p(c), _p(out)),
+               "bsh_diag_keyed")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().bsh_context_destroy(self.h)
+            self.h = c_void_p()
+
+
+def camera_struct(cam13) -> Camera:
+    c = Camera()
+    v = [float(x) for x in cam13]
+    for k in range(3):
+        c.eye[k], c.at[k], c.up[k] = v[k], v[3 + k], v[6 + k]
+    c.znear, c.zfar, c.fovy, c.aspect = v[9], v[10], v[11], v[12]
+    return c
+
+
+class Scene:
+    """A committed device scene (World after Commit) built from a banshee_amd.scenes.Scene."""
+
+    def __init__(self, ctx: Context, sc, max_leaf=4, sah_bins=16, threads=0):
+        self.ctx = ctx
+        self.sc = sc
+        self.h = c_void_p()
+        v = np.ascontiguousarray(sc.verts, np.float32)
+        n = np.ascontiguousarray(sc.normals, np.float32)
+        tv = np.ascontiguousarray(sc.tri_v, np.int32)
+        tn = np.ascontiguousarray(sc.tri_n, np.int32)
+        tm = np.ascontiguousarray(sc.tri_m, np.int32)
+        b = np.ascontiguousarray(sc.bundles, np.int32)
+        _check(lib().bsh_scene_create(ctx.h, _p(v), c_size_t(v.shape[0]), _p(n), c_size_t(n.shape[0]), _p(tv), _p(tn),
+                                      _p(tm), c_size_t(tm.shape[0]), _p(b), c_size_t(b.shape[0]),
+                                      ctypes.byref(self.h)), "bsh_scene_create")
+        mats = np.zeros((sc.materials.shape[0], 8), np.float32)
+        mats[:, :6] = sc.materials[:, :6]
+        _check(lib().bsh_scene_set_materials(self.h, _p(mats), c_size_t(mats.shape[0])), "bsh_scene_set_materials")
+        lights = np.ascontiguousarray(sc.lights, np.int32).reshape(-1, 2)
+        _check(lib().bsh_scene_set_lights(self.h, _p(lights), c_size_t(lights.shape[0])), "bsh_scene_set_lights")
+        self.set_camera(sc.camera)
+        bp = BvhParams(max_leaf, sah_bins, threads, 0)
+        _check(lib().bsh_scene_commit(self.h, ctypes.byref(bp)), "bsh_scene_commit")
+
+    def set_camera(self, cam13):
+        c = camera_struct(cam13)
+        _check(lib().bsh_scene_set_camera(self.h, ctypes.byref(c)), "bsh_scene_set_camera")
+
+    def info(self):
+        nodes, tris, nbytes = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        ms, depth = ctypes.c_double(), ctypes.c_int32()
+        _check(lib().bsh_scene_info(self.h, ctypes.byref(nodes), ctypes.byref(tris), ctypes.byref(nbytes),
+                                    ctypes.byref(ms), ctypes.byref(depth)), "bsh_scene_info")
+        return {"nodes": nodes.value, "tris": tris.value, "bytes": nbytes.value, "build_ms": ms.value,
+                "depth": depth.value}
+
+    # ---- queries on device buffers ----
+    def intersect_closest_dev(self, rays: DeviceArray, n: int, hits: DeviceArray, stream=None):
+        _check(lib().bsh_intersect_closest(self.h, rays.ptr, c_size_t(n), hits.ptr, c_void_p(stream)),
+               "bsh_intersect_closest")
+
+    def intersect_any_dev(self, rays: DeviceArray, n: int, occ: DeviceArray, stream=None):
+        _check(lib().bsh_intersect_any(self.h, rays.ptr, c_size_t(n), occ.ptr, c_void_p(stream)), "bsh_intersect_any")
+
+    # ---- host conveniences ----
+    def intersect(self, rays: np.ndarray, any_hit=False):
+        """rays: n x 8 float32. Returns (prim int32[n], thit float32[n,3]) like the oracles."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        if n == 0:
+            return np.zeros(0, np.int32), np.zeros((0, 3), np.float32)
+        dr = self.ctx.to_device(rays)
+        if any_hit:
+            occ = self.ctx.alloc(n, np.int32)
+            self.intersect_any_dev(dr, n, occ)
+            self.ctx.sync()
+            return occ.numpy(), np.zeros((n, 3), np.float32)
+        hits = self.ctx.alloc((n, 4), np.float32)
+        self.intersect_closest_dev(dr, n, hits)
+        self.ctx.sync()
+        h = hits.numpy()
+        prim = h[:, 0].view(np.int32).copy()
+        return prim, np.ascontiguousarray(h[:, 1:4])
+
+    def intersect_count(self, rays: np.ndarray, any_hit=False):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        dr = self.ctx.to_device(rays)
+        hits = self.ctx.alloc((n, 4), np.float32)
+        vis = self.ctx.alloc((n, 2), np.uint32)
+        _check(lib().bsh_intersect_count(self.h, dr.ptr, c_size_t(n), c_int(1 if any_hit else 0), hits.ptr, vis.ptr,
+                                         c_void_p(None)), "bsh_intersect_count")
+        self.ctx.sync()
+        h = hits.numpy()
+        return h[:, 0].view(np.int32).copy(), h[:, 1:4].copy(), vis.numpy()
+
+    def render(self, width, height, img_grid, max_depth, seed, crop=None, spp_range=None, chunk_spp=0,
+               bg=(0.0, 0.0, 0.0), per_sample=False, accum: DeviceArray | None = None, stream=None, download=True,
+               count=False):
+        """Returns (sum [h,w,3], sumsq [h,w,3], count [h,w], per_sample [h,w,s,3] or None)."""
+        p = RenderParams()
+        p.width, p.height = width, height
+        if crop is not None:
+            p.crop_x, p.crop_y, p.crop_w, p.crop_h = crop
+        cw = crop[2] if crop else width
+        ch = crop[3] if crop else height
+        p.img_grid = img_grid
+        spp = img_grid * img_grid
+        sb, sc = (spp_range if spp_range else (0, spp))
+        p.spp_begin, p.spp_count = sb, sc
+        p.max_depth = max_depth
+        p.seed = seed
+        p.chunk_spp = chunk_spp
+        for k in range(3):
+            p.bg[k] = bg[k]
+        p.flags = 4 if count else 0
+        own = accum is None
+        if own:
+            accum = self.ctx.alloc((ch, cw, 8), np.float32)
+        samp = self.ctx.alloc((ch, cw, sc, 3), np.float32) if per_sample else None
+        _check(lib().bsh_render(self.h, ctypes.byref(p), accum.ptr, samp.ptr if samp else c_void_p(None),
+                                c_void_p(stream)), "bsh_render")
+        if not download:
+            return None
+        self.ctx.sync()
+        a = accum.numpy()
+        return a[..., 0:3], a[..., 4:7], a[..., 3], (samp.numpy() if samp else None)
+
+    def stats(self) -> dict:
+        s = RenderStats()
+        _check(lib().bsh_render_stats_get(self.h, ctypes.byref(s)), "bsh_render_stats_get")
+        return {k: getattr(s, k) for k, _ in s._fields_ if k != "pad"}
+
+    def diag_light(self, light, p, u, w):
+        p, u, w = (np.ascontiguousarray(x, np.float32) for x in (p, u, w))
+        out = np.zeros((p.shape[0], 8), np.float32)
+        _check(lib().bsh_diag_light(self.h, c_int(light), c_size_t(p.shape[0]), _p(p), _p(u), _p(w), _p(out)),
+               "bsh_diag_light")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().bsh_scene_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,325 @@
+"""Seeded procedural scenes in the flat layout the C-ABI takes.
+
+Every generator uses numpy's PCG64 bit generator (``numpy.random.Generator(
+numpy.random.PCG64(seed))``) so a (generator, seed, size) triple always yields the
+same triangles.  Transforms are baked into vertices, so every bundle has the
+identity transform; a reference ``Mesh`` built from the same arrays therefore sees
+object space == world space (``Banshee/math/mathutils.h:129-157`` with identity
+matrices).
+
+Flat layout (all little-endian, C-contiguous):
+    verts    float32 [V,3]    positions
+    normals  float32 [N,3]    shading normals
+    tri_v    int32   [T,3]    global vertex indices
+    tri_n    int32   [T,3]    global normal indices
+    tri_m    int32   [T]      material index (``Mesh::Face::m``, mesh.h:57-63)
+    bundles  int32   [B,6]    tri_begin, tri_count, vert_begin, vert_count,
+                              norm_begin, norm_count  (one reference Mesh each)
+    materials float32 [M,8]   type, c0, c1, c2, eta, alpha, 0, 0
+                              type 0 Lambert(kd=c)        bsdf/lambert.h:45-135
+                              type 1 Microfacet(eta, ks=c, GGX(alpha))
+                                                          bsdf/microfacet.h:53-77
+                              type 2 Emissive(e=c)        material/emissive.h:45-99
+    lights   int32   [L,2]    (bundle, shape index in bundle): one AreaLight per
+                              emissive triangle, arealight.cpp:11-23
+    camera   float32 [13]     eye3, at3, up3, znear, zfar, fovy, aspect
+                              (PerscpectiveCamera ctor, perspective_camera.cpp:5-23)
+The global primitive index of a triangle is its row in ``tri_v``; bundles are
+concatenated in order, which is the order ``Bvh::Build`` numbers shapes
+(bvh.cpp:26-51).
+"""
+from __future__ import annotations
+
+import hashlib
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+LAMBERT, MICROFACET, EMISSIVE = 0, 1, 2
+
+
+@dataclass
+class Scene:
+    verts: np.ndarray
+    normals: np.ndarray
+    tri_v: np.ndarray
+    tri_n: np.ndarray
+    tri_m: np.ndarray
+    bundles: np.ndarray
+    materials: np.ndarray
+    lights: np.ndarray
+    camera: np.ndarray
+    name: str = ""
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def ntris(self) -> int:
+        return int(self.tri_v.shape[0])
+
+    def digest(self) -> str:
+        h = hashlib.sha256()
+        for a in (self.verts, self.normals, self.tri_v, self.tri_n, self.tri_m, self.bundles,
+                  self.materials, self.lights, self.camera):
+            h.update(np.ascontiguousarray(a).tobytes())
+        return h.hexdigest()[:16]
+
+    def with_camera(self, camera) -> "Scene":
+        s = Scene(**{k: getattr(self, k) for k in self.__dataclass_fields__})
+        s.camera = np.asarray(camera, np.float32)
+        return s
+
+
+class _Builder:
+    def __init__(self):
+        self.v, self.n, self.tv, self.tn, self.tm, self.b = [], [], [], [], [], []
+        self.nv = 0
+        self.nt = 0
+        self.mats = []
+        self.lights = []
+
+    def material(self, type_, c, eta=0.0, alpha=0.0) -> int:
+        self.mats.append([type_, c[0], c[1], c[2], eta, alpha, 0.0, 0.0])
+        return len(self.mats) - 1
+
+    def bundle(self, verts, normals, tris, mat) -> int:
+        verts = np.asarray(verts, np.float32).reshape(-1, 3)
+        normals = np.asarray(normals, np.float32).reshape(-1, 3)
+        assert verts.shape == normals.shape
+        tris = np.asarray(tris, np.int64).reshape(-1, 3)
+        mats = np.broadcast_to(np.asarray(mat, np.int32), (tris.shape[0],))
+        vb = self.nv
+        self.v.append(verts)
+        self.n.append(normals)
+        self.tv.append((tris + vb).astype(np.int32))
+        self.tn.append((tris + vb).astype(np.int32))
+        self.tm.append(mats.astype(np.int32))
+        self.b.append([self.nt, tris.shape[0], vb, verts.shape[0], vb, verts.shape[0]])
+        self.nv += verts.shape[0]
+        self.nt += tris.shape[0]
+        return len(self.b) - 1
+
+    def quad(self, corners, normal, mat) -> int:
+        """corners: 4 points in order around the quad; two triangles (0,1,2), (0,2,3)."""
+        c = np.asarray(corners, np.float32)
+        n = np.tile(np.asarray(normal, np.float32), (4, 1))
+        return self.bundle(c, n, [[0, 1, 2], [0, 2, 3]], mat)
+
+    def area_lights(self, bundle: int):
+        for s in range(self.b[bundle][1]):
+            self.lights.append([bundle, s])
+
+    def build(self, camera, name, meta=None) -> Scene:
+        return Scene(
+            verts=np.ascontiguousarray(np.concatenate(self.v), np.float32),
+            normals=np.ascontiguousarray(np.concatenate(self.n), np.float32),
+            tri_v=np.ascontiguousarray(np.concatenate(self.tv), np.int32),
+            tri_n=np.ascontiguousarray(np.concatenate(self.tn), np.int32),
+            tri_m=np.ascontiguousarray(np.concatenate(self.tm), np.int32),
+            bundles=np.ascontiguousarray(np.asarray(self.b, np.int32).reshape(-1, 6)),
+            materials=np.ascontiguousarray(np.asarray(self.mats, np.float32).reshape(-1, 8)),
+            lights=np.ascontiguousarray(np.asarray(self.lights, np.int32).reshape(-1, 2)),
+            camera=np.asarray(camera, np.float32),
+            name=name,
+            meta=meta or {},
+        )
+
+
+def box_mesh(lo, hi):
+    """Axis-aligned box: 24 vertices (flat faces), 12 triangles, outward normals."""
+    lo = np.asarray(lo, np.float32)
+    hi = np.asarray(hi, np.float32)
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    faces = [
+        ([[x0, y0, z0], [x0, y0, z1], [x0, y1, z1], [x0, y1, z0]], [-1, 0, 0]),
+        ([[x1, y0, z0], [x1, y1, z0], [x1, y1, z1], [x1, y0, z1]], [1, 0, 0]),
+        ([[x0, y0, z0], [x1, y0, z0], [x1, y0, z1], [x0, y0, z1]], [0, -1, 0]),
+        ([[x0, y1, z0], [x0, y1, z1], [x1, y1, z1], [x1, y1, z0]], [0, 1, 0]),
+        ([[x0, y0, z0], [x0, y1, z0], [x1, y1, z0], [x1, y0, z0]], [0, 0, -1]),
+        ([[x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], [0, 0, 1]),
+    ]
+    v, n, t = [], [], []
+    for k, (c, nn) in enumerate(faces):
+        v.extend(c)
+        n.extend([nn] * 4)
+        b = 4 * k
+        t.extend([[b, b + 1, b + 2], [b, b + 2, b + 3]])
+    return np.asarray(v, np.float32), np.asarray(n, np.float32), np.asarray(t, np.int32)
+
+
+def cornell_box(width=256, height=256, seed=0, light_radiance=15.0) -> Scene:
+    """Procedural Cornell-style box: 5 walls + 2 axis-aligned boxes (34 triangles),
+    Lambert albedo U[0.2,0.8]^3 per object, one 0.5x0.5 quad area light of
+    radiance 15 just under the ceiling (2 emissive triangles => 2 AreaLights)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    B = _Builder()
+
+    def albedo():
+        return rng.uniform(0.2, 0.8, 3).astype(np.float32)
+
+    walls = [
+        ([[0, 0, 0], [2, 0, 0], [2, 0, 2], [0, 0, 2]], [0, 1, 0]),      # floor
+        ([[0, 2, 0], [0, 2, 2], [2, 2, 2], [2, 2, 0]], [0, -1, 0]),     # ceiling
+        ([[0, 0, 0], [0, 2, 0], [2, 2, 0], [2, 0, 0]], [0, 0, 1]),      # back
+        ([[0, 0, 0], [0, 0, 2], [0, 2, 2], [0, 2, 0]], [1, 0, 0]),      # left
+        ([[2, 0, 0], [2, 2, 0], [2, 2, 2], [2, 0, 2]], [-1, 0, 0]),     # right
+    ]
+    for c, n in walls:
+        B.quad(c, n, B.material(LAMBERT, albedo()))
+    for lo, hi in (((0.3, 0.0, 0.9), (0.9, 0.6, 1.5)), ((1.1, 0.0, 0.3), (1.7, 1.2, 0.9))):
+        v, n, t = box_mesh(lo, hi)
+        B.bundle(v, n, t, B.material(LAMBERT, albedo()))
+    em = B.material(EMISSIVE, (light_radiance,) * 3)
+    lb = B.quad([[0.75, 1.98, 0.75], [1.25, 1.98, 0.75], [1.25, 1.98, 1.25], [0.75, 1.98, 1.25]],
+                [0, -1, 0], em)
+    B.area_lights(lb)
+    cam = [1, 1, 3.5, 1, 1, 0, 0, 1, 0, 0.01, 10000.0, math.pi / 4, width / height]
+    return B.build(cam, "cornell", {"seed": seed})
+
+
+def icosphere(subdiv: int):
+    """Unit icosphere by repeated midpoint subdivision of an icosahedron."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+         [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+         [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+         [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    verts = [np.asarray(p, np.float64) / np.linalg.norm(p) for p in v]
+    for _ in range(subdiv):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in f:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        f = nf
+    return np.asarray(verts, np.float64), np.asarray(f, np.int32)
+
+
+def _value_noise(rng, x, z, extent, octaves=4, base_cells=8):
+    """Sum of octaves of smoothstep-interpolated lattice noise in [-1, 1]."""
+    h = np.zeros_like(x, np.float64)
+    amp, total = 1.0, 0.0
+    cells = base_cells
+    for _ in range(octaves):
+        lat = rng.uniform(-1.0, 1.0, (cells + 1, cells + 1))
+        u = (x + extent) / (2 * extent) * cells
+        w = (z + extent) / (2 * extent) * cells
+        i0 = np.clip(np.floor(u).astype(np.int64), 0, cells - 1)
+        j0 = np.clip(np.floor(w).astype(np.int64), 0, cells - 1)
+        fu = u - i0
+        fw = w - j0
+        su = fu * fu * (3 - 2 * fu)
+        sw = fw * fw * (3 - 2 * fw)
+        a = lat[i0, j0] * (1 - su) + lat[i0 + 1, j0] * su
+        b = lat[i0, j0 + 1] * (1 - su) + lat[i0 + 1, j0 + 1] * su
+        h += amp * (a * (1 - sw) + b * sw)
+        total += amp
+        amp *= 0.5
+        cells *= 2
+    return h / total
+
+
+def _random_rotations(rng, n):
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - w * z)
+    R[:, 0, 2] = 2 * (x * z + w * y)
+    R[:, 1, 0] = 2 * (x * y + w * z)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - w * x)
+    R[:, 2, 0] = 2 * (x * z - w * y)
+    R[:, 2, 1] = 2 * (y * z + w * x)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def terrain_scene(grid=512, spheres=2000, subdiv=3, seed=1, width=1920, height=1080,
+                  extent=50.0, amplitude=3.0, light_radiance=80.0) -> Scene:
+    """Seeded heightfield (grid x grid quads over [-extent, extent]^2, value-noise
+    displaced, amplitude 3) plus `spheres` random-scaled/rotated icospheres
+    (`subdiv` subdivisions) resting above it. Per sphere: 50% Lambert (albedo
+    U[0.2,0.8]^3), 50% Microfacet(eta=1.5, ks=U[0.2,0.8]^3, GGX alpha=U[0.05,1]).
+    Two 4x4 quad area lights (4 light triangles). Each light triangle has area 8, so its
+    area pdf 1/8 clears the reference's MINPDF cut (0.05, ditracer.cpp:103); a triangle
+    larger than 20 would never be light-sampled."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    B = _Builder()
+    # heightfield
+    g = grid
+    xs = np.linspace(-extent, extent, g + 1)
+    X, Z = np.meshgrid(xs, xs, indexing="ij")
+    H = amplitude * _value_noise(rng, X, Z, extent)
+    step = 2 * extent / g
+    dHx = np.gradient(H, step, axis=0)
+    dHz = np.gradient(H, step, axis=1)
+    N = np.stack([-dHx, np.ones_like(H), -dHz], -1)
+    N /= np.linalg.norm(N, axis=-1, keepdims=True)
+    V = np.stack([X, H, Z], -1).reshape(-1, 3)
+    idx = np.arange((g + 1) * (g + 1)).reshape(g + 1, g + 1)
+    a = idx[:-1, :-1].ravel()
+    b = idx[1:, :-1].ravel()
+    c = idx[1:, 1:].ravel()
+    d = idx[:-1, 1:].ravel()
+    tris = np.empty((2 * g * g, 3), np.int64)
+    tris[0::2] = np.stack([a, d, c], -1)
+    tris[1::2] = np.stack([a, c, b], -1)
+    B.bundle(V, N.reshape(-1, 3), tris, B.material(LAMBERT, rng.uniform(0.2, 0.8, 3)))
+    # icospheres
+    sv, sf = icosphere(subdiv)
+    if spheres > 0:
+        scale = rng.uniform(0.3, 2.0, spheres)
+        R = _random_rotations(rng, spheres)
+        px = rng.uniform(-0.9 * extent, 0.9 * extent, spheres)
+        pz = rng.uniform(-0.9 * extent, 0.9 * extent, spheres)
+        lift = rng.uniform(0.0, 5.0, spheres)
+        is_ggx = rng.uniform(0.0, 1.0, spheres) < 0.5
+        col = rng.uniform(0.2, 0.8, (spheres, 3))
+        alpha = rng.uniform(0.05, 1.0, spheres)
+        # ground height under each sphere from the same bilinear mesh interpolation
+        gi = np.clip(((px + extent) / step).astype(np.int64), 0, g - 1)
+        gj = np.clip(((pz + extent) / step).astype(np.int64), 0, g - 1)
+        ground = np.maximum.reduce([H[gi, gj], H[gi + 1, gj], H[gi, gj + 1], H[gi + 1, gj + 1]])
+        py = ground + scale + lift + 0.05
+        rot_n = np.einsum("nij,vj->nvi", R, sv)
+        pos = rot_n * scale[:, None, None] + np.stack([px, py, pz], -1)[:, None, :]
+        for k in range(spheres):
+            if is_ggx[k]:
+                m = B.material(MICROFACET, col[k], 1.5, alpha[k])
+            else:
+                m = B.material(LAMBERT, col[k])
+            B.bundle(pos[k], rot_n[k], sf, m)
+    # two quad lights facing down
+    em = B.material(EMISSIVE, (light_radiance,) * 3)
+    for cx, cy, cz, s in ((-15.0, 22.0, -10.0, 4.0), (20.0, 20.0, 15.0, 4.0)):
+        h = s / 2
+        lb = B.quad([[cx - h, cy, cz - h], [cx + h, cy, cz - h], [cx + h, cy, cz + h], [cx - h, cy, cz + h]],
+                    [0, -1, 0], em)
+        B.area_lights(lb)
+    cam = [0, 30, 75, 0, 0, 0, 0, 1, 0, 0.01, 10000.0, math.pi / 4, width / height]
+    return B.build(cam, f"terrain{grid}_{spheres}", {"seed": seed, "grid": grid, "spheres": spheres,
+                                                    "subdiv": subdiv})
+
+
+def terrain_for_tris(target_tris: int, seed: int, subdiv=3, width=1920, height=1080) -> Scene:
+    """The terrain generator scaled to about `target_tris` triangles: the heightfield
+    keeps the cfg1 share (1/6 of the triangles), the rest are icospheres."""
+    per_sphere = 20 * 4 ** subdiv
+    hf = max(8, target_tris // 6)
+    grid = max(4, int(round(math.sqrt(hf / 2))))
+    rest = max(0, target_tris - 2 * grid * grid)
+    spheres = max(1, rest // per_sphere)
+    return terrain_scene(grid=grid, spheres=spheres, subdiv=subdiv, seed=seed, width=width, height=height)

@@ -1,0 +1,219 @@
+/*
+ * banshee_hip.h -- C ABI of the MI355X (gfx950) global-illumination hot path.
+ *
+ * Plain C: opaque handles, caller-owned buffers, int status codes, no exceptions
+ * cross this boundary. Device buffers are hipMalloc'd memory (from bsh_device_alloc,
+ * torch, or the caller's own allocator); "stream" is a hipStream_t (NULL = the
+ * legacy default stream).
+ *
+ * Each entry point names the reference interface it replaces (paths relative to
+ * FireRays/Banshee/ of the reference). The reference's plug-in points are C++
+ * virtual interfaces; INTEGRATION.md shows the adapters a maintainer adds on the
+ * reference side (a HipAccelerator : Intersectable and a HipImageRenderer : Renderer).
+ *
+ * Threading: one context per device; a scene is immutable after bsh_scene_commit;
+ * calls on different streams may run concurrently from different threads, except
+ * that bsh_render / bsh_bake_sh / bsh_batch_run on one scene serialise on the
+ * scene's own work buffers.
+ */
+#ifndef BANSHEE_HIP_H
+#define BANSHEE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int bsh_status;
+enum
+{
+    BSH_OK = 0,
+    BSH_ERR_INVALID = 1,     /* bad argument or scene (the reference throws std::runtime_error, arealight.cpp:16-19) */
+    BSH_ERR_HIP = 2,         /* a HIP runtime call failed */
+    BSH_ERR_UNSUPPORTED = 3, /* configuration this build does not implement */
+    BSH_ERR_NO_DEVICE = 4,   /* no usable gfx950 device */
+    BSH_ERR_STATE = 5        /* call order violated (e.g. render before commit) */
+};
+
+typedef struct bsh_context bsh_context;
+typedef struct bsh_scene bsh_scene;
+
+/* Ray, 32 B: the reference `ray` (math/ray.h:42-62) minus its unused id.
+   Valid interval (tmin, tmax]: a hit is accepted iff tmin < t <= tmax (primitive/mesh.cpp:33). */
+typedef struct
+{
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} bsh_ray;
+
+/* Closest hit, 16 B: global primitive index (bundle-concatenation order,
+   accelerator/bvh.cpp:26-51; -1 = miss), distance and the barycentrics (b1, b2) of
+   Mesh::IntersectFace (mesh.cpp:7-45): p = (1-b1-b2) p1 + b1 p2 + b2 p3. */
+typedef struct
+{
+    int32_t prim;
+    float t, b1, b2;
+} bsh_hit;
+
+/* Bundle = one reference Mesh (primitive/mesh.h:48-181): a contiguous triangle range
+   with its own vertex and normal ranges. Only identity transforms are supported:
+   callers bake ShapeBundle::SetTransform into the vertices. */
+typedef struct
+{
+    int32_t tri_begin, tri_count;
+    int32_t vert_begin, vert_count;
+    int32_t norm_begin, norm_count;
+} bsh_bundle;
+
+/* Material row. type 0: SimpleMaterial(Lambert(kd=color)) (bsdf/lambert.h:45-135);
+   type 1: SimpleMaterial(Microfacet(eta, ks=color, FresnelDielectric, GgxDistribution(alpha)))
+   (bsdf/microfacet.h:53-150, microfacet_distribution.h:196-286, fresnel.h:53-91);
+   type 2: Emissive(e=color) (material/emissive.h:45-99). */
+typedef struct
+{
+    float type, color[3], eta, alpha, pad[2];
+} bsh_material;
+
+enum { BSH_LAMBERT = 0, BSH_MICROFACET = 1, BSH_EMISSIVE = 2 };
+
+/* AreaLight(shapeidx, bundle, material) (light/arealight.cpp:11-23). As in the
+   reference, the bundle's light pointer is the LAST AreaLight created on it; the
+   BSDF-sampled MIS branch of DiTracer::GetDi matches hits by that pointer
+   (tracer/ditracer.cpp:202). */
+typedef struct
+{
+    int32_t bundle, shape;
+} bsh_area_light;
+
+/* PerscpectiveCamera(eye, at, up, zcap=(znear, zfar), fovy, aspect)
+   (camera/perspective_camera.cpp:5-23). */
+typedef struct
+{
+    float eye[3], at[3], up[3];
+    float znear, zfar, fovy, aspect;
+} bsh_camera;
+
+typedef struct
+{
+    int32_t max_leaf;   /* triangles per leaf, 1..8 (the reference uses 1, bvh.cpp:74) */
+    int32_t sah_bins;   /* bins per axis for the binned SAH, 4..64 */
+    int32_t threads;    /* host build threads, 0 = hardware concurrency */
+    int32_t reserved;
+} bsh_bvh_params;
+
+/* Frame parameters for bsh_render. The estimator is GiTracer(max_depth)
+   (tracer/gitracer.cpp:13-140) with DiTracer::GetDi NEE (ditracer.cpp:57-239);
+   image sampler StratifiedSampler(img_grid), light and BSDF samplers
+   StratifiedSampler(1) (sampler/stratified_sampler.cpp:7-24) fed by the keyed sample
+   stream. Camera sample g = (y*width + x)*spp + s, spp = img_grid^2. */
+typedef struct
+{
+    int32_t width, height;         /* full image plane (ImagePlane::resolution) */
+    int32_t crop_x, crop_y;        /* rendered window; crop_w = crop_h = 0 means the full frame */
+    int32_t crop_w, crop_h;
+    int32_t img_grid;              /* spp = img_grid^2 */
+    int32_t spp_begin, spp_count;  /* render samples [spp_begin, spp_begin+spp_count) of each pixel; count 0 = all */
+    int32_t max_depth;             /* GiTracer maxdepth_ */
+    uint32_t seed;
+    int32_t chunk_spp;             /* samples per pixel per wavefront chunk, 0 = auto */
+    float bg[3];                   /* World::bgcolor_ */
+    int32_t flags;                 /* BSH_RENDER_* */
+} bsh_render_params;
+
+enum
+{
+    BSH_RENDER_ACCUMULATE = 1,  /* add into accum instead of overwriting it */
+    BSH_RENDER_NO_QUIRKS = 2,   /* quirk-free estimator (no MINPDF cut, no clamp, solid-angle MIS); unsupported in this build */
+    BSH_RENDER_COUNT = 4        /* count BVH node visits and triangle tests (slower; for roofline accounting) */
+};
+
+/* Per-kernel timings and ray counts of the last bsh_render (profiling). */
+typedef struct
+{
+    uint64_t rays_closest, rays_any;   /* BVH queries issued (Intersectable::Intersect x2) */
+    uint64_t camera_samples;
+    uint64_t node_visits, tri_tests;   /* totals over all queries, BSH_RENDER_COUNT only */
+    double trace_ms, shade_ms, total_ms; /* summed HIP-event durations */
+    uint32_t trace_launches, shade_launches;
+    uint32_t pad;
+} bsh_render_stats;
+
+/* ---------------- context ---------------- */
+bsh_status bsh_context_create(int device, bsh_context** out);
+bsh_status bsh_context_destroy(bsh_context* ctx);
+const char* bsh_last_error(void);  /* thread-local message for the last failing call */
+
+/* Device memory helpers (hipMalloc/hipMemcpy wrappers) for hosts without a HIP runtime binding. */
+bsh_status bsh_device_alloc(bsh_context* ctx, size_t bytes, void** dptr);
+bsh_status bsh_device_free(bsh_context* ctx, void* dptr);
+bsh_status bsh_copy_to_device(bsh_context* ctx, void* dst, const void* src, size_t bytes);
+bsh_status bsh_copy_to_host(bsh_context* ctx, void* dst, const void* src, size_t bytes);
+bsh_status bsh_device_sync(bsh_context* ctx);
+
+/* ---------------- scene (World, world/world.h:51-88) ---------------- */
+/* Replaces building World::shapebundles_ from Mesh ctors (mesh.cpp:126-214): the arrays are
+   copied. Host pointers. tri_vidx/tri_nidx are global indices; tri_material indexes the
+   material table. */
+bsh_status bsh_scene_create(bsh_context* ctx, const float* verts, size_t nverts, const float* normals,
+                            size_t nnormals, const int32_t* tri_vidx, const int32_t* tri_nidx,
+                            const int32_t* tri_material, size_t ntris, const bsh_bundle* bundles,
+                            size_t nbundles, bsh_scene** out);
+/* World::materials_ */
+bsh_status bsh_scene_set_materials(bsh_scene* s, const bsh_material* mats, size_t n);
+/* World::lights_ (AreaLight only) */
+bsh_status bsh_scene_set_lights(bsh_scene* s, const bsh_area_light* lights, size_t n);
+/* World::camera_ */
+bsh_status bsh_scene_set_camera(bsh_scene* s, const bsh_camera* cam);
+/* World::Commit (world/world.cpp:5-16): host SAH build, flatten, upload. params may be NULL. */
+bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params);
+bsh_status bsh_scene_destroy(bsh_scene* s);
+/* Node count, triangle count, device bytes, build milliseconds, max depth. */
+bsh_status bsh_scene_info(const bsh_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* bytes, double* build_ms,
+                          int32_t* depth);
+
+/* ---------------- ray queries (Intersectable, accelerator/intersectable.h:41-49) ---------------- */
+/* Intersectable::Intersect(ray const&, Hit&) -> Bvh::Intersect closest hit (bvh.cpp:499-573).
+   Device pointers. */
+bsh_status bsh_intersect_closest(bsh_scene* s, const bsh_ray* rays, size_t n, bsh_hit* hits, void* stream);
+/* Intersectable::Intersect(ray const&) -> Bvh::Intersect any hit (bvh.cpp:575-647).
+   occluded[i] = 1 if any triangle has tmin < t <= tmax. Device pointers. */
+bsh_status bsh_intersect_any(bsh_scene* s, const bsh_ray* rays, size_t n, int32_t* occluded, void* stream);
+/* Counting mode: same queries, also writing per-ray node visits and triangle tests
+   (visits[2*i], visits[2*i+1]); used for the algorithmic-bytes roofline. */
+bsh_status bsh_intersect_count(bsh_scene* s, const bsh_ray* rays, size_t n, int any, bsh_hit* hits,
+                               uint32_t* visits, void* stream);
+
+/* ---------------- frame render (Renderer::Render, renderer/renderer.h:49) ---------------- */
+/* Replaces ImageRenderer/MtImageRenderer::Render(world) (imagerenderer.cpp:9-59,
+   mt_imagerenderer.cpp:16-117) for a GiTracer. accum: device float[crop_w*crop_h*8]:
+   (sum r, g, b, n, sum r^2, g^2, b^2, 0) per pixel, row-major, NOT y-flipped
+   (FileImagePlane flips only when writing a file, fileimageplane.cpp:29-34).
+   samples: optional device float[crop_w*crop_h*spp_count*3] receiving every camera
+   sample's returned radiance (GiTracer::GetLi's return value) in (pixel, s) order. */
+bsh_status bsh_render(bsh_scene* s, const bsh_render_params* p, float* accum, float* samples, void* stream);
+bsh_status bsh_render_stats_get(const bsh_scene* s, bsh_render_stats* out);
+
+/* ---------------- diagnostics (exercise device code paths in isolation) ---------------- */
+/* Device evaluation of Material::Sample / Evaluate / GetPdf for one material row, same
+   argument layout as the probes in tests: frames n x 12 (n, ng, dpdu, dpdv), view n x 3,
+   u n x 2, ea/eb n x 3; out n x 12. Host pointers. */
+bsh_status bsh_diag_bsdf(bsh_context* ctx, const bsh_material* mat, size_t n, const float* frames,
+                         const float* view, const float* u, const float* ea, const float* eb, float* out);
+/* Device evaluation of AreaLight::GetSample / GetPdf for scene light `light`. Host pointers.
+   p n x 3, u n x 2, w n x 3; out n x 8 (le3, pdf, d3, getpdf). */
+bsh_status bsh_diag_light(bsh_scene* s, int light, size_t n, const float* p, const float* u, const float* w,
+                          float* out);
+/* Device camera rays for image-plane samples in [0,1]^2 (PerscpectiveCamera::GenerateRay,
+   perspective_camera.cpp:25-38). Host pointers: samples n x 2, rays n x 8. */
+bsh_status bsh_diag_camera(bsh_context* ctx, const bsh_camera* cam, size_t n, const float* samples, float* rays);
+/* Device keyed sample stream values ks_u32(ks_base(seed, camsample), stream, counter). Host pointers. */
+bsh_status bsh_diag_keyed(bsh_context* ctx, uint32_t seed, size_t n, const uint32_t* camsample,
+                          const uint32_t* stream, const uint32_t* counter, uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BANSHEE_HIP_H */

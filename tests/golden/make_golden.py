@@ -1,0 +1,169 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference
+(oracle/_ref/liboracle_ref.so, built by `make -C oracle ref` from the reference's own
+sources). Run in a checkout where the reference is present:
+
+    python tests/golden/make_golden.py
+
+Every fixture holds inputs and the reference's outputs only, plus the digest of the
+generated scene it was made on, so a drift of the scene generator is detected.
+"""
+from __future__ import annotations
+
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from banshee_amd.scenes import cornell_box, terrain_for_tris  # noqa: E402
+from oracle_lib import Oracle  # noqa: E402
+
+# Fixture configurations (also read by the tests).
+CORNELL = dict(W=256, H=256, grid=4, nee=1, depth=5, seed=1)
+CORNELL_CROP = (120, 60, 16, 16)
+TERRAIN_TRIS = 100_000
+TERRAIN = dict(W=192, H=108, grid=2, nee=1, depth=8, seed=2)
+TERRAIN_CROP = (80, 40, 32, 24)
+
+
+def unit(rng, n):
+    v = rng.normal(size=(n, 3))
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+
+def orthovector(n):
+    p = np.zeros_like(n)
+    big = np.abs(n[:, 2]) > 0.707106781186547524401
+    k = np.sqrt(n[:, 1] ** 2 + n[:, 2] ** 2)
+    p[big, 1] = -n[big, 2] / k[big]
+    p[big, 2] = n[big, 1] / k[big]
+    k2 = np.sqrt(n[:, 0] ** 2 + n[:, 1] ** 2)
+    p[~big, 0] = -n[~big, 1] / k2[~big]
+    p[~big, 1] = n[~big, 0] / k2[~big]
+    return p.astype(np.float32)
+
+
+def bsdf_cases(rng, n):
+    nn = unit(rng, n)
+    ng = nn + 0.3 * unit(rng, n)
+    ng /= np.linalg.norm(ng, axis=1, keepdims=True)
+    flip = rng.uniform(size=n) < 0.3
+    ng[flip] *= -1
+    du = orthovector(nn)
+    dv = np.cross(nn, du).astype(np.float32)
+    frames = np.concatenate([nn, ng, du, dv], 1).astype(np.float32)
+    view = unit(rng, n)
+    # bias half the views into the normal's hemisphere
+    same = rng.uniform(size=n) < 0.5
+    s = np.sign(np.sum(view * nn, 1))
+    view[same] *= s[same, None]
+    u = rng.uniform(0, 1, (n, 2)).astype(np.float32)
+    ea = unit(rng, n)
+    eb = view.copy()
+    return frames, view, u, ea, eb
+
+
+def main():
+    o = Oracle("ref")
+    rng = np.random.Generator(np.random.PCG64(2024))
+    out = {}
+
+    # ---------------- samplers / rng ----------------
+    first_is_x = int(o.lib.oref_sample2d_first_is_x())
+    mc = {s: o.mcrng(s, 64) for s in (1, 7, 12345)}
+    strat = o.stratified(4, 3, 48)
+    np.savez_compressed(os.path.join(HERE, "rng.npz"), first_is_x=first_is_x, mc_seeds=np.array([1, 7, 12345]),
+                        mc=np.stack([mc[1], mc[7], mc[12345]]), strat_grid=4, strat_seed=3, strat=strat)
+
+    # ---------------- camera ----------------
+    cb = cornell_box()
+    tr = terrain_for_tris(TERRAIN_TRIS, seed=1, width=TERRAIN["W"], height=TERRAIN["H"])
+    samples = rng.uniform(0, 1, (256, 2)).astype(np.float32)
+    np.savez_compressed(os.path.join(HERE, "camera.npz"), cams=np.stack([cb.camera, tr.camera]), samples=samples,
+                        rays=np.stack([o.camera_rays(cb.camera, samples), o.camera_rays(tr.camera, samples)]))
+
+    # ---------------- BSDFs ----------------
+    mats = np.array([
+        [0, 0.7, 0.5, 0.3, 0, 0, 0, 0],
+        [1, 0.8, 0.6, 0.4, 1.5, 0.05, 0, 0],
+        [1, 0.5, 0.5, 0.5, 1.5, 0.3, 0, 0],
+        [1, 0.3, 0.7, 0.2, 1.5, 1.0, 0, 0],
+        [2, 15.0, 12.0, 9.0, 0, 0, 0, 0],
+    ], np.float32)
+    n = 512
+    cases = [bsdf_cases(rng, n) for _ in mats]
+    res = [o.bsdf(m, *c) for m, c in zip(mats, cases)]
+    np.savez_compressed(os.path.join(HERE, "bsdf.npz"), mats=mats,
+                        frames=np.stack([c[0] for c in cases]), view=np.stack([c[1] for c in cases]),
+                        u=np.stack([c[2] for c in cases]), ea=np.stack([c[3] for c in cases]),
+                        eb=np.stack([c[4] for c in cases]), out=np.stack(res))
+
+    # ---------------- area lights ----------------
+    sc = o.scene(cb)
+    p = rng.uniform([0.05, 0.05, 0.05], [1.95, 1.9, 1.95], (400, 3)).astype(np.float32)
+    u = rng.uniform(0, 1, (400, 2)).astype(np.float32)
+    tgt = rng.uniform([0.7, 1.98, 0.7], [1.3, 1.98, 1.3], (400, 3)).astype(np.float32)
+    w = tgt - p
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    w[::5] = unit(rng, len(w[::5]))
+    lres = np.stack([sc.light(li, p, u, w) for li in range(cb.lights.shape[0])])
+    np.savez_compressed(os.path.join(HERE, "light.npz"), digest=cb.digest(), p=p, u=u, w=w, out=lres)
+
+    # ---------------- spherical harmonics ----------------
+    dirs = np.concatenate([unit(rng, 1000), np.array([[0, 0, 1], [0, 0, -1], [1, 0, 0], [-1, 0, 0], [0, 1, 0],
+                                                      [0, -1, 0]], np.float32)])
+    ylm4 = o.sh_eval(dirs, 4)
+    cin = rng.normal(size=(9, 3)).astype(np.float32)
+    conv = o.sh_convolve(2, cin)
+    env = rng.uniform(0, 2, (16, 32, 3)).astype(np.float32)
+    proj = o.sh_project(env, 2)
+    const_proj = o.sh_project(np.ones((64, 128, 3), np.float32), 2)
+    nrm = unit(rng, 200)
+    irr = o.sh_irradiance(2, cin, nrm)
+    np.savez_compressed(os.path.join(HERE, "sh.npz"), dirs=dirs, ylm4=ylm4, cin=cin, conv=conv, env=env, proj=proj,
+                        const_proj=const_proj, normals=nrm, irr=irr)
+
+    # ---------------- GI on the Cornell box (cfg0) + recorded rays ----------------
+    C = CORNELL
+    t0 = time.time()
+    sc.record(max_rays=24000, stride=431)
+    S, Q, N, _, rays = sc.render_keyed(C["W"], C["H"], 0, 0, C["W"], C["H"], C["grid"], C["nee"], C["depth"],
+                                       C["seed"])
+    rr, kind, prim, thit = sc.fetch(24000)
+    print(f"cornell full frame {time.time() - t0:.1f}s rays={rays} recorded={len(rr)}")
+    x0, y0, cw, ch = CORNELL_CROP
+    _, _, _, PS, _ = sc.render_keyed(C["W"], C["H"], x0, y0, cw, ch, C["grid"], C["nee"], C["depth"], C["seed"],
+                                     per_sample=True)
+    np.savez_compressed(os.path.join(HERE, "gi_cornell.npz"), digest=cb.digest(), sum=S, sumsq=Q, n=N,
+                        ray_counts=rays, crop=np.array(CORNELL_CROP), crop_samples=PS,
+                        **{k: v for k, v in C.items()})
+    np.savez_compressed(os.path.join(HERE, "rays_cornell.npz"), digest=cb.digest(), rays=rr, kind=kind, prim=prim,
+                        thit=thit)
+
+    # ---------------- GI on a 100k-triangle terrain scene (cfg1 generator) ----------------
+    T = TERRAIN
+    st = o.scene(tr)
+    t0 = time.time()
+    x0, y0, cw, ch = TERRAIN_CROP
+    st.record(max_rays=24000, stride=23)
+    S, Q, N, PS, rays = st.render_keyed(T["W"], T["H"], x0, y0, cw, ch, T["grid"], T["nee"], T["depth"], T["seed"],
+                                        per_sample=True)
+    rr, kind, prim, thit = st.fetch(24000)
+    print(f"terrain crop {time.time() - t0:.1f}s rays={rays} recorded={len(rr)} tris={tr.ntris}")
+    np.savez_compressed(os.path.join(HERE, "gi_terrain.npz"), digest=tr.digest(), sum=S, sumsq=Q, n=N,
+                        ray_counts=rays, crop=np.array(TERRAIN_CROP), crop_samples=PS, tris=tr.ntris,
+                        **{k: v for k, v in T.items()})
+    np.savez_compressed(os.path.join(HERE, "rays_terrain.npz"), digest=tr.digest(), rays=rr, kind=kind, prim=prim,
+                        thit=thit)
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()

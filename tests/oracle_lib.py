@@ -1,0 +1,202 @@
+"""ctypes bindings for the two CPU oracles (test infrastructure only).
+
+* ``oracle/_ref/liboracle_ref.so`` -- the reference's own sources compiled in place
+  plus ``oracle/harness`` (prefix ``oref_``). Built only where the reference is present.
+* ``oracle/_build/libcpuref.so``  -- this repository's CPU restatement
+  (``oracle/cpu_ref``, prefix ``cref_``), pinned against the fixtures the first one wrote.
+
+Both expose the same entry points, so ``Oracle("ref")`` and ``Oracle("cpu")`` are
+interchangeable in tests.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "liboracle_ref.so")
+CPU_SO = os.path.join(ROOT, "oracle", "_build", "libcpuref.so")
+
+_libs = {}
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def available(kind: str) -> bool:
+    return os.path.exists(REF_SO if kind == "ref" else CPU_SO)
+
+
+def _load(kind):
+    if kind not in _libs:
+        path = REF_SO if kind == "ref" else CPU_SO
+        lib = ctypes.CDLL(path)
+        pre = "oref_" if kind == "ref" else "cref_"
+        vp = ctypes.c_void_p
+        i = ctypes.c_int
+        getattr(lib, pre + "scene_create").restype = vp
+        getattr(lib, pre + "scene_create").argtypes = [vp, i, vp, i, vp, vp, vp, i, vp, i, vp, i, vp, i, vp]
+        getattr(lib, pre + "scene_destroy").argtypes = [vp]
+        getattr(lib, pre + "intersect").argtypes = [vp, vp, i, i, vp, vp]
+        getattr(lib, pre + "render_keyed").argtypes = [vp, i, i, i, i, i, i, i, i, i, ctypes.c_uint, vp, vp, vp,
+                                                          vp, vp]
+        getattr(lib, pre + "time_render").restype = ctypes.c_double
+        getattr(lib, pre + "time_render").argtypes = [vp, i, i, i, i, i, ctypes.c_uint, i, vp]
+        getattr(lib, pre + "bsdf").argtypes = [vp, i, vp, vp, vp, vp, vp, vp]
+        getattr(lib, pre + "light").argtypes = [vp, i, i, vp, vp, vp, vp]
+        getattr(lib, pre + "camera_rays").argtypes = [vp, vp, i, vp]
+        getattr(lib, pre + "mcrng").argtypes = [ctypes.c_uint, i, vp]
+        getattr(lib, pre + "stratified").argtypes = [i, ctypes.c_uint, i, vp]
+        getattr(lib, pre + "sh_eval").argtypes = [vp, i, i, vp]
+        getattr(lib, pre + "sh_convolve").argtypes = [i, vp, vp]
+        getattr(lib, pre + "sh_project").argtypes = [vp, i, i, i, vp]
+        getattr(lib, pre + "sh_irradiance").argtypes = [i, vp, vp, i, vp]
+        if kind == "ref":
+            lib.oref_record_begin.argtypes = [vp, i, i]
+            lib.oref_record_fetch.argtypes = [vp, vp, vp, vp, vp]
+            lib.oref_record_fetch.restype = i
+            lib.oref_intersect_fill.argtypes = [vp, vp, i, vp]
+        _libs[kind] = (lib, pre)
+    return _libs[kind]
+
+
+def f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+class Oracle:
+    def __init__(self, kind: str):
+        self.kind = kind
+        self.lib, self.pre = _load(kind)
+
+    def _fn(self, name):
+        return getattr(self.lib, self.pre + name)
+
+    # ---- scene ----
+    def scene(self, sc):
+        return OracleScene(self, sc)
+
+    # ---- stateless probes ----
+    def bsdf(self, mat, frames, view, u, ea, eb):
+        n = frames.shape[0]
+        out = np.zeros((n, 12), np.float32)
+        keep = [f32(mat), f32(frames), f32(view), f32(u), f32(ea), f32(eb)]
+        self._fn("bsdf")(_p(keep[0]), n, *[_p(k) for k in keep[1:]], _p(out))
+        return out
+
+    def camera_rays(self, cam, samples):
+        samples = f32(samples)
+        out = np.zeros((samples.shape[0], 8), np.float32)
+        c = f32(cam)
+        self._fn("camera_rays")(_p(c), _p(samples), samples.shape[0], _p(out))
+        return out
+
+    def mcrng(self, seed, n):
+        out = np.zeros(n, np.uint32)
+        self._fn("mcrng")(seed, n, _p(out))
+        return out
+
+    def stratified(self, grid, seed, n):
+        out = np.zeros((n, 2), np.float32)
+        self._fn("stratified")(grid, seed, n, _p(out))
+        return out
+
+    def sh_eval(self, dirs, lmax):
+        dirs = f32(dirs)
+        out = np.zeros((dirs.shape[0], (lmax + 1) ** 2), np.float32)
+        self._fn("sh_eval")(_p(dirs), dirs.shape[0], lmax, _p(out))
+        return out
+
+    def sh_convolve(self, lmax, cin):
+        cin = f32(cin)
+        out = np.zeros_like(cin)
+        self._fn("sh_convolve")(lmax, _p(cin), _p(out))
+        return out
+
+    def sh_project(self, env, lmax):
+        env = f32(env)
+        H, W = env.shape[:2]
+        out = np.zeros(((lmax + 1) ** 2, 3), np.float32)
+        self._fn("sh_project")(_p(env), W, H, lmax, _p(out))
+        return out
+
+    def sh_irradiance(self, lmax, coeffs, normals):
+        coeffs, normals = f32(coeffs), f32(normals)
+        out = np.zeros((normals.shape[0], 3), np.float32)
+        self._fn("sh_irradiance")(lmax, _p(coeffs), _p(normals), normals.shape[0], _p(out))
+        return out
+
+
+class OracleScene:
+    def __init__(self, orc: Oracle, sc):
+        self.o = orc
+        self.sc = sc
+        self._keep = [f32(sc.verts), f32(sc.normals), i32(sc.tri_v), i32(sc.tri_n), i32(sc.tri_m),
+                      i32(sc.bundles), f32(sc.materials), i32(sc.lights), f32(sc.camera)]
+        k = self._keep
+        self.h = orc._fn("scene_create")(
+            _p(k[0]), k[0].shape[0], _p(k[1]), k[1].shape[0], _p(k[2]), _p(k[3]), _p(k[4]), k[2].shape[0],
+            _p(k[5]), k[5].shape[0], _p(k[6]), k[6].shape[0], _p(k[7]), k[7].shape[0], _p(k[8]))
+        if not self.h:
+            raise RuntimeError("oracle scene creation failed")
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.o._fn("scene_destroy")(self.h)
+        except Exception:
+            pass
+
+    def intersect(self, rays, any_hit=False):
+        rays = f32(rays)
+        n = rays.shape[0]
+        prim = np.zeros(n, np.int32)
+        thit = np.zeros((n, 3), np.float32)
+        self.o._fn("intersect")(self.h, _p(rays), n, 1 if any_hit else 0, _p(prim), _p(thit))
+        return prim, thit
+
+    def intersect_fill(self, rays):
+        rays = f32(rays)
+        out = np.zeros((rays.shape[0], 20), np.float32)
+        self.o.lib.oref_intersect_fill(self.h, _p(rays), rays.shape[0], _p(out))
+        return out
+
+    def render_keyed(self, W, H, x0, y0, tw, th, img_grid, nee_grid, depth, seed, per_sample=False):
+        spp = img_grid * img_grid
+        s = np.zeros((th, tw, 3), np.float32)
+        q = np.zeros((th, tw, 3), np.float32)
+        c = np.zeros((th, tw), np.float32)
+        ps = np.zeros((th, tw, spp, 3), np.float32) if per_sample else None
+        rays = np.zeros(2, np.int64)
+        self.o._fn("render_keyed")(self.h, W, H, x0, y0, tw, th, img_grid, nee_grid, depth, seed, _p(s), _p(q),
+                                   _p(c), _p(ps), _p(rays))
+        return s, q, c, ps, rays
+
+    def time_render(self, W, H, img_grid, nee_grid, depth, seed, mt=True):
+        rays = np.zeros(2, np.int64)
+        t = self.o._fn("time_render")(self.h, W, H, img_grid, nee_grid, depth, seed, 1 if mt else 0, _p(rays))
+        return t, rays
+
+    def light(self, idx, p, u, w):
+        p, u, w = f32(p), f32(u), f32(w)
+        out = np.zeros((p.shape[0], 8), np.float32)
+        self.o._fn("light")(self.h, idx, p.shape[0], _p(p), _p(u), _p(w), _p(out))
+        return out
+
+    def record(self, max_rays, stride):
+        self.o.lib.oref_record_begin(self.h, max_rays, stride)
+
+    def fetch(self, max_rays):
+        rays = np.zeros((max_rays, 8), np.float32)
+        kind = np.zeros(max_rays, np.int32)
+        prim = np.zeros(max_rays, np.int32)
+        thit = np.zeros((max_rays, 3), np.float32)
+        n = self.o.lib.oref_record_fetch(self.h, _p(rays), _p(kind), _p(prim), _p(thit))
+        return rays[:n], kind[:n], prim[:n], thit[:n]
