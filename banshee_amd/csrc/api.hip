@@ -6,6 +6,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <vector>
 
 #include "internal.h"
 
@@ -343,13 +344,131 @@ bsh_status bsh_intersect_count(bsh_scene* s, const bsh_ray* rays, size_t n, int 
 
 bsh_status bsh_render(bsh_scene* s, const bsh_render_params* p, float* accum, float* samples, void* stream)
 {
-    return render_frame(s, p, accum, samples, (hipStream_t)stream);
+    JobDev J;
+    bsh_status r = make_render_job(s, p, accum, samples, J);
+    if (r != BSH_OK) return r;
+    std::vector<JobDev> jobs(1, J);
+    return run_jobs(s->ctx, jobs, (p->flags & BSH_RENDER_COUNT) != 0, nullptr, 0, (hipStream_t)stream);
+}
+
+bsh_status bsh_bake_sh(bsh_scene* s, const float* probes, size_t nprobes, const bsh_probe_params* p, float* coeffs,
+                       float* radiance, void* stream)
+{
+    JobDev J;
+    bsh_status r = make_probe_job(s, probes, nprobes, p, coeffs, radiance, J);
+    if (r != BSH_OK) return r;
+    if (nprobes == 0) return BSH_OK;
+    std::vector<JobDev> jobs(1, J);
+    return run_jobs(s->ctx, jobs, (p->flags & BSH_RENDER_COUNT) != 0, nullptr, 0, (hipStream_t)stream);
 }
 
 bsh_status bsh_render_stats_get(const bsh_scene* s, bsh_render_stats* out)
 {
     if (!s || !out) return fail(BSH_ERR_INVALID, "bsh_render_stats_get: NULL argument");
-    *out = s->stats;
+    *out = s->ctx->stats;
+    return BSH_OK;
+}
+
+bsh_status bsh_context_stats_get(const bsh_context* ctx, bsh_render_stats* out)
+{
+    if (!ctx || !out) return fail(BSH_ERR_INVALID, "bsh_context_stats_get: NULL argument");
+    *out = ctx->stats;
+    return BSH_OK;
+}
+
+bsh_status bsh_context_set_tuning(bsh_context* ctx, const bsh_tuning* t)
+{
+    if (!ctx || !t) return fail(BSH_ERR_INVALID, "bsh_context_set_tuning: NULL argument");
+    if (t->paths_per_chunk < 0 || t->trace_blocks_per_cu < 0 || t->trace_blocks_per_cu > 5)
+        return fail(BSH_ERR_INVALID, "bsh_context_set_tuning: trace_blocks_per_cu must be 0..5 (LDS limit)");
+    ctx->tuning = *t;
+    return BSH_OK;
+}
+
+bsh_status bsh_context_get_tuning(const bsh_context* ctx, bsh_tuning* t)
+{
+    if (!ctx || !t) return fail(BSH_ERR_INVALID, "bsh_context_get_tuning: NULL argument");
+    *t = ctx->tuning;
+    return BSH_OK;
+}
+
+static bsh_status build_jobs(bsh_context* ctx, const bsh_job* jobs, size_t njobs, std::vector<JobDev>& out, bool& count)
+{
+    out.clear();
+    count = false;
+    for (size_t k = 0; k < njobs; ++k)
+    {
+        const bsh_job& j = jobs[k];
+        if (!j.scene || j.scene->ctx != ctx)
+            return fail(BSH_ERR_INVALID, "bsh_batch_run: job " + std::to_string(k) + " has no scene of this context");
+        JobDev J;
+        bsh_status r;
+        if (j.kind == BSH_JOB_RENDER)
+        {
+            r = make_render_job(j.scene, &j.render, j.accum, j.samples, J);
+            count = count || (j.render.flags & BSH_RENDER_COUNT);
+        }
+        else if (j.kind == BSH_JOB_BAKE_SH)
+        {
+            r = make_probe_job(j.scene, j.probes, j.nprobes, &j.probe, j.coeffs, j.radiance, J);
+            if (r == BSH_OK && j.nprobes == 0) continue;
+        }
+        else
+        {
+            return fail(BSH_ERR_INVALID, "bsh_batch_run: unknown job kind");
+        }
+        if (r != BSH_OK) return r;
+        out.push_back(J);
+    }
+    return BSH_OK;
+}
+
+bsh_status bsh_batch_run(bsh_context* ctx, const bsh_job* jobs, size_t njobs, const bsh_tuning* tuning, void* stream)
+{
+    if (!ctx || (njobs && !jobs)) return fail(BSH_ERR_INVALID, "bsh_batch_run: NULL argument");
+    std::vector<JobDev> hj;
+    bool count = false;
+    bsh_status r = build_jobs(ctx, jobs, njobs, hj, count);
+    if (r != BSH_OK) return r;
+    const bsh_tuning& t = tuning ? *tuning : ctx->tuning;
+    return run_jobs(ctx, hj, count, &t, t.use_graph, (hipStream_t)stream);
+}
+
+bsh_status bsh_autotune(bsh_context* ctx, const bsh_job* jobs, size_t njobs, const int32_t* paths_per_chunk,
+                        size_t n_paths, const int32_t* blocks_per_cu, size_t n_blocks, bsh_tuning* best,
+                        double* best_ms)
+{
+    if (!ctx || (njobs && !jobs)) return fail(BSH_ERR_INVALID, "bsh_autotune: NULL argument");
+    static const int32_t kPaths[] = {1 << 19, 1 << 20, 1 << 21, 1 << 22};
+    static const int32_t kBlocks[] = {2, 3, 4, 5};
+    if (!paths_per_chunk || n_paths == 0) { paths_per_chunk = kPaths; n_paths = 4; }
+    if (!blocks_per_cu || n_blocks == 0) { blocks_per_cu = kBlocks; n_blocks = 4; }
+    std::vector<JobDev> hj;
+    bool count = false;
+    bsh_status r = build_jobs(ctx, jobs, njobs, hj, count);
+    if (r != BSH_OK) return r;
+    bsh_tuning bt = ctx->tuning;
+    double bms = 1e300;
+    for (size_t a = 0; a < n_paths; ++a)
+        for (size_t b = 0; b < n_blocks; ++b)
+        {
+            if (blocks_per_cu[b] < 1 || blocks_per_cu[b] > 5 || paths_per_chunk[a] < 1)
+                return fail(BSH_ERR_INVALID, "bsh_autotune: candidate out of range");
+            bsh_tuning t{paths_per_chunk[a], blocks_per_cu[b], 0, 0};
+            std::vector<JobDev> copy = hj;
+            r = run_jobs(ctx, copy, false, &t, 0, nullptr);
+            if (r != BSH_OK) return r;
+            double ms = ctx->stats.total_ms;
+            if (ms < bms)
+            {
+                bms = ms;
+                bt = t;
+            }
+        }
+    bt.use_graph = ctx->tuning.use_graph;
+    ctx->tuning = bt;
+    if (best) *best = bt;
+    if (best_ms) *best_ms = bms;
     return BSH_OK;
 }
 

@@ -6,17 +6,13 @@
 #include <string>
 #include <vector>
 
+#include <mutex>
+
 #include "../../include/banshee_hip.h"
 #include "bsh_math.h"
 #include "bvh_build.h"
 #include "scene_dev.h"
 
-struct bsh_context
-{
-    int device = 0;
-    int num_cus = 0;
-    size_t lds_per_cu = 0;
-};
 
 namespace bsh {
 
@@ -80,7 +76,43 @@ struct RenderWork
     DevBuf<unsigned long long> stats;  // closest, any
 };
 
+
+// One job of a batch: a camera frame (type 0) or a probe set (type 1), with the device
+// scene it runs on. Copied to the device as the batch's job table.
+struct JobDev
+{
+    SceneDev s;
+    CameraDev cam;
+    int type;
+    int W, H, cx, cy, cw, ch, grid, spp, s_begin, s_count;
+    int depth;
+    uint32_t seed;
+    float bg[3];
+    int accumulate;
+    float* accum;
+    float* samples;
+    const float* probes;
+    int nprobes, N, g, lmax;
+    float tmin;
+    float* coeffs;
+    float* radiance;
+};
+
+bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& jobs, bool count, const bsh_tuning* tune, int use_graph,
+                    hipStream_t st);
 }  // namespace bsh
+
+struct bsh_context
+{
+    int device = 0;
+    int num_cus = 0;
+    size_t lds_per_cu = 0;
+    std::mutex mu;                 // serialises work on the shared buffers below
+    bsh::RenderWork work;
+    bsh::DevBuf<bsh::JobDev> jobs;
+    bsh_tuning tuning{1 << 21, 5, 0, 0};
+    bsh_render_stats stats{};
+};
 
 struct bsh_scene
 {
@@ -103,8 +135,6 @@ struct bsh_scene
     int max_depth = 0;
     double build_ms = 0.0;
 
-    bsh::RenderWork work;
-    bsh_render_stats stats{};
 
     bsh::SceneDev dev() const
     {
@@ -125,5 +155,7 @@ struct bsh_scene
 namespace bsh {
 bsh_status launch_intersect(bsh_scene* s, const bsh_ray* rays, size_t n, int any, bsh_hit* hits, int32_t* occ,
                             uint32_t* visits, hipStream_t st);
-bsh_status render_frame(bsh_scene* s, const bsh_render_params* p, float* accum, float* samples, hipStream_t st);
+bsh_status make_render_job(bsh_scene* s, const bsh_render_params* p, float* accum, float* samples, JobDev& J);
+bsh_status make_probe_job(bsh_scene* s, const float* probes, size_t nprobes, const bsh_probe_params* p,
+                          float* coeffs, float* radiance, JobDev& J);
 }  // namespace bsh

@@ -1,15 +1,25 @@
-// Wavefront GI: the per-pixel recursion of ImageRenderer + GiTracer::GetLi +
-// DiTracer::GetDi (imagerenderer.cpp:61-101, gitracer.cpp:13-140, ditracer.cpp:57-239)
-// restructured as kernels over all in-flight camera samples of a chunk:
+// Wavefront GI over a job table.
 //
-//   raygen -> [trace -> shade] x max_depth -> trace -> shade(final) -> accumulate
+// The per-pixel recursion of ImageRenderer + GiTracer::GetLi + DiTracer::GetDi
+// (imagerenderer.cpp:61-101, gitracer.cpp:13-140, ditracer.cpp:57-239) is restructured
+// as kernels over every in-flight path of a chunk:
+//
+//   raygen(per job segment) -> [trace -> shade] x depth -> trace -> shade(final)
+//                           -> accumulate / SH-project (per job segment)
+//
+// A job is either a camera frame (bsh_render) or a probe set (bsh_bake_sh); a batch
+// (bsh_batch_run) is a list of jobs over any number of committed scenes. Every path
+// carries its job index, so one chunk mixes scenes freely. Chunks are cut at pixel /
+// probe boundaries, so the per-pixel sums and per-probe projections are formed in the
+// same order whatever the batch composition: a job gives bit-identical results alone or
+// batched.
 //
 // shade(b) first resolves the next-event (NEE) rays its previous pass issued, then
-// shades the closest hit of the extension ray, emits the new shadow (any-hit),
-// MIS (closest-hit) and extension rays into two compacted queues (one atomic per
-// wave and queue), and finalises terminated paths. trace() is a persistent kernel
-// whose waves take 64 rays at a time from the closest queue, then from the any
-// queue, so every wave runs a single traversal variant.
+// shades the closest hit of the extension ray, emits the new shadow (any-hit), MIS
+// (closest-hit) and extension rays into two compacted queues (one atomic per wave and
+// queue), and finalises terminated paths. trace() is a persistent kernel whose waves
+// take 64 rays at a time from the closest queue, then the any queue, so every wave runs
+// a single traversal variant.
 //
 // Random numbers come from the keyed sample stream; the counter of every draw is a
 // fixed function of (bounce, light), which reproduces the reference's draw order
@@ -18,16 +28,17 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
+#include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "internal.h"
 #include "shading.h"
+#include "sh_dev.h"
 
 namespace bsh {
 
 constexpr int kBlock = 256;
-constexpr int kMaxLights = 16;
 
 enum : uint32_t
 {
@@ -42,20 +53,12 @@ enum : uint32_t { Q_EXT = 0, Q_NEE_CLOSEST = 1, Q_NEE_ANY = 2 };
 
 struct RenderDev
 {
-    SceneDev s;
-    CameraDev cam;
-    int W, H, cx, cy, cw, ch;
-    int grid, spp, s_begin, s_count;
-    int depth;
-    uint32_t seed;
-    float bg0, bg1, bg2;
-    int spc;       // samples per pixel in this chunk
-    int s0;        // first sample of this chunk
-    uint32_t P;    // paths in this chunk
-    int nl, K;
+    const JobDev* jobs;
+    uint32_t P;        // paths in this chunk
+    int Kshift;        // NEE slots per path = 1 << Kshift
     float4 *ray_o, *ray_d, *thr, *thr_nee, *rad, *out;
     int4* hit;
-    uint4* meta;
+    uint4* meta;       // x: key base, y: flags | bounce << 8 | nee lights << 16 | job << 24, z/w: BSDF sample
     uint32_t* nee_mask;
     float4 *nee_a, *nee_b, *nee_wi;
     int4* nee_res;
@@ -73,46 +76,95 @@ __global__ __launch_bounds__(kBlock) void k_zero(uint32_t* p, int n)
     if (i < n) p[i] = 0u;
 }
 
-// ImageRenderer sample loop + PerscpectiveCamera::GenerateRay (imagerenderer.cpp:78-87,
-// perspective_camera.cpp:25-38). Every path emits exactly one closest ray, at queue slot i.
-__global__ __launch_bounds__(kBlock) void k_raygen(RenderDev R)
+// Every path of a chunk enqueues exactly one primary ray at its own slot.
+__global__ void k_set_count(uint32_t* counters, unsigned long long* stats, uint32_t P)
 {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i == 0)
+    if (threadIdx.x == 0)
     {
-        R.counters[0] = R.P;
-        atomicAdd(&R.stats[0], (unsigned long long)R.P);
+        counters[0] = P;
+        stats[0] += (unsigned long long)P;
     }
-    if (i >= R.P) return;
-    uint32_t q = i / (uint32_t)R.spc;
-    int s = R.s0 + (int)(i - q * (uint32_t)R.spc);
-    int x = R.cx + (int)(q % (uint32_t)R.cw);
-    int y = R.cy + (int)(q / (uint32_t)R.cw);
-    uint32_t g = ((uint32_t)y * (uint32_t)R.W + (uint32_t)x) * (uint32_t)R.spp + (uint32_t)s;
-    uint32_t base = ks_base(R.seed, g);
-    float ux, uy;
-    ks_sample2d(base, KS_IMG, 0, ux, uy);
-    int idx = s % (R.grid * R.grid);
-    float cell = 1.f / (float)R.grid;
-    float sx = ((float)(idx % R.grid) + ux) * cell;
-    float sy = ((float)(idx / R.grid) + uy) * cell;
-    float imx = (float)x / (float)R.W + (1.f / (float)R.W) * sx;
-    float imy = (float)y / (float)R.H + (1.f / (float)R.H) * sy;
-    float hx = imx - 0.5f, hy = imy - 0.5f;
-    float csx = hx * R.cam.dimx, csy = hy * R.cam.dimy;
-    f3 fwd = mk(R.cam.fwd[0], R.cam.fwd[1], R.cam.fwd[2]);
-    f3 right = mk(R.cam.right[0], R.cam.right[1], R.cam.right[2]);
-    f3 up = mk(R.cam.up[0], R.cam.up[1], R.cam.up[2]);
-    f3 d = normalize(R.cam.znear * fwd + csx * right + csy * up);
-    f3 o = mk(R.cam.eye[0], R.cam.eye[1], R.cam.eye[2]);
+}
+
+__device__ __forceinline__ void init_path(const RenderDev& R, uint32_t i, f3 o, f3 d, float tmin, float tmax,
+                                          uint32_t base, uint32_t job)
+{
     R.ray_o[i] = st3(o, 0.f);
     R.ray_d[i] = st3(d, 0.f);
     R.thr[i] = make_float4(1.f, 1.f, 1.f, 0.f);
     R.rad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    R.meta[i] = make_uint4(base, F_EXT, 0u, 0u);
-    R.qc_o[0][i] = st3(o, R.cam.znear);
-    R.qc_d[0][i] = st3(d, R.cam.zfar);
+    R.meta[i] = make_uint4(base, F_EXT | (job << 24), 0u, 0u);
+    R.qc_o[0][i] = st3(o, tmin);
+    R.qc_d[0][i] = st3(d, tmax);
     R.qc_own[0][i] = (int)((Q_EXT << 29) | i);
+}
+
+// Camera raygen for pixels [u0, u0+nu) of camera job `job`, written at chunk offset c0:
+// ImageRenderer sample loop + PerscpectiveCamera::GenerateRay (imagerenderer.cpp:78-87,
+// perspective_camera.cpp:25-38).
+__global__ __launch_bounds__(kBlock) void k_raygen(RenderDev R, uint32_t job, uint32_t u0, uint32_t nu, uint32_t c0)
+{
+    const JobDev& J = R.jobs[job];
+    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nu * (uint32_t)J.s_count) return;
+    uint32_t i = c0 + t;
+    uint32_t q = u0 + t / (uint32_t)J.s_count;                 // pixel within the crop
+    int s = J.s_begin + (int)(t % (uint32_t)J.s_count);
+    int x = J.cx + (int)(q % (uint32_t)J.cw);
+    int y = J.cy + (int)(q / (uint32_t)J.cw);
+    uint32_t g = ((uint32_t)y * (uint32_t)J.W + (uint32_t)x) * (uint32_t)J.spp + (uint32_t)s;
+    uint32_t base = ks_base(J.seed, g);
+    float ux, uy;
+    ks_sample2d(base, KS_IMG, 0, ux, uy);
+    int idx = s % (J.grid * J.grid);
+    float cell = 1.f / (float)J.grid;
+    float sx = ((float)(idx % J.grid) + ux) * cell;
+    float sy = ((float)(idx / J.grid) + uy) * cell;
+    float imx = (float)x / (float)J.W + (1.f / (float)J.W) * sx;
+    float imy = (float)y / (float)J.H + (1.f / (float)J.H) * sy;
+    float hx = imx - 0.5f, hy = imy - 0.5f;
+    float csx = hx * J.cam.dimx, csy = hy * J.cam.dimy;
+    f3 fwd = mk(J.cam.fwd[0], J.cam.fwd[1], J.cam.fwd[2]);
+    f3 right = mk(J.cam.right[0], J.cam.right[1], J.cam.right[2]);
+    f3 up = mk(J.cam.up[0], J.cam.up[1], J.cam.up[2]);
+    f3 d = normalize(J.cam.znear * fwd + csx * right + csy * up);
+    f3 o = mk(J.cam.eye[0], J.cam.eye[1], J.cam.eye[2]);
+    init_path(R, i, o, d, J.cam.znear, J.cam.zfar, base, job);
+}
+
+// Stratified uniform-sphere direction of probe ray j (sqrt(N) = g cells per axis),
+// drawn from stream KS_PROBE.
+__device__ __forceinline__ f3 probe_dir(uint32_t base, int j, int g)
+{
+    float ux, uy;
+    ks_sample2d(base, KS_PROBE, 0, ux, uy);
+    float cell = 1.f / (float)g;
+    float u = ((float)(j % g) + ux) * cell;
+    float v = ((float)(j / g) + uy) * cell;
+    float z = 1.f - 2.f * u;
+    float r = sqrtf(fmaxf(0.f, 1.f - z * z));
+    float phi = 2.f * kPi * v;
+    return mk(r * cosf(phi), r * sinf(phi), z);
+}
+
+__global__ __launch_bounds__(kBlock) void k_raygen_probe(RenderDev R, uint32_t job, uint32_t u0, uint32_t nu,
+                                                       uint32_t c0)
+{
+    const JobDev& J = R.jobs[job];
+    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nu * (uint32_t)J.N) return;
+    uint32_t i = c0 + t;
+    uint32_t p = u0 + t / (uint32_t)J.N;
+    int j = (int)(t % (uint32_t)J.N);
+    uint32_t base = ks_base(J.seed, p * (uint32_t)J.N + (uint32_t)j);
+    f3 d = probe_dir(base, j, J.g);
+    f3 o = mk(J.probes[3 * p], J.probes[3 * p + 1], J.probes[3 * p + 2]);
+    init_path(R, i, o, d, J.tmin, 1000000.f, base, job);
+}
+
+__device__ __forceinline__ uint32_t path_of(const RenderDev& R, uint32_t kind, uint32_t idx)
+{
+    return kind == Q_EXT ? idx : (idx >> R.Kshift);
 }
 
 template <bool kCount>
@@ -141,11 +193,13 @@ __global__ __launch_bounds__(kBlock) void k_trace_queue(RenderDev R, int L, int 
                 float4 a = qo[q], b = qd[q];
                 uint32_t own = (uint32_t)qown[q];
                 uint32_t kind = own >> 29, idx = own & ((1u << 29) - 1u);
+                uint32_t job = R.meta[path_of(R, kind, idx)].y >> 24;
+                const SceneDev S = R.jobs[job].s;
                 TraceResult res;
                 TraceCounts cnt{0u, 0u};
                 if (pass == 0)
                 {
-                    bool h = traverse<false, kCount>(R.s, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, my, kBlock,
+                    bool h = traverse<false, kCount>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, my, kBlock,
                                                      res, cnt);
                     int4 o = make_int4(h ? res.prim : -1, __float_as_int(res.t), __float_as_int(res.b1),
                                        __float_as_int(res.b2));
@@ -154,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_queue(RenderDev R, int L, int 
                 }
                 else
                 {
-                    bool h = traverse<true, kCount>(R.s, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, my, kBlock,
+                    bool h = traverse<true, kCount>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, my, kBlock,
                                                     res, cnt);
                     R.nee_res[idx] = make_int4(h ? 1 : 0, 0, 0, 0);
                 }
@@ -186,7 +240,7 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const int K = R.K;
+    const int K = 1 << R.Kshift;
     bool emit_ext = false;
     uint32_t emit_mask = 0;
     f3 org = mk(0.f, 0.f, 0.f), ext_d = mk(0.f, 0.f, 0.f);
@@ -196,6 +250,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
         uint32_t fl = meta.y & 0xffu;
         int pb = (int)((meta.y >> 8) & 0xffu);
         int ncnt = (int)((meta.y >> 16) & 0xffu);
+        const uint32_t job = meta.y >> 24;
+        const JobDev& J = R.jobs[job];
+        const SceneDev S = J.s;
+        const int nl = S.nlights;
         if (!(fl & F_DONE))
         {
             f3 rad = ld3(R.rad[i]);
@@ -222,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                         {
                             f3 p1, p2, p3, n1, n2, n3;
                             int mat, bl;
-                            tri_verts(R.s, r.x, p1, p2, p3, n1, n2, n3, mat, bl);
+                            tri_verts(S, r.x, p1, p2, p3, n1, n2, n3, mat, bl);
                             if (bl == Lid)   // shadowhit.bundle->GetAreaLight() == &light, ditracer.cpp:202
                             {
                                 Frame hh = fill_hit(p1, p2, p3, n1, n2, n3, __int_as_float(r.z), __int_as_float(r.w));
@@ -232,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                                 if (dot(hh.n, -wi) > 0.f)
                                 {
                                     float d2inv = 1.f / sqnorm(d);
-                                    Mat lm = load_mat(R.s, mat);
+                                    Mat lm = load_mat(S, mat);
                                     if (lm.type == 2) le = emissive_le(lm.c, hh.n, -wi) * d2inv;
                                     else le = mk(0.f, 0.f, 0.f) * d2inv;  // Material::GetLe default
                                 }
@@ -245,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                         }
                     }
                     if (pb == 0) rad = rad + T * local;
-                    else rad = rad + (T * local) * (float)R.nl;
+                    else rad = rad + (T * local) * (float)nl;
                 }
                 fl &= ~F_NEE;
             }
@@ -258,7 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                 {
                     if (b == 0)
                     {
-                        rad = mk(R.bg0, R.bg1, R.bg2);
+                        rad = mk(J.bg[0], J.bg[1], J.bg[2]);
                         fl |= F_NOCLAMP;
                     }
                     fl |= F_TERM;
@@ -267,9 +325,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                 {
                     f3 p1, p2, p3, n1, n2, n3;
                     int mat, bl;
-                    tri_verts(R.s, hr.x, p1, p2, p3, n1, n2, n3, mat, bl);
+                    tri_verts(S, hr.x, p1, p2, p3, n1, n2, n3, mat, bl);
                     Frame h = fill_hit(p1, p2, p3, n1, n2, n3, __int_as_float(hr.z), __int_as_float(hr.w));
-                    Mat m = load_mat(R.s, mat);
+                    Mat m = load_mat(S, mat);
                     f3 wo = -ld3(R.ray_d[i]);
                     uint32_t base = meta.x;
                     f3 T = ld3(R.thr[i]);
@@ -294,16 +352,16 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                     {
                         org = h.p;
                         // ---- next-event estimation (gitracer.cpp:70-84 -> ditracer.cpp:57-239) ----
-                        if (R.nl > 0)
+                        if (nl > 0)
                         {
-                            int jn = b == 0 ? R.nl : 1;
+                            int jn = b == 0 ? nl : 1;
                             int Lsel = 0;
-                            if (b != 0) Lsel = (int)(ks_rand(base, rand_pick_counter(b)) % (uint32_t)R.nl);
+                            if (b != 0) Lsel = (int)(ks_rand(base, rand_pick_counter(b)) % (uint32_t)nl);
                             for (int j = 0; j < jn; ++j)
                             {
                                 int Lid = b == 0 ? j : Lsel;
-                                const LightDev L = R.s.lights[Lid];
-                                uint32_t lc = b == 0 ? (uint32_t)(2 * j) : (uint32_t)(2 * R.nl + 2 * (b - 1));
+                                const LightDev L = S.lights[Lid];
+                                uint32_t lc = b == 0 ? (uint32_t)(2 * j) : (uint32_t)(2 * nl + 2 * (b - 1));
                                 uint32_t bc = 2u + lc;
                                 float lx, ly, bx, by;
                                 ks_sample2d(base, KS_LIGHT, lc, lx, ly);
@@ -312,7 +370,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                                 // light sample + shadow ray
                                 f3 ld;
                                 float lpdf = 0.f;
-                                f3 le = light_sample(R.s, L, h.p, lx, ly, ld, lpdf);
+                                f3 le = light_sample(S, L, h.p, lx, ly, ld, lpdf);
                                 if (lpdf > kMinPdf && sqnorm(le) > 0.f)
                                 {
                                     f3 wi = normalize(ld);
@@ -333,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                                 f3 wi = normalize(wis);
                                 if (sqnorm(fb) > 0.f && bpdf > kMinPdf)
                                 {
-                                    float lp = light_pdf(R.s, L, h.p, wi);
+                                    float lp = light_pdf(S, L, h.p, wi);
                                     if (!(lp < kMinPdf))
                                     {
                                         float w = power_heuristic(bpdf, lp);
@@ -351,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                             ncnt = jn;
                         }
                         // ---- continuation: BSDF sample, throughput, Russian roulette ----
-                        if (b + 1 < R.depth)
+                        if (b + 1 < J.depth)
                         {
                             f3 wi;
                             float pdf = 0.f;
@@ -410,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                 fl |= F_DONE;
             }
             R.rad[i] = st3(rad, 0.f);
-            meta.y = fl | ((uint32_t)b << 8) | ((uint32_t)ncnt << 16);
+            meta.y = fl | ((uint32_t)b << 8) | ((uint32_t)ncnt << 16) | (job << 24);
             R.meta[i] = meta;
         }
     }
@@ -468,27 +526,31 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_accumulate(RenderDev R, float* accum, float* samples, int first_chunk,
-                                                       int accumulate)
+// One thread per pixel of a camera segment: FileImagePlane::WriteSample's buf += L,
+// buf.w += 1 (fileimageplane.cpp:29-34) over the pixel's samples in order, plus sum L^2.
+__global__ __launch_bounds__(kBlock) void k_accumulate(RenderDev R, uint32_t job, uint32_t u0, uint32_t nu,
+                                                       uint32_t c0)
 {
-    uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    uint32_t npix = (uint32_t)R.cw * (uint32_t)R.ch;
-    if (q >= npix) return;
+    const JobDev& J = R.jobs[job];
+    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nu) return;
+    uint32_t q = u0 + t;
+    float4* acc = reinterpret_cast<float4*>(J.accum);
     float4 s1, s2;
-    if (first_chunk && !accumulate)
+    if (J.accumulate)
+    {
+        s1 = acc[2 * q];
+        s2 = acc[2 * q + 1];
+    }
+    else
     {
         s1 = make_float4(0.f, 0.f, 0.f, 0.f);
         s2 = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    else
+    const uint32_t first = c0 + t * (uint32_t)J.s_count;
+    for (int k = 0; k < J.s_count; ++k)
     {
-        s1 = reinterpret_cast<float4*>(accum)[2 * q];
-        s2 = reinterpret_cast<float4*>(accum)[2 * q + 1];
-    }
-    for (int k = 0; k < R.spc; ++k)
-    {
-        float4 L = R.out[(size_t)q * R.spc + k];
-        // FileImagePlane::WriteSample: buf += L; buf.w += 1 (fileimageplane.cpp:29-34)
+        float4 L = R.out[first + k];
         s1.x += L.x;
         s1.y += L.y;
         s1.z += L.z;
@@ -496,21 +558,74 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(RenderDev R, float* accum
         s2.x += L.x * L.x;
         s2.y += L.y * L.y;
         s2.z += L.z * L.z;
-        if (samples)
+        if (J.samples)
         {
-            size_t o = ((size_t)q * R.s_count + (size_t)(R.s0 - R.s_begin + k)) * 3;
-            samples[o] = L.x;
-            samples[o + 1] = L.y;
-            samples[o + 2] = L.z;
+            size_t o = ((size_t)q * J.s_count + k) * 3;
+            J.samples[o] = L.x;
+            J.samples[o + 1] = L.y;
+            J.samples[o + 2] = L.z;
         }
     }
-    reinterpret_cast<float4*>(accum)[2 * q] = s1;
-    reinterpret_cast<float4*>(accum)[2 * q + 1] = s2;
+    acc[2 * q] = s1;
+    acc[2 * q + 1] = s2;
 }
 
-static bsh_status ensure_work(bsh_scene* s, size_t P, int K, int nl)
+// One wave per probe of a probe segment: lane l accumulates rays l, l+64, ... in order,
+// then a fixed butterfly reduction; c_i = (4 pi / N) sum_j L_j Y_i(w_j).
+template <int LMAX>
+__global__ __launch_bounds__(kBlock) void k_project_sh(RenderDev R, uint32_t job, uint32_t u0, uint32_t nu,
+                                                     uint32_t c0)
 {
-    RenderWork& w = s->work;
+    constexpr int NT = (LMAX + 1) * (LMAX + 1);
+    const JobDev& J = R.jobs[job];
+    uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= nu) return;
+    const uint32_t p = u0 + wave;
+    const int N = J.N;
+    float acc[3 * NT];
+#pragma unroll
+    for (int k = 0; k < 3 * NT; ++k) acc[k] = 0.f;
+    for (int j = lane; j < N; j += 64)
+    {
+        float4 L = R.out[c0 + wave * (uint32_t)N + (uint32_t)j];
+        uint32_t base = ks_base(J.seed, p * (uint32_t)N + (uint32_t)j);
+        f3 d = probe_dir(base, j, J.g);
+        float y[kShMaxTerms];
+        sh_evaluate(d, LMAX, y);
+#pragma unroll
+        for (int k = 0; k < NT; ++k)
+        {
+            acc[3 * k + 0] += L.x * y[k];
+            acc[3 * k + 1] += L.y * y[k];
+            acc[3 * k + 2] += L.z * y[k];
+        }
+        if (J.radiance)
+        {
+            size_t o = ((size_t)p * N + j) * 3;
+            J.radiance[o] = L.x;
+            J.radiance[o + 1] = L.y;
+            J.radiance[o + 2] = L.z;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < 3 * NT; ++k) acc[k] += __shfl_xor(acc[k], off);
+    if (lane == 0)
+    {
+        float scale = 4.f * kPi / (float)N;
+#pragma unroll
+        for (int k = 0; k < 3 * NT; ++k) J.coeffs[(size_t)p * 3 * NT + k] = acc[k] * scale;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------
+static bsh_status ensure_work(bsh_context* c, size_t P, int K, int nlmax)
+{
+    RenderWork& w = c->work;
     BSH_HIP(w.ray_o.alloc(P));
     BSH_HIP(w.ray_d.alloc(P));
     BSH_HIP(w.thr.alloc(P));
@@ -520,12 +635,12 @@ static bsh_status ensure_work(bsh_scene* s, size_t P, int K, int nl)
     BSH_HIP(w.hit.alloc(P));
     BSH_HIP(w.meta.alloc(P));
     BSH_HIP(w.nee_mask.alloc(P));
-    size_t slots = P * (size_t)std::max(K, 1);
+    size_t slots = P * (size_t)K;
     BSH_HIP(w.nee_a.alloc(slots));
     BSH_HIP(w.nee_b.alloc(slots));
     BSH_HIP(w.nee_wi.alloc(slots));
     BSH_HIP(w.nee_res.alloc(slots));
-    size_t qc = P * (size_t)(1 + nl), qa = P * (size_t)std::max(nl, 1);
+    size_t qc = P * (size_t)(1 + nlmax), qa = P * (size_t)std::max(nlmax, 1);
     for (int k = 0; k < 2; ++k)
     {
         BSH_HIP(w.qc_o[k].alloc(qc));
@@ -536,12 +651,139 @@ static bsh_status ensure_work(bsh_scene* s, size_t P, int K, int nl)
         BSH_HIP(w.qa_own[k].alloc(qa));
     }
     BSH_HIP(w.stats.alloc(4));
-    w.paths = P;
-    w.slots = K;
     return BSH_OK;
 }
 
-bsh_status render_frame(bsh_scene* s, const bsh_render_params* p, float* accum, float* samples, hipStream_t st)
+// HIP events around every trace and shade launch on the launch stream, read once at the end.
+struct LaunchTimer
+{
+    std::vector<hipEvent_t> evs;
+    size_t ei = 0;
+    bsh_status init(size_t pairs)
+    {
+        evs.resize(3 * pairs + 2);
+        for (auto& e : evs) BSH_HIP(hipEventCreate(&e));
+        return BSH_OK;
+    }
+    hipEvent_t next() { return evs[ei++]; }
+    ~LaunchTimer()
+    {
+        for (auto& e : evs) (void)hipEventDestroy(e);
+    }
+};
+
+#define BSH_TRY(x)                   \
+    do                               \
+    {                                \
+        bsh_status r_ = (x);         \
+        if (r_ != BSH_OK) return r_; \
+    } while (0)
+
+// A run of consecutive units (pixels or probes) of one job inside a chunk.
+struct Segment
+{
+    uint32_t job, u0, nu, c0;
+};
+struct Chunk
+{
+    uint32_t P = 0;
+    int depth = 0;
+    std::vector<Segment> segs;
+};
+
+static std::vector<Chunk> plan_chunks(const std::vector<JobDev>& jobs, size_t pmax)
+{
+    std::vector<Chunk> out;
+    Chunk cur;
+    for (uint32_t j = 0; j < jobs.size(); ++j)
+    {
+        const JobDev& J = jobs[j];
+        uint32_t unit = J.type == 0 ? (uint32_t)J.s_count : (uint32_t)J.N;
+        uint32_t nunits = J.type == 0 ? (uint32_t)(J.cw * J.ch) : (uint32_t)J.nprobes;
+        uint32_t u = 0;
+        while (u < nunits)
+        {
+            size_t room = pmax - cur.P;
+            if (room < unit)
+            {
+                out.push_back(cur);
+                cur = Chunk();
+                room = pmax;
+            }
+            uint32_t take = (uint32_t)std::min<size_t>(nunits - u, room / unit);
+            cur.segs.push_back(Segment{j, u, take, cur.P});
+            cur.P += take * unit;
+            cur.depth = std::max(cur.depth, J.depth);
+            u += take;
+        }
+    }
+    if (cur.P) out.push_back(cur);
+    return out;
+}
+
+static dim3 grid_for(size_t n) { return dim3((uint32_t)std::max<size_t>(1, (n + kBlock - 1) / kBlock)); }
+
+// Enqueues every kernel of a batch on `st`. With T == nullptr no events are recorded
+// (the form captured into a hipGraph).
+static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<Chunk>& chunks,
+                                const std::vector<JobDev>& hjobs, bool count, const bsh_tuning& tune, hipStream_t st,
+                                LaunchTimer* T)
+{
+    RenderWork& w = c->work;
+    const int trace_blocks = c->num_cus * std::max(1, tune.trace_blocks_per_cu);
+    for (const Chunk& ch : chunks)
+    {
+        const int launches = ch.depth + 1;
+        R.P = ch.P;
+        hipLaunchKernelGGL(k_zero, grid_for(4 * launches), dim3(kBlock), 0, st, w.counters.p, 4 * launches);
+        for (const Segment& sg : ch.segs)
+        {
+            const JobDev& J = hjobs[sg.job];
+            size_t n = (size_t)sg.nu * (J.type == 0 ? J.s_count : J.N);
+            if (J.type == 0) hipLaunchKernelGGL(k_raygen, grid_for(n), dim3(kBlock), 0, st, R, sg.job, sg.u0, sg.nu, sg.c0);
+            else hipLaunchKernelGGL(k_raygen_probe, grid_for(n), dim3(kBlock), 0, st, R, sg.job, sg.u0, sg.nu, sg.c0);
+        }
+        hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, st, w.counters.p, w.stats.p, ch.P);
+        dim3 pg = grid_for(ch.P);
+        for (int b = 0; b <= ch.depth; ++b)
+        {
+            if (T) BSH_HIP(hipEventRecord(T->next(), st));
+            if (count)
+                hipLaunchKernelGGL(k_trace_queue<true>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1);
+            else
+                hipLaunchKernelGGL(k_trace_queue<false>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1);
+            if (T) BSH_HIP(hipEventRecord(T->next(), st));
+            hipLaunchKernelGGL(k_shade, pg, dim3(kBlock), 0, st, R, b, b + 1 <= ch.depth ? b + 1 : b);
+            if (T) BSH_HIP(hipEventRecord(T->next(), st));
+        }
+        for (const Segment& sg : ch.segs)
+        {
+            const JobDev& J = hjobs[sg.job];
+            if (J.type == 0)
+            {
+                hipLaunchKernelGGL(k_accumulate, grid_for(sg.nu), dim3(kBlock), 0, st, R, sg.job, sg.u0, sg.nu, sg.c0);
+            }
+            else
+            {
+                dim3 wg = grid_for((size_t)sg.nu * 64);
+                switch (J.lmax)
+                {
+                case 0: hipLaunchKernelGGL(k_project_sh<0>, wg, dim3(kBlock), 0, st, R, sg.job, sg.u0, sg.nu, sg.c0); break;
+                case 1: hipLaunchKernelGGL(k_project_sh<1>, wg, dim3(kBlock), 0, st, R, sg.job, sg.u0, sg.nu, sg.c0); break;
+                case 2: hipLaunchKernelGGL(k_project_sh<2>, wg, dim3(kBlock), 0, st, R, sg.job, sg.u0, sg.nu, sg.c0); break;
+                case 3: hipLaunchKernelGGL(k_project_sh<3>, wg, dim3(kBlock), 0, st, R, sg.job, sg.u0, sg.nu, sg.c0); break;
+                default: hipLaunchKernelGGL(k_project_sh<4>, wg, dim3(kBlock), 0, st, R, sg.job, sg.u0, sg.nu, sg.c0); break;
+                }
+            }
+        }
+    }
+    BSH_HIP(hipGetLastError());
+    return BSH_OK;
+}
+
+constexpr int kMaxLights = 16;
+
+bsh_status make_render_job(bsh_scene* s, const bsh_render_params* p, float* accum, float* samples, JobDev& J)
 {
     if (!s || !p || !accum) return fail(BSH_ERR_INVALID, "bsh_render: NULL argument");
     if (!s->committed) return fail(BSH_ERR_STATE, "bsh_render: scene not committed");
@@ -549,8 +791,7 @@ bsh_status render_frame(bsh_scene* s, const bsh_render_params* p, float* accum, 
     if (p->flags & BSH_RENDER_NO_QUIRKS) return fail(BSH_ERR_UNSUPPORTED, "bsh_render: quirk-free mode not built");
     if (p->width <= 0 || p->height <= 0 || p->img_grid <= 0 || p->max_depth <= 0 || p->max_depth > 250)
         return fail(BSH_ERR_INVALID, "bsh_render: bad frame parameters");
-    int nl = (int)s->lights.size();
-    if (nl > kMaxLights) return fail(BSH_ERR_UNSUPPORTED, "bsh_render: more than 16 area lights");
+    if (s->lights.size() > (size_t)kMaxLights) return fail(BSH_ERR_UNSUPPORTED, "bsh_render: more than 16 area lights");
     int cw = p->crop_w > 0 ? p->crop_w : p->width, ch = p->crop_h > 0 ? p->crop_h : p->height;
     int cx = p->crop_w > 0 ? p->crop_x : 0, cy = p->crop_h > 0 ? p->crop_y : 0;
     if (cx < 0 || cy < 0 || cx + cw > p->width || cy + ch > p->height)
@@ -560,39 +801,103 @@ bsh_status render_frame(bsh_scene* s, const bsh_render_params* p, float* accum, 
     if (sb < 0 || sc <= 0 || sb + sc > spp) return fail(BSH_ERR_INVALID, "bsh_render: bad sample range");
     if ((uint64_t)p->width * p->height * spp > 0xffffffffull)
         return fail(BSH_ERR_UNSUPPORTED, "bsh_render: more than 2^32 camera samples per frame");
-    size_t npix = (size_t)cw * ch;
-    int spc = p->chunk_spp > 0 ? p->chunk_spp : (int)std::max<size_t>(1, ((size_t)1 << 21) / npix);
-    spc = std::min(spc, sc);
-    size_t P = npix * (size_t)spc;
-    int K = 2 * std::max(nl, 1);
-    if (P * (size_t)K >= ((size_t)1 << 29)) return fail(BSH_ERR_UNSUPPORTED, "bsh_render: chunk too large");
-    BSH_HIP(hipSetDevice(s->ctx->device));
-    bsh_status e = ensure_work(s, P, K, nl);
-    if (e != BSH_OK) return e;
-    RenderWork& w = s->work;
-    const int launches = p->max_depth + 1;
-    BSH_HIP(w.counters.alloc((size_t)4 * launches));
+    J = JobDev{};
+    J.s = s->dev();
+    J.cam = make_camera(s->cam);
+    J.type = 0;
+    J.W = p->width;
+    J.H = p->height;
+    J.cx = cx;
+    J.cy = cy;
+    J.cw = cw;
+    J.ch = ch;
+    J.grid = p->img_grid;
+    J.spp = spp;
+    J.s_begin = sb;
+    J.s_count = sc;
+    J.depth = p->max_depth;
+    J.seed = p->seed;
+    for (int k = 0; k < 3; ++k) J.bg[k] = p->bg[k];
+    J.accumulate = (p->flags & BSH_RENDER_ACCUMULATE) ? 1 : 0;
+    J.accum = accum;
+    J.samples = samples;
+    return BSH_OK;
+}
+
+bsh_status make_probe_job(bsh_scene* s, const float* probes, size_t nprobes, const bsh_probe_params* p,
+                          float* coeffs, float* radiance, JobDev& J)
+{
+    if (!s || !p || !coeffs || (nprobes && !probes)) return fail(BSH_ERR_INVALID, "bsh_bake_sh: NULL argument");
+    if (!s->committed) return fail(BSH_ERR_STATE, "bsh_bake_sh: scene not committed");
+    if (p->lmax < 0 || p->lmax > kShMaxL) return fail(BSH_ERR_UNSUPPORTED, "bsh_bake_sh: lmax must be 0..4");
+    int N = p->rays_per_probe;
+    int g = (int)std::lround(std::sqrt((double)(N > 0 ? N : 0)));
+    if (N <= 0 || g * g != N) return fail(BSH_ERR_INVALID, "bsh_bake_sh: rays_per_probe must be a square");
+    if (p->max_depth <= 0 || p->max_depth > 250) return fail(BSH_ERR_INVALID, "bsh_bake_sh: bad max_depth");
+    if ((uint64_t)nprobes * (uint64_t)N > 0xffffffffull)
+        return fail(BSH_ERR_UNSUPPORTED, "bsh_bake_sh: more than 2^32 probe rays");
+    if (s->lights.size() > (size_t)kMaxLights) return fail(BSH_ERR_UNSUPPORTED, "bsh_bake_sh: more than 16 area lights");
+    J = JobDev{};
+    J.s = s->dev();
+    J.type = 1;
+    J.depth = p->max_depth;
+    J.seed = p->seed;
+    for (int k = 0; k < 3; ++k) J.bg[k] = p->bg[k];
+    J.probes = probes;
+    J.nprobes = (int)nprobes;
+    J.N = N;
+    J.g = g;
+    J.lmax = p->lmax;
+    J.tmin = p->tmin;
+    J.coeffs = coeffs;
+    J.radiance = radiance;
+    return BSH_OK;
+}
+
+bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, const bsh_tuning* tune_in, int use_graph,
+                    hipStream_t st)
+{
+    if (hjobs.empty()) return BSH_OK;
+    if (hjobs.size() > 256) return fail(BSH_ERR_UNSUPPORTED, "at most 256 jobs per batch");
+    std::lock_guard<std::mutex> lock(c->mu);
+    BSH_HIP(hipSetDevice(c->device));
+    bsh_tuning tune = tune_in ? *tune_in : c->tuning;
+    if (tune.paths_per_chunk <= 0) tune.paths_per_chunk = 1 << 21;
+    if (tune.trace_blocks_per_cu <= 0) tune.trace_blocks_per_cu = 5;
+    int nlmax = 1;
+    uint32_t maxunit = 1;
+    uint64_t samples = 0;
+    for (const JobDev& J : hjobs)
+    {
+        nlmax = std::max(nlmax, J.s.nlights);
+        maxunit = std::max<uint32_t>(maxunit, J.type == 0 ? (uint32_t)J.s_count : (uint32_t)J.N);
+        samples += J.type == 0 ? (uint64_t)J.cw * J.ch * J.s_count : (uint64_t)J.nprobes * J.N;
+    }
+    int Kshift = 1;
+    while ((1 << Kshift) < 2 * nlmax) ++Kshift;
+    const int K = 1 << Kshift;
+    size_t pmax = std::max<size_t>((size_t)tune.paths_per_chunk, maxunit);
+    if (pmax * (size_t)K >= ((size_t)1 << 29)) pmax = (((size_t)1 << 29) - 1) / K;
+    if (pmax < maxunit) return fail(BSH_ERR_UNSUPPORTED, "unit larger than the NEE slot space");
+    std::vector<Chunk> chunks = plan_chunks(hjobs, pmax);
+    size_t P = 0;
+    int maxdepth = 1;
+    size_t pairs = 0;
+    for (const Chunk& ch : chunks)
+    {
+        P = std::max<size_t>(P, ch.P);
+        maxdepth = std::max(maxdepth, ch.depth);
+        pairs += ch.depth + 1;
+    }
+    BSH_TRY(ensure_work(c, P, K, nlmax));
+    RenderWork& w = c->work;
+    BSH_HIP(w.counters.alloc((size_t)4 * (maxdepth + 1)));
+    BSH_HIP(c->jobs.alloc(hjobs.size()));
+    BSH_HIP(hipMemcpyAsync(c->jobs.p, hjobs.data(), hjobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, st));
 
     RenderDev R;
-    R.s = s->dev();
-    R.cam = make_camera(s->cam);
-    R.W = p->width;
-    R.H = p->height;
-    R.cx = cx;
-    R.cy = cy;
-    R.cw = cw;
-    R.ch = ch;
-    R.grid = p->img_grid;
-    R.spp = spp;
-    R.s_begin = sb;
-    R.s_count = sc;
-    R.depth = p->max_depth;
-    R.seed = p->seed;
-    R.bg0 = p->bg[0];
-    R.bg1 = p->bg[1];
-    R.bg2 = p->bg[2];
-    R.nl = nl;
-    R.K = K;
+    R.jobs = c->jobs.p;
+    R.Kshift = Kshift;
     R.ray_o = w.ray_o.p; R.ray_d = w.ray_d.p; R.thr = w.thr.p; R.thr_nee = w.thr_nee.p; R.rad = w.rad.p; R.out = w.out.p;
     R.hit = w.hit.p; R.meta = w.meta.p; R.nee_mask = w.nee_mask.p;
     R.nee_a = w.nee_a.p; R.nee_b = w.nee_b.p; R.nee_wi = w.nee_wi.p; R.nee_res = w.nee_res.p;
@@ -603,70 +908,95 @@ bsh_status render_frame(bsh_scene* s, const bsh_render_params* p, float* accum, 
     }
     R.counters = w.counters.p;
     R.stats = w.stats.p;
+    R.P = 0;
 
-    const bool count_visits = (p->flags & BSH_RENDER_COUNT) != 0;
-    int trace_blocks = s->ctx->num_cus * 5;
-    const int nchunks = (sc + spc - 1) / spc;
-    // Event triplets around every trace/shade pair; read once after the frame.
-    std::vector<hipEvent_t> evs((size_t)3 * nchunks * launches + 2);
-    for (auto& ev : evs) BSH_HIP(hipEventCreate(&ev));
-    size_t ei = 0;
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, st, (uint32_t*)w.stats.p, 8);
-    BSH_HIP(hipEventRecord(evs[ei++], st));
-    for (int s0 = sb; s0 < sb + sc; s0 += spc)
+    bsh_render_stats& S = c->stats;
+    S = bsh_render_stats{};
+    if (use_graph)
     {
-        int n = std::min(spc, sb + sc - s0);
-        R.spc = n;
-        R.s0 = s0;
-        R.P = (uint32_t)(npix * (size_t)n);
-        dim3 pg((R.P + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_zero, dim3((4 * launches + kBlock - 1) / kBlock), dim3(kBlock), 0, st, w.counters.p,
-                           4 * launches);
-        hipLaunchKernelGGL(k_raygen, pg, dim3(kBlock), 0, st, R);
-        for (int b = 0; b <= p->max_depth; ++b)
+        // Capture the whole batch (every chunk's raygen, bounces, accumulation) as one graph
+        // and launch it once. Counters are cleared by kernels, never by memset nodes.
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        BSH_HIP(hipEventCreate(&e0));
+        BSH_HIP(hipEventCreate(&e1));
+        hipStream_t cs = nullptr;
+        BSH_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        bsh_status rc = BSH_OK;
+        // order the capture stream after prior work on `st` (job table upload, stats clear)
+        (void)hipEventRecord(e0, st);
+        (void)hipStreamWaitEvent(cs, e0, 0);
+        (void)hipStreamSynchronize(cs);
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess)
         {
-            BSH_HIP(hipEventRecord(evs[ei++], st));
-            if (count_visits)
-                hipLaunchKernelGGL(k_trace_queue<true>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1);
-            else
-                hipLaunchKernelGGL(k_trace_queue<false>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1);
-            BSH_HIP(hipEventRecord(evs[ei++], st));
-            hipLaunchKernelGGL(k_shade, pg, dim3(kBlock), 0, st, R, b, b + 1 <= p->max_depth ? b + 1 : b);
-            BSH_HIP(hipEventRecord(evs[ei++], st));
+            rc = enqueue_batch(c, R, chunks, hjobs, count, tune, cs, nullptr);
+            hipError_t e2 = hipStreamEndCapture(cs, &graph);
+            if (rc == BSH_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
         }
-        hipLaunchKernelGGL(k_accumulate, dim3((uint32_t)((npix + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, R,
-                           accum, samples, s0 == sb ? 1 : 0, (p->flags & BSH_RENDER_ACCUMULATE) ? 1 : 0);
+        else
+        {
+            rc = hip_fail(e, "hipStreamBeginCapture");
+        }
+        if (rc == BSH_OK)
+        {
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+        }
+        if (rc == BSH_OK)
+        {
+            (void)hipEventRecord(e0, cs);
+            e = hipGraphLaunch(exec, cs);
+            if (e != hipSuccess) rc = hip_fail(e, "hipGraphLaunch");
+            (void)hipEventRecord(e1, cs);
+            if (rc == BSH_OK)
+            {
+                e = hipEventSynchronize(e1);
+                if (e != hipSuccess) rc = hip_fail(e, "graph execution");
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            S.total_ms = ms;
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipStreamDestroy(cs);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc != BSH_OK) return rc;
+        S.trace_launches = S.shade_launches = (uint32_t)pairs;
     }
-    BSH_HIP(hipEventRecord(evs[ei++], st));
-    BSH_HIP(hipGetLastError());
-    BSH_HIP(hipEventSynchronize(evs[ei - 1]));
-    double trace_ms = 0.0, shade_ms = 0.0;
-    uint32_t ntrace = 0, nshade = 0;
-    for (size_t k = 1; k + 2 < ei; k += 3)
+    else
     {
-        float a = 0.f, b2 = 0.f;
-        BSH_HIP(hipEventElapsedTime(&a, evs[k], evs[k + 1]));
-        BSH_HIP(hipEventElapsedTime(&b2, evs[k + 1], evs[k + 2]));
-        trace_ms += a;
-        shade_ms += b2;
-        ++ntrace;
-        ++nshade;
+        LaunchTimer T;
+        BSH_TRY(T.init(pairs));
+        BSH_HIP(hipEventRecord(T.next(), st));
+        BSH_TRY(enqueue_batch(c, R, chunks, hjobs, count, tune, st, &T));
+        BSH_HIP(hipEventRecord(T.next(), st));
+        BSH_HIP(hipEventSynchronize(T.evs[T.ei - 1]));
+        for (size_t k = 1; k + 2 < T.ei; k += 3)
+        {
+            float a = 0.f, b = 0.f;
+            BSH_HIP(hipEventElapsedTime(&a, T.evs[k], T.evs[k + 1]));
+            BSH_HIP(hipEventElapsedTime(&b, T.evs[k + 1], T.evs[k + 2]));
+            S.trace_ms += a;
+            S.shade_ms += b;
+            S.trace_launches++;
+            S.shade_launches++;
+        }
+        float tot = 0.f;
+        BSH_HIP(hipEventElapsedTime(&tot, T.evs[0], T.evs[T.ei - 1]));
+        S.total_ms = tot;
     }
-    float tot = 0.f;
-    BSH_HIP(hipEventElapsedTime(&tot, evs[0], evs[ei - 1]));
-    for (auto& ev : evs) (void)hipEventDestroy(ev);
     unsigned long long st2[4];
     BSH_HIP(hipMemcpy(st2, w.stats.p, sizeof(st2), hipMemcpyDeviceToHost));
-    s->stats.rays_closest = st2[0];
-    s->stats.rays_any = st2[1];
-    s->stats.node_visits = count_visits ? st2[2] : 0;
-    s->stats.tri_tests = count_visits ? st2[3] : 0;
-    s->stats.camera_samples = (uint64_t)npix * (uint64_t)sc;
-    s->stats.trace_ms = trace_ms;
-    s->stats.shade_ms = shade_ms;
-    s->stats.total_ms = tot;
-    s->stats.trace_launches = ntrace;
-    s->stats.shade_launches = nshade;
+    S.rays_closest = st2[0];
+    S.rays_any = st2[1];
+    S.node_visits = count ? st2[2] : 0;
+    S.tri_tests = count ? st2[3] : 0;
+    S.camera_samples = samples;
     return BSH_OK;
 }
 

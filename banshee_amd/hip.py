@@ -61,6 +61,23 @@ class RenderParams(ctypes.Structure):
                 ("bg", ctypes.c_float * 3), ("flags", ctypes.c_int32)]
 
 
+class ProbeParams(ctypes.Structure):
+    _fields_ = [("rays_per_probe", ctypes.c_int32), ("lmax", ctypes.c_int32), ("max_depth", ctypes.c_int32),
+                ("seed", ctypes.c_uint32), ("chunk_probes", ctypes.c_int32), ("tmin", ctypes.c_float),
+                ("bg", ctypes.c_float * 3), ("flags", ctypes.c_int32)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("paths_per_chunk", ctypes.c_int32), ("trace_blocks_per_cu", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("scene", c_void_p), ("kind", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("render", RenderParams), ("accum", c_void_p), ("samples", c_void_p), ("probe", ProbeParams),
+                ("probes", c_void_p), ("nprobes", c_size_t), ("coeffs", c_void_p), ("radiance", c_void_p)]
+
+
 class RenderStats(ctypes.Structure):
     _fields_ = [("rays_closest", ctypes.c_uint64), ("rays_any", ctypes.c_uint64),
                 ("camera_samples", ctypes.c_uint64), ("node_visits", ctypes.c_uint64),
@@ -76,6 +93,8 @@ EXPORTS = [
     "bsh_scene_set_lights", "bsh_scene_set_camera", "bsh_scene_commit", "bsh_scene_destroy", "bsh_scene_info",
     "bsh_intersect_closest", "bsh_intersect_any", "bsh_intersect_count", "bsh_render", "bsh_render_stats_get",
     "bsh_diag_bsdf", "bsh_diag_light", "bsh_diag_camera", "bsh_diag_keyed",
+    "bsh_bake_sh", "bsh_sh_project_env", "bsh_sh_irradiance", "bsh_diag_sh_eval",
+    "bsh_batch_run", "bsh_context_set_tuning", "bsh_context_get_tuning", "bsh_context_stats_get", "bsh_autotune",
 ]
 
 _lib = None
@@ -192,10 +211,141 @@ class Context:
                "bsh_diag_keyed")
         return out
 
+    def diag_sh_eval(self, dirs, lmax):
+        d = np.ascontiguousarray(dirs, np.float32)
+        out = np.zeros((d.shape[0], (lmax + 1) ** 2), np.float32)
+        _check(lib().bsh_diag_sh_eval(self.h, c_int(lmax), c_size_t(d.shape[0]), _p(d), _p(out)), "bsh_diag_sh_eval")
+        return out
+
+    def sh_project_env(self, env, lmax):
+        env = np.ascontiguousarray(env, np.float32)
+        H, W = env.shape[:2]
+        de = self.to_device(env)
+        dc = self.alloc(((lmax + 1) ** 2, 3), np.float32)
+        _check(lib().bsh_sh_project_env(self.h, de.ptr, c_int(W), c_int(H), c_int(lmax), dc.ptr, c_void_p(None)),
+               "bsh_sh_project_env")
+        self.sync()
+        return dc.numpy()
+
+    def sh_irradiance(self, lmax, coeffs, normals):
+        dc = self.to_device(np.ascontiguousarray(coeffs, np.float32))
+        dn = self.to_device(np.ascontiguousarray(normals, np.float32))
+        n = dn.shape[0]
+        do = self.alloc((n, 3), np.float32)
+        _check(lib().bsh_sh_irradiance(self.h, c_int(lmax), dc.ptr, dn.ptr, c_size_t(n), do.ptr, c_void_p(None)),
+               "bsh_sh_irradiance")
+        self.sync()
+        return do.numpy()
+
+    # ---- batching ----
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().bsh_context_get_tuning(self.h, ctypes.byref(t)), "bsh_context_get_tuning")
+        return {k: getattr(t, k) for k, _ in t._fields_ if k != "reserved"}
+
+    def set_tuning(self, paths_per_chunk=0, trace_blocks_per_cu=0, use_graph=0):
+        t = Tuning(paths_per_chunk, trace_blocks_per_cu, use_graph, 0)
+        _check(lib().bsh_context_set_tuning(self.h, ctypes.byref(t)), "bsh_context_set_tuning")
+
+    def stats(self) -> dict:
+        st = RenderStats()
+        _check(lib().bsh_context_stats_get(self.h, ctypes.byref(st)), "bsh_context_stats_get")
+        return {k: getattr(st, k) for k, _ in st._fields_ if k != "pad"}
+
+    def batch_run(self, jobs, tuning: dict | None = None, stream=None):
+        """jobs: list of BatchJob. Runs them in one wavefront (one hipGraph when use_graph)."""
+        arr = (Job * len(jobs))(*[j.struct() for j in jobs])
+        t = None
+        if tuning is not None:
+            t = Tuning(tuning.get("paths_per_chunk", 0), tuning.get("trace_blocks_per_cu", 0),
+                       tuning.get("use_graph", 0), 0)
+        _check(lib().bsh_batch_run(self.h, arr, c_size_t(len(jobs)), ctypes.byref(t) if t else None,
+                                   c_void_p(stream)), "bsh_batch_run")
+
+    def autotune(self, jobs, paths=None, blocks=None):
+        arr = (Job * len(jobs))(*[j.struct() for j in jobs])
+        pa = np.ascontiguousarray(paths if paths is not None else [], np.int32)
+        bl = np.ascontiguousarray(blocks if blocks is not None else [], np.int32)
+        best = Tuning()
+        ms = ctypes.c_double()
+        _check(lib().bsh_autotune(self.h, arr, c_size_t(len(jobs)), _p(pa) if pa.size else None, c_size_t(pa.size),
+                                  _p(bl) if bl.size else None, c_size_t(bl.size), ctypes.byref(best),
+                                  ctypes.byref(ms)), "bsh_autotune")
+        return {"paths_per_chunk": best.paths_per_chunk, "trace_blocks_per_cu": best.trace_blocks_per_cu}, ms.value
+
     def close(self):
         if self.h:
             lib().bsh_context_destroy(self.h)
             self.h = c_void_p()
+
+
+def render_params(width, height, img_grid, max_depth, seed, crop=None, spp_range=None, chunk_spp=0,
+                  bg=(0.0, 0.0, 0.0), count=False, accumulate=False) -> RenderParams:
+    p = RenderParams()
+    p.width, p.height = width, height
+    if crop is not None:
+        p.crop_x, p.crop_y, p.crop_w, p.crop_h = crop
+    p.img_grid = img_grid
+    spp = img_grid * img_grid
+    sb, sc = (spp_range if spp_range else (0, spp))
+    p.spp_begin, p.spp_count = sb, sc
+    p.max_depth = max_depth
+    p.seed = seed
+    p.chunk_spp = chunk_spp
+    for k in range(3):
+        p.bg[k] = bg[k]
+    p.flags = (4 if count else 0) | (1 if accumulate else 0)
+    return p
+
+
+class BatchJob:
+    """A render or probe-bake job for Context.batch_run; owns its device outputs."""
+
+    def __init__(self, scene: "Scene", kind: str, **kw):
+        self.scene = scene
+        self.kind = kind
+        ctx = scene.ctx
+        if kind == "render":
+            self.params = render_params(**{k: v for k, v in kw.items() if k != "per_sample"})
+            crop = kw.get("crop")
+            cw = crop[2] if crop else kw["width"]
+            ch = crop[3] if crop else kw["height"]
+            sc = self.params.spp_count
+            self.accum = ctx.alloc((ch, cw, 8), np.float32)
+            self.samples = ctx.alloc((ch, cw, sc, 3), np.float32) if kw.get("per_sample") else None
+        else:
+            self.probes = ctx.to_device(np.ascontiguousarray(kw["probes"], np.float32).reshape(-1, 3))
+            n = self.probes.shape[0]
+            pp = ProbeParams()
+            pp.rays_per_probe, pp.lmax, pp.max_depth, pp.seed = kw["rays_per_probe"], kw["lmax"], kw["max_depth"], \
+                kw["seed"]
+            pp.tmin = kw.get("tmin", 0.0)
+            self.pparams = pp
+            self.coeffs = ctx.alloc((n, (kw["lmax"] + 1) ** 2, 3), np.float32)
+            self.radiance = ctx.alloc((n, kw["rays_per_probe"], 3), np.float32) if kw.get("radiance") else None
+
+    def struct(self) -> Job:
+        j = Job()
+        j.scene = self.scene.h
+        if self.kind == "render":
+            j.kind = 0
+            j.render = self.params
+            j.accum = self.accum.ptr
+            j.samples = self.samples.ptr if self.samples else None
+        else:
+            j.kind = 1
+            j.probe = self.pparams
+            j.probes = self.probes.ptr
+            j.nprobes = self.probes.shape[0]
+            j.coeffs = self.coeffs.ptr
+            j.radiance = self.radiance.ptr if self.radiance else None
+        return j
+
+    def result(self):
+        if self.kind == "render":
+            a = self.accum.numpy()
+            return a[..., 0:3], a[..., 4:7], a[..., 3], (self.samples.numpy() if self.samples else None)
+        return self.coeffs.numpy(), (self.radiance.numpy() if self.radiance else None)
 
 
 def camera_struct(cam13) -> Camera:
@@ -315,6 +465,27 @@ class Scene:
         self.ctx.sync()
         a = accum.numpy()
         return a[..., 0:3], a[..., 4:7], a[..., 3], (samp.numpy() if samp else None)
+
+    def bake_sh(self, probes, rays_per_probe, lmax, max_depth, seed, tmin=0.0, chunk_probes=0, radiance=False,
+                count=False, coeffs: DeviceArray | None = None, download=True):
+        """Returns (coeffs [nprobes, (lmax+1)^2, 3], per-ray radiance [nprobes, N, 3] or None)."""
+        probes = np.ascontiguousarray(probes, np.float32).reshape(-1, 3)
+        n = probes.shape[0]
+        dp = self.ctx.to_device(probes)
+        nt = (lmax + 1) ** 2
+        if coeffs is None:
+            coeffs = self.ctx.alloc((n, nt, 3), np.float32)
+        rad = self.ctx.alloc((n, rays_per_probe, 3), np.float32) if radiance else None
+        p = ProbeParams()
+        p.rays_per_probe, p.lmax, p.max_depth, p.seed = rays_per_probe, lmax, max_depth, seed
+        p.chunk_probes, p.tmin = chunk_probes, tmin
+        p.flags = 4 if count else 0
+        _check(lib().bsh_bake_sh(self.h, dp.ptr, c_size_t(n), ctypes.byref(p), coeffs.ptr,
+                                 rad.ptr if rad else c_void_p(None), c_void_p(None)), "bsh_bake_sh")
+        if not download:
+            return None
+        self.ctx.sync()
+        return coeffs.numpy(), (rad.numpy() if rad else None)
 
     def stats(self) -> dict:
         s = RenderStats()

@@ -323,3 +323,35 @@ def terrain_for_tris(target_tris: int, seed: int, subdiv=3, width=1920, height=1
     rest = max(0, target_tris - 2 * grid * grid)
     spheres = max(1, rest // per_sphere)
     return terrain_scene(grid=grid, spheres=spheres, subdiv=subdiv, seed=seed, width=width, height=height)
+
+
+def batch_jobs(n=64, seed=100, min_tris=1_000, max_tris=2_000_000, min_res=128, max_res=1024, min_grid=2,
+               max_grid=8, min_depth=4, max_depth=12):
+    """The batched workload: n independent jobs of the terrain generator with triangle counts
+    log-uniform in [min_tris, max_tris], square resolutions in [min_res, max_res] (multiples
+    of 16), spp = grid^2 with grid in [min_grid, max_grid] (4..64 spp) and depth in
+    [min_depth, max_depth]; job k uses scene seed seed + k. Returns a list of dicts."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(n):
+        tris = int(round(math.exp(rng.uniform(math.log(min_tris), math.log(max_tris)))))
+        res = int(rng.integers(min_res // 16, max_res // 16 + 1)) * 16
+        grid = int(rng.integers(min_grid, max_grid + 1))
+        depth = int(rng.integers(min_depth, max_depth + 1))
+        out.append({"tris": tris, "res": res, "grid": grid, "depth": depth, "seed": seed + k})
+    return out
+
+
+def probe_grid(scene: Scene, n=16, margin=0.05, lift=0.5):
+    """n^3 probe positions on a regular grid inside the scene's bounding box, each moved up
+    until it is `lift` above the heightfield's top vertex of its column so that probes sit
+    in free space."""
+    lo = scene.verts.min(0)
+    hi = scene.verts.max(0)
+    ext = hi - lo
+    t = (np.arange(n) + 0.5) / n
+    xs = lo[0] + ext[0] * (margin + (1 - 2 * margin) * t)
+    ys = lo[1] + ext[1] * (margin + (1 - 2 * margin) * t)
+    zs = lo[2] + ext[2] * (margin + (1 - 2 * margin) * t)
+    P = np.stack(np.meshgrid(xs, ys, zs, indexing="ij"), -1).reshape(-1, 3)
+    return P.astype(np.float32)

@@ -12,9 +12,9 @@
  * reference side (a HipAccelerator : Intersectable and a HipImageRenderer : Renderer).
  *
  * Threading: one context per device; a scene is immutable after bsh_scene_commit;
- * calls on different streams may run concurrently from different threads, except
- * that bsh_render / bsh_bake_sh / bsh_batch_run on one scene serialise on the
- * scene's own work buffers.
+ * ray queries on different streams may run concurrently from different threads;
+ * bsh_render / bsh_bake_sh / bsh_batch_run share the context's wavefront buffers and
+ * serialise on a per-context lock (they return after their work has completed).
  */
 #ifndef BANSHEE_HIP_H
 #define BANSHEE_HIP_H
@@ -129,7 +129,8 @@ enum
     BSH_RENDER_COUNT = 4        /* count BVH node visits and triangle tests (slower; for roofline accounting) */
 };
 
-/* Per-kernel timings and ray counts of the last bsh_render (profiling). */
+/* Per-kernel timings and ray counts of the last bsh_render / bsh_bake_sh / bsh_batch_run
+   on a context (profiling). */
 typedef struct
 {
     uint64_t rays_closest, rays_any;   /* BVH queries issued (Intersectable::Intersect x2) */
@@ -193,9 +194,91 @@ bsh_status bsh_intersect_count(bsh_scene* s, const bsh_ray* rays, size_t n, int 
    samples: optional device float[crop_w*crop_h*spp_count*3] receiving every camera
    sample's returned radiance (GiTracer::GetLi's return value) in (pixel, s) order. */
 bsh_status bsh_render(bsh_scene* s, const bsh_render_params* p, float* accum, float* samples, void* stream);
+/* Stats of the last render / bake / batch on the scene's context. */
 bsh_status bsh_render_stats_get(const bsh_scene* s, bsh_render_stats* out);
 
+/* ---------------- spherical harmonics (math/sh.h, math/shproject.h, tracer/shtracer.h) ---------------- */
+/* Probe baking (no reference counterpart; it composes GiTracer::GetLi with ShEvaluate,
+   sh.cpp:97-153): for every probe, rays_per_probe stratified directions over the sphere
+   (uniform pdf 1/(4 pi)), each estimated with the same GiTracer(max_depth) as bsh_render,
+   projected as c_i = (4 pi / N) sum_j L(w_j) Y_i(w_j). Probe ray j of probe p is keyed
+   as sample g = p*N + j. */
+typedef struct
+{
+    int32_t rays_per_probe;   /* N, a perfect square (stratified sqrt(N) x sqrt(N) grid) */
+    int32_t lmax;             /* 0..4; the reference's order-3 probes use lmax = 2 (9 coefficients) */
+    int32_t max_depth;
+    uint32_t seed;
+    int32_t chunk_probes;     /* probes per wavefront chunk, 0 = auto */
+    float tmin;               /* probe ray tmin */
+    float bg[3];
+    int32_t flags;            /* BSH_RENDER_COUNT honoured */
+} bsh_probe_params;
+
+/* probes: device float[nprobes*3]; coeffs: device float[nprobes*(lmax+1)^2*3] (coefficient-major,
+   RGB inner); radiance: optional device float[nprobes*N*3] receiving every probe ray's GetLi. */
+bsh_status bsh_bake_sh(bsh_scene* s, const float* probes, size_t nprobes, const bsh_probe_params* p, float* coeffs,
+                       float* radiance, void* stream);
+/* ShProjectEnvironmentMap(envmap, W, H, lmax, coeffs) (math/shproject.cpp:73-130): Riemann sum over a
+   lat-long map with the reference's y-up direction. env: device float[H*W*3]; coeffs: device
+   float[(lmax+1)^2*3], overwritten. */
+bsh_status bsh_sh_project_env(bsh_context* ctx, const float* env, int width, int height, int lmax, float* coeffs,
+                              void* stream);
+/* ShTracer::GetE (tracer/shtracer.cpp:20-36): irradiance sum_i Y_i(n) ShConvolveCosTheta(c)_i 0.7/pi
+   (math/sh.cpp:160-180). coeffs device float[(lmax+1)^2*3]; normals/out device float[n*3]. */
+bsh_status bsh_sh_irradiance(bsh_context* ctx, int lmax, const float* coeffs, const float* normals, size_t n,
+                             float* out, void* stream);
+
+/* ---------------- batching (no reference counterpart) ---------------- */
+/* Launch-shape parameters of the wavefront: paths in flight per chunk (queue capacity)
+   and persistent trace workgroups per CU; use_graph captures the whole batch as one
+   hipGraph and launches it once. */
+typedef struct
+{
+    int32_t paths_per_chunk;
+    int32_t trace_blocks_per_cu;
+    int32_t use_graph;
+    int32_t reserved;
+} bsh_tuning;
+
+enum { BSH_JOB_RENDER = 0, BSH_JOB_BAKE_SH = 1 };
+
+/* One entry of a batch: exactly the arguments of bsh_render (kind BSH_JOB_RENDER) or of
+   bsh_bake_sh (kind BSH_JOB_BAKE_SH), on any committed scene of the context. */
+typedef struct
+{
+    bsh_scene* scene;
+    int32_t kind;
+    int32_t reserved;
+    bsh_render_params render;
+    float* accum;
+    float* samples;
+    bsh_probe_params probe;
+    const float* probes;
+    size_t nprobes;
+    float* coeffs;
+    float* radiance;
+} bsh_job;
+
+/* Runs every job in one wavefront: paths of all jobs share the chunks, queues and
+   persistent trace kernel (a chunk mixes scenes), optionally captured as one hipGraph.
+   Each job's output is bit-identical to running it alone. tuning NULL = context tuning. */
+bsh_status bsh_batch_run(bsh_context* ctx, const bsh_job* jobs, size_t njobs, const bsh_tuning* tuning,
+                         void* stream);
+bsh_status bsh_context_set_tuning(bsh_context* ctx, const bsh_tuning* t);
+bsh_status bsh_context_get_tuning(const bsh_context* ctx, bsh_tuning* t);
+bsh_status bsh_context_stats_get(const bsh_context* ctx, bsh_render_stats* out);
+/* Autotuner: times the batch under every candidate (paths_per_chunk x trace_blocks_per_cu),
+   ncand each, and stores the fastest as the context tuning (also returned in best).
+   The jobs' outputs are overwritten. NULL candidate arrays use the built-in sweep. */
+bsh_status bsh_autotune(bsh_context* ctx, const bsh_job* jobs, size_t njobs, const int32_t* paths_per_chunk,
+                        size_t n_paths, const int32_t* blocks_per_cu, size_t n_blocks, bsh_tuning* best,
+                        double* best_ms);
+
 /* ---------------- diagnostics (exercise device code paths in isolation) ---------------- */
+/* Device ShEvaluate (math/sh.cpp:97-153) on n unit directions. Host pointers: dirs n x 3,
+   out n x (lmax+1)^2. */
+bsh_status bsh_diag_sh_eval(bsh_context* ctx, int lmax, size_t n, const float* dirs, float* out);
 /* Device evaluation of Material::Sample / Evaluate / GetPdf for one material row, same
    argument layout as the probes in tests: frames n x 12 (n, ng, dpdu, dpdv), view n x 3,
    u n x 2, ea/eb n x 3; out n x 12. Host pointers. */

@@ -1208,4 +1208,60 @@ void cref_sh_irradiance(int lmax, float const* coeffs, float const* normals, int
     }
 }
 
+/* Probe baking restated (no reference counterpart): for probe p and ray j the keyed
+   sample g = p*N + j draws a stratified uniform-sphere direction (stream 4, draws 0/1)
+   and GiTracer::GetLi estimates L; coefficients c_i = (4 pi / N) sum_j L Y_i(w_j) are
+   accumulated in double. coeffs: nprobes x (lmax+1)^2 x 3; radiance (optional): nprobes x N x 3. */
+void cref_bake_sh(void* h, float const* probes, int nprobes, int N, int lmax, int depth, unsigned seed, float tmin,
+                  double* coeffs, float* radiance, int threads)
+{
+    const Scene& s = *(Scene*)h;
+    int g = (int)std::lround(std::sqrt((double)N));
+    int nt = (lmax + 1) * (lmax + 1);
+    int nth = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nth; ++t)
+        pool.emplace_back([&]() {
+            std::vector<float> y(nt);
+            for (;;)
+            {
+                int p = next.fetch_add(1);
+                if (p >= nprobes) break;
+                std::vector<double> acc(3 * nt, 0.0);
+                V3 o = at(probes, p);
+                for (int j = 0; j < N; ++j)
+                {
+                    Draws D;
+                    D.base = ks_base(seed, (uint32_t)p * (uint32_t)N + (uint32_t)j);
+                    for (int i = 0; i < 8; ++i) D.ctr[i] = 0;
+                    float ux, uy;
+                    D.sample2d(KS_PROBE, ux, uy);
+                    float cell = 1.f / (float)g;
+                    float u = ((float)(j % g) + ux) * cell, v = ((float)(j / g) + uy) * cell;
+                    float z = 1.f - 2.f * u;
+                    float r = std::sqrt(std::max(0.f, 1.f - z * z));
+                    float phi = 2.f * PI * v;
+                    V3 d = v3(r * cosf(phi), r * sinf(phi), z);
+                    V3 L = get_li(s, o, d, tmin, 1000000.f, D, depth, V3{0, 0, 0}, nullptr);
+                    if (radiance)
+                    {
+                        float* q = radiance + 3 * ((size_t)p * N + j);
+                        q[0] = L.x; q[1] = L.y; q[2] = L.z;
+                    }
+                    sh_eval_one(d, lmax, y.data());
+                    for (int k = 0; k < nt; ++k)
+                    {
+                        acc[3 * k] += (double)L.x * y[k];
+                        acc[3 * k + 1] += (double)L.y * y[k];
+                        acc[3 * k + 2] += (double)L.z * y[k];
+                    }
+                }
+                double sc = 4.0 * 3.14159265358979323846 / N;
+                for (int k = 0; k < 3 * nt; ++k) coeffs[(size_t)p * 3 * nt + k] = acc[k] * sc;
+            }
+        });
+    for (auto& th : pool) th.join();
+}
+
 }  // extern "C"

@@ -54,6 +54,8 @@ def _load(kind):
         getattr(lib, pre + "sh_convolve").argtypes = [i, vp, vp]
         getattr(lib, pre + "sh_project").argtypes = [vp, i, i, i, vp]
         getattr(lib, pre + "sh_irradiance").argtypes = [i, vp, vp, i, vp]
+        if kind == "cpu":
+            lib.cref_bake_sh.argtypes = [vp, vp, i, i, i, i, ctypes.c_uint, ctypes.c_float, vp, vp, i]
         if kind == "ref":
             lib.oref_record_begin.argtypes = [vp, i, i]
             lib.oref_record_fetch.argtypes = [vp, vp, vp, vp, vp]
@@ -183,6 +185,16 @@ class OracleScene:
         rays = np.zeros(2, np.int64)
         t = self.o._fn("time_render")(self.h, W, H, img_grid, nee_grid, depth, seed, 1 if mt else 0, _p(rays))
         return t, rays
+
+    def bake_sh(self, probes, N, lmax, depth, seed, tmin=0.0, threads=0):
+        """CPU restatement only: (coeffs float64 [P, nt, 3], radiance float32 [P, N, 3])."""
+        probes = f32(probes).reshape(-1, 3)
+        P = probes.shape[0]
+        nt = (lmax + 1) ** 2
+        c = np.zeros((P, nt, 3), np.float64)
+        r = np.zeros((P, N, 3), np.float32)
+        self.o.lib.cref_bake_sh(self.h, _p(probes), P, N, lmax, depth, seed, tmin, _p(c), _p(r), threads)
+        return c, r
 
     def light(self, idx, p, u, w):
         p, u, w = f32(p), f32(u), f32(w)

@@ -1,0 +1,111 @@
+"""Spherical harmonics on the GPU: basis, env-map projection, irradiance and probe baking."""
+import numpy as np
+import pytest
+
+import oracle_lib
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def sh_close(got, ref, rtol=1e-5, floor=1e-2):
+    """|got - ref| <= rtol * max(|ref|, floor * |c00|) per channel. Near-zero coefficients
+    carry only FP32 summation noise (about 1e-7 absolute for sums of ~1000 terms of size
+    ~0.01), so their tolerance is floored at 1e-2 of the DC term."""
+    c00 = np.abs(ref[..., 0:1, :])
+    scale = np.maximum(np.abs(ref), floor * c00) + 1e-12
+    return np.abs(got - ref) <= rtol * scale
+
+
+def env_projection_fp64(env, lmax=2):
+    """ShProjectEnvironmentMap (shproject.cpp:73-130) in float64 with the closed-form basis."""
+    H, W = env.shape[:2]
+    th = np.pi / H / 2 + np.arange(H) * np.pi / H
+    ph = 2 * np.pi / W / 2 + np.arange(W) * 2 * np.pi / W
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    w = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1)
+    Y = _sh_fp64(w)
+    wt = (np.sin(T) * (np.pi / H) * (2 * np.pi / W))[..., None]
+    return np.einsum("hwk,hwc->kc", Y * wt, env.astype(np.float64))
+
+
+def test_basis_bit_exact(ctx):
+    fx = golden("sh")
+    np.testing.assert_array_equal(ctx.diag_sh_eval(fx["dirs"], 4), fx["ylm4"])
+
+
+def test_env_projection(ctx):
+    fx = golden("sh")
+    got = ctx.sh_project_env(fx["env"], 2)
+    # against an FP64 projection of the same map: 1e-5 relative (floored at 1e-2 |c00|)
+    assert np.all(sh_close(got[None], env_projection_fp64(fx["env"])[None]))
+    # against the reference's own FP32 sequential sum: within that sum's rounding (1e-4)
+    assert np.all(sh_close(got[None], fx["proj"][None], rtol=1e-4))
+    const = ctx.sh_project_env(np.ones((64, 128, 3), np.float32), 2)
+    assert np.all(sh_close(const[None], env_projection_fp64(np.ones((64, 128, 3)))[None]))
+    assert np.all(sh_close(const[None], fx["const_proj"][None], rtol=1e-4))
+    assert abs(const[0, 0] - np.sqrt(4 * np.pi)) < 2e-3   # Riemann sum of Y00 over the sphere
+
+
+def test_irradiance(ctx):
+    fx = golden("sh")
+    got = ctx.sh_irradiance(2, fx["cin"], fx["normals"])
+    np.testing.assert_allclose(got, fx["irr"], rtol=2e-6, atol=1e-7)
+
+
+PROBES = np.array([[1.0, 1.0, 1.0], [0.5, 1.5, 1.5], [1.5, 0.3, 1.2], [0.2, 0.2, 1.9], [1.0, 1.7, 0.4]],
+                  np.float32)
+
+
+def _sh_fp64(d):
+    """Appendix-C closed forms of the reference basis (z pole, Condon-Shortley), lmax = 2."""
+    x, y, z = d[..., 0], d[..., 1], d[..., 2]
+    return np.stack([0.28209479177387814 * np.ones_like(x), -0.4886025119029199 * y, 0.4886025119029199 * z,
+                     -0.4886025119029199 * x, 1.0925484305920792 * x * y, -1.0925484305920792 * y * z,
+                     0.31539156525252005 * (3 * z * z - 1), -1.0925484305920792 * x * z,
+                     0.5462742152960396 * (x * x - y * y)], -1)
+
+
+def test_probe_projection_of_a_radiance_tape(ctx, cornell):
+    """Given the GPU's own per-ray radiance, its FP32 projection matches an FP64 projection."""
+    from banshee_amd import hip
+    s = hip.Scene(ctx, cornell)
+    N, seed = 1024, 7
+    c, rad = s.bake_sh(PROBES, N, 2, 5, seed, radiance=True)
+    # regenerate the stratified directions from the keyed stream in float64
+    from test_gpu_shading import _keyed_np
+    P = PROBES.shape[0]
+    g = 32
+    j = np.arange(N)
+    cs = (np.arange(P)[:, None] * N + j[None]).astype(np.uint32).ravel()
+    st = np.full(cs.shape, 4, np.uint32)
+    uy = _keyed_np(seed, cs, st, np.zeros_like(cs)).astype(np.float64) * 2.328306e-10
+    ux = _keyed_np(seed, cs, st, np.ones_like(cs)).astype(np.float64) * 2.328306e-10
+    u = ((j % g)[None].repeat(P, 0).ravel() + ux) / g
+    v = ((j // g)[None].repeat(P, 0).ravel() + uy) / g
+    z = 1 - 2 * u
+    r = np.sqrt(np.maximum(0, 1 - z * z))
+    d = np.stack([r * np.cos(2 * np.pi * v), r * np.sin(2 * np.pi * v), z], -1).reshape(P, N, 3)
+    Y = _sh_fp64(d)
+    ref = 4 * np.pi / N * np.einsum("pnk,pnc->pkc", Y, rad.astype(np.float64))
+    ok = sh_close(c, ref, rtol=1e-5)
+    assert ok.all(), np.abs(c - ref).max()
+
+
+@pytest.mark.skipif(not oracle_lib.available("cpu"), reason="CPU restatement not built")
+def test_probe_bake_matches_restatement(ctx, cornell):
+    from banshee_amd import hip
+    s = hip.Scene(ctx, cornell)
+    N, seed, depth = 1024, 11, 5
+    c, rad = s.bake_sh(PROBES, N, 2, depth, seed, radiance=True)
+    cpu = oracle_lib.Oracle("cpu").scene(cornell)
+    cr, rr = cpu.bake_sh(PROBES, N, 2, depth, seed)
+    close = np.all(np.abs(rad - rr) <= 1e-4 + 1e-3 * np.abs(rr), -1)
+    print(f"probe rays agreeing with the restatement: {close.mean():.5f}")
+    assert close.mean() >= 0.97
+    # coefficients: within 3 standard errors of the projection estimator
+    Y = np.abs(_sh_fp64(np.array([[0, 0, 1.0]])))  # scale only
+    se = 4 * np.pi / np.sqrt(N) * rr.std(axis=1)[:, None, :] * 1.0926 + 1e-6
+    assert np.all(np.abs(c - cr) <= 3 * se)
+    frac_tight = np.mean(sh_close(c, cr, rtol=1e-4))
+    print(f"coefficients within 1e-4 of the restatement: {frac_tight:.4f}")
