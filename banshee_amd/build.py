@@ -21,7 +21,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbanshee_hip.so")
 ROOT = os.path.dirname(HERE)
 
-HIP_SOURCES = ["api.hip", "trace.hip", "render.hip", "diag.hip", "sh.hip"]
+HIP_SOURCES = ["api.hip", "trace.hip", "render.hip", "diag.hip", "sh.hip", "guide.hip"]
 CXX_SOURCES = ["bvh_build.cpp"]
 
 ARCH = os.environ.get("BANSHEE_ARCH", "gfx950")

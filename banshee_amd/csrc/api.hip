@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <thread>
 #include <vector>
@@ -296,6 +297,17 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
     if (!s->h_lights.empty())
         BSH_HIP(hipMemcpy(s->d_lights.p, s->h_lights.data(), s->h_lights.size() * sizeof(LightDev),
                           hipMemcpyHostToDevice));
+    for (int k = 0; k < 3; ++k)
+    {
+        s->box_lo[k] = std::numeric_limits<float>::max();
+        s->box_hi[k] = -std::numeric_limits<float>::max();
+    }
+    for (size_t i = 0; i < s->verts.size() / 3; ++i)
+        for (int k = 0; k < 3; ++k)
+        {
+            s->box_lo[k] = std::min(s->box_lo[k], s->verts[3 * i + k]);
+            s->box_hi[k] = std::max(s->box_hi[k], s->verts[3 * i + k]);
+        }
     s->nnodes = bo.nodes.size();
     s->max_depth = bo.max_depth;
     s->build_ms = bo.build_ms;
@@ -360,6 +372,31 @@ bsh_status bsh_bake_sh(bsh_scene* s, const float* probes, size_t nprobes, const 
     if (nprobes == 0) return BSH_OK;
     std::vector<JobDev> jobs(1, J);
     return run_jobs(s->ctx, jobs, (p->flags & BSH_RENDER_COUNT) != 0, nullptr, 0, (hipStream_t)stream);
+}
+
+bsh_status bsh_render_guided(bsh_scene* s, const bsh_render_params* p, bsh_guide* guide, float* accum,
+                             float* records, size_t cap, size_t* nrecords, void* stream)
+{
+    if (!p || !(p->flags & BSH_RENDER_NO_QUIRKS))
+        return fail(BSH_ERR_INVALID, "bsh_render_guided: needs BSH_RENDER_NO_QUIRKS (the guided estimator is quirk-free)");
+    if (cap && !records) return fail(BSH_ERR_INVALID, "bsh_render_guided: NULL records with nonzero capacity");
+    JobDev J;
+    bsh_status r = make_render_job(s, p, accum, nullptr, J);
+    if (r != BSH_OK) return r;
+    if (guide)
+    {
+        J.flags |= JOB_GUIDED;
+        J.bsdf_frac = guide_bsdf_fraction(guide);
+    }
+    if (records && cap) J.flags |= JOB_RECORD;
+    GuideRun gr;
+    gr.guide = guide;
+    gr.records = records;
+    gr.cap = cap;
+    std::vector<JobDev> jobs(1, J);
+    r = run_jobs(s->ctx, jobs, (p->flags & BSH_RENDER_COUNT) != 0, nullptr, 0, (hipStream_t)stream, &gr);
+    if (nrecords) *nrecords = gr.written;
+    return r;
 }
 
 bsh_status bsh_render_stats_get(const bsh_scene* s, bsh_render_stats* out)

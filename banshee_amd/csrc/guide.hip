@@ -1,0 +1,759 @@
+// Learned path guiding: FP16-weight / FP32-accumulate MFMA kernels for the guide MLP
+// (inference, training forward, backward, weight gradients, Adam). See guide_dev.h for the
+// network and the vMF mixture. No reference counterpart.
+//
+// Matrix tiles use v_mfma_f32_16x16x16_f16 with the CDNA operand layout
+//   A (16x16, MxK): lane l holds A[l%16][4*(l/16) + i], i = 0..3
+//   B (16x16, KxN): lane l holds B[4*(l/16) + i][l%16]
+//   D (16x16):      lane l holds D[4*(l/16) + i][l%16]
+// (checked on the device by bsh_diag_mfma). Weights live in LDS as FP16 rows padded to
+// K+4 halves, so the 16 lanes of a fragment read hit distinct banks and 8-byte aligned.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+#include "guide_dev.h"
+
+namespace bsh {
+
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4v mfma16(h4 a, h4 b, f4v c) { return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0); }
+
+// ---- forward weight layout in FP16 (out-major rows Wt[n][k]) ----
+constexpr int kPad = 4;
+constexpr int kL1S = kGIn + kPad, kLHS = kGHid + kPad;             // LDS row strides (halves)
+constexpr int kF1 = 0;                                             // 64 rows x 96
+constexpr int kF2 = kF1 + kGHid * kGIn;                            // 64 x 64
+constexpr int kF3 = kF2 + kGHid * kGHid;
+constexpr int kF4 = kF3 + kGHid * kGHid;
+constexpr int kF5 = kF4 + kGHid * kGHid;                           // 48 x 64
+constexpr int kFTotal = kF5 + kGOut * kGHid;
+// ---- backward layout (in-major rows W[n_in][k_out]) for layers 5..2 ----
+constexpr int kB5 = 0;                                             // 64 rows x 48
+constexpr int kB4 = kB5 + kGHid * kGOut;                           // 64 x 64
+constexpr int kB3 = kB4 + kGHid * kGHid;
+constexpr int kB2 = kB3 + kGHid * kGHid;
+constexpr int kBTotal = kB2 + kGHid * kGHid;
+constexpr int kBiasTotal = 4 * kGHid + kGOut;
+// LDS image of the forward weights (padded rows)
+constexpr int kLF1 = 0, kLF2 = kLF1 + kGHid * kL1S, kLF3 = kLF2 + kGHid * kLHS, kLF4 = kLF3 + kGHid * kLHS,
+              kLF5 = kLF4 + kGHid * kLHS, kLFTotal = kLF5 + kGOut * kLHS;
+
+struct GuideNet
+{
+    const _Float16* wf;   // forward FP16 weights (kFTotal)
+    const _Float16* wb;   // backward FP16 weights (kBTotal)
+    const float* bias;    // b1..b5 (kBiasTotal)
+};
+
+__device__ __forceinline__ void load_fwd_weights(const GuideNet& g, _Float16* lw, float* lb)
+{
+    // unpadded global rows -> padded LDS rows
+    for (int t = threadIdx.x; t < kFTotal; t += blockDim.x)
+    {
+        int dst;
+        if (t < kF2) { int n = t / kGIn, k = t % kGIn; dst = kLF1 + n * kL1S + k; }
+        else if (t < kF5)
+        {
+            int u = (t - kF2), l = u / (kGHid * kGHid), r = u % (kGHid * kGHid);
+            int n = r / kGHid, k = r % kGHid;
+            dst = kLF2 + l * kGHid * kLHS + n * kLHS + k;
+        }
+        else { int u = t - kF5; int n = u / kGHid, k = u % kGHid; dst = kLF5 + n * kLHS + k; }
+        lw[dst] = g.wf[t];
+    }
+    for (int t = threadIdx.x; t < kBiasTotal; t += blockDim.x) lb[t] = g.bias[t];
+}
+
+// Y[16 x N] = act(X[16 x K] W + b) for one wave; X, Wt in LDS; Y to LDS (FP16) with ReLU.
+template <int K, int N>
+__device__ __forceinline__ void dense_relu(const _Float16* X, int xs, const _Float16* Wt, int ws, const float* b,
+                                           _Float16* Y, int ys, int lane)
+{
+    const int r = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int nt = 0; nt < N / 16; ++nt)
+    {
+        f4v acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < K / 16; ++ks)
+        {
+            h4 a = *reinterpret_cast<const h4*>(&X[r * xs + ks * 16 + 4 * q]);
+            h4 w = *reinterpret_cast<const h4*>(&Wt[(nt * 16 + r) * ws + ks * 16 + 4 * q]);
+            acc = mfma16(a, w, acc);
+        }
+        float bb = b[nt * 16 + r];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Y[(4 * q + i) * ys + nt * 16 + r] = (_Float16)fmaxf(acc[i] + bb, 0.f);
+    }
+}
+
+// Encodes 16 rows of raw inputs (8 floats per row: v0..v6, active) into X [16 x 96] FP16.
+__device__ __forceinline__ void encode_rows(const float4* in, int stride, uint32_t row0, uint32_t nrows, _Float16* X,
+                                            int lane)
+{
+    for (int t = lane; t < 16 * kGIn; t += 64)
+    {
+        int rr = t / kGIn, j = t % kGIn;
+        float f = 0.f;
+        uint32_t row = row0 + rr;
+        if (row < nrows)
+        {
+            float4 a = in[(size_t)stride * row], c = in[(size_t)stride * row + 1];
+            float v[kGRaw] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z};
+            f = guide_feature(v, j);
+        }
+        X[rr * kL1S + j] = (_Float16)f;
+    }
+}
+
+constexpr int kFwdWaves = 8;
+
+// Fused forward: rows of raw inputs -> 48 FP32 outputs per row. With kTrain the encoded
+// input and the four hidden activations are also written (FP16) for the backward pass.
+template <bool kTrain>
+__global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet g, const float4* in, int stride,
+                                                                 uint32_t nrows, float* out, _Float16* x0,
+                                                                 _Float16* hs)
+{
+    __shared__ _Float16 lw[kLFTotal];
+    __shared__ float lb[kBiasTotal];
+    __shared__ _Float16 bufA[kFwdWaves][16 * kL1S];
+    __shared__ _Float16 bufB[kFwdWaves][16 * kLHS];
+    load_fwd_weights(g, lw, lb);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = lane & 15, q = lane >> 4;
+    _Float16* A = bufA[wave];
+    _Float16* B = bufB[wave];
+    const uint32_t ntiles = (nrows + 15) / 16;
+    for (uint32_t tile = blockIdx.x * kFwdWaves + wave; tile < ntiles; tile += gridDim.x * kFwdWaves)
+    {
+        const uint32_t row0 = tile * 16;
+        encode_rows(in, stride, row0, nrows, A, lane);
+        __builtin_amdgcn_wave_barrier();   // wave-local LDS hand-off: LDS ops of one wave complete in order
+        if (kTrain)
+            for (int t = lane; t < 16 * kGIn; t += 64)
+            {
+                int rr = t / kGIn, j = t % kGIn;
+                if (row0 + rr < nrows) x0[(size_t)(row0 + rr) * kGIn + j] = A[rr * kL1S + j];
+            }
+        dense_relu<kGIn, kGHid>(A, kL1S, lw + kLF1, kL1S, lb, B, kLHS, lane);       // H1 -> B
+        if (kTrain)
+            for (int t = lane; t < 16 * kGHid; t += 64)
+            {
+                int rr = t / kGHid, j = t % kGHid;
+                if (row0 + rr < nrows) hs[((size_t)0 * nrows + row0 + rr) * kGHid + j] = B[rr * kLHS + j];
+            }
+        dense_relu<kGHid, kGHid>(B, kLHS, lw + kLF2, kLHS, lb + kGHid, A, kLHS, lane);   // H2 -> A
+        if (kTrain)
+            for (int t = lane; t < 16 * kGHid; t += 64)
+            {
+                int rr = t / kGHid, j = t % kGHid;
+                if (row0 + rr < nrows) hs[((size_t)1 * nrows + row0 + rr) * kGHid + j] = A[rr * kLHS + j];
+            }
+        dense_relu<kGHid, kGHid>(A, kLHS, lw + kLF3, kLHS, lb + 2 * kGHid, B, kLHS, lane);   // H3 -> B
+        if (kTrain)
+            for (int t = lane; t < 16 * kGHid; t += 64)
+            {
+                int rr = t / kGHid, j = t % kGHid;
+                if (row0 + rr < nrows) hs[((size_t)2 * nrows + row0 + rr) * kGHid + j] = B[rr * kLHS + j];
+            }
+        dense_relu<kGHid, kGHid>(B, kLHS, lw + kLF4, kLHS, lb + 3 * kGHid, A, kLHS, lane);   // H4 -> A
+        if (kTrain)
+            for (int t = lane; t < 16 * kGHid; t += 64)
+            {
+                int rr = t / kGHid, j = t % kGHid;
+                if (row0 + rr < nrows) hs[((size_t)3 * nrows + row0 + rr) * kGHid + j] = A[rr * kLHS + j];
+            }
+        // output layer, FP32 straight from the accumulators
+#pragma unroll
+        for (int nt = 0; nt < kGOut / 16; ++nt)
+        {
+            f4v acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < kGHid / 16; ++ks)
+            {
+                h4 a = *reinterpret_cast<const h4*>(&A[r * kLHS + ks * 16 + 4 * q]);
+                h4 w = *reinterpret_cast<const h4*>(&lw[kLF5 + (nt * 16 + r) * kLHS + ks * 16 + 4 * q]);
+                acc = mfma16(a, w, acc);
+            }
+            float bb = lb[4 * kGHid + nt * 16 + r];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+            {
+                uint32_t row = row0 + 4 * q + i;
+                if (row < nrows) out[(size_t)row * kGOut + nt * 16 + r] = acc[i] + bb;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Per-row loss -w log q(w_dir) and its gradient w.r.t. the 48 outputs. Weights are
+// normalised to mean 1 over the batch (s = w * rows / sum w) so FP16 gradients stay O(1);
+// the weight-gradient reduction divides by rows. rec: 3 float4 per row: (x, r), (n, active),
+// (dir, w).
+__global__ void k_guide_loss(const float* out, const float4* rec, uint32_t nrows, const float* wsum, _Float16* dout,
+                             float* loss_acc)
+{
+    uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    const float* o = out + (size_t)row * kGOut;
+    float4 c = rec[3 * row + 2];
+    f3 w = mk(c.x, c.y, c.z);
+    float wt = c.w;
+    float scale = (float)nrows / fmaxf(wsum[0], 1e-30f);
+    Lobes L = guide_lobes(o);
+    float v[kGLobes], q = 0.f;
+    for (int k = 0; k < kGLobes; ++k)
+    {
+        v[k] = vmf_pdf(L.mu[k], L.kappa[k], w);
+        q += L.w[k] * v[k];
+    }
+    float g[kGOut];
+    for (int k = 0; k < kGOut; ++k) g[k] = 0.f;
+    float s = wt * scale;
+    if (q > 0.f && s > 0.f)
+    {
+        for (int k = 0; k < kGLobes; ++k)
+        {
+            float gam = L.w[k] * v[k] / q;
+            g[k] = s * (L.w[k] - gam);
+            // d(-log q)/d mu_k = -gam kappa w, projected through mu = r / |r|
+            f3 r = mk(o[8 + 3 * k], o[9 + 3 * k], o[10 + 3 * k]);
+            float rn = sqrtf(sqnorm(r));
+            if (rn > 0.f)
+            {
+                f3 gm = w * (-s * gam * L.kappa[k]);
+                f3 gr = (gm - L.mu[k] * dot(L.mu[k], gm)) * (1.f / rn);
+                g[8 + 3 * k] = gr.x;
+                g[9 + 3 * k] = gr.y;
+                g[10 + 3 * k] = gr.z;
+            }
+            float lk = o[32 + k];
+            if (lk > kLogKappaLo && lk < kLogKappaHi)
+            {
+                float kap = L.kappa[k];
+                float e = expf(-2.f * kap);
+                float dlogv = 1.f / kap - 2.f * e / (1.f - e) + (dot(L.mu[k], w) - 1.f);
+                g[32 + k] = -s * gam * kap * dlogv;
+            }
+        }
+        atomicAdd(loss_acc, -s * logf(q));
+    }
+    for (int k = 0; k < kGOut; ++k) dout[(size_t)row * kGOut + k] = (_Float16)g[k];
+}
+
+__global__ void k_guide_wsum(const float4* rec, uint32_t nrows, float* wsum)
+{
+    __shared__ float red[256];
+    float s = 0.f;
+    for (uint32_t r = blockIdx.x * 256 + threadIdx.x; r < nrows; r += gridDim.x * 256) s += rec[3 * r + 2].w;
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1)
+    {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(wsum, red[0]);
+}
+
+// dX[rows x NIN] = (dY[rows x KOUT] W^T) * (H > 0), one wave per 16 rows; W (backward
+// layout, rows n_in of k_out) staged in LDS.
+template <int KOUT, int NIN>
+__global__ __launch_bounds__(256) void k_guide_bwd(const _Float16* wb, const _Float16* dy, const _Float16* h,
+                                                   uint32_t nrows, _Float16* dx)
+{
+    constexpr int S = KOUT + kPad;
+    __shared__ _Float16 lw[NIN * S];
+    for (int t = threadIdx.x; t < NIN * KOUT; t += 256) lw[(t / KOUT) * S + (t % KOUT)] = wb[t];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+    const uint32_t ntiles = (nrows + 15) / 16;
+    for (uint32_t tile = blockIdx.x * 4 + wave; tile < ntiles; tile += gridDim.x * 4)
+    {
+        uint32_t row0 = tile * 16;
+        uint32_t ra = row0 + r;
+#pragma unroll
+        for (int nt = 0; nt < NIN / 16; ++nt)
+        {
+            f4v acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < KOUT / 16; ++ks)
+            {
+                h4 a = {0, 0, 0, 0};
+                if (ra < nrows) a = *reinterpret_cast<const h4*>(&dy[(size_t)ra * KOUT + ks * 16 + 4 * q]);
+                h4 w = *reinterpret_cast<const h4*>(&lw[(nt * 16 + r) * S + ks * 16 + 4 * q]);
+                acc = mfma16(a, w, acc);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+            {
+                uint32_t row = row0 + 4 * q + i;
+                if (row < nrows)
+                {
+                    size_t o = (size_t)row * NIN + nt * 16 + r;
+                    dx[o] = (float)h[o] > 0.f ? (_Float16)acc[i] : (_Float16)0.f;
+                }
+            }
+        }
+    }
+}
+
+// Partial weight gradients over a chunk of rows: dW[k][n] = sum_rows X[row][k] dY[row][n]
+// (MFMA with the rows as the reduction dimension) and db[n] = sum_rows dY[row][n].
+// part: [gridDim.x][KIN*NOUT + NOUT]
+constexpr int kWgRows = 1024;
+template <int KIN, int NOUT>
+__global__ __launch_bounds__(64) void k_guide_wgrad(const _Float16* x, const _Float16* dy, uint32_t nrows, float* part)
+{
+    __shared__ _Float16 xs[16][KIN + kPad];
+    __shared__ _Float16 ys[16][NOUT + kPad];
+    const int lane = threadIdx.x, r = lane & 15, q = lane >> 4;
+    f4v acc[KIN / 16][NOUT / 16];
+#pragma unroll
+    for (int a = 0; a < KIN / 16; ++a)
+#pragma unroll
+        for (int b = 0; b < NOUT / 16; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;
+    const uint32_t r0 = blockIdx.x * kWgRows;
+    for (uint32_t s0 = r0; s0 < r0 + kWgRows && s0 < nrows; s0 += 16)
+    {
+        for (int t = lane; t < 16 * KIN; t += 64)
+        {
+            uint32_t row = s0 + t / KIN;
+            xs[t / KIN][t % KIN] = row < nrows ? x[(size_t)row * KIN + t % KIN] : (_Float16)0.f;
+        }
+        for (int t = lane; t < 16 * NOUT; t += 64)
+        {
+            uint32_t row = s0 + t / NOUT;
+            ys[t / NOUT][t % NOUT] = row < nrows ? dy[(size_t)row * NOUT + t % NOUT] : (_Float16)0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int mt = 0; mt < KIN / 16; ++mt)
+        {
+            h4 a;   // A[m = k_in][kk = row]: X[row 4q+i][16 mt + r]
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = xs[4 * q + i][mt * 16 + r];
+#pragma unroll
+            for (int nt = 0; nt < NOUT / 16; ++nt)
+            {
+                h4 b;   // B[kk = row][n]: dY[row 4q+i][16 nt + r]
+#pragma unroll
+                for (int i = 0; i < 4; ++i) b[i] = ys[4 * q + i][nt * 16 + r];
+                acc[mt][nt] = mfma16(a, b, acc[mt][nt]);
+            }
+        }
+        if (lane < NOUT)
+            for (int i = 0; i < 16; ++i) bsum += (float)ys[i][lane];
+        __syncthreads();
+    }
+    float* P = part + (size_t)blockIdx.x * (KIN * NOUT + NOUT);
+#pragma unroll
+    for (int mt = 0; mt < KIN / 16; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NOUT / 16; ++nt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)   // D[row m = 16 mt + 4q + i][col n = 16 nt + r]
+                P[(size_t)(mt * 16 + 4 * q + i) * NOUT + nt * 16 + r] = acc[mt][nt][i];
+    if (lane < NOUT) P[KIN * NOUT + lane] = bsum;
+}
+
+// Sums the partial gradients of one layer (chunk order fixed) into grad[W..., b...] and
+// undoes the loss scaling.
+__global__ void k_guide_reduce(const float* part, int nchunks, int n, float inv_rows, float* grad)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float s = 0.f;
+    for (int c = 0; c < nchunks; ++c) s += part[(size_t)c * n + t];
+    grad[t] = s * inv_rows;
+}
+
+__global__ void k_guide_zero(float* p, int n)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0.f;
+}
+
+// Adam on the FP32 master parameters, then refresh the FP16 forward / backward copies.
+__global__ void k_guide_adam(float* p, float* m, float* v, const float* grad, int n, float lr, float b1, float b2,
+                             float eps, float c1, float c2, _Float16* wf, _Float16* wb, float* bias)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float g = grad[t];
+    float mm = b1 * m[t] + (1.f - b1) * g;
+    float vv = b2 * v[t] + (1.f - b2) * g * g;
+    m[t] = mm;
+    v[t] = vv;
+    float x = p[t] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    p[t] = x;
+    // scatter into the FP16 layouts
+    _Float16 hx = (_Float16)x;
+    if (t < kGB1) { int k = t / kGHid, nn = t % kGHid; wf[kF1 + nn * kGIn + k] = hx; }
+    else if (t < kGW2) bias[t - kGB1] = x;
+    else if (t < kGB2) { int u = t - kGW2, k = u / kGHid, nn = u % kGHid; wf[kF2 + nn * kGHid + k] = hx; wb[kB2 + k * kGHid + nn] = hx; }
+    else if (t < kGW3) bias[kGHid + t - kGB2] = x;
+    else if (t < kGB3) { int u = t - kGW3, k = u / kGHid, nn = u % kGHid; wf[kF3 + nn * kGHid + k] = hx; wb[kB3 + k * kGHid + nn] = hx; }
+    else if (t < kGW4) bias[2 * kGHid + t - kGB3] = x;
+    else if (t < kGB4) { int u = t - kGW4, k = u / kGHid, nn = u % kGHid; wf[kF4 + nn * kGHid + k] = hx; wb[kB4 + k * kGHid + nn] = hx; }
+    else if (t < kGW5) bias[3 * kGHid + t - kGB4] = x;
+    else if (t < kGB5) { int u = t - kGW5, k = u / kGOut, nn = u % kGOut; wf[kF5 + nn * kGHid + k] = hx; wb[kB5 + k * kGOut + nn] = hx; }
+    else bias[4 * kGHid + t - kGB5] = x;
+}
+
+__global__ void k_diag_mfma(const _Float16* a, const _Float16* b, float* d)
+{
+    // a: 16x16 row-major (M x K), b: 16x16 row-major (K x N), d: 16x16 row-major
+    const int lane = threadIdx.x, r = lane & 15, q = lane >> 4;
+    h4 af, bf;
+    for (int i = 0; i < 4; ++i)
+    {
+        af[i] = a[r * 16 + 4 * q + i];
+        bf[i] = b[(4 * q + i) * 16 + r];
+    }
+    f4v acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma16(af, bf, acc);
+    for (int i = 0; i < 4; ++i) d[(4 * q + i) * 16 + r] = acc[i];
+}
+
+// Lobe decoding, mixture pdf and Jakob sampling on given network outputs (diagnostic).
+// out row: sampled direction (3), pdf at the sample, pdf at dirs[t], lobe weight sum, 2 unused.
+__global__ void k_diag_guide(uint32_t n, const float* o, const float* u, const float* dirs, float* out)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    Lobes L = guide_lobes(o + (size_t)t * kGOut);
+    f3 w = guide_sample(L, u[3 * t], u[3 * t + 1], u[3 * t + 2]);
+    f3 d = mk(dirs[3 * t], dirs[3 * t + 1], dirs[3 * t + 2]);
+    float ws = 0.f;
+    for (int k = 0; k < kGLobes; ++k) ws += L.w[k];
+    float* r = out + (size_t)t * 8;
+    r[0] = w.x;
+    r[1] = w.y;
+    r[2] = w.z;
+    r[3] = guide_pdf(L, w);
+    r[4] = guide_pdf(L, d);
+    r[5] = ws;
+    r[6] = 0.f;
+    r[7] = 0.f;
+}
+
+}  // namespace bsh
+
+// ---------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------
+using namespace bsh;
+
+struct bsh_guide
+{
+    bsh_context* ctx = nullptr;
+    bsh_guide_params prm{};
+    std::vector<float> host_params;
+    DevBuf<float> p, m, v, grad, bias, part, out, loss, wsum;
+    DevBuf<_Float16> wf, wb, x0, hs, d5, d4, d3, d2, d1;
+    int step = 0;
+    size_t cap = 0;
+    GuideNet net() const { return GuideNet{wf.p, wb.p, bias.p}; }
+};
+
+namespace bsh {
+
+// PCG32 (M. O'Neill, 2014) for the deterministic He initialisation.
+static uint32_t pcg32(uint64_t& st)
+{
+    uint64_t old = st;
+    st = old * 6364136223846793005ULL + 1442695040888963407ULL;
+    uint32_t x = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (x >> rot) | (x << ((32 - rot) & 31));
+}
+
+static bsh_status upload_params(bsh_guide* g)
+{
+    // fp32 master + derived fp16 copies, built on the host in the same layouts k_guide_adam writes
+    const std::vector<float>& P = g->host_params;
+    std::vector<_Float16> wf(kFTotal), wb(kBTotal);
+    std::vector<float> bias(kBiasTotal);
+    for (int k = 0; k < kGIn; ++k)
+        for (int n = 0; n < kGHid; ++n) wf[kF1 + n * kGIn + k] = (_Float16)P[kGW1 + k * kGHid + n];
+    const int wo[3] = {kGW2, kGW3, kGW4}, fo[3] = {kF2, kF3, kF4}, bo[3] = {kB2, kB3, kB4};
+    for (int l = 0; l < 3; ++l)
+        for (int k = 0; k < kGHid; ++k)
+            for (int n = 0; n < kGHid; ++n)
+            {
+                _Float16 h = (_Float16)P[wo[l] + k * kGHid + n];
+                wf[fo[l] + n * kGHid + k] = h;
+                wb[bo[l] + k * kGHid + n] = h;
+            }
+    for (int k = 0; k < kGHid; ++k)
+        for (int n = 0; n < kGOut; ++n)
+        {
+            _Float16 h = (_Float16)P[kGW5 + k * kGOut + n];
+            wf[kF5 + n * kGHid + k] = h;
+            wb[kB5 + k * kGOut + n] = h;
+        }
+    const int bsrc[5] = {kGB1, kGB2, kGB3, kGB4, kGB5};
+    for (int l = 0; l < 4; ++l)
+        for (int n = 0; n < kGHid; ++n) bias[l * kGHid + n] = P[bsrc[l] + n];
+    for (int n = 0; n < kGOut; ++n) bias[4 * kGHid + n] = P[kGB5 + n];
+    BSH_HIP(hipMemcpy(g->p.p, P.data(), kGParams * 4, hipMemcpyHostToDevice));
+    BSH_HIP(hipMemcpy(g->wf.p, wf.data(), kFTotal * 2, hipMemcpyHostToDevice));
+    BSH_HIP(hipMemcpy(g->wb.p, wb.data(), kBTotal * 2, hipMemcpyHostToDevice));
+    BSH_HIP(hipMemcpy(g->bias.p, bias.data(), kBiasTotal * 4, hipMemcpyHostToDevice));
+    return BSH_OK;
+}
+
+static bsh_status ensure_train(bsh_guide* g, size_t rows)
+{
+    if (rows <= g->cap) return BSH_OK;
+    BSH_HIP(g->x0.alloc(rows * kGIn));
+    BSH_HIP(g->hs.alloc(rows * kGHid * 4));
+    BSH_HIP(g->out.alloc(rows * kGOut));
+    BSH_HIP(g->d5.alloc(rows * kGOut));
+    BSH_HIP(g->d4.alloc(rows * kGHid));
+    BSH_HIP(g->d3.alloc(rows * kGHid));
+    BSH_HIP(g->d2.alloc(rows * kGHid));
+    BSH_HIP(g->d1.alloc(rows * kGHid));
+    size_t nchunks = (rows + kWgRows - 1) / kWgRows;
+    BSH_HIP(g->part.alloc(nchunks * (size_t)(kGIn * kGHid + kGHid)));
+    g->cap = rows;
+    return BSH_OK;
+}
+
+float guide_bsdf_fraction(const bsh_guide* g) { return g->prm.bsdf_fraction; }
+
+bsh_status guide_forward(bsh_guide* g, const float4* in, int stride, uint32_t n, float* out, hipStream_t st)
+{
+    if (n == 0) return BSH_OK;
+    uint32_t tiles = (n + 15) / 16;
+    uint32_t blocks = std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 2);
+    hipLaunchKernelGGL(k_guide_forward<false>, dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), in, stride, n, out,
+                       (_Float16*)nullptr, (_Float16*)nullptr);
+    BSH_HIP(hipGetLastError());
+    return BSH_OK;
+}
+
+template <int K, int N>
+static void wgrad_layer(bsh_guide* g, const _Float16* x, const _Float16* dy, uint32_t n, int param_off, hipStream_t st)
+{
+    int nchunks = (int)((n + kWgRows - 1) / kWgRows);
+    hipLaunchKernelGGL((k_guide_wgrad<K, N>), dim3(nchunks), dim3(64), 0, st, x, dy, n, g->part.p);
+    int cnt = K * N + N;
+    hipLaunchKernelGGL(k_guide_reduce, dim3((cnt + 255) / 256), dim3(256), 0, st, g->part.p, nchunks, cnt,
+                       1.f / (float)n, g->grad.p + param_off);
+}
+
+// Forward + backward on `rec` (3 float4 per row); leaves the batch gradient in g->grad
+// and the weighted mean loss (times rows) in g->loss.
+static bsh_status guide_gradients(bsh_guide* g, const float4* rec, uint32_t n, hipStream_t st)
+{
+    bsh_status r = ensure_train(g, n);
+    if (r != BSH_OK) return r;
+    hipLaunchKernelGGL(k_guide_zero, dim3(1), dim3(64), 0, st, g->wsum.p, 1);
+    hipLaunchKernelGGL(k_guide_zero, dim3(1), dim3(64), 0, st, g->loss.p, 1);
+    hipLaunchKernelGGL(k_guide_wsum, dim3(std::min<uint32_t>((n + 255) / 256, 1024)), dim3(256), 0, st, rec, n,
+                       g->wsum.p);
+    uint32_t tiles = (n + 15) / 16;
+    uint32_t blocks = std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 2);
+    hipLaunchKernelGGL(k_guide_forward<true>, dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), rec, 3, n,
+                       g->out.p, g->x0.p, g->hs.p);
+    hipLaunchKernelGGL(k_guide_loss, dim3((n + 255) / 256), dim3(256), 0, st, g->out.p, rec, n, g->wsum.p, g->d5.p,
+                       g->loss.p);
+    const _Float16* H1 = g->hs.p;
+    const _Float16* H2 = g->hs.p + (size_t)n * kGHid;
+    const _Float16* H3 = g->hs.p + (size_t)2 * n * kGHid;
+    const _Float16* H4 = g->hs.p + (size_t)3 * n * kGHid;
+    uint32_t bb = std::min<uint32_t>((tiles + 3) / 4, (uint32_t)g->ctx->num_cus * 8);
+    hipLaunchKernelGGL((k_guide_bwd<kGOut, kGHid>), dim3(bb), dim3(256), 0, st, g->wb.p + kB5, g->d5.p, H4, n, g->d4.p);
+    hipLaunchKernelGGL((k_guide_bwd<kGHid, kGHid>), dim3(bb), dim3(256), 0, st, g->wb.p + kB4, g->d4.p, H3, n, g->d3.p);
+    hipLaunchKernelGGL((k_guide_bwd<kGHid, kGHid>), dim3(bb), dim3(256), 0, st, g->wb.p + kB3, g->d3.p, H2, n, g->d2.p);
+    hipLaunchKernelGGL((k_guide_bwd<kGHid, kGHid>), dim3(bb), dim3(256), 0, st, g->wb.p + kB2, g->d2.p, H1, n, g->d1.p);
+    wgrad_layer<kGHid, kGOut>(g, H4, g->d5.p, n, kGW5, st);
+    wgrad_layer<kGHid, kGHid>(g, H3, g->d4.p, n, kGW4, st);
+    wgrad_layer<kGHid, kGHid>(g, H2, g->d3.p, n, kGW3, st);
+    wgrad_layer<kGHid, kGHid>(g, H1, g->d2.p, n, kGW2, st);
+    wgrad_layer<kGIn, kGHid>(g, g->x0.p, g->d1.p, n, kGW1, st);
+    BSH_HIP(hipGetLastError());
+    return BSH_OK;
+}
+
+}  // namespace bsh
+
+extern "C" {
+
+size_t bsh_guide_num_params(void) { return (size_t)kGParams; }
+
+bsh_status bsh_guide_create(bsh_context* ctx, const bsh_guide_params* prm, bsh_guide** out)
+{
+    if (!ctx || !out) return fail(BSH_ERR_INVALID, "bsh_guide_create: NULL argument");
+    BSH_HIP(hipSetDevice(ctx->device));
+    bsh_guide* g = new bsh_guide();
+    g->ctx = ctx;
+    bsh_guide_params d{1u, 5e-3f, 0.9f, 0.999f, 1e-8f, 0.5f, {0, 0}};
+    g->prm = prm ? *prm : d;
+    g->host_params.assign(kGParams, 0.f);
+    // He-uniform weights (limit sqrt(6 / fan_in)), zero biases, from PCG32(seed)
+    uint64_t st = 0x853c49e6748fea9bULL ^ ((uint64_t)g->prm.seed << 1);
+    auto U = [&]() { return (pcg32(st) >> 8) * (1.f / 16777216.f); };
+    const int wo[5] = {kGW1, kGW2, kGW3, kGW4, kGW5}, fin[5] = {kGIn, kGHid, kGHid, kGHid, kGHid},
+              fout[5] = {kGHid, kGHid, kGHid, kGHid, kGOut};
+    for (int l = 0; l < 5; ++l)
+    {
+        float lim = std::sqrt(6.f / (l == 0 ? 91.f : (float)fin[l]));
+        for (int k = 0; k < fin[l] * fout[l]; ++k) g->host_params[wo[l] + k] = (2.f * U() - 1.f) * lim;
+    }
+    // the padded input rows 91..95 are always zero; keep their weights zero
+    for (int k = 91; k < kGIn; ++k)
+        for (int n = 0; n < kGHid; ++n) g->host_params[kGW1 + k * kGHid + n] = 0.f;
+    bsh_status r = BSH_OK;
+    auto alloc = [&]() -> bsh_status {
+        BSH_HIP(g->p.alloc(kGParams));
+        BSH_HIP(g->m.alloc(kGParams));
+        BSH_HIP(g->v.alloc(kGParams));
+        BSH_HIP(g->grad.alloc(kGParams));
+        BSH_HIP(g->bias.alloc(kBiasTotal));
+        BSH_HIP(g->wf.alloc(kFTotal));
+        BSH_HIP(g->wb.alloc(kBTotal));
+        BSH_HIP(g->loss.alloc(1));
+        BSH_HIP(g->wsum.alloc(1));
+        BSH_HIP(hipMemset(g->m.p, 0, kGParams * 4));
+        BSH_HIP(hipMemset(g->v.p, 0, kGParams * 4));
+        return BSH_OK;
+    };
+    r = alloc();
+    if (r == BSH_OK) r = upload_params(g);
+    if (r != BSH_OK)
+    {
+        delete g;
+        return r;
+    }
+    *out = g;
+    return BSH_OK;
+}
+
+bsh_status bsh_guide_destroy(bsh_guide* g)
+{
+    delete g;
+    return BSH_OK;
+}
+
+bsh_status bsh_guide_get_params(const bsh_guide* g, float* host)
+{
+    if (!g || !host) return fail(BSH_ERR_INVALID, "bsh_guide_get_params: NULL argument");
+    BSH_HIP(hipMemcpy(host, g->p.p, kGParams * 4, hipMemcpyDeviceToHost));
+    return BSH_OK;
+}
+
+bsh_status bsh_guide_set_params(bsh_guide* g, const float* host)
+{
+    if (!g || !host) return fail(BSH_ERR_INVALID, "bsh_guide_set_params: NULL argument");
+    g->host_params.assign(host, host + kGParams);
+    BSH_HIP(hipMemset(g->m.p, 0, kGParams * 4));
+    BSH_HIP(hipMemset(g->v.p, 0, kGParams * 4));
+    g->step = 0;
+    return upload_params(g);
+}
+
+bsh_status bsh_guide_forward(bsh_guide* g, const float* inputs, size_t n, float* out, void* stream)
+{
+    if (!g || (n && (!inputs || !out))) return fail(BSH_ERR_INVALID, "bsh_guide_forward: NULL argument");
+    if (n >= ((size_t)1 << 31)) return fail(BSH_ERR_UNSUPPORTED, "bsh_guide_forward: too many rows");
+    BSH_HIP(hipSetDevice(g->ctx->device));
+    return guide_forward(g, reinterpret_cast<const float4*>(inputs), 2, (uint32_t)n, out, (hipStream_t)stream);
+}
+
+bsh_status bsh_guide_gradient(bsh_guide* g, const float* records, size_t n, float* grad, float* loss)
+{
+    if (!g || !records || n == 0) return fail(BSH_ERR_INVALID, "bsh_guide_gradient: bad argument");
+    if (n >= ((size_t)1 << 30)) return fail(BSH_ERR_UNSUPPORTED, "bsh_guide_gradient: too many rows");
+    BSH_HIP(hipSetDevice(g->ctx->device));
+    bsh_status r = guide_gradients(g, reinterpret_cast<const float4*>(records), (uint32_t)n, nullptr);
+    if (r != BSH_OK) return r;
+    BSH_HIP(hipDeviceSynchronize());
+    if (grad) BSH_HIP(hipMemcpy(grad, g->grad.p, kGParams * 4, hipMemcpyDeviceToHost));
+    if (loss)
+    {
+        float l = 0.f;
+        BSH_HIP(hipMemcpy(&l, g->loss.p, 4, hipMemcpyDeviceToHost));
+        *loss = l / (float)n;
+    }
+    return BSH_OK;
+}
+
+bsh_status bsh_guide_train_step(bsh_guide* g, const float* records, size_t n, float* loss, void* stream)
+{
+    if (!g || !records) return fail(BSH_ERR_INVALID, "bsh_guide_train_step: NULL argument");
+    if (n == 0) return BSH_OK;
+    if (n >= ((size_t)1 << 30)) return fail(BSH_ERR_UNSUPPORTED, "bsh_guide_train_step: too many rows");
+    BSH_HIP(hipSetDevice(g->ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    bsh_status r = guide_gradients(g, reinterpret_cast<const float4*>(records), (uint32_t)n, st);
+    if (r != BSH_OK) return r;
+    g->step++;
+    float c1 = 1.f - std::pow(g->prm.beta1, (float)g->step), c2 = 1.f - std::pow(g->prm.beta2, (float)g->step);
+    hipLaunchKernelGGL(k_guide_adam, dim3((kGParams + 255) / 256), dim3(256), 0, st, g->p.p, g->m.p, g->v.p, g->grad.p,
+                       kGParams, g->prm.lr, g->prm.beta1, g->prm.beta2, g->prm.eps, c1, c2, g->wf.p, g->wb.p,
+                       g->bias.p);
+    BSH_HIP(hipGetLastError());
+    if (loss)
+    {
+        float l = 0.f;
+        BSH_HIP(hipMemcpyAsync(&l, g->loss.p, 4, hipMemcpyDeviceToHost, st));
+        BSH_HIP(hipStreamSynchronize(st));
+        *loss = l / (float)n;
+    }
+    return BSH_OK;
+}
+
+bsh_status bsh_diag_mfma(bsh_context* ctx, const uint16_t* a, const uint16_t* b, float* d)
+{
+    if (!ctx || !a || !b || !d) return fail(BSH_ERR_INVALID, "bsh_diag_mfma: NULL argument");
+    BSH_HIP(hipSetDevice(ctx->device));
+    DevBuf<_Float16> da, db;
+    DevBuf<float> dd;
+    BSH_HIP(da.alloc(256));
+    BSH_HIP(db.alloc(256));
+    BSH_HIP(dd.alloc(256));
+    BSH_HIP(hipMemcpy(da.p, a, 512, hipMemcpyHostToDevice));
+    BSH_HIP(hipMemcpy(db.p, b, 512, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_diag_mfma, dim3(1), dim3(64), 0, 0, da.p, db.p, dd.p);
+    BSH_HIP(hipGetLastError());
+    BSH_HIP(hipMemcpy(d, dd.p, 1024, hipMemcpyDeviceToHost));
+    return BSH_OK;
+}
+
+bsh_status bsh_diag_guide(bsh_context* ctx, size_t n, const float* outputs, const float* u, const float* dirs,
+                          float* out)
+{
+    if (!ctx || (n && (!outputs || !u || !dirs || !out))) return fail(BSH_ERR_INVALID, "bsh_diag_guide: NULL argument");
+    if (n == 0) return BSH_OK;
+    if (n >= ((size_t)1 << 31)) return fail(BSH_ERR_UNSUPPORTED, "bsh_diag_guide: too many rows");
+    BSH_HIP(hipSetDevice(ctx->device));
+    DevBuf<float> d_o, d_u, d_d, d_r;
+    BSH_HIP(d_o.alloc(n * kGOut));
+    BSH_HIP(d_u.alloc(n * 3));
+    BSH_HIP(d_d.alloc(n * 3));
+    BSH_HIP(d_r.alloc(n * 8));
+    BSH_HIP(hipMemcpy(d_o.p, outputs, n * kGOut * 4, hipMemcpyHostToDevice));
+    BSH_HIP(hipMemcpy(d_u.p, u, n * 12, hipMemcpyHostToDevice));
+    BSH_HIP(hipMemcpy(d_d.p, dirs, n * 12, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_diag_guide, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, 0, (uint32_t)n, d_o.p, d_u.p,
+                       d_d.p, d_r.p);
+    BSH_HIP(hipGetLastError());
+    BSH_HIP(hipMemcpy(out, d_r.p, n * 32, hipMemcpyDeviceToHost));
+    return BSH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,114 @@
+// Learned path-guiding distribution: a mixture of 8 von Mises-Fisher lobes whose
+// parameters come from a small MLP of (position, normal, roughness). No reference
+// counterpart; the plain BSDF-sampling tracer stays the fallback and error judge.
+//
+// Network: 7 raw inputs (position normalised to the scene box, shading normal, GGX alpha
+// or 1 for Lambert) -> features [raw 7, sin/cos(2^f pi v) for f = 0..5] = 91, padded to 96
+// -> 4 hidden layers of 64 ReLU units -> 48 outputs of which 40 are used:
+//   o[0..7]   lobe logits (softmax -> weights pi_k)
+//   o[8..31]  lobe mean directions, unnormalised (mu_k = o / |o|)
+//   o[32..39] log concentrations, kappa_k = exp(clamp(o, -2, 6))
+// vMF density (stable form): kappa / (2 pi (1 - e^{-2 kappa})) exp(kappa (mu.w - 1)).
+// Sampling a lobe: W. Jakob, "Numerically stable sampling of the von Mises-Fisher
+// distribution on S^2 (and other tricks)", 2012.
+#pragma once
+
+#include "bsh_math.h"
+
+namespace bsh {
+
+constexpr int kGIn = 96;       // padded encoded input width
+constexpr int kGHid = 64;      // hidden width
+constexpr int kGOut = 48;      // padded output width
+constexpr int kGLobes = 8;
+constexpr int kGRaw = 7;
+constexpr int kGBands = 6;
+constexpr float kLogKappaLo = -2.f, kLogKappaHi = 6.f;
+
+// Parameter layout of the FP32 master vector (weights input-major W[k][n], then bias).
+constexpr int kGW1 = 0, kGB1 = kGW1 + kGIn * kGHid;
+constexpr int kGW2 = kGB1 + kGHid, kGB2 = kGW2 + kGHid * kGHid;
+constexpr int kGW3 = kGB2 + kGHid, kGB3 = kGW3 + kGHid * kGHid;
+constexpr int kGW4 = kGB3 + kGHid, kGB4 = kGW4 + kGHid * kGHid;
+constexpr int kGW5 = kGB4 + kGHid, kGB5 = kGW5 + kGHid * kGOut;
+constexpr int kGParams = kGB5 + kGOut;
+
+// Feature j of the encoded input for raw values v[0..6].
+BSH_HD float guide_feature(const float* v, int j)
+{
+    if (j < kGRaw) return v[j];
+    if (j >= kGRaw + 2 * kGRaw * kGBands) return 0.f;
+    int t = j - kGRaw;
+    int band = t / (2 * kGRaw);
+    int r = t % (2 * kGRaw);
+    int d = r >> 1;
+    float a = (float)(1 << band) * kPi * v[d];
+    return (r & 1) ? cosf(a) : sinf(a);
+}
+
+struct Lobes
+{
+    float w[kGLobes];      // mixture weights
+    f3 mu[kGLobes];
+    float kappa[kGLobes];
+};
+
+BSH_HD Lobes guide_lobes(const float* o)
+{
+    Lobes L;
+    float mx = o[0];
+    for (int k = 1; k < kGLobes; ++k) mx = fmaxf(mx, o[k]);
+    float s = 0.f;
+    for (int k = 0; k < kGLobes; ++k)
+    {
+        L.w[k] = expf(o[k] - mx);
+        s += L.w[k];
+    }
+    for (int k = 0; k < kGLobes; ++k)
+    {
+        L.w[k] = L.w[k] / s;
+        f3 r = mk(o[8 + 3 * k], o[9 + 3 * k], o[10 + 3 * k]);
+        float n = sqrtf(sqnorm(r));
+        L.mu[k] = n > 0.f ? r * (1.f / n) : mk(0.f, 0.f, 1.f);
+        L.kappa[k] = expf(clampf(o[32 + k], kLogKappaLo, kLogKappaHi));
+    }
+    return L;
+}
+
+BSH_HD float vmf_pdf(f3 mu, float kappa, f3 w)
+{
+    return kappa / (2.f * kPi * (1.f - expf(-2.f * kappa))) * expf(kappa * (dot(mu, w) - 1.f));
+}
+
+BSH_HD float guide_pdf(const Lobes& L, f3 w)
+{
+    float p = 0.f;
+    for (int k = 0; k < kGLobes; ++k) p += L.w[k] * vmf_pdf(L.mu[k], L.kappa[k], w);
+    return p;
+}
+
+// Lobe chosen by u0 through the weight CDF, then Jakob's stable vMF sample with (u1, u2).
+BSH_HD f3 guide_sample(const Lobes& L, float u0, float u1, float u2)
+{
+    int k = kGLobes - 1;
+    float c = 0.f;
+    for (int j = 0; j < kGLobes; ++j)
+    {
+        c += L.w[j];
+        if (u0 < c)
+        {
+            k = j;
+            break;
+        }
+    }
+    float kap = L.kappa[k];
+    float w = 1.f + logf(u1 + (1.f - u1) * expf(-2.f * kap)) / kap;
+    w = clampf(w, -1.f, 1.f);
+    float s = sqrtf(fmaxf(0.f, 1.f - w * w));
+    f3 t1 = orthovector(L.mu[k]);
+    f3 t2 = cross(L.mu[k], t1);
+    float phi = 2.f * kPi * u2;
+    return normalize(L.mu[k] * w + t1 * (s * cosf(phi)) + t2 * (s * sinf(phi)));
+}
+
+}  // namespace bsh

@@ -72,6 +72,9 @@ struct RenderWork
     DevBuf<int4> nee_res;
     DevBuf<float4> qc_o[2], qc_d[2], qa_o[2], qa_d[2];   // closest / any-hit queues
     DevBuf<int> qc_own[2], qa_own[2];
+    DevBuf<float4> gin, rec;          // guide network inputs (2 per path), training records (10 per path)
+    DevBuf<float> gout;               // guide network outputs (48 per path)
+    DevBuf<uint32_t> rec_count;
     DevBuf<uint32_t> counters;     // per launch: count, work, (pad) x 4
     DevBuf<unsigned long long> stats;  // closest, any
 };
@@ -96,10 +99,25 @@ struct JobDev
     float tmin;
     float* coeffs;
     float* radiance;
+    int flags;             // JOB_NO_QUIRKS, JOB_GUIDED, JOB_RECORD
+    float bsdf_frac;       // guided: probability of sampling the BSDF
+    float glo[3], ginv[3]; // guided: scene box for the network's position input
 };
 
+enum : int { JOB_NO_QUIRKS = 1, JOB_GUIDED = 2, JOB_RECORD = 4 };
+
+struct GuideRun
+{
+    bsh_guide* guide = nullptr;   // network used by JOB_GUIDED jobs
+    float* records = nullptr;     // training records out (12 floats each)
+    size_t cap = 0;
+    size_t written = 0;
+};
+bsh_status guide_forward(bsh_guide* g, const float4* in, int stride, uint32_t n, float* out, hipStream_t st);
+float guide_bsdf_fraction(const bsh_guide* g);
+
 bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& jobs, bool count, const bsh_tuning* tune, int use_graph,
-                    hipStream_t st);
+                    hipStream_t st, GuideRun* gr = nullptr);
 }  // namespace bsh
 
 struct bsh_context
@@ -131,6 +149,7 @@ struct bsh_scene
     bsh::DevBuf<int4> d_shade;
     bsh::DevBuf<bsh::LightDev> d_lights;
     std::vector<bsh::LightDev> h_lights;
+    float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {0.f, 0.f, 0.f};
     uint64_t nnodes = 0;
     int max_depth = 0;
     double build_ms = 0.0;

@@ -35,6 +35,7 @@
 #include "internal.h"
 #include "shading.h"
 #include "sh_dev.h"
+#include "guide_dev.h"
 
 namespace bsh {
 
@@ -66,6 +67,12 @@ struct RenderDev
     int *qc_own[2], *qa_own[2];
     uint32_t* counters;            // per launch L: closest count, closest work, any count, any work
     unsigned long long* stats;     // closest, any, node visits, triangle tests
+    float4* gin;                   // guide inputs, 2 per path
+    const float* gout;             // guide outputs, 48 per path
+    float4* rec;                   // per-path training records: 2 x 5 float4
+    float* rec_out;                // appended records, 12 floats each
+    uint32_t rec_cap;
+    uint32_t* rec_count;
 };
 
 __device__ __forceinline__ uint32_t rand_pick_counter(int b) { return (uint32_t)((b - 1) + (b > 4 ? b - 4 : 0)); }
@@ -97,6 +104,11 @@ __device__ __forceinline__ void init_path(const RenderDev& R, uint32_t i, f3 o, 
     R.qc_o[0][i] = st3(o, tmin);
     R.qc_d[0][i] = st3(d, tmax);
     R.qc_own[0][i] = (int)((Q_EXT << 29) | i);
+    if (R.rec)
+    {
+        R.rec[(size_t)i * 10 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+        R.rec[(size_t)i * 10 + 7] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
 }
 
 // Camera raygen for pixels [u0, u0+nu) of camera job `job`, written at chunk offset c0:
@@ -165,6 +177,36 @@ __global__ __launch_bounds__(kBlock) void k_raygen_probe(RenderDev R, uint32_t j
 __device__ __forceinline__ uint32_t path_of(const RenderDev& R, uint32_t kind, uint32_t idx)
 {
     return kind == Q_EXT ? idx : (idx >> R.Kshift);
+}
+
+// Network inputs of every path whose extension ray hit something (guided jobs): position in
+// the scene box, roughness (GGX alpha, 1 otherwise), shading normal facing the viewer.
+__global__ __launch_bounds__(kBlock) void k_guide_inputs(RenderDev R)
+{
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= R.P) return;
+    uint4 meta = R.meta[i];
+    const JobDev& J = R.jobs[meta.y >> 24];
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), c = make_float4(0.f, 0.f, 0.f, 0.f);
+    if ((J.flags & JOB_GUIDED) && (meta.y & F_EXT) && !(meta.y & F_DONE))
+    {
+        int4 hr = R.hit[i];
+        if (hr.x >= 0)
+        {
+            f3 p1, p2, p3, n1, n2, n3;
+            int mat, bl;
+            tri_verts(J.s, hr.x, p1, p2, p3, n1, n2, n3, mat, bl);
+            Frame h = fill_hit(p1, p2, p3, n1, n2, n3, __int_as_float(hr.z), __int_as_float(hr.w));
+            Mat m = load_mat(J.s, mat);
+            f3 wo = -ld3(R.ray_d[i]);
+            f3 n = dot(h.n, wo) < 0.f ? -h.n : h.n;
+            a = make_float4((h.p.x - J.glo[0]) * J.ginv[0], (h.p.y - J.glo[1]) * J.ginv[1],
+                            (h.p.z - J.glo[2]) * J.ginv[2], m.type == 1 ? m.alpha : 1.f);
+            c = make_float4(n.x, n.y, n.z, 1.f);
+        }
+    }
+    R.gin[2 * i] = a;
+    R.gin[2 * i + 1] = c;
 }
 
 template <bool kCount>
@@ -237,6 +279,9 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
     return v;
 }
 
+// kFull = false compiles only the reference estimator (no quirk-free, guided or record
+// code), keeping the plain path's registers low.
+template <bool kFull>
 __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -254,9 +299,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
         const JobDev& J = R.jobs[job];
         const SceneDev S = J.s;
         const int nl = S.nlights;
+        const int jflags = kFull ? J.flags : 0;
+        const bool nq = (jflags & JOB_NO_QUIRKS) != 0;
         if (!(fl & F_DONE))
         {
             f3 rad = ld3(R.rad[i]);
+            const bool had_nee = (fl & F_NEE) != 0;
             // ---- 1. resolve the NEE rays issued at bounce pb (DiTracer::GetDi tail) ----
             if (fl & F_NEE)
             {
@@ -281,7 +329,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                             f3 p1, p2, p3, n1, n2, n3;
                             int mat, bl;
                             tri_verts(S, r.x, p1, p2, p3, n1, n2, n3, mat, bl);
-                            if (bl == Lid)   // shadowhit.bundle->GetAreaLight() == &light, ditracer.cpp:202
+                            // shadowhit.bundle->GetAreaLight() == &light (ditracer.cpp:202); quirk-free:
+                            // the hit must be this light's own triangle
+                            bool mine = nq ? (r.x == S.lights[Lid].tri) : (bl == Lid);
+                            if (mine)
                             {
                                 Frame hh = fill_hit(p1, p2, p3, n1, n2, n3, __int_as_float(r.z), __int_as_float(r.w));
                                 f3 wi = ld3(R.nee_wi[ks + 1]);
@@ -291,7 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                                 {
                                     float d2inv = 1.f / sqnorm(d);
                                     Mat lm = load_mat(S, mat);
-                                    if (lm.type == 2) le = emissive_le(lm.c, hh.n, -wi) * d2inv;
+                                    if (lm.type == 2) le = emissive_le(lm.c, hh.n, -wi) * (nq ? 1.f : d2inv);
                                     else le = mk(0.f, 0.f, 0.f) * d2inv;  // Material::GetLe default
                                 }
                                 if (sqnorm(le) > 0.f)
@@ -307,6 +358,13 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                 }
                 fl &= ~F_NEE;
             }
+            // training record of the vertex shaded in the previous pass: radiance so far
+            if ((jflags & JOB_RECORD) && b >= 1 && b <= 2)
+            {
+                float4 v = R.rec[(size_t)i * 10 + (b - 1) * 5 + 2];
+                if (v.w > 0.f) R.rec[(size_t)i * 10 + (b - 1) * 5 + 4] = st3(rad, 1.f);
+            }
+            (void)had_nee;
             // ---- 2. shade the extension hit (GiTracer bounce loop body) ----
             if (fl & F_EXT)
             {
@@ -348,6 +406,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                         rad = rad + emissive_le(m.c, h.n, wo);   // gitracer.cpp:61-68
                         fl |= F_TERM;
                     }
+                    else if (nq && m.type == 2)
+                    {
+                        fl |= F_TERM;   // quirk-free: emitters do not scatter; NEE counted their light
+                    }
                     else
                     {
                         org = h.p;
@@ -369,15 +431,26 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                                 size_t ks = (size_t)i * K + 2 * j;
                                 // light sample + shadow ray
                                 f3 ld;
-                                float lpdf = 0.f;
-                                f3 le = light_sample(S, L, h.p, lx, ly, ld, lpdf);
-                                if (lpdf > kMinPdf && sqnorm(le) > 0.f)
+                                float lpdf = 0.f, cosl = 0.f;
+                                f3 le = light_sample(S, L, h.p, lx, ly, ld, lpdf, &cosl);
+                                if ((nq ? lpdf > 0.f : lpdf > kMinPdf) && sqnorm(le) > 0.f)
                                 {
                                     f3 wi = normalize(ld);
                                     float dist = sqrtf(sqnorm(ld));
                                     f3 f = mat_eval(m, h, wi, wo);
-                                    float bp = mat_pdf(m, h, wi, wo);
-                                    float w = power_heuristic(lpdf, bp);
+                                    // the reference asks GetPdf(light dir, view), the density of the
+                                    // view direction; MIS needs the density of sampling the light
+                                    // direction from the view, GetPdf(view, light dir)
+                                    float bp = nq ? mat_pdf(m, h, wo, wi) : mat_pdf(m, h, wi, wo);
+                                    // quirk-free: both MIS weights in solid angle (the reference
+                                    // weighs the area pdf against the BSDF's solid-angle pdf)
+                                    float lw = nq ? lpdf * sqnorm(ld) / cosl : lpdf;
+                                    float w = power_heuristic(lw, bp);
+                                    // GetLe / d^2 is the reference's whole area-to-solid-angle factor,
+                                    // which treats the emitter as uniform radiance e; the emitter's
+                                    // radiance is GetLe = e cos_l (what the camera and the BSDF branch
+                                    // see), so the quirk-free estimator adds the Jacobian's cos_l
+                                    if (nq) le = le * cosl;
                                     f3 c = le * f * fabsf(dot(h.n, wi)) * w * (1.f / lpdf);
                                     R.nee_a[ks] = st3(c, 0.f);
                                     R.nee_wi[ks] = st3(wi, dist - 0.01f);
@@ -389,10 +462,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                                 int type = 0;
                                 f3 fb = mat_sample(m, h, bx, by, wo, wis, bpdf, type);
                                 f3 wi = normalize(wis);
-                                if (sqnorm(fb) > 0.f && bpdf > kMinPdf)
+                                if (sqnorm(fb) > 0.f && (nq ? bpdf > 0.f : bpdf > kMinPdf))
                                 {
                                     float lp = light_pdf(S, L, h.p, wi);
-                                    if (!(lp < kMinPdf))
+                                    if (nq ? lp > 0.f : !(lp < kMinPdf))
                                     {
                                         float w = power_heuristic(bpdf, lp);
                                         R.nee_a[ks + 1] = st3(fb, fabsf(dot(h.n, wi)));
@@ -414,8 +487,44 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                             f3 wi;
                             float pdf = 0.f;
                             int type = 0;
-                            f3 f = mat_sample(m, h, ux, uy, wo, wi, pdf, type);
-                            if (sqnorm(f) == 0.f || pdf < kMinPdf)
+                            f3 f;
+                            bool stop;
+                            if (nq)
+                            {
+                                // quirk-free continuation: a fresh sample per bounce and, for guided
+                                // jobs, the one-sample MIS mixture alpha BSDF + (1 - alpha) guide; the
+                                // integrand is always Evaluate(view, dir) with pdf GetPdf(view, dir)
+                                const uint32_t c6 = 6u * (uint32_t)b;
+                                float gx, gy;
+                                ks_sample2d(base, KS_GUIDE, c6, gx, gy);
+                                bool guided = (jflags & JOB_GUIDED) != 0;
+                                float alpha = guided ? J.bsdf_frac : 1.f;
+                                float usel = ks_float(ks_u32(base, KS_GUIDE, c6 + 2));
+                                Lobes G;
+                                if (guided) G = guide_lobes(R.gout + (size_t)i * kGOut);
+                                if (!guided || usel < alpha)
+                                {
+                                    (void)mat_sample(m, h, gx, gy, wo, wi, pdf, type);
+                                    wi = normalize(wi);
+                                }
+                                else
+                                {
+                                    float u0 = ks_float(ks_u32(base, KS_GUIDE, c6 + 3));
+                                    float u1 = ks_float(ks_u32(base, KS_GUIDE, c6 + 4));
+                                    float u2 = ks_float(ks_u32(base, KS_GUIDE, c6 + 5));
+                                    wi = guide_sample(G, u0, u1, u2);
+                                }
+                                f = mat_eval(m, h, wo, wi);
+                                float pb = mat_pdf(m, h, wo, wi);
+                                pdf = alpha * pb + (guided ? (1.f - alpha) * guide_pdf(G, wi) : 0.f);
+                                stop = !(pdf > 0.f) || sqnorm(f) == 0.f;
+                            }
+                            else
+                            {
+                                f = mat_sample(m, h, ux, uy, wo, wi, pdf, type);
+                                stop = sqnorm(f) == 0.f || pdf < kMinPdf;
+                            }
+                            if (stop)
                             {
                                 fl |= F_TERM;
                             }
@@ -423,6 +532,19 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                             {
                                 f3 nd = normalize(wi);
                                 T = T * (f * (fabsf(dot(h.n, wi)) / pdf));
+                                if ((jflags & JOB_RECORD) && b <= 1)
+                                {
+                                    // training record: network input, sampled direction, mixture pdf,
+                                    // throughput after this vertex
+                                    f3 nn = dot(h.n, wo) < 0.f ? -h.n : h.n;
+                                    float4* rc = R.rec + (size_t)i * 10 + b * 5;
+                                    rc[0] = make_float4((h.p.x - J.glo[0]) * J.ginv[0], (h.p.y - J.glo[1]) * J.ginv[1],
+                                                        (h.p.z - J.glo[2]) * J.ginv[2], m.type == 1 ? m.alpha : 1.f);
+                                    rc[1] = make_float4(nn.x, nn.y, nn.z, 1.f);
+                                    rc[2] = make_float4(nd.x, nd.y, nd.z, pdf);
+                                    rc[3] = st3(T, 0.f);
+                                    rc[4] = make_float4(0.f, 0.f, 0.f, 0.f);
+                                }
                                 bool alive = true;
                                 if (b > 3)
                                 {
@@ -458,7 +580,32 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
             if ((fl & F_TERM) && !(fl & F_NEE))
             {
                 f3 o = rad;
-                if (!(fl & F_NOCLAMP))
+                if (jflags & JOB_RECORD)
+                {
+                    for (int k = 0; k < 2; ++k)
+                    {
+                        float4* rc = R.rec + (size_t)i * 10 + k * 5;
+                        float4 d = rc[2], snap = rc[4];
+                        if (!(d.w > 0.f) || !(snap.w > 0.f)) continue;
+                        float4 T = rc[3];
+                        f3 L = mk(T.x > 0.f ? (rad.x - snap.x) / T.x : 0.f, T.y > 0.f ? (rad.y - snap.y) / T.y : 0.f,
+                                  T.z > 0.f ? (rad.z - snap.z) / T.z : 0.f);
+                        float lum = 0.2126f * L.x + 0.7152f * L.y + 0.0722f * L.z;
+                        if (lum > 0.f)
+                        {
+                            uint32_t slot = atomicAdd(R.rec_count, 1u);
+                            if (slot < R.rec_cap)
+                            {
+                                float4* q = reinterpret_cast<float4*>(R.rec_out) + (size_t)slot * 3;
+                                q[0] = rc[0];
+                                q[1] = rc[1];
+                                q[2] = make_float4(d.x, d.y, d.z, lum / d.w);
+                            }
+                        }
+                        rc[2].w = 0.f;
+                    }
+                }
+                if (!(fl & F_NOCLAMP) && !nq)
                 {
                     f3 v = rad * (1.f / 1.f);
                     o = mk(clampf(v.x, 0.f, kMaxRadiance), clampf(v.y, 0.f, kMaxRadiance),
@@ -727,7 +874,7 @@ static dim3 grid_for(size_t n) { return dim3((uint32_t)std::max<size_t>(1, (n + 
 // (the form captured into a hipGraph).
 static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<Chunk>& chunks,
                                 const std::vector<JobDev>& hjobs, bool count, const bsh_tuning& tune, hipStream_t st,
-                                LaunchTimer* T)
+                                LaunchTimer* T, GuideRun* gr)
 {
     RenderWork& w = c->work;
     const int trace_blocks = c->num_cus * std::max(1, tune.trace_blocks_per_cu);
@@ -735,6 +882,8 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
     {
         const int launches = ch.depth + 1;
         R.P = ch.P;
+        bool full = false;
+        for (const Segment& sg : ch.segs) full = full || hjobs[sg.job].flags != 0;
         hipLaunchKernelGGL(k_zero, grid_for(4 * launches), dim3(kBlock), 0, st, w.counters.p, 4 * launches);
         for (const Segment& sg : ch.segs)
         {
@@ -753,7 +902,16 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
             else
                 hipLaunchKernelGGL(k_trace_queue<false>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1);
             if (T) BSH_HIP(hipEventRecord(T->next(), st));
-            hipLaunchKernelGGL(k_shade, pg, dim3(kBlock), 0, st, R, b, b + 1 <= ch.depth ? b + 1 : b);
+            if (gr && gr->guide && b < ch.depth)
+            {
+                hipLaunchKernelGGL(k_guide_inputs, pg, dim3(kBlock), 0, st, R);
+                bsh_status gs = guide_forward(gr->guide, R.gin, 2, ch.P, c->work.gout.p, st);
+                if (gs != BSH_OK) return gs;
+            }
+            if (full)
+                hipLaunchKernelGGL(k_shade<true>, pg, dim3(kBlock), 0, st, R, b, b + 1 <= ch.depth ? b + 1 : b);
+            else
+                hipLaunchKernelGGL(k_shade<false>, pg, dim3(kBlock), 0, st, R, b, b + 1 <= ch.depth ? b + 1 : b);
             if (T) BSH_HIP(hipEventRecord(T->next(), st));
         }
         for (const Segment& sg : ch.segs)
@@ -788,7 +946,6 @@ bsh_status make_render_job(bsh_scene* s, const bsh_render_params* p, float* accu
     if (!s || !p || !accum) return fail(BSH_ERR_INVALID, "bsh_render: NULL argument");
     if (!s->committed) return fail(BSH_ERR_STATE, "bsh_render: scene not committed");
     if (!s->has_cam) return fail(BSH_ERR_STATE, "bsh_render: no camera");
-    if (p->flags & BSH_RENDER_NO_QUIRKS) return fail(BSH_ERR_UNSUPPORTED, "bsh_render: quirk-free mode not built");
     if (p->width <= 0 || p->height <= 0 || p->img_grid <= 0 || p->max_depth <= 0 || p->max_depth > 250)
         return fail(BSH_ERR_INVALID, "bsh_render: bad frame parameters");
     if (s->lights.size() > (size_t)kMaxLights) return fail(BSH_ERR_UNSUPPORTED, "bsh_render: more than 16 area lights");
@@ -821,6 +978,13 @@ bsh_status make_render_job(bsh_scene* s, const bsh_render_params* p, float* accu
     J.accumulate = (p->flags & BSH_RENDER_ACCUMULATE) ? 1 : 0;
     J.accum = accum;
     J.samples = samples;
+    J.flags = (p->flags & BSH_RENDER_NO_QUIRKS) ? JOB_NO_QUIRKS : 0;
+    for (int k = 0; k < 3; ++k)
+    {
+        J.glo[k] = s->box_lo[k];
+        float e = s->box_hi[k] - s->box_lo[k];
+        J.ginv[k] = e > 0.f ? 1.f / e : 0.f;
+    }
     return BSH_OK;
 }
 
@@ -855,7 +1019,7 @@ bsh_status make_probe_job(bsh_scene* s, const float* probes, size_t nprobes, con
 }
 
 bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, const bsh_tuning* tune_in, int use_graph,
-                    hipStream_t st)
+                    hipStream_t st, GuideRun* gr)
 {
     if (hjobs.empty()) return BSH_OK;
     if (hjobs.size() > 256) return fail(BSH_ERR_UNSUPPORTED, "at most 256 jobs per batch");
@@ -892,6 +1056,18 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     BSH_TRY(ensure_work(c, P, K, nlmax));
     RenderWork& w = c->work;
     BSH_HIP(w.counters.alloc((size_t)4 * (maxdepth + 1)));
+    const bool guided = gr && gr->guide;
+    const bool record = gr && gr->records && gr->cap > 0;
+    if (guided)
+    {
+        BSH_HIP(w.gin.alloc(2 * P));
+        BSH_HIP(w.gout.alloc((size_t)kGOut * P));
+    }
+    if (record)
+    {
+        BSH_HIP(w.rec.alloc(10 * P));
+        BSH_HIP(w.rec_count.alloc(1));
+    }
     BSH_HIP(c->jobs.alloc(hjobs.size()));
     BSH_HIP(hipMemcpyAsync(c->jobs.p, hjobs.data(), hjobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, st));
 
@@ -909,6 +1085,13 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     R.counters = w.counters.p;
     R.stats = w.stats.p;
     R.P = 0;
+    R.gin = guided ? w.gin.p : nullptr;
+    R.gout = guided ? w.gout.p : nullptr;
+    R.rec = record ? w.rec.p : nullptr;
+    R.rec_out = record ? gr->records : nullptr;
+    R.rec_cap = record ? (uint32_t)std::min<size_t>(gr->cap, 0xffffffffu) : 0u;
+    R.rec_count = record ? w.rec_count.p : nullptr;
+    if (record) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, w.rec_count.p, 1);
 
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, st, (uint32_t*)w.stats.p, 8);
     bsh_render_stats& S = c->stats;
@@ -932,7 +1115,7 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess)
         {
-            rc = enqueue_batch(c, R, chunks, hjobs, count, tune, cs, nullptr);
+            rc = enqueue_batch(c, R, chunks, hjobs, count, tune, cs, nullptr, gr);
             hipError_t e2 = hipStreamEndCapture(cs, &graph);
             if (rc == BSH_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
         }
@@ -973,7 +1156,7 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         LaunchTimer T;
         BSH_TRY(T.init(pairs));
         BSH_HIP(hipEventRecord(T.next(), st));
-        BSH_TRY(enqueue_batch(c, R, chunks, hjobs, count, tune, st, &T));
+        BSH_TRY(enqueue_batch(c, R, chunks, hjobs, count, tune, st, &T, gr));
         BSH_HIP(hipEventRecord(T.next(), st));
         BSH_HIP(hipEventSynchronize(T.evs[T.ei - 1]));
         for (size_t k = 1; k + 2 < T.ei; k += 3)
@@ -997,6 +1180,12 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     S.node_visits = count ? st2[2] : 0;
     S.tri_tests = count ? st2[3] : 0;
     S.camera_samples = samples;
+    if (record)
+    {
+        uint32_t nrec = 0;
+        BSH_HIP(hipMemcpy(&nrec, w.rec_count.p, 4, hipMemcpyDeviceToHost));
+        gr->written = std::min<size_t>(nrec, gr->cap);
+    }
     return BSH_OK;
 }
 

@@ -48,7 +48,8 @@ __device__ __forceinline__ float light_pdf(const SceneDev& s, const LightDev& L,
 }
 
 // AreaLight::GetSample -> Mesh::GetSampleOnShape/FillSample (arealight.cpp:25-53, mesh.cpp:99-123,323-336)
-__device__ __forceinline__ f3 light_sample(const SceneDev& s, const LightDev& L, f3 p, float ux, float uy, f3& d, float& pdf)
+__device__ __forceinline__ f3 light_sample(const SceneDev& s, const LightDev& L, f3 p, float ux, float uy, f3& d, float& pdf,
+                                          float* cos_l = nullptr)
 {
     f3 p1, p2, p3, n1, n2, n3;
     int mat, bl;
@@ -59,7 +60,9 @@ __device__ __forceinline__ f3 light_sample(const SceneDev& s, const LightDev& L,
     f3 sp = w * p1 + a * p2 + b * p3;
     f3 sn = normalize(w * n1 + a * n2 + b * n3);
     d = sp - p;
-    if (L.pdf > 0.f && dot(sn, -normalize(d)) > 0.f)
+    float cl = dot(sn, -normalize(d));
+    if (cos_l) *cos_l = cl;
+    if (L.pdf > 0.f && cl > 0.f)
     {
         pdf = L.pdf;
         float d2inv = 1.f / sqnorm(d);

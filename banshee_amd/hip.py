@@ -78,6 +78,12 @@ class Job(ctypes.Structure):
                 ("probes", c_void_p), ("nprobes", c_size_t), ("coeffs", c_void_p), ("radiance", c_void_p)]
 
 
+class GuideParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint32), ("lr", ctypes.c_float), ("beta1", ctypes.c_float),
+                ("beta2", ctypes.c_float), ("eps", ctypes.c_float), ("bsdf_fraction", ctypes.c_float),
+                ("reserved", ctypes.c_int32 * 2)]
+
+
 class RenderStats(ctypes.Structure):
     _fields_ = [("rays_closest", ctypes.c_uint64), ("rays_any", ctypes.c_uint64),
                 ("camera_samples", ctypes.c_uint64), ("node_visits", ctypes.c_uint64),
@@ -95,6 +101,8 @@ EXPORTS = [
     "bsh_diag_bsdf", "bsh_diag_light", "bsh_diag_camera", "bsh_diag_keyed",
     "bsh_bake_sh", "bsh_sh_project_env", "bsh_sh_irradiance", "bsh_diag_sh_eval",
     "bsh_batch_run", "bsh_context_set_tuning", "bsh_context_get_tuning", "bsh_context_stats_get", "bsh_autotune",
+    "bsh_guide_num_params", "bsh_guide_create", "bsh_guide_destroy", "bsh_guide_get_params", "bsh_guide_set_params",
+    "bsh_guide_forward", "bsh_guide_gradient", "bsh_guide_train_step", "bsh_render_guided", "bsh_diag_mfma", "bsh_diag_guide",
 ]
 
 _lib = None
@@ -109,7 +117,9 @@ def lib():
         L.bsh_last_error.restype = ctypes.c_char_p
         for name in EXPORTS:
             f = getattr(L, name)
-            if name != "bsh_last_error":
+            if name == "bsh_guide_num_params":
+                f.restype = c_size_t
+            elif name != "bsh_last_error":
                 f.restype = c_int
         _lib = L
     return _lib
@@ -237,6 +247,24 @@ class Context:
         self.sync()
         return do.numpy()
 
+    def diag_mfma(self, a, b):
+        """One 16x16x16 FP16 MFMA tile on the device: returns a @ b in FP32 (a, b float16 16x16)."""
+        a = np.ascontiguousarray(a, np.float16).view(np.uint16)
+        b = np.ascontiguousarray(b, np.float16).view(np.uint16)
+        d = np.zeros((16, 16), np.float32)
+        _check(lib().bsh_diag_mfma(self.h, _p(a), _p(b), _p(d)), "bsh_diag_mfma")
+        return d
+
+    def diag_guide(self, outputs, u, dirs):
+        """Device lobe decoding / mixture pdf / vMF sampling on given outputs (n x 48).
+        Returns n x 8: sample xyz, pdf(sample), pdf(dirs), sum of lobe weights."""
+        o = np.ascontiguousarray(outputs, np.float32).reshape(-1, 48)
+        u = np.ascontiguousarray(u, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros((o.shape[0], 8), np.float32)
+        _check(lib().bsh_diag_guide(self.h, c_size_t(o.shape[0]), _p(o), _p(u), _p(d), _p(out)), "bsh_diag_guide")
+        return out
+
     # ---- batching ----
     def tuning(self) -> dict:
         t = Tuning()
@@ -346,6 +374,75 @@ class BatchJob:
             a = self.accum.numpy()
             return a[..., 0:3], a[..., 4:7], a[..., 3], (self.samples.numpy() if self.samples else None)
         return self.coeffs.numpy(), (self.radiance.numpy() if self.radiance else None)
+
+
+RECORD_FLOATS = 12
+GUIDE_OUT = 48
+
+
+class Guide:
+    """The path-guiding network (vMF mixture MLP) owned by a Context."""
+
+    def __init__(self, ctx: Context, seed=1, lr=5e-3, beta1=0.9, beta2=0.99, eps=1e-8, bsdf_fraction=0.5):
+        self.ctx = ctx
+        self.h = c_void_p()
+        p = GuideParams()
+        p.seed, p.lr, p.beta1, p.beta2, p.eps, p.bsdf_fraction = seed, lr, beta1, beta2, eps, bsdf_fraction
+        _check(lib().bsh_guide_create(ctx.h, ctypes.byref(p), ctypes.byref(self.h)), "bsh_guide_create")
+
+    @staticmethod
+    def num_params() -> int:
+        return int(lib().bsh_guide_num_params())
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.num_params(), np.float32)
+        _check(lib().bsh_guide_get_params(self.h, _p(out)), "bsh_guide_get_params")
+        return out
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.num_params()
+        _check(lib().bsh_guide_set_params(self.h, _p(w)), "bsh_guide_set_params")
+
+    def forward(self, inputs):
+        """inputs: n x 8 raw rows (position in [0,1]^3, roughness, normal, unused) -> n x 48 outputs."""
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 8)
+        n = x.shape[0]
+        dx = self.ctx.to_device(x)
+        do = self.ctx.alloc((n, GUIDE_OUT), np.float32)
+        _check(lib().bsh_guide_forward(self.h, dx.ptr, c_size_t(n), do.ptr, c_void_p(None)), "bsh_guide_forward")
+        self.ctx.sync()
+        return do.numpy()
+
+    def gradient(self, records):
+        """records: n x 12 (host array or DeviceArray) -> (gradient float32[num_params], loss)."""
+        dr = records if isinstance(records, DeviceArray) else self.ctx.to_device(
+            np.ascontiguousarray(records, np.float32).reshape(-1, RECORD_FLOATS))
+        n = dr.shape[0]
+        g = np.zeros(self.num_params(), np.float32)
+        loss = ctypes.c_float()
+        _check(lib().bsh_guide_gradient(self.h, dr.ptr, c_size_t(n), _p(g), ctypes.byref(loss)), "bsh_guide_gradient")
+        return g, loss.value
+
+    def train_step(self, records, n=None):
+        dr = records if isinstance(records, DeviceArray) else self.ctx.to_device(
+            np.ascontiguousarray(records, np.float32).reshape(-1, RECORD_FLOATS))
+        n = dr.shape[0] if n is None else n
+        loss = ctypes.c_float()
+        _check(lib().bsh_guide_train_step(self.h, dr.ptr, c_size_t(n), ctypes.byref(loss), c_void_p(None)),
+               "bsh_guide_train_step")
+        return loss.value
+
+    def close(self):
+        if self.h:
+            lib().bsh_guide_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def camera_struct(cam13) -> Camera:
@@ -486,6 +583,29 @@ class Scene:
             return None
         self.ctx.sync()
         return coeffs.numpy(), (rad.numpy() if rad else None)
+
+    def render_guided(self, width, height, img_grid, max_depth, seed, guide: "Guide | None" = None, crop=None,
+                      spp_range=None, chunk_spp=0, accum: DeviceArray | None = None, records: DeviceArray | None = None,
+                      download=True):
+        """Quirk-free render whose continuation samples the BSDF/guide mixture. Returns
+        (sum, sumsq, count, nrecords) when download, else nrecords."""
+        p = render_params(width, height, img_grid, max_depth, seed, crop=crop, spp_range=spp_range,
+                          chunk_spp=chunk_spp)
+        p.flags |= 2
+        cw = crop[2] if crop else width
+        ch = crop[3] if crop else height
+        if accum is None:
+            accum = self.ctx.alloc((ch, cw, 8), np.float32)
+        nrec = c_size_t(0)
+        cap = records.shape[0] if records is not None else 0
+        _check(lib().bsh_render_guided(self.h, ctypes.byref(p), guide.h if guide else None, accum.ptr,
+                                       records.ptr if records is not None else None, c_size_t(cap),
+                                       ctypes.byref(nrec), c_void_p(None)), "bsh_render_guided")
+        if not download:
+            return nrec.value
+        self.ctx.sync()
+        a = accum.numpy()
+        return a[..., 0:3], a[..., 4:7], a[..., 3], nrec.value
 
     def stats(self) -> dict:
         s = RenderStats()

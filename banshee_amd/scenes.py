@@ -355,3 +355,88 @@ def probe_grid(scene: Scene, n=16, margin=0.05, lift=0.5):
     zs = lo[2] + ext[2] * (margin + (1 - 2 * margin) * t)
     P = np.stack(np.meshgrid(xs, ys, zs, indexing="ij"), -1).reshape(-1, 3)
     return P.astype(np.float32)
+
+
+def lit_floor(albedo=0.5, light_radiance=10.0, target=(0.3, 0.0, 0.2), fovy=1e-3, width=64, height=64) -> Scene:
+    """A Lambert floor (y = 0, 20 x 20) under one emissive triangle facing down at y = 1.5,
+    seen through a very narrow camera aimed at `target`, so every camera sample lands within
+    a few millimetres of that point. With depth 1 the image estimates the reflected direct
+    light there, which has a closed form (quadrature over the emitter)."""
+    B = _Builder()
+    B.quad([[-10, 0, -10], [-10, 0, 10], [10, 0, 10], [10, 0, -10]], [0, 1, 0],
+           B.material(LAMBERT, (albedo, albedo, albedo)))
+    em = B.material(EMISSIVE, (light_radiance,) * 3)
+    lt = np.asarray([[-0.5, 1.5, -0.5], [0.5, 1.5, -0.5], [0.1, 1.5, 0.6]], np.float32)
+    lb = B.bundle(lt, np.tile([0, -1, 0], (3, 1)), [[0, 1, 2]], em)
+    B.area_lights(lb)
+    cam = [0.0, 1.0, 3.0, target[0], target[1], target[2], 0, 1, 0, 0.01, 10000.0, fovy, width / height]
+    return B.build(cam, "lit_floor", {"light": lt, "albedo": albedo, "radiance": light_radiance})
+
+
+def _grid_quad(B, origin, du, dv, nu, nv, normal, mat, keep=None):
+    """A planar quad origin + s du + t dv, s, t in [0,1], split into nu x nv cells (2 triangles
+    each); keep[nu, nv] drops the cells where it is False (holes)."""
+    o = np.asarray(origin, np.float64)
+    du = np.asarray(du, np.float64)
+    dv = np.asarray(dv, np.float64)
+    s = np.linspace(0, 1, nu + 1)
+    t = np.linspace(0, 1, nv + 1)
+    S, T = np.meshgrid(s, t, indexing="ij")
+    V = (o + S[..., None] * du + T[..., None] * dv).reshape(-1, 3)
+    idx = np.arange((nu + 1) * (nv + 1)).reshape(nu + 1, nv + 1)
+    a, b = idx[:-1, :-1], idx[1:, :-1]
+    c, d = idx[1:, 1:], idx[:-1, 1:]
+    m = np.ones((nu, nv), bool) if keep is None else keep
+    tris = np.concatenate([np.stack([a[m], b[m], c[m]], -1), np.stack([a[m], c[m], d[m]], -1)])
+    N = np.tile(np.asarray(normal, np.float32), (V.shape[0], 1))
+    return B.bundle(V, N, tris, mat)
+
+
+def occluded_room(target_tris=500_000, seed=3, width=1024, height=1024, hole_fraction=0.05,
+                  light_radiance=400.0, subdiv=3) -> Scene:
+    """The guiding workload: an 8 x 3 x 6 room split at z = 3 by a wall with `hole_fraction`
+    of its 64 x 24 cells removed at random. The only light (a 1.2 x 1.2 quad under the
+    ceiling) is in the far half; the camera stands in the near half with its back to the
+    holed wall, so everything it sees is lit only through the holes. Walls, floor and ceiling are finely tessellated Lambert surfaces
+    (albedo U[0.5,0.8]^3); the rest of the triangle budget is icospheres resting on the floor
+    in both halves, Microfacet(eta=1.5, ks=U[0.5,0.9]^3, GGX alpha=U[0.1,0.6])."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    B = _Builder()
+    X, Y, Z, ZW = 8.0, 3.0, 6.0, 3.0
+
+    def lam():
+        return B.material(LAMBERT, rng.uniform(0.5, 0.8, 3))
+    # room shell, tessellated (~16k triangles)
+    _grid_quad(B, (0, 0, 0), (X, 0, 0), (0, 0, Z), 64, 48, (0, 1, 0), lam())          # floor
+    _grid_quad(B, (0, Y, 0), (0, 0, Z), (X, 0, 0), 48, 64, (0, -1, 0), lam())         # ceiling
+    _grid_quad(B, (0, 0, 0), (0, Y, 0), (X, 0, 0), 24, 64, (0, 0, 1), lam())          # near wall
+    _grid_quad(B, (0, 0, Z), (X, 0, 0), (0, Y, 0), 64, 24, (0, 0, -1), lam())         # far wall
+    _grid_quad(B, (0, 0, 0), (0, 0, Z), (0, Y, 0), 48, 24, (1, 0, 0), lam())          # left
+    _grid_quad(B, (X, 0, 0), (0, Y, 0), (0, 0, Z), 24, 48, (-1, 0, 0), lam())         # right
+    keep = rng.uniform(0, 1, (64, 24)) >= hole_fraction
+    _grid_quad(B, (0, 0, ZW), (X, 0, 0), (0, Y, 0), 64, 24, (0, 0, -1), lam(), keep)  # dividing wall
+    # icospheres on the floor, clear of the walls
+    sv, sf = icosphere(subdiv)
+    per = sf.shape[0]
+    shell = B.nt
+    n = max(0, (target_tris - shell - 2) // per)
+    if n:
+        r = rng.uniform(0.1, 0.35, n)
+        half = rng.uniform(0, 1, n) < 0.5
+        px = rng.uniform(0.4, X - 0.4, n)
+        pz = np.where(half, rng.uniform(0.4, ZW - 0.4, n), rng.uniform(ZW + 0.4, Z - 0.4, n))
+        R = _random_rotations(rng, n)
+        col = rng.uniform(0.5, 0.9, (n, 3))
+        alpha = rng.uniform(0.1, 0.6, n)
+        rot_n = np.einsum("nij,vj->nvi", R, sv)
+        pos = rot_n * r[:, None, None] + np.stack([px, r + 0.001, pz], -1)[:, None, :]
+        for k in range(n):
+            B.bundle(pos[k], rot_n[k], sf, B.material(MICROFACET, col[k], 1.5, alpha[k]))
+    em = B.material(EMISSIVE, (light_radiance,) * 3)
+    cx, cz, h = 4.0, 4.8, 0.6
+    lb = B.quad([[cx - h, Y - 0.02, cz - h], [cx + h, Y - 0.02, cz - h], [cx + h, Y - 0.02, cz + h],
+                 [cx - h, Y - 0.02, cz + h]], [0, -1, 0], em)
+    B.area_lights(lb)
+    # in the near half, backed up to the holed wall and looking away from it
+    cam = [4.0, 1.8, 2.9, 3.6, 0.9, 0.0, 0, 1, 0, 0.01, 10000.0, math.pi / 2.5, width / height]
+    return B.build(cam, "occluded_room", {"seed": seed, "holes": int((~keep).sum()), "spheres": n})

@@ -82,7 +82,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              f"(the path shards by pixels/samples/probes with one final reduction, see DESIGN.md)",
+              f"(the path would shard by pixels, samples or probes with one final reduction)",
               file=sys.stderr)
         sys.exit(2)
 

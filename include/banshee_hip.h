@@ -8,8 +8,9 @@
  *
  * Each entry point names the reference interface it replaces (paths relative to
  * FireRays/Banshee/ of the reference). The reference's plug-in points are C++
- * virtual interfaces; INTEGRATION.md shows the adapters a maintainer adds on the
- * reference side (a HipAccelerator : Intersectable and a HipImageRenderer : Renderer).
+ * virtual interfaces; the reference side binds this ABI through two thin adapters,
+ * a HipAccelerator : Intersectable (per-ray callers) and a HipImageRenderer : Renderer
+ * (whole frames).
  *
  * Threading: one context per device; a scene is immutable after bsh_scene_commit;
  * ray queries on different streams may run concurrently from different threads;
@@ -125,7 +126,9 @@ typedef struct
 enum
 {
     BSH_RENDER_ACCUMULATE = 1,  /* add into accum instead of overwriting it */
-    BSH_RENDER_NO_QUIRKS = 2,   /* quirk-free estimator (no MINPDF cut, no clamp, solid-angle MIS); unsupported in this build */
+    BSH_RENDER_NO_QUIRKS = 2,   /* quirk-free estimator: no MINPDF cut or clamp, MIS in solid angle with GetPdf(view, dir),
+                                   light hits matched by triangle, emitters end paths after bounce 0, a fresh BSDF
+                                   sample per bounce (the unbiased judge of the guided tracer) */
     BSH_RENDER_COUNT = 4        /* count BVH node visits and triangle tests (slower; for roofline accounting) */
 };
 
@@ -275,7 +278,50 @@ bsh_status bsh_autotune(bsh_context* ctx, const bsh_job* jobs, size_t njobs, con
                         size_t n_paths, const int32_t* blocks_per_cu, size_t n_blocks, bsh_tuning* best,
                         double* best_ms);
 
+/* ---------------- learned path guiding (no reference counterpart) ---------------- */
+/* A vMF-mixture proposal (8 lobes) predicted by an MLP of position, normal and roughness
+   (7 inputs, 6-band frequency encoding -> 96, 4 x 64 ReLU, 48 outputs; FP16 weights,
+   FP32 accumulation on MFMA), trained online with Adam on the tracer's own radiance
+   samples by minimising -sum_s (L_s / p_s) log q(w_s). */
+typedef struct bsh_guide bsh_guide;
+typedef struct
+{
+    uint32_t seed;        /* He-uniform initialisation seed */
+    float lr, beta1, beta2, eps;   /* Adam */
+    float bsdf_fraction;  /* one-sample MIS: probability of sampling the BSDF instead of the guide */
+    int32_t reserved[2];
+} bsh_guide_params;
+
+size_t bsh_guide_num_params(void);
+bsh_status bsh_guide_create(bsh_context* ctx, const bsh_guide_params* p, bsh_guide** out);
+bsh_status bsh_guide_destroy(bsh_guide* g);
+/* FP32 master parameters (bsh_guide_num_params floats, layout W1[96][64], b1, W2[64][64], b2,
+   W3, b3, W4, b4, W5[64][48], b5); set resets the Adam state. Host pointers. */
+bsh_status bsh_guide_get_params(const bsh_guide* g, float* host);
+bsh_status bsh_guide_set_params(bsh_guide* g, const float* host);
+/* Network outputs for n rows of raw inputs (8 floats per row: position in [0,1]^3,
+   roughness, normal, unused). Device pointers; out n x 48. */
+bsh_status bsh_guide_forward(bsh_guide* g, const float* inputs, size_t n, float* out, void* stream);
+/* Training records: 12 floats per row (position, roughness, normal, unused, direction,
+   weight L/p). Device records; grad host float[num_params]; loss host (weighted mean). */
+bsh_status bsh_guide_gradient(bsh_guide* g, const float* records, size_t n, float* grad, float* loss);
+/* One Adam step on the records. loss may be NULL. */
+bsh_status bsh_guide_train_step(bsh_guide* g, const float* records, size_t n, float* loss, void* stream);
+/* Renders like bsh_render (flags must include BSH_RENDER_NO_QUIRKS) with the continuation
+   direction drawn from the one-sample MIS mixture of BSDF and guide (guide may be NULL: BSDF
+   only, the plain quirk-free tracer), and appends training records of the first two path
+   vertices to `records` (device float[cap*12]); *nrecords receives the count written. */
+bsh_status bsh_render_guided(bsh_scene* s, const bsh_render_params* p, bsh_guide* guide, float* accum,
+                             float* records, size_t cap, size_t* nrecords, void* stream);
+
 /* ---------------- diagnostics (exercise device code paths in isolation) ---------------- */
+/* One v_mfma_f32_16x16x16_f16 tile: d = a b for 16x16 row-major FP16 a, b (bit patterns). */
+bsh_status bsh_diag_mfma(bsh_context* ctx, const uint16_t* a, const uint16_t* b, float* d);
+/* Guide distribution on given network outputs (host pointers): outputs n x 48, uniforms u n x 3,
+   probe directions n x 3 -> out n x 8 = (vMF-mixture sample xyz, pdf at the sample, pdf at
+   dirs, sum of lobe weights, 0, 0). */
+bsh_status bsh_diag_guide(bsh_context* ctx, size_t n, const float* outputs, const float* u, const float* dirs,
+                          float* out);
 /* Device ShEvaluate (math/sh.cpp:97-153) on n unit directions. Host pointers: dirs n x 3,
    out n x (lmax+1)^2. */
 bsh_status bsh_diag_sh_eval(bsh_context* ctx, int lmax, size_t n, const float* dirs, float* out);

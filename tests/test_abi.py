@@ -10,7 +10,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     src = open(os.path.join(ROOT, "include", "banshee_hip.h")).read()
-    return sorted(set(re.findall(r"^(?:bsh_status|const char\*)\s+(bsh_\w+)\(", src, re.M)))
+    return sorted(set(re.findall(r"^(?:bsh_status|const char\*|size_t)\s+(bsh_\w+)\(", src, re.M)))
 
 
 def test_header_matches_binding_list():
