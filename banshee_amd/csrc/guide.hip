@@ -71,128 +71,127 @@ __device__ __forceinline__ void load_fwd_weights(const GuideNet& g, _Float16* lw
     for (int t = threadIdx.x; t < kBiasTotal; t += blockDim.x) lb[t] = g.bias[t];
 }
 
-// Y[16 x N] = act(X[16 x K] W + b) for one wave; X, Wt in LDS; Y to LDS (FP16) with ReLU.
-template <int K, int N>
-__device__ __forceinline__ void dense_relu(const _Float16* X, int xs, const _Float16* Wt, int ws, const float* b,
-                                           _Float16* Y, int ys, int lane)
+// Feature j of the encoded input of raw values v0..v6 = (a.xyzw, c.xyz): j < 7 the raw value,
+// then for t = j - 7 < 84: band b = t / 14, dimension d = (t % 14) / 2, sin (t even) or cos
+// (t odd) of 2^b pi v_d; zero for j >= 91. Evaluated with the hardware sine/cosine, which
+// take revolutions: sin(2^b pi v) = sin(2 pi fract(2^(b-1) v)), the power-of-two scaling and
+// fract being exact.
+__device__ __forceinline__ float encode_feature(float4 a, float4 c, int j)
 {
-    const int r = lane & 15, q = lane >> 4;
+    if (j >= kGRaw + 2 * kGRaw * kGBands) return 0.f;
+    int t = j - kGRaw;
+    int band = t / (2 * kGRaw);
+    int r = t - band * (2 * kGRaw);
+    int d = j < kGRaw ? j : r >> 1;
+    // v[d] by selects on register values (an array indexed by d would live in scratch memory)
+    float lo = d == 0 ? a.x : d == 1 ? a.y : d == 2 ? a.z : a.w;
+    float hi = d == 4 ? c.x : d == 5 ? c.y : c.z;
+    float x = d < 4 ? lo : hi;
+    if (j < kGRaw) return x;
+    float rev = __builtin_amdgcn_fractf(ldexpf(x, band - 1));
+    return (r & 1) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev);
+}
+
+// One dense layer on the transposed product Y^T = W^T X^T for a 16-row tile: the weights
+// are the A operand (LDS, out-major rows), the activations the B operand. The MFMA result
+// fragment (lane l holds Y^T[16 mt + 4(l/16) + i][row l%16]) is exactly the B fragment of
+// the next layer's k-step mt, so activations never leave the registers.
+template <int K, int N>
+__device__ __forceinline__ void dense_t(const h4 (&x)[K / 16], const _Float16* Wt, int ws, const float* b, h4 (&y)[N / 16],
+                                        int r, int q)
+{
 #pragma unroll
-    for (int nt = 0; nt < N / 16; ++nt)
+    for (int mt = 0; mt < N / 16; ++mt)
     {
         f4v acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int ks = 0; ks < K / 16; ++ks)
         {
-            h4 a = *reinterpret_cast<const h4*>(&X[r * xs + ks * 16 + 4 * q]);
-            h4 w = *reinterpret_cast<const h4*>(&Wt[(nt * 16 + r) * ws + ks * 16 + 4 * q]);
-            acc = mfma16(a, w, acc);
+            h4 w = *reinterpret_cast<const h4*>(&Wt[(mt * 16 + r) * ws + ks * 16 + 4 * q]);
+            acc = mfma16(w, x[ks], acc);
         }
-        float bb = b[nt * 16 + r];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) Y[(4 * q + i) * ys + nt * 16 + r] = (_Float16)fmaxf(acc[i] + bb, 0.f);
+        float4 bb = *reinterpret_cast<const float4*>(&b[mt * 16 + 4 * q]);
+        h4 o;
+        o[0] = (_Float16)fmaxf(acc[0] + bb.x, 0.f);
+        o[1] = (_Float16)fmaxf(acc[1] + bb.y, 0.f);
+        o[2] = (_Float16)fmaxf(acc[2] + bb.z, 0.f);
+        o[3] = (_Float16)fmaxf(acc[3] + bb.w, 0.f);
+        y[mt] = o;
     }
 }
 
-// Encodes 16 rows of raw inputs (8 floats per row: v0..v6, active) into X [16 x 96] FP16.
-__device__ __forceinline__ void encode_rows(const float4* in, int stride, uint32_t row0, uint32_t nrows, _Float16* X,
-                                            int lane)
+template <int N>
+__device__ __forceinline__ void store_rows(_Float16* dst, uint32_t row, uint32_t nrows, const h4 (&y)[N / 16], int q)
 {
-    for (int t = lane; t < 16 * kGIn; t += 64)
-    {
-        int rr = t / kGIn, j = t % kGIn;
-        float f = 0.f;
-        uint32_t row = row0 + rr;
-        if (row < nrows)
-        {
-            float4 a = in[(size_t)stride * row], c = in[(size_t)stride * row + 1];
-            float v[kGRaw] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z};
-            f = guide_feature(v, j);
-        }
-        X[rr * kL1S + j] = (_Float16)f;
-    }
+    if (row >= nrows) return;
+#pragma unroll
+    for (int mt = 0; mt < N / 16; ++mt) *reinterpret_cast<h4*>(&dst[(size_t)row * N + mt * 16 + 4 * q]) = y[mt];
 }
 
 constexpr int kFwdWaves = 8;
 
-// Fused forward: rows of raw inputs -> 48 FP32 outputs per row. With kTrain the encoded
-// input and the four hidden activations are also written (FP16) for the backward pass.
+// Fused forward: rows of raw inputs (`stride` float4 per row, the first two holding
+// v0..v6) -> 48 FP32 outputs per row. Weights and biases are staged in LDS once per block;
+// each wave runs 16-row tiles with all activations in registers. With kTrain the encoded
+// input and the four hidden activations are also written (FP16, row-major) for backward.
 template <bool kTrain>
 __global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet g, const float4* in, int stride,
                                                                  uint32_t nrows, float* out, _Float16* x0,
                                                                  _Float16* hs)
 {
     __shared__ _Float16 lw[kLFTotal];
-    __shared__ float lb[kBiasTotal];
-    __shared__ _Float16 bufA[kFwdWaves][16 * kL1S];
-    __shared__ _Float16 bufB[kFwdWaves][16 * kLHS];
+    __shared__ __align__(16) float lb[kBiasTotal];
     load_fwd_weights(g, lw, lb);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = lane & 15, q = lane >> 4;
-    _Float16* A = bufA[wave];
-    _Float16* B = bufB[wave];
     const uint32_t ntiles = (nrows + 15) / 16;
     for (uint32_t tile = blockIdx.x * kFwdWaves + wave; tile < ntiles; tile += gridDim.x * kFwdWaves)
     {
-        const uint32_t row0 = tile * 16;
-        encode_rows(in, stride, row0, nrows, A, lane);
-        __builtin_amdgcn_wave_barrier();   // wave-local LDS hand-off: LDS ops of one wave complete in order
-        if (kTrain)
-            for (int t = lane; t < 16 * kGIn; t += 64)
-            {
-                int rr = t / kGIn, j = t % kGIn;
-                if (row0 + rr < nrows) x0[(size_t)(row0 + rr) * kGIn + j] = A[rr * kL1S + j];
-            }
-        dense_relu<kGIn, kGHid>(A, kL1S, lw + kLF1, kL1S, lb, B, kLHS, lane);       // H1 -> B
-        if (kTrain)
-            for (int t = lane; t < 16 * kGHid; t += 64)
-            {
-                int rr = t / kGHid, j = t % kGHid;
-                if (row0 + rr < nrows) hs[((size_t)0 * nrows + row0 + rr) * kGHid + j] = B[rr * kLHS + j];
-            }
-        dense_relu<kGHid, kGHid>(B, kLHS, lw + kLF2, kLHS, lb + kGHid, A, kLHS, lane);   // H2 -> A
-        if (kTrain)
-            for (int t = lane; t < 16 * kGHid; t += 64)
-            {
-                int rr = t / kGHid, j = t % kGHid;
-                if (row0 + rr < nrows) hs[((size_t)1 * nrows + row0 + rr) * kGHid + j] = A[rr * kLHS + j];
-            }
-        dense_relu<kGHid, kGHid>(A, kLHS, lw + kLF3, kLHS, lb + 2 * kGHid, B, kLHS, lane);   // H3 -> B
-        if (kTrain)
-            for (int t = lane; t < 16 * kGHid; t += 64)
-            {
-                int rr = t / kGHid, j = t % kGHid;
-                if (row0 + rr < nrows) hs[((size_t)2 * nrows + row0 + rr) * kGHid + j] = B[rr * kLHS + j];
-            }
-        dense_relu<kGHid, kGHid>(B, kLHS, lw + kLF4, kLHS, lb + 3 * kGHid, A, kLHS, lane);   // H4 -> A
-        if (kTrain)
-            for (int t = lane; t < 16 * kGHid; t += 64)
-            {
-                int rr = t / kGHid, j = t % kGHid;
-                if (row0 + rr < nrows) hs[((size_t)3 * nrows + row0 + rr) * kGHid + j] = A[rr * kLHS + j];
-            }
-        // output layer, FP32 straight from the accumulators
+        // an offset the compiler cannot see through keeps the weight reads inside the loop:
+        // hoisted, all five layers' fragments would take ~180 registers per lane
+        int zoff = 0;
+        asm volatile("" : "+v"(zoff));
+        const _Float16* W = lw + zoff;
+        const uint32_t row = tile * 16 + r;
+        const bool live = row < nrows;
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), c = a;
+        if (live)
+        {
+            a = in[(size_t)stride * row];
+            c = in[(size_t)stride * row + 1];
+        }
+        h4 x[kGIn / 16];
 #pragma unroll
-        for (int nt = 0; nt < kGOut / 16; ++nt)
+        for (int ks = 0; ks < kGIn / 16; ++ks)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) x[ks][i] = live ? (_Float16)encode_feature(a, c, ks * 16 + 4 * q + i) : (_Float16)0.f;
+        if (kTrain) store_rows<kGIn>(x0, row, nrows, x, q);
+        h4 h1[kGHid / 16], h2[kGHid / 16];
+        dense_t<kGIn, kGHid>(x, W + kLF1, kL1S, lb, h1, r, q);
+        if (kTrain) store_rows<kGHid>(hs, row, nrows, h1, q);
+        dense_t<kGHid, kGHid>(h1, W + kLF2, kLHS, lb + kGHid, h2, r, q);
+        if (kTrain) store_rows<kGHid>(hs + (size_t)nrows * kGHid, row, nrows, h2, q);
+        dense_t<kGHid, kGHid>(h2, W + kLF3, kLHS, lb + 2 * kGHid, h1, r, q);
+        if (kTrain) store_rows<kGHid>(hs + (size_t)2 * nrows * kGHid, row, nrows, h1, q);
+        dense_t<kGHid, kGHid>(h1, W + kLF4, kLHS, lb + 3 * kGHid, h2, r, q);
+        if (kTrain) store_rows<kGHid>(hs + (size_t)3 * nrows * kGHid, row, nrows, h2, q);
+        // output layer, FP32 straight from the accumulators: lane holds outputs 16 mt + 4q + i of its row
+#pragma unroll
+        for (int mt = 0; mt < kGOut / 16; ++mt)
         {
             f4v acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int ks = 0; ks < kGHid / 16; ++ks)
             {
-                h4 a = *reinterpret_cast<const h4*>(&A[r * kLHS + ks * 16 + 4 * q]);
-                h4 w = *reinterpret_cast<const h4*>(&lw[kLF5 + (nt * 16 + r) * kLHS + ks * 16 + 4 * q]);
-                acc = mfma16(a, w, acc);
+                h4 w = *reinterpret_cast<const h4*>(&W[kLF5 + (mt * 16 + r) * kLHS + ks * 16 + 4 * q]);
+                acc = mfma16(w, h2[ks], acc);
             }
-            float bb = lb[4 * kGHid + nt * 16 + r];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-            {
-                uint32_t row = row0 + 4 * q + i;
-                if (row < nrows) out[(size_t)row * kGOut + nt * 16 + r] = acc[i] + bb;
-            }
+            float4 bb = *reinterpret_cast<const float4*>(&lb[4 * kGHid + mt * 16 + 4 * q]);
+            if (live)
+                *reinterpret_cast<float4*>(&out[(size_t)row * kGOut + mt * 16 + 4 * q]) =
+                    make_float4(acc[0] + bb.x, acc[1] + bb.y, acc[2] + bb.z, acc[3] + bb.w);
         }
-        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -308,64 +307,88 @@ __global__ __launch_bounds__(256) void k_guide_bwd(const _Float16* wb, const _Fl
     }
 }
 
-// Partial weight gradients over a chunk of rows: dW[k][n] = sum_rows X[row][k] dY[row][n]
-// (MFMA with the rows as the reduction dimension) and db[n] = sum_rows dY[row][n].
+// Partial weight gradients over a chunk of kWgRows rows: dW[k][n] = sum_rows X[row][k] dY[row][n]
+// (MFMA with the rows as the reduction dimension) and db[n] = sum_rows dY[row][n]. Four waves
+// take a quarter of the chunk each, staging 16-row tiles in wave-private LDS; their sums are
+// added in wave order, so the result does not depend on scheduling.
 // part: [gridDim.x][KIN*NOUT + NOUT]
-constexpr int kWgRows = 1024;
+constexpr int kWgRows = 512;
 template <int KIN, int NOUT>
-__global__ __launch_bounds__(64) void k_guide_wgrad(const _Float16* x, const _Float16* dy, uint32_t nrows, float* part)
+__global__ __launch_bounds__(256) void k_guide_wgrad(const _Float16* x, const _Float16* dy, uint32_t nrows, float* part)
 {
-    __shared__ _Float16 xs[16][KIN + kPad];
-    __shared__ _Float16 ys[16][NOUT + kPad];
-    const int lane = threadIdx.x, r = lane & 15, q = lane >> 4;
+    constexpr int XS = KIN + kPad, YS = NOUT + kPad;
+    __shared__ __align__(16) _Float16 xs[4][16 * XS];
+    __shared__ __align__(16) _Float16 ys[4][16 * YS];
+    __shared__ float red[KIN * NOUT + NOUT];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
     f4v acc[KIN / 16][NOUT / 16];
 #pragma unroll
     for (int a = 0; a < KIN / 16; ++a)
 #pragma unroll
         for (int b = 0; b < NOUT / 16; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
     float bsum = 0.f;
-    const uint32_t r0 = blockIdx.x * kWgRows;
-    for (uint32_t s0 = r0; s0 < r0 + kWgRows && s0 < nrows; s0 += 16)
+    _Float16* X = xs[wave];
+    _Float16* Y = ys[wave];
+    const uint32_t r0 = blockIdx.x * kWgRows + wave * (kWgRows / 4);
+    const uint32_t r1 = min(r0 + kWgRows / 4, nrows);
+    for (uint32_t s0 = r0; s0 < r1; s0 += 16)
     {
-        for (int t = lane; t < 16 * KIN; t += 64)
+        for (int t = lane; t < 16 * (KIN / 4); t += 64)
         {
-            uint32_t row = s0 + t / KIN;
-            xs[t / KIN][t % KIN] = row < nrows ? x[(size_t)row * KIN + t % KIN] : (_Float16)0.f;
+            int rr = t / (KIN / 4), c = (t % (KIN / 4)) * 4;
+            uint32_t row = s0 + rr;
+            h4 v = {0, 0, 0, 0};
+            if (row < nrows) v = *reinterpret_cast<const h4*>(&x[(size_t)row * KIN + c]);
+            *reinterpret_cast<h4*>(&X[rr * XS + c]) = v;
         }
-        for (int t = lane; t < 16 * NOUT; t += 64)
+        for (int t = lane; t < 16 * (NOUT / 4); t += 64)
         {
-            uint32_t row = s0 + t / NOUT;
-            ys[t / NOUT][t % NOUT] = row < nrows ? dy[(size_t)row * NOUT + t % NOUT] : (_Float16)0.f;
+            int rr = t / (NOUT / 4), c = (t % (NOUT / 4)) * 4;
+            uint32_t row = s0 + rr;
+            h4 v = {0, 0, 0, 0};
+            if (row < nrows) v = *reinterpret_cast<const h4*>(&dy[(size_t)row * NOUT + c]);
+            *reinterpret_cast<h4*>(&Y[rr * YS + c]) = v;
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();   // wave-private tiles: LDS ops of one wave complete in order
 #pragma unroll
         for (int mt = 0; mt < KIN / 16; ++mt)
         {
             h4 a;   // A[m = k_in][kk = row]: X[row 4q+i][16 mt + r]
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = xs[4 * q + i][mt * 16 + r];
+            for (int i = 0; i < 4; ++i) a[i] = X[(4 * q + i) * XS + mt * 16 + r];
 #pragma unroll
             for (int nt = 0; nt < NOUT / 16; ++nt)
             {
                 h4 b;   // B[kk = row][n]: dY[row 4q+i][16 nt + r]
 #pragma unroll
-                for (int i = 0; i < 4; ++i) b[i] = ys[4 * q + i][nt * 16 + r];
+                for (int i = 0; i < 4; ++i) b[i] = Y[(4 * q + i) * YS + nt * 16 + r];
                 acc[mt][nt] = mfma16(a, b, acc[mt][nt]);
             }
         }
         if (lane < NOUT)
-            for (int i = 0; i < 16; ++i) bsum += (float)ys[i][lane];
+            for (int i = 0; i < 16; ++i) bsum += (float)Y[i * YS + lane];
+        __builtin_amdgcn_wave_barrier();
+    }
+    for (int w = 0; w < 4; ++w)
+    {
+        if (wave == w)
+        {
+#pragma unroll
+            for (int mt = 0; mt < KIN / 16; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < NOUT / 16; ++nt)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)   // D[row m = 16 mt + 4q + i][col n = 16 nt + r]
+                    {
+                        int o = (mt * 16 + 4 * q + i) * NOUT + nt * 16 + r;
+                        red[o] = (w == 0 ? 0.f : red[o]) + acc[mt][nt][i];
+                    }
+            if (lane < NOUT) red[KIN * NOUT + lane] = (w == 0 ? 0.f : red[KIN * NOUT + lane]) + bsum;
+        }
         __syncthreads();
     }
     float* P = part + (size_t)blockIdx.x * (KIN * NOUT + NOUT);
-#pragma unroll
-    for (int mt = 0; mt < KIN / 16; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < NOUT / 16; ++nt)
-#pragma unroll
-            for (int i = 0; i < 4; ++i)   // D[row m = 16 mt + 4q + i][col n = 16 nt + r]
-                P[(size_t)(mt * 16 + 4 * q + i) * NOUT + nt * 16 + r] = acc[mt][nt][i];
-    if (lane < NOUT) P[KIN * NOUT + lane] = bsum;
+    for (int t = threadIdx.x; t < KIN * NOUT + NOUT; t += 256) P[t] = red[t];
 }
 
 // Sums the partial gradients of one layer (chunk order fixed) into grad[W..., b...] and
@@ -538,7 +561,7 @@ bsh_status guide_forward(bsh_guide* g, const float4* in, int stride, uint32_t n,
 {
     if (n == 0) return BSH_OK;
     uint32_t tiles = (n + 15) / 16;
-    uint32_t blocks = std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 2);
+    uint32_t blocks = std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
     hipLaunchKernelGGL(k_guide_forward<false>, dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), in, stride, n, out,
                        (_Float16*)nullptr, (_Float16*)nullptr);
     BSH_HIP(hipGetLastError());
@@ -549,7 +572,7 @@ template <int K, int N>
 static void wgrad_layer(bsh_guide* g, const _Float16* x, const _Float16* dy, uint32_t n, int param_off, hipStream_t st)
 {
     int nchunks = (int)((n + kWgRows - 1) / kWgRows);
-    hipLaunchKernelGGL((k_guide_wgrad<K, N>), dim3(nchunks), dim3(64), 0, st, x, dy, n, g->part.p);
+    hipLaunchKernelGGL((k_guide_wgrad<K, N>), dim3(nchunks), dim3(256), 0, st, x, dy, n, g->part.p);
     int cnt = K * N + N;
     hipLaunchKernelGGL(k_guide_reduce, dim3((cnt + 255) / 256), dim3(256), 0, st, g->part.p, nchunks, cnt,
                        1.f / (float)n, g->grad.p + param_off);
@@ -566,7 +589,7 @@ static bsh_status guide_gradients(bsh_guide* g, const float4* rec, uint32_t n, h
     hipLaunchKernelGGL(k_guide_wsum, dim3(std::min<uint32_t>((n + 255) / 256, 1024)), dim3(256), 0, st, rec, n,
                        g->wsum.p);
     uint32_t tiles = (n + 15) / 16;
-    uint32_t blocks = std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 2);
+    uint32_t blocks = std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
     hipLaunchKernelGGL(k_guide_forward<true>, dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), rec, 3, n,
                        g->out.p, g->x0.p, g->hs.p);
     hipLaunchKernelGGL(k_guide_loss, dim3((n + 255) / 256), dim3(256), 0, st, g->out.p, rec, n, g->wsum.p, g->d5.p,

@@ -33,19 +33,6 @@ constexpr int kGW4 = kGB3 + kGHid, kGB4 = kGW4 + kGHid * kGHid;
 constexpr int kGW5 = kGB4 + kGHid, kGB5 = kGW5 + kGHid * kGOut;
 constexpr int kGParams = kGB5 + kGOut;
 
-// Feature j of the encoded input for raw values v[0..6].
-BSH_HD float guide_feature(const float* v, int j)
-{
-    if (j < kGRaw) return v[j];
-    if (j >= kGRaw + 2 * kGRaw * kGBands) return 0.f;
-    int t = j - kGRaw;
-    int band = t / (2 * kGRaw);
-    int r = t % (2 * kGRaw);
-    int d = r >> 1;
-    float a = (float)(1 << band) * kPi * v[d];
-    return (r & 1) ? cosf(a) : sinf(a);
-}
-
 struct Lobes
 {
     float w[kGLobes];      // mixture weights

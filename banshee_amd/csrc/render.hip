@@ -599,7 +599,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                                 float4* q = reinterpret_cast<float4*>(R.rec_out) + (size_t)slot * 3;
                                 q[0] = rc[0];
                                 q[1] = rc[1];
-                                q[2] = make_float4(d.x, d.y, d.z, lum / d.w);
+                                // target density of the guide: incident radiance times the
+                                // cosine (the whole integrand of a diffuse vertex; the network
+                                // sees no view direction, so it cannot learn a glossy lobe)
+                                float4 nn = rc[1];
+                                float cs = fabsf(nn.x * d.x + nn.y * d.y + nn.z * d.z);
+                                q[2] = make_float4(d.x, d.y, d.z, lum * cs / d.w);
                             }
                         }
                         rc[2].w = 0.f;

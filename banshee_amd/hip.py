@@ -586,11 +586,11 @@ class Scene:
 
     def render_guided(self, width, height, img_grid, max_depth, seed, guide: "Guide | None" = None, crop=None,
                       spp_range=None, chunk_spp=0, accum: DeviceArray | None = None, records: DeviceArray | None = None,
-                      download=True):
+                      download=True, accumulate=False):
         """Quirk-free render whose continuation samples the BSDF/guide mixture. Returns
         (sum, sumsq, count, nrecords) when download, else nrecords."""
         p = render_params(width, height, img_grid, max_depth, seed, crop=crop, spp_range=spp_range,
-                          chunk_spp=chunk_spp)
+                          chunk_spp=chunk_spp, accumulate=accumulate)
         p.flags |= 2
         cw = crop[2] if crop else width
         ch = crop[3] if crop else height

@@ -395,9 +395,9 @@ def _grid_quad(B, origin, du, dv, nu, nv, normal, mat, keep=None):
 def occluded_room(target_tris=500_000, seed=3, width=1024, height=1024, hole_fraction=0.05,
                   light_radiance=400.0, subdiv=3) -> Scene:
     """The guiding workload: an 8 x 3 x 6 room split at z = 3 by a wall with `hole_fraction`
-    of its 64 x 24 cells removed at random. The only light (a 1.2 x 1.2 quad under the
-    ceiling) is in the far half; the camera stands in the near half with its back to the
-    holed wall, so everything it sees is lit only through the holes. Walls, floor and ceiling are finely tessellated Lambert surfaces
+    of its 64 x 24 cells removed at random. The only light (a 1.2 x 1.2 quad) is in the far
+    half and faces the far wall; the camera stands in the near half with its back to the
+    holed wall, so everything it sees is lit indirectly, through the holes. Walls, floor and ceiling are finely tessellated Lambert surfaces
     (albedo U[0.5,0.8]^3); the rest of the triangle budget is icospheres resting on the floor
     in both halves, Microfacet(eta=1.5, ks=U[0.5,0.9]^3, GGX alpha=U[0.1,0.6])."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -432,10 +432,12 @@ def occluded_room(target_tris=500_000, seed=3, width=1024, height=1024, hole_fra
         pos = rot_n * r[:, None, None] + np.stack([px, r + 0.001, pz], -1)[:, None, :]
         for k in range(n):
             B.bundle(pos[k], rot_n[k], sf, B.material(MICROFACET, col[k], 1.5, alpha[k]))
+    # the light faces the far wall: no point of the near half is in front of it, so the near
+    # half receives only light that has bounced in the far half and come through the holes
     em = B.material(EMISSIVE, (light_radiance,) * 3)
-    cx, cz, h = 4.0, 4.8, 0.6
-    lb = B.quad([[cx - h, Y - 0.02, cz - h], [cx + h, Y - 0.02, cz - h], [cx + h, Y - 0.02, cz + h],
-                 [cx - h, Y - 0.02, cz + h]], [0, -1, 0], em)
+    cx, cy, cz, h = 4.0, 1.5, 4.2, 0.6
+    lb = B.quad([[cx - h, cy - h, cz], [cx + h, cy - h, cz], [cx + h, cy + h, cz], [cx - h, cy + h, cz]],
+                [0, 0, 1], em)
     B.area_lights(lb)
     # in the near half, backed up to the holed wall and looking away from it
     cam = [4.0, 1.8, 2.9, 3.6, 0.9, 0.0, 0, 1, 0, 0.01, 10000.0, math.pi / 2.5, width / height]

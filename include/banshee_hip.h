@@ -282,7 +282,7 @@ bsh_status bsh_autotune(bsh_context* ctx, const bsh_job* jobs, size_t njobs, con
 /* A vMF-mixture proposal (8 lobes) predicted by an MLP of position, normal and roughness
    (7 inputs, 6-band frequency encoding -> 96, 4 x 64 ReLU, 48 outputs; FP16 weights,
    FP32 accumulation on MFMA), trained online with Adam on the tracer's own radiance
-   samples by minimising -sum_s (L_s / p_s) log q(w_s). */
+   samples by minimising -sum_s (L_s |cos_s| / p_s) log q(w_s). */
 typedef struct bsh_guide bsh_guide;
 typedef struct
 {
@@ -303,7 +303,8 @@ bsh_status bsh_guide_set_params(bsh_guide* g, const float* host);
    roughness, normal, unused). Device pointers; out n x 48. */
 bsh_status bsh_guide_forward(bsh_guide* g, const float* inputs, size_t n, float* out, void* stream);
 /* Training records: 12 floats per row (position, roughness, normal, unused, direction,
-   weight L/p). Device records; grad host float[num_params]; loss host (weighted mean). */
+   weight). bsh_render_guided writes weight = lum(L_in) |n.w| / p_mix. Device records; grad
+   host float[num_params]; loss host (weighted mean). */
 bsh_status bsh_guide_gradient(bsh_guide* g, const float* records, size_t n, float* grad, float* loss);
 /* One Adam step on the records. loss may be NULL. */
 bsh_status bsh_guide_train_step(bsh_guide* g, const float* records, size_t n, float* loss, void* stream);

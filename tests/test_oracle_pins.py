@@ -5,7 +5,7 @@ fixtures; these tests pin both against what the mathematics fixes, so that a wro
 index or dropped term in either would show: Moller-Trumbore on hand triangles, the BVH
 against brute force, Lambert and GGX closed forms and sample/pdf consistency, the area
 light's pdfs, glibc rand / MWC sequences from their published definitions, the camera
-frustum, and the SH tables of Appendix-style closed forms (Y_lm, the cosine-lobe
+frustum, and the closed forms of the SH tables (Y_lm, the cosine-lobe
 convolution A_l = pi, 2 pi/3, pi/4, 0, -pi/24).
 """
 import math
