@@ -241,6 +241,9 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
         if (params->max_leaf > 0) bp.max_leaf = params->max_leaf;
         if (params->sah_bins > 0) bp.bins = params->sah_bins;
         bp.threads = params->threads;
+        if (params->width != 0 && params->width != 2 && params->width != 4)
+            return fail(BSH_ERR_INVALID, "bsh_scene_commit: BVH width must be 0, 2 or 4");
+        if (params->width != 0) bp.width = params->width;
     }
     BuildOutput bo;
     build_bvh(tv.data(), nt, bp, bo);
@@ -281,8 +284,10 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
         mats4[2 * i] = make_float4(m.type, m.color[0], m.color[1], m.color[2]);
         mats4[2 * i + 1] = make_float4(m.eta, m.alpha, 0.f, 0.f);
     }
-    BSH_HIP(s->d_nodes.alloc(bo.nodes.size()));
-    BSH_HIP(hipMemcpy(s->d_nodes.p, bo.nodes.data(), bo.nodes.size() * sizeof(GpuNode), hipMemcpyHostToDevice));
+    const size_t node_f4 = bo.width == 4 ? bo.nodes4.size() * 8 : bo.nodes.size() * 4;
+    BSH_HIP(s->d_nodes.alloc(node_f4));
+    BSH_HIP(hipMemcpy(s->d_nodes.p, bo.width == 4 ? (const void*)bo.nodes4.data() : (const void*)bo.nodes.data(),
+                      node_f4 * sizeof(float4), hipMemcpyHostToDevice));
     BSH_HIP(s->d_tris.alloc(tris.size()));
     BSH_HIP(hipMemcpy(s->d_tris.p, tris.data(), tris.size() * sizeof(float4), hipMemcpyHostToDevice));
     BSH_HIP(s->d_shade.alloc(shade.size()));
@@ -308,7 +313,8 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
             s->box_lo[k] = std::min(s->box_lo[k], s->verts[3 * i + k]);
             s->box_hi[k] = std::max(s->box_hi[k], s->verts[3 * i + k]);
         }
-    s->nnodes = bo.nodes.size();
+    s->nnodes = bo.width == 4 ? bo.nodes4.size() : bo.nodes.size();
+    s->width = bo.width;
     s->max_depth = bo.max_depth;
     s->build_ms = bo.build_ms;
     s->committed = true;
@@ -325,16 +331,48 @@ bsh_status bsh_scene_destroy(bsh_scene* s)
     return BSH_OK;
 }
 
+bsh_status bsh_bvh_build_host(const float* tri_verts, size_t ntris, const bsh_bvh_params* params, void* nodes,
+                              size_t nodes_cap, int32_t* tri_order, size_t* nnodes, int32_t* width, int32_t* depth)
+{
+    if (!tri_verts || ntris == 0 || !nnodes) return fail(BSH_ERR_INVALID, "bsh_bvh_build_host: bad argument");
+    BuildParams bp;
+    if (params)
+    {
+        if (params->max_leaf > 0) bp.max_leaf = params->max_leaf;
+        if (params->sah_bins > 0) bp.bins = params->sah_bins;
+        bp.threads = params->threads;
+        if (params->width != 0 && params->width != 2 && params->width != 4)
+            return fail(BSH_ERR_INVALID, "bsh_bvh_build_host: BVH width must be 0, 2 or 4");
+        if (params->width != 0) bp.width = params->width;
+    }
+    BuildOutput bo;
+    build_bvh(tri_verts, ntris, bp, bo);
+    const size_t n = bo.width == 4 ? bo.nodes4.size() : bo.nodes.size();
+    const size_t nb = bo.width == 4 ? sizeof(GpuNode4) : sizeof(GpuNode);
+    *nnodes = n;
+    if (width) *width = bo.width;
+    if (depth) *depth = bo.max_depth;
+    if (nodes)
+    {
+        if (nodes_cap < n) return fail(BSH_ERR_INVALID, "bsh_bvh_build_host: node buffer too small");
+        std::memcpy(nodes, bo.width == 4 ? (const void*)bo.nodes4.data() : (const void*)bo.nodes.data(), n * nb);
+    }
+    if (tri_order) std::memcpy(tri_order, bo.tri_order.data(), ntris * sizeof(int32_t));
+    return BSH_OK;
+}
+
 bsh_status bsh_scene_info(const bsh_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* bytes, double* build_ms,
-                          int32_t* depth)
+                          int32_t* depth, int32_t* width)
 {
     if (!s) return fail(BSH_ERR_INVALID, "bsh_scene_info: NULL scene");
     if (nodes) *nodes = s->nnodes;
     if (tris) *tris = s->tri_m.size();
     if (bytes)
-        *bytes = s->nnodes * sizeof(GpuNode) + s->tri_m.size() * (48 + 32) + (s->verts.size() + s->normals.size()) / 3 * 16;
+        *bytes = s->nnodes * (s->width == 4 ? sizeof(GpuNode4) : sizeof(GpuNode)) + s->tri_m.size() * (48 + 32) +
+                 (s->verts.size() + s->normals.size()) / 3 * 16;
     if (build_ms) *build_ms = s->build_ms;
     if (depth) *depth = s->max_depth;
+    if (width) *width = s->width;
     return BSH_OK;
 }
 

@@ -99,6 +99,17 @@ public:
         ncount_ = 0;
         int r = -1;
         if (n_ > 0) r = build(0, (int)n_, root, croot, 0);
+        out.width = 2;
+        if (p_.width == 4 && r >= 0 && nodes_[r].left >= 0)
+        {
+            flatten4(r, out);
+            if (3 * out.max_depth + 1 <= kMaxStack4)
+            {
+                out.width = 4;
+                return;
+            }
+            out.nodes4.clear();
+        }
         flatten(r, root, out);
     }
 
@@ -300,6 +311,77 @@ private:
             }
         }
         write(out.nodes[me], c[0], c[1]);
+        return me;
+    }
+
+    void flatten4(int root, BuildOutput& out)
+    {
+        out.nodes4.clear();
+        out.tri_order.assign(idx_.begin(), idx_.end());
+        out.max_depth = 0;
+        out.nodes4.reserve(ncount_.load() / 2 + 1);
+        emit4(root, 0, out);
+    }
+
+    // Children of a 4-wide node: start from the BVH2 node's two children and open the
+    // internal one of largest area until there are four or only leaves remain.
+    int emit4(int t, int depth, BuildOutput& out)
+    {
+        out.max_depth = std::max(out.max_depth, depth);
+        int me = (int)out.nodes4.size();
+        out.nodes4.emplace_back();
+        int kids[4] = {nodes_[t].left, nodes_[t].right, -1, -1};
+        int nk = 2;
+        while (nk < 4)
+        {
+            int best = -1;
+            float ba = -1.f;
+            for (int k = 0; k < nk; ++k)
+                if (nodes_[kids[k]].left >= 0 && nodes_[kids[k]].box.area() > ba)
+                {
+                    ba = nodes_[kids[k]].box.area();
+                    best = k;
+                }
+            if (best < 0) break;
+            int open = kids[best];
+            kids[best] = nodes_[open].left;
+            kids[nk++] = nodes_[open].right;
+        }
+        ChildRef c[4];
+        for (int k = 0; k < 4; ++k)
+        {
+            if (k >= nk)
+            {
+                c[k].ref = 0;
+                c[k].count = -1;
+                c[k].box.reset();
+                continue;
+            }
+            const TNode& ch = nodes_[kids[k]];
+            c[k].box = ch.box;
+            if (ch.left < 0)
+            {
+                c[k].ref = ch.first;
+                c[k].count = ch.count;
+            }
+            else
+            {
+                c[k].ref = emit4(kids[k], depth + 1, out);
+                c[k].count = 0;
+            }
+        }
+        GpuNode4& g = out.nodes4[me];
+        for (int k = 0; k < 4; ++k)
+        {
+            float lo[3], hi[3];
+            if (c[k].count < 0) { for (int a = 0; a < 3; ++a) { lo[a] = 1.f; hi[a] = -1.f; } }
+            else pad(c[k].box, lo, hi);
+            g.lox[k] = lo[0]; g.hix[k] = hi[0];
+            g.loy[k] = lo[1]; g.hiy[k] = hi[1];
+            g.loz[k] = lo[2]; g.hiz[k] = hi[2];
+            g.ref[k] = c[k].ref;
+            g.cnt[k] = c[k].count;
+        }
         return me;
     }
 

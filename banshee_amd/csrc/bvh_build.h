@@ -28,21 +28,38 @@ struct GpuNode
 };
 static_assert(sizeof(GpuNode) == 64, "node is 64 B");
 
+// 4-wide node, 128 B = 8 x float4: per child slab bounds, then refs and counts (same
+// meaning as in GpuNode). Collapsed from the binned-SAH BVH2 by repeatedly opening the
+// internal child of largest surface area, so one visit tests up to four boxes.
+struct GpuNode4
+{
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    int32_t ref[4];
+    int32_t cnt[4];
+};
+static_assert(sizeof(GpuNode4) == 128, "node is 128 B");
+
 // Stack entries the traversal kernel reserves per lane. The builder caps the depth of
 // internal nodes at kMaxStack - 2 so that the near-first traversal, which pushes at
 // most one far child per internal ancestor, never needs more.
 constexpr int kMaxStack = 32;
+// The 4-wide traversal pushes at most 3 children per level; the builder falls back to the
+// binary layout when 3 x (4-wide depth) + 1 would exceed this.
+constexpr int kMaxStack4 = 48;
 
 struct BuildParams
 {
     int max_leaf = 4;
     int bins = 16;
     int threads = 0;
+    int width = 2;      // 2: GpuNode, 4: GpuNode4
 };
 
 struct BuildOutput
 {
-    std::vector<GpuNode> nodes;
+    std::vector<GpuNode> nodes;       // width 2
+    std::vector<GpuNode4> nodes4;     // width 4
+    int width = 2;                    // the layout actually built
     std::vector<int32_t> tri_order;   // leaf order -> global primitive index
     int max_depth = 0;
     double build_ms = 0.0;

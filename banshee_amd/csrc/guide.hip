@@ -626,6 +626,11 @@ bsh_status bsh_guide_create(bsh_context* ctx, const bsh_guide_params* prm, bsh_g
     g->ctx = ctx;
     bsh_guide_params d{1u, 5e-3f, 0.9f, 0.999f, 1e-8f, 0.5f, {0, 0}};
     g->prm = prm ? *prm : d;
+    if (!(g->prm.bsdf_fraction > 0.f && g->prm.bsdf_fraction <= 1.f))
+    {
+        delete g;
+        return fail(BSH_ERR_INVALID, "bsh_guide_create: the BSDF fraction must be in (0, 1] (unbiasedness)");
+    }
     g->host_params.assign(kGParams, 0.f);
     // He-uniform weights (limit sqrt(6 / fan_in)), zero biases, from PCG32(seed)
     uint64_t st = 0x853c49e6748fea9bULL ^ ((uint64_t)g->prm.seed << 1);
@@ -687,6 +692,15 @@ bsh_status bsh_guide_set_params(bsh_guide* g, const float* host)
     BSH_HIP(hipMemset(g->v.p, 0, kGParams * 4));
     g->step = 0;
     return upload_params(g);
+}
+
+bsh_status bsh_guide_set_bsdf_fraction(bsh_guide* g, float bsdf_fraction)
+{
+    if (!g) return fail(BSH_ERR_INVALID, "bsh_guide_set_bsdf_fraction: NULL guide");
+    if (!(bsdf_fraction > 0.f && bsdf_fraction <= 1.f))
+        return fail(BSH_ERR_INVALID, "bsh_guide_set_bsdf_fraction: the BSDF fraction must be in (0, 1]");
+    g->prm.bsdf_fraction = bsdf_fraction;
+    return BSH_OK;
 }
 
 bsh_status bsh_guide_forward(bsh_guide* g, const float* inputs, size_t n, float* out, void* stream)

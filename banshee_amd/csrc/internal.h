@@ -77,6 +77,7 @@ struct RenderWork
     DevBuf<uint32_t> rec_count;
     DevBuf<uint32_t> counters;     // per launch: count, work, (pad) x 4
     DevBuf<unsigned long long> stats;  // closest, any
+    DevBuf<uint32_t> stk_ovf;          // deep traversal-stack entries of the trace kernel
 };
 
 
@@ -144,13 +145,14 @@ struct bsh_scene
     bool has_cam = false;
     bool committed = false;
 
-    bsh::DevBuf<bsh::GpuNode> d_nodes;
+    bsh::DevBuf<float4> d_nodes;   // GpuNode (width 2) or GpuNode4 (width 4) as float4s
     bsh::DevBuf<float4> d_tris, d_verts, d_normals, d_mats;
     bsh::DevBuf<int4> d_shade;
     bsh::DevBuf<bsh::LightDev> d_lights;
     std::vector<bsh::LightDev> h_lights;
     float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {0.f, 0.f, 0.f};
     uint64_t nnodes = 0;
+    int width = 2;
     int max_depth = 0;
     double build_ms = 0.0;
 
@@ -158,7 +160,8 @@ struct bsh_scene
     bsh::SceneDev dev() const
     {
         bsh::SceneDev s;
-        s.nodes = reinterpret_cast<const float4*>(d_nodes.p);
+        s.nodes = d_nodes.p;
+        s.width = width;
         s.tris = d_tris.p;
         s.shade = d_shade.p;
         s.verts = d_verts.p;

@@ -40,6 +40,12 @@
 namespace bsh {
 
 constexpr int kBlock = 256;
+// LDS entries of the wavefront trace kernel's traversal stack: all of the binary layout's
+// kMaxStack (32 x 4 B x 256 threads = 32 KB per block, 5 blocks per CU); the 4-wide
+// layout's deeper entries go to per-thread global slots. Measured on the 3M-triangle
+// terrain, more waves per SIMD with a smaller LDS part did not trace faster.
+constexpr int kTraceLds = 32;
+constexpr int kMaxTraceBlocksPerCU = 5;
 
 enum : uint32_t
 {
@@ -67,6 +73,7 @@ struct RenderDev
     int *qc_own[2], *qa_own[2];
     uint32_t* counters;            // per launch L: closest count, closest work, any count, any work
     unsigned long long* stats;     // closest, any, node visits, triangle tests
+    uint32_t* stk_ovf;             // traversal stack entries beyond kTraceLds, [entry][trace thread]
     float4* gin;                   // guide inputs, 2 per path
     const float* gout;             // guide outputs, 48 per path
     float4* rec;                   // per-path training records: 2 x 5 float4
@@ -212,9 +219,11 @@ __global__ __launch_bounds__(kBlock) void k_guide_inputs(RenderDev R)
 template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_trace_queue(RenderDev R, int L, int buf)
 {
-    __shared__ uint32_t stk[kMaxStack * kBlock];
+    __shared__ uint32_t stk[kTraceLds * kBlock];
     const int lane = threadIdx.x & 63;
     uint32_t* my = stk + threadIdx.x;
+    const int ostride = gridDim.x * kBlock;
+    uint32_t* ovf = R.stk_ovf + blockIdx.x * kBlock + threadIdx.x;
     uint32_t vis_nodes = 0, vis_tris = 0;
     for (int pass = 0; pass < 2; ++pass)
     {
@@ -241,8 +250,11 @@ __global__ __launch_bounds__(kBlock) void k_trace_queue(RenderDev R, int L, int 
                 TraceCounts cnt{0u, 0u};
                 if (pass == 0)
                 {
-                    bool h = traverse<false, kCount>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, my, kBlock,
-                                                     res, cnt);
+                    bool h = S.width == 4
+                                 ? traverse4<false, kCount, kTraceLds>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w,
+                                                                       b.w, my, kBlock, res, cnt, ovf, ostride)
+                                 : traverse<false, kCount, kTraceLds>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w,
+                                                                      b.w, my, kBlock, res, cnt, ovf, ostride);
                     int4 o = make_int4(h ? res.prim : -1, __float_as_int(res.t), __float_as_int(res.b1),
                                        __float_as_int(res.b2));
                     if (kind == Q_EXT) R.hit[idx] = o;
@@ -250,8 +262,11 @@ __global__ __launch_bounds__(kBlock) void k_trace_queue(RenderDev R, int L, int 
                 }
                 else
                 {
-                    bool h = traverse<true, kCount>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, my, kBlock,
-                                                    res, cnt);
+                    bool h = S.width == 4
+                                 ? traverse4<true, kCount, kTraceLds>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w,
+                                                                      b.w, my, kBlock, res, cnt, ovf, ostride)
+                                 : traverse<true, kCount, kTraceLds>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w,
+                                                                     b.w, my, kBlock, res, cnt, ovf, ostride);
                     R.nee_res[idx] = make_int4(h ? 1 : 0, 0, 0, 0);
                 }
                 vis_nodes += cnt.nodes;
@@ -1032,7 +1047,7 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     BSH_HIP(hipSetDevice(c->device));
     bsh_tuning tune = tune_in ? *tune_in : c->tuning;
     if (tune.paths_per_chunk <= 0) tune.paths_per_chunk = 1 << 21;
-    if (tune.trace_blocks_per_cu <= 0) tune.trace_blocks_per_cu = 5;
+    if (tune.trace_blocks_per_cu <= 0) tune.trace_blocks_per_cu = kMaxTraceBlocksPerCU;
     int nlmax = 1;
     uint32_t maxunit = 1;
     uint64_t samples = 0;
@@ -1073,6 +1088,8 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         BSH_HIP(w.rec.alloc(10 * P));
         BSH_HIP(w.rec_count.alloc(1));
     }
+    const size_t trace_threads = (size_t)c->num_cus * std::max(1, tune.trace_blocks_per_cu) * kBlock;
+    BSH_HIP(w.stk_ovf.alloc(trace_threads * (std::max(kMaxStack, kMaxStack4) - kTraceLds)));
     BSH_HIP(c->jobs.alloc(hjobs.size()));
     BSH_HIP(hipMemcpyAsync(c->jobs.p, hjobs.data(), hjobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, st));
 
@@ -1089,6 +1106,7 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     }
     R.counters = w.counters.p;
     R.stats = w.stats.p;
+    R.stk_ovf = w.stk_ovf.p;
     R.P = 0;
     R.gin = guided ? w.gin.p : nullptr;
     R.gout = guided ? w.gout.p : nullptr;

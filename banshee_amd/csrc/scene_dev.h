@@ -31,6 +31,7 @@ struct SceneDev
     const LightDev* lights;
     int nlights;
     int ntris;
+    int width;                // BVH layout: 2 (GpuNode, 4 float4) or 4 (GpuNode4, 8 float4)
 };
 
 struct TraceResult
@@ -45,15 +46,19 @@ struct TraceCounts
 };
 
 // Stack-based BVH2 traversal with both child boxes in the parent, near child first,
-// leaves intersected as soon as their box is hit, and a per-lane LDS stack of
-// kMaxStack entries laid out [entry][lane] (stride = blockDim.x) so a wave's pushes
-// and pops hit 64 distinct banks. Closest hit (any == false) replaces
-// Bvh::Intersect(ray, Hit&) (bvh.cpp:499-573): accept tmin < t <= current t; any hit
-// replaces Bvh::Intersect(ray) (bvh.cpp:575-647) and returns at the first accept.
-template <bool kAny, bool kCount>
+// leaves intersected as soon as their box is hit, and a per-lane stack of kMaxStack
+// entries: the first kLds in LDS laid out [entry][lane] (stride = blockDim.x) so a wave's
+// pushes and pops hit 64 distinct banks, the rest (deep entries, rarely touched) in the
+// lane's own global slots `ovf` (stride ostride). A smaller LDS part lets more waves share
+// a CU. Closest hit (any == false) replaces Bvh::Intersect(ray, Hit&) (bvh.cpp:499-573):
+// accept tmin < t <= current t; any hit replaces Bvh::Intersect(ray) (bvh.cpp:575-647) and
+// returns at the first accept.
+template <bool kAny, bool kCount, int kLds = kMaxStack>
 __device__ __forceinline__ bool traverse(const SceneDev& s, f3 o, f3 d, float tmin, float tmax, uint32_t* stk,
-                                         int stride, TraceResult& res, TraceCounts& cnt)
+                                         int stride, TraceResult& res, TraceCounts& cnt, uint32_t* ovf = nullptr,
+                                         int ostride = 0)
 {
+    static_assert(kLds >= 1 && kLds <= kMaxStack, "LDS stack part out of range");
     // Slabs are evaluated as b*inv - o*inv with one FMA; a zero direction component
     // would give inf - inf = NaN there, so the box test uses |d| >= 1e-20 (the
     // triangle test keeps the exact direction).
@@ -130,7 +135,9 @@ __device__ __forceinline__ bool traverse(const SceneDev& s, f3 o, f3 d, float tm
             uint32_t farc = first0 ? (uint32_t)r.y : (uint32_t)r.x;
             // The builder caps internal depth at kMaxStack - 2, so the stack never fills;
             // were it to, the far child is dropped rather than written out of bounds.
-            if (sp < kMaxStack) stk[(sp++) * stride] = farc;
+            if (sp < kLds) stk[sp * stride] = farc;
+            else if (kLds < kMaxStack && sp < kMaxStack) ovf[(sp - kLds) * ostride] = farc;
+            if (sp < kMaxStack) ++sp;
             node = nearc;
         }
         else if (h0)
@@ -144,8 +151,139 @@ __device__ __forceinline__ bool traverse(const SceneDev& s, f3 o, f3 d, float tm
         else
         {
             if (sp == 0) break;
-            node = stk[(--sp) * stride];
+            --sp;
+            node = (kLds == kMaxStack || sp < kLds) ? stk[sp * stride] : ovf[(sp - kLds) * ostride];
         }
+    }
+    res.prim = prim;
+    res.t = tcur;
+    res.b1 = hb1;
+    res.b2 = hb2;
+    return prim >= 0;
+}
+
+// Stack push/pop over the LDS part (kLds entries) and the global overflow slots.
+template <int kLds, int kCap>
+__device__ __forceinline__ void stack_push(uint32_t* stk, int stride, uint32_t* ovf, int ostride, int& sp, uint32_t v)
+{
+    if (sp < kLds) stk[sp * stride] = v;
+    else if (kLds < kCap && sp < kCap) ovf[(sp - kLds) * ostride] = v;
+    if (sp < kCap) ++sp;
+}
+
+template <int kLds, int kCap>
+__device__ __forceinline__ uint32_t stack_pop(const uint32_t* stk, int stride, const uint32_t* ovf, int ostride, int& sp)
+{
+    --sp;
+    return (kLds >= kCap || sp < kLds) ? stk[sp * stride] : ovf[(sp - kLds) * ostride];
+}
+
+// 4-wide traversal over GpuNode4: one visit loads the node's 128 B, tests four boxes,
+// intersects leaf children at once, then visits the hit internal children nearest first
+// (a 5-comparator sorting network on their entry distances; the far ones are pushed,
+// farthest first). Same accept rules as traverse(). The builder bounds the stack depth
+// by kMaxStack4 (at most three pushes per level).
+template <bool kAny, bool kCount, int kLds>
+__device__ __forceinline__ bool traverse4(const SceneDev& s, f3 o, f3 d, float tmin, float tmax, uint32_t* stk,
+                                          int stride, TraceResult& res, TraceCounts& cnt, uint32_t* ovf,
+                                          int ostride)
+{
+    const float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
+    const float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
+    const float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
+    const float ix = 1.f / dx, iy = 1.f / dy, iz = 1.f / dz;
+    const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
+    const float kInf = __builtin_inff();
+    float tcur = tmax;
+    int prim = -1;
+    float hb1 = 0.f, hb2 = 0.f;
+    uint32_t node = 0;
+    int sp = 0;
+    for (;;)
+    {
+        const float4* nd = s.nodes + 8 * (size_t)node;
+        const float4 LX = nd[0], HX = nd[1], LY = nd[2], HY = nd[3], LZ = nd[4], HZ = nd[5];
+        const int4 REF = reinterpret_cast<const int4*>(nd)[6];
+        const int4 CNT = reinterpret_cast<const int4*>(nd)[7];
+        if (kCount) cnt.nodes++;
+        const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, hx[4] = {HX.x, HX.y, HX.z, HX.w};
+        const float ly[4] = {LY.x, LY.y, LY.z, LY.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w};
+        const float lz[4] = {LZ.x, LZ.y, LZ.z, LZ.w}, hz[4] = {HZ.x, HZ.y, HZ.z, HZ.w};
+        const int ref[4] = {REF.x, REF.y, REF.z, REF.w}, cn[4] = {CNT.x, CNT.y, CNT.z, CNT.w};
+        float tn[4];
+        bool hit[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+            float x0 = __builtin_fmaf(lx[c], ix, -ox), x1 = __builtin_fmaf(hx[c], ix, -ox);
+            float y0 = __builtin_fmaf(ly[c], iy, -oy), y1 = __builtin_fmaf(hy[c], iy, -oy);
+            float z0 = __builtin_fmaf(lz[c], iz, -oz), z1 = __builtin_fmaf(hz[c], iz, -oz);
+            tn[c] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tcur));
+            hit[c] = (tn[c] <= tf) && (cn[c] >= 0);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+            if (hit[c] && cn[c] > 0)
+            {
+                for (int q = ref[c]; q < ref[c] + cn[c]; ++q)
+                {
+                    const float4 v0 = s.tris[3 * q + 0];
+                    const float4 e1 = s.tris[3 * q + 1];
+                    const float4 e2 = s.tris[3 * q + 2];
+                    if (kCount) cnt.tris++;
+                    float t, u, v;
+                    if (intersect_face(ld3(v0), ld3(e1), ld3(e2), o, d, tmin, kAny ? tmax : tcur, t, u, v))
+                    {
+                        prim = __builtin_bit_cast(int, v0.w);
+                        if (kAny)
+                        {
+                            res.prim = prim;
+                            res.t = t;
+                            res.b1 = u;
+                            res.b2 = v;
+                            return true;
+                        }
+                        tcur = t;
+                        hb1 = u;
+                        hb2 = v;
+                    }
+                }
+                hit[c] = false;
+            }
+        }
+        float key[4];
+        uint32_t id[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+            key[c] = (hit[c] && tn[c] <= tcur) ? tn[c] : kInf;
+            id[c] = (uint32_t)ref[c];
+        }
+#define BSH_CSWAP(a, b)                                       \
+    if (key[b] < key[a])                                      \
+    {                                                         \
+        float tk = key[a]; key[a] = key[b]; key[b] = tk;      \
+        uint32_t ti = id[a]; id[a] = id[b]; id[b] = ti;       \
+    }
+        BSH_CSWAP(0, 1)
+        BSH_CSWAP(2, 3)
+        BSH_CSWAP(0, 2)
+        BSH_CSWAP(1, 3)
+        BSH_CSWAP(1, 2)
+#undef BSH_CSWAP
+        const int n = (key[0] < kInf) + (key[1] < kInf) + (key[2] < kInf) + (key[3] < kInf);
+        if (n == 0)
+        {
+            if (sp == 0) break;
+            node = stack_pop<kLds, kMaxStack4>(stk, stride, ovf, ostride, sp);
+            continue;
+        }
+#pragma unroll
+        for (int k = 3; k >= 1; --k)
+            if (k < n) stack_push<kLds, kMaxStack4>(stk, stride, ovf, ostride, sp, id[k]);
+        node = id[0];
     }
     res.prim = prim;
     res.t = tcur;

@@ -12,15 +12,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect(SceneDev s, const bsh
                                                            bsh_hit* __restrict__ hits, int32_t* __restrict__ occ,
                                                            uint32_t* __restrict__ visits)
 {
-    __shared__ uint32_t stk[kMaxStack * kTraceBlock];
+    // whole stack in LDS: 48 entries x 4 B x 256 threads = 48 KB, enough for either layout
+    __shared__ uint32_t stk[kMaxStack4 * kTraceBlock];
     uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= n) return;
     const float4* r4 = reinterpret_cast<const float4*>(rays);
     float4 a = r4[2 * i], b = r4[2 * i + 1];
     TraceResult res;
     TraceCounts cnt{0u, 0u};
-    bool h = traverse<kAny, kCount>(s, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, stk + threadIdx.x, kTraceBlock,
-                                    res, cnt);
+    bool h = s.width == 4 ? traverse4<kAny, kCount, kMaxStack4>(s, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w,
+                                                                 stk + threadIdx.x, kTraceBlock, res, cnt, nullptr, 0)
+                          : traverse<kAny, kCount>(s, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w,
+                                                   stk + threadIdx.x, kTraceBlock, res, cnt);
     if (kAny && occ) occ[i] = h ? 1 : 0;
     if (hits)
     {

@@ -50,7 +50,7 @@ class Camera(ctypes.Structure):
 
 class BvhParams(ctypes.Structure):
     _fields_ = [("max_leaf", ctypes.c_int32), ("sah_bins", ctypes.c_int32), ("threads", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("width", ctypes.c_int32)]
 
 
 class RenderParams(ctypes.Structure):
@@ -102,7 +102,8 @@ EXPORTS = [
     "bsh_bake_sh", "bsh_sh_project_env", "bsh_sh_irradiance", "bsh_diag_sh_eval",
     "bsh_batch_run", "bsh_context_set_tuning", "bsh_context_get_tuning", "bsh_context_stats_get", "bsh_autotune",
     "bsh_guide_num_params", "bsh_guide_create", "bsh_guide_destroy", "bsh_guide_get_params", "bsh_guide_set_params",
-    "bsh_guide_forward", "bsh_guide_gradient", "bsh_guide_train_step", "bsh_render_guided", "bsh_diag_mfma", "bsh_diag_guide",
+    "bsh_guide_forward", "bsh_guide_gradient", "bsh_guide_train_step", "bsh_render_guided", "bsh_diag_mfma",
+    "bsh_diag_guide", "bsh_bvh_build_host", "bsh_guide_set_bsdf_fraction",
 ]
 
 _lib = None
@@ -390,6 +391,9 @@ class Guide:
         p.seed, p.lr, p.beta1, p.beta2, p.eps, p.bsdf_fraction = seed, lr, beta1, beta2, eps, bsdf_fraction
         _check(lib().bsh_guide_create(ctx.h, ctypes.byref(p), ctypes.byref(self.h)), "bsh_guide_create")
 
+    def set_bsdf_fraction(self, a: float):
+        _check(lib().bsh_guide_set_bsdf_fraction(self.h, ctypes.c_float(a)), "bsh_guide_set_bsdf_fraction")
+
     @staticmethod
     def num_params() -> int:
         return int(lib().bsh_guide_num_params())
@@ -424,14 +428,18 @@ class Guide:
         _check(lib().bsh_guide_gradient(self.h, dr.ptr, c_size_t(n), _p(g), ctypes.byref(loss)), "bsh_guide_gradient")
         return g, loss.value
 
-    def train_step(self, records, n=None):
+    def train_step(self, records, n=None, offset=0, want_loss=True):
+        """One Adam step on rows [offset, offset + n) of `records` (host array or DeviceArray).
+        Returns the weighted mean loss (want_loss=False skips its device-to-host read)."""
         dr = records if isinstance(records, DeviceArray) else self.ctx.to_device(
             np.ascontiguousarray(records, np.float32).reshape(-1, RECORD_FLOATS))
-        n = dr.shape[0] if n is None else n
+        n = dr.shape[0] - offset if n is None else n
+        assert 0 <= offset and offset + n <= dr.shape[0]
+        ptr = c_void_p(dr.ptr.value + offset * RECORD_FLOATS * 4)
         loss = ctypes.c_float()
-        _check(lib().bsh_guide_train_step(self.h, dr.ptr, c_size_t(n), ctypes.byref(loss), c_void_p(None)),
-               "bsh_guide_train_step")
-        return loss.value
+        _check(lib().bsh_guide_train_step(self.h, ptr, c_size_t(n), ctypes.byref(loss) if want_loss else None,
+                                          c_void_p(None)), "bsh_guide_train_step")
+        return loss.value if want_loss else None
 
     def close(self):
         if self.h:
@@ -443,6 +451,24 @@ class Guide:
             self.close()
         except Exception:
             pass
+
+
+def bvh_build_host(tri_verts, max_leaf=4, sah_bins=16, threads=0, width=0):
+    """Host-only BVH build (no device): returns (nodes float32 [n, 16 or 32], tri_order, width, depth).
+    Node rows are the raw 64-B (width 2) or 128-B (width 4) records; the int fields are
+    returned by viewing the row as int32."""
+    tv = np.ascontiguousarray(tri_verts, np.float32).reshape(-1, 9)
+    bp = BvhParams(max_leaf, sah_bins, threads, width)
+    n, w, d = c_size_t(), ctypes.c_int32(), ctypes.c_int32()
+    _check(lib().bsh_bvh_build_host(_p(tv), c_size_t(tv.shape[0]), ctypes.byref(bp), None, c_size_t(0), None,
+                                    ctypes.byref(n), ctypes.byref(w), ctypes.byref(d)), "bsh_bvh_build_host")
+    cols = 32 if w.value == 4 else 16
+    nodes = np.zeros((n.value, cols), np.float32)
+    order = np.zeros(tv.shape[0], np.int32)
+    _check(lib().bsh_bvh_build_host(_p(tv), c_size_t(tv.shape[0]), ctypes.byref(bp), _p(nodes), c_size_t(n.value),
+                                    _p(order), ctypes.byref(n), ctypes.byref(w), ctypes.byref(d)),
+           "bsh_bvh_build_host")
+    return nodes, order, w.value, d.value
 
 
 def camera_struct(cam13) -> Camera:
@@ -457,7 +483,7 @@ def camera_struct(cam13) -> Camera:
 class Scene:
     """A committed device scene (World after Commit) built from a banshee_amd.scenes.Scene."""
 
-    def __init__(self, ctx: Context, sc, max_leaf=4, sah_bins=16, threads=0):
+    def __init__(self, ctx: Context, sc, max_leaf=4, sah_bins=16, threads=0, width=0):
         self.ctx = ctx
         self.sc = sc
         self.h = c_void_p()
@@ -476,7 +502,7 @@ class Scene:
         lights = np.ascontiguousarray(sc.lights, np.int32).reshape(-1, 2)
         _check(lib().bsh_scene_set_lights(self.h, _p(lights), c_size_t(lights.shape[0])), "bsh_scene_set_lights")
         self.set_camera(sc.camera)
-        bp = BvhParams(max_leaf, sah_bins, threads, 0)
+        bp = BvhParams(max_leaf, sah_bins, threads, width)
         _check(lib().bsh_scene_commit(self.h, ctypes.byref(bp)), "bsh_scene_commit")
 
     def set_camera(self, cam13):
@@ -485,11 +511,11 @@ class Scene:
 
     def info(self):
         nodes, tris, nbytes = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
-        ms, depth = ctypes.c_double(), ctypes.c_int32()
+        ms, depth, width = ctypes.c_double(), ctypes.c_int32(), ctypes.c_int32()
         _check(lib().bsh_scene_info(self.h, ctypes.byref(nodes), ctypes.byref(tris), ctypes.byref(nbytes),
-                                    ctypes.byref(ms), ctypes.byref(depth)), "bsh_scene_info")
+                                    ctypes.byref(ms), ctypes.byref(depth), ctypes.byref(width)), "bsh_scene_info")
         return {"nodes": nodes.value, "tris": tris.value, "bytes": nbytes.value, "build_ms": ms.value,
-                "depth": depth.value}
+                "depth": depth.value, "width": width.value}
 
     # ---- queries on device buffers ----
     def intersect_closest_dev(self, rays: DeviceArray, n: int, hits: DeviceArray, stream=None):

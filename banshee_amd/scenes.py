@@ -392,10 +392,10 @@ def _grid_quad(B, origin, du, dv, nu, nv, normal, mat, keep=None):
     return B.bundle(V, N, tris, mat)
 
 
-def occluded_room(target_tris=500_000, seed=3, width=1024, height=1024, hole_fraction=0.05,
+def occluded_room(target_tris=500_000, seed=3, width=1024, height=1024, hole_fraction=0.05, holes=4,
                   light_radiance=400.0, subdiv=3) -> Scene:
-    """The guiding workload: an 8 x 3 x 6 room split at z = 3 by a wall with `hole_fraction`
-    of its 64 x 24 cells removed at random. The only light (a 1.2 x 1.2 quad) is in the far
+    """The guiding workload: an 8 x 3 x 6 room split at z = 3 by a wall of 64 x 24 cells with
+    `holes` rectangular openings, together `hole_fraction` of its area, at random places. The only light (a 1.2 x 1.2 quad) is in the far
     half and faces the far wall; the camera stands in the near half with its back to the
     holed wall, so everything it sees is lit indirectly, through the holes. Walls, floor and ceiling are finely tessellated Lambert surfaces
     (albedo U[0.5,0.8]^3); the rest of the triangle budget is icospheres resting on the floor
@@ -413,7 +413,24 @@ def occluded_room(target_tris=500_000, seed=3, width=1024, height=1024, hole_fra
     _grid_quad(B, (0, 0, Z), (X, 0, 0), (0, Y, 0), 64, 24, (0, 0, -1), lam())         # far wall
     _grid_quad(B, (0, 0, 0), (0, 0, Z), (0, Y, 0), 48, 24, (1, 0, 0), lam())          # left
     _grid_quad(B, (X, 0, 0), (0, Y, 0), (0, 0, Z), 24, 48, (-1, 0, 0), lam())         # right
-    keep = rng.uniform(0, 1, (64, 24)) >= hole_fraction
+    keep = np.ones((64, 24), bool)
+    cells = int(round(hole_fraction * 64 * 24))
+    per = [cells // holes + (1 if k < cells % holes else 0) for k in range(holes)]
+    for c in per:
+        # a near-square opening of c cells (w x h, w >= h), placed where it does not overlap
+        h = max(1, int(math.floor(math.sqrt(c))))
+        w = int(math.ceil(c / h))
+        for _ in range(1000):
+            x0 = int(rng.integers(1, 64 - w))
+            y0 = int(rng.integers(1, 24 - h))
+            if keep[x0 - 1:x0 + w + 1, y0 - 1:y0 + h + 1].all():
+                break
+        n = 0
+        for i in range(w):
+            for j in range(h):
+                if n < c:
+                    keep[x0 + i, y0 + j] = False
+                    n += 1
     _grid_quad(B, (0, 0, ZW), (X, 0, 0), (0, Y, 0), 64, 24, (0, 0, -1), lam(), keep)  # dividing wall
     # icospheres on the floor, clear of the walls
     sv, sf = icosphere(subdiv)
@@ -441,4 +458,5 @@ def occluded_room(target_tris=500_000, seed=3, width=1024, height=1024, hole_fra
     B.area_lights(lb)
     # in the near half, backed up to the holed wall and looking away from it
     cam = [4.0, 1.8, 2.9, 3.6, 0.9, 0.0, 0, 1, 0, 0.01, 10000.0, math.pi / 2.5, width / height]
-    return B.build(cam, "occluded_room", {"seed": seed, "holes": int((~keep).sum()), "spheres": n})
+    return B.build(cam, "occluded_room", {"seed": seed, "openings": holes, "hole_cells": int((~keep).sum()),
+                                          "spheres": n})

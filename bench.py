@@ -8,8 +8,9 @@ each step renders the full frame into a device accumulation buffer.
 
 value = BVH queries (closest + any hit) per second over the timed steps, in Mrays/s.
 The roofline object prices the dominant kernel (the wavefront trace kernel) by its
-algorithmic bytes: B_ray = 32 (ray) + 16 (hit) + 64 * node visits + 48 * triangle
-tests, with visits counted by the same kernel in counting mode on a 4-spp frame.
+algorithmic bytes: B_ray = 32 (ray) + 16 (hit) + S_node * node visits + 48 * triangle
+tests (S_node = 64 B for the binary layout, 128 B for the 4-wide one), with visits
+counted by the same kernel in counting mode on a 4-spp frame.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 """
@@ -73,12 +74,87 @@ def cpu_baseline(sc_full_args) -> dict | None:
                       f"{n} rays in {dt:.2f} s on {cores} threads of {cpu_model()}"}
 
 
+# Guided vs plain (quirk-free) path tracing: the occluded room, 500k triangles, 1024x1024, depth 8.
+G_RES, G_DEPTH, G_TARGET = 1024, 8, 0.02
+G_PASS_GRID, G_CHECK = 2, 8               # 4 spp per pass; RMSE estimated every 8 passes (32 spp)
+G_ALPHA, G_LR, G_STEPS, G_BATCH = 0.3, 5e-3, 2, 1 << 18
+
+
+def rmse_estimate(a) -> float:
+    """RMSE of the per-pixel mean over pixels and channels, from the per-pixel sample
+    variance (both tracers are unbiased, so the MSE of a pixel is its variance of the mean)."""
+    import numpy as np
+    n = a[..., 3:4].astype(np.float64)
+    m = a[..., 0:3] / n
+    var = np.maximum(a[..., 4:7] / n - m * m, 0.0) / np.maximum(n - 1.0, 1.0)
+    return float(np.sqrt(np.mean(var)))
+
+
+def time_to_rmse(ctx, scene, guide, max_passes, seed0):
+    """Renders passes of G_PASS_GRID^2 spp into one accumulation buffer until the RMSE
+    estimate reaches G_TARGET. With a guide, every pass also writes training records and
+    runs G_STEPS Adam steps on G_BATCH of them; every pass's samples count in the image.
+    Returns (seconds of render + training, spp, rmse, training seconds). The RMSE checks
+    are outside the clock."""
+    import numpy as np
+    from banshee_amd import hip
+    acc = ctx.alloc((G_RES, G_RES, 8), np.float32)
+    cap = G_RES * G_RES * G_PASS_GRID ** 2 * 2 if guide is not None else 0
+    recs = ctx.alloc((cap, hip.RECORD_FLOATS), np.float32) if cap else None
+    ctx.sync()
+    clock = train = 0.0
+    e = float("inf")
+    k = 0
+    while k < max_passes:
+        t = time.perf_counter()
+        n = scene.render_guided(G_RES, G_RES, G_PASS_GRID, G_DEPTH, seed0 + k, guide=guide, records=recs,
+                                accum=acc, accumulate=k > 0, download=False)
+        if guide is not None and n:
+            t2 = time.perf_counter()
+            n = min(n, cap)
+            for s in range(G_STEPS):
+                off = (s * G_BATCH) % max(1, n - G_BATCH + 1) if n > G_BATCH else 0
+                guide.train_step(recs, min(G_BATCH, n), offset=off, want_loss=False)
+            ctx.sync()
+            train += time.perf_counter() - t2
+        ctx.sync()
+        clock += time.perf_counter() - t
+        k += 1
+        if k % G_CHECK == 0:
+            e = rmse_estimate(acc.numpy())
+            if e <= G_TARGET:
+                break
+    return clock, k * G_PASS_GRID ** 2, e, train
+
+
+def guided_comparison(ctx, bvh_width) -> dict:
+    from banshee_amd import hip
+    from banshee_amd.scenes import occluded_room
+    sc = occluded_room(width=G_RES, height=G_RES)
+    scene = hip.Scene(ctx, sc, width=bvh_width)
+    scene.render_guided(G_RES, G_RES, 1, G_DEPTH, 1, download=False)   # warm the wavefront buffers
+    ctx.sync()
+    tp, sp, ep, _ = time_to_rmse(ctx, scene, None, 4096, 100000)
+    guide = hip.Guide(ctx, seed=1, bsdf_fraction=G_ALPHA, lr=G_LR, beta2=0.99)
+    tg, sg, eg, tt = time_to_rmse(ctx, scene, guide, 4096, 200000)
+    return {"workload": f"occluded room ({sc.ntris} tris, the light behind a wall with 4 openings totalling 5% of its "
+                        f"area, the camera's side lit only through them), "
+                        f"{G_RES}x{G_RES}, depth {G_DEPTH}, quirk-free estimator",
+            "target_rmse": G_TARGET, "plain_s": round(tp, 3), "plain_spp": sp, "plain_rmse": round(ep, 5),
+            "guided_s": round(tg, 3), "guided_spp": sg, "guided_rmse": round(eg, 5),
+            "guided_train_s": round(tt, 3), "time_ratio": round(tg / tp, 3),
+            "guide": {"bsdf_fraction": G_ALPHA, "lr": G_LR, "adam_steps_per_pass": G_STEPS, "batch": G_BATCH,
+                      "spp_per_pass": G_PASS_GRID ** 2}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain time-to-RMSE comparison")
+    ap.add_argument("--bvh-width", type=int, default=2, choices=(2, 4), help="BVH node width")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -92,7 +168,7 @@ def main():
     t0 = time.time()
     sc = terrain_scene(width=W, height=H)
     ctx = hip.Context(0)
-    scene = hip.Scene(ctx, sc)
+    scene = hip.Scene(ctx, sc, width=args.bvh_width)
     info = scene.info()
     setup_s = time.time() - t0
     accum = ctx.alloc((H, W, 8), "float32")
@@ -104,7 +180,8 @@ def main():
     crays = cst["rays_closest"] + cst["rays_any"]
     v_node = cst["node_visits"] / crays
     v_tri = cst["tri_tests"] / crays
-    b_ray = 32 + 16 + 64 * v_node + 48 * v_tri
+    s_node = 128 if info["width"] == 4 else 64
+    b_ray = 32 + 16 + s_node * v_node + 48 * v_tri
 
     for _ in range(args.warmup):
         scene.render(W, H, GRID, DEPTH, SEED, accum=accum, download=False)
@@ -144,6 +221,7 @@ def main():
         "config": {"workload": "plain wavefront GPU path tracing, seeded terrain scene (heightfield 512x512 + 2000 "
                                "icospheres subdiv 3, GGX/Lambert 50/50, 2 quad area lights), 1920x1080, 64 spp, depth 8",
                    "tris": int(info["tris"]), "bvh_nodes": int(info["nodes"]), "bvh_depth": int(info["depth"]),
+                   "bvh_width": int(info["width"]),
                    "samples_per_s": round(samples / dt, 1), "rays_per_step": rays // args.steps,
                    "setup_s": round(setup_s, 2), "bvh_build_ms": round(info["build_ms"], 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -153,6 +231,8 @@ def main():
                      "tri_tests_per_ray": round(v_tri, 2), "trace_share": round(trace_ms / (dt * 1e3), 3)},
         "cpu_baseline": None,
     }
+    if not args.no_guided:
+        out["guided"] = guided_comparison(ctx, args.bvh_width)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(None)
     print(json.dumps(out), flush=True)

@@ -101,7 +101,8 @@ typedef struct
     int32_t max_leaf;   /* triangles per leaf, 1..8 (the reference uses 1, bvh.cpp:74) */
     int32_t sah_bins;   /* bins per axis for the binned SAH, 4..64 */
     int32_t threads;    /* host build threads, 0 = hardware concurrency */
-    int32_t reserved;
+    int32_t width;      /* node width: 2 (64-B binary nodes) or 4 (128-B 4-wide nodes collapsed from the
+                           binary SAH tree); 0 = 2. A tree too deep for the 4-wide stack stays binary. */
 } bsh_bvh_params;
 
 /* Frame parameters for bsh_render. The estimator is GiTracer(max_depth)
@@ -173,9 +174,15 @@ bsh_status bsh_scene_set_camera(bsh_scene* s, const bsh_camera* cam);
 /* World::Commit (world/world.cpp:5-16): host SAH build, flatten, upload. params may be NULL. */
 bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params);
 bsh_status bsh_scene_destroy(bsh_scene* s);
-/* Node count, triangle count, device bytes, build milliseconds, max depth. */
+/* The host half of bsh_scene_commit on its own (no device needed): builds the BVH of ntris
+   triangles (tri_verts: 9 floats each) and, when nodes is not NULL, copies the flattened nodes
+   (GpuNode 64 B or GpuNode4 128 B, per *width; nodes_cap in nodes) and the leaf order
+   (tri_order: ntris ints). Call with nodes = NULL first to learn *nnodes. */
+bsh_status bsh_bvh_build_host(const float* tri_verts, size_t ntris, const bsh_bvh_params* params, void* nodes,
+                              size_t nodes_cap, int32_t* tri_order, size_t* nnodes, int32_t* width, int32_t* depth);
+/* Node count, triangle count, device bytes, build milliseconds, max depth, BVH width built. */
 bsh_status bsh_scene_info(const bsh_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* bytes, double* build_ms,
-                          int32_t* depth);
+                          int32_t* depth, int32_t* width);
 
 /* ---------------- ray queries (Intersectable, accelerator/intersectable.h:41-49) ---------------- */
 /* Intersectable::Intersect(ray const&, Hit&) -> Bvh::Intersect closest hit (bvh.cpp:499-573).
@@ -288,7 +295,7 @@ typedef struct
 {
     uint32_t seed;        /* He-uniform initialisation seed */
     float lr, beta1, beta2, eps;   /* Adam */
-    float bsdf_fraction;  /* one-sample MIS: probability of sampling the BSDF instead of the guide */
+    float bsdf_fraction;  /* one-sample MIS: probability of sampling the BSDF instead of the guide, in (0, 1] */
     int32_t reserved[2];
 } bsh_guide_params;
 
@@ -299,6 +306,8 @@ bsh_status bsh_guide_destroy(bsh_guide* g);
    W3, b3, W4, b4, W5[64][48], b5); set resets the Adam state. Host pointers. */
 bsh_status bsh_guide_get_params(const bsh_guide* g, float* host);
 bsh_status bsh_guide_set_params(bsh_guide* g, const float* host);
+/* The one-sample MIS BSDF fraction used by later bsh_render_guided calls, in (0, 1]. */
+bsh_status bsh_guide_set_bsdf_fraction(bsh_guide* g, float bsdf_fraction);
 /* Network outputs for n rows of raw inputs (8 floats per row: position in [0,1]^3,
    roughness, normal, unused). Device pointers; out n x 48. */
 bsh_status bsh_guide_forward(bsh_guide* g, const float* inputs, size_t n, float* out, void* stream);

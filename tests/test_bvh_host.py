@@ -1,0 +1,136 @@
+"""The host BVH builder and both flattened node layouts (CPU only, through the C ABI).
+
+Invariants: every primitive is in exactly one leaf; every child box contains its subtree's
+triangles (boxes are padded outward); inner references are in range; the depth respects the
+traversal stack bound. A numpy traversal of the flattened nodes (visiting every child whose
+box the ray enters) finds the same closest hits as FP64 brute force, so a wrong box, ref,
+count or collapse step in either layout fails here before it reaches a GPU.
+"""
+import numpy as np
+import pytest
+
+from banshee_amd import hip
+from banshee_amd.scenes import terrain_for_tris
+
+pytestmark = pytest.mark.skipif(not __import__("os").path.exists(hip.LIB_PATH), reason="library not built")
+
+
+def tri_verts(sc):
+    return sc.verts[sc.tri_v].reshape(-1, 9).astype(np.float32)
+
+
+def decode(nodes, width):
+    """Per node: lo [k,3], hi [k,3], ref [k], cnt [k] for k = width children."""
+    if width == 2:
+        f = nodes
+        i = nodes.view(np.int32)
+        lo = np.stack([np.stack([f[:, 0], f[:, 2], f[:, 8]], -1), np.stack([f[:, 4], f[:, 6], f[:, 10]], -1)], 1)
+        hi = np.stack([np.stack([f[:, 1], f[:, 3], f[:, 9]], -1), np.stack([f[:, 5], f[:, 7], f[:, 11]], -1)], 1)
+        return lo, hi, i[:, 12:14], i[:, 14:16]
+    f = nodes.reshape(-1, 8, 4)
+    i = nodes.view(np.int32).reshape(-1, 8, 4)
+    lo = np.stack([f[:, 0], f[:, 2], f[:, 4]], -1)
+    hi = np.stack([f[:, 1], f[:, 3], f[:, 5]], -1)
+    return lo, hi, i[:, 6], i[:, 7]
+
+
+@pytest.fixture(scope="module")
+def scene():
+    return terrain_for_tris(4000, seed=9, width=64, height=64)
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_layout_invariants(scene, width):
+    tv = tri_verts(scene)
+    nodes, order, w, depth = hip.bvh_build_host(tv, width=width)
+    assert w == width
+    assert sorted(order.tolist()) == list(range(len(tv)))
+    lo, hi, ref, cnt = decode(nodes, w)
+    n = nodes.shape[0]
+    seen = np.zeros(len(tv), np.int32)
+    tris = tv.reshape(-1, 3, 3)
+    for k in range(n):
+        for c in range(w):
+            if cnt[k, c] > 0:
+                prims = order[ref[k, c]:ref[k, c] + cnt[k, c]]
+                seen[prims] += 1
+                p = tris[prims].reshape(-1, 3)
+                assert np.all(p >= lo[k, c]) and np.all(p <= hi[k, c])
+            elif cnt[k, c] == 0:
+                assert 0 < ref[k, c] < n
+                child = ref[k, c]
+                occupied = cnt[child] >= 0
+                assert np.all(lo[child][occupied] >= lo[k, c]) and np.all(hi[child][occupied] <= hi[k, c])
+    assert np.all(seen == 1)
+    bound = 32 if w == 2 else 48
+    assert (depth + 2 if w == 2 else 3 * depth + 1) <= bound
+
+
+def traverse_np(nodes, w, order, tris, o, d, tmin, tmax):
+    lo, hi, ref, cnt = decode(nodes, w)
+    inv = 1.0 / np.where(np.abs(d) < 1e-20, np.copysign(1e-20, d), d)
+    best, bt = -1, tmax
+    stack = [0]
+    while stack:
+        k = stack.pop()
+        for c in range(w):
+            if cnt[k, c] < 0:
+                continue
+            t0 = (lo[k, c] - o) * inv
+            t1 = (hi[k, c] - o) * inv
+            tn = max(np.max(np.minimum(t0, t1)), tmin)
+            tf = min(np.min(np.maximum(t0, t1)), bt)
+            if tn > tf:
+                continue
+            if cnt[k, c] == 0:
+                stack.append(ref[k, c])
+                continue
+            for q in range(ref[k, c], ref[k, c] + cnt[k, c]):
+                p1, p2, p3 = tris[order[q]].astype(np.float64)
+                e1, e2 = p2 - p1, p3 - p1
+                s1 = np.cross(d, e2)
+                det = s1 @ e1
+                if det == 0:
+                    continue
+                dd = o - p1
+                b1 = (dd @ s1) / det
+                s2 = np.cross(dd, e1)
+                b2 = (d @ s2) / det
+                t = (e2 @ s2) / det
+                if b1 >= 0 and b1 <= 1 and b2 >= 0 and b1 + b2 <= 1 and tmin < t <= bt:
+                    best, bt = order[q], t
+    return best, bt
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_layout_traversal_matches_brute_force(scene, width):
+    tv = tri_verts(scene)
+    nodes, order, w, _ = hip.bvh_build_host(tv, width=width)
+    tris = tv.reshape(-1, 3, 3)
+    rng = np.random.default_rng(width)
+    lo, hi = scene.verts.min(0), scene.verts.max(0)
+    from test_oracle_pins import brute_force
+    n = 150
+    o = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, np.full((n, 1), 0.01), d, np.full((n, 1), 1e4)], 1).astype(np.float32)
+    bp, bt = brute_force(scene, rays)
+    r64 = rays.astype(np.float64)
+    hits = 0
+    for i in range(n):
+        p, t = traverse_np(nodes, w, order, tris, r64[i, 0:3], r64[i, 4:7], r64[i, 3], r64[i, 7])
+        assert (p >= 0) == (bp[i] >= 0), i
+        if p >= 0:
+            hits += 1
+            assert abs(t - bt[i]) <= 1e-9 * max(1.0, bt[i]), (i, t, bt[i])
+    assert hits > n // 4
+
+
+def test_four_wide_collapses_levels(scene):
+    tv = tri_verts(scene)
+    n2, _, _, d2 = hip.bvh_build_host(tv, width=2)
+    n4, _, w, d4 = hip.bvh_build_host(tv, width=4)
+    assert w == 4
+    assert len(n4) < 0.6 * len(n2)
+    assert d4 < d2

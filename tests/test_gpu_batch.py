@@ -12,9 +12,9 @@ def _jobs(ctx):
     cfgs = batch_jobs(n=5, seed=300, min_tris=1_000, max_tris=40_000, min_res=32, max_res=96, min_grid=2,
                       max_grid=4, min_depth=3, max_depth=8)
     scenes, jobs = [], []
-    for c in cfgs:
+    for k, c in enumerate(cfgs):
         sc = terrain_for_tris(c["tris"], seed=c["seed"], width=c["res"], height=c["res"])
-        s = hip.Scene(ctx, sc)
+        s = hip.Scene(ctx, sc, width=4 if k % 2 else 2)   # one chunk mixes both node layouts
         scenes.append(s)
         jobs.append(("render", s, dict(width=c["res"], height=c["res"], img_grid=c["grid"], max_depth=c["depth"],
                                        seed=c["seed"])))

@@ -8,23 +8,26 @@ from conftest import golden
 pytestmark = pytest.mark.gpu
 
 
-def _scene(ctx, sc):
+def _scene(ctx, sc, width=2):
     from banshee_amd import hip
-    return hip.Scene(ctx, sc)
+    s = hip.Scene(ctx, sc, width=width)
+    assert s.info()["width"] == width
+    return s
 
 
+@pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("name", ["cornell", "terrain"])
-def test_recorded_rays_match_reference(ctx, cornell, terrain100k, name):
+def test_recorded_rays_match_reference(ctx, cornell, terrain100k, name, width):
     fx = golden(f"rays_{name}")
     sc = cornell if name == "cornell" else terrain100k
     assert str(fx["digest"]) == sc.digest(), "scene generator drifted from the fixture"
-    s = _scene(ctx, sc)
+    s = _scene(ctx, sc, width)
     rays, kind, prim, thit = fx["rays"], fx["kind"], fx["prim"], fx["thit"]
     cl = kind == 0
     gp, gt = s.intersect(rays[cl])
     rp, rt = prim[cl], thit[cl]
     same = gp == rp
-    print(f"{name}: closest {cl.sum()} rays, prim agreement {same.mean():.6f}")
+    print(f"{name} width {width}: closest {cl.sum()} rays, prim agreement {same.mean():.6f}")
     assert same.mean() >= 0.9999 or (~same).sum() <= 1
     hit = same & (rp >= 0)
     # identical Moller-Trumbore arithmetic: t and barycentrics must agree to 1e-5 relative
@@ -77,14 +80,15 @@ def _brute_force_fp32(sc, rays):
     return best, tb
 
 
-def test_brute_force_cornell(ctx, cornell):
+@pytest.mark.parametrize("width", [2, 4])
+def test_brute_force_cornell(ctx, cornell, width):
     rng = np.random.default_rng(5)
     n = 20000
     o = rng.uniform(0.05, 1.95, (n, 3))
     d = rng.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     rays = np.concatenate([o, np.full((n, 1), 0.01), d, np.full((n, 1), 1e6)], 1).astype(np.float32)
-    s = _scene(ctx, cornell)
+    s = _scene(ctx, cornell, width)
     gp, gt = s.intersect(rays)
     bp, bt = _brute_force_fp32(cornell, rays)
     same = gp == bp
@@ -96,8 +100,9 @@ def test_brute_force_cornell(ctx, cornell):
     assert np.array_equal(occ == 1, bp >= 0) or (occ != (bp >= 0)).sum() <= len(bad)
 
 
-def test_edge_cases(ctx, cornell):
-    s = _scene(ctx, cornell)
+@pytest.mark.parametrize("width", [2, 4])
+def test_edge_cases(ctx, cornell, width):
+    s = _scene(ctx, cornell, width)
     # tmax == t accepted, tmin == t rejected (mesh.cpp:33); floor at y = 0
     ray_down = np.array([[1.0, 1.0, 1.0, 0.01, 0.0, -1.0, 0.0, 1.0]], np.float32)
     p, t = s.intersect(ray_down)
