@@ -425,12 +425,14 @@ bsh_status bsh_render_guided(bsh_scene* s, const bsh_render_params* p, bsh_guide
     {
         J.flags |= JOB_GUIDED;
         J.bsdf_frac = guide_bsdf_fraction(guide);
+        J.guide_bounces = guide_bounces(guide);
     }
     if (records && cap) J.flags |= JOB_RECORD;
     GuideRun gr;
     gr.guide = guide;
     gr.records = records;
     gr.cap = cap;
+    gr.bounces = guide ? guide_bounces(guide) : 0;
     std::vector<JobDev> jobs(1, J);
     r = run_jobs(s->ctx, jobs, (p->flags & BSH_RENDER_COUNT) != 0, nullptr, 0, (hipStream_t)stream, &gr);
     if (nrecords) *nrecords = gr.written;

@@ -134,10 +134,12 @@ constexpr int kFwdWaves = 8;
 // v0..v6) -> 48 FP32 outputs per row. Weights and biases are staged in LDS once per block;
 // each wave runs 16-row tiles with all activations in registers. With kTrain the encoded
 // input and the four hidden activations are also written (FP16, row-major) for backward.
-template <bool kTrain>
+// kHalf: write only the kGUsed outputs the mixture uses, in FP16 (the wavefront's form);
+// seg (when set): rows come in segments of kGuideSeg of which the first seg[s] are live.
+template <bool kTrain, bool kHalf>
 __global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet g, const float4* in, int stride,
-                                                                 uint32_t nrows, float* out, _Float16* x0,
-                                                                 _Float16* hs)
+                                                                 uint32_t nrows, const uint32_t* seg, void* outp,
+                                                                 _Float16* x0, _Float16* hs)
 {
     __shared__ _Float16 lw[kLFTotal];
     __shared__ __align__(16) float lb[kBiasTotal];
@@ -154,7 +156,13 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet g, co
         asm volatile("" : "+v"(zoff));
         const _Float16* W = lw + zoff;
         const uint32_t row = tile * 16 + r;
-        const bool live = row < nrows;
+        uint32_t seg_live = kGuideSeg;
+        if (seg)
+        {
+            seg_live = seg[(tile * 16) / kGuideSeg];          // the same for the whole tile
+            if ((tile * 16) % kGuideSeg >= seg_live) continue;  // tile past the segment's rows
+        }
+        const bool live = row < nrows && row % kGuideSeg < seg_live;
         float4 a = make_float4(0.f, 0.f, 0.f, 0.f), c = a;
         if (live)
         {
@@ -188,12 +196,26 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet g, co
                 acc = mfma16(w, h2[ks], acc);
             }
             float4 bb = *reinterpret_cast<const float4*>(&lb[4 * kGHid + mt * 16 + 4 * q]);
-            if (live)
-                *reinterpret_cast<float4*>(&out[(size_t)row * kGOut + mt * 16 + 4 * q]) =
+            if (!live) continue;
+            if (kHalf)
+            {
+                if (mt * 16 + 4 * q < kGUsed)
+                {
+                    h4 o = {(_Float16)(acc[0] + bb.x), (_Float16)(acc[1] + bb.y), (_Float16)(acc[2] + bb.z),
+                            (_Float16)(acc[3] + bb.w)};
+                    *reinterpret_cast<h4*>(&static_cast<_Float16*>(outp)[(size_t)row * kGUsed + mt * 16 + 4 * q]) = o;
+                }
+            }
+            else
+            {
+                *reinterpret_cast<float4*>(&static_cast<float*>(outp)[(size_t)row * kGOut + mt * 16 + 4 * q]) =
                     make_float4(acc[0] + bb.x, acc[1] + bb.y, acc[2] + bb.z, acc[3] + bb.w);
+            }
         }
     }
 }
+
+constexpr float kMaxRecordWeight = 256.f;
 
 // Per-row loss -w log q(w_dir) and its gradient w.r.t. the 48 outputs. Weights are
 // normalised to mean 1 over the batch (s = w * rows / sum w) so FP16 gradients stay O(1);
@@ -218,8 +240,10 @@ __global__ void k_guide_loss(const float* out, const float4* rec, uint32_t nrows
     }
     float g[kGOut];
     for (int k = 0; k < kGOut; ++k) g[k] = 0.f;
-    float s = wt * scale;
-    if (q > 0.f && s > 0.f)
+    // weights normalised to mean 1, capped so one rare record cannot overflow the FP16
+    // gradients (this shapes the proposal only; the rendered estimate stays unbiased)
+    float s = fminf(wt * scale, kMaxRecordWeight);
+    if (q > 0.f && s > 0.f && isfinite(q) && isfinite(s))
     {
         for (int k = 0; k < kGLobes; ++k)
         {
@@ -250,19 +274,40 @@ __global__ void k_guide_loss(const float* out, const float4* rec, uint32_t nrows
     for (int k = 0; k < kGOut; ++k) dout[(size_t)row * kGOut + k] = (_Float16)g[k];
 }
 
-__global__ void k_guide_wsum(const float4* rec, uint32_t nrows, float* wsum)
+// Sum of the record weights in a fixed order, so that training is reproducible bit for bit:
+// kWsumParts blocks each reduce a contiguous range with a fixed tree (pass 1), then one block
+// adds the partial sums the same way (pass 2).
+constexpr int kWsumBlock = 256, kWsumParts = 256;
+__device__ __forceinline__ float block_sum(float s, float* red)
 {
-    __shared__ float red[256];
-    float s = 0.f;
-    for (uint32_t r = blockIdx.x * 256 + threadIdx.x; r < nrows; r += gridDim.x * 256) s += rec[3 * r + 2].w;
     red[threadIdx.x] = s;
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1)
+    for (int st = kWsumBlock / 2; st > 0; st >>= 1)
     {
         if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(wsum, red[0]);
+    return red[0];
+}
+
+__global__ __launch_bounds__(kWsumBlock) void k_guide_wsum(const float4* rec, uint32_t nrows, float* part)
+{
+    __shared__ float red[kWsumBlock];
+    const uint32_t per = (nrows + kWsumParts - 1) / kWsumParts;
+    const uint32_t r0 = blockIdx.x * per, r1 = min(r0 + per, nrows);
+    float s = 0.f;
+    for (uint32_t r = r0 + threadIdx.x; r < r1; r += kWsumBlock) s += rec[3 * r + 2].w;
+    float t = block_sum(s, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kWsumBlock) void k_guide_wsum_final(const float* part, float* wsum)
+{
+    __shared__ float red[kWsumBlock];
+    float s = 0.f;
+    for (int k = threadIdx.x; k < kWsumParts; k += kWsumBlock) s += part[k];
+    float t = block_sum(s, red);
+    if (threadIdx.x == 0) wsum[0] = t;
 }
 
 // dX[rows x NIN] = (dY[rows x KOUT] W^T) * (H > 0), one wave per 16 rows; W (backward
@@ -391,8 +436,20 @@ __global__ __launch_bounds__(256) void k_guide_wgrad(const _Float16* x, const _F
     for (int t = threadIdx.x; t < KIN * NOUT + NOUT; t += 256) P[t] = red[t];
 }
 
-// Sums the partial gradients of one layer (chunk order fixed) into grad[W..., b...] and
-// undoes the loss scaling.
+// Sums the partial gradients of one layer into grad[W..., b...] in a fixed order and undoes
+// the loss scaling, in two passes so that no thread walks all chunks: pass 1 sums groups of
+// kRedGroup chunks (blockIdx.y = group), pass 2 sums the groups.
+constexpr int kRedGroup = 16;
+__global__ void k_guide_reduce_groups(const float* part, int nchunks, int n, float* out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int c0 = blockIdx.y * kRedGroup, c1 = min(c0 + kRedGroup, nchunks);
+    float s = 0.f;
+    for (int c = c0; c < c1; ++c) s += part[(size_t)c * n + t];
+    out[(size_t)blockIdx.y * n + t] = s;
+}
+
 __global__ void k_guide_reduce(const float* part, int nchunks, int n, float inv_rows, float* grad)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -415,6 +472,7 @@ __global__ void k_guide_adam(float* p, float* m, float* v, const float* grad, in
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     float g = grad[t];
+    if (!isfinite(g)) g = 0.f;   // an overflowed entry leaves its parameter as it is
     float mm = b1 * m[t] + (1.f - b1) * g;
     float vv = b2 * v[t] + (1.f - b2) * g * g;
     m[t] = mm;
@@ -457,7 +515,7 @@ __global__ void k_diag_guide(uint32_t n, const float* o, const float* u, const f
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     Lobes L = guide_lobes(o + (size_t)t * kGOut);
-    f3 w = guide_sample(L, u[3 * t], u[3 * t + 1], u[3 * t + 2]);
+    f3 w = guide_sample(L, o + (size_t)t * kGOut, u[3 * t], u[3 * t + 1], u[3 * t + 2]);
     f3 d = mk(dirs[3 * t], dirs[3 * t + 1], dirs[3 * t + 2]);
     float ws = 0.f;
     for (int k = 0; k < kGLobes; ++k) ws += L.w[k];
@@ -484,7 +542,7 @@ struct bsh_guide
     bsh_context* ctx = nullptr;
     bsh_guide_params prm{};
     std::vector<float> host_params;
-    DevBuf<float> p, m, v, grad, bias, part, out, loss, wsum;
+    DevBuf<float> p, m, v, grad, bias, part, part2, out, loss, wsum;
     DevBuf<_Float16> wf, wb, x0, hs, d5, d4, d3, d2, d1;
     int step = 0;
     size_t cap = 0;
@@ -551,19 +609,33 @@ static bsh_status ensure_train(bsh_guide* g, size_t rows)
     BSH_HIP(g->d1.alloc(rows * kGHid));
     size_t nchunks = (rows + kWgRows - 1) / kWgRows;
     BSH_HIP(g->part.alloc(nchunks * (size_t)(kGIn * kGHid + kGHid)));
+    BSH_HIP(g->part2.alloc(((nchunks + kRedGroup - 1) / kRedGroup) * (size_t)(kGIn * kGHid + kGHid)));
     g->cap = rows;
     return BSH_OK;
 }
 
 float guide_bsdf_fraction(const bsh_guide* g) { return g->prm.bsdf_fraction; }
+int guide_bounces(const bsh_guide* g) { return g->prm.guided_bounces; }
 
 bsh_status guide_forward(bsh_guide* g, const float4* in, int stride, uint32_t n, float* out, hipStream_t st)
 {
     if (n == 0) return BSH_OK;
     uint32_t tiles = (n + 15) / 16;
     uint32_t blocks = std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
-    hipLaunchKernelGGL(k_guide_forward<false>, dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), in, stride, n, out,
-                       (_Float16*)nullptr, (_Float16*)nullptr);
+    hipLaunchKernelGGL((k_guide_forward<false, false>), dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), in,
+                       stride, n, (const uint32_t*)nullptr, (void*)out, (_Float16*)nullptr, (_Float16*)nullptr);
+    BSH_HIP(hipGetLastError());
+    return BSH_OK;
+}
+
+bsh_status guide_forward_rows(bsh_guide* g, const float4* in, const uint32_t* seg_counts, uint32_t rows,
+                              _Float16* out, hipStream_t st)
+{
+    if (rows == 0) return BSH_OK;
+    uint32_t tiles = (rows + 15) / 16;
+    uint32_t blocks = std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
+    hipLaunchKernelGGL((k_guide_forward<false, true>), dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), in, 2,
+                       rows, seg_counts, (void*)out, (_Float16*)nullptr, (_Float16*)nullptr);
     BSH_HIP(hipGetLastError());
     return BSH_OK;
 }
@@ -574,8 +646,19 @@ static void wgrad_layer(bsh_guide* g, const _Float16* x, const _Float16* dy, uin
     int nchunks = (int)((n + kWgRows - 1) / kWgRows);
     hipLaunchKernelGGL((k_guide_wgrad<K, N>), dim3(nchunks), dim3(256), 0, st, x, dy, n, g->part.p);
     int cnt = K * N + N;
-    hipLaunchKernelGGL(k_guide_reduce, dim3((cnt + 255) / 256), dim3(256), 0, st, g->part.p, nchunks, cnt,
-                       1.f / (float)n, g->grad.p + param_off);
+    if (nchunks > kRedGroup)
+    {
+        int groups = (nchunks + kRedGroup - 1) / kRedGroup;
+        hipLaunchKernelGGL(k_guide_reduce_groups, dim3((cnt + 255) / 256, groups), dim3(256), 0, st, g->part.p,
+                           nchunks, cnt, g->part2.p);
+        hipLaunchKernelGGL(k_guide_reduce, dim3((cnt + 255) / 256), dim3(256), 0, st, g->part2.p, groups, cnt,
+                           1.f / (float)n, g->grad.p + param_off);
+    }
+    else
+    {
+        hipLaunchKernelGGL(k_guide_reduce, dim3((cnt + 255) / 256), dim3(256), 0, st, g->part.p, nchunks, cnt,
+                           1.f / (float)n, g->grad.p + param_off);
+    }
 }
 
 // Forward + backward on `rec` (3 float4 per row); leaves the batch gradient in g->grad
@@ -586,12 +669,12 @@ static bsh_status guide_gradients(bsh_guide* g, const float4* rec, uint32_t n, h
     if (r != BSH_OK) return r;
     hipLaunchKernelGGL(k_guide_zero, dim3(1), dim3(64), 0, st, g->wsum.p, 1);
     hipLaunchKernelGGL(k_guide_zero, dim3(1), dim3(64), 0, st, g->loss.p, 1);
-    hipLaunchKernelGGL(k_guide_wsum, dim3(std::min<uint32_t>((n + 255) / 256, 1024)), dim3(256), 0, st, rec, n,
-                       g->wsum.p);
+    hipLaunchKernelGGL(k_guide_wsum, dim3(kWsumParts), dim3(kWsumBlock), 0, st, rec, n, g->wsum.p + 1);
+    hipLaunchKernelGGL(k_guide_wsum_final, dim3(1), dim3(kWsumBlock), 0, st, g->wsum.p + 1, g->wsum.p);
     uint32_t tiles = (n + 15) / 16;
     uint32_t blocks = std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
-    hipLaunchKernelGGL(k_guide_forward<true>, dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), rec, 3, n,
-                       g->out.p, g->x0.p, g->hs.p);
+    hipLaunchKernelGGL((k_guide_forward<true, false>), dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), rec, 3, n,
+                       (const uint32_t*)nullptr, (void*)g->out.p, g->x0.p, g->hs.p);
     hipLaunchKernelGGL(k_guide_loss, dim3((n + 255) / 256), dim3(256), 0, st, g->out.p, rec, n, g->wsum.p, g->d5.p,
                        g->loss.p);
     const _Float16* H1 = g->hs.p;
@@ -624,12 +707,13 @@ bsh_status bsh_guide_create(bsh_context* ctx, const bsh_guide_params* prm, bsh_g
     BSH_HIP(hipSetDevice(ctx->device));
     bsh_guide* g = new bsh_guide();
     g->ctx = ctx;
-    bsh_guide_params d{1u, 5e-3f, 0.9f, 0.999f, 1e-8f, 0.5f, {0, 0}};
+    bsh_guide_params d{1u, 5e-3f, 0.9f, 0.999f, 1e-8f, 0.5f, 0, 0};
     g->prm = prm ? *prm : d;
-    if (!(g->prm.bsdf_fraction > 0.f && g->prm.bsdf_fraction <= 1.f))
+    if (!(g->prm.bsdf_fraction > 0.f && g->prm.bsdf_fraction <= 1.f) || g->prm.guided_bounces < 0)
     {
         delete g;
-        return fail(BSH_ERR_INVALID, "bsh_guide_create: the BSDF fraction must be in (0, 1] (unbiasedness)");
+        return fail(BSH_ERR_INVALID, "bsh_guide_create: the BSDF fraction must be in (0, 1] (unbiasedness) and "
+                                     "guided_bounces >= 0");
     }
     g->host_params.assign(kGParams, 0.f);
     // He-uniform weights (limit sqrt(6 / fan_in)), zero biases, from PCG32(seed)
@@ -655,7 +739,7 @@ bsh_status bsh_guide_create(bsh_context* ctx, const bsh_guide_params* prm, bsh_g
         BSH_HIP(g->wf.alloc(kFTotal));
         BSH_HIP(g->wb.alloc(kBTotal));
         BSH_HIP(g->loss.alloc(1));
-        BSH_HIP(g->wsum.alloc(1));
+        BSH_HIP(g->wsum.alloc(1 + kWsumParts));   // the sum, then the partial sums
         BSH_HIP(hipMemset(g->m.p, 0, kGParams * 4));
         BSH_HIP(hipMemset(g->v.p, 0, kGParams * 4));
         return BSH_OK;
@@ -750,6 +834,23 @@ bsh_status bsh_guide_train_step(bsh_guide* g, const float* records, size_t n, fl
         BSH_HIP(hipMemcpyAsync(&l, g->loss.p, 4, hipMemcpyDeviceToHost, st));
         BSH_HIP(hipStreamSynchronize(st));
         *loss = l / (float)n;
+    }
+    return BSH_OK;
+}
+
+bsh_status bsh_guide_train(bsh_guide* g, const float* records, size_t n, int steps, size_t batch, float* loss,
+                           void* stream)
+{
+    if (!g || !records) return fail(BSH_ERR_INVALID, "bsh_guide_train: NULL argument");
+    if (steps < 0 || batch == 0) return fail(BSH_ERR_INVALID, "bsh_guide_train: bad steps or batch");
+    if (n == 0 || steps == 0) return BSH_OK;
+    const size_t rows = std::min(batch, n);
+    for (int s = 0; s < steps; ++s)
+    {
+        // consecutive batches over the records, wrapping so every batch is full
+        size_t off = n > batch ? ((size_t)s * batch) % (n - batch + 1) : 0;
+        bsh_status r = bsh_guide_train_step(g, records + off * 12, rows, s + 1 == steps ? loss : nullptr, stream);
+        if (r != BSH_OK) return r;
     }
     return BSH_OK;
 }

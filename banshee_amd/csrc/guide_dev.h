@@ -20,6 +20,7 @@ namespace bsh {
 constexpr int kGIn = 96;       // padded encoded input width
 constexpr int kGHid = 64;      // hidden width
 constexpr int kGOut = 48;      // padded output width
+constexpr int kGUsed = 40;     // outputs the mixture uses
 constexpr int kGLobes = 8;
 constexpr int kGRaw = 7;
 constexpr int kGBands = 6;
@@ -40,26 +41,34 @@ struct Lobes
     float kappa[kGLobes];
 };
 
-BSH_HD Lobes guide_lobes(const float* o)
+template <class T>
+BSH_HD Lobes guide_lobes(const T* o)
 {
     Lobes L;
-    float mx = o[0];
-    for (int k = 1; k < kGLobes; ++k) mx = fmaxf(mx, o[k]);
+    float mx = (float)o[0];
+    for (int k = 1; k < kGLobes; ++k) mx = fmaxf(mx, (float)o[k]);
     float s = 0.f;
     for (int k = 0; k < kGLobes; ++k)
     {
-        L.w[k] = expf(o[k] - mx);
+        L.w[k] = expf((float)o[k] - mx);
         s += L.w[k];
     }
     for (int k = 0; k < kGLobes; ++k)
     {
         L.w[k] = L.w[k] / s;
-        f3 r = mk(o[8 + 3 * k], o[9 + 3 * k], o[10 + 3 * k]);
+        f3 r = mk((float)o[8 + 3 * k], (float)o[9 + 3 * k], (float)o[10 + 3 * k]);
         float n = sqrtf(sqnorm(r));
         L.mu[k] = n > 0.f ? r * (1.f / n) : mk(0.f, 0.f, 1.f);
-        L.kappa[k] = expf(clampf(o[32 + k], kLogKappaLo, kLogKappaHi));
+        L.kappa[k] = expf(clampf((float)o[32 + k], kLogKappaLo, kLogKappaHi));
     }
     return L;
+}
+
+BSH_HD bool lobes_finite(const Lobes& L)
+{
+    float s = 0.f;
+    for (int k = 0; k < kGLobes; ++k) s += L.w[k] + L.kappa[k] + L.mu[k].x + L.mu[k].y + L.mu[k].z;
+    return isfinite(s);
 }
 
 BSH_HD float vmf_pdf(f3 mu, float kappa, f3 w)
@@ -75,27 +84,34 @@ BSH_HD float guide_pdf(const Lobes& L, f3 w)
 }
 
 // Lobe chosen by u0 through the weight CDF, then Jakob's stable vMF sample with (u1, u2).
-BSH_HD f3 guide_sample(const Lobes& L, float u0, float u1, float u2)
+// The chosen lobe's direction and concentration are decoded again from the raw outputs `o`
+// (the same operations as guide_lobes, so the same values): indexing the Lobes registers by
+// a computed lobe would move them to scratch memory.
+template <class T>
+BSH_HD f3 guide_sample(const Lobes& L, const T* o, float u0, float u1, float u2)
 {
     int k = kGLobes - 1;
     float c = 0.f;
+    bool found = false;
+#pragma unroll
     for (int j = 0; j < kGLobes; ++j)
     {
         c += L.w[j];
-        if (u0 < c)
-        {
-            k = j;
-            break;
-        }
+        bool take = !found && u0 < c;
+        k = take ? j : k;
+        found = found || take;
     }
-    float kap = L.kappa[k];
+    f3 r = mk((float)o[8 + 3 * k], (float)o[9 + 3 * k], (float)o[10 + 3 * k]);
+    float n = sqrtf(sqnorm(r));
+    f3 mu = n > 0.f ? r * (1.f / n) : mk(0.f, 0.f, 1.f);
+    float kap = expf(clampf((float)o[32 + k], kLogKappaLo, kLogKappaHi));
     float w = 1.f + logf(u1 + (1.f - u1) * expf(-2.f * kap)) / kap;
     w = clampf(w, -1.f, 1.f);
     float s = sqrtf(fmaxf(0.f, 1.f - w * w));
-    f3 t1 = orthovector(L.mu[k]);
-    f3 t2 = cross(L.mu[k], t1);
+    f3 t1 = orthovector(mu);
+    f3 t2 = cross(mu, t1);
     float phi = 2.f * kPi * u2;
-    return normalize(L.mu[k] * w + t1 * (s * cosf(phi)) + t2 * (s * sinf(phi)));
+    return normalize(mu * w + t1 * (s * cosf(phi)) + t2 * (s * sinf(phi)));
 }
 
 }  // namespace bsh

@@ -72,8 +72,10 @@ struct RenderWork
     DevBuf<int4> nee_res;
     DevBuf<float4> qc_o[2], qc_d[2], qa_o[2], qa_d[2];   // closest / any-hit queues
     DevBuf<int> qc_own[2], qa_own[2];
-    DevBuf<float4> gin, rec;          // guide network inputs (2 per path), training records (10 per path)
-    DevBuf<float> gout;               // guide network outputs (48 per path)
+    DevBuf<float4> gin, rec;          // guide network inputs (2 per row), training records (per path)
+    DevBuf<_Float16> gout;            // guide network outputs (kGUsed per row)
+    DevBuf<int> grow;                 // per path: its guide row this pass
+    DevBuf<uint32_t> gseg;            // guide rows per block segment this pass
     DevBuf<uint32_t> rec_count;
     DevBuf<uint32_t> counters;     // per launch: count, work, (pad) x 4
     DevBuf<unsigned long long> stats;  // closest, any
@@ -102,6 +104,7 @@ struct JobDev
     float* radiance;
     int flags;             // JOB_NO_QUIRKS, JOB_GUIDED, JOB_RECORD
     float bsdf_frac;       // guided: probability of sampling the BSDF
+    int guide_bounces;     // guided: vertices 0 .. guide_bounces-1 use the guide (0 = all)
     float glo[3], ginv[3]; // guided: scene box for the network's position input
 };
 
@@ -113,9 +116,17 @@ struct GuideRun
     float* records = nullptr;     // training records out (12 floats each)
     size_t cap = 0;
     size_t written = 0;
+    int bounces = 0;              // guided vertices per path (0 = all)
 };
 bsh_status guide_forward(bsh_guide* g, const float4* in, int stride, uint32_t n, float* out, hipStream_t st);
+// Inference for the wavefront: rows of 2-float4 inputs in segments of kGuideSeg rows, of
+// which the first seg_counts[s] (device) are live; writes the kGUsed used outputs per row in
+// FP16. Tiles past a segment's count are skipped.
+constexpr int kGuideSeg = 256;
+bsh_status guide_forward_rows(bsh_guide* g, const float4* in, const uint32_t* seg_counts, uint32_t rows,
+                              _Float16* out, hipStream_t st);
 float guide_bsdf_fraction(const bsh_guide* g);
+int guide_bounces(const bsh_guide* g);
 
 bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& jobs, bool count, const bsh_tuning* tune, int use_graph,
                     hipStream_t st, GuideRun* gr = nullptr);

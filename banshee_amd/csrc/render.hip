@@ -45,6 +45,9 @@ constexpr int kBlock = 256;
 // layout's deeper entries go to per-thread global slots. Measured on the 3M-triangle
 // terrain, more waves per SIMD with a smaller LDS part did not trace faster.
 constexpr int kTraceLds = 32;
+// Training records are kept for the first kRecVerts vertices of a path, 5 float4 each.
+constexpr int kRecVerts = 4, kRecStride = 5 * kRecVerts;
+static_assert(kGuideSeg == kBlock, "guide row segments are the blocks of k_guide_inputs");
 constexpr int kMaxTraceBlocksPerCU = 5;
 
 enum : uint32_t
@@ -74,9 +77,11 @@ struct RenderDev
     uint32_t* counters;            // per launch L: closest count, closest work, any count, any work
     unsigned long long* stats;     // closest, any, node visits, triangle tests
     uint32_t* stk_ovf;             // traversal stack entries beyond kTraceLds, [entry][trace thread]
-    float4* gin;                   // guide inputs, 2 per path
-    const float* gout;             // guide outputs, 48 per path
-    float4* rec;                   // per-path training records: 2 x 5 float4
+    float4* gin;                   // guide inputs, 2 per row, rows compacted over the guided vertices
+    const _Float16* gout;          // guide outputs, kGUsed per row (FP16)
+    int* grow;                     // per path: its row this pass, -1 when the vertex is not guided
+    uint32_t* gseg;                // per block of kBlock paths: guided rows packed at the front of its segment
+    float4* rec;                   // per-path training records: kRecVerts x 5 float4
     float* rec_out;                // appended records, 12 floats each
     uint32_t rec_cap;
     uint32_t* rec_count;
@@ -113,8 +118,7 @@ __device__ __forceinline__ void init_path(const RenderDev& R, uint32_t i, f3 o, 
     R.qc_own[0][i] = (int)((Q_EXT << 29) | i);
     if (R.rec)
     {
-        R.rec[(size_t)i * 10 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
-        R.rec[(size_t)i * 10 + 7] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int k = 0; k < kRecVerts; ++k) R.rec[(size_t)i * kRecStride + 5 * k + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
 
@@ -186,20 +190,27 @@ __device__ __forceinline__ uint32_t path_of(const RenderDev& R, uint32_t kind, u
     return kind == Q_EXT ? idx : (idx >> R.Kshift);
 }
 
-// Network inputs of every path whose extension ray hit something (guided jobs): position in
-// the scene box, roughness (GGX alpha, 1 otherwise), shading normal facing the viewer.
-__global__ __launch_bounds__(kBlock) void k_guide_inputs(RenderDev R)
+// Network inputs of every path whose extension ray hit something at a guided vertex of pass b:
+// position in the scene box, roughness (GGX alpha, 1 otherwise), shading normal facing the
+// viewer. Each block packs its rows at the front of its own kBlock-row segment (a block-wide
+// prefix count, no global atomics) and stores how many it wrote; each path records its row.
+__global__ __launch_bounds__(kBlock) void k_guide_inputs(RenderDev R, int b)
 {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= R.P) return;
-    uint4 meta = R.meta[i];
-    const JobDev& J = R.jobs[meta.y >> 24];
+    __shared__ uint32_t wsum[kBlock / 64];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float4 a = make_float4(0.f, 0.f, 0.f, 0.f), c = make_float4(0.f, 0.f, 0.f, 0.f);
-    if ((J.flags & JOB_GUIDED) && (meta.y & F_EXT) && !(meta.y & F_DONE))
+    bool want = false;
+    uint4 meta = make_uint4(0u, 0u, 0u, 0u);
+    if (i < R.P) meta = R.meta[i];
+    const JobDev& J = R.jobs[meta.y >> 24];
+    if (i < R.P && (J.flags & JOB_GUIDED) && (J.guide_bounces == 0 || b < J.guide_bounces) && (meta.y & F_EXT) &&
+        !(meta.y & F_DONE))
     {
         int4 hr = R.hit[i];
         if (hr.x >= 0)
         {
+            want = true;
             f3 p1, p2, p3, n1, n2, n3;
             int mat, bl;
             tri_verts(J.s, hr.x, p1, p2, p3, n1, n2, n3, mat, bl);
@@ -212,8 +223,25 @@ __global__ __launch_bounds__(kBlock) void k_guide_inputs(RenderDev R)
             c = make_float4(n.x, n.y, n.z, 1.f);
         }
     }
-    R.gin[2 * i] = a;
-    R.gin[2 * i + 1] = c;
+    const unsigned long long m = __ballot(want);
+    if (lane == 0) wsum[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t base = blockIdx.x * kBlock, total = 0;
+    for (int w = 0; w < kBlock / 64; ++w)
+    {
+        if (w < wave) base += wsum[w];
+        total += wsum[w];
+    }
+    if (threadIdx.x == 0) R.gseg[blockIdx.x] = total;
+    if (i >= R.P) return;
+    int row = -1;
+    if (want)
+    {
+        row = (int)(base + __popcll(m & ((1ull << lane) - 1ull)));
+        R.gin[2 * row] = a;
+        R.gin[2 * row + 1] = c;
+    }
+    R.grow[i] = row;
 }
 
 template <bool kCount>
@@ -303,6 +331,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
     const int K = 1 << R.Kshift;
     bool emit_ext = false;
     uint32_t emit_mask = 0;
+    uint32_t rec_mask = 0;               // training records this path emits at its end
+    float rec_w[kRecVerts];
     f3 org = mk(0.f, 0.f, 0.f), ext_d = mk(0.f, 0.f, 0.f);
     if (i < R.P)
     {
@@ -374,10 +404,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                 fl &= ~F_NEE;
             }
             // training record of the vertex shaded in the previous pass: radiance so far
-            if ((jflags & JOB_RECORD) && b >= 1 && b <= 2)
+            if ((jflags & JOB_RECORD) && b >= 1 && b <= kRecVerts)
             {
-                float4 v = R.rec[(size_t)i * 10 + (b - 1) * 5 + 2];
-                if (v.w > 0.f) R.rec[(size_t)i * 10 + (b - 1) * 5 + 4] = st3(rad, 1.f);
+                float4 v = R.rec[(size_t)i * kRecStride + (b - 1) * 5 + 2];
+                if (v.w > 0.f) R.rec[(size_t)i * kRecStride + (b - 1) * 5 + 4] = st3(rad, 1.f);
             }
             (void)had_nee;
             // ---- 2. shade the extension hit (GiTracer bounce loop body) ----
@@ -512,11 +542,20 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                                 const uint32_t c6 = 6u * (uint32_t)b;
                                 float gx, gy;
                                 ks_sample2d(base, KS_GUIDE, c6, gx, gy);
-                                bool guided = (jflags & JOB_GUIDED) != 0;
+                                bool guided = (jflags & JOB_GUIDED) != 0 && (J.guide_bounces == 0 || b < J.guide_bounces);
+                                Lobes G;
+                                const int grow = guided ? R.grow[i] : -1;
+                                guided = grow >= 0;
+                                if (guided)
+                                {
+                                    G = guide_lobes(R.gout + (size_t)grow * kGUsed);
+                                    // a vertex whose network output is not finite samples the BSDF
+                                    // alone; the choice depends on the vertex only, so the estimator
+                                    // stays unbiased
+                                    guided = lobes_finite(G);
+                                }
                                 float alpha = guided ? J.bsdf_frac : 1.f;
                                 float usel = ks_float(ks_u32(base, KS_GUIDE, c6 + 2));
-                                Lobes G;
-                                if (guided) G = guide_lobes(R.gout + (size_t)i * kGOut);
                                 if (!guided || usel < alpha)
                                 {
                                     (void)mat_sample(m, h, gx, gy, wo, wi, pdf, type);
@@ -527,7 +566,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                                     float u0 = ks_float(ks_u32(base, KS_GUIDE, c6 + 3));
                                     float u1 = ks_float(ks_u32(base, KS_GUIDE, c6 + 4));
                                     float u2 = ks_float(ks_u32(base, KS_GUIDE, c6 + 5));
-                                    wi = guide_sample(G, u0, u1, u2);
+                                    wi = guide_sample(G, R.gout + (size_t)grow * kGUsed, u0, u1, u2);
                                 }
                                 f = mat_eval(m, h, wo, wi);
                                 float pb = mat_pdf(m, h, wo, wi);
@@ -547,12 +586,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                             {
                                 f3 nd = normalize(wi);
                                 T = T * (f * (fabsf(dot(h.n, wi)) / pdf));
-                                if ((jflags & JOB_RECORD) && b <= 1)
+                                if ((jflags & JOB_RECORD) && b < kRecVerts)
                                 {
                                     // training record: network input, sampled direction, mixture pdf,
                                     // throughput after this vertex
                                     f3 nn = dot(h.n, wo) < 0.f ? -h.n : h.n;
-                                    float4* rc = R.rec + (size_t)i * 10 + b * 5;
+                                    float4* rc = R.rec + (size_t)i * kRecStride + b * 5;
                                     rc[0] = make_float4((h.p.x - J.glo[0]) * J.ginv[0], (h.p.y - J.glo[1]) * J.ginv[1],
                                                         (h.p.z - J.glo[2]) * J.ginv[2], m.type == 1 ? m.alpha : 1.f);
                                     rc[1] = make_float4(nn.x, nn.y, nn.z, 1.f);
@@ -597,9 +636,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                 f3 o = rad;
                 if (jflags & JOB_RECORD)
                 {
-                    for (int k = 0; k < 2; ++k)
+#pragma unroll
+                    for (int k = 0; k < kRecVerts; ++k)
                     {
-                        float4* rc = R.rec + (size_t)i * 10 + k * 5;
+                        const float4* rc = R.rec + (size_t)i * kRecStride + k * 5;
                         float4 d = rc[2], snap = rc[4];
                         if (!(d.w > 0.f) || !(snap.w > 0.f)) continue;
                         float4 T = rc[3];
@@ -608,21 +648,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                         float lum = 0.2126f * L.x + 0.7152f * L.y + 0.0722f * L.z;
                         if (lum > 0.f)
                         {
-                            uint32_t slot = atomicAdd(R.rec_count, 1u);
-                            if (slot < R.rec_cap)
-                            {
-                                float4* q = reinterpret_cast<float4*>(R.rec_out) + (size_t)slot * 3;
-                                q[0] = rc[0];
-                                q[1] = rc[1];
-                                // target density of the guide: incident radiance times the
-                                // cosine (the whole integrand of a diffuse vertex; the network
-                                // sees no view direction, so it cannot learn a glossy lobe)
-                                float4 nn = rc[1];
-                                float cs = fabsf(nn.x * d.x + nn.y * d.y + nn.z * d.z);
-                                q[2] = make_float4(d.x, d.y, d.z, lum * cs / d.w);
-                            }
+                            // target density of the guide: incident radiance times the cosine
+                            // (the whole integrand of a diffuse vertex; the network sees no view
+                            // direction, so it cannot learn a glossy lobe)
+                            float4 nn = rc[1];
+                            float cs = fabsf(nn.x * d.x + nn.y * d.y + nn.z * d.z);
+                            rec_w[k] = lum * cs / d.w;
+                            rec_mask |= 1u << k;
                         }
-                        rc[2].w = 0.f;
                     }
                 }
                 if (!(fl & F_NOCLAMP) && !nq)
@@ -689,6 +722,32 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                 R.qa_own[buf][ap] = (int)((Q_NEE_ANY << 29) | (uint32_t)ks);
                 ++ap;
             }
+        }
+    }
+    // ---- training records of finished paths: one atomic per wave ----
+    if (kFull)
+    {
+        const uint32_t nrec = (uint32_t)__popc(rec_mask);
+        const uint32_t rincl = wave_incl_scan(nrec);
+        const uint32_t rtotal = __shfl(rincl, 63);
+        uint32_t rbase = 0;
+        if (lane == 63 && rtotal) rbase = atomicAdd(R.rec_count, rtotal);
+        rbase = __shfl(rbase, 63);
+        uint32_t slot = rbase + rincl - nrec;
+#pragma unroll
+        for (int k = 0; k < kRecVerts; ++k)
+        {
+            if (!(rec_mask & (1u << k))) continue;
+            if (slot < R.rec_cap)
+            {
+                const float4* rc = R.rec + (size_t)i * kRecStride + k * 5;
+                float4* q = reinterpret_cast<float4*>(R.rec_out) + (size_t)slot * 3;
+                float4 d = rc[2];
+                q[0] = rc[0];
+                q[1] = rc[1];
+                q[2] = make_float4(d.x, d.y, d.z, rec_w[k]);
+            }
+            ++slot;
         }
     }
 }
@@ -922,10 +981,10 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
             else
                 hipLaunchKernelGGL(k_trace_queue<false>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1);
             if (T) BSH_HIP(hipEventRecord(T->next(), st));
-            if (gr && gr->guide && b < ch.depth)
+            if (gr && gr->guide && b < ch.depth && (gr->bounces == 0 || b < gr->bounces))
             {
-                hipLaunchKernelGGL(k_guide_inputs, pg, dim3(kBlock), 0, st, R);
-                bsh_status gs = guide_forward(gr->guide, R.gin, 2, ch.P, c->work.gout.p, st);
+                hipLaunchKernelGGL(k_guide_inputs, pg, dim3(kBlock), 0, st, R, b);
+                bsh_status gs = guide_forward_rows(gr->guide, R.gin, w.gseg.p, pg.x * kBlock, w.gout.p, st);
                 if (gs != BSH_OK) return gs;
             }
             if (full)
@@ -1080,12 +1139,15 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     const bool record = gr && gr->records && gr->cap > 0;
     if (guided)
     {
-        BSH_HIP(w.gin.alloc(2 * P));
-        BSH_HIP(w.gout.alloc((size_t)kGOut * P));
+        const size_t Pb = (P + kBlock - 1) / kBlock * kBlock;   // rows live in whole segments
+        BSH_HIP(w.gin.alloc(2 * Pb));
+        BSH_HIP(w.gout.alloc((size_t)kGUsed * Pb));
+        BSH_HIP(w.grow.alloc(P));
+        BSH_HIP(w.gseg.alloc(Pb / kBlock));
     }
     if (record)
     {
-        BSH_HIP(w.rec.alloc(10 * P));
+        BSH_HIP(w.rec.alloc((size_t)kRecStride * P));
         BSH_HIP(w.rec_count.alloc(1));
     }
     const size_t trace_threads = (size_t)c->num_cus * std::max(1, tune.trace_blocks_per_cu) * kBlock;
@@ -1110,6 +1172,8 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     R.P = 0;
     R.gin = guided ? w.gin.p : nullptr;
     R.gout = guided ? w.gout.p : nullptr;
+    R.grow = guided ? w.grow.p : nullptr;
+    R.gseg = guided ? w.gseg.p : nullptr;
     R.rec = record ? w.rec.p : nullptr;
     R.rec_out = record ? gr->records : nullptr;
     R.rec_cap = record ? (uint32_t)std::min<size_t>(gr->cap, 0xffffffffu) : 0u;

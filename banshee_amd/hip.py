@@ -81,7 +81,7 @@ class Job(ctypes.Structure):
 class GuideParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint32), ("lr", ctypes.c_float), ("beta1", ctypes.c_float),
                 ("beta2", ctypes.c_float), ("eps", ctypes.c_float), ("bsdf_fraction", ctypes.c_float),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("guided_bounces", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class RenderStats(ctypes.Structure):
@@ -103,7 +103,7 @@ EXPORTS = [
     "bsh_batch_run", "bsh_context_set_tuning", "bsh_context_get_tuning", "bsh_context_stats_get", "bsh_autotune",
     "bsh_guide_num_params", "bsh_guide_create", "bsh_guide_destroy", "bsh_guide_get_params", "bsh_guide_set_params",
     "bsh_guide_forward", "bsh_guide_gradient", "bsh_guide_train_step", "bsh_render_guided", "bsh_diag_mfma",
-    "bsh_diag_guide", "bsh_bvh_build_host", "bsh_guide_set_bsdf_fraction",
+    "bsh_diag_guide", "bsh_bvh_build_host", "bsh_guide_set_bsdf_fraction", "bsh_guide_train",
 ]
 
 _lib = None
@@ -384,11 +384,13 @@ GUIDE_OUT = 48
 class Guide:
     """The path-guiding network (vMF mixture MLP) owned by a Context."""
 
-    def __init__(self, ctx: Context, seed=1, lr=5e-3, beta1=0.9, beta2=0.99, eps=1e-8, bsdf_fraction=0.5):
+    def __init__(self, ctx: Context, seed=1, lr=5e-3, beta1=0.9, beta2=0.99, eps=1e-8, bsdf_fraction=0.5,
+                 guided_bounces=0):
         self.ctx = ctx
         self.h = c_void_p()
         p = GuideParams()
         p.seed, p.lr, p.beta1, p.beta2, p.eps, p.bsdf_fraction = seed, lr, beta1, beta2, eps, bsdf_fraction
+        p.guided_bounces = guided_bounces
         _check(lib().bsh_guide_create(ctx.h, ctypes.byref(p), ctypes.byref(self.h)), "bsh_guide_create")
 
     def set_bsdf_fraction(self, a: float):
@@ -439,6 +441,13 @@ class Guide:
         loss = ctypes.c_float()
         _check(lib().bsh_guide_train_step(self.h, ptr, c_size_t(n), ctypes.byref(loss) if want_loss else None,
                                           c_void_p(None)), "bsh_guide_train_step")
+        return loss.value if want_loss else None
+
+    def train(self, records: DeviceArray, n: int, steps: int, batch: int, want_loss=False):
+        """`steps` Adam steps on consecutive batches of the first n device records."""
+        loss = ctypes.c_float()
+        _check(lib().bsh_guide_train(self.h, records.ptr, c_size_t(n), c_int(steps), c_size_t(batch),
+                                     ctypes.byref(loss) if want_loss else None, c_void_p(None)), "bsh_guide_train")
         return loss.value if want_loss else None
 
     def close(self):

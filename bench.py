@@ -76,8 +76,11 @@ def cpu_baseline(sc_full_args) -> dict | None:
 
 # Guided vs plain (quirk-free) path tracing: the occluded room, 500k triangles, 1024x1024, depth 8.
 G_RES, G_DEPTH, G_TARGET = 1024, 8, 0.02
-G_PASS_GRID, G_CHECK = 2, 8               # 4 spp per pass; RMSE estimated every 8 passes (32 spp)
-G_ALPHA, G_LR, G_STEPS, G_BATCH = 0.3, 5e-3, 2, 1 << 18
+G_PASS_GRID, G_CHECK = 2, 4               # 4 spp per pass; RMSE estimated every 4 passes (16 spp)
+G_ALPHA, G_WARM, G_LR, G_BATCH = 0.5, 8, 5e-3, 1 << 18
+G_STEPS, G_STEPS_LATE, G_LATE = 4, 1, 24   # Adam steps per pass: G_STEPS, from pass G_LATE on G_STEPS_LATE
+# the BSDF fraction falls linearly from 1 to G_ALPHA over the first G_WARM passes, while the
+# untrained guide would only add variance
 
 
 def rmse_estimate(a) -> float:
@@ -99,7 +102,7 @@ def time_to_rmse(ctx, scene, guide, max_passes, seed0):
     import numpy as np
     from banshee_amd import hip
     acc = ctx.alloc((G_RES, G_RES, 8), np.float32)
-    cap = G_RES * G_RES * G_PASS_GRID ** 2 * 2 if guide is not None else 0
+    cap = G_RES * G_RES * G_PASS_GRID ** 2 * 4 if guide is not None else 0   # up to 4 records per path
     recs = ctx.alloc((cap, hip.RECORD_FLOATS), np.float32) if cap else None
     ctx.sync()
     clock = train = 0.0
@@ -107,14 +110,13 @@ def time_to_rmse(ctx, scene, guide, max_passes, seed0):
     k = 0
     while k < max_passes:
         t = time.perf_counter()
+        if guide is not None:
+            guide.set_bsdf_fraction(max(G_ALPHA, 1.0 - (1.0 - G_ALPHA) * k / G_WARM))
         n = scene.render_guided(G_RES, G_RES, G_PASS_GRID, G_DEPTH, seed0 + k, guide=guide, records=recs,
                                 accum=acc, accumulate=k > 0, download=False)
         if guide is not None and n:
             t2 = time.perf_counter()
-            n = min(n, cap)
-            for s in range(G_STEPS):
-                off = (s * G_BATCH) % max(1, n - G_BATCH + 1) if n > G_BATCH else 0
-                guide.train_step(recs, min(G_BATCH, n), offset=off, want_loss=False)
+            guide.train(recs, min(n, cap), G_STEPS if k < G_LATE else G_STEPS_LATE, G_BATCH)
             ctx.sync()
             train += time.perf_counter() - t2
         ctx.sync()
@@ -135,7 +137,7 @@ def guided_comparison(ctx, bvh_width) -> dict:
     scene.render_guided(G_RES, G_RES, 1, G_DEPTH, 1, download=False)   # warm the wavefront buffers
     ctx.sync()
     tp, sp, ep, _ = time_to_rmse(ctx, scene, None, 4096, 100000)
-    guide = hip.Guide(ctx, seed=1, bsdf_fraction=G_ALPHA, lr=G_LR, beta2=0.99)
+    guide = hip.Guide(ctx, seed=1, bsdf_fraction=1.0, lr=G_LR, beta2=0.99)
     tg, sg, eg, tt = time_to_rmse(ctx, scene, guide, 4096, 200000)
     return {"workload": f"occluded room ({sc.ntris} tris, the light behind a wall with 4 openings totalling 5% of its "
                         f"area, the camera's side lit only through them), "
@@ -143,7 +145,8 @@ def guided_comparison(ctx, bvh_width) -> dict:
             "target_rmse": G_TARGET, "plain_s": round(tp, 3), "plain_spp": sp, "plain_rmse": round(ep, 5),
             "guided_s": round(tg, 3), "guided_spp": sg, "guided_rmse": round(eg, 5),
             "guided_train_s": round(tt, 3), "time_ratio": round(tg / tp, 3),
-            "guide": {"bsdf_fraction": G_ALPHA, "lr": G_LR, "adam_steps_per_pass": G_STEPS, "batch": G_BATCH,
+            "guide": {"bsdf_fraction": G_ALPHA, "warmup_passes": G_WARM, "lr": G_LR,
+                      "adam_steps_per_pass": [G_STEPS, G_STEPS_LATE, G_LATE], "batch": G_BATCH,
                       "spp_per_pass": G_PASS_GRID ** 2}}
 
 
@@ -154,7 +157,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain time-to-RMSE comparison")
-    ap.add_argument("--bvh-width", type=int, default=2, choices=(2, 4), help="BVH node width")
+    ap.add_argument("--bvh-width", type=int, default=4, choices=(2, 4), help="BVH node width")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "

@@ -102,7 +102,7 @@ typedef struct
     int32_t sah_bins;   /* bins per axis for the binned SAH, 4..64 */
     int32_t threads;    /* host build threads, 0 = hardware concurrency */
     int32_t width;      /* node width: 2 (64-B binary nodes) or 4 (128-B 4-wide nodes collapsed from the
-                           binary SAH tree); 0 = 2. A tree too deep for the 4-wide stack stays binary. */
+                           binary SAH tree); 0 = 4. A tree too deep for the 4-wide stack stays binary. */
 } bsh_bvh_params;
 
 /* Frame parameters for bsh_render. The estimator is GiTracer(max_depth)
@@ -296,7 +296,9 @@ typedef struct
     uint32_t seed;        /* He-uniform initialisation seed */
     float lr, beta1, beta2, eps;   /* Adam */
     float bsdf_fraction;  /* one-sample MIS: probability of sampling the BSDF instead of the guide, in (0, 1] */
-    int32_t reserved[2];
+    int32_t guided_bounces; /* path vertices 0 .. guided_bounces-1 use the guide, later ones the BSDF alone
+                               (and skip the network); 0 = every vertex */
+    int32_t reserved;
 } bsh_guide_params;
 
 size_t bsh_guide_num_params(void);
@@ -317,10 +319,15 @@ bsh_status bsh_guide_forward(bsh_guide* g, const float* inputs, size_t n, float*
 bsh_status bsh_guide_gradient(bsh_guide* g, const float* records, size_t n, float* grad, float* loss);
 /* One Adam step on the records. loss may be NULL. */
 bsh_status bsh_guide_train_step(bsh_guide* g, const float* records, size_t n, float* loss, void* stream);
+/* `steps` Adam steps on consecutive batches of min(batch, n) records (wrapping), e.g. the
+   records one guided pass wrote. loss (may be NULL) receives the last step's loss. */
+bsh_status bsh_guide_train(bsh_guide* g, const float* records, size_t n, int steps, size_t batch, float* loss,
+                           void* stream);
 /* Renders like bsh_render (flags must include BSH_RENDER_NO_QUIRKS) with the continuation
    direction drawn from the one-sample MIS mixture of BSDF and guide (guide may be NULL: BSDF
-   only, the plain quirk-free tracer), and appends training records of the first two path
-   vertices to `records` (device float[cap*12]); *nrecords receives the count written. */
+   only, the plain quirk-free tracer), and appends training records of the first four path
+   vertices to `records` (device float[cap*12]; records beyond cap are dropped); *nrecords
+   receives the count written. */
 bsh_status bsh_render_guided(bsh_scene* s, const bsh_render_params* p, bsh_guide* guide, float* accum,
                              float* records, size_t cap, size_t* nrecords, void* stream);
 

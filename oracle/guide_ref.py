@@ -15,13 +15,15 @@ Network (parameters in one flat FP32 vector, weights input-major W[k][n], then b
   features(7 raw) = [raw, sin/cos(2^f pi raw_d), f = 0..5] -> 91, zero-padded to 96
   -> 4 x (64, ReLU) -> 48 outputs: logits[0:8], mu raw[8:32], log kappa[32:40] in [-2, 6].
 Loss over records (position, roughness, normal, direction, weight w):
-  L = sum_r (w_r / sum w) * (-log q(dir_r)),  q = sum_k pi_k vMF(dir; mu_k, kappa_k).
+  L = sum_r (c_r / n) * (-log q(dir_r)),  q = sum_k pi_k vMF(dir; mu_k, kappa_k),
+  c_r = min(n w_r / sum w, 256)  (weights normalised to mean 1, then capped).
 """
 from __future__ import annotations
 
 import numpy as np
 
 N_IN, N_HID, N_OUT, LOBES, RAW, BANDS = 96, 64, 48, 8, 7, 6
+MAX_RECORD_WEIGHT = 256.0
 LK_LO, LK_HI = -2.0, 6.0
 SIZES = [(N_IN, N_HID), (N_HID, N_HID), (N_HID, N_HID), (N_HID, N_HID), (N_HID, N_OUT)]
 
@@ -155,11 +157,16 @@ def split_records(rec):
     return raw, rec[:, 8:11], rec[:, 11]
 
 
+def record_coeffs(wt):
+    n = len(wt)
+    return np.minimum(wt * n / wt.sum(), MAX_RECORD_WEIGHT) / n
+
+
 def loss(p, rec, fp16=False):
     raw, d, wt = split_records(rec)
     o, _, _ = forward(p, raw, fp16)
     q = mixture_pdf(o, d)
-    c = wt / wt.sum()
+    c = record_coeffs(wt)
     ok = (q > 0) & (c > 0)
     return float(np.sum(c[ok] * -np.log(q[ok])))
 
@@ -189,7 +196,7 @@ def loss_and_grad(p, rec, fp16=False):
     """(loss, gradient in the flat parameter layout), by backpropagation in FP64."""
     raw, d, wt = split_records(rec)
     o, acts, pre = forward(p, raw, fp16)
-    c = wt / wt.sum()
+    c = record_coeffs(wt)
     q = mixture_pdf(o, d)
     ok = (q > 0) & (c > 0)
     L = float(np.sum(c[ok] * -np.log(q[ok])))

@@ -171,3 +171,17 @@ def test_training_on_device_recovers_a_lobe(ctx):
     x = raw_inputs(rng, 64)
     q = G.mixture_pdf(g.forward(x).astype(np.float64), np.repeat(mu[None], 64, 0))
     assert np.all(q > 0.5 * kappa / (2 * np.pi))
+
+
+def test_multi_step_train_equals_single_steps(ctx):
+    """bsh_guide_train(steps, batch) is the same Adam steps as train_step on the same slices."""
+    from banshee_amd import hip
+    rec = records(np.random.default_rng(12), 3000)
+    d = ctx.to_device(rec)
+    g1 = hip.Guide(ctx, seed=6)
+    g2 = hip.Guide(ctx, seed=6)
+    g1.train(d, 3000, 3, 1024)
+    for s in range(3):
+        off = (s * 1024) % (3000 - 1024 + 1)
+        g2.train_step(d, 1024, offset=off)
+    assert np.array_equal(g1.get_params(), g2.get_params())

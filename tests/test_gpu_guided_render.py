@@ -110,3 +110,37 @@ def test_guided_trained_is_unbiased(ctx, cornell):
     z, _ = block_z(plain, guided)
     assert np.max(np.abs(z)) < 5.0, np.max(np.abs(z))
     assert abs(np.mean(z)) < 0.5
+
+
+def test_non_finite_guide_falls_back_to_the_bsdf(ctx, cornell):
+    """A guide whose outputs are NaN must not bias the image: every vertex then samples the
+    BSDF alone, which is the plain quirk-free tracer sample for sample."""
+    from banshee_amd import hip
+    S = hip.Scene(ctx, cornell)
+    g = hip.Guide(ctx, seed=1, bsdf_fraction=0.3)
+    g.set_params(np.full(hip.Guide.num_params(), np.nan, np.float32))
+    plain = S.render_guided(64, 64, 4, 5, 41, crop=None)
+    guided = S.render_guided(64, 64, 4, 5, 41, guide=g)
+    assert np.array_equal(plain[0], guided[0]) and np.array_equal(plain[1], guided[1])
+
+
+def test_training_survives_extreme_weights(ctx):
+    """One record with an enormous weight (a rare sample with a tiny pdf) neither overflows
+    the FP16 gradients into the parameters nor stops learning."""
+    from banshee_amd import hip
+    rng = np.random.default_rng(8)
+    g = hip.Guide(ctx, seed=3, lr=5e-3, beta2=0.99)
+    rec = np.zeros((4096, 12), np.float32)
+    rec[:, 0:3] = rng.random((4096, 3))
+    rec[:, 3] = 0.5
+    nn = rng.normal(size=(4096, 3))
+    rec[:, 4:7] = nn / np.linalg.norm(nn, axis=1, keepdims=True)
+    rec[:, 7] = 1
+    d = rng.normal(size=(4096, 3))
+    rec[:, 8:11] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rec[:, 11] = 1.0
+    rec[7, 11] = 1e30
+    for _ in range(5):
+        loss = g.train_step(rec)
+    assert np.isfinite(loss)
+    assert np.all(np.isfinite(g.get_params()))
