@@ -309,6 +309,22 @@ __global__ __launch_bounds__(kBlock) void k_trace_queue(RenderDev R, int L, int 
     }
 }
 
+// Adds the guided rows of this pass (the per-block segment counts) to stats[4], in a fixed order.
+__global__ __launch_bounds__(kBlock) void k_sum_rows(const uint32_t* seg, uint32_t nseg, unsigned long long* total)
+{
+    __shared__ unsigned long long red[kBlock];
+    unsigned long long s = 0;
+    for (uint32_t k = threadIdx.x; k < nseg; k += kBlock) s += seg[k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1)
+    {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) total[0] += red[0];
+}
+
 // Inclusive prefix sum over the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {
@@ -876,7 +892,7 @@ static bsh_status ensure_work(bsh_context* c, size_t P, int K, int nlmax)
         BSH_HIP(w.qa_d[k].alloc(qa));
         BSH_HIP(w.qa_own[k].alloc(qa));
     }
-    BSH_HIP(w.stats.alloc(4));
+    BSH_HIP(w.stats.alloc(5));
     return BSH_OK;
 }
 
@@ -887,7 +903,7 @@ struct LaunchTimer
     size_t ei = 0;
     bsh_status init(size_t pairs)
     {
-        evs.resize(3 * pairs + 2);
+        evs.resize(5 * pairs + 2);
         for (auto& e : evs) BSH_HIP(hipEventCreate(&e));
         return BSH_OK;
     }
@@ -981,12 +997,19 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
             else
                 hipLaunchKernelGGL(k_trace_queue<false>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1);
             if (T) BSH_HIP(hipEventRecord(T->next(), st));
-            if (gr && gr->guide && b < ch.depth && (gr->bounces == 0 || b < gr->bounces))
+            const bool infer = gr && gr->guide && b < ch.depth && (gr->bounces == 0 || b < gr->bounces);
+            if (infer)
             {
                 hipLaunchKernelGGL(k_guide_inputs, pg, dim3(kBlock), 0, st, R, b);
+                hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(kBlock), 0, st, w.gseg.p, pg.x, w.stats.p + 4);
+            }
+            if (T) BSH_HIP(hipEventRecord(T->next(), st));
+            if (infer)
+            {
                 bsh_status gs = guide_forward_rows(gr->guide, R.gin, w.gseg.p, pg.x * kBlock, w.gout.p, st);
                 if (gs != BSH_OK) return gs;
             }
+            if (T) BSH_HIP(hipEventRecord(T->next(), st));
             if (full)
                 hipLaunchKernelGGL(k_shade<true>, pg, dim3(kBlock), 0, st, R, b, b + 1 <= ch.depth ? b + 1 : b);
             else
@@ -1180,7 +1203,7 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     R.rec_count = record ? w.rec_count.p : nullptr;
     if (record) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, w.rec_count.p, 1);
 
-    hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, st, (uint32_t*)w.stats.p, 8);
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, st, (uint32_t*)w.stats.p, 10);
     bsh_render_stats& S = c->stats;
     S = bsh_render_stats{};
     if (use_graph)
@@ -1246,22 +1269,30 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         BSH_TRY(enqueue_batch(c, R, chunks, hjobs, count, tune, st, &T, gr));
         BSH_HIP(hipEventRecord(T.next(), st));
         BSH_HIP(hipEventSynchronize(T.evs[T.ei - 1]));
-        for (size_t k = 1; k + 2 < T.ei; k += 3)
+        // per pass: before trace, after trace, after the guide inputs, after inference, after shade
+        for (size_t k = 1; k + 4 < T.ei; k += 5)
         {
-            float a = 0.f, b = 0.f;
+            float a = 0.f, g = 0.f, b = 0.f;
             BSH_HIP(hipEventElapsedTime(&a, T.evs[k], T.evs[k + 1]));
-            BSH_HIP(hipEventElapsedTime(&b, T.evs[k + 1], T.evs[k + 2]));
+            BSH_HIP(hipEventElapsedTime(&g, T.evs[k + 2], T.evs[k + 3]));
+            BSH_HIP(hipEventElapsedTime(&b, T.evs[k + 3], T.evs[k + 4]));
             S.trace_ms += a;
             S.shade_ms += b;
             S.trace_launches++;
             S.shade_launches++;
+            if (guided && g > 0.f)
+            {
+                S.guide_ms += g;
+                S.guide_launches++;
+            }
         }
         float tot = 0.f;
         BSH_HIP(hipEventElapsedTime(&tot, T.evs[0], T.evs[T.ei - 1]));
         S.total_ms = tot;
     }
-    unsigned long long st2[4];
+    unsigned long long st2[5];
     BSH_HIP(hipMemcpy(st2, w.stats.p, sizeof(st2), hipMemcpyDeviceToHost));
+    S.guide_rows = st2[4];
     S.rays_closest = st2[0];
     S.rays_any = st2[1];
     S.node_visits = count ? st2[2] : 0;

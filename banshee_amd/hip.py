@@ -89,7 +89,8 @@ class RenderStats(ctypes.Structure):
                 ("camera_samples", ctypes.c_uint64), ("node_visits", ctypes.c_uint64),
                 ("tri_tests", ctypes.c_uint64), ("trace_ms", ctypes.c_double), ("shade_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("trace_launches", ctypes.c_uint32),
-                ("shade_launches", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+                ("shade_launches", ctypes.c_uint32), ("guide_launches", ctypes.c_uint32),
+                ("guide_rows", ctypes.c_uint64), ("guide_ms", ctypes.c_double)]
 
 
 # Every symbol include/banshee_hip.h declares.
@@ -279,7 +280,7 @@ class Context:
     def stats(self) -> dict:
         st = RenderStats()
         _check(lib().bsh_context_stats_get(self.h, ctypes.byref(st)), "bsh_context_stats_get")
-        return {k: getattr(st, k) for k, _ in st._fields_ if k != "pad"}
+        return {k: getattr(st, k) for k, _ in st._fields_}
 
     def batch_run(self, jobs, tuning: dict | None = None, stream=None):
         """jobs: list of BatchJob. Runs them in one wavefront (one hipGraph when use_graph)."""
@@ -645,7 +646,7 @@ class Scene:
     def stats(self) -> dict:
         s = RenderStats()
         _check(lib().bsh_render_stats_get(self.h, ctypes.byref(s)), "bsh_render_stats_get")
-        return {k: getattr(s, k) for k, _ in s._fields_ if k != "pad"}
+        return {k: getattr(s, k) for k, _ in s._fields_}
 
     def diag_light(self, light, p, u, w):
         p, u, w = (np.ascontiguousarray(x, np.float32) for x in (p, u, w))

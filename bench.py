@@ -29,6 +29,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GI Mrays/s; time-to-target-RMSE guided vs plain PT, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+MFMA_FP16_PEAK_TFLOPS = 2500.0
+MLP_FLOP_PER_ROW = 2 * (96 * 64 + 3 * 64 * 64 + 64 * 48)   # one forward row of the guide network
 W, H, GRID, DEPTH, SEED = 1920, 1080, 8, 8, 1
 
 
@@ -78,7 +80,8 @@ def cpu_baseline(sc_full_args) -> dict | None:
 G_RES, G_DEPTH, G_TARGET = 1024, 8, 0.02
 G_PASS_GRID, G_CHECK = 2, 4               # 4 spp per pass; RMSE estimated every 4 passes (16 spp)
 G_ALPHA, G_WARM, G_LR, G_BATCH = 0.5, 8, 5e-3, 1 << 18
-G_STEPS, G_STEPS_LATE, G_LATE = 4, 1, 24   # Adam steps per pass: G_STEPS, from pass G_LATE on G_STEPS_LATE
+G_STEPS, G_STEPS_LATE, G_LATE = 8, 1, 12   # Adam steps per pass: G_STEPS, from pass G_LATE on G_STEPS_LATE
+G_BOUNCES = 0                              # guided path vertices (0 = all)
 # the BSDF fraction falls linearly from 1 to G_ALPHA over the first G_WARM passes, while the
 # untrained guide would only add variance
 
@@ -97,8 +100,8 @@ def time_to_rmse(ctx, scene, guide, max_passes, seed0):
     """Renders passes of G_PASS_GRID^2 spp into one accumulation buffer until the RMSE
     estimate reaches G_TARGET. With a guide, every pass also writes training records and
     runs G_STEPS Adam steps on G_BATCH of them; every pass's samples count in the image.
-    Returns (seconds of render + training, spp, rmse, training seconds). The RMSE checks
-    are outside the clock."""
+    Returns (seconds of render + training, spp, rmse, training seconds, guided network rows,
+    milliseconds of inference launches). The RMSE checks are outside the clock."""
     import numpy as np
     from banshee_amd import hip
     acc = ctx.alloc((G_RES, G_RES, 8), np.float32)
@@ -108,12 +111,17 @@ def time_to_rmse(ctx, scene, guide, max_passes, seed0):
     clock = train = 0.0
     e = float("inf")
     k = 0
+    rows = 0
+    infer_ms = 0.0
     while k < max_passes:
         t = time.perf_counter()
         if guide is not None:
             guide.set_bsdf_fraction(max(G_ALPHA, 1.0 - (1.0 - G_ALPHA) * k / G_WARM))
         n = scene.render_guided(G_RES, G_RES, G_PASS_GRID, G_DEPTH, seed0 + k, guide=guide, records=recs,
                                 accum=acc, accumulate=k > 0, download=False)
+        st = scene.stats()
+        rows += st["guide_rows"]
+        infer_ms += st["guide_ms"]
         if guide is not None and n:
             t2 = time.perf_counter()
             guide.train(recs, min(n, cap), G_STEPS if k < G_LATE else G_STEPS_LATE, G_BATCH)
@@ -126,7 +134,7 @@ def time_to_rmse(ctx, scene, guide, max_passes, seed0):
             e = rmse_estimate(acc.numpy())
             if e <= G_TARGET:
                 break
-    return clock, k * G_PASS_GRID ** 2, e, train
+    return clock, k * G_PASS_GRID ** 2, e, train, rows, infer_ms
 
 
 def guided_comparison(ctx, bvh_width) -> dict:
@@ -136,18 +144,34 @@ def guided_comparison(ctx, bvh_width) -> dict:
     scene = hip.Scene(ctx, sc, width=bvh_width)
     scene.render_guided(G_RES, G_RES, 1, G_DEPTH, 1, download=False)   # warm the wavefront buffers
     ctx.sync()
-    tp, sp, ep, _ = time_to_rmse(ctx, scene, None, 4096, 100000)
-    guide = hip.Guide(ctx, seed=1, bsdf_fraction=1.0, lr=G_LR, beta2=0.99)
-    tg, sg, eg, tt = time_to_rmse(ctx, scene, guide, 4096, 200000)
+    tp, sp, ep, _, _, _ = time_to_rmse(ctx, scene, None, 4096, 100000)
+    guide = hip.Guide(ctx, seed=1, bsdf_fraction=1.0, lr=G_LR, beta2=0.99, guided_bounces=G_BOUNCES)
+    tg, sg, eg, tt, rows, infer_ms = time_to_rmse(ctx, scene, guide, 4096, 200000)
+    tflops = rows * MLP_FLOP_PER_ROW / (infer_ms / 1e3) / 1e12 if infer_ms > 0 else 0.0
     return {"workload": f"occluded room ({sc.ntris} tris, the light behind a wall with 4 openings totalling 5% of its "
                         f"area, the camera's side lit only through them), "
                         f"{G_RES}x{G_RES}, depth {G_DEPTH}, quirk-free estimator",
             "target_rmse": G_TARGET, "plain_s": round(tp, 3), "plain_spp": sp, "plain_rmse": round(ep, 5),
             "guided_s": round(tg, 3), "guided_spp": sg, "guided_rmse": round(eg, 5),
             "guided_train_s": round(tt, 3), "time_ratio": round(tg / tp, 3),
-            "guide": {"bsdf_fraction": G_ALPHA, "warmup_passes": G_WARM, "lr": G_LR,
+            "mlp_roofline": {"bound": "mfma", "kernel": "k_guide_forward (inference, FP16 MFMA)",
+                             "achieved": round(tflops, 1), "peak": MFMA_FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": round(tflops / MFMA_FP16_PEAK_TFLOPS, 4), "rows": int(rows),
+                             "flop_per_row": MLP_FLOP_PER_ROW, "inference_ms": round(infer_ms, 2)},
+            "guide": {"bsdf_fraction": G_ALPHA, "warmup_passes": G_WARM, "lr": G_LR, "guided_bounces": G_BOUNCES,
                       "adam_steps_per_pass": [G_STEPS, G_STEPS_LATE, G_LATE], "batch": G_BATCH,
                       "spp_per_pass": G_PASS_GRID ** 2}}
+
+
+def measured_traffic(width: int):
+    """HBM bytes per trace launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    of this workload (calibrated against a copy kernel), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_trace_queue_terrain3M_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(rf"traffic_bytes_per_launch\[width={width}\]: (\d+)", open(path).read())
+    return int(m.group(1)) if m else None
 
 
 def main():
@@ -158,6 +182,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain time-to-RMSE comparison")
     ap.add_argument("--bvh-width", type=int, default=4, choices=(2, 4), help="BVH node width")
+    ap.add_argument("--guided-only", action="store_true",
+                    help="run only the guided-vs-plain comparison and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -167,6 +193,10 @@ def main():
 
     from banshee_amd import hip
     from banshee_amd.scenes import terrain_scene
+
+    if args.guided_only:
+        print(json.dumps(guided_comparison(hip.Context(0), args.bvh_width)), flush=True)
+        return
 
     t0 = time.time()
     sc = terrain_scene(width=W, height=H)
@@ -228,7 +258,8 @@ def main():
                    "samples_per_s": round(samples / dt, 1), "rays_per_step": rays // args.steps,
                    "setup_s": round(setup_s, 2), "bvh_build_ms": round(info["build_ms"], 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(int(info["width"])),
+                     "traffic_source": "profiles/k_trace_queue_terrain3M_pmc.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
                      "kernel": "k_trace_queue", "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                      "bytes_per_ray": round(b_ray, 1), "node_visits_per_ray": round(v_node, 2),
                      "tri_tests_per_ray": round(v_tri, 2), "trace_share": round(trace_ms / (dt * 1e3), 3)},

@@ -142,7 +142,9 @@ typedef struct
     uint64_t node_visits, tri_tests;   /* totals over all queries, BSH_RENDER_COUNT only */
     double trace_ms, shade_ms, total_ms; /* summed HIP-event durations */
     uint32_t trace_launches, shade_launches;
-    uint32_t pad;
+    uint32_t guide_launches;           /* guide network inference launches (bsh_render_guided) */
+    uint64_t guide_rows;               /* path vertices run through the guide network */
+    double guide_ms;                   /* summed duration of the inference launches */
 } bsh_render_stats;
 
 /* ---------------- context ---------------- */
