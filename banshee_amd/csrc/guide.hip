@@ -71,145 +71,208 @@ __device__ __forceinline__ void load_fwd_weights(const GuideNet& g, _Float16* lw
     for (int t = threadIdx.x; t < kBiasTotal; t += blockDim.x) lb[t] = g.bias[t];
 }
 
-// Feature j of the encoded input of raw values v0..v6 = (a.xyzw, c.xyz): j < 7 the raw value,
-// then for t = j - 7 < 84: band b = t / 14, dimension d = (t % 14) / 2, sin (t even) or cos
-// (t odd) of 2^b pi v_d; zero for j >= 91. Evaluated with the hardware sine/cosine, which
-// take revolutions: sin(2^b pi v) = sin(2 pi fract(2^(b-1) v)), the power-of-two scaling and
-// fract being exact.
-__device__ __forceinline__ float encode_feature(float4 a, float4 c, int j)
+// Encoded input of one row for the lane holding k-positions 4q..4q+3 of every 16-feature
+// block (the B-fragment layout): block b holds band b, position p < 14 the sin (p even) or cos
+// (p odd) of 2^b pi v_{p/2}, positions 14 and 15 the raw values v_{2b}, v_{2b+1} (zero past
+// v6). So lane group q < 3 owns dimensions 2q and 2q+1 in every band and q = 3 dimension 6
+// plus the raw slots. Band 0 uses the hardware sine/cosine on the exact revolution argument
+// fract(v/2); each further band doubles the angle: sin 2x = 2 sin x cos x,
+// cos 2x = (cos x - sin x)(cos x + sin x).
+template <int B>
+__device__ __forceinline__ float raw_slot(float4 a, float4 c)
 {
-    if (j >= kGRaw + 2 * kGRaw * kGBands) return 0.f;
-    int t = j - kGRaw;
-    int band = t / (2 * kGRaw);
-    int r = t - band * (2 * kGRaw);
-    int d = j < kGRaw ? j : r >> 1;
-    // v[d] by selects on register values (an array indexed by d would live in scratch memory)
-    float lo = d == 0 ? a.x : d == 1 ? a.y : d == 2 ? a.z : a.w;
-    float hi = d == 4 ? c.x : d == 5 ? c.y : c.z;
-    float x = d < 4 ? lo : hi;
-    if (j < kGRaw) return x;
-    float rev = __builtin_amdgcn_fractf(ldexpf(x, band - 1));
-    return (r & 1) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev);
+    return B == 0 ? a.x : B == 1 ? a.y : B == 2 ? a.z : B == 3 ? a.w : B == 4 ? c.x : B == 5 ? c.y : B == 6 ? c.z : 0.f;
 }
 
-// One dense layer on the transposed product Y^T = W^T X^T for a 16-row tile: the weights
-// are the A operand (LDS, out-major rows), the activations the B operand. The MFMA result
-// fragment (lane l holds Y^T[16 mt + 4(l/16) + i][row l%16]) is exactly the B fragment of
-// the next layer's k-step mt, so activations never leave the registers.
-template <int K, int N>
-__device__ __forceinline__ void dense_t(const h4 (&x)[K / 16], const _Float16* Wt, int ws, const float* b, h4 (&y)[N / 16],
-                                        int r, int q)
+template <int BAND>
+__device__ __forceinline__ void encode_band(float4 a, float4 c, int q, float& sA, float& cA, float& sB, float& cB,
+                                            h4 (&x)[kGIn / 16])
+{
+    const bool tail = q == 3;
+    x[BAND][0] = (_Float16)sA;
+    x[BAND][1] = (_Float16)cA;
+    x[BAND][2] = (_Float16)(tail ? raw_slot<2 * BAND>(a, c) : sB);
+    x[BAND][3] = (_Float16)(tail ? raw_slot<2 * BAND + 1>(a, c) : cB);
+    float s2 = 2.f * sA * cA, c2 = (cA - sA) * (cA + sA);
+    sA = s2;
+    cA = c2;
+    s2 = 2.f * sB * cB;
+    c2 = (cB - sB) * (cB + sB);
+    sB = s2;
+    cB = c2;
+}
+
+__device__ __forceinline__ void encode_row(float4 a, float4 c, int q, bool live, h4 (&x)[kGIn / 16])
+{
+    // the lane's two dimensions (q = 3: dimension 6 and an unused second one)
+    float vA = q == 0 ? a.x : q == 1 ? a.z : q == 2 ? c.x : c.z;
+    float vB = q == 0 ? a.y : q == 1 ? a.w : q == 2 ? c.y : 0.f;
+    float rA = __builtin_amdgcn_fractf(0.5f * vA), rB = __builtin_amdgcn_fractf(0.5f * vB);
+    float sA = __builtin_amdgcn_sinf(rA), cA = __builtin_amdgcn_cosf(rA);
+    float sB = __builtin_amdgcn_sinf(rB), cB = __builtin_amdgcn_cosf(rB);
+    encode_band<0>(a, c, q, sA, cA, sB, cB, x);
+    encode_band<1>(a, c, q, sA, cA, sB, cB, x);
+    encode_band<2>(a, c, q, sA, cA, sB, cB, x);
+    encode_band<3>(a, c, q, sA, cA, sB, cB, x);
+    encode_band<4>(a, c, q, sA, cA, sB, cB, x);
+    encode_band<5>(a, c, q, sA, cA, sB, cB, x);
+    static_assert(kGIn / 16 == kGBands, "one 16-feature block per band");
+    if (!live)
+#pragma unroll
+        for (int k = 0; k < kGIn / 16; ++k) x[k] = h4{0, 0, 0, 0};
+}
+
+// One dense layer for kT 16-row tiles on the transposed product Y^T = W^T X^T: the weights are
+// the A operand (LDS, out-major rows), the activations the B operand. Each weight fragment is
+// read from LDS once and used for all kT tiles. The MFMA result fragment (lane l holds
+// Y^T[16 mt + 4(l/16) + i][row l%16]) is exactly the B fragment of the next layer's k-step mt,
+// so activations never leave the registers.
+template <int K, int N, int kT>
+__device__ __forceinline__ void dense_t(const h4 (&x)[kT][K / 16], const _Float16* Wt, int ws, const float* b,
+                                        h4 (&y)[kT][N / 16], int r, int q)
 {
 #pragma unroll
     for (int mt = 0; mt < N / 16; ++mt)
     {
-        f4v acc = {0.f, 0.f, 0.f, 0.f};
+        f4v acc[kT];
+#pragma unroll
+        for (int t = 0; t < kT; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int ks = 0; ks < K / 16; ++ks)
         {
             h4 w = *reinterpret_cast<const h4*>(&Wt[(mt * 16 + r) * ws + ks * 16 + 4 * q]);
-            acc = mfma16(w, x[ks], acc);
+#pragma unroll
+            for (int t = 0; t < kT; ++t) acc[t] = mfma16(w, x[t][ks], acc[t]);
         }
         float4 bb = *reinterpret_cast<const float4*>(&b[mt * 16 + 4 * q]);
-        h4 o;
-        o[0] = (_Float16)fmaxf(acc[0] + bb.x, 0.f);
-        o[1] = (_Float16)fmaxf(acc[1] + bb.y, 0.f);
-        o[2] = (_Float16)fmaxf(acc[2] + bb.z, 0.f);
-        o[3] = (_Float16)fmaxf(acc[3] + bb.w, 0.f);
-        y[mt] = o;
+#pragma unroll
+        for (int t = 0; t < kT; ++t)
+        {
+            h4 o;
+            o[0] = (_Float16)fmaxf(acc[t][0] + bb.x, 0.f);
+            o[1] = (_Float16)fmaxf(acc[t][1] + bb.y, 0.f);
+            o[2] = (_Float16)fmaxf(acc[t][2] + bb.z, 0.f);
+            o[3] = (_Float16)fmaxf(acc[t][3] + bb.w, 0.f);
+            y[t][mt] = o;
+        }
     }
 }
 
 template <int N>
-__device__ __forceinline__ void store_rows(_Float16* dst, uint32_t row, uint32_t nrows, const h4 (&y)[N / 16], int q)
+__device__ __forceinline__ void store_rows(_Float16* dst, uint32_t row, bool live, const h4 (&y)[N / 16], int q)
 {
-    if (row >= nrows) return;
+    if (!live) return;
 #pragma unroll
     for (int mt = 0; mt < N / 16; ++mt) *reinterpret_cast<h4*>(&dst[(size_t)row * N + mt * 16 + 4 * q]) = y[mt];
 }
 
 constexpr int kFwdWaves = 8;
+constexpr int kFwdTiles = 2;   // 16-row tiles per wave per step (weight reads shared; 4 costs occupancy)
 
 // Fused forward: rows of raw inputs (`stride` float4 per row, the first two holding
-// v0..v6) -> 48 FP32 outputs per row. Weights and biases are staged in LDS once per block;
-// each wave runs 16-row tiles with all activations in registers. With kTrain the encoded
-// input and the four hidden activations are also written (FP16, row-major) for backward.
-// kHalf: write only the kGUsed outputs the mixture uses, in FP16 (the wavefront's form);
-// seg (when set): rows come in segments of kGuideSeg of which the first seg[s] are live.
+// v0..v6) -> the network's outputs. Weights and biases are staged in LDS once per block; each
+// wave runs kFwdTiles 16-row tiles at a time with all activations in registers. With kTrain
+// the encoded input and the four hidden activations are also written (FP16, row-major) for
+// backward. kHalf: write only the kGUsed outputs the mixture uses, in FP16 (the wavefront's
+// form), else all kGOut in FP32. seg (when set): rows come in segments of kGuideSeg of which
+// the first seg[s] are live; tile groups past a segment's rows are skipped.
 template <bool kTrain, bool kHalf>
 __global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet g, const float4* in, int stride,
                                                                  uint32_t nrows, const uint32_t* seg, void* outp,
                                                                  _Float16* x0, _Float16* hs)
 {
+    constexpr int kGroup = 16 * kFwdTiles;
+    static_assert(kGuideSeg % kGroup == 0, "tile groups must not straddle segments");
     __shared__ _Float16 lw[kLFTotal];
     __shared__ __align__(16) float lb[kBiasTotal];
     load_fwd_weights(g, lw, lb);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = lane & 15, q = lane >> 4;
-    const uint32_t ntiles = (nrows + 15) / 16;
-    for (uint32_t tile = blockIdx.x * kFwdWaves + wave; tile < ntiles; tile += gridDim.x * kFwdWaves)
+    const uint32_t ngroups = (nrows + kGroup - 1) / kGroup;
+    for (uint32_t grp = blockIdx.x * kFwdWaves + wave; grp < ngroups; grp += gridDim.x * kFwdWaves)
     {
         // an offset the compiler cannot see through keeps the weight reads inside the loop:
         // hoisted, all five layers' fragments would take ~180 registers per lane
         int zoff = 0;
         asm volatile("" : "+v"(zoff));
         const _Float16* W = lw + zoff;
-        const uint32_t row = tile * 16 + r;
+        const uint32_t row0 = grp * kGroup;
         uint32_t seg_live = kGuideSeg;
         if (seg)
         {
-            seg_live = seg[(tile * 16) / kGuideSeg];          // the same for the whole tile
-            if ((tile * 16) % kGuideSeg >= seg_live) continue;  // tile past the segment's rows
+            seg_live = seg[row0 / kGuideSeg];                 // the same for the whole group
+            if (row0 % kGuideSeg >= seg_live) continue;       // group past the segment's rows
         }
-        const bool live = row < nrows && row % kGuideSeg < seg_live;
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), c = a;
-        if (live)
+        uint32_t row[kFwdTiles];
+        bool live[kFwdTiles];
+        h4 x[kFwdTiles][kGIn / 16];
+#pragma unroll
+        for (int t = 0; t < kFwdTiles; ++t)
         {
-            a = in[(size_t)stride * row];
-            c = in[(size_t)stride * row + 1];
+            row[t] = row0 + 16 * t + r;
+            live[t] = row[t] < nrows && row[t] % kGuideSeg < seg_live;
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), c = a;
+            if (live[t])
+            {
+                a = in[(size_t)stride * row[t]];
+                c = in[(size_t)stride * row[t] + 1];
+            }
+            encode_row(a, c, q, live[t], x[t]);
+            if (kTrain) store_rows<kGIn>(x0, row[t], live[t], x[t], q);
         }
-        h4 x[kGIn / 16];
+        h4 h1[kFwdTiles][kGHid / 16], h2[kFwdTiles][kGHid / 16];
+        dense_t<kGIn, kGHid, kFwdTiles>(x, W + kLF1, kL1S, lb, h1, r, q);
+        if (kTrain)
 #pragma unroll
-        for (int ks = 0; ks < kGIn / 16; ++ks)
+            for (int t = 0; t < kFwdTiles; ++t) store_rows<kGHid>(hs, row[t], live[t], h1[t], q);
+        dense_t<kGHid, kGHid, kFwdTiles>(h1, W + kLF2, kLHS, lb + kGHid, h2, r, q);
+        if (kTrain)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) x[ks][i] = live ? (_Float16)encode_feature(a, c, ks * 16 + 4 * q + i) : (_Float16)0.f;
-        if (kTrain) store_rows<kGIn>(x0, row, nrows, x, q);
-        h4 h1[kGHid / 16], h2[kGHid / 16];
-        dense_t<kGIn, kGHid>(x, W + kLF1, kL1S, lb, h1, r, q);
-        if (kTrain) store_rows<kGHid>(hs, row, nrows, h1, q);
-        dense_t<kGHid, kGHid>(h1, W + kLF2, kLHS, lb + kGHid, h2, r, q);
-        if (kTrain) store_rows<kGHid>(hs + (size_t)nrows * kGHid, row, nrows, h2, q);
-        dense_t<kGHid, kGHid>(h2, W + kLF3, kLHS, lb + 2 * kGHid, h1, r, q);
-        if (kTrain) store_rows<kGHid>(hs + (size_t)2 * nrows * kGHid, row, nrows, h1, q);
-        dense_t<kGHid, kGHid>(h1, W + kLF4, kLHS, lb + 3 * kGHid, h2, r, q);
-        if (kTrain) store_rows<kGHid>(hs + (size_t)3 * nrows * kGHid, row, nrows, h2, q);
+            for (int t = 0; t < kFwdTiles; ++t) store_rows<kGHid>(hs + (size_t)nrows * kGHid, row[t], live[t], h2[t], q);
+        dense_t<kGHid, kGHid, kFwdTiles>(h2, W + kLF3, kLHS, lb + 2 * kGHid, h1, r, q);
+        if (kTrain)
+#pragma unroll
+            for (int t = 0; t < kFwdTiles; ++t)
+                store_rows<kGHid>(hs + (size_t)2 * nrows * kGHid, row[t], live[t], h1[t], q);
+        dense_t<kGHid, kGHid, kFwdTiles>(h1, W + kLF4, kLHS, lb + 3 * kGHid, h2, r, q);
+        if (kTrain)
+#pragma unroll
+            for (int t = 0; t < kFwdTiles; ++t)
+                store_rows<kGHid>(hs + (size_t)3 * nrows * kGHid, row[t], live[t], h2[t], q);
         // output layer, FP32 straight from the accumulators: lane holds outputs 16 mt + 4q + i of its row
 #pragma unroll
         for (int mt = 0; mt < kGOut / 16; ++mt)
         {
-            f4v acc = {0.f, 0.f, 0.f, 0.f};
+            f4v acc[kFwdTiles];
+#pragma unroll
+            for (int t = 0; t < kFwdTiles; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int ks = 0; ks < kGHid / 16; ++ks)
             {
                 h4 w = *reinterpret_cast<const h4*>(&W[kLF5 + (mt * 16 + r) * kLHS + ks * 16 + 4 * q]);
-                acc = mfma16(w, h2[ks], acc);
+#pragma unroll
+                for (int t = 0; t < kFwdTiles; ++t) acc[t] = mfma16(w, h2[t][ks], acc[t]);
             }
             float4 bb = *reinterpret_cast<const float4*>(&lb[4 * kGHid + mt * 16 + 4 * q]);
-            if (!live) continue;
-            if (kHalf)
+#pragma unroll
+            for (int t = 0; t < kFwdTiles; ++t)
             {
-                if (mt * 16 + 4 * q < kGUsed)
+                if (!live[t]) continue;
+                if (kHalf)
                 {
-                    h4 o = {(_Float16)(acc[0] + bb.x), (_Float16)(acc[1] + bb.y), (_Float16)(acc[2] + bb.z),
-                            (_Float16)(acc[3] + bb.w)};
-                    *reinterpret_cast<h4*>(&static_cast<_Float16*>(outp)[(size_t)row * kGUsed + mt * 16 + 4 * q]) = o;
+                    if (mt * 16 + 4 * q < kGUsed)
+                    {
+                        h4 o = {(_Float16)(acc[t][0] + bb.x), (_Float16)(acc[t][1] + bb.y), (_Float16)(acc[t][2] + bb.z),
+                                (_Float16)(acc[t][3] + bb.w)};
+                        *reinterpret_cast<h4*>(&static_cast<_Float16*>(outp)[(size_t)row[t] * kGUsed + mt * 16 + 4 * q]) = o;
+                    }
                 }
-            }
-            else
-            {
-                *reinterpret_cast<float4*>(&static_cast<float*>(outp)[(size_t)row * kGOut + mt * 16 + 4 * q]) =
-                    make_float4(acc[0] + bb.x, acc[1] + bb.y, acc[2] + bb.z, acc[3] + bb.w);
+                else
+                {
+                    *reinterpret_cast<float4*>(&static_cast<float*>(outp)[(size_t)row[t] * kGOut + mt * 16 + 4 * q]) =
+                        make_float4(acc[t][0] + bb.x, acc[t][1] + bb.y, acc[t][2] + bb.z, acc[t][3] + bb.w);
+                }
             }
         }
     }
@@ -620,8 +683,8 @@ int guide_bounces(const bsh_guide* g) { return g->prm.guided_bounces; }
 bsh_status guide_forward(bsh_guide* g, const float4* in, int stride, uint32_t n, float* out, hipStream_t st)
 {
     if (n == 0) return BSH_OK;
-    uint32_t tiles = (n + 15) / 16;
-    uint32_t blocks = std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
+    uint32_t groups = (n + 16 * kFwdTiles - 1) / (16 * kFwdTiles);
+    uint32_t blocks = std::min<uint32_t>((groups + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
     hipLaunchKernelGGL((k_guide_forward<false, false>), dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), in,
                        stride, n, (const uint32_t*)nullptr, (void*)out, (_Float16*)nullptr, (_Float16*)nullptr);
     BSH_HIP(hipGetLastError());
@@ -632,8 +695,8 @@ bsh_status guide_forward_rows(bsh_guide* g, const float4* in, const uint32_t* se
                               _Float16* out, hipStream_t st)
 {
     if (rows == 0) return BSH_OK;
-    uint32_t tiles = (rows + 15) / 16;
-    uint32_t blocks = std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
+    uint32_t groups = (rows + 16 * kFwdTiles - 1) / (16 * kFwdTiles);
+    uint32_t blocks = std::min<uint32_t>((groups + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
     hipLaunchKernelGGL((k_guide_forward<false, true>), dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), in, 2,
                        rows, seg_counts, (void*)out, (_Float16*)nullptr, (_Float16*)nullptr);
     BSH_HIP(hipGetLastError());
@@ -672,7 +735,8 @@ static bsh_status guide_gradients(bsh_guide* g, const float4* rec, uint32_t n, h
     hipLaunchKernelGGL(k_guide_wsum, dim3(kWsumParts), dim3(kWsumBlock), 0, st, rec, n, g->wsum.p + 1);
     hipLaunchKernelGGL(k_guide_wsum_final, dim3(1), dim3(kWsumBlock), 0, st, g->wsum.p + 1, g->wsum.p);
     uint32_t tiles = (n + 15) / 16;
-    uint32_t blocks = std::min<uint32_t>((tiles + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
+    uint32_t groups = (n + 16 * kFwdTiles - 1) / (16 * kFwdTiles);
+    uint32_t blocks = std::min<uint32_t>((groups + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
     hipLaunchKernelGGL((k_guide_forward<true, false>), dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), rec, 3, n,
                        (const uint32_t*)nullptr, (void*)g->out.p, g->x0.p, g->hs.p);
     hipLaunchKernelGGL(k_guide_loss, dim3((n + 255) / 256), dim3(256), 0, st, g->out.p, rec, n, g->wsum.p, g->d5.p,
@@ -726,9 +790,10 @@ bsh_status bsh_guide_create(bsh_context* ctx, const bsh_guide_params* prm, bsh_g
         float lim = std::sqrt(6.f / (l == 0 ? 91.f : (float)fin[l]));
         for (int k = 0; k < fin[l] * fout[l]; ++k) g->host_params[wo[l] + k] = (2.f * U() - 1.f) * lim;
     }
-    // the padded input rows 91..95 are always zero; keep their weights zero
-    for (int k = 91; k < kGIn; ++k)
-        for (int n = 0; n < kGHid; ++n) g->host_params[kGW1 + k * kGHid + n] = 0.f;
+    // the padding features are always zero; keep their weights zero
+    for (int k = 0; k < kGIn; ++k)
+        if (guide_pad_feature(k))
+            for (int n = 0; n < kGHid; ++n) g->host_params[kGW1 + k * kGHid + n] = 0.f;
     bsh_status r = BSH_OK;
     auto alloc = [&]() -> bsh_status {
         BSH_HIP(g->p.alloc(kGParams));

@@ -2,8 +2,10 @@
 // parameters come from a small MLP of (position, normal, roughness). No reference
 // counterpart; the plain BSDF-sampling tracer stays the fallback and error judge.
 //
-// Network: 7 raw inputs (position normalised to the scene box, shading normal, GGX alpha
-// or 1 for Lambert) -> features [raw 7, sin/cos(2^f pi v) for f = 0..5] = 91, padded to 96
+// Network: 7 raw inputs v0..v6 (position normalised to the scene box, GGX alpha or 1 for
+// Lambert, shading normal) -> 96 encoded features in 6 blocks of 16, block b = band b:
+// position p < 14 is sin (p even) or cos (p odd) of 2^b pi v_{p/2}, positions 14 and 15 the
+// raw values v_{2b}, v_{2b+1} (zero past v6: 91 features and 5 zeros)
 // -> 4 hidden layers of 64 ReLU units -> 48 outputs of which 40 are used:
 //   o[0..7]   lobe logits (softmax -> weights pi_k)
 //   o[8..31]  lobe mean directions, unnormalised (mu_k = o / |o|)
@@ -33,6 +35,13 @@ constexpr int kGW3 = kGB2 + kGHid, kGB3 = kGW3 + kGHid * kGHid;
 constexpr int kGW4 = kGB3 + kGHid, kGB4 = kGW4 + kGHid * kGHid;
 constexpr int kGW5 = kGB4 + kGHid, kGB5 = kGW5 + kGHid * kGOut;
 constexpr int kGParams = kGB5 + kGOut;
+
+// The five always-zero encoded features (raw slots past v6).
+BSH_HD inline bool guide_pad_feature(int j)
+{
+    int b = j / 16, p = j % 16;
+    return p >= 14 && 2 * b + (p - 14) >= kGRaw;
+}
 
 struct Lobes
 {

@@ -12,7 +12,9 @@ network's definition, and the tests pin it from outside itself:
 * Training with it on records drawn from a known vMF lobe recovers that lobe.
 
 Network (parameters in one flat FP32 vector, weights input-major W[k][n], then bias):
-  features(7 raw) = [raw, sin/cos(2^f pi raw_d), f = 0..5] -> 91, zero-padded to 96
+  features(7 raw): 6 blocks of 16, block b = band b; position p < 14 is sin (p even) or cos
+  (p odd) of 2^b pi raw_{p/2}, positions 14, 15 the raw values raw_{2b}, raw_{2b+1} (zero
+  past raw_6): 91 features and 5 zeros
   -> 4 x (64, ReLU) -> 48 outputs: logits[0:8], mu raw[8:32], log kappa[32:40] in [-2, 6].
 Loss over records (position, roughness, normal, direction, weight w):
   L = sum_r (c_r / n) * (-log q(dir_r)),  q = sum_k pi_k vMF(dir; mu_k, kappa_k),
@@ -53,19 +55,22 @@ def pack(Ws, bs):
     return np.concatenate([np.concatenate([W.ravel(), b]) for W, b in zip(Ws, bs)])
 
 
+PAD_FEATURES = [16 * b + p for b in range(BANDS) for p in (14, 15) if 2 * b + p - 14 >= RAW]
+
+
 def features(raw):
-    """raw: n x 7 -> n x 96."""
+    """raw: n x 7 -> n x 96 in the block-per-band order."""
     raw = np.asarray(raw, np.float64)
     n = raw.shape[0]
     F = np.zeros((n, N_IN))
-    F[:, :RAW] = raw
-    j = RAW
     for band in range(BANDS):
         for d in range(RAW):
             a = (2.0 ** band) * np.pi * raw[:, d]
-            F[:, j] = np.sin(a)
-            F[:, j + 1] = np.cos(a)
-            j += 2
+            F[:, 16 * band + 2 * d] = np.sin(a)
+            F[:, 16 * band + 2 * d + 1] = np.cos(a)
+        for e in (2 * band, 2 * band + 1):
+            if e < RAW:
+                F[:, 16 * band + 14 + (e - 2 * band)] = raw[:, e]
     return F
 
 
@@ -221,7 +226,7 @@ def he_init(seed=0):
         lim = np.sqrt(6.0 / (91 if li == 0 else k))
         W = rng.uniform(-lim, lim, (k, n))
         if li == 0:
-            W[91:] = 0
+            W[PAD_FEATURES] = 0
         Ws.append(W)
         bs.append(rng.uniform(-0.1, 0.1, n))
     return pack(Ws, bs).astype(np.float32)

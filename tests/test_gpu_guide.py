@@ -118,7 +118,8 @@ def test_gradient_matches_fp64(ctx, n):
     for (k, m), (wo, bo) in zip(G.SIZES, offs):
         assert rel(grad[wo:wo + k * m], ref[wo:wo + k * m]) <= 1e-2, ("W", wo, rel(grad[wo:wo + k * m], ref[wo:wo + k * m]))
         assert rel(grad[bo:bo + m], ref[bo:bo + m]) <= 1e-2, ("b", bo, rel(grad[bo:bo + m], ref[bo:bo + m]))
-    assert np.all(grad[91 * 64:96 * 64] == 0)   # padded input features carry no gradient
+    for j in G.PAD_FEATURES:                      # padding features carry no gradient
+        assert np.all(grad[j * 64:(j + 1) * 64] == 0)
 
 
 def test_adam_step_and_fp16_copies(ctx):

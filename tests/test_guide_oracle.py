@@ -92,8 +92,7 @@ def test_backward_matches_finite_differences():
     assert np.isclose(L, G.loss(p, rec))
     offs, _ = G.offsets()
     for (k, n), (wo, bo) in zip(G.SIZES, offs):
-        rows = 91 if wo == 0 else k
-        idx = list(rng.integers(wo, wo + rows * n, 6)) + list(rng.integers(bo, bo + n, 3))
+        idx = list(rng.integers(wo, wo + k * n, 6)) + list(rng.integers(bo, bo + n, 3))
         for i in idx:
             h = 1e-6 * max(1.0, abs(p[i]))
             pp, pm = p.copy(), p.copy()
@@ -180,3 +179,15 @@ def test_emitter_quadrature_matches_lambert_formula():
         q = emitter_irradiance(p, [0, 1, 0], tri, [0, -1, 0], cos_power=1)
         e = lambert_polygon_irradiance(p, [0, 1, 0], tri)
         assert abs(q - e) < 2e-5 * e, (p, q, e)
+
+
+def test_feature_order():
+    """Block b of the encoding is band b: (sin, cos) of 2^b pi v_d at 2d, 2d+1, then the raw
+    slots v_2b, v_2b+1; exactly five features are zero padding."""
+    v = np.array([[0.1, 0.2, 0.3, 0.4, -0.5, 0.6, 0.7]])
+    F = G.features(v)[0]
+    assert np.isclose(F[16 * 3 + 2 * 4], np.sin(8 * np.pi * -0.5))
+    assert np.isclose(F[16 * 5 + 2 * 6 + 1], np.cos(32 * np.pi * 0.7))
+    assert F[16 * 0 + 14] == 0.1 and F[16 * 2 + 15] == 0.6 and F[16 * 3 + 14] == 0.7
+    assert G.PAD_FEATURES == [63, 78, 79, 94, 95]
+    assert np.all(F[G.PAD_FEATURES] == 0)
