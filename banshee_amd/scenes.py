@@ -311,7 +311,7 @@ def terrain_scene(grid=512, spheres=2000, subdiv=3, seed=1, width=1920, height=1
         B.area_lights(lb)
     cam = [0, 30, 75, 0, 0, 0, 0, 1, 0, 0.01, 10000.0, math.pi / 4, width / height]
     return B.build(cam, f"terrain{grid}_{spheres}", {"seed": seed, "grid": grid, "spheres": spheres,
-                                                    "subdiv": subdiv})
+                                                    "subdiv": subdiv, "extent": extent})
 
 
 def terrain_for_tris(target_tris: int, seed: int, subdiv=3, width=1920, height=1080) -> Scene:
@@ -343,9 +343,9 @@ def batch_jobs(n=64, seed=100, min_tris=1_000, max_tris=2_000_000, min_res=128, 
 
 
 def probe_grid(scene: Scene, n=16, margin=0.05, lift=0.5):
-    """n^3 probe positions on a regular grid inside the scene's bounding box, each moved up
-    until it is `lift` above the heightfield's top vertex of its column so that probes sit
-    in free space."""
+    """n^3 probe positions on a regular grid inside the scene's bounding box. For a terrain
+    scene, a probe below the heightfield is moved up to `lift` above the highest corner of the
+    heightfield cell under it, so that every probe sits above the ground."""
     lo = scene.verts.min(0)
     hi = scene.verts.max(0)
     ext = hi - lo
@@ -354,6 +354,15 @@ def probe_grid(scene: Scene, n=16, margin=0.05, lift=0.5):
     ys = lo[1] + ext[1] * (margin + (1 - 2 * margin) * t)
     zs = lo[2] + ext[2] * (margin + (1 - 2 * margin) * t)
     P = np.stack(np.meshgrid(xs, ys, zs, indexing="ij"), -1).reshape(-1, 3)
+    if "grid" in scene.meta and "extent" in scene.meta:
+        g, e = scene.meta["grid"], scene.meta["extent"]
+        b = scene.bundles[0]                      # the heightfield, (g + 1)^2 vertices in x-major order
+        H = scene.verts[b[2]:b[2] + b[3], 1].reshape(g + 1, g + 1)
+        step = 2 * e / g
+        i = np.clip(((P[:, 0] + e) / step).astype(np.int64), 0, g - 1)
+        j = np.clip(((P[:, 2] + e) / step).astype(np.int64), 0, g - 1)
+        top = np.maximum.reduce([H[i, j], H[i + 1, j], H[i, j + 1], H[i + 1, j + 1]])
+        P[:, 1] = np.maximum(P[:, 1], top + lift)
     return P.astype(np.float32)
 
 

@@ -174,6 +174,91 @@ def measured_traffic(width: int):
     return int(m.group(1)) if m else None
 
 
+def batch_workload(args) -> dict:
+    """The batched workload: 64 seeded terrain scenes (1k-2M triangles, log-uniform),
+    128^2-1024^2, 4-64 spp, depth 4-12, rendered in one persistent wavefront captured as one
+    hipGraph with autotuned chunk size and trace grid; compared with rendering each job alone."""
+    from banshee_amd import hip
+    from banshee_amd.scenes import batch_jobs, terrain_for_tris
+    ctx = hip.Context(0)
+    t0 = time.time()
+    cfgs = batch_jobs(n=64, seed=100)
+    scenes, jobs = [], []
+    for c in cfgs:
+        sc = terrain_for_tris(c["tris"], seed=c["seed"], width=c["res"], height=c["res"])
+        s = hip.Scene(ctx, sc, width=args.bvh_width)
+        scenes.append(s)
+        jobs.append(hip.BatchJob(s, "render", width=c["res"], height=c["res"], img_grid=c["grid"],
+                                 max_depth=c["depth"], seed=c["seed"]))
+    setup_s = time.time() - t0
+    tris = sum(s.info()["tris"] for s in scenes)
+    t1 = time.time()
+    best, best_ms = ctx.autotune(jobs, paths=[1 << 19, 1 << 20, 1 << 21], blocks=[3, 4, 5])
+    tune_s = time.time() - t1
+    tuning = dict(best, use_graph=1)
+    for _ in range(args.warmup):
+        ctx.batch_run(jobs, tuning=tuning)
+    ctx.sync()
+    rays = samples = 0
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.batch_run(jobs, tuning=tuning)
+        st = ctx.stats()
+        rays += st["rays_closest"] + st["rays_any"]
+        samples += st["camera_samples"]
+    ctx.sync()
+    dt = time.perf_counter() - t2
+    # the same jobs one by one (bsh_render each, same tuning, no graph)
+    ctx.set_tuning(best["paths_per_chunk"], best["trace_blocks_per_cu"], 0)
+    t3 = time.perf_counter()
+    for j in jobs:
+        j.scene.render(j.params.width, j.params.height, j.params.img_grid, j.params.max_depth, j.params.seed,
+                       accum=j.accum, download=False)
+    ctx.sync()
+    solo_s = time.perf_counter() - t3
+    return {"metric": "batched GI Mrays/s (64 scenes, one hipGraph launch)", "value": round(rays / dt / 1e6, 2),
+            "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "64 seeded terrain scenes, tris log-uniform 1k-2M, 128^2-1024^2, 4-64 spp, "
+                                   "depth 4-12, one persistent wavefront as one hipGraph",
+                       "tris_total": int(tris), "samples_per_step": samples // args.steps,
+                       "samples_per_s": round(samples / dt, 1), "tuning": best,
+                       "autotune_s": round(tune_s, 2), "setup_s": round(setup_s, 2),
+                       "one_by_one_s": round(solo_s, 3), "batched_s": round(dt / args.steps, 3),
+                       "batch_speedup": round(solo_s / (dt / args.steps), 3)}}
+
+
+def probe_workload(args) -> dict:
+    """SH probe baking: 4096 probes on a 16^3 grid in a 2M-triangle terrain scene, 1024 rays
+    per probe, depth 6, order-3 SH (9 coefficients x RGB), all probes in one wavefront."""
+    from banshee_amd import hip
+    from banshee_amd.scenes import probe_grid, terrain_for_tris
+    ctx = hip.Context(0)
+    sc = terrain_for_tris(2_000_000, seed=7)
+    scene = hip.Scene(ctx, sc, width=args.bvh_width)
+    probes = probe_grid(sc, 16)
+    coeffs = ctx.alloc((len(probes), 9, 3), "float32")
+    for _ in range(args.warmup):
+        scene.bake_sh(probes, 1024, 2, 6, 1, coeffs=coeffs, download=False)
+    ctx.sync()
+    rays = paths = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        scene.bake_sh(probes, 1024, 2, 6, 1 + k, coeffs=coeffs, download=False)
+        st = scene.stats()
+        rays += st["rays_closest"] + st["rays_any"]
+        paths += st["camera_samples"]
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    return {"metric": "SH probe baking Mrays/s", "value": round(rays / dt / 1e6, 2), "unit": "Mrays/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "4096 probes (16^3 grid) in a 2M-triangle terrain scene, 1024 rays/probe, "
+                                   "depth 6, order-3 SH", "tris": int(sc.ntris),
+                       "probe_paths_per_s": round(paths / dt, 1), "probes_per_s": round(len(probes) * args.steps / dt, 1)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +269,8 @@ def main():
     ap.add_argument("--bvh-width", type=int, default=4, choices=(2, 4), help="BVH node width")
     ap.add_argument("--guided-only", action="store_true",
                     help="run only the guided-vs-plain comparison and print its JSON (for profiling)")
+    ap.add_argument("--workload", default="gi", choices=("gi", "batch", "probes"),
+                    help="gi: the headline line (default); batch: 64 scenes in one launch; probes: SH baking")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -196,6 +283,12 @@ def main():
 
     if args.guided_only:
         print(json.dumps(guided_comparison(hip.Context(0), args.bvh_width)), flush=True)
+        return
+    if args.workload == "batch":
+        print(json.dumps(batch_workload(args)), flush=True)
+        return
+    if args.workload == "probes":
+        print(json.dumps(probe_workload(args)), flush=True)
         return
 
     t0 = time.time()
