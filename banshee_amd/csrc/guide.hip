@@ -284,13 +284,14 @@ constexpr float kMaxRecordWeight = 256.f;
 // normalised to mean 1 over the batch (s = w * rows / sum w) so FP16 gradients stay O(1);
 // the weight-gradient reduction divides by rows. rec: 3 float4 per row: (x, r), (n, active),
 // (dir, w).
-__global__ void k_guide_loss(const float* out, const float4* rec, uint32_t nrows, const float* wsum, _Float16* dout,
+__global__ void k_guide_loss(const float* out, const float4* rec, int srow, uint32_t nrows, const float* wsum,
+                             _Float16* dout,
                              float* loss_acc)
 {
     uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= nrows) return;
     const float* o = out + (size_t)row * kGOut;
-    float4 c = rec[3 * row + 2];
+    float4 c = rec[(size_t)srow * row + 2];
     f3 w = mk(c.x, c.y, c.z);
     float wt = c.w;
     float scale = (float)nrows / fmaxf(wsum[0], 1e-30f);
@@ -353,13 +354,13 @@ __device__ __forceinline__ float block_sum(float s, float* red)
     return red[0];
 }
 
-__global__ __launch_bounds__(kWsumBlock) void k_guide_wsum(const float4* rec, uint32_t nrows, float* part)
+__global__ __launch_bounds__(kWsumBlock) void k_guide_wsum(const float4* rec, int srow, uint32_t nrows, float* part)
 {
     __shared__ float red[kWsumBlock];
     const uint32_t per = (nrows + kWsumParts - 1) / kWsumParts;
     const uint32_t r0 = blockIdx.x * per, r1 = min(r0 + per, nrows);
     float s = 0.f;
-    for (uint32_t r = r0 + threadIdx.x; r < r1; r += kWsumBlock) s += rec[3 * r + 2].w;
+    for (uint32_t r = r0 + threadIdx.x; r < r1; r += kWsumBlock) s += rec[(size_t)srow * r + 2].w;
     float t = block_sum(s, red);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
@@ -726,20 +727,20 @@ static void wgrad_layer(bsh_guide* g, const _Float16* x, const _Float16* dy, uin
 
 // Forward + backward on `rec` (3 float4 per row); leaves the batch gradient in g->grad
 // and the weighted mean loss (times rows) in g->loss.
-static bsh_status guide_gradients(bsh_guide* g, const float4* rec, uint32_t n, hipStream_t st)
+static bsh_status guide_gradients(bsh_guide* g, const float4* rec, int srow, uint32_t n, hipStream_t st)
 {
     bsh_status r = ensure_train(g, n);
     if (r != BSH_OK) return r;
     hipLaunchKernelGGL(k_guide_zero, dim3(1), dim3(64), 0, st, g->wsum.p, 1);
     hipLaunchKernelGGL(k_guide_zero, dim3(1), dim3(64), 0, st, g->loss.p, 1);
-    hipLaunchKernelGGL(k_guide_wsum, dim3(kWsumParts), dim3(kWsumBlock), 0, st, rec, n, g->wsum.p + 1);
+    hipLaunchKernelGGL(k_guide_wsum, dim3(kWsumParts), dim3(kWsumBlock), 0, st, rec, srow, n, g->wsum.p + 1);
     hipLaunchKernelGGL(k_guide_wsum_final, dim3(1), dim3(kWsumBlock), 0, st, g->wsum.p + 1, g->wsum.p);
     uint32_t tiles = (n + 15) / 16;
     uint32_t groups = (n + 16 * kFwdTiles - 1) / (16 * kFwdTiles);
     uint32_t blocks = std::min<uint32_t>((groups + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
-    hipLaunchKernelGGL((k_guide_forward<true, false>), dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), rec, 3, n,
+    hipLaunchKernelGGL((k_guide_forward<true, false>), dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), rec, srow, n,
                        (const uint32_t*)nullptr, (void*)g->out.p, g->x0.p, g->hs.p);
-    hipLaunchKernelGGL(k_guide_loss, dim3((n + 255) / 256), dim3(256), 0, st, g->out.p, rec, n, g->wsum.p, g->d5.p,
+    hipLaunchKernelGGL(k_guide_loss, dim3((n + 255) / 256), dim3(256), 0, st, g->out.p, rec, srow, n, g->wsum.p, g->d5.p,
                        g->loss.p);
     const _Float16* H1 = g->hs.p;
     const _Float16* H2 = g->hs.p + (size_t)n * kGHid;
@@ -865,7 +866,7 @@ bsh_status bsh_guide_gradient(bsh_guide* g, const float* records, size_t n, floa
     if (!g || !records || n == 0) return fail(BSH_ERR_INVALID, "bsh_guide_gradient: bad argument");
     if (n >= ((size_t)1 << 30)) return fail(BSH_ERR_UNSUPPORTED, "bsh_guide_gradient: too many rows");
     BSH_HIP(hipSetDevice(g->ctx->device));
-    bsh_status r = guide_gradients(g, reinterpret_cast<const float4*>(records), (uint32_t)n, nullptr);
+    bsh_status r = guide_gradients(g, reinterpret_cast<const float4*>(records), 3, (uint32_t)n, nullptr);
     if (r != BSH_OK) return r;
     BSH_HIP(hipDeviceSynchronize());
     if (grad) BSH_HIP(hipMemcpy(grad, g->grad.p, kGParams * 4, hipMemcpyDeviceToHost));
@@ -878,14 +879,11 @@ bsh_status bsh_guide_gradient(bsh_guide* g, const float* records, size_t n, floa
     return BSH_OK;
 }
 
-bsh_status bsh_guide_train_step(bsh_guide* g, const float* records, size_t n, float* loss, void* stream)
+// One Adam step on n records read `stride` records apart.
+static bsh_status train_step(bsh_guide* g, const float* records, size_t stride, size_t n, float* loss,
+                             hipStream_t st)
 {
-    if (!g || !records) return fail(BSH_ERR_INVALID, "bsh_guide_train_step: NULL argument");
-    if (n == 0) return BSH_OK;
-    if (n >= ((size_t)1 << 30)) return fail(BSH_ERR_UNSUPPORTED, "bsh_guide_train_step: too many rows");
-    BSH_HIP(hipSetDevice(g->ctx->device));
-    hipStream_t st = (hipStream_t)stream;
-    bsh_status r = guide_gradients(g, reinterpret_cast<const float4*>(records), (uint32_t)n, st);
+    bsh_status r = guide_gradients(g, reinterpret_cast<const float4*>(records), (int)(3 * stride), (uint32_t)n, st);
     if (r != BSH_OK) return r;
     g->step++;
     float c1 = 1.f - std::pow(g->prm.beta1, (float)g->step), c2 = 1.f - std::pow(g->prm.beta2, (float)g->step);
@@ -903,18 +901,33 @@ bsh_status bsh_guide_train_step(bsh_guide* g, const float* records, size_t n, fl
     return BSH_OK;
 }
 
+bsh_status bsh_guide_train_step(bsh_guide* g, const float* records, size_t n, float* loss, void* stream)
+{
+    if (!g || !records) return fail(BSH_ERR_INVALID, "bsh_guide_train_step: NULL argument");
+    if (n == 0) return BSH_OK;
+    if (n >= ((size_t)1 << 30)) return fail(BSH_ERR_UNSUPPORTED, "bsh_guide_train_step: too many rows");
+    BSH_HIP(hipSetDevice(g->ctx->device));
+    return train_step(g, records, 1, n, loss, (hipStream_t)stream);
+}
+
 bsh_status bsh_guide_train(bsh_guide* g, const float* records, size_t n, int steps, size_t batch, float* loss,
                            void* stream)
 {
     if (!g || !records) return fail(BSH_ERR_INVALID, "bsh_guide_train: NULL argument");
     if (steps < 0 || batch == 0) return fail(BSH_ERR_INVALID, "bsh_guide_train: bad steps or batch");
     if (n == 0 || steps == 0) return BSH_OK;
-    const size_t rows = std::min(batch, n);
+    if (n >= ((size_t)1 << 34)) return fail(BSH_ERR_UNSUPPORTED, "bsh_guide_train: too many records");
+    BSH_HIP(hipSetDevice(g->ctx->device));
+    // step s takes every stride-th record from offset s mod stride: each batch spans the whole
+    // record array (records are in path order, i.e. image order), and successive steps see
+    // different records
+    const size_t stride = std::max<size_t>(1, n / batch);
     for (int s = 0; s < steps; ++s)
     {
-        // consecutive batches over the records, wrapping so every batch is full
-        size_t off = n > batch ? ((size_t)s * batch) % (n - batch + 1) : 0;
-        bsh_status r = bsh_guide_train_step(g, records + off * 12, rows, s + 1 == steps ? loss : nullptr, stream);
+        const size_t off = (size_t)s % stride;
+        const size_t rows = std::min(batch, (n - off + stride - 1) / stride);
+        bsh_status r = train_step(g, records + off * 12, stride, rows, s + 1 == steps ? loss : nullptr,
+                                  (hipStream_t)stream);
         if (r != BSH_OK) return r;
     }
     return BSH_OK;

@@ -76,7 +76,8 @@ struct RenderWork
     DevBuf<_Float16> gout;            // guide network outputs (kGUsed per row)
     DevBuf<int> grow;                 // per path: its guide row this pass
     DevBuf<uint32_t> gseg;            // guide rows per block segment this pass
-    DevBuf<uint32_t> rec_count;
+    DevBuf<uint32_t> rec_count;       // records written so far (running over the chunks)
+    DevBuf<uint32_t> rec_blk, rec_off; // per block of paths: records, output offset
     DevBuf<uint32_t> counters;     // per launch: count, work, (pad) x 4
     DevBuf<unsigned long long> stats;  // closest, any
     DevBuf<uint32_t> stk_ovf;          // deep traversal-stack entries of the trace kernel

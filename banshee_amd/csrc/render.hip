@@ -347,8 +347,6 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
     const int K = 1 << R.Kshift;
     bool emit_ext = false;
     uint32_t emit_mask = 0;
-    uint32_t rec_mask = 0;               // training records this path emits at its end
-    float rec_w[kRecVerts];
     f3 org = mk(0.f, 0.f, 0.f), ext_d = mk(0.f, 0.f, 0.f);
     if (i < R.P)
     {
@@ -669,8 +667,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                             // direction, so it cannot learn a glossy lobe)
                             float4 nn = rc[1];
                             float cs = fabsf(nn.x * d.x + nn.y * d.y + nn.z * d.z);
-                            rec_w[k] = lum * cs / d.w;
-                            rec_mask |= 1u << k;
+                            // the record's weight goes in the free .w of its throughput; the
+                            // chunk's compaction pass emits the records with a weight > 0
+                            reinterpret_cast<float*>(R.rec + (size_t)i * kRecStride + k * 5 + 3)[3] = lum * cs / d.w;
                         }
                     }
                 }
@@ -740,31 +739,100 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
             }
         }
     }
-    // ---- training records of finished paths: one atomic per wave ----
-    if (kFull)
+}
+
+// ---- training records: each chunk's records are compacted in (path, vertex) order ----
+// A record of path i, vertex k is emitted iff its direction was stored (pdf > 0) and its
+// weight (throughput .w) is positive. Three passes, no atomics, so the order and the
+// contents are the same run after run: per-block counts, one scan over the blocks (plus the
+// running total of earlier chunks), and the writes.
+__device__ __forceinline__ int rec_emitted(const RenderDev& R, uint32_t i, int k)
+{
+    const float4* rc = R.rec + (size_t)i * kRecStride + k * 5;
+    return rc[2].w > 0.f && rc[3].w > 0.f;
+}
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total)
+{
+    __shared__ uint32_t wsum[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = wave_incl_scan(v);
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t base = 0;
+    total = 0;
+    for (int w = 0; w < kBlock / 64; ++w)
     {
-        const uint32_t nrec = (uint32_t)__popc(rec_mask);
-        const uint32_t rincl = wave_incl_scan(nrec);
-        const uint32_t rtotal = __shfl(rincl, 63);
-        uint32_t rbase = 0;
-        if (lane == 63 && rtotal) rbase = atomicAdd(R.rec_count, rtotal);
-        rbase = __shfl(rbase, 63);
-        uint32_t slot = rbase + rincl - nrec;
-#pragma unroll
-        for (int k = 0; k < kRecVerts; ++k)
+        if (w < wave) base += wsum[w];
+        total += wsum[w];
+    }
+    __syncthreads();
+    return base + incl - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rec_count(RenderDev R, uint32_t* blk)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t n = 0;
+    if (i < R.P)
+        for (int k = 0; k < kRecVerts; ++k) n += rec_emitted(R, i, k);
+    uint32_t total;
+    (void)block_excl_scan(n, total);
+    if (threadIdx.x == 0) blk[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of the per-block counts, offset by the running total.
+constexpr int kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void k_rec_scan(const uint32_t* blk, uint32_t nblk, uint32_t* running,
+                                                         uint32_t* off)
+{
+    __shared__ uint32_t ws[kScanBlock / 64];
+    const uint32_t per = (nblk + kScanBlock - 1) / kScanBlock;
+    const uint32_t b0 = threadIdx.x * per, b1 = min(b0 + per, nblk);
+    uint32_t sum = 0;
+    for (uint32_t b = b0; b < b1; ++b) sum += blk[b];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = wave_incl_scan(sum);
+    if (lane == 63) ws[wave] = incl;
+    const uint32_t start = *running;
+    __syncthreads();
+    uint32_t base = start, total = 0;
+    for (int w = 0; w < kScanBlock / 64; ++w)
+    {
+        if (w < wave) base += ws[w];
+        total += ws[w];
+    }
+    uint32_t o = base + incl - sum;
+    for (uint32_t b = b0; b < b1; ++b)
+    {
+        off[b] = o;
+        o += blk[b];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *running = start + total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rec_write(RenderDev R, const uint32_t* off)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t mask = 0;
+    if (i < R.P)
+        for (int k = 0; k < kRecVerts; ++k) mask |= (uint32_t)rec_emitted(R, i, k) << k;
+    uint32_t total;
+    uint32_t slot = off[blockIdx.x] + block_excl_scan((uint32_t)__popc(mask), total);
+    for (int k = 0; k < kRecVerts; ++k)
+    {
+        if (!(mask & (1u << k))) continue;
+        if (slot < R.rec_cap)
         {
-            if (!(rec_mask & (1u << k))) continue;
-            if (slot < R.rec_cap)
-            {
-                const float4* rc = R.rec + (size_t)i * kRecStride + k * 5;
-                float4* q = reinterpret_cast<float4*>(R.rec_out) + (size_t)slot * 3;
-                float4 d = rc[2];
-                q[0] = rc[0];
-                q[1] = rc[1];
-                q[2] = make_float4(d.x, d.y, d.z, rec_w[k]);
-            }
-            ++slot;
+            const float4* rc = R.rec + (size_t)i * kRecStride + k * 5;
+            float4* q = reinterpret_cast<float4*>(R.rec_out) + (size_t)slot * 3;
+            float4 d = rc[2];
+            q[0] = rc[0];
+            q[1] = rc[1];
+            q[2] = make_float4(d.x, d.y, d.z, rc[3].w);
         }
+        ++slot;
     }
 }
 
@@ -1016,6 +1084,13 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
                 hipLaunchKernelGGL(k_shade<false>, pg, dim3(kBlock), 0, st, R, b, b + 1 <= ch.depth ? b + 1 : b);
             if (T) BSH_HIP(hipEventRecord(T->next(), st));
         }
+        if (R.rec_out)
+        {
+            hipLaunchKernelGGL(k_rec_count, pg, dim3(kBlock), 0, st, R, w.rec_blk.p);
+            hipLaunchKernelGGL(k_rec_scan, dim3(1), dim3(kScanBlock), 0, st, w.rec_blk.p, pg.x, w.rec_count.p,
+                               w.rec_off.p);
+            hipLaunchKernelGGL(k_rec_write, pg, dim3(kBlock), 0, st, R, w.rec_off.p);
+        }
         for (const Segment& sg : ch.segs)
         {
             const JobDev& J = hjobs[sg.job];
@@ -1172,6 +1247,8 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     {
         BSH_HIP(w.rec.alloc((size_t)kRecStride * P));
         BSH_HIP(w.rec_count.alloc(1));
+        BSH_HIP(w.rec_blk.alloc((P + kBlock - 1) / kBlock));
+        BSH_HIP(w.rec_off.alloc((P + kBlock - 1) / kBlock));
     }
     const size_t trace_threads = (size_t)c->num_cus * std::max(1, tune.trace_blocks_per_cu) * kBlock;
     BSH_HIP(w.stk_ovf.alloc(trace_threads * (std::max(kMaxStack, kMaxStack4) - kTraceLds)));

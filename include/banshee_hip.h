@@ -321,8 +321,9 @@ bsh_status bsh_guide_forward(bsh_guide* g, const float* inputs, size_t n, float*
 bsh_status bsh_guide_gradient(bsh_guide* g, const float* records, size_t n, float* grad, float* loss);
 /* One Adam step on the records. loss may be NULL. */
 bsh_status bsh_guide_train_step(bsh_guide* g, const float* records, size_t n, float* loss, void* stream);
-/* `steps` Adam steps on consecutive batches of min(batch, n) records (wrapping), e.g. the
-   records one guided pass wrote. loss (may be NULL) receives the last step's loss. */
+/* `steps` Adam steps on batches of up to `batch` records, e.g. the records one guided pass
+   wrote: with stride = max(1, n / batch), step s uses records s mod stride, + stride, ...
+   so every batch spans the whole array. loss (may be NULL) receives the last step's loss. */
 bsh_status bsh_guide_train(bsh_guide* g, const float* records, size_t n, int steps, size_t batch, float* loss,
                            void* stream);
 /* Renders like bsh_render (flags must include BSH_RENDER_NO_QUIRKS) with the continuation

@@ -96,6 +96,11 @@ def test_records_are_well_formed(ctx, cornell):
     assert np.allclose(np.linalg.norm(r[:, 8:11], axis=1), 1, atol=1e-4)
     assert np.all(np.isfinite(r[:, 11])) and np.all(r[:, 11] > 0)
     assert np.all(r[:, 7] == 1)
+    # records come out in (path, vertex) order, the same every run
+    recs2 = ctx.alloc((cap, hip.RECORD_FLOATS), np.float32)
+    n2 = S.render_guided(128, 128, 2, 5, 3, records=recs2, download=False)
+    ctx.sync()
+    assert n2 == n and np.array_equal(recs2.numpy()[:n], r)
 
 
 def test_guided_trained_is_unbiased(ctx, cornell):
