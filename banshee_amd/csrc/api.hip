@@ -412,15 +412,11 @@ bsh_status bsh_bake_sh(bsh_scene* s, const float* probes, size_t nprobes, const 
     return run_jobs(s->ctx, jobs, (p->flags & BSH_RENDER_COUNT) != 0, nullptr, 0, (hipStream_t)stream);
 }
 
-bsh_status bsh_render_guided(bsh_scene* s, const bsh_render_params* p, bsh_guide* guide, float* accum,
-                             float* records, size_t cap, size_t* nrecords, void* stream)
+// Runs one quirk-free job (a camera frame or a probe set) whose continuation samples the
+// BSDF/guide mixture when guide is set, appending training records when records is set.
+static bsh_status run_guided_job(bsh_scene* s, JobDev& J, bool count, bsh_guide* guide, float* records, size_t cap,
+                                 size_t* nrecords, hipStream_t st)
 {
-    if (!p || !(p->flags & BSH_RENDER_NO_QUIRKS))
-        return fail(BSH_ERR_INVALID, "bsh_render_guided: needs BSH_RENDER_NO_QUIRKS (the guided estimator is quirk-free)");
-    if (cap && !records) return fail(BSH_ERR_INVALID, "bsh_render_guided: NULL records with nonzero capacity");
-    JobDev J;
-    bsh_status r = make_render_job(s, p, accum, nullptr, J);
-    if (r != BSH_OK) return r;
     if (guide)
     {
         J.flags |= JOB_GUIDED;
@@ -434,9 +430,37 @@ bsh_status bsh_render_guided(bsh_scene* s, const bsh_render_params* p, bsh_guide
     gr.cap = cap;
     gr.bounces = guide ? guide_bounces(guide) : 0;
     std::vector<JobDev> jobs(1, J);
-    r = run_jobs(s->ctx, jobs, (p->flags & BSH_RENDER_COUNT) != 0, nullptr, 0, (hipStream_t)stream, &gr);
+    bsh_status r = run_jobs(s->ctx, jobs, count, nullptr, 0, st, &gr);
     if (nrecords) *nrecords = gr.written;
     return r;
+}
+
+bsh_status bsh_render_guided(bsh_scene* s, const bsh_render_params* p, bsh_guide* guide, float* accum,
+                             float* records, size_t cap, size_t* nrecords, void* stream)
+{
+    if (nrecords) *nrecords = 0;
+    if (!p || !(p->flags & BSH_RENDER_NO_QUIRKS))
+        return fail(BSH_ERR_INVALID, "bsh_render_guided: needs BSH_RENDER_NO_QUIRKS (the guided estimator is quirk-free)");
+    if (cap && !records) return fail(BSH_ERR_INVALID, "bsh_render_guided: NULL records with nonzero capacity");
+    JobDev J;
+    bsh_status r = make_render_job(s, p, accum, nullptr, J);
+    if (r != BSH_OK) return r;
+    return run_guided_job(s, J, (p->flags & BSH_RENDER_COUNT) != 0, guide, records, cap, nrecords, (hipStream_t)stream);
+}
+
+bsh_status bsh_bake_sh_guided(bsh_scene* s, const float* probes, size_t nprobes, const bsh_probe_params* p,
+                              bsh_guide* guide, float* coeffs, float* radiance, float* records, size_t cap,
+                              size_t* nrecords, void* stream)
+{
+    if (nrecords) *nrecords = 0;
+    if (!p || !(p->flags & BSH_RENDER_NO_QUIRKS))
+        return fail(BSH_ERR_INVALID, "bsh_bake_sh_guided: needs BSH_RENDER_NO_QUIRKS (the guided estimator is quirk-free)");
+    if (cap && !records) return fail(BSH_ERR_INVALID, "bsh_bake_sh_guided: NULL records with nonzero capacity");
+    JobDev J;
+    bsh_status r = make_probe_job(s, probes, nprobes, p, coeffs, radiance, J);
+    if (r != BSH_OK) return r;
+    if (nprobes == 0) return BSH_OK;
+    return run_guided_job(s, J, (p->flags & BSH_RENDER_COUNT) != 0, guide, records, cap, nrecords, (hipStream_t)stream);
 }
 
 bsh_status bsh_render_stats_get(const bsh_scene* s, bsh_render_stats* out)

@@ -1118,6 +1118,17 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
 
 constexpr int kMaxLights = 16;
 
+// The guide's position input is the hit point mapped to [0,1]^3 over the scene's box.
+static void set_guide_box(const bsh_scene* s, JobDev& J)
+{
+    for (int k = 0; k < 3; ++k)
+    {
+        J.glo[k] = s->box_lo[k];
+        float e = s->box_hi[k] - s->box_lo[k];
+        J.ginv[k] = e > 0.f ? 1.f / e : 0.f;
+    }
+}
+
 bsh_status make_render_job(bsh_scene* s, const bsh_render_params* p, float* accum, float* samples, JobDev& J)
 {
     if (!s || !p || !accum) return fail(BSH_ERR_INVALID, "bsh_render: NULL argument");
@@ -1156,12 +1167,7 @@ bsh_status make_render_job(bsh_scene* s, const bsh_render_params* p, float* accu
     J.accum = accum;
     J.samples = samples;
     J.flags = (p->flags & BSH_RENDER_NO_QUIRKS) ? JOB_NO_QUIRKS : 0;
-    for (int k = 0; k < 3; ++k)
-    {
-        J.glo[k] = s->box_lo[k];
-        float e = s->box_hi[k] - s->box_lo[k];
-        J.ginv[k] = e > 0.f ? 1.f / e : 0.f;
-    }
+    set_guide_box(s, J);
     return BSH_OK;
 }
 
@@ -1192,6 +1198,8 @@ bsh_status make_probe_job(bsh_scene* s, const float* probes, size_t nprobes, con
     J.tmin = p->tmin;
     J.coeffs = coeffs;
     J.radiance = radiance;
+    J.flags = (p->flags & BSH_RENDER_NO_QUIRKS) ? JOB_NO_QUIRKS : 0;
+    set_guide_box(s, J);
     return BSH_OK;
 }
 

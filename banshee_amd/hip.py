@@ -105,6 +105,7 @@ EXPORTS = [
     "bsh_guide_num_params", "bsh_guide_create", "bsh_guide_destroy", "bsh_guide_get_params", "bsh_guide_set_params",
     "bsh_guide_forward", "bsh_guide_gradient", "bsh_guide_train_step", "bsh_render_guided", "bsh_diag_mfma",
     "bsh_diag_guide", "bsh_bvh_build_host", "bsh_guide_set_bsdf_fraction", "bsh_guide_train",
+    "bsh_bake_sh_guided",
 ]
 
 _lib = None
@@ -600,8 +601,9 @@ class Scene:
         return a[..., 0:3], a[..., 4:7], a[..., 3], (samp.numpy() if samp else None)
 
     def bake_sh(self, probes, rays_per_probe, lmax, max_depth, seed, tmin=0.0, chunk_probes=0, radiance=False,
-                count=False, coeffs: DeviceArray | None = None, download=True):
-        """Returns (coeffs [nprobes, (lmax+1)^2, 3], per-ray radiance [nprobes, N, 3] or None)."""
+                count=False, coeffs: DeviceArray | None = None, download=True, quirk_free=False):
+        """Returns (coeffs [nprobes, (lmax+1)^2, 3], per-ray radiance [nprobes, N, 3] or None).
+        quirk_free selects the unbiased estimator (BSH_RENDER_NO_QUIRKS) instead of GiTracer's."""
         probes = np.ascontiguousarray(probes, np.float32).reshape(-1, 3)
         n = probes.shape[0]
         dp = self.ctx.to_device(probes)
@@ -612,13 +614,45 @@ class Scene:
         p = ProbeParams()
         p.rays_per_probe, p.lmax, p.max_depth, p.seed = rays_per_probe, lmax, max_depth, seed
         p.chunk_probes, p.tmin = chunk_probes, tmin
-        p.flags = 4 if count else 0
+        p.flags = (4 if count else 0) | (2 if quirk_free else 0)
         _check(lib().bsh_bake_sh(self.h, dp.ptr, c_size_t(n), ctypes.byref(p), coeffs.ptr,
                                  rad.ptr if rad else c_void_p(None), c_void_p(None)), "bsh_bake_sh")
         if not download:
             return None
         self.ctx.sync()
         return coeffs.numpy(), (rad.numpy() if rad else None)
+
+    def bake_sh_guided(self, probes, rays_per_probe, lmax, max_depth, seed, guide: "Guide | None" = None,
+                       tmin=0.0, chunk_probes=0, radiance=False, coeffs: DeviceArray | None = None,
+                       records: DeviceArray | None = None, download=True):
+        """Quirk-free probe bake whose continuation samples the BSDF/guide mixture. Returns
+        (coeffs, per-ray radiance or None, nrecords) when download, else nrecords. probes may
+        be a host array or a DeviceArray of shape [n, 3]."""
+        if isinstance(probes, DeviceArray):
+            dp = probes
+            n = probes.shape[0]
+        else:
+            probes = np.ascontiguousarray(probes, np.float32).reshape(-1, 3)
+            n = probes.shape[0]
+            dp = self.ctx.to_device(probes)
+        nt = (lmax + 1) ** 2
+        if coeffs is None:
+            coeffs = self.ctx.alloc((n, nt, 3), np.float32)
+        rad = self.ctx.alloc((n, rays_per_probe, 3), np.float32) if radiance else None
+        p = ProbeParams()
+        p.rays_per_probe, p.lmax, p.max_depth, p.seed = rays_per_probe, lmax, max_depth, seed
+        p.chunk_probes, p.tmin = chunk_probes, tmin
+        p.flags = 2
+        nrec = c_size_t(0)
+        cap = records.shape[0] if records is not None else 0
+        _check(lib().bsh_bake_sh_guided(self.h, dp.ptr, c_size_t(n), ctypes.byref(p), guide.h if guide else None,
+                                        coeffs.ptr, rad.ptr if rad else c_void_p(None),
+                                        records.ptr if records is not None else None, c_size_t(cap),
+                                        ctypes.byref(nrec), c_void_p(None)), "bsh_bake_sh_guided")
+        if not download:
+            return nrec.value
+        self.ctx.sync()
+        return coeffs.numpy(), (rad.numpy() if rad else None), nrec.value
 
     def render_guided(self, width, height, img_grid, max_depth, seed, guide: "Guide | None" = None, crop=None,
                       spp_range=None, chunk_spp=0, accum: DeviceArray | None = None, records: DeviceArray | None = None,

@@ -251,12 +251,65 @@ def probe_workload(args) -> dict:
         paths += st["camera_samples"]
     ctx.sync()
     dt = time.perf_counter() - t0
+    guided = None if args.no_guided else probe_guided_comparison(ctx, scene, probes)
     return {"metric": "SH probe baking Mrays/s", "value": round(rays / dt / 1e6, 2), "unit": "Mrays/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "4096 probes (16^3 grid) in a 2M-triangle terrain scene, 1024 rays/probe, "
                                    "depth 6, order-3 SH", "tris": int(sc.ntris),
-                       "probe_paths_per_s": round(paths / dt, 1), "probes_per_s": round(len(probes) * args.steps / dt, 1)}}
+                       "probe_paths_per_s": round(paths / dt, 1), "probes_per_s": round(len(probes) * args.steps / dt, 1)},
+            "guided": guided}
+
+
+P_RAYS, P_DEPTH, P_TRAIN_PASSES, P_STEPS = 1024, 6, 12, 8
+
+
+def probe_guided_comparison(ctx, scene, probes) -> dict:
+    """Plain vs guided proposal for the probe bake, both with the quirk-free estimator. The
+    guide trains on the probe paths' own records over P_TRAIN_PASSES bakes (BSDF fraction
+    falling from 1 to 0.5), then two independent bakes of each kind give the variance of a
+    coefficient as mean((c_1 - c_2)^2) / 2 over all probes and coefficients; each bake is
+    timed alone. efficiency = (var_plain t_plain) / (var_guided t_guided), training excluded
+    (reported on its own)."""
+    import numpy as np
+    from banshee_amd import hip
+    n = len(probes)
+    dp = ctx.to_device(np.ascontiguousarray(probes, np.float32))
+    c = [ctx.alloc((n, 9, 3), "float32") for _ in range(2)]
+
+    def bake_pair(guide, seed0):
+        times, out = [], []
+        for k in range(2):
+            ctx.sync()
+            t = time.perf_counter()
+            scene.bake_sh_guided(dp, P_RAYS, 2, P_DEPTH, seed0 + k, guide=guide, coeffs=c[k], download=False)
+            ctx.sync()
+            times.append(time.perf_counter() - t)
+            out.append(c[k].numpy().astype(np.float64))
+        return float(np.mean((out[0] - out[1]) ** 2) / 2), float(np.mean(times)), out[0]
+
+    bake_pair(None, 50)   # warm
+    var_p, t_p, cp = bake_pair(None, 300)
+    g = hip.Guide(ctx, seed=1, bsdf_fraction=1.0, lr=G_LR, beta2=0.99)
+    cap = n * P_RAYS * 4
+    recs = ctx.alloc((cap, hip.RECORD_FLOATS), np.float32)
+    ctx.sync()
+    t = time.perf_counter()
+    for k in range(P_TRAIN_PASSES):
+        g.set_bsdf_fraction(max(0.5, 1.0 - 0.5 * k / (P_TRAIN_PASSES // 2)))
+        nr = scene.bake_sh_guided(dp, P_RAYS, 2, P_DEPTH, 1000 + k, guide=g, records=recs, download=False)
+        if nr:
+            g.train(recs, min(nr, cap), P_STEPS, G_BATCH)
+    ctx.sync()
+    t_train = time.perf_counter() - t
+    var_g, t_g, cg = bake_pair(g, 400)
+    dc = float(np.mean(cg[:, 0, :]) / max(np.mean(cp[:, 0, :]), 1e-30))
+    return {"estimator": "quirk-free, probe rays uniform over the sphere, guided continuation at surface vertices",
+            "rays_per_probe": P_RAYS, "depth": P_DEPTH, "train_passes": P_TRAIN_PASSES,
+            "plain_bake_ms": round(t_p * 1e3, 2), "guided_bake_ms": round(t_g * 1e3, 2),
+            "train_s": round(t_train, 3), "coeff_var_plain": var_p, "coeff_var_guided": var_g,
+            "var_ratio": round(var_g / var_p, 4), "time_ratio": round(t_g / t_p, 4),
+            "efficiency": round(var_p * t_p / (var_g * t_g), 3), "dc_ratio_guided_over_plain": round(dc, 4)}
 
 
 def main():

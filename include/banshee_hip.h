@@ -142,7 +142,7 @@ typedef struct
     uint64_t node_visits, tri_tests;   /* totals over all queries, BSH_RENDER_COUNT only */
     double trace_ms, shade_ms, total_ms; /* summed HIP-event durations */
     uint32_t trace_launches, shade_launches;
-    uint32_t guide_launches;           /* guide network inference launches (bsh_render_guided) */
+    uint32_t guide_launches;           /* guide network inference launches (guided render or bake) */
     uint64_t guide_rows;               /* path vertices run through the guide network */
     double guide_ms;                   /* summed duration of the inference launches */
 } bsh_render_stats;
@@ -224,7 +224,7 @@ typedef struct
     int32_t chunk_probes;     /* probes per wavefront chunk, 0 = auto */
     float tmin;               /* probe ray tmin */
     float bg[3];
-    int32_t flags;            /* BSH_RENDER_COUNT honoured */
+    int32_t flags;            /* BSH_RENDER_COUNT and BSH_RENDER_NO_QUIRKS honoured */
 } bsh_probe_params;
 
 /* probes: device float[nprobes*3]; coeffs: device float[nprobes*(lmax+1)^2*3] (coefficient-major,
@@ -310,7 +310,7 @@ bsh_status bsh_guide_destroy(bsh_guide* g);
    W3, b3, W4, b4, W5[64][48], b5); set resets the Adam state. Host pointers. */
 bsh_status bsh_guide_get_params(const bsh_guide* g, float* host);
 bsh_status bsh_guide_set_params(bsh_guide* g, const float* host);
-/* The one-sample MIS BSDF fraction used by later bsh_render_guided calls, in (0, 1]. */
+/* The one-sample MIS BSDF fraction used by later guided render and bake calls, in (0, 1]. */
 bsh_status bsh_guide_set_bsdf_fraction(bsh_guide* g, float bsdf_fraction);
 /* Network outputs for n rows of raw inputs (8 floats per row: position in [0,1]^3,
    roughness, normal, unused). Device pointers; out n x 48. */
@@ -329,10 +329,20 @@ bsh_status bsh_guide_train(bsh_guide* g, const float* records, size_t n, int ste
 /* Renders like bsh_render (flags must include BSH_RENDER_NO_QUIRKS) with the continuation
    direction drawn from the one-sample MIS mixture of BSDF and guide (guide may be NULL: BSDF
    only, the plain quirk-free tracer), and appends training records of the first four path
-   vertices to `records` (device float[cap*12]; records beyond cap are dropped); *nrecords
-   receives the count written. */
+   vertices to `records` (device float[cap*12]) in (path, vertex) order, the same on every
+   run; records beyond cap are dropped and *nrecords receives the count written. */
 bsh_status bsh_render_guided(bsh_scene* s, const bsh_render_params* p, bsh_guide* guide, float* accum,
                              float* records, size_t cap, size_t* nrecords, void* stream);
+/* Probe baking like bsh_bake_sh with the quirk-free estimator (p->flags must include
+   BSH_RENDER_NO_QUIRKS) and the guided continuation of bsh_render_guided at every surface
+   vertex of a probe path. Each probe ray stays a stratified uniform-sphere direction, so the
+   projection c_i = (4 pi / N) sum_j L(w_j) Y_i(w_j) is unchanged and stays unbiased; the
+   guide only lowers the variance of each L(w_j). guide may be NULL (the plain quirk-free
+   bake); records, cap and nrecords as in bsh_render_guided, the first record vertex being
+   the probe ray's hit. */
+bsh_status bsh_bake_sh_guided(bsh_scene* s, const float* probes, size_t nprobes, const bsh_probe_params* p,
+                              bsh_guide* guide, float* coeffs, float* radiance, float* records, size_t cap,
+                              size_t* nrecords, void* stream);
 
 /* ---------------- diagnostics (exercise device code paths in isolation) ---------------- */
 /* One v_mfma_f32_16x16x16_f16 tile: d = a b for 16x16 row-major FP16 a, b (bit patterns). */

@@ -6,6 +6,8 @@
 * Guided rendering (one-sample MIS of BSDF and learned vMF mixture) has the same mean as
   the plain BSDF-sampling tracer, with an untrained and with a trained guide.
 * Training records written by the tracer are well formed.
+* What training buys: on a small occluded room the trained guide lowers the per-sample
+  variance well below the plain tracer's, while the same network untrained does not.
 """
 import numpy as np
 import pytest
@@ -149,3 +151,46 @@ def test_training_survives_extreme_weights(ctx):
         loss = g.train_step(rec)
     assert np.isfinite(loss)
     assert np.all(np.isfinite(g.get_params()))
+
+
+def mean_sample_variance(res):
+    s, s2, n = res[0].astype(np.float64), res[1].astype(np.float64), res[2].astype(np.float64)[..., None]
+    mean = s / n
+    return float(np.mean(np.maximum(s2 / n - mean * mean, 0) * n / np.maximum(n - 1, 1)))
+
+
+def test_trained_guide_lowers_variance(ctx):
+    """The occluded room at 120k triangles, 128^2: the camera's half is lit only through four
+    openings in a wall. The guide trains online for 64 passes of 4 spp (BSDF fraction falling
+    from 1 to 0.5 over the first 8, eight Adam steps per pass), then plain, untrained-guided
+    and trained-guided renders of 36 spp are compared: the trained guide's mean per-sample
+    variance is below 0.75 of the plain tracer's and below 0.5 of the untrained guide's, and
+    the image means agree. Training and rendering are deterministic, so these are fixed
+    numbers for this code (0.65 and 0.34 when written)."""
+    from banshee_amd import hip
+    from banshee_amd.scenes import occluded_room
+    W = 128
+    sc = occluded_room(target_tris=120_000, width=W, height=W)
+    S = hip.Scene(ctx, sc)
+    g = hip.Guide(ctx, seed=1, bsdf_fraction=1.0, lr=5e-3, beta2=0.99)
+    cap = W * W * 4 * 4
+    recs = ctx.alloc((cap, hip.RECORD_FLOATS), np.float32)
+    for it in range(64):
+        g.set_bsdf_fraction(max(0.5, 1.0 - 0.5 * it / 8))
+        n = S.render_guided(W, W, 2, 8, 500 + it, guide=g, records=recs, download=False)
+        if n:
+            g.train(recs, min(n, cap), 8, 1 << 16)
+    ctx.sync()
+    g0 = hip.Guide(ctx, seed=1, bsdf_fraction=0.5)
+    plain = S.render_guided(W, W, 6, 8, 901)
+    trained = S.render_guided(W, W, 6, 8, 902, guide=g)
+    untrained = S.render_guided(W, W, 6, 8, 903, guide=g0)
+    vp, vt, vu = (mean_sample_variance(r) for r in (plain, trained, untrained))
+    print(f"mean per-sample variance: plain {vp:.4g}, untrained guide {vu:.4g}, trained guide {vt:.4g}")
+    assert vt < 0.75 * vp, (vt, vp)
+    assert vt < 0.5 * vu, (vt, vu)
+    # the image means agree (unbiasedness block by block is pinned on the Cornell box above)
+    mp, sp = pixel_stats(plain)
+    mt, st = pixel_stats(trained)
+    z = (mp.mean(axis=(0, 1)) - mt.mean(axis=(0, 1))) / np.sqrt(sp.sum(axis=(0, 1)) + st.sum(axis=(0, 1))) * W * W
+    assert np.all(np.abs(z) < 4.5), z

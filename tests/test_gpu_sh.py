@@ -58,7 +58,8 @@ PROBES = np.array([[1.0, 1.0, 1.0], [0.5, 1.5, 1.5], [1.5, 0.3, 1.2], [0.2, 0.2,
 
 
 def _sh_fp64(d):
-    """Appendix-C closed forms of the reference basis (z pole, Condon-Shortley), lmax = 2."""
+    """Closed forms of the reference basis for lmax = 2 (math/sh.cpp:97-153: z pole,
+    Condon-Shortley phase, index l^2 + l + m, m < 0 from sin(|m| phi))."""
     x, y, z = d[..., 0], d[..., 1], d[..., 2]
     return np.stack([0.28209479177387814 * np.ones_like(x), -0.4886025119029199 * y, 0.4886025119029199 * z,
                      -0.4886025119029199 * x, 1.0925484305920792 * x * y, -1.0925484305920792 * y * z,
@@ -66,16 +67,11 @@ def _sh_fp64(d):
                      0.5462742152960396 * (x * x - y * y)], -1)
 
 
-def test_probe_projection_of_a_radiance_tape(ctx, cornell):
-    """Given the GPU's own per-ray radiance, its FP32 projection matches an FP64 projection."""
-    from banshee_amd import hip
-    s = hip.Scene(ctx, cornell)
-    N, seed = 1024, 7
-    c, rad = s.bake_sh(PROBES, N, 2, 5, seed, radiance=True)
-    # regenerate the stratified directions from the keyed stream in float64
+def probe_dirs(seed, P, N):
+    """The stratified probe-ray directions [P, N, 3] regenerated in float64 from the keyed
+    stream (stream 4, counters 0 and 1 give the jitter of cell j of probe p, sample p*N + j)."""
     from test_gpu_shading import _keyed_np
-    P = PROBES.shape[0]
-    g = 32
+    g = int(round(np.sqrt(N)))
     j = np.arange(N)
     cs = (np.arange(P)[:, None] * N + j[None]).astype(np.uint32).ravel()
     st = np.full(cs.shape, 4, np.uint32)
@@ -85,8 +81,16 @@ def test_probe_projection_of_a_radiance_tape(ctx, cornell):
     v = ((j // g)[None].repeat(P, 0).ravel() + uy) / g
     z = 1 - 2 * u
     r = np.sqrt(np.maximum(0, 1 - z * z))
-    d = np.stack([r * np.cos(2 * np.pi * v), r * np.sin(2 * np.pi * v), z], -1).reshape(P, N, 3)
-    Y = _sh_fp64(d)
+    return np.stack([r * np.cos(2 * np.pi * v), r * np.sin(2 * np.pi * v), z], -1).reshape(P, N, 3)
+
+
+def test_probe_projection_of_a_radiance_tape(ctx, cornell):
+    """Given the GPU's own per-ray radiance, its FP32 projection matches an FP64 projection."""
+    from banshee_amd import hip
+    s = hip.Scene(ctx, cornell)
+    N, seed = 1024, 7
+    c, rad = s.bake_sh(PROBES, N, 2, 5, seed, radiance=True)
+    Y = _sh_fp64(probe_dirs(seed, PROBES.shape[0], N))
     ref = 4 * np.pi / N * np.einsum("pnk,pnc->pkc", Y, rad.astype(np.float64))
     ok = sh_close(c, ref, rtol=1e-5)
     assert ok.all(), np.abs(c - ref).max()
