@@ -482,6 +482,8 @@ bsh_status bsh_context_set_tuning(bsh_context* ctx, const bsh_tuning* t)
     if (!ctx || !t) return fail(BSH_ERR_INVALID, "bsh_context_set_tuning: NULL argument");
     if (t->paths_per_chunk < 0 || t->trace_blocks_per_cu < 0 || t->trace_blocks_per_cu > 5)
         return fail(BSH_ERR_INVALID, "bsh_context_set_tuning: trace_blocks_per_cu must be 0..5 (LDS limit)");
+    if (t->refill_lanes < -1 || t->refill_lanes > 64)
+        return fail(BSH_ERR_INVALID, "bsh_context_set_tuning: refill_lanes must be -1..64");
     ctx->tuning = *t;
     return BSH_OK;
 }
@@ -540,7 +542,7 @@ bsh_status bsh_autotune(bsh_context* ctx, const bsh_job* jobs, size_t njobs, con
                         double* best_ms)
 {
     if (!ctx || (njobs && !jobs)) return fail(BSH_ERR_INVALID, "bsh_autotune: NULL argument");
-    static const int32_t kPaths[] = {1 << 19, 1 << 20, 1 << 21, 1 << 22};
+    static const int32_t kPaths[] = {1 << 21, 1 << 22, 1 << 23, 1 << 24};
     static const int32_t kBlocks[] = {2, 3, 4, 5};
     if (!paths_per_chunk || n_paths == 0) { paths_per_chunk = kPaths; n_paths = 4; }
     if (!blocks_per_cu || n_blocks == 0) { blocks_per_cu = kBlocks; n_blocks = 4; }
@@ -555,7 +557,7 @@ bsh_status bsh_autotune(bsh_context* ctx, const bsh_job* jobs, size_t njobs, con
         {
             if (blocks_per_cu[b] < 1 || blocks_per_cu[b] > 5 || paths_per_chunk[a] < 1)
                 return fail(BSH_ERR_INVALID, "bsh_autotune: candidate out of range");
-            bsh_tuning t{paths_per_chunk[a], blocks_per_cu[b], 0, 0};
+            bsh_tuning t{paths_per_chunk[a], blocks_per_cu[b], 0, ctx->tuning.refill_lanes};
             std::vector<JobDev> copy = hj;
             r = run_jobs(ctx, copy, false, &t, 0, nullptr);
             if (r != BSH_OK) return r;

@@ -49,6 +49,10 @@ constexpr int kTraceLds = 32;
 constexpr int kRecVerts = 4, kRecStride = 5 * kRecVerts;
 static_assert(kGuideSeg == kBlock, "guide row segments are the blocks of k_guide_inputs");
 constexpr int kMaxTraceBlocksPerCU = 5;
+// Idle lanes of a wave that trigger a refill in k_trace_refill (when the tuning leaves it 0).
+// Measured on the 3M-triangle terrain at 1080p: 8 lanes 733 Mrays/s, 16 1058, 32 1314,
+// 48 1381, 56 1393, 64 1353; whole 64-ray batches per wave (k_trace_queue) 1065.
+constexpr int kDefaultRefill = 56;
 
 enum : uint32_t
 {
@@ -300,6 +304,187 @@ __global__ __launch_bounds__(kBlock) void k_trace_queue(RenderDev R, int L, int 
                 vis_nodes += cnt.nodes;
                 vis_tris += cnt.tris;
             }
+        }
+    }
+    if (kCount)
+    {
+        atomicAdd(&R.stats[2], (unsigned long long)vis_nodes);
+        atomicAdd(&R.stats[3], (unsigned long long)vis_tris);
+    }
+}
+
+// 4-wide traversal with per-lane refill. The closest-hit queue and the any-hit queue form one
+// index range [0, nc + na) drawn by one work counter. Every lane holds its own ray state and
+// advances it by one node visit per loop iteration; a finished lane idles until `refill` or
+// more lanes of its wave are idle (or the whole wave is), when the wave claims exactly that
+// many new rays with one atomic and hands them to the idle lanes in lane order. Lanes of one
+// wave may hold closest and any-hit rays of different scenes at once. Node visits and
+// accept rules are those of traverse4, so every ray's result is the same.
+template <bool kCount>
+__global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(RenderDev R, int L, int buf, int refill)
+{
+    __shared__ uint32_t stk[kTraceLds * kBlock];
+    const int lane = threadIdx.x & 63;
+    uint32_t* my = stk + threadIdx.x;
+    const int ostride = gridDim.x * kBlock;
+    uint32_t* ovf = R.stk_ovf + blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t nc = R.counters[4 * L + 0];
+    const uint32_t total = nc + R.counters[4 * L + 2];
+    uint32_t* work = &R.counters[4 * L + 1];
+    const float kInf = __builtin_inff();
+    uint32_t vis_nodes = 0, vis_tris = 0;
+    bool active = false, any = false, exhausted = false;
+    f3 o = mk(0.f, 0.f, 0.f), d = mk(0.f, 0.f, 0.f);
+    float ix = 0.f, iy = 0.f, iz = 0.f, ox = 0.f, oy = 0.f, oz = 0.f, tmin = 0.f, tcur = 0.f, hb1 = 0.f, hb2 = 0.f;
+    int prim = -1, sp = 0;
+    uint32_t node = 0, own = 0;
+    const float4* nodes = nullptr;
+    const float4* tris = nullptr;
+    for (;;)
+    {
+        // ---- refill the idle lanes (wave-uniform decision) ----
+        const unsigned long long idle = __ballot(!active);
+        const int nidle = __popcll(idle);
+        if (!exhausted && (nidle >= refill || nidle == 64))
+        {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(work, (uint32_t)nidle);
+            base = __shfl(base, 0);
+            if (base + (uint32_t)nidle >= total) exhausted = true;
+            if (!active)
+            {
+                const uint32_t q = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (q < total)
+                {
+                    any = q >= nc;
+                    const uint32_t k = any ? q - nc : q;
+                    const float4 a = any ? R.qa_o[buf][k] : R.qc_o[buf][k];
+                    const float4 b = any ? R.qa_d[buf][k] : R.qc_d[buf][k];
+                    own = (uint32_t)(any ? R.qa_own[buf][k] : R.qc_own[buf][k]);
+                    const uint32_t job = R.meta[path_of(R, own >> 29, own & ((1u << 29) - 1u))].y >> 24;
+                    nodes = R.jobs[job].s.nodes;
+                    tris = R.jobs[job].s.tris;
+                    o = mk(a.x, a.y, a.z);
+                    d = mk(b.x, b.y, b.z);
+                    const float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
+                    const float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
+                    const float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
+                    ix = 1.f / dx;
+                    iy = 1.f / dy;
+                    iz = 1.f / dz;
+                    ox = o.x * ix;
+                    oy = o.y * iy;
+                    oz = o.z * iz;
+                    tmin = a.w;
+                    tcur = b.w;
+                    prim = -1;
+                    hb1 = hb2 = 0.f;
+                    node = 0;
+                    sp = 0;
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) break;
+        if (!active) continue;
+        // ---- one node visit (traverse4's loop body) ----
+        if (kCount) vis_nodes++;
+        const float4* nd = nodes + 8 * (size_t)node;
+        const float4 LX = nd[0], HX = nd[1], LY = nd[2], HY = nd[3], LZ = nd[4], HZ = nd[5];
+        const int4 REF = reinterpret_cast<const int4*>(nd)[6];
+        const int4 CNT = reinterpret_cast<const int4*>(nd)[7];
+        const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, hx[4] = {HX.x, HX.y, HX.z, HX.w};
+        const float ly[4] = {LY.x, LY.y, LY.z, LY.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w};
+        const float lz[4] = {LZ.x, LZ.y, LZ.z, LZ.w}, hz[4] = {HZ.x, HZ.y, HZ.z, HZ.w};
+        const int ref[4] = {REF.x, REF.y, REF.z, REF.w}, cn[4] = {CNT.x, CNT.y, CNT.z, CNT.w};
+        float tn[4];
+        bool hit[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+            float x0 = __builtin_fmaf(lx[c], ix, -ox), x1 = __builtin_fmaf(hx[c], ix, -ox);
+            float y0 = __builtin_fmaf(ly[c], iy, -oy), y1 = __builtin_fmaf(hy[c], iy, -oy);
+            float z0 = __builtin_fmaf(lz[c], iz, -oz), z1 = __builtin_fmaf(hz[c], iz, -oz);
+            tn[c] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tcur));
+            hit[c] = (tn[c] <= tf) && (cn[c] >= 0);
+        }
+        bool done = false;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+            if (!done && hit[c] && cn[c] > 0)
+            {
+                for (int q = ref[c]; q < ref[c] + cn[c]; ++q)
+                {
+                    const float4 v0 = tris[3 * q + 0];
+                    const float4 e1 = tris[3 * q + 1];
+                    const float4 e2 = tris[3 * q + 2];
+                    if (kCount) vis_tris++;
+                    float t, u, v;
+                    if (intersect_face(ld3(v0), ld3(e1), ld3(e2), o, d, tmin, tcur, t, u, v))
+                    {
+                        prim = __builtin_bit_cast(int, v0.w);
+                        tcur = t;
+                        hb1 = u;
+                        hb2 = v;
+                        if (any)
+                        {
+                            done = true;
+                            break;
+                        }
+                    }
+                }
+            }
+            if (cn[c] > 0) hit[c] = false;
+        }
+        if (!done)
+        {
+            float key[4];
+            uint32_t id[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+            {
+                key[c] = (hit[c] && tn[c] <= tcur) ? tn[c] : kInf;
+                id[c] = (uint32_t)ref[c];
+            }
+#define BSH_CSWAP(a, b)                                       \
+    if (key[b] < key[a])                                      \
+    {                                                         \
+        float tk = key[a]; key[a] = key[b]; key[b] = tk;      \
+        uint32_t ti = id[a]; id[a] = id[b]; id[b] = ti;       \
+    }
+            BSH_CSWAP(0, 1)
+            BSH_CSWAP(2, 3)
+            BSH_CSWAP(0, 2)
+            BSH_CSWAP(1, 3)
+            BSH_CSWAP(1, 2)
+#undef BSH_CSWAP
+            const int n = (key[0] < kInf) + (key[1] < kInf) + (key[2] < kInf) + (key[3] < kInf);
+            if (n == 0)
+            {
+                if (sp == 0) done = true;
+                else node = stack_pop<kTraceLds, kMaxStack4>(my, kBlock, ovf, ostride, sp);
+            }
+            else
+            {
+#pragma unroll
+                for (int k = 3; k >= 1; --k)
+                    if (k < n) stack_push<kTraceLds, kMaxStack4>(my, kBlock, ovf, ostride, sp, id[k]);
+                node = id[0];
+            }
+        }
+        if (done)
+        {
+            const uint32_t kind = own >> 29, idx = own & ((1u << 29) - 1u);
+            if (any) R.nee_res[idx] = make_int4(prim >= 0 ? 1 : 0, 0, 0, 0);
+            else
+            {
+                const int4 res = make_int4(prim, __float_as_int(tcur), __float_as_int(hb1), __float_as_int(hb2));
+                if (kind == Q_EXT) R.hit[idx] = res;
+                else R.nee_res[idx] = res;
+            }
+            active = false;
         }
     }
     if (kCount)
@@ -1041,12 +1226,17 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
 {
     RenderWork& w = c->work;
     const int trace_blocks = c->num_cus * std::max(1, tune.trace_blocks_per_cu);
+    const int refill = tune.refill_lanes == 0 ? kDefaultRefill : tune.refill_lanes;
     for (const Chunk& ch : chunks)
     {
         const int launches = ch.depth + 1;
         R.P = ch.P;
-        bool full = false;
-        for (const Segment& sg : ch.segs) full = full || hjobs[sg.job].flags != 0;
+        bool full = false, wide = refill > 0;
+        for (const Segment& sg : ch.segs)
+        {
+            full = full || hjobs[sg.job].flags != 0;
+            wide = wide && hjobs[sg.job].s.width == 4;
+        }
         hipLaunchKernelGGL(k_zero, grid_for(4 * launches), dim3(kBlock), 0, st, w.counters.p, 4 * launches);
         for (const Segment& sg : ch.segs)
         {
@@ -1060,7 +1250,14 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
         for (int b = 0; b <= ch.depth; ++b)
         {
             if (T) BSH_HIP(hipEventRecord(T->next(), st));
-            if (count)
+            if (wide)
+            {
+                if (count)
+                    hipLaunchKernelGGL(k_trace_refill<true>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1, refill);
+                else
+                    hipLaunchKernelGGL(k_trace_refill<false>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1, refill);
+            }
+            else if (count)
                 hipLaunchKernelGGL(k_trace_queue<true>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1);
             else
                 hipLaunchKernelGGL(k_trace_queue<false>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1);
@@ -1211,7 +1408,7 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     std::lock_guard<std::mutex> lock(c->mu);
     BSH_HIP(hipSetDevice(c->device));
     bsh_tuning tune = tune_in ? *tune_in : c->tuning;
-    if (tune.paths_per_chunk <= 0) tune.paths_per_chunk = 1 << 21;
+    if (tune.paths_per_chunk <= 0) tune.paths_per_chunk = 1 << 23;
     if (tune.trace_blocks_per_cu <= 0) tune.trace_blocks_per_cu = kMaxTraceBlocksPerCU;
     int nlmax = 1;
     uint32_t maxunit = 1;

@@ -78,9 +78,10 @@ def cpu_baseline(sc_full_args) -> dict | None:
 
 # Guided vs plain (quirk-free) path tracing: the occluded room, 500k triangles, 1024x1024, depth 8.
 G_RES, G_DEPTH, G_TARGET = 1024, 8, 0.02
-G_PASS_GRID, G_CHECK = 2, 4               # 4 spp per pass; RMSE estimated every 4 passes (16 spp)
-G_ALPHA, G_WARM, G_LR, G_BATCH = 0.5, 8, 5e-3, 1 << 18
-G_STEPS, G_STEPS_LATE, G_LATE = 8, 1, 12   # Adam steps per pass: G_STEPS, from pass G_LATE on G_STEPS_LATE
+G_PASS_GRID, G_CHECK = 2, 1               # 4 spp per pass; RMSE estimated after every pass (outside the clock)
+G_ALPHA, G_WARM, G_LR, G_BATCH = 0.5, 6, 5e-3, 1 << 18
+G_STEPS, G_STEPS_LATE, G_LATE = 8, 0, 10   # Adam steps per pass: G_STEPS, from pass G_LATE on G_STEPS_LATE
+# (passes that train no step write no records)
 G_BOUNCES = 0                              # guided path vertices (0 = all)
 # the BSDF fraction falls linearly from 1 to G_ALPHA over the first G_WARM passes, while the
 # untrained guide would only add variance
@@ -96,12 +97,13 @@ def rmse_estimate(a) -> float:
     return float(np.sqrt(np.mean(var)))
 
 
-def time_to_rmse(ctx, scene, guide, max_passes, seed0):
+def time_to_rmse(ctx, scene, guide, max_passes, seed0) -> dict:
     """Renders passes of G_PASS_GRID^2 spp into one accumulation buffer until the RMSE
     estimate reaches G_TARGET. With a guide, every pass also writes training records and
     runs G_STEPS Adam steps on G_BATCH of them; every pass's samples count in the image.
-    Returns (seconds of render + training, spp, rmse, training seconds, guided network rows,
-    milliseconds of inference launches). The RMSE checks are outside the clock."""
+    Returns the seconds of render + training, spp, rmse, training seconds, guided network rows,
+    and the summed launch times (ms) of trace, shade and inference with the BVH queries. The
+    RMSE checks are outside the clock."""
     import numpy as np
     from banshee_amd import hip
     acc = ctx.alloc((G_RES, G_RES, 8), np.float32)
@@ -111,20 +113,21 @@ def time_to_rmse(ctx, scene, guide, max_passes, seed0):
     clock = train = 0.0
     e = float("inf")
     k = 0
-    rows = 0
-    infer_ms = 0.0
+    tot = {"guide_rows": 0, "guide_ms": 0.0, "trace_ms": 0.0, "shade_ms": 0.0, "rays": 0}
     while k < max_passes:
         t = time.perf_counter()
+        steps = (G_STEPS if k < G_LATE else G_STEPS_LATE) if guide is not None else 0
         if guide is not None:
             guide.set_bsdf_fraction(max(G_ALPHA, 1.0 - (1.0 - G_ALPHA) * k / G_WARM))
-        n = scene.render_guided(G_RES, G_RES, G_PASS_GRID, G_DEPTH, seed0 + k, guide=guide, records=recs,
-                                accum=acc, accumulate=k > 0, download=False)
+        n = scene.render_guided(G_RES, G_RES, G_PASS_GRID, G_DEPTH, seed0 + k, guide=guide,
+                                records=recs if steps > 0 else None, accum=acc, accumulate=k > 0, download=False)
         st = scene.stats()
-        rows += st["guide_rows"]
-        infer_ms += st["guide_ms"]
-        if guide is not None and n:
+        for key in ("guide_rows", "guide_ms", "trace_ms", "shade_ms"):
+            tot[key] += st[key]
+        tot["rays"] += st["rays_closest"] + st["rays_any"]
+        if steps > 0 and n:
             t2 = time.perf_counter()
-            guide.train(recs, min(n, cap), G_STEPS if k < G_LATE else G_STEPS_LATE, G_BATCH)
+            guide.train(recs, min(n, cap), steps, G_BATCH)
             ctx.sync()
             train += time.perf_counter() - t2
         ctx.sync()
@@ -134,7 +137,19 @@ def time_to_rmse(ctx, scene, guide, max_passes, seed0):
             e = rmse_estimate(acc.numpy())
             if e <= G_TARGET:
                 break
-    return clock, k * G_PASS_GRID ** 2, e, train, rows, infer_ms
+    spp = k * G_PASS_GRID ** 2
+    return {"s": clock, "spp": spp, "rmse": e, "train_s": train, "rows": tot["guide_rows"],
+            "infer_ms": tot["guide_ms"], "ms_per_spp": clock * 1e3 / spp,
+            "trace_ms_per_spp": tot["trace_ms"] / spp, "shade_ms_per_spp": tot["shade_ms"] / spp,
+            "rays_per_sample": tot["rays"] / (spp * G_RES * G_RES)}
+
+
+def make_context(args):
+    from banshee_amd import hip
+    ctx = hip.Context(0)
+    if args.refill_lanes or args.paths_per_chunk:
+        ctx.set_tuning(paths_per_chunk=args.paths_per_chunk, refill_lanes=args.refill_lanes)
+    return ctx
 
 
 def guided_comparison(ctx, bvh_width) -> dict:
@@ -144,16 +159,20 @@ def guided_comparison(ctx, bvh_width) -> dict:
     scene = hip.Scene(ctx, sc, width=bvh_width)
     scene.render_guided(G_RES, G_RES, 1, G_DEPTH, 1, download=False)   # warm the wavefront buffers
     ctx.sync()
-    tp, sp, ep, _, _, _ = time_to_rmse(ctx, scene, None, 4096, 100000)
+    p = time_to_rmse(ctx, scene, None, 4096, 100000)
     guide = hip.Guide(ctx, seed=1, bsdf_fraction=1.0, lr=G_LR, beta2=0.99, guided_bounces=G_BOUNCES)
-    tg, sg, eg, tt, rows, infer_ms = time_to_rmse(ctx, scene, guide, 4096, 200000)
+    g = time_to_rmse(ctx, scene, guide, 4096, 200000)
+    tp, sp, ep = p["s"], p["spp"], p["rmse"]
+    tg, sg, eg, tt, rows, infer_ms = g["s"], g["spp"], g["rmse"], g["train_s"], g["rows"], g["infer_ms"]
     tflops = rows * MLP_FLOP_PER_ROW / (infer_ms / 1e3) / 1e12 if infer_ms > 0 else 0.0
+    keys = ("ms_per_spp", "trace_ms_per_spp", "shade_ms_per_spp", "rays_per_sample")
     return {"workload": f"occluded room ({sc.ntris} tris, the light behind a wall with 4 openings totalling 5% of its "
                         f"area, the camera's side lit only through them), "
                         f"{G_RES}x{G_RES}, depth {G_DEPTH}, quirk-free estimator",
             "target_rmse": G_TARGET, "plain_s": round(tp, 3), "plain_spp": sp, "plain_rmse": round(ep, 5),
             "guided_s": round(tg, 3), "guided_spp": sg, "guided_rmse": round(eg, 5),
             "guided_train_s": round(tt, 3), "time_ratio": round(tg / tp, 3),
+            "per_spp": {"plain": {k: round(p[k], 4) for k in keys}, "guided": {k: round(g[k], 4) for k in keys}},
             "mlp_roofline": {"bound": "mfma", "kernel": "k_guide_forward (inference, FP16 MFMA)",
                              "achieved": round(tflops, 1), "peak": MFMA_FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
                              "frac": round(tflops / MFMA_FP16_PEAK_TFLOPS, 4), "rows": int(rows),
@@ -180,7 +199,7 @@ def batch_workload(args) -> dict:
     hipGraph with autotuned chunk size and trace grid; compared with rendering each job alone."""
     from banshee_amd import hip
     from banshee_amd.scenes import batch_jobs, terrain_for_tris
-    ctx = hip.Context(0)
+    ctx = make_context(args)
     t0 = time.time()
     cfgs = batch_jobs(n=64, seed=100)
     scenes, jobs = [], []
@@ -193,7 +212,7 @@ def batch_workload(args) -> dict:
     setup_s = time.time() - t0
     tris = sum(s.info()["tris"] for s in scenes)
     t1 = time.time()
-    best, best_ms = ctx.autotune(jobs, paths=[1 << 19, 1 << 20, 1 << 21], blocks=[3, 4, 5])
+    best, best_ms = ctx.autotune(jobs, paths=[1 << 21, 1 << 22, 1 << 23], blocks=[3, 4, 5])
     tune_s = time.time() - t1
     tuning = dict(best, use_graph=1)
     for _ in range(args.warmup):
@@ -209,7 +228,7 @@ def batch_workload(args) -> dict:
     ctx.sync()
     dt = time.perf_counter() - t2
     # the same jobs one by one (bsh_render each, same tuning, no graph)
-    ctx.set_tuning(best["paths_per_chunk"], best["trace_blocks_per_cu"], 0)
+    ctx.set_tuning(best["paths_per_chunk"], best["trace_blocks_per_cu"], 0, best["refill_lanes"])
     t3 = time.perf_counter()
     for j in jobs:
         j.scene.render(j.params.width, j.params.height, j.params.img_grid, j.params.max_depth, j.params.seed,
@@ -234,7 +253,7 @@ def probe_workload(args) -> dict:
     per probe, depth 6, order-3 SH (9 coefficients x RGB), all probes in one wavefront."""
     from banshee_amd import hip
     from banshee_amd.scenes import probe_grid, terrain_for_tris
-    ctx = hip.Context(0)
+    ctx = make_context(args)
     sc = terrain_for_tris(2_000_000, seed=7)
     scene = hip.Scene(ctx, sc, width=args.bvh_width)
     probes = probe_grid(sc, 16)
@@ -320,6 +339,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain time-to-RMSE comparison")
     ap.add_argument("--bvh-width", type=int, default=4, choices=(2, 4), help="BVH node width")
+    ap.add_argument("--paths-per-chunk", type=int, default=0,
+                    help="paths in flight per wavefront chunk (0 = library default, 2^23)")
+    ap.add_argument("--refill-lanes", type=int, default=0,
+                    help="idle lanes per wave that trigger a ray refill in the 4-wide trace kernel "
+                         "(0 = library default, -1 = whole 64-ray batches per wave)")
     ap.add_argument("--guided-only", action="store_true",
                     help="run only the guided-vs-plain comparison and print its JSON (for profiling)")
     ap.add_argument("--workload", default="gi", choices=("gi", "batch", "probes"),
@@ -335,7 +359,7 @@ def main():
     from banshee_amd.scenes import terrain_scene
 
     if args.guided_only:
-        print(json.dumps(guided_comparison(hip.Context(0), args.bvh_width)), flush=True)
+        print(json.dumps(guided_comparison(make_context(args), args.bvh_width)), flush=True)
         return
     if args.workload == "batch":
         print(json.dumps(batch_workload(args)), flush=True)
@@ -346,7 +370,7 @@ def main():
 
     t0 = time.time()
     sc = terrain_scene(width=W, height=H)
-    ctx = hip.Context(0)
+    ctx = make_context(args)
     scene = hip.Scene(ctx, sc, width=args.bvh_width)
     info = scene.info()
     setup_s = time.time() - t0
@@ -406,7 +430,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(int(info["width"])),
                      "traffic_source": "profiles/k_trace_queue_terrain3M_pmc.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
-                     "kernel": "k_trace_queue", "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+                     "kernel": "k_trace_refill" if int(info["width"]) == 4 and args.refill_lanes != -1 else "k_trace_queue",
+                     "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                      "bytes_per_ray": round(b_ray, 1), "node_visits_per_ray": round(v_node, 2),
                      "tri_tests_per_ray": round(v_tri, 2), "trace_share": round(trace_ms / (dt * 1e3), 3)},
         "cpu_baseline": None,

@@ -244,13 +244,16 @@ bsh_status bsh_sh_irradiance(bsh_context* ctx, int lmax, const float* coeffs, co
 /* ---------------- batching (no reference counterpart) ---------------- */
 /* Launch-shape parameters of the wavefront: paths in flight per chunk (queue capacity)
    and persistent trace workgroups per CU; use_graph captures the whole batch as one
-   hipGraph and launches it once. */
+   hipGraph and launches it once. refill_lanes: on 4-wide BVHs each trace lane holds its own
+   ray and a wave claims new rays once this many of its 64 lanes are idle (1..64; 0 = the
+   default, 56); -1 traces in whole 64-ray batches per wave instead (also the form used when
+   a chunk holds a binary BVH). Results do not depend on it. */
 typedef struct
 {
     int32_t paths_per_chunk;
     int32_t trace_blocks_per_cu;
     int32_t use_graph;
-    int32_t reserved;
+    int32_t refill_lanes;
 } bsh_tuning;
 
 enum { BSH_JOB_RENDER = 0, BSH_JOB_BAKE_SH = 1 };
