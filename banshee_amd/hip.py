@@ -447,7 +447,8 @@ class Guide:
         return loss.value if want_loss else None
 
     def train(self, records: DeviceArray, n: int, steps: int, batch: int, want_loss=False):
-        """`steps` Adam steps on consecutive batches of the first n device records."""
+        """`steps` Adam steps on batches of up to `batch` of the first n device records: step s
+        takes every stride-th record from offset s mod stride, stride = max(1, n // batch)."""
         loss = ctypes.c_float()
         _check(lib().bsh_guide_train(self.h, records.ptr, c_size_t(n), c_int(steps), c_size_t(batch),
                                      ctypes.byref(loss) if want_loss else None, c_void_p(None)), "bsh_guide_train")

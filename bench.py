@@ -186,7 +186,7 @@ def measured_traffic(width: int):
     """HBM bytes per trace launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
     of this workload (calibrated against a copy kernel), or None."""
     import re
-    path = os.path.join(ROOT, "profiles", "k_trace_queue_terrain3M_pmc.md")
+    path = os.path.join(ROOT, "profiles", "k_trace_refill_terrain3M_pmc.md")
     if not os.path.exists(path):
         return None
     m = re.search(rf"traffic_bytes_per_launch\[width={width}\]: (\d+)", open(path).read())
@@ -429,7 +429,7 @@ def main():
                    "setup_s": round(setup_s, 2), "bvh_build_ms": round(info["build_ms"], 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(int(info["width"])),
-                     "traffic_source": "profiles/k_trace_queue_terrain3M_pmc.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
+                     "traffic_source": "profiles/k_trace_refill_terrain3M_pmc.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
                      "kernel": "k_trace_refill" if int(info["width"]) == 4 and args.refill_lanes != -1 else "k_trace_queue",
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                      "bytes_per_ray": round(b_ray, 1), "node_visits_per_ray": round(v_node, 2),
