@@ -449,8 +449,8 @@ bsh_status bsh_render_guided(bsh_scene* s, const bsh_render_params* p, bsh_guide
 }
 
 bsh_status bsh_bake_sh_guided(bsh_scene* s, const float* probes, size_t nprobes, const bsh_probe_params* p,
-                              bsh_guide* guide, float* coeffs, float* radiance, float* records, size_t cap,
-                              size_t* nrecords, void* stream)
+                              bsh_guide* guide, float* coeffs, float* radiance, float* ray_dirs, float* records,
+                              size_t cap, size_t* nrecords, void* stream)
 {
     if (nrecords) *nrecords = 0;
     if (!p || !(p->flags & BSH_RENDER_NO_QUIRKS))
@@ -460,6 +460,8 @@ bsh_status bsh_bake_sh_guided(bsh_scene* s, const float* probes, size_t nprobes,
     bsh_status r = make_probe_job(s, probes, nprobes, p, coeffs, radiance, J);
     if (r != BSH_OK) return r;
     if (nprobes == 0) return BSH_OK;
+    J.ray_dirs = ray_dirs;
+    if (guide) J.flags |= JOB_PROBE_DIR;
     return run_guided_job(s, J, (p->flags & BSH_RENDER_COUNT) != 0, guide, records, cap, nrecords, (hipStream_t)stream);
 }
 

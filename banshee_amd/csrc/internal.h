@@ -73,6 +73,7 @@ struct RenderWork
     DevBuf<float4> qc_o[2], qc_d[2], qa_o[2], qa_d[2];   // closest / any-hit queues
     DevBuf<int> qc_own[2], qa_own[2];
     DevBuf<float4> gin, rec;          // guide network inputs (2 per row), training records (per path)
+    DevBuf<float4> pdir;              // guided probe jobs: per path, probe ray direction and weight
     DevBuf<_Float16> gout;            // guide network outputs (kGUsed per row)
     DevBuf<int> grow;                 // per path: its guide row this pass
     DevBuf<uint32_t> gseg;            // guide rows per block segment this pass
@@ -103,13 +104,16 @@ struct JobDev
     float tmin;
     float* coeffs;
     float* radiance;
-    int flags;             // JOB_NO_QUIRKS, JOB_GUIDED, JOB_RECORD
+    float* ray_dirs;       // probe jobs: optional per-ray (direction, projection weight)
+    int flags;             // JOB_NO_QUIRKS, JOB_GUIDED, JOB_RECORD, JOB_PROBE_DIR
     float bsdf_frac;       // guided: probability of sampling the BSDF
     int guide_bounces;     // guided: vertices 0 .. guide_bounces-1 use the guide (0 = all)
     float glo[3], ginv[3]; // guided: scene box for the network's position input
 };
 
-enum : int { JOB_NO_QUIRKS = 1, JOB_GUIDED = 2, JOB_RECORD = 4 };
+// JOB_PROBE_DIR: a guided probe job whose probe rays are drawn from the uniform/guide mixture
+// (the probe position is the path's vertex -1; its training record takes record slot 0).
+enum : int { JOB_NO_QUIRKS = 1, JOB_GUIDED = 2, JOB_RECORD = 4, JOB_PROBE_DIR = 8 };
 
 struct GuideRun
 {

@@ -89,6 +89,7 @@ struct RenderDev
     float* rec_out;                // appended records, 12 floats each
     uint32_t rec_cap;
     uint32_t* rec_count;
+    float4* pdir;                  // guided probe jobs: per path, probe ray direction and projection weight
 };
 
 __device__ __forceinline__ uint32_t rand_pick_counter(int b) { return (uint32_t)((b - 1) + (b > 4 ? b - 4 : 0)); }
@@ -174,8 +175,35 @@ __device__ __forceinline__ f3 probe_dir(uint32_t base, int j, int g)
     return mk(r * cosf(phi), r * sinf(phi), z);
 }
 
+// Network inputs of the probes [u0, u0+nu) of a guided probe job, rows row0.. : the probe
+// position in the scene box, roughness 0 and a zero normal (no surface has either), so the
+// network tells the free-space vertex of a probe from surface vertices.
+__global__ __launch_bounds__(kBlock) void k_probe_guide_inputs(RenderDev R, uint32_t job, uint32_t u0, uint32_t nu,
+                                                              uint32_t row0)
+{
+    const JobDev& J = R.jobs[job];
+    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= nu) return;
+    const uint32_t p = u0 + t;
+    R.gin[2 * (row0 + t)] = make_float4((J.probes[3 * p] - J.glo[0]) * J.ginv[0],
+                                        (J.probes[3 * p + 1] - J.glo[1]) * J.ginv[1],
+                                        (J.probes[3 * p + 2] - J.glo[2]) * J.ginv[2], 0.f);
+    R.gin[2 * (row0 + t) + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Keyed-stream counters of the probe vertex's guide draws (the per-bounce guide draws use
+// counters 6b .. 6b+5).
+constexpr uint32_t kProbeGuideCtr = 0xFFF000u;
+
+// Probe ray j of probe p. Plain: the stratified uniform direction of cell j, weight 1. With
+// JOB_PROBE_DIR: with probability alpha (the guide's BSDF fraction, here the uniform
+// fraction) that direction, else a sample of the probe's vMF mixture (network row row0 + p - u0);
+// the projection weight is 1 / (4 pi p_mix), p_mix = alpha / (4 pi) + (1 - alpha) q. Summed
+// over the N rays, the cell-restricted uniform densities add up to the uniform density, so
+// (4 pi / N) sum_j w_j L_j Y(w_j) stays an unbiased projection. A probe whose network output is
+// not finite keeps the plain directions and weight 1.
 __global__ __launch_bounds__(kBlock) void k_raygen_probe(RenderDev R, uint32_t job, uint32_t u0, uint32_t nu,
-                                                       uint32_t c0)
+                                                       uint32_t c0, uint32_t row0)
 {
     const JobDev& J = R.jobs[job];
     uint32_t t = blockIdx.x * kBlock + threadIdx.x;
@@ -186,7 +214,38 @@ __global__ __launch_bounds__(kBlock) void k_raygen_probe(RenderDev R, uint32_t j
     uint32_t base = ks_base(J.seed, p * (uint32_t)J.N + (uint32_t)j);
     f3 d = probe_dir(base, j, J.g);
     f3 o = mk(J.probes[3 * p], J.probes[3 * p + 1], J.probes[3 * p + 2]);
+    float pmix = 0.25f / kPi;
+    float w = 1.f;
+    if (J.flags & JOB_PROBE_DIR)
+    {
+        const _Float16* row = R.gout + (size_t)(row0 + (p - u0)) * kGUsed;
+        const Lobes G = guide_lobes(row);
+        if (lobes_finite(G))
+        {
+            const float alpha = J.bsdf_frac;
+            const float usel = ks_float(ks_u32(base, KS_GUIDE, kProbeGuideCtr));
+            if (!(usel < alpha))
+                d = guide_sample(G, row, ks_float(ks_u32(base, KS_GUIDE, kProbeGuideCtr + 1)),
+                                 ks_float(ks_u32(base, KS_GUIDE, kProbeGuideCtr + 2)),
+                                 ks_float(ks_u32(base, KS_GUIDE, kProbeGuideCtr + 3)));
+            pmix = alpha * (0.25f / kPi) + (1.f - alpha) * guide_pdf(G, d);
+            w = 1.f / (4.f * kPi * pmix);
+        }
+        R.pdir[i] = make_float4(d.x, d.y, d.z, w);
+    }
     init_path(R, i, o, d, J.tmin, 1000000.f, base, job);
+    if ((J.flags & JOB_PROBE_DIR) && (J.flags & JOB_RECORD))
+    {
+        // the probe vertex's training record (slot 0): radiance before it is 0 and its
+        // throughput 1, so the incident radiance is the path's whole radiance; normal 0 and
+        // .w = 0 mark a record without a cosine factor
+        float4* rc = R.rec + (size_t)i * kRecStride;
+        rc[0] = make_float4((o.x - J.glo[0]) * J.ginv[0], (o.y - J.glo[1]) * J.ginv[1], (o.z - J.glo[2]) * J.ginv[2], 0.f);
+        rc[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        rc[2] = make_float4(d.x, d.y, d.z, pmix);
+        rc[3] = make_float4(1.f, 1.f, 1.f, 0.f);
+        rc[4] = make_float4(0.f, 0.f, 0.f, 1.f);
+    }
 }
 
 __device__ __forceinline__ uint32_t path_of(const RenderDev& R, uint32_t kind, uint32_t idx)
@@ -602,11 +661,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                 }
                 fl &= ~F_NEE;
             }
+            // record slot of path vertex b: b, or b + 1 on guided probe paths, whose slot 0 is
+            // the probe itself
+            const int roff = (jflags & JOB_PROBE_DIR) ? 1 : 0;
             // training record of the vertex shaded in the previous pass: radiance so far
-            if ((jflags & JOB_RECORD) && b >= 1 && b <= kRecVerts)
+            if ((jflags & JOB_RECORD) && b >= 1 && b - 1 + roff < kRecVerts)
             {
-                float4 v = R.rec[(size_t)i * kRecStride + (b - 1) * 5 + 2];
-                if (v.w > 0.f) R.rec[(size_t)i * kRecStride + (b - 1) * 5 + 4] = st3(rad, 1.f);
+                float4 v = R.rec[(size_t)i * kRecStride + (b - 1 + roff) * 5 + 2];
+                if (v.w > 0.f) R.rec[(size_t)i * kRecStride + (b - 1 + roff) * 5 + 4] = st3(rad, 1.f);
             }
             (void)had_nee;
             // ---- 2. shade the extension hit (GiTracer bounce loop body) ----
@@ -785,12 +847,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                             {
                                 f3 nd = normalize(wi);
                                 T = T * (f * (fabsf(dot(h.n, wi)) / pdf));
-                                if ((jflags & JOB_RECORD) && b < kRecVerts)
+                                if ((jflags & JOB_RECORD) && b + roff < kRecVerts)
                                 {
                                     // training record: network input, sampled direction, mixture pdf,
                                     // throughput after this vertex
                                     f3 nn = dot(h.n, wo) < 0.f ? -h.n : h.n;
-                                    float4* rc = R.rec + (size_t)i * kRecStride + b * 5;
+                                    float4* rc = R.rec + (size_t)i * kRecStride + (b + roff) * 5;
                                     rc[0] = make_float4((h.p.x - J.glo[0]) * J.ginv[0], (h.p.y - J.glo[1]) * J.ginv[1],
                                                         (h.p.z - J.glo[2]) * J.ginv[2], m.type == 1 ? m.alpha : 1.f);
                                     rc[1] = make_float4(nn.x, nn.y, nn.z, 1.f);
@@ -851,7 +913,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                             // (the whole integrand of a diffuse vertex; the network sees no view
                             // direction, so it cannot learn a glossy lobe)
                             float4 nn = rc[1];
-                            float cs = fabsf(nn.x * d.x + nn.y * d.y + nn.z * d.z);
+                            float cs = nn.w > 0.f ? fabsf(nn.x * d.x + nn.y * d.y + nn.z * d.z) : 1.f;
                             // the record's weight goes in the free .w of its throughput; the
                             // chunk's compaction pass emits the records with a weight > 0
                             reinterpret_cast<float*>(R.rec + (size_t)i * kRecStride + k * 5 + 3)[3] = lum * cs / d.w;
@@ -1083,18 +1145,32 @@ __global__ __launch_bounds__(kBlock) void k_project_sh(RenderDev R, uint32_t job
     for (int k = 0; k < 3 * NT; ++k) acc[k] = 0.f;
     for (int j = lane; j < N; j += 64)
     {
-        float4 L = R.out[c0 + wave * (uint32_t)N + (uint32_t)j];
-        uint32_t base = ks_base(J.seed, p * (uint32_t)N + (uint32_t)j);
-        f3 d = probe_dir(base, j, J.g);
+        const uint32_t i = c0 + wave * (uint32_t)N + (uint32_t)j;
+        const float4 L = R.out[i];
+        f3 d;
+        float w = 1.f;
+        if (J.flags & JOB_PROBE_DIR)
+        {
+            const float4 dw = R.pdir[i];
+            d = mk(dw.x, dw.y, dw.z);
+            w = dw.w;
+        }
+        else
+        {
+            d = probe_dir(ks_base(J.seed, p * (uint32_t)N + (uint32_t)j), j, J.g);
+        }
         float y[kShMaxTerms];
         sh_evaluate(d, LMAX, y);
+        const float lx = L.x * w, ly = L.y * w, lz = L.z * w;
 #pragma unroll
         for (int k = 0; k < NT; ++k)
         {
-            acc[3 * k + 0] += L.x * y[k];
-            acc[3 * k + 1] += L.y * y[k];
-            acc[3 * k + 2] += L.z * y[k];
+            acc[3 * k + 0] += lx * y[k];
+            acc[3 * k + 1] += ly * y[k];
+            acc[3 * k + 2] += lz * y[k];
         }
+        if (J.ray_dirs)
+            reinterpret_cast<float4*>(J.ray_dirs)[(size_t)p * N + j] = make_float4(d.x, d.y, d.z, w);
         if (J.radiance)
         {
             size_t o = ((size_t)p * N + j) * 3;
@@ -1238,12 +1314,33 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
             wide = wide && hjobs[sg.job].s.width == 4;
         }
         hipLaunchKernelGGL(k_zero, grid_for(4 * launches), dim3(kBlock), 0, st, w.counters.p, 4 * launches);
-        for (const Segment& sg : ch.segs)
+        // guided probe jobs: the network rows of their probes (vertex -1 of every probe path)
+        // come first, one row per probe, so the probe rays can be drawn from the mixture
+        std::vector<uint32_t> seg_row(ch.segs.size(), 0u);
+        uint32_t prow = 0;
+        for (size_t k = 0; k < ch.segs.size(); ++k)
         {
+            const Segment& sg = ch.segs[k];
+            if (hjobs[sg.job].type != 1 || !(hjobs[sg.job].flags & JOB_PROBE_DIR)) continue;
+            seg_row[k] = prow;
+            hipLaunchKernelGGL(k_probe_guide_inputs, grid_for(sg.nu), dim3(kBlock), 0, st, R, sg.job, sg.u0, sg.nu, prow);
+            prow += sg.nu;
+        }
+        if (prow)
+        {
+            if (!gr || !gr->guide) return fail(BSH_ERR_STATE, "guided probe job without a guide");
+            bsh_status gs = guide_forward_rows(gr->guide, R.gin, nullptr, prow, w.gout.p, st);
+            if (gs != BSH_OK) return gs;
+        }
+        for (size_t k = 0; k < ch.segs.size(); ++k)
+        {
+            const Segment& sg = ch.segs[k];
             const JobDev& J = hjobs[sg.job];
             size_t n = (size_t)sg.nu * (J.type == 0 ? J.s_count : J.N);
             if (J.type == 0) hipLaunchKernelGGL(k_raygen, grid_for(n), dim3(kBlock), 0, st, R, sg.job, sg.u0, sg.nu, sg.c0);
-            else hipLaunchKernelGGL(k_raygen_probe, grid_for(n), dim3(kBlock), 0, st, R, sg.job, sg.u0, sg.nu, sg.c0);
+            else
+                hipLaunchKernelGGL(k_raygen_probe, grid_for(n), dim3(kBlock), 0, st, R, sg.job, sg.u0, sg.nu, sg.c0,
+                                   seg_row[k]);
         }
         hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, st, w.counters.p, w.stats.p, ch.P);
         dim3 pg = grid_for(ch.P);
@@ -1395,6 +1492,7 @@ bsh_status make_probe_job(bsh_scene* s, const float* probes, size_t nprobes, con
     J.tmin = p->tmin;
     J.coeffs = coeffs;
     J.radiance = radiance;
+    J.ray_dirs = nullptr;
     J.flags = (p->flags & BSH_RENDER_NO_QUIRKS) ? JOB_NO_QUIRKS : 0;
     set_guide_box(s, J);
     return BSH_OK;
@@ -1448,6 +1546,9 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         BSH_HIP(w.grow.alloc(P));
         BSH_HIP(w.gseg.alloc(Pb / kBlock));
     }
+    bool probe_dir = false;
+    for (const JobDev& J : hjobs) probe_dir = probe_dir || (J.flags & JOB_PROBE_DIR) != 0;
+    if (probe_dir) BSH_HIP(w.pdir.alloc(P));
     if (record)
     {
         BSH_HIP(w.rec.alloc((size_t)kRecStride * P));
@@ -1479,6 +1580,7 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     R.gout = guided ? w.gout.p : nullptr;
     R.grow = guided ? w.grow.p : nullptr;
     R.gseg = guided ? w.gseg.p : nullptr;
+    R.pdir = probe_dir ? w.pdir.p : nullptr;
     R.rec = record ? w.rec.p : nullptr;
     R.rec_out = record ? gr->records : nullptr;
     R.rec_cap = record ? (uint32_t)std::min<size_t>(gr->cap, 0xffffffffu) : 0u;

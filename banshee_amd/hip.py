@@ -626,10 +626,12 @@ class Scene:
 
     def bake_sh_guided(self, probes, rays_per_probe, lmax, max_depth, seed, guide: "Guide | None" = None,
                        tmin=0.0, chunk_probes=0, radiance=False, coeffs: DeviceArray | None = None,
-                       records: DeviceArray | None = None, download=True):
-        """Quirk-free probe bake whose continuation samples the BSDF/guide mixture. Returns
-        (coeffs, per-ray radiance or None, nrecords) when download, else nrecords. probes may
-        be a host array or a DeviceArray of shape [n, 3]."""
+                       records: DeviceArray | None = None, download=True, ray_dirs=False):
+        """Quirk-free probe bake guided at every path vertex, the probe included (its rays are
+        drawn from the uniform/guide mixture and weighted). Returns (coeffs, per-ray radiance or
+        None, nrecords) when download, else nrecords; with ray_dirs, (coeffs, radiance, dirs
+        [n, N, 4] = direction and projection weight, nrecords). probes may be a host array or a
+        DeviceArray of shape [n, 3]."""
         if isinstance(probes, DeviceArray):
             dp = probes
             n = probes.shape[0]
@@ -641,6 +643,7 @@ class Scene:
         if coeffs is None:
             coeffs = self.ctx.alloc((n, nt, 3), np.float32)
         rad = self.ctx.alloc((n, rays_per_probe, 3), np.float32) if radiance else None
+        dirs = self.ctx.alloc((n, rays_per_probe, 4), np.float32) if ray_dirs else None
         p = ProbeParams()
         p.rays_per_probe, p.lmax, p.max_depth, p.seed = rays_per_probe, lmax, max_depth, seed
         p.chunk_probes, p.tmin = chunk_probes, tmin
@@ -649,11 +652,14 @@ class Scene:
         cap = records.shape[0] if records is not None else 0
         _check(lib().bsh_bake_sh_guided(self.h, dp.ptr, c_size_t(n), ctypes.byref(p), guide.h if guide else None,
                                         coeffs.ptr, rad.ptr if rad else c_void_p(None),
+                                        dirs.ptr if dirs else c_void_p(None),
                                         records.ptr if records is not None else None, c_size_t(cap),
                                         ctypes.byref(nrec), c_void_p(None)), "bsh_bake_sh_guided")
         if not download:
             return nrec.value
         self.ctx.sync()
+        if ray_dirs:
+            return coeffs.numpy(), (rad.numpy() if rad else None), dirs.numpy(), nrec.value
         return coeffs.numpy(), (rad.numpy() if rad else None), nrec.value
 
     def render_guided(self, width, height, img_grid, max_depth, seed, guide: "Guide | None" = None, crop=None,

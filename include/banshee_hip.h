@@ -337,15 +337,20 @@ bsh_status bsh_guide_train(bsh_guide* g, const float* records, size_t n, int ste
 bsh_status bsh_render_guided(bsh_scene* s, const bsh_render_params* p, bsh_guide* guide, float* accum,
                              float* records, size_t cap, size_t* nrecords, void* stream);
 /* Probe baking like bsh_bake_sh with the quirk-free estimator (p->flags must include
-   BSH_RENDER_NO_QUIRKS) and the guided continuation of bsh_render_guided at every surface
-   vertex of a probe path. Each probe ray stays a stratified uniform-sphere direction, so the
-   projection c_i = (4 pi / N) sum_j L(w_j) Y_i(w_j) is unchanged and stays unbiased; the
-   guide only lowers the variance of each L(w_j). guide may be NULL (the plain quirk-free
-   bake); records, cap and nrecords as in bsh_render_guided, the first record vertex being
-   the probe ray's hit. */
+   BSH_RENDER_NO_QUIRKS), guided at every vertex of a probe path. The probe itself is vertex
+   -1: the network is queried at the probe position (roughness 0, normal 0), and probe ray j
+   is, with probability bsdf_fraction, the stratified uniform direction of cell j, else a
+   sample of the probe's mixture; its projection weight is w_j = 1 / (4 pi p_mix(w_j)),
+   p_mix = bsdf_fraction / (4 pi) + (1 - bsdf_fraction) q, and c_i = (4 pi / N)
+   sum_j w_j L(w_j) Y_i(w_j) stays unbiased. Surface vertices continue as in
+   bsh_render_guided. guide may be NULL (the plain quirk-free bake, w_j = 1). radiance as in
+   bsh_bake_sh (L(w_j), unweighted); ray_dirs: optional device float[nprobes*N*4] receiving
+   every probe ray's (direction, w_j). records, cap and nrecords as in bsh_render_guided; on
+   a guided bake record slot 0 of a path is the probe (normal 0, the record's 8th float 0,
+   weight lum(L) / p_mix: no cosine factor), then its first three surface vertices. */
 bsh_status bsh_bake_sh_guided(bsh_scene* s, const float* probes, size_t nprobes, const bsh_probe_params* p,
-                              bsh_guide* guide, float* coeffs, float* radiance, float* records, size_t cap,
-                              size_t* nrecords, void* stream);
+                              bsh_guide* guide, float* coeffs, float* radiance, float* ray_dirs, float* records,
+                              size_t cap, size_t* nrecords, void* stream);
 
 /* ---------------- diagnostics (exercise device code paths in isolation) ---------------- */
 /* One v_mfma_f32_16x16x16_f16 tile: d = a b for 16x16 row-major FP16 a, b (bit patterns). */
