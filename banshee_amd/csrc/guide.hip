@@ -21,7 +21,18 @@
 namespace bsh {
 
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 hp2 __attribute__((ext_vector_type(2)));
 typedef float f4v __attribute__((ext_vector_type(4)));
+
+// ReLU of an FP32 accumulator fragment, rounded to FP16: two paired conversions, then two
+// packed FP16 maxima (max(round(x), 0) = round(max(x, 0)) for every x).
+__device__ __forceinline__ h4 relu_h4(f4v a)
+{
+    const hp2 z = {(_Float16)0.f, (_Float16)0.f};
+    const hp2 lo = __builtin_elementwise_max((hp2){(_Float16)a[0], (_Float16)a[1]}, z);
+    const hp2 hi = __builtin_elementwise_max((hp2){(_Float16)a[2], (_Float16)a[3]}, z);
+    return (h4){lo[0], lo[1], hi[0], hi[1]};
+}
 
 __device__ __forceinline__ f4v mfma16(h4 a, h4 b, f4v c) { return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0); }
 
@@ -134,9 +145,11 @@ __device__ __forceinline__ void dense_t(const h4 (&x)[kT][K / 16], const _Float1
 #pragma unroll
     for (int mt = 0; mt < N / 16; ++mt)
     {
+        // the accumulators start at the bias (lane rows 16 mt + 4q + i), so no add follows
+        const float4 bb = *reinterpret_cast<const float4*>(&b[mt * 16 + 4 * q]);
         f4v acc[kT];
 #pragma unroll
-        for (int t = 0; t < kT; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < kT; ++t) acc[t] = f4v{bb.x, bb.y, bb.z, bb.w};
 #pragma unroll
         for (int ks = 0; ks < K / 16; ++ks)
         {
@@ -144,17 +157,8 @@ __device__ __forceinline__ void dense_t(const h4 (&x)[kT][K / 16], const _Float1
 #pragma unroll
             for (int t = 0; t < kT; ++t) acc[t] = mfma16(w, x[t][ks], acc[t]);
         }
-        float4 bb = *reinterpret_cast<const float4*>(&b[mt * 16 + 4 * q]);
 #pragma unroll
-        for (int t = 0; t < kT; ++t)
-        {
-            h4 o;
-            o[0] = (_Float16)fmaxf(acc[t][0] + bb.x, 0.f);
-            o[1] = (_Float16)fmaxf(acc[t][1] + bb.y, 0.f);
-            o[2] = (_Float16)fmaxf(acc[t][2] + bb.z, 0.f);
-            o[3] = (_Float16)fmaxf(acc[t][3] + bb.w, 0.f);
-            y[t][mt] = o;
-        }
+        for (int t = 0; t < kT; ++t) y[t][mt] = relu_h4(acc[t]);
     }
 }
 
@@ -244,9 +248,10 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet g, co
 #pragma unroll
         for (int mt = 0; mt < kGOut / 16; ++mt)
         {
+            const float4 bb = *reinterpret_cast<const float4*>(&lb[4 * kGHid + mt * 16 + 4 * q]);
             f4v acc[kFwdTiles];
 #pragma unroll
-            for (int t = 0; t < kFwdTiles; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+            for (int t = 0; t < kFwdTiles; ++t) acc[t] = f4v{bb.x, bb.y, bb.z, bb.w};
 #pragma unroll
             for (int ks = 0; ks < kGHid / 16; ++ks)
             {
@@ -254,7 +259,6 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet g, co
 #pragma unroll
                 for (int t = 0; t < kFwdTiles; ++t) acc[t] = mfma16(w, h2[t][ks], acc[t]);
             }
-            float4 bb = *reinterpret_cast<const float4*>(&lb[4 * kGHid + mt * 16 + 4 * q]);
 #pragma unroll
             for (int t = 0; t < kFwdTiles; ++t)
             {
@@ -263,15 +267,14 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet g, co
                 {
                     if (mt * 16 + 4 * q < kGUsed)
                     {
-                        h4 o = {(_Float16)(acc[t][0] + bb.x), (_Float16)(acc[t][1] + bb.y), (_Float16)(acc[t][2] + bb.z),
-                                (_Float16)(acc[t][3] + bb.w)};
+                        h4 o = {(_Float16)acc[t][0], (_Float16)acc[t][1], (_Float16)acc[t][2], (_Float16)acc[t][3]};
                         *reinterpret_cast<h4*>(&static_cast<_Float16*>(outp)[(size_t)row[t] * kGUsed + mt * 16 + 4 * q]) = o;
                     }
                 }
                 else
                 {
                     *reinterpret_cast<float4*>(&static_cast<float*>(outp)[(size_t)row[t] * kGOut + mt * 16 + 4 * q]) =
-                        make_float4(acc[t][0] + bb.x, acc[t][1] + bb.y, acc[t][2] + bb.z, acc[t][3] + bb.w);
+                        make_float4(acc[t][0], acc[t][1], acc[t][2], acc[t][3]);
                 }
             }
         }
