@@ -2,12 +2,15 @@
 // (inference, training forward, backward, weight gradients, Adam). See guide_dev.h for the
 // network and the vMF mixture. No reference counterpart.
 //
-// Matrix tiles use v_mfma_f32_16x16x16_f16 with the CDNA operand layout
-//   A (16x16, MxK): lane l holds A[l%16][4*(l/16) + i], i = 0..3
-//   B (16x16, KxN): lane l holds B[4*(l/16) + i][l%16]
-//   D (16x16):      lane l holds D[4*(l/16) + i][l%16]
-// (checked on the device by bsh_diag_mfma). Weights live in LDS as FP16 rows padded to
-// K+4 halves, so the 16 lanes of a fragment read hit distinct banks and 8-byte aligned.
+// Matrix tiles use the CDNA operand layouts
+//   v_mfma_f32_16x16x16_f16 (backward, weight gradients):
+//     A (16x16, MxK): lane l holds A[l%16][4*(l/16) + i], i = 0..3
+//     B (16x16, KxN): lane l holds B[4*(l/16) + i][l%16]
+//     D (16x16):      lane l holds D[4*(l/16) + i][l%16]
+//   v_mfma_f32_16x16x32_f16 (forward; twice the work per issue on gfx950):
+//     A (16x32): lane l holds A[l%16][8*(l/16) + i], i = 0..7; B (32x16): B[8*(l/16) + i][l%16];
+//     D as above
+// (the first checked on the device by bsh_diag_mfma, both by the forward tests).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,10 +38,24 @@ __device__ __forceinline__ h4 relu_h4(f4v a)
 }
 
 __device__ __forceinline__ f4v mfma16(h4 a, h4 b, f4v c) { return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0); }
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f4v mfma32(h8 a, h8 b, f4v c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ h8 cat8(h4 lo, h4 hi) { return (h8){lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]}; }
 
 // ---- forward weight layout in FP16 (out-major rows Wt[n][k]) ----
-constexpr int kPad = 4;
+// In LDS each row is padded to a 16-byte multiple, and within every 32-column block the
+// input features are permuted (perm32) so that the 8 halves lane group g reads for a 16x32
+// A fragment, columns 32 kb + 8g .. 8g+7, are the features 32 kb + 4g + 0..3 and
+// 32 kb + 16 + 4g + 0..3: exactly what the lane holds of the two 16-feature output fragments
+// 2 kb and 2 kb + 1 of the previous layer (or of bands 2 kb, 2 kb + 1 of the encoding), so
+// the B operand is their concatenation.
+constexpr int kPad = 8;
 constexpr int kL1S = kGIn + kPad, kLHS = kGHid + kPad;             // LDS row strides (halves)
+static_assert(kGIn % 32 == 0 && kGHid % 32 == 0, "16x32 tiles");
+__host__ __device__ constexpr int perm32(int k)
+{
+    return (k & ~31) | (((k >> 2) & 3) << 3) | (((k >> 4) & 1) << 2) | (k & 3);
+}
 constexpr int kF1 = 0;                                             // 64 rows x 96
 constexpr int kF2 = kF1 + kGHid * kGIn;                            // 64 x 64
 constexpr int kF3 = kF2 + kGHid * kGHid;
@@ -65,19 +82,22 @@ struct GuideNet
 
 __device__ __forceinline__ void load_fwd_weights(const GuideNet& g, _Float16* lw, float* lb)
 {
-    // unpadded global rows -> padded LDS rows
-    for (int t = threadIdx.x; t < kFTotal; t += blockDim.x)
+    // unpadded global rows -> padded, column-permuted LDS rows, 4 halves at a time (perm32
+    // keeps runs of 4 consecutive features together)
+    const h4* src = reinterpret_cast<const h4*>(g.wf);
+    for (int u = threadIdx.x; u < kFTotal / 4; u += blockDim.x)
     {
+        const int t = 4 * u;
         int dst;
-        if (t < kF2) { int n = t / kGIn, k = t % kGIn; dst = kLF1 + n * kL1S + k; }
+        if (t < kF2) { int n = t / kGIn, k = t % kGIn; dst = kLF1 + n * kL1S + perm32(k); }
         else if (t < kF5)
         {
-            int u = (t - kF2), l = u / (kGHid * kGHid), r = u % (kGHid * kGHid);
+            int v = (t - kF2), l = v / (kGHid * kGHid), r = v % (kGHid * kGHid);
             int n = r / kGHid, k = r % kGHid;
-            dst = kLF2 + l * kGHid * kLHS + n * kLHS + k;
+            dst = kLF2 + l * kGHid * kLHS + n * kLHS + perm32(k);
         }
-        else { int u = t - kF5; int n = u / kGHid, k = u % kGHid; dst = kLF5 + n * kLHS + k; }
-        lw[dst] = g.wf[t];
+        else { int v = t - kF5; int n = v / kGHid, k = v % kGHid; dst = kLF5 + n * kLHS + perm32(k); }
+        *reinterpret_cast<h4*>(&lw[dst]) = src[u];
     }
     for (int t = threadIdx.x; t < kBiasTotal; t += blockDim.x) lb[t] = g.bias[t];
 }
@@ -133,11 +153,11 @@ __device__ __forceinline__ void encode_row(float4 a, float4 c, int q, bool live,
         for (int k = 0; k < kGIn / 16; ++k) x[k] = h4{0, 0, 0, 0};
 }
 
-// One dense layer for kT 16-row tiles on the transposed product Y^T = W^T X^T: the weights are
-// the A operand (LDS, out-major rows), the activations the B operand. Each weight fragment is
-// read from LDS once and used for all kT tiles. The MFMA result fragment (lane l holds
-// Y^T[16 mt + 4(l/16) + i][row l%16]) is exactly the B fragment of the next layer's k-step mt,
-// so activations never leave the registers.
+// One dense layer for kT 16-row tiles on the transposed product Y^T = W^T X^T with 16x32 tiles:
+// the weights are the A operand (LDS, out-major rows, columns permuted by perm32), the
+// activations the B operand. Each weight fragment is read from LDS once and used for all kT
+// tiles. The MFMA result fragment (lane l holds Y^T[16 mt + 4(l/16) + i][row l%16]) is half of
+// the next layer's B fragment for k-block mt/2, so activations never leave the registers.
 template <int K, int N, int kT>
 __device__ __forceinline__ void dense_t(const h4 (&x)[kT][K / 16], const _Float16* Wt, int ws, const float* b,
                                         h4 (&y)[kT][N / 16], int r, int q)
@@ -151,11 +171,11 @@ __device__ __forceinline__ void dense_t(const h4 (&x)[kT][K / 16], const _Float1
 #pragma unroll
         for (int t = 0; t < kT; ++t) acc[t] = f4v{bb.x, bb.y, bb.z, bb.w};
 #pragma unroll
-        for (int ks = 0; ks < K / 16; ++ks)
+        for (int kb = 0; kb < K / 32; ++kb)
         {
-            h4 w = *reinterpret_cast<const h4*>(&Wt[(mt * 16 + r) * ws + ks * 16 + 4 * q]);
+            const h8 w = *reinterpret_cast<const h8*>(&Wt[(mt * 16 + r) * ws + kb * 32 + 8 * q]);
 #pragma unroll
-            for (int t = 0; t < kT; ++t) acc[t] = mfma16(w, x[t][ks], acc[t]);
+            for (int t = 0; t < kT; ++t) acc[t] = mfma32(w, cat8(x[t][2 * kb], x[t][2 * kb + 1]), acc[t]);
         }
 #pragma unroll
         for (int t = 0; t < kT; ++t) y[t][mt] = relu_h4(acc[t]);
@@ -187,7 +207,7 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet g, co
 {
     constexpr int kGroup = 16 * kFwdTiles;
     static_assert(kGuideSeg % kGroup == 0, "tile groups must not straddle segments");
-    __shared__ _Float16 lw[kLFTotal];
+    __shared__ __align__(16) _Float16 lw[kLFTotal];
     __shared__ __align__(16) float lb[kBiasTotal];
     load_fwd_weights(g, lw, lb);
     __syncthreads();
@@ -253,11 +273,11 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet g, co
 #pragma unroll
             for (int t = 0; t < kFwdTiles; ++t) acc[t] = f4v{bb.x, bb.y, bb.z, bb.w};
 #pragma unroll
-            for (int ks = 0; ks < kGHid / 16; ++ks)
+            for (int kb = 0; kb < kGHid / 32; ++kb)
             {
-                h4 w = *reinterpret_cast<const h4*>(&W[kLF5 + (mt * 16 + r) * kLHS + ks * 16 + 4 * q]);
+                const h8 w = *reinterpret_cast<const h8*>(&W[kLF5 + (mt * 16 + r) * kLHS + kb * 32 + 8 * q]);
 #pragma unroll
-                for (int t = 0; t < kFwdTiles; ++t) acc[t] = mfma16(w, h2[t][ks], acc[t]);
+                for (int t = 0; t < kFwdTiles; ++t) acc[t] = mfma32(w, cat8(h2[t][2 * kb], h2[t][2 * kb + 1]), acc[t]);
             }
 #pragma unroll
             for (int t = 0; t < kFwdTiles; ++t)
