@@ -49,6 +49,8 @@ constexpr int kTraceLds = 32;
 constexpr int kRecVerts = 4, kRecStride = 5 * kRecVerts;
 static_assert(kGuideSeg == kBlock, "guide row segments are the blocks of k_guide_inputs");
 constexpr int kMaxTraceBlocksPerCU = 5;
+// Statistics buffer: 4 counters, then the guided-row partials, kRowStride apart (128 B).
+constexpr int kStatRows = 4, kRowParts = 64, kRowStride = 16, kStatTotal = kStatRows + kRowParts * kRowStride;
 // Idle lanes of a wave that trigger a refill in k_trace_refill (when the tuning leaves it 0).
 // Measured on the 3M-triangle terrain at 1080p: 8 lanes 733 Mrays/s, 16 1058, 32 1314,
 // 48 1381, 56 1393, 64 1353; whole 64-ray batches per wave (k_trace_queue) 1065.
@@ -79,7 +81,7 @@ struct RenderDev
     float4 *qc_o[2], *qc_d[2], *qa_o[2], *qa_d[2];
     int *qc_own[2], *qa_own[2];
     uint32_t* counters;            // per launch L: closest count, closest work, any count, any work
-    unsigned long long* stats;     // closest, any, node visits, triangle tests
+    unsigned long long* stats;     // closest, any, node visits, triangle tests, then kRowParts guided-row partials
     uint32_t* stk_ovf;             // traversal stack entries beyond kTraceLds, [entry][trace thread]
     float4* gin;                   // guide inputs, 2 per row, rows compacted over the guided vertices
     const _Float16* gout;          // guide outputs, kGUsed per row (FP16)
@@ -256,7 +258,8 @@ __device__ __forceinline__ uint32_t path_of(const RenderDev& R, uint32_t kind, u
 // Network inputs of every path whose extension ray hit something at a guided vertex of pass b:
 // position in the scene box, roughness (GGX alpha, 1 otherwise), shading normal facing the
 // viewer. Each block packs its rows at the front of its own kBlock-row segment (a block-wide
-// prefix count, no global atomics) and stores how many it wrote; each path records its row.
+// prefix count) and stores how many it wrote; each path records its row. The block's count
+// is also added to the guided-row statistic (one integer atomic per block).
 __global__ __launch_bounds__(kBlock) void k_guide_inputs(RenderDev R, int b)
 {
     __shared__ uint32_t wsum[kBlock / 64];
@@ -295,7 +298,13 @@ __global__ __launch_bounds__(kBlock) void k_guide_inputs(RenderDev R, int b)
         if (w < wave) base += wsum[w];
         total += wsum[w];
     }
-    if (threadIdx.x == 0) R.gseg[blockIdx.x] = total;
+    if (threadIdx.x == 0)
+    {
+        R.gseg[blockIdx.x] = total;
+        // guided rows: an exact integer sum over kRowParts partial counters on separate cache
+        // lines (one shared counter serialises the blocks' atomics)
+        if (total) atomicAdd(&R.stats[kStatRows + kRowStride * (blockIdx.x % kRowParts)], (unsigned long long)total);
+    }
     if (i >= R.P) return;
     int row = -1;
     if (want)
@@ -551,22 +560,6 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
         atomicAdd(&R.stats[2], (unsigned long long)vis_nodes);
         atomicAdd(&R.stats[3], (unsigned long long)vis_tris);
     }
-}
-
-// Adds the guided rows of this pass (the per-block segment counts) to stats[4], in a fixed order.
-__global__ __launch_bounds__(kBlock) void k_sum_rows(const uint32_t* seg, uint32_t nseg, unsigned long long* total)
-{
-    __shared__ unsigned long long red[kBlock];
-    unsigned long long s = 0;
-    for (uint32_t k = threadIdx.x; k < nseg; k += kBlock) s += seg[k];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int st = kBlock / 2; st > 0; st >>= 1)
-    {
-        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) total[0] += red[0];
 }
 
 // Inclusive prefix sum over the 64 lanes of a wave.
@@ -1221,7 +1214,7 @@ static bsh_status ensure_work(bsh_context* c, size_t P, int K, int nlmax)
         BSH_HIP(w.qa_d[k].alloc(qa));
         BSH_HIP(w.qa_own[k].alloc(qa));
     }
-    BSH_HIP(w.stats.alloc(5));
+    BSH_HIP(w.stats.alloc(kStatTotal));
     return BSH_OK;
 }
 
@@ -1363,7 +1356,6 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
             if (infer)
             {
                 hipLaunchKernelGGL(k_guide_inputs, pg, dim3(kBlock), 0, st, R, b);
-                hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(kBlock), 0, st, w.gseg.p, pg.x, w.stats.p + 4);
             }
             if (T) BSH_HIP(hipEventRecord(T->next(), st));
             if (infer)
@@ -1587,7 +1579,7 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     R.rec_count = record ? w.rec_count.p : nullptr;
     if (record) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, w.rec_count.p, 1);
 
-    hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, st, (uint32_t*)w.stats.p, 10);
+    hipLaunchKernelGGL(k_zero, grid_for(2 * kStatTotal), dim3(kBlock), 0, st, (uint32_t*)w.stats.p, 2 * kStatTotal);
     bsh_render_stats& S = c->stats;
     S = bsh_render_stats{};
     if (use_graph)
@@ -1674,9 +1666,10 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         BSH_HIP(hipEventElapsedTime(&tot, T.evs[0], T.evs[T.ei - 1]));
         S.total_ms = tot;
     }
-    unsigned long long st2[5];
+    unsigned long long st2[kStatTotal];
     BSH_HIP(hipMemcpy(st2, w.stats.p, sizeof(st2), hipMemcpyDeviceToHost));
-    S.guide_rows = st2[4];
+    S.guide_rows = 0;
+    for (int k = 0; k < kRowParts; ++k) S.guide_rows += st2[kStatRows + kRowStride * k];
     S.rays_closest = st2[0];
     S.rays_any = st2[1];
     S.node_visits = count ? st2[2] : 0;
