@@ -79,7 +79,7 @@ def cpu_baseline(sc_full_args) -> dict | None:
 # Guided vs plain (quirk-free) path tracing: the occluded room, 500k triangles, 1024x1024, depth 8.
 G_RES, G_DEPTH, G_TARGET = 1024, 8, 0.02
 G_PASS_GRID, G_CHECK = 2, 1               # 4 spp per pass; RMSE estimated after every pass (outside the clock)
-G_ALPHA, G_WARM, G_LR, G_BATCH = 0.5, 6, 5e-3, 1 << 18
+G_ALPHA, G_WARM, G_LR, G_BATCH = 0.5, 4, 5e-3, 1 << 18
 G_STEPS, G_STEPS_LATE, G_LATE = 8, 0, 10   # Adam steps per pass: G_STEPS, from pass G_LATE on G_STEPS_LATE
 # (passes that train no step write no records)
 G_BOUNCES = 0                              # guided path vertices (0 = all)
