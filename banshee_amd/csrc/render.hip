@@ -92,6 +92,8 @@ struct RenderDev
     uint32_t rec_cap;
     uint32_t* rec_count;
     float4* pdir;                  // guided probe jobs: per path, probe ray direction and projection weight
+    const float4* nodes1;          // when every job of the run traces the same scene: its nodes and
+    const float4* tris1;           // triangles (else null: each ray looks its scene up by its job)
 };
 
 __device__ __forceinline__ uint32_t rand_pick_counter(int b) { return (uint32_t)((b - 1) + (b > 4 ? b - 4 : 0)); }
@@ -429,9 +431,17 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
                     const float4 a = any ? R.qa_o[buf][k] : R.qc_o[buf][k];
                     const float4 b = any ? R.qa_d[buf][k] : R.qc_d[buf][k];
                     own = (uint32_t)(any ? R.qa_own[buf][k] : R.qc_own[buf][k]);
-                    const uint32_t job = R.meta[path_of(R, own >> 29, own & ((1u << 29) - 1u))].y >> 24;
-                    nodes = R.jobs[job].s.nodes;
-                    tris = R.jobs[job].s.tris;
+                    if (R.nodes1)
+                    {
+                        nodes = R.nodes1;
+                        tris = R.tris1;
+                    }
+                    else
+                    {
+                        const uint32_t job = R.meta[path_of(R, own >> 29, own & ((1u << 29) - 1u))].y >> 24;
+                        nodes = R.jobs[job].s.nodes;
+                        tris = R.jobs[job].s.tris;
+                    }
                     o = mk(a.x, a.y, a.z);
                     d = mk(b.x, b.y, b.z);
                     const float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
@@ -1573,6 +1583,10 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     R.grow = guided ? w.grow.p : nullptr;
     R.gseg = guided ? w.gseg.p : nullptr;
     R.pdir = probe_dir ? w.pdir.p : nullptr;
+    R.nodes1 = hjobs.empty() ? nullptr : hjobs[0].s.nodes;
+    R.tris1 = hjobs.empty() ? nullptr : hjobs[0].s.tris;
+    for (const JobDev& J : hjobs)
+        if (J.s.nodes != R.nodes1 || J.s.tris != R.tris1) R.nodes1 = R.tris1 = nullptr;
     R.rec = record ? w.rec.p : nullptr;
     R.rec_out = record ? gr->records : nullptr;
     R.rec_cap = record ? (uint32_t)std::min<size_t>(gr->cap, 0xffffffffu) : 0u;
