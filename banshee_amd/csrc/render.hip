@@ -938,55 +938,73 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
         }
     }
     // ---- queue append: one atomic per wave and queue ----
-    uint32_t ncl = (emit_ext ? 1u : 0u), nany = 0;
-    for (int k = 0; k < K; ++k)
-        if (emit_mask & (1u << k)) { if (k & 1) ncl++; else nany++; }
-    uint32_t packed = ncl | (nany << 16);
-    uint32_t incl = wave_incl_scan(packed);
-    uint32_t total = __shfl(incl, 63);
-    uint32_t excl = incl - packed;
+    // The trace kernel claims rays in queue order, so the wave writes them grouped: its
+    // extension rays by direction octant, then its NEE rays slot by slot (the same light for
+    // every lane of a run), instead of lane by lane. Rays claimed together then share an
+    // origin region and a direction class. Hits are written per owner, so the order changes
+    // no result.
     const int lane = threadIdx.x & 63;
-    uint32_t cbase = 0, abase = 0;
-    if (lane == 63)
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t ncl = (uint32_t)__popcll(__ballot(emit_ext)), nany = 0;
+    for (int k = 0; k < K; ++k)
     {
-        uint32_t tc = total & 0xffffu, ta = total >> 16;
-        if (tc) cbase = atomicAdd(&R.counters[4 * Lout + 0], tc);
-        if (ta) abase = atomicAdd(&R.counters[4 * Lout + 2], ta);
-        if (tc) atomicAdd(&R.stats[0], (unsigned long long)tc);
-        if (ta) atomicAdd(&R.stats[1], (unsigned long long)ta);
+        const uint32_t c = (uint32_t)__popcll(__ballot((emit_mask >> k) & 1u));
+        if (k & 1) ncl += c;
+        else nany += c;
     }
-    cbase = __shfl(cbase, 63);
-    abase = __shfl(abase, 63);
+    uint32_t cbase = 0, abase = 0;
+    if (lane == 0)
+    {
+        if (ncl) cbase = atomicAdd(&R.counters[4 * Lout + 0], ncl);
+        if (nany) abase = atomicAdd(&R.counters[4 * Lout + 2], nany);
+        if (ncl) atomicAdd(&R.stats[0], (unsigned long long)ncl);
+        if (nany) atomicAdd(&R.stats[1], (unsigned long long)nany);
+    }
+    cbase = __shfl(cbase, 0);
+    abase = __shfl(abase, 0);
     if (ncl | nany)
     {
         const int buf = Lout & 1;
-        uint32_t cp = cbase + (excl & 0xffffu), ap = abase + (excl >> 16);
-        if (emit_ext)
+        uint32_t cofs = 0, aofs = 0;
+        const int oct = (ext_d.x < 0.f ? 1 : 0) | (ext_d.y < 0.f ? 2 : 0) | (ext_d.z < 0.f ? 4 : 0);
+        for (int o = 0; o < 8; ++o)
         {
-            R.qc_o[buf][cp] = st3(org, 0.01f);
-            R.qc_d[buf][cp] = st3(ext_d, 1000000.f);
-            R.qc_own[buf][cp] = (int)((Q_EXT << 29) | i);
-            ++cp;
+            const bool mine = emit_ext && oct == o;
+            const unsigned long long m = __ballot(mine);
+            if (mine)
+            {
+                const uint32_t cp = cbase + cofs + (uint32_t)__popcll(m & below);
+                R.qc_o[buf][cp] = st3(org, 0.01f);
+                R.qc_d[buf][cp] = st3(ext_d, 1000000.f);
+                R.qc_own[buf][cp] = (int)((Q_EXT << 29) | i);
+            }
+            cofs += (uint32_t)__popcll(m);
         }
         for (int k = 0; k < K; ++k)
         {
-            if (!(emit_mask & (1u << k))) continue;
-            size_t ks = (size_t)i * K + k;
-            float4 wd = R.nee_wi[ks];
-            if (k & 1)
+            const bool mine = ((emit_mask >> k) & 1u) != 0;
+            const unsigned long long m = __ballot(mine);
+            if (mine)
             {
-                R.qc_o[buf][cp] = st3(org, 0.01f);
-                R.qc_d[buf][cp] = wd;
-                R.qc_own[buf][cp] = (int)((Q_NEE_CLOSEST << 29) | (uint32_t)ks);
-                ++cp;
+                size_t ks = (size_t)i * K + k;
+                float4 wd = R.nee_wi[ks];
+                if (k & 1)
+                {
+                    const uint32_t cp = cbase + cofs + (uint32_t)__popcll(m & below);
+                    R.qc_o[buf][cp] = st3(org, 0.01f);
+                    R.qc_d[buf][cp] = wd;
+                    R.qc_own[buf][cp] = (int)((Q_NEE_CLOSEST << 29) | (uint32_t)ks);
+                }
+                else
+                {
+                    const uint32_t ap = abase + aofs + (uint32_t)__popcll(m & below);
+                    R.qa_o[buf][ap] = st3(org, 0.01f);
+                    R.qa_d[buf][ap] = wd;
+                    R.qa_own[buf][ap] = (int)((Q_NEE_ANY << 29) | (uint32_t)ks);
+                }
             }
-            else
-            {
-                R.qa_o[buf][ap] = st3(org, 0.01f);
-                R.qa_d[buf][ap] = wd;
-                R.qa_own[buf][ap] = (int)((Q_NEE_ANY << 29) | (uint32_t)ks);
-                ++ap;
-            }
+            if (k & 1) cofs += (uint32_t)__popcll(m);
+            else aofs += (uint32_t)__popcll(m);
         }
     }
 }
