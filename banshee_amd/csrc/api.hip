@@ -228,6 +228,13 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
         d.pdf = 1.f / d.area;                                              // FillSample, mesh.cpp:121
         d.pad = 0.f;
         d.e = make_float4(m.color[0], m.color[1], m.color[2], 0.f);
+        const int32_t* tn = &s->tri_n[3 * tri];
+        const float* N = s->normals.data();
+        for (int k = 0; k < 3; ++k)
+        {
+            d.p[k] = make_float4(V[3 * tv[k]], V[3 * tv[k] + 1], V[3 * tv[k] + 2], 0.f);
+            d.n[k] = make_float4(N[3 * tn[k]], N[3 * tn[k] + 1], N[3 * tn[k] + 2], 0.f);
+        }
         s->h_lights.push_back(d);
     }
     // BVH

@@ -46,12 +46,12 @@ __global__ void k_diag_light(SceneDev s, int light, uint32_t n, const float* p, 
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const LightDev L = s.lights[light];
+    const LightDev* L = s.lights + light;
     f3 pp = mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
     f3 d;
     float pdf = 0.f;
-    f3 le = light_sample(s, L, pp, u[2 * i], u[2 * i + 1], d, pdf);
-    float gp = light_pdf(s, L, pp, mk(w[3 * i], w[3 * i + 1], w[3 * i + 2]));
+    f3 le = light_sample(L, pp, u[2 * i], u[2 * i + 1], d, pdf);
+    float gp = light_pdf(L, pp, mk(w[3 * i], w[3 * i + 1], w[3 * i + 2]));
     float* o = out + 8 * i;
     o[0] = le.x; o[1] = le.y; o[2] = le.z; o[3] = pdf;
     o[4] = d.x; o[5] = d.y; o[6] = d.z; o[7] = gp;

@@ -731,7 +731,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                             for (int j = 0; j < jn; ++j)
                             {
                                 int Lid = b == 0 ? j : Lsel;
-                                const LightDev L = S.lights[Lid];
+                                const LightDev* L = S.lights + Lid;
                                 uint32_t lc = b == 0 ? (uint32_t)(2 * j) : (uint32_t)(2 * nl + 2 * (b - 1));
                                 uint32_t bc = 2u + lc;
                                 float lx, ly, bx, by;
@@ -741,7 +741,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                                 // light sample + shadow ray
                                 f3 ld;
                                 float lpdf = 0.f, cosl = 0.f;
-                                f3 le = light_sample(S, L, h.p, lx, ly, ld, lpdf, &cosl);
+                                f3 le = light_sample(L, h.p, lx, ly, ld, lpdf, &cosl);
                                 if ((nq ? lpdf > 0.f : lpdf > kMinPdf) && sqnorm(le) > 0.f)
                                 {
                                     f3 wi = normalize(ld);
@@ -773,7 +773,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                                 f3 wi = normalize(wis);
                                 if (sqnorm(fb) > 0.f && (nq ? bpdf > 0.f : bpdf > kMinPdf))
                                 {
-                                    float lp = light_pdf(S, L, h.p, wi);
+                                    float lp = light_pdf(L, h.p, wi);
                                     if (nq ? lp > 0.f : !(lp < kMinPdf))
                                     {
                                         float w = power_heuristic(bpdf, lp);

@@ -18,6 +18,11 @@ struct LightDev
     float pdf;
     float pad;
     float4 e;
+    // the light triangle's vertices and vertex normals, copied from the scene's arrays so a
+    // light sample or pdf reads them with one load instead of going through the triangle's
+    // index record
+    float4 p[3];
+    float4 n[3];
 };
 
 struct SceneDev

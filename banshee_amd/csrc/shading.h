@@ -31,11 +31,11 @@ __device__ __forceinline__ void tri_verts(const SceneDev& s, int prim, f3& p1, f
 }
 
 // AreaLight::GetPdf -> Mesh::GetPdfOnShape (arealight.cpp:55-58, mesh.cpp:354-378)
-__device__ __forceinline__ float light_pdf(const SceneDev& s, const LightDev& L, f3 p, f3 w)
+__device__ __forceinline__ float light_pdf(const LightDev* Lp, f3 p, f3 w)
 {
-    f3 p1, p2, p3, n1, n2, n3;
-    int mat, bl;
-    tri_verts(s, L.tri, p1, p2, p3, n1, n2, n3, mat, bl);
+    const LightDev& L = *Lp;
+    const f3 p1 = ld3(L.p[0]), p2 = ld3(L.p[1]), p3 = ld3(L.p[2]);
+    const f3 n1 = ld3(L.n[0]), n2 = ld3(L.n[1]), n3 = ld3(L.n[2]);
     float t, a, b;
     if (!intersect_face(p1, p2 - p1, p3 - p1, p, w, 0.001f, 100000.f, t, a, b)) return 0.f;
     Frame h = fill_hit(p1, p2, p3, n1, n2, n3, a, b);
@@ -48,12 +48,12 @@ __device__ __forceinline__ float light_pdf(const SceneDev& s, const LightDev& L,
 }
 
 // AreaLight::GetSample -> Mesh::GetSampleOnShape/FillSample (arealight.cpp:25-53, mesh.cpp:99-123,323-336)
-__device__ __forceinline__ f3 light_sample(const SceneDev& s, const LightDev& L, f3 p, float ux, float uy, f3& d, float& pdf,
+__device__ __forceinline__ f3 light_sample(const LightDev* Lp, f3 p, float ux, float uy, f3& d, float& pdf,
                                           float* cos_l = nullptr)
 {
-    f3 p1, p2, p3, n1, n2, n3;
-    int mat, bl;
-    tri_verts(s, L.tri, p1, p2, p3, n1, n2, n3, mat, bl);
+    const LightDev& L = *Lp;
+    const f3 p1 = ld3(L.p[0]), p2 = ld3(L.p[1]), p3 = ld3(L.p[2]);
+    const f3 n1 = ld3(L.n[0]), n2 = ld3(L.n[1]), n3 = ld3(L.n[2]);
     float a = sqrtf(ux) * (1.f - uy);
     float b = sqrtf(ux) * uy;
     float w = 1.f - a - b;
