@@ -17,9 +17,14 @@
 // shade(b) first resolves the next-event (NEE) rays its previous pass issued, then
 // shades the closest hit of the extension ray, emits the new shadow (any-hit), MIS
 // (closest-hit) and extension rays into two compacted queues (one atomic per wave and
-// queue), and finalises terminated paths. trace() is a persistent kernel whose waves
-// take 64 rays at a time from the closest queue, then the any queue, so every wave runs
-// a single traversal variant.
+// queue; each wave's rays grouped by direction octant and NEE slot), and finalises
+// terminated paths. trace() is a persistent kernel: on 4-wide BVHs (k_trace_refill) each
+// lane holds its own ray, closest and any-hit alike, and a wave claims new rays for its idle
+// lanes in bulk; on binary BVHs (k_trace_queue) waves take 64 rays at a time from the
+// closest queue, then the any queue.
+//
+// Guided jobs add, per bounce, the guide network's inputs of the extension hits and one
+// inference launch before the shade, and training records compacted after each chunk.
 //
 // Random numbers come from the keyed sample stream; the counter of every draw is a
 // fixed function of (bounce, light), which reproduces the reference's draw order

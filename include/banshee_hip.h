@@ -14,8 +14,9 @@
  *
  * Threading: one context per device; a scene is immutable after bsh_scene_commit;
  * ray queries on different streams may run concurrently from different threads;
- * bsh_render / bsh_bake_sh / bsh_batch_run share the context's wavefront buffers and
- * serialise on a per-context lock (they return after their work has completed).
+ * bsh_render / bsh_bake_sh / bsh_render_guided / bsh_bake_sh_guided / bsh_batch_run share
+ * the context's wavefront buffers and serialise on a per-context lock (they return after
+ * their work has completed).
  */
 #ifndef BANSHEE_HIP_H
 #define BANSHEE_HIP_H
