@@ -44,6 +44,28 @@
 
 namespace bsh {
 
+// Device-memory views for the trace kernel's hot loads and its stack: through a global (or
+// LDS) address-space pointer the compiler emits global (or LDS) instructions; through a
+// generic pointer it emits flat ones, which count against both the memory and the LDS wait
+// counters, so the stack's LDS accesses would wait on outstanding node fetches.
+typedef float nf4 __attribute__((ext_vector_type(4)));
+typedef int ni4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const nf4 gnf4;
+typedef __attribute__((address_space(1))) const ni4 gni4;
+__device__ __forceinline__ float4 gload(const float4* p, size_t i)
+{
+    const nf4 v = ((const gnf4*)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int4 gload_i(const float4* p, size_t i)
+{
+    const ni4 v = ((const gni4*)p)[i];
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+typedef __attribute__((address_space(1))) uint32_t gu32;
+typedef __attribute__((address_space(3))) uint32_t lu32;
+
+
 constexpr int kBlock = 256;
 // LDS entries of the wavefront trace kernel's traversal stack: all of the binary layout's
 // kMaxStack (32 x 4 B x 256 threads = 32 KB per block, 5 blocks per CU); the 4-wide
@@ -400,9 +422,9 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
 {
     __shared__ uint32_t stk[kTraceLds * kBlock];
     const int lane = threadIdx.x & 63;
-    uint32_t* my = stk + threadIdx.x;
+    lu32* my = (lu32*)(stk + threadIdx.x);
     const int ostride = gridDim.x * kBlock;
-    uint32_t* ovf = R.stk_ovf + blockIdx.x * kBlock + threadIdx.x;
+    gu32* ovf = (gu32*)(R.stk_ovf + blockIdx.x * kBlock + threadIdx.x);
     const uint32_t nc = R.counters[4 * L + 0];
     const uint32_t total = nc + R.counters[4 * L + 2];
     uint32_t* work = &R.counters[4 * L + 1];
@@ -433,8 +455,8 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
                 {
                     any = q >= nc;
                     const uint32_t k = any ? q - nc : q;
-                    const float4 a = any ? R.qa_o[buf][k] : R.qc_o[buf][k];
-                    const float4 b = any ? R.qa_d[buf][k] : R.qc_d[buf][k];
+                    const float4 a = gload(any ? R.qa_o[buf] : R.qc_o[buf], k);
+                    const float4 b = gload(any ? R.qa_d[buf] : R.qc_d[buf], k);
                     own = (uint32_t)(any ? R.qa_own[buf][k] : R.qc_own[buf][k]);
                     if (R.nodes1)
                     {
@@ -472,10 +494,11 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
         if (!active) continue;
         // ---- one node visit (traverse4's loop body) ----
         if (kCount) vis_nodes++;
-        const float4* nd = nodes + 8 * (size_t)node;
-        const float4 LX = nd[0], HX = nd[1], LY = nd[2], HY = nd[3], LZ = nd[4], HZ = nd[5];
-        const int4 REF = reinterpret_cast<const int4*>(nd)[6];
-        const int4 CNT = reinterpret_cast<const int4*>(nd)[7];
+        const size_t nb = 8 * (size_t)node;
+        const float4 LX = gload(nodes, nb), HX = gload(nodes, nb + 1), LY = gload(nodes, nb + 2),
+                     HY = gload(nodes, nb + 3), LZ = gload(nodes, nb + 4), HZ = gload(nodes, nb + 5);
+        const int4 REF = gload_i(nodes, nb + 6);
+        const int4 CNT = gload_i(nodes, nb + 7);
         const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, hx[4] = {HX.x, HX.y, HX.z, HX.w};
         const float ly[4] = {LY.x, LY.y, LY.z, LY.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w};
         const float lz[4] = {LZ.x, LZ.y, LZ.z, LZ.w}, hz[4] = {HZ.x, HZ.y, HZ.z, HZ.w};
@@ -500,9 +523,9 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
             {
                 for (int q = ref[c]; q < ref[c] + cn[c]; ++q)
                 {
-                    const float4 v0 = tris[3 * q + 0];
-                    const float4 e1 = tris[3 * q + 1];
-                    const float4 e2 = tris[3 * q + 2];
+                    const float4 v0 = gload(tris, 3 * (size_t)q + 0);
+                    const float4 e1 = gload(tris, 3 * (size_t)q + 1);
+                    const float4 e2 = gload(tris, 3 * (size_t)q + 2);
                     if (kCount) vis_tris++;
                     float t, u, v;
                     if (intersect_face(ld3(v0), ld3(e1), ld3(e2), o, d, tmin, tcur, t, u, v))

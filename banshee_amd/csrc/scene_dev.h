@@ -168,16 +168,18 @@ __device__ __forceinline__ bool traverse(const SceneDev& s, f3 o, f3 d, float tm
 }
 
 // Stack push/pop over the LDS part (kLds entries) and the global overflow slots.
-template <int kLds, int kCap>
-__device__ __forceinline__ void stack_push(uint32_t* stk, int stride, uint32_t* ovf, int ostride, int& sp, uint32_t v)
+// The pointer types are template parameters so a kernel can pass address-space-qualified
+// pointers (LDS for stk, global for ovf) and get LDS and global instructions instead of flat.
+template <int kLds, int kCap, class SP, class OP>
+__device__ __forceinline__ void stack_push(SP stk, int stride, OP ovf, int ostride, int& sp, uint32_t v)
 {
     if (sp < kLds) stk[sp * stride] = v;
     else if (kLds < kCap && sp < kCap) ovf[(sp - kLds) * ostride] = v;
     if (sp < kCap) ++sp;
 }
 
-template <int kLds, int kCap>
-__device__ __forceinline__ uint32_t stack_pop(const uint32_t* stk, int stride, const uint32_t* ovf, int ostride, int& sp)
+template <int kLds, int kCap, class SP, class OP>
+__device__ __forceinline__ uint32_t stack_pop(SP stk, int stride, OP ovf, int ostride, int& sp)
 {
     --sp;
     return (kLds >= kCap || sp < kLds) ? stk[sp * stride] : ovf[(sp - kLds) * ostride];
