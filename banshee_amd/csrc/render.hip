@@ -44,26 +44,7 @@
 
 namespace bsh {
 
-// Device-memory views for the trace kernel's hot loads and its stack: through a global (or
-// LDS) address-space pointer the compiler emits global (or LDS) instructions; through a
-// generic pointer it emits flat ones, which count against both the memory and the LDS wait
-// counters, so the stack's LDS accesses would wait on outstanding node fetches.
-typedef float nf4 __attribute__((ext_vector_type(4)));
-typedef int ni4 __attribute__((ext_vector_type(4)));
-typedef __attribute__((address_space(1))) const nf4 gnf4;
-typedef __attribute__((address_space(1))) const ni4 gni4;
-__device__ __forceinline__ float4 gload(const float4* p, size_t i)
-{
-    const nf4 v = ((const gnf4*)p)[i];
-    return make_float4(v.x, v.y, v.z, v.w);
-}
-__device__ __forceinline__ int4 gload_i(const float4* p, size_t i)
-{
-    const ni4 v = ((const gni4*)p)[i];
-    return make_int4(v.x, v.y, v.z, v.w);
-}
-typedef __attribute__((address_space(1))) uint32_t gu32;
-typedef __attribute__((address_space(3))) uint32_t lu32;
+
 
 
 constexpr int kBlock = 256;

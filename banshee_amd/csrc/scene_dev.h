@@ -8,6 +8,27 @@
 
 namespace bsh {
 
+// Device-memory views for hot loads and the traversal stack: through a global (or
+// LDS) address-space pointer the compiler emits global (or LDS) instructions; through a
+// generic pointer it emits flat ones, which count against both the memory and the LDS wait
+// counters, so the stack's LDS accesses would wait on outstanding node fetches.
+typedef float nf4 __attribute__((ext_vector_type(4)));
+typedef int ni4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const nf4 gnf4;
+typedef __attribute__((address_space(1))) const ni4 gni4;
+__device__ __forceinline__ float4 gload(const float4* p, size_t i)
+{
+    const nf4 v = ((const gnf4*)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int4 gload_i(const float4* p, size_t i)
+{
+    const ni4 v = ((const gni4*)p)[i];
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+typedef __attribute__((address_space(1))) uint32_t gu32;
+typedef __attribute__((address_space(3))) uint32_t lu32;
+
 // One area light (AreaLight, light/arealight.h:43-81): emissive triangle `tri`,
 // emission e of its material, and the triangle's area and area pdf exactly as
 // Mesh::GetShapeSurfaceArea / FillSample compute them (mesh.cpp:304-321, 121).
@@ -77,12 +98,15 @@ __device__ __forceinline__ bool traverse(const SceneDev& s, f3 o, f3 d, float tm
     float hb1 = 0.f, hb2 = 0.f;
     uint32_t node = 0;
     int sp = 0;
+    // the stack is in LDS and its overflow in device memory (see gload)
+    lu32* const lstk = (lu32*)stk;
+    gu32* const govf = (gu32*)ovf;
     for (;;)
     {
-        const float4 a = s.nodes[4 * node + 0];
-        const float4 b = s.nodes[4 * node + 1];
-        const float4 c = s.nodes[4 * node + 2];
-        const int4 r = reinterpret_cast<const int4*>(s.nodes)[4 * node + 3];
+        const float4 a = gload(s.nodes, 4 * (size_t)node + 0);
+        const float4 b = gload(s.nodes, 4 * (size_t)node + 1);
+        const float4 c = gload(s.nodes, 4 * (size_t)node + 2);
+        const int4 r = gload_i(s.nodes, 4 * (size_t)node + 3);
         if (kCount) cnt.nodes++;
         // child 0
         float lx0 = __builtin_fmaf(a.x, ix, -ox), hx0 = __builtin_fmaf(a.y, ix, -ox);
@@ -109,9 +133,9 @@ __device__ __forceinline__ bool traverse(const SceneDev& s, f3 o, f3 d, float tm
                 int first = k == 0 ? r.x : r.y;
                 for (int q = first; q < first + cnt_k; ++q)
                 {
-                    const float4 v0 = s.tris[3 * q + 0];
-                    const float4 e1 = s.tris[3 * q + 1];
-                    const float4 e2 = s.tris[3 * q + 2];
+                    const float4 v0 = gload(s.tris, 3 * (size_t)q + 0);
+                    const float4 e1 = gload(s.tris, 3 * (size_t)q + 1);
+                    const float4 e2 = gload(s.tris, 3 * (size_t)q + 2);
                     if (kCount) cnt.tris++;
                     float t, u, v;
                     if (intersect_face(ld3(v0), ld3(e1), ld3(e2), o, d, tmin, kAny ? tmax : tcur, t, u, v))
@@ -140,8 +164,8 @@ __device__ __forceinline__ bool traverse(const SceneDev& s, f3 o, f3 d, float tm
             uint32_t farc = first0 ? (uint32_t)r.y : (uint32_t)r.x;
             // The builder caps internal depth at kMaxStack - 2, so the stack never fills;
             // were it to, the far child is dropped rather than written out of bounds.
-            if (sp < kLds) stk[sp * stride] = farc;
-            else if (kLds < kMaxStack && sp < kMaxStack) ovf[(sp - kLds) * ostride] = farc;
+            if (sp < kLds) lstk[sp * stride] = farc;
+            else if (kLds < kMaxStack && sp < kMaxStack) govf[(sp - kLds) * ostride] = farc;
             if (sp < kMaxStack) ++sp;
             node = nearc;
         }
@@ -157,7 +181,7 @@ __device__ __forceinline__ bool traverse(const SceneDev& s, f3 o, f3 d, float tm
         {
             if (sp == 0) break;
             --sp;
-            node = (kLds == kMaxStack || sp < kLds) ? stk[sp * stride] : ovf[(sp - kLds) * ostride];
+            node = (kLds == kMaxStack || sp < kLds) ? lstk[sp * stride] : govf[(sp - kLds) * ostride];
         }
     }
     res.prim = prim;
@@ -208,10 +232,11 @@ __device__ __forceinline__ bool traverse4(const SceneDev& s, f3 o, f3 d, float t
     int sp = 0;
     for (;;)
     {
-        const float4* nd = s.nodes + 8 * (size_t)node;
-        const float4 LX = nd[0], HX = nd[1], LY = nd[2], HY = nd[3], LZ = nd[4], HZ = nd[5];
-        const int4 REF = reinterpret_cast<const int4*>(nd)[6];
-        const int4 CNT = reinterpret_cast<const int4*>(nd)[7];
+        const size_t nb = 8 * (size_t)node;
+        const float4 LX = gload(s.nodes, nb), HX = gload(s.nodes, nb + 1), LY = gload(s.nodes, nb + 2),
+                     HY = gload(s.nodes, nb + 3), LZ = gload(s.nodes, nb + 4), HZ = gload(s.nodes, nb + 5);
+        const int4 REF = gload_i(s.nodes, nb + 6);
+        const int4 CNT = gload_i(s.nodes, nb + 7);
         if (kCount) cnt.nodes++;
         const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, hx[4] = {HX.x, HX.y, HX.z, HX.w};
         const float ly[4] = {LY.x, LY.y, LY.z, LY.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w};
@@ -236,9 +261,9 @@ __device__ __forceinline__ bool traverse4(const SceneDev& s, f3 o, f3 d, float t
             {
                 for (int q = ref[c]; q < ref[c] + cn[c]; ++q)
                 {
-                    const float4 v0 = s.tris[3 * q + 0];
-                    const float4 e1 = s.tris[3 * q + 1];
-                    const float4 e2 = s.tris[3 * q + 2];
+                    const float4 v0 = gload(s.tris, 3 * (size_t)q + 0);
+                    const float4 e1 = gload(s.tris, 3 * (size_t)q + 1);
+                    const float4 e2 = gload(s.tris, 3 * (size_t)q + 2);
                     if (kCount) cnt.tris++;
                     float t, u, v;
                     if (intersect_face(ld3(v0), ld3(e1), ld3(e2), o, d, tmin, kAny ? tmax : tcur, t, u, v))
@@ -284,12 +309,12 @@ __device__ __forceinline__ bool traverse4(const SceneDev& s, f3 o, f3 d, float t
         if (n == 0)
         {
             if (sp == 0) break;
-            node = stack_pop<kLds, kMaxStack4>(stk, stride, ovf, ostride, sp);
+            node = stack_pop<kLds, kMaxStack4>((lu32*)stk, stride, (gu32*)ovf, ostride, sp);
             continue;
         }
 #pragma unroll
         for (int k = 3; k >= 1; --k)
-            if (k < n) stack_push<kLds, kMaxStack4>(stk, stride, ovf, ostride, sp, id[k]);
+            if (k < n) stack_push<kLds, kMaxStack4>((lu32*)stk, stride, (gu32*)ovf, ostride, sp, id[k]);
         node = id[0];
     }
     res.prim = prim;
