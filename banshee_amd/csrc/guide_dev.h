@@ -43,6 +43,15 @@ BSH_HD inline bool guide_pad_feature(int j)
     return p >= 14 && 2 * b + (p - 14) >= kGRaw;
 }
 
+// Device-only: the decode, pdf and sampler use the hardware exp2 / log2 / reciprocal /
+// reciprocal square root (relative error about 1e-6, far below the FP16 precision of the
+// network outputs they read). The sampler and the pdf work from the same decoded lobes, so
+// the density evaluated is the density sampled, to float rounding.
+__device__ __forceinline__ float gexp(float x) { return __expf(x); }
+__device__ __forceinline__ float glog(float x) { return __logf(x); }
+__device__ __forceinline__ float grcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float grsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
 struct Lobes
 {
     float w[kGLobes];      // mixture weights
@@ -50,8 +59,17 @@ struct Lobes
     float kappa[kGLobes];
 };
 
+// Unit mean direction of lobe k from its raw outputs (0,0,1 when they are all zero).
 template <class T>
-BSH_HD Lobes guide_lobes(const T* o)
+__device__ __forceinline__ f3 lobe_mu(const T* o, int k)
+{
+    const f3 r = mk((float)o[8 + 3 * k], (float)o[9 + 3 * k], (float)o[10 + 3 * k]);
+    const float n2 = sqnorm(r);
+    return n2 > 0.f ? r * grsq(n2) : mk(0.f, 0.f, 1.f);
+}
+
+template <class T>
+__device__ __forceinline__ Lobes guide_lobes(const T* o)
 {
     Lobes L;
     float mx = (float)o[0];
@@ -59,33 +77,32 @@ BSH_HD Lobes guide_lobes(const T* o)
     float s = 0.f;
     for (int k = 0; k < kGLobes; ++k)
     {
-        L.w[k] = expf((float)o[k] - mx);
+        L.w[k] = gexp((float)o[k] - mx);
         s += L.w[k];
     }
+    const float is = grcp(s);
     for (int k = 0; k < kGLobes; ++k)
     {
-        L.w[k] = L.w[k] / s;
-        f3 r = mk((float)o[8 + 3 * k], (float)o[9 + 3 * k], (float)o[10 + 3 * k]);
-        float n = sqrtf(sqnorm(r));
-        L.mu[k] = n > 0.f ? r * (1.f / n) : mk(0.f, 0.f, 1.f);
-        L.kappa[k] = expf(clampf((float)o[32 + k], kLogKappaLo, kLogKappaHi));
+        L.w[k] = L.w[k] * is;
+        L.mu[k] = lobe_mu(o, k);
+        L.kappa[k] = gexp(clampf((float)o[32 + k], kLogKappaLo, kLogKappaHi));
     }
     return L;
 }
 
-BSH_HD bool lobes_finite(const Lobes& L)
+__device__ __forceinline__ bool lobes_finite(const Lobes& L)
 {
     float s = 0.f;
     for (int k = 0; k < kGLobes; ++k) s += L.w[k] + L.kappa[k] + L.mu[k].x + L.mu[k].y + L.mu[k].z;
     return isfinite(s);
 }
 
-BSH_HD float vmf_pdf(f3 mu, float kappa, f3 w)
+__device__ __forceinline__ float vmf_pdf(f3 mu, float kappa, f3 w)
 {
-    return kappa / (2.f * kPi * (1.f - expf(-2.f * kappa))) * expf(kappa * (dot(mu, w) - 1.f));
+    return kappa * grcp(2.f * kPi * (1.f - gexp(-2.f * kappa))) * gexp(kappa * (dot(mu, w) - 1.f));
 }
 
-BSH_HD float guide_pdf(const Lobes& L, f3 w)
+__device__ __forceinline__ float guide_pdf(const Lobes& L, f3 w)
 {
     float p = 0.f;
     for (int k = 0; k < kGLobes; ++k) p += L.w[k] * vmf_pdf(L.mu[k], L.kappa[k], w);
@@ -97,7 +114,7 @@ BSH_HD float guide_pdf(const Lobes& L, f3 w)
 // (the same operations as guide_lobes, so the same values): indexing the Lobes registers by
 // a computed lobe would move them to scratch memory.
 template <class T>
-BSH_HD f3 guide_sample(const Lobes& L, const T* o, float u0, float u1, float u2)
+__device__ __forceinline__ f3 guide_sample(const Lobes& L, const T* o, float u0, float u1, float u2)
 {
     int k = kGLobes - 1;
     float c = 0.f;
@@ -110,17 +127,15 @@ BSH_HD f3 guide_sample(const Lobes& L, const T* o, float u0, float u1, float u2)
         k = take ? j : k;
         found = found || take;
     }
-    f3 r = mk((float)o[8 + 3 * k], (float)o[9 + 3 * k], (float)o[10 + 3 * k]);
-    float n = sqrtf(sqnorm(r));
-    f3 mu = n > 0.f ? r * (1.f / n) : mk(0.f, 0.f, 1.f);
-    float kap = expf(clampf((float)o[32 + k], kLogKappaLo, kLogKappaHi));
-    float w = 1.f + logf(u1 + (1.f - u1) * expf(-2.f * kap)) / kap;
+    const f3 mu = lobe_mu(o, k);
+    const float kap = gexp(clampf((float)o[32 + k], kLogKappaLo, kLogKappaHi));
+    float w = 1.f + glog(u1 + (1.f - u1) * gexp(-2.f * kap)) * grcp(kap);
     w = clampf(w, -1.f, 1.f);
     float s = sqrtf(fmaxf(0.f, 1.f - w * w));
     f3 t1 = orthovector(mu);
     f3 t2 = cross(mu, t1);
-    float phi = 2.f * kPi * u2;
-    return normalize(mu * w + t1 * (s * cosf(phi)) + t2 * (s * sinf(phi)));
+    // azimuth 2 pi u2; the hardware sine / cosine take revolutions: sin(2 pi u2) = amdgcn_sin(u2)
+    return normalize(mu * w + t1 * (s * __builtin_amdgcn_cosf(u2)) + t2 * (s * __builtin_amdgcn_sinf(u2)));
 }
 
 }  // namespace bsh
