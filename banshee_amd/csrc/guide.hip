@@ -306,10 +306,10 @@ constexpr float kMaxRecordWeight = 256.f;
 // Per-row loss -w log q(w_dir) and its gradient w.r.t. the 48 outputs. Weights are
 // normalised to mean 1 over the batch (s = w * rows / sum w) so FP16 gradients stay O(1);
 // the weight-gradient reduction divides by rows. rec: 3 float4 per row: (x, r), (n, active),
-// (dir, w).
+// (dir, w). T: the gradient's type (FP16 for the MFMA backward, FP32 in FP32 mode).
+template <class T>
 __global__ void k_guide_loss(const float* out, const float4* rec, int srow, uint32_t nrows, const float* wsum,
-                             _Float16* dout,
-                             float* loss_acc)
+                             T* dout, float* loss_acc)
 {
     uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= nrows) return;
@@ -358,7 +358,7 @@ __global__ void k_guide_loss(const float* out, const float4* rec, int srow, uint
         }
         atomicAdd(loss_acc, -s * logf(q));
     }
-    for (int k = 0; k < kGOut; ++k) dout[(size_t)row * kGOut + k] = (_Float16)g[k];
+    for (int k = 0; k < kGOut; ++k) dout[(size_t)row * kGOut + k] = (T)g[k];
 }
 
 // Sum of the record weights in a fixed order, so that training is reproducible bit for bit:
@@ -580,6 +580,81 @@ __global__ void k_guide_adam(float* p, float* m, float* v, const float* grad, in
     else bias[4 * kGHid + t - kGB5] = x;
 }
 
+// ---- FP32 mode (verification) ----
+// The same network, loss and backward computed from the FP32 master parameters (input-major
+// W[k][n], then the bias) with FP32 features, activations and gradients and plain FMA
+// arithmetic: one thread per output element, every sum in a fixed order. It checks the FP16
+// kernels' feature order, layer wiring and gradient formulas without their rounding and is
+// not a fast path. The features use the accurate sinpi/cospi of 2^b v (an exact scaling)
+// rather than the FP16 encoder's double-angle recurrence.
+__global__ void k_f32_encode(const float4* in, int stride, uint32_t nrows, float* X)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nrows * kGIn) return;
+    const size_t row = t / kGIn;
+    const int f = (int)(t % kGIn), b = f / 16, p = f % 16;
+    const float4 a = in[(size_t)stride * row], c = in[(size_t)stride * row + 1];
+    const float v[kGRaw] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z};
+    float x = 0.f;
+    if (p < 14)
+    {
+        const float arg = ldexpf(v[p / 2], b);
+        x = (p & 1) ? cospif(arg) : sinpif(arg);
+    }
+    else if (2 * b + p - 14 < kGRaw)
+        x = v[2 * b + p - 14];
+    X[t] = x;
+}
+
+// Y[row][n] = act(b[n] + sum_k X[row][k] W[k][n]), k ascending; act = ReLU or identity.
+__global__ void k_f32_dense(const float* X, int K, const float* W, const float* b, int N, uint32_t nrows, int relu,
+                            float* Y)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nrows * N) return;
+    const size_t row = t / N;
+    const int n = (int)(t % N);
+    const float* x = X + row * K;
+    float acc = b[n];
+    for (int k = 0; k < K; ++k) acc = fmaf(x[k], W[(size_t)k * N + n], acc);
+    Y[t] = relu ? fmaxf(acc, 0.f) : acc;
+}
+
+// dX[row][i] = (H[row][i] > 0) sum_j dY[row][j] W[i][j]: the gradient through a layer whose
+// input-major weights are W[NIN][KOUT] and through the ReLU that produced its input H.
+__global__ void k_f32_bwd(const float* dY, int KOUT, const float* W, const float* H, int NIN, uint32_t nrows,
+                          float* dX)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nrows * NIN) return;
+    const size_t row = t / NIN;
+    const int i = (int)(t % NIN);
+    float s = 0.f;
+    if (H[t] > 0.f)
+        for (int j = 0; j < KOUT; ++j) s = fmaf(dY[row * KOUT + j], W[(size_t)i * KOUT + j], s);
+    dX[t] = s;
+}
+
+// One layer's gradient block in the parameter layout: W[k][n] = inv_rows sum_rows X[row][k] dY[row][n]
+// (threads 0 .. K*N-1), then b[n] = inv_rows sum_rows dY[row][n], rows ascending.
+__global__ void k_f32_wgrad(const float* X, int K, const float* dY, int N, uint32_t nrows, float inv_rows, float* grad)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= K * N + N) return;
+    float s = 0.f;
+    if (t < K * N)
+    {
+        const int k = t / N, n = t % N;
+        for (uint32_t r = 0; r < nrows; ++r) s = fmaf(X[(size_t)r * K + k], dY[(size_t)r * N + n], s);
+    }
+    else
+    {
+        const int n = t - K * N;
+        for (uint32_t r = 0; r < nrows; ++r) s += dY[(size_t)r * N + n];
+    }
+    grad[t] = s * inv_rows;
+}
+
 __global__ void k_diag_mfma(const _Float16* a, const _Float16* b, float* d)
 {
     // a: 16x16 row-major (M x K), b: 16x16 row-major (K x N), d: 16x16 row-major
@@ -763,8 +838,8 @@ static bsh_status guide_gradients(bsh_guide* g, const float4* rec, int srow, uin
     uint32_t blocks = std::min<uint32_t>((groups + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
     hipLaunchKernelGGL((k_guide_forward<true, false>), dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), rec, srow, n,
                        (const uint32_t*)nullptr, (void*)g->out.p, g->x0.p, g->hs.p);
-    hipLaunchKernelGGL(k_guide_loss, dim3((n + 255) / 256), dim3(256), 0, st, g->out.p, rec, srow, n, g->wsum.p, g->d5.p,
-                       g->loss.p);
+    hipLaunchKernelGGL(k_guide_loss<_Float16>, dim3((n + 255) / 256), dim3(256), 0, st, g->out.p, rec, srow, n,
+                       g->wsum.p, g->d5.p, g->loss.p);
     const _Float16* H1 = g->hs.p;
     const _Float16* H2 = g->hs.p + (size_t)n * kGHid;
     const _Float16* H3 = g->hs.p + (size_t)2 * n * kGHid;
@@ -779,6 +854,79 @@ static bsh_status guide_gradients(bsh_guide* g, const float4* rec, int srow, uin
     wgrad_layer<kGHid, kGHid>(g, H2, g->d3.p, n, kGW3, st);
     wgrad_layer<kGHid, kGHid>(g, H1, g->d2.p, n, kGW2, st);
     wgrad_layer<kGIn, kGHid>(g, g->x0.p, g->d1.p, n, kGW1, st);
+    BSH_HIP(hipGetLastError());
+    return BSH_OK;
+}
+
+// FP32 mode: the buffers of one call (features, the four hidden activations, outputs, the
+// output gradient, two ping-pong hidden gradients, the parameter gradient). Allocated per
+// call: FP32 mode is a verification path.
+constexpr size_t kF32MaxRows = (size_t)1 << 22;
+struct F32Work
+{
+    DevBuf<float> X, H, out, dY, dA, dB, grad;
+};
+
+static dim3 f32_grid(size_t threads) { return dim3((uint32_t)((threads + 255) / 256)); }
+
+static bsh_status f32_forward(bsh_guide* g, const float4* in, int stride, uint32_t n, F32Work& w, float* out,
+                              hipStream_t st)
+{
+    BSH_HIP(w.X.alloc((size_t)n * kGIn));
+    BSH_HIP(w.H.alloc((size_t)n * kGHid * 4));
+    const float* P = g->p.p;
+    const size_t hs = (size_t)n * kGHid;
+    hipLaunchKernelGGL(k_f32_encode, f32_grid((size_t)n * kGIn), dim3(256), 0, st, in, stride, n, w.X.p);
+    hipLaunchKernelGGL(k_f32_dense, f32_grid(hs), dim3(256), 0, st, (const float*)w.X.p, kGIn, P + kGW1, P + kGB1,
+                       kGHid, n, 1, w.H.p);
+    const int wo[3] = {kGW2, kGW3, kGW4}, bo[3] = {kGB2, kGB3, kGB4};
+    for (int l = 0; l < 3; ++l)
+        hipLaunchKernelGGL(k_f32_dense, f32_grid(hs), dim3(256), 0, st, (const float*)(w.H.p + l * hs), kGHid,
+                           P + wo[l], P + bo[l], kGHid, n, 1, w.H.p + (l + 1) * hs);
+    hipLaunchKernelGGL(k_f32_dense, f32_grid((size_t)n * kGOut), dim3(256), 0, st, (const float*)(w.H.p + 3 * hs),
+                       kGHid, P + kGW5, P + kGB5, kGOut, n, 0, out);
+    BSH_HIP(hipGetLastError());
+    return BSH_OK;
+}
+
+// FP32-mode forward + backward on `rec` (3 float4 per row): the batch gradient in w.grad,
+// the weighted loss (times rows) in g->loss. The loss weights and their normalisation are
+// the FP16 path's (same kernels).
+static bsh_status f32_gradients(bsh_guide* g, const float4* rec, uint32_t n, F32Work& w, hipStream_t st)
+{
+    BSH_HIP(w.out.alloc((size_t)n * kGOut));
+    BSH_HIP(w.dY.alloc((size_t)n * kGOut));
+    BSH_HIP(w.dA.alloc((size_t)n * kGHid));
+    BSH_HIP(w.dB.alloc((size_t)n * kGHid));
+    BSH_HIP(w.grad.alloc(kGParams));
+    hipLaunchKernelGGL(k_guide_zero, dim3(1), dim3(64), 0, st, g->wsum.p, 1);
+    hipLaunchKernelGGL(k_guide_zero, dim3(1), dim3(64), 0, st, g->loss.p, 1);
+    hipLaunchKernelGGL(k_guide_wsum, dim3(kWsumParts), dim3(kWsumBlock), 0, st, rec, 3, n, g->wsum.p + 1);
+    hipLaunchKernelGGL(k_guide_wsum_final, dim3(1), dim3(kWsumBlock), 0, st, g->wsum.p + 1, g->wsum.p);
+    bsh_status r = f32_forward(g, rec, 3, n, w, w.out.p, st);
+    if (r != BSH_OK) return r;
+    hipLaunchKernelGGL(k_guide_loss<float>, dim3((n + 255) / 256), dim3(256), 0, st, (const float*)w.out.p, rec, 3, n,
+                       (const float*)g->wsum.p, w.dY.p, g->loss.p);
+    const float* P = g->p.p;
+    const size_t hs = (size_t)n * kGHid;
+    const float* H[4] = {w.H.p, w.H.p + hs, w.H.p + 2 * hs, w.H.p + 3 * hs};
+    const float inv = 1.f / (float)n;
+    auto wgrad = [&](const float* X, int K, const float* dY, int N, int off) {
+        hipLaunchKernelGGL(k_f32_wgrad, f32_grid((size_t)K * N + N), dim3(256), 0, st, X, K, dY, N, n, inv,
+                           w.grad.p + off);
+    };
+    auto bwd = [&](const float* dY, int KOUT, int woff, const float* Hin, float* dX) {
+        hipLaunchKernelGGL(k_f32_bwd, f32_grid(hs), dim3(256), 0, st, dY, KOUT, P + woff, Hin, kGHid, n, dX);
+    };
+    wgrad(H[3], kGHid, w.dY.p, kGOut, kGW5);
+    bwd(w.dY.p, kGOut, kGW5, H[3], w.dA.p);    // layer 4's pre-activation gradient
+    wgrad(H[2], kGHid, w.dA.p, kGHid, kGW4);
+    bwd(w.dA.p, kGHid, kGW4, H[2], w.dB.p);    // layer 3
+    wgrad(H[1], kGHid, w.dB.p, kGHid, kGW3);
+    bwd(w.dB.p, kGHid, kGW3, H[1], w.dA.p);    // layer 2
+    wgrad(H[0], kGHid, w.dA.p, kGHid, kGW2);
+    bwd(w.dA.p, kGHid, kGW2, H[0], w.dB.p);    // layer 1
+    wgrad(w.X.p, kGIn, w.dB.p, kGHid, kGW1);
     BSH_HIP(hipGetLastError());
     return BSH_OK;
 }
@@ -893,6 +1041,38 @@ bsh_status bsh_guide_gradient(bsh_guide* g, const float* records, size_t n, floa
     if (r != BSH_OK) return r;
     BSH_HIP(hipDeviceSynchronize());
     if (grad) BSH_HIP(hipMemcpy(grad, g->grad.p, kGParams * 4, hipMemcpyDeviceToHost));
+    if (loss)
+    {
+        float l = 0.f;
+        BSH_HIP(hipMemcpy(&l, g->loss.p, 4, hipMemcpyDeviceToHost));
+        *loss = l / (float)n;
+    }
+    return BSH_OK;
+}
+
+bsh_status bsh_guide_forward_fp32(bsh_guide* g, const float* inputs, size_t n, float* out, void* stream)
+{
+    if (!g || (n && (!inputs || !out))) return fail(BSH_ERR_INVALID, "bsh_guide_forward_fp32: NULL argument");
+    if (n == 0) return BSH_OK;
+    if (n > kF32MaxRows) return fail(BSH_ERR_UNSUPPORTED, "bsh_guide_forward_fp32: more than 2^22 rows");
+    BSH_HIP(hipSetDevice(g->ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    F32Work w;
+    bsh_status r = f32_forward(g, reinterpret_cast<const float4*>(inputs), 2, (uint32_t)n, w, out, st);
+    BSH_HIP(hipStreamSynchronize(st));   // the work buffers are released on return
+    return r;
+}
+
+bsh_status bsh_guide_gradient_fp32(bsh_guide* g, const float* records, size_t n, float* grad, float* loss)
+{
+    if (!g || !records || n == 0) return fail(BSH_ERR_INVALID, "bsh_guide_gradient_fp32: bad argument");
+    if (n > kF32MaxRows) return fail(BSH_ERR_UNSUPPORTED, "bsh_guide_gradient_fp32: more than 2^22 rows");
+    BSH_HIP(hipSetDevice(g->ctx->device));
+    F32Work w;
+    bsh_status r = f32_gradients(g, reinterpret_cast<const float4*>(records), (uint32_t)n, w, nullptr);
+    BSH_HIP(hipDeviceSynchronize());
+    if (r != BSH_OK) return r;
+    if (grad) BSH_HIP(hipMemcpy(grad, w.grad.p, kGParams * 4, hipMemcpyDeviceToHost));
     if (loss)
     {
         float l = 0.f;

@@ -105,7 +105,7 @@ EXPORTS = [
     "bsh_guide_num_params", "bsh_guide_create", "bsh_guide_destroy", "bsh_guide_get_params", "bsh_guide_set_params",
     "bsh_guide_forward", "bsh_guide_gradient", "bsh_guide_train_step", "bsh_render_guided", "bsh_diag_mfma",
     "bsh_diag_guide", "bsh_bvh_build_host", "bsh_guide_set_bsdf_fraction", "bsh_guide_train",
-    "bsh_bake_sh_guided",
+    "bsh_bake_sh_guided", "bsh_guide_forward_fp32", "bsh_guide_gradient_fp32",
 ]
 
 _lib = None
@@ -413,24 +413,28 @@ class Guide:
         assert w.size == self.num_params()
         _check(lib().bsh_guide_set_params(self.h, _p(w)), "bsh_guide_set_params")
 
-    def forward(self, inputs):
-        """inputs: n x 8 raw rows (position in [0,1]^3, roughness, normal, unused) -> n x 48 outputs."""
+    def forward(self, inputs, fp32=False):
+        """inputs: n x 8 raw rows (position in [0,1]^3, roughness, normal, unused) -> n x 48 outputs.
+        fp32=True: the FP32 verification mode (bsh_guide_forward_fp32)."""
         x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 8)
         n = x.shape[0]
         dx = self.ctx.to_device(x)
         do = self.ctx.alloc((n, GUIDE_OUT), np.float32)
-        _check(lib().bsh_guide_forward(self.h, dx.ptr, c_size_t(n), do.ptr, c_void_p(None)), "bsh_guide_forward")
+        name = "bsh_guide_forward_fp32" if fp32 else "bsh_guide_forward"
+        _check(getattr(lib(), name)(self.h, dx.ptr, c_size_t(n), do.ptr, c_void_p(None)), name)
         self.ctx.sync()
         return do.numpy()
 
-    def gradient(self, records):
-        """records: n x 12 (host array or DeviceArray) -> (gradient float32[num_params], loss)."""
+    def gradient(self, records, fp32=False):
+        """records: n x 12 (host array or DeviceArray) -> (gradient float32[num_params], loss).
+        fp32=True: the FP32 verification mode (bsh_guide_gradient_fp32)."""
         dr = records if isinstance(records, DeviceArray) else self.ctx.to_device(
             np.ascontiguousarray(records, np.float32).reshape(-1, RECORD_FLOATS))
         n = dr.shape[0]
         g = np.zeros(self.num_params(), np.float32)
         loss = ctypes.c_float()
-        _check(lib().bsh_guide_gradient(self.h, dr.ptr, c_size_t(n), _p(g), ctypes.byref(loss)), "bsh_guide_gradient")
+        name = "bsh_guide_gradient_fp32" if fp32 else "bsh_guide_gradient"
+        _check(getattr(lib(), name)(self.h, dr.ptr, c_size_t(n), _p(g), ctypes.byref(loss)), name)
         return g, loss.value
 
     def train_step(self, records, n=None, offset=0, want_loss=True):

@@ -323,6 +323,13 @@ bsh_status bsh_guide_forward(bsh_guide* g, const float* inputs, size_t n, float*
    weight). bsh_render_guided writes weight = lum(L_in) |n.w| / p_mix. Device records; grad
    host float[num_params]; loss host (weighted mean). */
 bsh_status bsh_guide_gradient(bsh_guide* g, const float* records, size_t n, float* grad, float* loss);
+/* FP32 mode of bsh_guide_forward and bsh_guide_gradient, for verification: the same network,
+   loss and backward from the FP32 master parameters with FP32 features, activations and
+   gradients and plain FMA arithmetic (no MFMA, no FP16 rounding), every sum in a fixed
+   order. Same arguments; at most 2^22 rows (BSH_ERR_UNSUPPORTED beyond). Not a fast path:
+   work buffers are allocated per call and the call returns after the work is done. */
+bsh_status bsh_guide_forward_fp32(bsh_guide* g, const float* inputs, size_t n, float* out, void* stream);
+bsh_status bsh_guide_gradient_fp32(bsh_guide* g, const float* records, size_t n, float* grad, float* loss);
 /* One Adam step on the records. loss may be NULL. */
 bsh_status bsh_guide_train_step(bsh_guide* g, const float* records, size_t n, float* loss, void* stream);
 /* `steps` Adam steps on batches of up to `batch` records, e.g. the records one guided pass

@@ -122,6 +122,71 @@ def test_gradient_matches_fp64(ctx, n):
         assert np.all(grad[j * 64:(j + 1) * 64] == 0)
 
 
+def test_fp32_mode_forward_matches_fp64(ctx):
+    """FP32 mode (bsh_guide_forward_fp32): the network from the FP32 master parameters
+    without FP16 rounding agrees with the FP64 restatement to relative L2 <= 1e-5, and the
+    FP16 kernels' outputs lie within their own bound of it."""
+    from banshee_amd import hip
+    g = hip.Guide(ctx, seed=3)
+    p = g.get_params()
+    rng = np.random.default_rng(11)
+    for n in (1, 15, 1000, 4099):
+        x = raw_inputs(rng, n)
+        out32 = g.forward(x, fp32=True)
+        ref64, _, _ = G.forward(p, x[:, :7], fp16=False)
+        assert rel(out32, ref64) <= 1e-5, (n, rel(out32, ref64))
+        assert rel(g.forward(x), out32) <= 3e-2
+
+
+def fp32_gradient_check(g, rec, tol):
+    p = g.get_params()
+    grad, loss = g.gradient(rec, fp32=True)
+    L, ref = G.loss_and_grad(p, rec, fp16=False)
+    assert abs(loss - L) <= tol * abs(L) + 1e-7, (loss, L)
+    offs, _ = G.offsets()
+    worst = 0.0
+    for (k, m), (wo, bo) in zip(G.SIZES, offs):
+        for a, b in ((wo, wo + k * m), (bo, bo + m)):
+            e = rel(grad[a:b], ref[a:b])
+            worst = max(worst, e)
+            assert e <= tol, (a, e)
+    for j in G.PAD_FEATURES:
+        assert np.all(grad[j * 64:(j + 1) * 64] == 0)
+    return worst
+
+
+@pytest.mark.parametrize("n", [16, 1000, 5000])
+def test_fp32_mode_gradient_matches_fp64(ctx, n):
+    """FP32 mode (bsh_guide_gradient_fp32): loss and every parameter block's gradient agree
+    with the FP64 analytic backward (itself pinned by finite differences) to relative
+    L2 <= 1e-5."""
+    from banshee_amd import hip
+    g = hip.Guide(ctx, seed=7)
+    worst = fp32_gradient_check(g, records(np.random.default_rng(n), n), 1e-5)
+    print(f"n={n}: worst block relative error {worst:.2e}")
+
+
+def test_fp32_mode_gradient_of_a_trained_guide(ctx):
+    """The same check after training on records from one sharp lobe, where the network's
+    concentrations are large (kappa up to e^6) and its mixture weights uneven."""
+    from banshee_amd import hip
+    g = hip.Guide(ctx, seed=12, lr=5e-3, beta2=0.99)
+    rng = np.random.default_rng(5)
+    mu = np.array([0.3, -0.5, 0.81])
+    mu /= np.linalg.norm(mu)
+    d = rng.normal(size=(8192, 3)) * 0.1 + mu
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rec = records(rng, 8192, d)
+    dr = ctx.to_device(rec)
+    for _ in range(150):
+        g.train_step(dr, want_loss=False)
+    out = g.forward(rec[:, :8], fp32=True)
+    lk = float(np.max(out[:, 32:40]))
+    assert lk > 1.0, lk                            # concentrated lobes
+    worst = fp32_gradient_check(g, records(np.random.default_rng(6), 2000, d[:2000]), 1e-5)
+    print(f"trained (max log kappa {lk:.2f}): worst block relative error {worst:.2e}")
+
+
 def test_adam_step_and_fp16_copies(ctx):
     """One Adam step moves every parameter with a clear gradient by lr * sign(g) (the first
     Adam step), and the next forward uses the refreshed FP16 copies."""
