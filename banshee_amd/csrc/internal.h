@@ -79,7 +79,8 @@ struct RenderWork
     DevBuf<uint32_t> gseg;            // guide rows per block segment this pass
     DevBuf<uint32_t> rec_count;       // records written so far (running over the chunks)
     DevBuf<uint32_t> rec_blk, rec_off; // per block of paths: records, output offset
-    DevBuf<uint32_t> counters;     // per launch: count, work, (pad) x 4
+    DevBuf<uint32_t> alive[2];     // paths still live after a shade, listed for the next bounce
+    DevBuf<uint32_t> counters;     // per launch: count, work, (pad) x 4; then the live-path counts
     DevBuf<unsigned long long> stats;  // closest, any
     DevBuf<uint32_t> stk_ovf;          // deep traversal-stack entries of the trace kernel
 };

@@ -89,6 +89,8 @@ struct RenderDev
     float4 *qc_o[2], *qc_d[2], *qa_o[2], *qa_d[2];
     int *qc_own[2], *qa_own[2];
     uint32_t* counters;            // per launch L: closest count, closest work, any count, any work
+    uint32_t* alive[2];            // live paths: shade b reads list b & 1, writes list (b + 1) & 1
+    uint32_t* alive_n;             // per bounce: length of its live-path list (b >= 1)
     unsigned long long* stats;     // closest, any, node visits, triangle tests, then kRowParts guided-row partials
     uint32_t* stk_ovf;             // traversal stack entries beyond kTraceLds, [entry][trace thread]
     float4* gin;                   // guide inputs, 2 per row, rows compacted over the guided vertices
@@ -599,12 +601,23 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 template <bool kFull>
 __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
 {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    // Bounce 0 shades every path of the chunk; a later bounce only the paths the previous
+    // shade left live, in the order it listed them (its wave order, so neighbouring paths
+    // stay together). A wave then holds live paths only, instead of the survivors scattered
+    // over all paths.
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t i = t;
+    bool have = t < R.P;
+    if (b > 0)
+    {
+        have = t < R.alive_n[b];
+        i = have ? R.alive[b & 1][t] : 0u;
+    }
     const int K = 1 << R.Kshift;
-    bool emit_ext = false;
+    bool emit_ext = false, keep = false;
     uint32_t emit_mask = 0;
     f3 org = mk(0.f, 0.f, 0.f), ext_d = mk(0.f, 0.f, 0.f);
-    if (i < R.P)
+    if (have)
     {
         uint4 meta = R.meta[i];
         uint32_t fl = meta.y & 0xffu;
@@ -944,6 +957,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
             R.rad[i] = st3(rad, 0.f);
             meta.y = fl | ((uint32_t)b << 8) | ((uint32_t)ncnt << 16) | (job << 24);
             R.meta[i] = meta;
+            keep = !(fl & F_DONE);
         }
     }
     // ---- queue append: one atomic per wave and queue ----
@@ -954,6 +968,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
     // no result.
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (1ull << lane) - 1ull;
+    {
+        // the live-path list of the next bounce, one atomic per wave
+        const unsigned long long m = __ballot(keep);
+        uint32_t base = 0;
+        if (lane == 0 && m) base = atomicAdd(&R.alive_n[b + 1], (uint32_t)__popcll(m));
+        base = __shfl(base, 0);
+        if (keep) R.alive[(b + 1) & 1][base + (uint32_t)__popcll(m & below)] = i;
+    }
     uint32_t ncl = (uint32_t)__popcll(__ballot(emit_ext)), nany = 0;
     for (int k = 0; k < K; ++k)
     {
@@ -1236,6 +1258,8 @@ static bsh_status ensure_work(bsh_context* c, size_t P, int K, int nlmax)
     BSH_HIP(w.hit.alloc(P));
     BSH_HIP(w.meta.alloc(P));
     BSH_HIP(w.nee_mask.alloc(P));
+    BSH_HIP(w.alive[0].alloc(P));
+    BSH_HIP(w.alive[1].alloc(P));
     size_t slots = P * (size_t)K;
     BSH_HIP(w.nee_a.alloc(slots));
     BSH_HIP(w.nee_b.alloc(slots));
@@ -1343,7 +1367,7 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
             full = full || hjobs[sg.job].flags != 0;
             wide = wide && hjobs[sg.job].s.width == 4;
         }
-        hipLaunchKernelGGL(k_zero, grid_for(4 * launches), dim3(kBlock), 0, st, w.counters.p, 4 * launches);
+        hipLaunchKernelGGL(k_zero, grid_for(w.counters.n), dim3(kBlock), 0, st, w.counters.p, (int)w.counters.n);
         // guided probe jobs: the network rows of their probes (vertex -1 of every probe path)
         // come first, one row per probe, so the probe rays can be drawn from the mixture
         std::vector<uint32_t> seg_row(ch.segs.size(), 0u);
@@ -1564,7 +1588,9 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     }
     BSH_TRY(ensure_work(c, P, K, nlmax));
     RenderWork& w = c->work;
-    BSH_HIP(w.counters.alloc((size_t)4 * (maxdepth + 1)));
+    // 4 queue counters per launch, then the live-path counts of bounces 0 .. maxdepth + 1;
+    // every chunk clears the whole buffer
+    BSH_HIP(w.counters.alloc((size_t)5 * (maxdepth + 1) + 1));
     const bool guided = gr && gr->guide;
     const bool record = gr && gr->records && gr->cap > 0;
     if (guided)
@@ -1602,6 +1628,9 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         R.qa_o[k] = w.qa_o[k].p; R.qa_d[k] = w.qa_d[k].p; R.qa_own[k] = w.qa_own[k].p;
     }
     R.counters = w.counters.p;
+    R.alive_n = w.counters.p + 4 * (maxdepth + 1);
+    R.alive[0] = w.alive[0].p;
+    R.alive[1] = w.alive[1].p;
     R.stats = w.stats.p;
     R.stk_ovf = w.stk_ovf.p;
     R.P = 0;
