@@ -57,6 +57,8 @@ constexpr int kTraceLds = 32;
 constexpr int kRecVerts = 4, kRecStride = 5 * kRecVerts;
 static_assert(kGuideSeg == kBlock, "guide row segments are the blocks of k_guide_inputs");
 constexpr int kMaxTraceBlocksPerCU = 5;
+// The shade kernel is held to 128 VGPRs: 4 waves per SIMD (at 129 it would run 3).
+constexpr int kShadeWavesPerSimd = 4;
 // Statistics buffer: 4 counters, then the guided-row partials, kRowStride apart (128 B).
 constexpr int kStatRows = 4, kRowParts = 64, kRowStride = 16, kStatTotal = kStatRows + kRowParts * kRowStride;
 // Idle lanes of a wave that trigger a refill in k_trace_refill (when the tuning leaves it 0).
@@ -597,9 +599,11 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 }
 
 // kFull = false compiles only the reference estimator (no quirk-free, guided or record
-// code), keeping the plain path's registers low.
-template <bool kFull>
-__global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
+// code), keeping the plain path's registers low. kOne: the run has a single job, so its
+// record and scene pointers are read with scalar loads at the start instead of per lane
+// after the path's meta word.
+template <bool kFull, bool kOne>
+__global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev R, int b, int Lout)
 {
     // Bounce 0 shades every path of the chunk; a later bounce only the paths the previous
     // shade left live, in the order it listed them (its wave order, so neighbouring paths
@@ -623,7 +627,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
         uint32_t fl = meta.y & 0xffu;
         int pb = (int)((meta.y >> 8) & 0xffu);
         int ncnt = (int)((meta.y >> 16) & 0xffu);
-        const uint32_t job = meta.y >> 24;
+        const uint32_t job = kOne ? 0u : meta.y >> 24;
         const JobDev& J = R.jobs[job];
         const SceneDev S = J.s;
         const int nl = S.nlights;
@@ -899,7 +903,6 @@ __global__ __launch_bounds__(kBlock) void k_shade(RenderDev R, int b, int Lout)
                                     emit_ext = true;
                                     ext_d = nd;
                                     R.thr[i] = st3(T, 0.f);
-                                    R.ray_o[i] = st3(h.p, 0.f);
                                     R.ray_d[i] = st3(nd, 0.f);
                                     fl |= F_EXT;
                                 }
@@ -1357,6 +1360,7 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
     RenderWork& w = c->work;
     const int trace_blocks = c->num_cus * std::max(1, tune.trace_blocks_per_cu);
     const int refill = tune.refill_lanes == 0 ? kDefaultRefill : tune.refill_lanes;
+    const bool one = hjobs.size() == 1;   // every path of the run belongs to job 0
     for (const Chunk& ch : chunks)
     {
         const int launches = ch.depth + 1;
@@ -1425,10 +1429,11 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
                 if (gs != BSH_OK) return gs;
             }
             if (T) BSH_HIP(hipEventRecord(T->next(), st));
-            if (full)
-                hipLaunchKernelGGL(k_shade<true>, pg, dim3(kBlock), 0, st, R, b, b + 1 <= ch.depth ? b + 1 : b);
-            else
-                hipLaunchKernelGGL(k_shade<false>, pg, dim3(kBlock), 0, st, R, b, b + 1 <= ch.depth ? b + 1 : b);
+            const int lo = b + 1 <= ch.depth ? b + 1 : b;
+            if (full && one) hipLaunchKernelGGL((k_shade<true, true>), pg, dim3(kBlock), 0, st, R, b, lo);
+            else if (full) hipLaunchKernelGGL((k_shade<true, false>), pg, dim3(kBlock), 0, st, R, b, lo);
+            else if (one) hipLaunchKernelGGL((k_shade<false, true>), pg, dim3(kBlock), 0, st, R, b, lo);
+            else hipLaunchKernelGGL((k_shade<false, false>), pg, dim3(kBlock), 0, st, R, b, lo);
             if (T) BSH_HIP(hipEventRecord(T->next(), st));
         }
         if (R.rec_out)
