@@ -267,29 +267,32 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
         tris[3 * k + 1] = make_float4(e1.x, e1.y, e1.z, 0.f);
         tris[3 * k + 2] = make_float4(e2.x, e2.y, e2.z, 0.f);
     }
-    std::vector<int4> shade(2 * std::max<size_t>(nt, 1));
+    // per-primitive shading records: vertices, normals and material copied in (kShadeF4 float4)
+    std::vector<float4> stri((size_t)kShadeF4 * std::max<size_t>(nt, 1), make_float4(0.f, 0.f, 0.f, 0.f));
     {
+        auto v4 = [&](const std::vector<float>& a, int32_t k, float w) {
+            return make_float4(a[3 * (size_t)k], a[3 * (size_t)k + 1], a[3 * (size_t)k + 2], w);
+        };
         size_t b = 0;
         for (size_t i = 0; i < nt; ++i)
         {
             while (b + 1 < s->bundles.size() && (int32_t)i >= s->bundles[b].tri_begin + s->bundles[b].tri_count) ++b;
-            shade[2 * i] = make_int4(s->tri_v[3 * i], s->tri_v[3 * i + 1], s->tri_v[3 * i + 2], s->tri_m[i]);
-            shade[2 * i + 1] = make_int4(s->tri_n[3 * i], s->tri_n[3 * i + 1], s->tri_n[3 * i + 2],
-                                         s->bundles.empty() ? -1 : bundle_light[b]);
+            const int32_t bl = s->bundles.empty() ? -1 : bundle_light[b];
+            const bsh_material& m = s->mats[s->tri_m[i]];
+            float4* r = &stri[(size_t)kShadeF4 * i];
+            int32_t mi = s->tri_m[i];
+            float mw, bw;
+            std::memcpy(&mw, &mi, 4);
+            std::memcpy(&bw, &bl, 4);
+            r[0] = v4(s->verts, s->tri_v[3 * i], mw);
+            r[1] = v4(s->verts, s->tri_v[3 * i + 1], bw);
+            r[2] = v4(s->verts, s->tri_v[3 * i + 2], 0.f);
+            r[3] = v4(s->normals, s->tri_n[3 * i], 0.f);
+            r[4] = v4(s->normals, s->tri_n[3 * i + 1], 0.f);
+            r[5] = v4(s->normals, s->tri_n[3 * i + 2], 0.f);
+            r[6] = make_float4(m.type, m.color[0], m.color[1], m.color[2]);
+            r[7] = make_float4(m.eta, m.alpha, 0.f, 0.f);
         }
-    }
-    auto pack3 = [](const std::vector<float>& v) {
-        std::vector<float4> o(std::max<size_t>(v.size() / 3, 1));
-        for (size_t i = 0; i < v.size() / 3; ++i) o[i] = make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.f);
-        return o;
-    };
-    std::vector<float4> verts4 = pack3(s->verts), norms4 = pack3(s->normals);
-    std::vector<float4> mats4(2 * std::max<size_t>(s->mats.size(), 1));
-    for (size_t i = 0; i < s->mats.size(); ++i)
-    {
-        const bsh_material& m = s->mats[i];
-        mats4[2 * i] = make_float4(m.type, m.color[0], m.color[1], m.color[2]);
-        mats4[2 * i + 1] = make_float4(m.eta, m.alpha, 0.f, 0.f);
     }
     const size_t node_f4 = bo.width == 4 ? bo.nodes4.size() * 8 : bo.nodes.size() * 4;
     BSH_HIP(s->d_nodes.alloc(node_f4));
@@ -297,14 +300,8 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
                       node_f4 * sizeof(float4), hipMemcpyHostToDevice));
     BSH_HIP(s->d_tris.alloc(tris.size()));
     BSH_HIP(hipMemcpy(s->d_tris.p, tris.data(), tris.size() * sizeof(float4), hipMemcpyHostToDevice));
-    BSH_HIP(s->d_shade.alloc(shade.size()));
-    BSH_HIP(hipMemcpy(s->d_shade.p, shade.data(), shade.size() * sizeof(int4), hipMemcpyHostToDevice));
-    BSH_HIP(s->d_verts.alloc(verts4.size()));
-    BSH_HIP(hipMemcpy(s->d_verts.p, verts4.data(), verts4.size() * sizeof(float4), hipMemcpyHostToDevice));
-    BSH_HIP(s->d_normals.alloc(norms4.size()));
-    BSH_HIP(hipMemcpy(s->d_normals.p, norms4.data(), norms4.size() * sizeof(float4), hipMemcpyHostToDevice));
-    BSH_HIP(s->d_mats.alloc(mats4.size()));
-    BSH_HIP(hipMemcpy(s->d_mats.p, mats4.data(), mats4.size() * sizeof(float4), hipMemcpyHostToDevice));
+    BSH_HIP(s->d_stri.alloc(stri.size()));
+    BSH_HIP(hipMemcpy(s->d_stri.p, stri.data(), stri.size() * sizeof(float4), hipMemcpyHostToDevice));
     BSH_HIP(s->d_lights.alloc(std::max<size_t>(s->h_lights.size(), 1)));
     if (!s->h_lights.empty())
         BSH_HIP(hipMemcpy(s->d_lights.p, s->h_lights.data(), s->h_lights.size() * sizeof(LightDev),

@@ -163,8 +163,7 @@ struct bsh_scene
     bool committed = false;
 
     bsh::DevBuf<float4> d_nodes;   // GpuNode (width 2) or GpuNode4 (width 4) as float4s
-    bsh::DevBuf<float4> d_tris, d_verts, d_normals, d_mats;
-    bsh::DevBuf<int4> d_shade;
+    bsh::DevBuf<float4> d_tris, d_stri;
     bsh::DevBuf<bsh::LightDev> d_lights;
     std::vector<bsh::LightDev> h_lights;
     float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {0.f, 0.f, 0.f};
@@ -180,10 +179,7 @@ struct bsh_scene
         s.nodes = d_nodes.p;
         s.width = width;
         s.tris = d_tris.p;
-        s.shade = d_shade.p;
-        s.verts = d_verts.p;
-        s.normals = d_normals.p;
-        s.mats = d_mats.p;
+        s.stri = d_stri.p;
         s.lights = d_lights.p;
         s.nlights = (int)lights.size();
         s.ntris = (int)tri_v.size() / 3;

@@ -292,10 +292,9 @@ __global__ __launch_bounds__(kBlock) void k_guide_inputs(RenderDev R, int b)
         {
             want = true;
             f3 p1, p2, p3, n1, n2, n3;
-            int mat, bl;
-            tri_verts(J.s, hr.x, p1, p2, p3, n1, n2, n3, mat, bl);
+            Mat m;
+            tri_verts(J.s, hr.x, p1, p2, p3, n1, n2, n3, m);
             Frame h = fill_hit(p1, p2, p3, n1, n2, n3, __int_as_float(hr.z), __int_as_float(hr.w));
-            Mat m = load_mat(J.s, mat);
             f3 wo = -ld3(R.ray_d[i]);
             f3 n = dot(h.n, wo) < 0.f ? -h.n : h.n;
             a = make_float4((h.p.x - J.glo[0]) * J.ginv[0], (h.p.y - J.glo[1]) * J.ginv[1],
@@ -624,6 +623,9 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
     if (have)
     {
         uint4 meta = R.meta[i];
+        // the extension hit is read with the meta word rather than after it: one level less
+        // in the chain of dependent loads (wasted only by paths with just NEE rays pending)
+        const int4 hr = R.hit[i];
         uint32_t fl = meta.y & 0xffu;
         int pb = (int)((meta.y >> 8) & 0xffu);
         int ncnt = (int)((meta.y >> 16) & 0xffu);
@@ -658,14 +660,14 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                         int Lid = (int)nb.z;
                         if (r.x >= 0)
                         {
-                            f3 p1, p2, p3, n1, n2, n3;
-                            int mat, bl;
-                            tri_verts(S, r.x, p1, p2, p3, n1, n2, n3, mat, bl);
                             // shadowhit.bundle->GetAreaLight() == &light (ditracer.cpp:202); quirk-free:
                             // the hit must be this light's own triangle
-                            bool mine = nq ? (r.x == S.lights[Lid].tri) : (bl == Lid);
+                            bool mine = nq ? (r.x == S.lights[Lid].tri) : (tri_bundle_light(S, r.x) == Lid);
                             if (mine)
                             {
+                                f3 p1, p2, p3, n1, n2, n3;
+                                Mat lm;
+                                tri_verts(S, r.x, p1, p2, p3, n1, n2, n3, lm);
                                 Frame hh = fill_hit(p1, p2, p3, n1, n2, n3, __int_as_float(r.z), __int_as_float(r.w));
                                 f3 wi = ld3(R.nee_wi[ks + 1]);
                                 f3 d = hh.p - ps;
@@ -673,7 +675,6 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                 if (dot(hh.n, -wi) > 0.f)
                                 {
                                     float d2inv = 1.f / sqnorm(d);
-                                    Mat lm = load_mat(S, mat);
                                     if (lm.type == 2) le = emissive_le(lm.c, hh.n, -wi) * (nq ? 1.f : d2inv);
                                     else le = mk(0.f, 0.f, 0.f) * d2inv;  // Material::GetLe default
                                 }
@@ -704,7 +705,6 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
             if (fl & F_EXT)
             {
                 fl &= ~F_EXT;
-                int4 hr = R.hit[i];
                 if (hr.x < 0)
                 {
                     if (b == 0)
@@ -717,10 +717,9 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                 else
                 {
                     f3 p1, p2, p3, n1, n2, n3;
-                    int mat, bl;
-                    tri_verts(S, hr.x, p1, p2, p3, n1, n2, n3, mat, bl);
+                    Mat m;
+                    tri_verts(S, hr.x, p1, p2, p3, n1, n2, n3, m);
                     Frame h = fill_hit(p1, p2, p3, n1, n2, n3, __int_as_float(hr.z), __int_as_float(hr.w));
-                    Mat m = load_mat(S, mat);
                     f3 wo = -ld3(R.ray_d[i]);
                     uint32_t base = meta.x;
                     f3 T = ld3(R.thr[i]);

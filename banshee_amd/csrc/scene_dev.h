@@ -46,14 +46,14 @@ struct LightDev
     float4 n[3];
 };
 
+constexpr int kShadeF4 = 8;   // float4 per primitive shading record
+
 struct SceneDev
 {
     const float4* nodes;      // GpuNode as 4 x float4
     const float4* tris;       // leaf order: (p1, prim bits), (e1, 0), (e2, 0)
-    const int4* shade;        // per prim: (v0, v1, v2, material), (n0, n1, n2, bundle light)
-    const float4* verts;
-    const float4* normals;
-    const float4* mats;       // per material: (type, c.xyz), (eta, alpha, 0, 0)
+    const float4* stri;       // per prim, kShadeF4 float4: p1 (w: material index), p2 (w: bundle
+                              // light), p3, n1, n2, n3, its material (type, c.xyz), (eta, alpha, 0, 0)
     const LightDev* lights;
     int nlights;
     int ntris;

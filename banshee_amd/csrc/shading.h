@@ -5,29 +5,30 @@
 
 namespace bsh {
 
-__device__ __forceinline__ Mat load_mat(const SceneDev& s, int m)
+// A primitive's shading record: its three vertices, three normals and its material, copied
+// from the scene's arrays at commit. The loads of one record are independent of each other,
+// so a hit reaches everything shading reads after one level of loads.
+__device__ __forceinline__ void tri_verts(const SceneDev& s, int prim, f3& p1, f3& p2, f3& p3, f3& n1, f3& n2, f3& n3,
+                                          Mat& m)
 {
-    float4 a = s.mats[2 * m], b = s.mats[2 * m + 1];
-    Mat r;
-    r.type = (int)a.x;
-    r.c = mk(a.y, a.z, a.w);
-    r.eta = b.x;
-    r.alpha = b.y;
-    return r;
+    const float4* r = s.stri + (size_t)kShadeF4 * prim;
+    const float4 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4], f = r[5], g = r[6], h = r[7];
+    p1 = ld3(a);
+    p2 = ld3(b);
+    p3 = ld3(c);
+    n1 = ld3(d);
+    n2 = ld3(e);
+    n3 = ld3(f);
+    m.type = (int)g.x;
+    m.c = mk(g.y, g.z, g.w);
+    m.eta = h.x;
+    m.alpha = h.y;
 }
 
-__device__ __forceinline__ void tri_verts(const SceneDev& s, int prim, f3& p1, f3& p2, f3& p3, f3& n1, f3& n2, f3& n3,
-                                          int& mat, int& blight)
+// The area light of the primitive's bundle, -1 when it has none.
+__device__ __forceinline__ int tri_bundle_light(const SceneDev& s, int prim)
 {
-    int4 a = s.shade[2 * prim], b = s.shade[2 * prim + 1];
-    p1 = ld3(s.verts[a.x]);
-    p2 = ld3(s.verts[a.y]);
-    p3 = ld3(s.verts[a.z]);
-    n1 = ld3(s.normals[b.x]);
-    n2 = ld3(s.normals[b.y]);
-    n3 = ld3(s.normals[b.z]);
-    mat = a.w;
-    blight = b.w;
+    return __float_as_int(s.stri[(size_t)kShadeF4 * prim + 1].w);
 }
 
 // AreaLight::GetPdf -> Mesh::GetPdfOnShape (arealight.cpp:55-58, mesh.cpp:354-378)
