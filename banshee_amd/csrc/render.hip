@@ -90,7 +90,9 @@ struct RenderDev
     int4* nee_res;
     float4 *qc_o[2], *qc_d[2], *qa_o[2], *qa_d[2];
     int *qc_own[2], *qa_own[2];
-    uint32_t* counters;            // per launch L: closest count, closest work, any count, any work
+    uint32_t* counters;            // per launch L: closest count, any count (one 64-bit word, so a
+                                   // shade wave claims both with one atomic), closest or joint work,
+                                   // any work
     uint32_t* alive[2];            // live paths: shade b reads list b & 1, writes list (b + 1) & 1
     uint32_t* alive_n;             // per bounce: length of its live-path list (b >= 1)
     unsigned long long* stats;     // closest, any, node visits, triangle tests, then kRowParts guided-row partials
@@ -119,10 +121,23 @@ __global__ __launch_bounds__(kBlock) void k_zero(uint32_t* p, int n)
 // Every path of a chunk enqueues exactly one primary ray at its own slot.
 __global__ void k_set_count(uint32_t* counters, unsigned long long* stats, uint32_t P)
 {
+    (void)stats;
+    if (threadIdx.x == 0) counters[0] = P;
+}
+
+// After a chunk: its closest and any-hit ray totals, from every launch's queue counts.
+__global__ void k_chunk_rays(const uint32_t* counters, int launches, unsigned long long* stats)
+{
     if (threadIdx.x == 0)
     {
-        counters[0] = P;
-        stats[0] += (unsigned long long)P;
+        unsigned long long c = 0, a = 0;
+        for (int L = 0; L < launches; ++L)
+        {
+            c += counters[4 * L + 0];
+            a += counters[4 * L + 1];
+        }
+        stats[0] += c;
+        stats[1] += a;
     }
 }
 
@@ -340,8 +355,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_queue(RenderDev R, int L, int 
     uint32_t vis_nodes = 0, vis_tris = 0;
     for (int pass = 0; pass < 2; ++pass)
     {
-        const uint32_t count = R.counters[4 * L + 2 * pass];
-        uint32_t* work = &R.counters[4 * L + 2 * pass + 1];
+        const uint32_t count = R.counters[4 * L + pass];
+        uint32_t* work = &R.counters[4 * L + 2 + pass];
         const float4* qo = pass == 0 ? R.qc_o[buf] : R.qa_o[buf];
         const float4* qd = pass == 0 ? R.qc_d[buf] : R.qa_d[buf];
         const int* qown = pass == 0 ? R.qc_own[buf] : R.qa_own[buf];
@@ -410,8 +425,8 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
     const int ostride = gridDim.x * kBlock;
     gu32* ovf = (gu32*)(R.stk_ovf + blockIdx.x * kBlock + threadIdx.x);
     const uint32_t nc = R.counters[4 * L + 0];
-    const uint32_t total = nc + R.counters[4 * L + 2];
-    uint32_t* work = &R.counters[4 * L + 1];
+    const uint32_t total = nc + R.counters[4 * L + 1];
+    uint32_t* work = &R.counters[4 * L + 2];
     const float kInf = __builtin_inff();
     uint32_t vis_nodes = 0, vis_tris = 0;
     bool active = false, any = false, exhausted = false;
@@ -985,16 +1000,15 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
         if (k & 1) ncl += c;
         else nany += c;
     }
-    uint32_t cbase = 0, abase = 0;
-    if (lane == 0)
-    {
-        if (ncl) cbase = atomicAdd(&R.counters[4 * Lout + 0], ncl);
-        if (nany) abase = atomicAdd(&R.counters[4 * Lout + 2], nany);
-        if (ncl) atomicAdd(&R.stats[0], (unsigned long long)ncl);
-        if (nany) atomicAdd(&R.stats[1], (unsigned long long)nany);
-    }
-    cbase = __shfl(cbase, 0);
-    abase = __shfl(abase, 0);
+    // both queues' slots with one 64-bit atomic: closest in the low word, any-hit in the high
+    // word (neither count reaches 2^32, so no carry crosses); the ray totals of the run's
+    // statistics are summed from these counters once per chunk
+    unsigned long long both = 0;
+    if (lane == 0 && (ncl | nany))
+        both = atomicAdd(reinterpret_cast<unsigned long long*>(&R.counters[4 * Lout]),
+                         ((unsigned long long)nany << 32) | ncl);
+    both = __shfl(both, 0);
+    const uint32_t cbase = (uint32_t)both, abase = (uint32_t)(both >> 32);
     if (ncl | nany)
     {
         const int buf = Lout & 1;
@@ -1435,6 +1449,7 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
             else hipLaunchKernelGGL((k_shade<false, false>), pg, dim3(kBlock), 0, st, R, b, lo);
             if (T) BSH_HIP(hipEventRecord(T->next(), st));
         }
+        hipLaunchKernelGGL(k_chunk_rays, dim3(1), dim3(64), 0, st, (const uint32_t*)w.counters.p, launches, w.stats.p);
         if (R.rec_out)
         {
             hipLaunchKernelGGL(k_rec_count, pg, dim3(kBlock), 0, st, R, w.rec_blk.p);
@@ -1592,9 +1607,10 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     }
     BSH_TRY(ensure_work(c, P, K, nlmax));
     RenderWork& w = c->work;
-    // 4 queue counters per launch, then the live-path counts of bounces 0 .. maxdepth + 1;
-    // every chunk clears the whole buffer
-    BSH_HIP(w.counters.alloc((size_t)5 * (maxdepth + 1) + 1));
+    // 4 queue counters per launch, then, from the next 256-B boundary (a cache line of their
+    // own), the live-path counts of bounces 0 .. maxdepth + 1; every chunk clears the buffer
+    const size_t alive_off = ((size_t)4 * (maxdepth + 1) + 63) / 64 * 64;
+    BSH_HIP(w.counters.alloc(alive_off + maxdepth + 2));
     const bool guided = gr && gr->guide;
     const bool record = gr && gr->records && gr->cap > 0;
     if (guided)
@@ -1632,7 +1648,7 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         R.qa_o[k] = w.qa_o[k].p; R.qa_d[k] = w.qa_d[k].p; R.qa_own[k] = w.qa_own[k].p;
     }
     R.counters = w.counters.p;
-    R.alive_n = w.counters.p + 4 * (maxdepth + 1);
+    R.alive_n = w.counters.p + alive_off;
     R.alive[0] = w.alive[0].p;
     R.alive[1] = w.alive[1].p;
     R.stats = w.stats.p;
