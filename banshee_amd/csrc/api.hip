@@ -490,6 +490,8 @@ bsh_status bsh_context_set_tuning(bsh_context* ctx, const bsh_tuning* t)
         return fail(BSH_ERR_INVALID, "bsh_context_set_tuning: trace_blocks_per_cu must be 0..5 (LDS limit)");
     if (t->refill_lanes < -1 || t->refill_lanes > 64)
         return fail(BSH_ERR_INVALID, "bsh_context_set_tuning: refill_lanes must be -1..64");
+    if (t->claim_rays != 0 && (t->claim_rays < 64 || t->claim_rays > 4096))
+        return fail(BSH_ERR_INVALID, "bsh_context_set_tuning: claim_rays must be 0 or 64..4096");
     ctx->tuning = *t;
     return BSH_OK;
 }
@@ -563,7 +565,7 @@ bsh_status bsh_autotune(bsh_context* ctx, const bsh_job* jobs, size_t njobs, con
         {
             if (blocks_per_cu[b] < 1 || blocks_per_cu[b] > 5 || paths_per_chunk[a] < 1)
                 return fail(BSH_ERR_INVALID, "bsh_autotune: candidate out of range");
-            bsh_tuning t{paths_per_chunk[a], blocks_per_cu[b], 0, ctx->tuning.refill_lanes};
+            bsh_tuning t{paths_per_chunk[a], blocks_per_cu[b], 0, ctx->tuning.refill_lanes, ctx->tuning.claim_rays};
             std::vector<JobDev> copy = hj;
             r = run_jobs(ctx, copy, false, &t, 0, nullptr);
             if (r != BSH_OK) return r;

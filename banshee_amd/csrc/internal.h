@@ -146,7 +146,7 @@ struct bsh_context
     std::mutex mu;                 // serialises work on the shared buffers below
     bsh::RenderWork work;
     bsh::DevBuf<bsh::JobDev> jobs;
-    bsh_tuning tuning{1 << 23, 5, 0, 0};
+    bsh_tuning tuning{1 << 23, 5, 0, 0, 0};
     bsh_render_stats stats{};
 };
 

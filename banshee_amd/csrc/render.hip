@@ -65,6 +65,11 @@ constexpr int kStatRows = 4, kRowParts = 64, kRowStride = 16, kStatTotal = kStat
 // Measured on the 3M-triangle terrain at 1080p: 8 lanes 733 Mrays/s, 16 1058, 32 1314,
 // 48 1381, 56 1393, 64 1353; whole 64-ray batches per wave (k_trace_queue) 1065.
 constexpr int kDefaultRefill = 56;
+// Queue entries a trace wave claims per atomic on the work counter when the run spans several
+// scenes and the tuning leaves claim_rays 0 (a single scene claims exactly the idle lanes).
+// Measured: 64 seeded scenes (1k-2M triangles) 2875 -> 3193 Mrays/s; on one 3M-triangle
+// scene 128 cost 3 % (the small late-bounce launches lose balance).
+constexpr uint32_t kClaimRays = 128;
 
 enum : uint32_t
 {
@@ -417,7 +422,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_queue(RenderDev R, int L, int 
 // wave may hold closest and any-hit rays of different scenes at once. Node visits and
 // accept rules are those of traverse4, so every ray's result is the same.
 template <bool kCount>
-__global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(RenderDev R, int L, int buf, int refill)
+__global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(RenderDev R, int L, int buf, int refill,
+                                                                               uint32_t claim_rays)
 {
     __shared__ uint32_t stk[kTraceLds * kBlock];
     const int lane = threadIdx.x & 63;
@@ -430,6 +436,8 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
     const float kInf = __builtin_inff();
     uint32_t vis_nodes = 0, vis_tris = 0;
     bool active = false, any = false, exhausted = false;
+    // the wave's reserve: queue entries [rb, re) it has claimed and not yet handed to a lane
+    uint32_t rb = 0, re = 0;
     f3 o = mk(0.f, 0.f, 0.f), d = mk(0.f, 0.f, 0.f);
     float ix = 0.f, iy = 0.f, iz = 0.f, ox = 0.f, oy = 0.f, oz = 0.f, tmin = 0.f, tcur = 0.f, hb1 = 0.f, hb2 = 0.f;
     int prim = -1, sp = 0;
@@ -439,17 +447,28 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
     for (;;)
     {
         // ---- refill the idle lanes (wave-uniform decision) ----
+        // Idle lanes take entries from the reserve first. A wave whose reserve is short claims
+        // claim_rays more with one atomic (0: exactly its idle lanes, so no reserve is kept):
+        // with many small scenes, whose rays finish fast, the counter then sees a fraction of
+        // the refills, and a wave still takes its rays in queue order.
         const unsigned long long idle = __ballot(!active);
-        const int nidle = __popcll(idle);
-        if (!exhausted && (nidle >= refill || nidle == 64))
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if ((!exhausted || rb < re) && (nidle >= (uint32_t)refill || nidle == 64u))
         {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(work, (uint32_t)nidle);
-            base = __shfl(base, 0);
-            if (base + (uint32_t)nidle >= total) exhausted = true;
+            const uint32_t from_res = min(re - rb, nidle);
+            const uint32_t want = claim_rays ? claim_rays : nidle;   // >= the lanes it must cover
+            uint32_t nb = 0;
+            const bool claim = from_res < nidle && !exhausted;
+            if (claim)
+            {
+                if (lane == 0) nb = atomicAdd(work, want);
+                nb = __builtin_amdgcn_readfirstlane(__shfl(nb, 0));
+                if (nb + want >= total) exhausted = true;
+            }
             if (!active)
             {
-                const uint32_t q = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                const uint32_t q = rank < from_res ? rb + rank : claim ? nb + (rank - from_res) : total;
                 if (q < total)
                 {
                     any = q >= nc;
@@ -488,6 +507,15 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
                     active = true;
                 }
             }
+            // the reserve left (rb <= re always: a claim hands out at most the idle lanes it
+            // was sized for)
+            if (claim)
+            {
+                rb = nb + (nidle - from_res);
+                re = nb + want;
+            }
+            else
+                rb += from_res;
         }
         if (__ballot(active) == 0ull) break;
         if (!active) continue;
@@ -1374,6 +1402,9 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
     const int trace_blocks = c->num_cus * std::max(1, tune.trace_blocks_per_cu);
     const int refill = tune.refill_lanes == 0 ? kDefaultRefill : tune.refill_lanes;
     const bool one = hjobs.size() == 1;   // every path of the run belongs to job 0
+    // per-atomic trace claims: the tuning's, else exactly the idle lanes on one scene and
+    // kClaimRays when the run's jobs trace several (R.nodes1 is set only for one scene)
+    const uint32_t claim = tune.claim_rays > 0 ? (uint32_t)tune.claim_rays : R.nodes1 ? 0u : kClaimRays;
     for (const Chunk& ch : chunks)
     {
         const int launches = ch.depth + 1;
@@ -1421,9 +1452,11 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
             if (wide)
             {
                 if (count)
-                    hipLaunchKernelGGL(k_trace_refill<true>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1, refill);
+                    hipLaunchKernelGGL(k_trace_refill<true>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1, refill,
+                                       claim);
                 else
-                    hipLaunchKernelGGL(k_trace_refill<false>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1, refill);
+                    hipLaunchKernelGGL(k_trace_refill<false>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1, refill,
+                                       claim);
             }
             else if (count)
                 hipLaunchKernelGGL(k_trace_queue<true>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1);

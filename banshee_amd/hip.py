@@ -69,7 +69,7 @@ class ProbeParams(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("paths_per_chunk", ctypes.c_int32), ("trace_blocks_per_cu", ctypes.c_int32),
-                ("use_graph", ctypes.c_int32), ("refill_lanes", ctypes.c_int32)]
+                ("use_graph", ctypes.c_int32), ("refill_lanes", ctypes.c_int32), ("claim_rays", ctypes.c_int32)]
 
 
 class Job(ctypes.Structure):
@@ -274,8 +274,8 @@ class Context:
         _check(lib().bsh_context_get_tuning(self.h, ctypes.byref(t)), "bsh_context_get_tuning")
         return {k: getattr(t, k) for k, _ in t._fields_ if k != "reserved"}
 
-    def set_tuning(self, paths_per_chunk=0, trace_blocks_per_cu=0, use_graph=0, refill_lanes=0):
-        t = Tuning(paths_per_chunk, trace_blocks_per_cu, use_graph, refill_lanes)
+    def set_tuning(self, paths_per_chunk=0, trace_blocks_per_cu=0, use_graph=0, refill_lanes=0, claim_rays=0):
+        t = Tuning(paths_per_chunk, trace_blocks_per_cu, use_graph, refill_lanes, claim_rays)
         _check(lib().bsh_context_set_tuning(self.h, ctypes.byref(t)), "bsh_context_set_tuning")
 
     def stats(self) -> dict:
@@ -289,7 +289,7 @@ class Context:
         t = None
         if tuning is not None:
             t = Tuning(tuning.get("paths_per_chunk", 0), tuning.get("trace_blocks_per_cu", 0),
-                       tuning.get("use_graph", 0), tuning.get("refill_lanes", 0))
+                       tuning.get("use_graph", 0), tuning.get("refill_lanes", 0), tuning.get("claim_rays", 0))
         _check(lib().bsh_batch_run(self.h, arr, c_size_t(len(jobs)), ctypes.byref(t) if t else None,
                                    c_void_p(stream)), "bsh_batch_run")
 
@@ -303,7 +303,7 @@ class Context:
                                   _p(bl) if bl.size else None, c_size_t(bl.size), ctypes.byref(best),
                                   ctypes.byref(ms)), "bsh_autotune")
         return {"paths_per_chunk": best.paths_per_chunk, "trace_blocks_per_cu": best.trace_blocks_per_cu,
-                "refill_lanes": best.refill_lanes}, ms.value
+                "refill_lanes": best.refill_lanes, "claim_rays": best.claim_rays}, ms.value
 
     def close(self):
         if self.h:

@@ -228,7 +228,7 @@ def batch_workload(args) -> dict:
     ctx.sync()
     dt = time.perf_counter() - t2
     # the same jobs one by one (bsh_render each, same tuning, no graph)
-    ctx.set_tuning(best["paths_per_chunk"], best["trace_blocks_per_cu"], 0, best["refill_lanes"])
+    ctx.set_tuning(best["paths_per_chunk"], best["trace_blocks_per_cu"], 0, best["refill_lanes"], best["claim_rays"])
     t3 = time.perf_counter()
     for j in jobs:
         j.scene.render(j.params.width, j.params.height, j.params.img_grid, j.params.max_depth, j.params.seed,

@@ -248,13 +248,18 @@ bsh_status bsh_sh_irradiance(bsh_context* ctx, int lmax, const float* coeffs, co
    hipGraph and launches it once. refill_lanes: on 4-wide BVHs each trace lane holds its own
    ray and a wave claims new rays once this many of its 64 lanes are idle (1..64; 0 = the
    default, 56); -1 traces in whole 64-ray batches per wave instead (also the form used when
-   a chunk holds a binary BVH). Results do not depend on it. */
+   a chunk holds a binary BVH). claim_rays: queue entries a trace wave claims per atomic on
+   the launch's work counter, keeping what its idle lanes do not take for its next refills
+   (64..4096; 0 = automatic: exactly the idle lanes when every job of the run shares one
+   scene, 128 when the run spans several, whose small scenes refill often enough for the
+   shared counter to limit). Results do not depend on refill_lanes or claim_rays. */
 typedef struct
 {
     int32_t paths_per_chunk;
     int32_t trace_blocks_per_cu;
     int32_t use_graph;
     int32_t refill_lanes;
+    int32_t claim_rays;
 } bsh_tuning;
 
 enum { BSH_JOB_RENDER = 0, BSH_JOB_BAKE_SH = 1 };

@@ -75,3 +75,25 @@ def test_autotune_picks_a_candidate(ctx):
     assert ms > 0
     assert ctx.tuning()["paths_per_chunk"] == best["paths_per_chunk"]
     ctx.set_tuning(1 << 21, 5, 0)
+
+
+@pytest.mark.parametrize("claim", [64, 200, 4096])
+def test_claim_size_changes_no_result(ctx, claim):
+    """Trace waves claiming claim_rays queue entries per atomic (a reserve kept per wave) give
+    the same batch outputs as the automatic claim; out-of-range sizes are rejected."""
+    from banshee_amd import hip
+    scenes, specs = _jobs(ctx)
+    ref = []
+    for spec in specs:
+        j = _make(spec)
+        ctx.batch_run([j], tuning={"paths_per_chunk": 1 << 14, "trace_blocks_per_cu": 2})
+        ref.append(j.result())
+    jobs = [_make(spec) for spec in specs]
+    ctx.batch_run(jobs, tuning={"paths_per_chunk": 1 << 14, "trace_blocks_per_cu": 2, "claim_rays": claim})
+    for j, r in zip(jobs, ref):
+        for a, b in zip(j.result(), r):
+            if a is not None:
+                np.testing.assert_array_equal(a, b)
+    with pytest.raises(hip.BshError):
+        ctx.set_tuning(0, 0, 0, 0, 32)
+    ctx.set_tuning(1 << 21, 5, 0)
