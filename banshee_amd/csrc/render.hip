@@ -1005,22 +1005,17 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
             keep = !(fl & F_DONE);
         }
     }
-    // ---- queue append: one atomic per wave and queue ----
+    // ---- queue append and the next bounce's live-path list: one atomic per block each ----
     // The trace kernel claims rays in queue order, so the wave writes them grouped: its
     // extension rays by direction octant, then its NEE rays slot by slot (the same light for
     // every lane of a run), instead of lane by lane. Rays claimed together then share an
     // origin region and a direction class. Hits are written per owner, so the order changes
-    // no result.
-    const int lane = threadIdx.x & 63;
+    // no result. The block's four waves add their counts in LDS and one thread claims the
+    // block's slots: every claim is a returning atomic on one of two hot cache lines, and
+    // per wave they limited the kernel.
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned long long below = (1ull << lane) - 1ull;
-    {
-        // the live-path list of the next bounce, one atomic per wave
-        const unsigned long long m = __ballot(keep);
-        uint32_t base = 0;
-        if (lane == 0 && m) base = atomicAdd(&R.alive_n[b + 1], (uint32_t)__popcll(m));
-        base = __shfl(base, 0);
-        if (keep) R.alive[(b + 1) & 1][base + (uint32_t)__popcll(m & below)] = i;
-    }
+    const unsigned long long mkeep = __ballot(keep);
     uint32_t ncl = (uint32_t)__popcll(__ballot(emit_ext)), nany = 0;
     for (int k = 0; k < K; ++k)
     {
@@ -1028,15 +1023,42 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
         if (k & 1) ncl += c;
         else nany += c;
     }
-    // both queues' slots with one 64-bit atomic: closest in the low word, any-hit in the high
-    // word (neither count reaches 2^32, so no carry crosses); the ray totals of the run's
-    // statistics are summed from these counters once per chunk
-    unsigned long long both = 0;
-    if (lane == 0 && (ncl | nany))
-        both = atomicAdd(reinterpret_cast<unsigned long long*>(&R.counters[4 * Lout]),
-                         ((unsigned long long)nany << 32) | ncl);
-    both = __shfl(both, 0);
-    const uint32_t cbase = (uint32_t)both, abase = (uint32_t)(both >> 32);
+    __shared__ uint32_t s_cnt[kBlock / 64][3];   // per wave: closest rays, any-hit rays, live paths
+    __shared__ unsigned long long s_qbase;
+    __shared__ uint32_t s_lbase;
+    if (lane == 0)
+    {
+        s_cnt[wave][0] = ncl;
+        s_cnt[wave][1] = nany;
+        s_cnt[wave][2] = (uint32_t)__popcll(mkeep);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        uint32_t tc = 0, ta = 0, tl = 0;
+        for (int w = 0; w < kBlock / 64; ++w)
+        {
+            tc += s_cnt[w][0];
+            ta += s_cnt[w][1];
+            tl += s_cnt[w][2];
+        }
+        // both queues with one 64-bit atomic: closest in the low word, any-hit in the high
+        // word (neither count reaches 2^32, so no carry crosses); the run's ray totals are
+        // summed from these counters once per chunk
+        s_qbase = (tc | ta) ? atomicAdd(reinterpret_cast<unsigned long long*>(&R.counters[4 * Lout]),
+                                        ((unsigned long long)ta << 32) | tc)
+                            : 0ull;
+        s_lbase = tl ? atomicAdd(&R.alive_n[b + 1], tl) : 0u;
+    }
+    __syncthreads();
+    uint32_t cbase = (uint32_t)s_qbase, abase = (uint32_t)(s_qbase >> 32), lbase = s_lbase;
+    for (int w = 0; w < wave; ++w)
+    {
+        cbase += s_cnt[w][0];
+        abase += s_cnt[w][1];
+        lbase += s_cnt[w][2];
+    }
+    if (keep) R.alive[(b + 1) & 1][lbase + (uint32_t)__popcll(mkeep & below)] = i;
     if (ncl | nany)
     {
         const int buf = Lout & 1;
