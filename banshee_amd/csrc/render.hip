@@ -293,18 +293,26 @@ __device__ __forceinline__ uint32_t path_of(const RenderDev& R, uint32_t kind, u
 // position in the scene box, roughness (GGX alpha, 1 otherwise), shading normal facing the
 // viewer. Each block packs its rows at the front of its own kBlock-row segment (a block-wide
 // prefix count) and stores how many it wrote; each path records its row. The block's count
-// is also added to the guided-row statistic (one integer atomic per block).
+// is also added to the guided-row statistic (one integer atomic per block). From bounce 1 on
+// the threads walk the live-path list the shade of this bounce will walk, not all paths.
 __global__ __launch_bounds__(kBlock) void k_guide_inputs(RenderDev R, int b)
 {
     __shared__ uint32_t wsum[kBlock / 64];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t i = t;
+    bool have = t < R.P;
+    if (b > 0)
+    {
+        have = t < R.alive_n[b];
+        i = have ? R.alive[b & 1][t] : 0u;
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float4 a = make_float4(0.f, 0.f, 0.f, 0.f), c = make_float4(0.f, 0.f, 0.f, 0.f);
     bool want = false;
     uint4 meta = make_uint4(0u, 0u, 0u, 0u);
-    if (i < R.P) meta = R.meta[i];
+    if (have) meta = R.meta[i];
     const JobDev& J = R.jobs[meta.y >> 24];
-    if (i < R.P && (J.flags & JOB_GUIDED) && (J.guide_bounces == 0 || b < J.guide_bounces) && (meta.y & F_EXT) &&
+    if (have && (J.flags & JOB_GUIDED) && (J.guide_bounces == 0 || b < J.guide_bounces) && (meta.y & F_EXT) &&
         !(meta.y & F_DONE))
     {
         int4 hr = R.hit[i];
@@ -338,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void k_guide_inputs(RenderDev R, int b)
         // lines (one shared counter serialises the blocks' atomics)
         if (total) atomicAdd(&R.stats[kStatRows + kRowStride * (blockIdx.x % kRowParts)], (unsigned long long)total);
     }
-    if (i >= R.P) return;
+    if (!have) return;
     int row = -1;
     if (want)
     {
