@@ -145,6 +145,14 @@ struct bsh_context
     size_t lds_per_cu = 0;
     std::mutex mu;                 // serialises work on the shared buffers below
     bsh::RenderWork work;
+    // second wavefront lane: runs with two or more chunks process alternate chunks on it, on
+    // its own stream, so one chunk's kernels fill the other's trace tail
+    bsh::RenderWork work2;
+    hipStream_t lane_stream = nullptr;
+    ~bsh_context()
+    {
+        if (lane_stream) (void)hipStreamDestroy(lane_stream);
+    }
     bsh::DevBuf<bsh::JobDev> jobs;
     bsh_tuning tuning{1 << 23, 5, 0, 0, 0};
     bsh_render_stats stats{};

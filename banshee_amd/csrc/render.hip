@@ -130,7 +130,8 @@ __global__ void k_set_count(uint32_t* counters, unsigned long long* stats, uint3
     if (threadIdx.x == 0) counters[0] = P;
 }
 
-// After a chunk: its closest and any-hit ray totals, from every launch's queue counts.
+// After a chunk: its closest and any-hit ray totals, from every launch's queue counts (added
+// atomically: the chunks of two lanes may finish at the same time).
 __global__ void k_chunk_rays(const uint32_t* counters, int launches, unsigned long long* stats)
 {
     if (threadIdx.x == 0)
@@ -141,8 +142,8 @@ __global__ void k_chunk_rays(const uint32_t* counters, int launches, unsigned lo
             c += counters[4 * L + 0];
             a += counters[4 * L + 1];
         }
-        stats[0] += c;
-        stats[1] += a;
+        atomicAdd(&stats[0], c);
+        atomicAdd(&stats[1], a);
     }
 }
 
@@ -1320,9 +1321,8 @@ __global__ __launch_bounds__(kBlock) void k_project_sh(RenderDev R, uint32_t job
 // ---------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------
-static bsh_status ensure_work(bsh_context* c, size_t P, int K, int nlmax)
+static bsh_status ensure_work(RenderWork& w, size_t P, int K, int nlmax)
 {
-    RenderWork& w = c->work;
     BSH_HIP(w.ray_o.alloc(P));
     BSH_HIP(w.ray_d.alloc(P));
     BSH_HIP(w.thr.alloc(P));
@@ -1424,19 +1424,57 @@ static dim3 grid_for(size_t n) { return dim3((uint32_t)std::max<size_t>(1, (n + 
 
 // Enqueues every kernel of a batch on `st`. With T == nullptr no events are recorded
 // (the form captured into a hipGraph).
-static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<Chunk>& chunks,
-                                const std::vector<JobDev>& hjobs, bool count, const bsh_tuning& tune, hipStream_t st,
-                                LaunchTimer* T, GuideRun* gr)
+// Orders a second stream after the first (fork) and the first after the second (join).
+struct ForkJoin
 {
-    RenderWork& w = c->work;
+    hipEvent_t f = nullptr, j = nullptr;
+    bsh_status fork(hipStream_t a, hipStream_t b)
+    {
+        BSH_HIP(hipEventCreateWithFlags(&f, hipEventDisableTiming));
+        BSH_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+        BSH_HIP(hipEventRecord(f, a));
+        BSH_HIP(hipStreamWaitEvent(b, f, 0));
+        return BSH_OK;
+    }
+    bsh_status join(hipStream_t a, hipStream_t b)
+    {
+        BSH_HIP(hipEventRecord(j, b));
+        BSH_HIP(hipStreamWaitEvent(a, j, 0));
+        return BSH_OK;
+    }
+    ~ForkJoin()
+    {
+        if (f) (void)hipEventDestroy(f);
+        if (j) (void)hipEventDestroy(j);
+    }
+};
+
+// One wavefront lane: its device view, its buffers and the stream its chunks run on.
+struct Lane
+{
+    RenderDev R;
+    RenderWork* w;
+    hipStream_t st;
+};
+
+// Enqueues every chunk of the run; chunk k goes to lane k mod nlanes.
+static bsh_status enqueue_batch(bsh_context* c, const Lane* lanes, int nlanes, const std::vector<Chunk>& chunks,
+                                const std::vector<JobDev>& hjobs, bool count, const bsh_tuning& tune, LaunchTimer* T,
+                                GuideRun* gr)
+{
     const int trace_blocks = c->num_cus * std::max(1, tune.trace_blocks_per_cu);
     const int refill = tune.refill_lanes == 0 ? kDefaultRefill : tune.refill_lanes;
     const bool one = hjobs.size() == 1;   // every path of the run belongs to job 0
     // per-atomic trace claims: the tuning's, else exactly the idle lanes on one scene and
     // kClaimRays when the run's jobs trace several (R.nodes1 is set only for one scene)
-    const uint32_t claim = tune.claim_rays > 0 ? (uint32_t)tune.claim_rays : R.nodes1 ? 0u : kClaimRays;
-    for (const Chunk& ch : chunks)
+    const uint32_t claim = tune.claim_rays > 0 ? (uint32_t)tune.claim_rays : lanes[0].R.nodes1 ? 0u : kClaimRays;
+    for (size_t ci = 0; ci < chunks.size(); ++ci)
     {
+        const Chunk& ch = chunks[ci];
+        const Lane& ln = lanes[ci % (size_t)nlanes];
+        RenderDev R = ln.R;
+        RenderWork& w = *ln.w;
+        hipStream_t st = ln.st;
         const int launches = ch.depth + 1;
         R.P = ch.P;
         bool full = false, wide = refill > 0;
@@ -1474,7 +1512,7 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
                 hipLaunchKernelGGL(k_raygen_probe, grid_for(n), dim3(kBlock), 0, st, R, sg.job, sg.u0, sg.nu, sg.c0,
                                    seg_row[k]);
         }
-        hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, st, w.counters.p, w.stats.p, ch.P);
+        hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, st, w.counters.p, R.stats, ch.P);
         dim3 pg = grid_for(ch.P);
         for (int b = 0; b <= ch.depth; ++b)
         {
@@ -1512,7 +1550,7 @@ static bsh_status enqueue_batch(bsh_context* c, RenderDev R, const std::vector<C
             else hipLaunchKernelGGL((k_shade<false, false>), pg, dim3(kBlock), 0, st, R, b, lo);
             if (T) BSH_HIP(hipEventRecord(T->next(), st));
         }
-        hipLaunchKernelGGL(k_chunk_rays, dim3(1), dim3(64), 0, st, (const uint32_t*)w.counters.p, launches, w.stats.p);
+        hipLaunchKernelGGL(k_chunk_rays, dim3(1), dim3(64), 0, st, (const uint32_t*)w.counters.p, launches, R.stats);
         if (R.rec_out)
         {
             hipLaunchKernelGGL(k_rec_count, pg, dim3(kBlock), 0, st, R, w.rec_blk.p);
@@ -1668,68 +1706,87 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         maxdepth = std::max(maxdepth, ch.depth);
         pairs += ch.depth + 1;
     }
-    BSH_TRY(ensure_work(c, P, K, nlmax));
-    RenderWork& w = c->work;
+    const bool guided = gr && gr->guide;
+    const bool record = gr && gr->records && gr->cap > 0;
+    // Two lanes when the run has chunks to alternate and nothing ties the chunks to one order
+    // (training records take a running offset).
+    const int nlanes = (!record && chunks.size() >= 2) ? 2 : 1;
+    if (nlanes == 2 && !c->lane_stream) BSH_HIP(hipStreamCreateWithFlags(&c->lane_stream, hipStreamNonBlocking));
+    bool probe_dir = false;
+    for (const JobDev& J : hjobs) probe_dir = probe_dir || (J.flags & JOB_PROBE_DIR) != 0;
     // 4 queue counters per launch, then, from the next 256-B boundary (a cache line of their
     // own), the live-path counts of bounces 0 .. maxdepth + 1; every chunk clears the buffer
     const size_t alive_off = ((size_t)4 * (maxdepth + 1) + 63) / 64 * 64;
-    BSH_HIP(w.counters.alloc(alive_off + maxdepth + 2));
-    const bool guided = gr && gr->guide;
-    const bool record = gr && gr->records && gr->cap > 0;
-    if (guided)
-    {
-        const size_t Pb = (P + kBlock - 1) / kBlock * kBlock;   // rows live in whole segments
-        BSH_HIP(w.gin.alloc(2 * Pb));
-        BSH_HIP(w.gout.alloc((size_t)kGUsed * Pb));
-        BSH_HIP(w.grow.alloc(P));
-        BSH_HIP(w.gseg.alloc(Pb / kBlock));
-    }
-    bool probe_dir = false;
-    for (const JobDev& J : hjobs) probe_dir = probe_dir || (J.flags & JOB_PROBE_DIR) != 0;
-    if (probe_dir) BSH_HIP(w.pdir.alloc(P));
-    if (record)
-    {
-        BSH_HIP(w.rec.alloc((size_t)kRecStride * P));
-        BSH_HIP(w.rec_count.alloc(1));
-        BSH_HIP(w.rec_blk.alloc((P + kBlock - 1) / kBlock));
-        BSH_HIP(w.rec_off.alloc((P + kBlock - 1) / kBlock));
-    }
     const size_t trace_threads = (size_t)c->num_cus * std::max(1, tune.trace_blocks_per_cu) * kBlock;
-    BSH_HIP(w.stk_ovf.alloc(trace_threads * (std::max(kMaxStack, kMaxStack4) - kTraceLds)));
     BSH_HIP(c->jobs.alloc(hjobs.size()));
     BSH_HIP(hipMemcpyAsync(c->jobs.p, hjobs.data(), hjobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, st));
+    BSH_TRY(ensure_work(c->work, P, K, nlmax));   // lane 0's buffers also hold the run's statistics
+    RenderWork& w = c->work;
 
-    RenderDev R;
-    R.jobs = c->jobs.p;
-    R.Kshift = Kshift;
-    R.ray_o = w.ray_o.p; R.ray_d = w.ray_d.p; R.thr = w.thr.p; R.thr_nee = w.thr_nee.p; R.rad = w.rad.p; R.out = w.out.p;
-    R.hit = w.hit.p; R.meta = w.meta.p; R.nee_mask = w.nee_mask.p;
-    R.nee_a = w.nee_a.p; R.nee_b = w.nee_b.p; R.nee_wi = w.nee_wi.p; R.nee_res = w.nee_res.p;
-    for (int k = 0; k < 2; ++k)
+    // every lane's buffers and device view (the statistics are lane 0's, shared)
+    auto setup_lane = [&](RenderWork& lw, RenderDev& R) -> bsh_status {
+        BSH_TRY(ensure_work(lw, P, K, nlmax));
+        BSH_HIP(lw.counters.alloc(alive_off + maxdepth + 2));
+        if (guided)
+        {
+            const size_t Pb = (P + kBlock - 1) / kBlock * kBlock;   // rows live in whole segments
+            BSH_HIP(lw.gin.alloc(2 * Pb));
+            BSH_HIP(lw.gout.alloc((size_t)kGUsed * Pb));
+            BSH_HIP(lw.grow.alloc(P));
+            BSH_HIP(lw.gseg.alloc(Pb / kBlock));
+        }
+        if (probe_dir) BSH_HIP(lw.pdir.alloc(P));
+        if (record)
+        {
+            BSH_HIP(lw.rec.alloc((size_t)kRecStride * P));
+            BSH_HIP(lw.rec_count.alloc(1));
+            BSH_HIP(lw.rec_blk.alloc((P + kBlock - 1) / kBlock));
+            BSH_HIP(lw.rec_off.alloc((P + kBlock - 1) / kBlock));
+        }
+        BSH_HIP(lw.stk_ovf.alloc(trace_threads * (std::max(kMaxStack, kMaxStack4) - kTraceLds)));
+        R.jobs = c->jobs.p;
+        R.Kshift = Kshift;
+        R.ray_o = lw.ray_o.p; R.ray_d = lw.ray_d.p; R.thr = lw.thr.p; R.thr_nee = lw.thr_nee.p; R.rad = lw.rad.p;
+        R.out = lw.out.p;
+        R.hit = lw.hit.p; R.meta = lw.meta.p; R.nee_mask = lw.nee_mask.p;
+        R.nee_a = lw.nee_a.p; R.nee_b = lw.nee_b.p; R.nee_wi = lw.nee_wi.p; R.nee_res = lw.nee_res.p;
+        for (int k = 0; k < 2; ++k)
+        {
+            R.qc_o[k] = lw.qc_o[k].p; R.qc_d[k] = lw.qc_d[k].p; R.qc_own[k] = lw.qc_own[k].p;
+            R.qa_o[k] = lw.qa_o[k].p; R.qa_d[k] = lw.qa_d[k].p; R.qa_own[k] = lw.qa_own[k].p;
+        }
+        R.counters = lw.counters.p;
+        R.alive_n = lw.counters.p + alive_off;
+        R.alive[0] = lw.alive[0].p;
+        R.alive[1] = lw.alive[1].p;
+        R.stats = w.stats.p;
+        R.stk_ovf = lw.stk_ovf.p;
+        R.P = 0;
+        R.gin = guided ? lw.gin.p : nullptr;
+        R.gout = guided ? lw.gout.p : nullptr;
+        R.grow = guided ? lw.grow.p : nullptr;
+        R.gseg = guided ? lw.gseg.p : nullptr;
+        R.pdir = probe_dir ? lw.pdir.p : nullptr;
+        R.nodes1 = hjobs.empty() ? nullptr : hjobs[0].s.nodes;
+        R.tris1 = hjobs.empty() ? nullptr : hjobs[0].s.tris;
+        for (const JobDev& J : hjobs)
+            if (J.s.nodes != R.nodes1 || J.s.tris != R.tris1) R.nodes1 = R.tris1 = nullptr;
+        R.rec = record ? lw.rec.p : nullptr;
+        R.rec_out = record ? gr->records : nullptr;
+        R.rec_cap = record ? (uint32_t)std::min<size_t>(gr->cap, 0xffffffffu) : 0u;
+        R.rec_count = record ? lw.rec_count.p : nullptr;
+        return BSH_OK;
+    };
+    Lane lanes[2];
+    lanes[0].w = &c->work;
+    lanes[0].st = st;
+    BSH_TRY(setup_lane(c->work, lanes[0].R));
+    if (nlanes == 2)
     {
-        R.qc_o[k] = w.qc_o[k].p; R.qc_d[k] = w.qc_d[k].p; R.qc_own[k] = w.qc_own[k].p;
-        R.qa_o[k] = w.qa_o[k].p; R.qa_d[k] = w.qa_d[k].p; R.qa_own[k] = w.qa_own[k].p;
+        lanes[1].w = &c->work2;
+        lanes[1].st = c->lane_stream;
+        BSH_TRY(setup_lane(c->work2, lanes[1].R));
     }
-    R.counters = w.counters.p;
-    R.alive_n = w.counters.p + alive_off;
-    R.alive[0] = w.alive[0].p;
-    R.alive[1] = w.alive[1].p;
-    R.stats = w.stats.p;
-    R.stk_ovf = w.stk_ovf.p;
-    R.P = 0;
-    R.gin = guided ? w.gin.p : nullptr;
-    R.gout = guided ? w.gout.p : nullptr;
-    R.grow = guided ? w.grow.p : nullptr;
-    R.gseg = guided ? w.gseg.p : nullptr;
-    R.pdir = probe_dir ? w.pdir.p : nullptr;
-    R.nodes1 = hjobs.empty() ? nullptr : hjobs[0].s.nodes;
-    R.tris1 = hjobs.empty() ? nullptr : hjobs[0].s.tris;
-    for (const JobDev& J : hjobs)
-        if (J.s.nodes != R.nodes1 || J.s.tris != R.tris1) R.nodes1 = R.tris1 = nullptr;
-    R.rec = record ? w.rec.p : nullptr;
-    R.rec_out = record ? gr->records : nullptr;
-    R.rec_cap = record ? (uint32_t)std::min<size_t>(gr->cap, 0xffffffffu) : 0u;
-    R.rec_count = record ? w.rec_count.p : nullptr;
     if (record) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, w.rec_count.p, 1);
 
     hipLaunchKernelGGL(k_zero, grid_for(2 * kStatTotal), dim3(kBlock), 0, st, (uint32_t*)w.stats.p, 2 * kStatTotal);
@@ -1754,7 +1811,14 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess)
         {
-            rc = enqueue_batch(c, R, chunks, hjobs, count, tune, cs, nullptr, gr);
+            // the second lane joins the capture through an event recorded on the capture stream
+            // and is joined back before it ends, so the graph holds both lanes' chains
+            Lane gl[2] = {lanes[0], lanes[1]};
+            gl[0].st = cs;
+            ForkJoin fj;
+            if (nlanes == 2) rc = fj.fork(cs, c->lane_stream);
+            if (rc == BSH_OK) rc = enqueue_batch(c, gl, nlanes, chunks, hjobs, count, tune, nullptr, gr);
+            if (rc == BSH_OK && nlanes == 2) rc = fj.join(cs, c->lane_stream);
             hipError_t e2 = hipStreamEndCapture(cs, &graph);
             if (rc == BSH_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
         }
@@ -1795,7 +1859,12 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         LaunchTimer T;
         BSH_TRY(T.init(pairs));
         BSH_HIP(hipEventRecord(T.next(), st));
-        BSH_TRY(enqueue_batch(c, R, chunks, hjobs, count, tune, st, &T, gr));
+        // the second lane starts after everything before it on `st` (job table, cleared
+        // statistics) and `st` continues after both lanes
+        ForkJoin fj;
+        if (nlanes == 2) BSH_TRY(fj.fork(st, c->lane_stream));
+        BSH_TRY(enqueue_batch(c, lanes, nlanes, chunks, hjobs, count, tune, &T, gr));
+        if (nlanes == 2) BSH_TRY(fj.join(st, c->lane_stream));
         BSH_HIP(hipEventRecord(T.next(), st));
         BSH_HIP(hipEventSynchronize(T.evs[T.ei - 1]));
         // per pass: before trace, after trace, after the guide inputs, after inference, after shade
