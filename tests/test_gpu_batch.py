@@ -94,6 +94,7 @@ def test_claim_size_changes_no_result(ctx, claim):
         for a, b in zip(j.result(), r):
             if a is not None:
                 np.testing.assert_array_equal(a, b)
+    old = ctx.tuning()
     with pytest.raises(hip.BshError):
         ctx.set_tuning(0, 0, 0, 0, 32)
-    ctx.set_tuning(1 << 21, 5, 0)
+    assert ctx.tuning() == old

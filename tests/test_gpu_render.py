@@ -63,3 +63,27 @@ def test_full_cornell_frame(ctx, cornell):
     assert st["rays_any"] == pytest.approx(ref_rays[1], rel=0.01)
     assert st["rays_closest"] <= ref_rays[0]
     assert st["rays_closest"] >= 0.9 * ref_rays[0]
+
+
+@pytest.mark.parametrize("size,paths", [(256, 50_000), (256, 1 << 13), (512, 1 << 20)])
+def test_chunking_and_lanes_change_no_result(ctx, terrain100k, size, paths):
+    """The same frame with the default chunking (256^2 x 16 spp: one chunk; 512^2 x 16 spp =
+    2^22 paths: split in two for the two wavefront lanes) and split into many chunks
+    (alternate chunks on the two lanes and streams) gives the same sums bit for bit, and
+    the same ray counts."""
+    from banshee_amd import hip
+    s = hip.Scene(ctx, terrain100k, width=4)
+    old = ctx.tuning()
+    try:
+        ctx.set_tuning(0, 5, 0)
+        S0, Q0, N0, _ = s.render(size, size, 4, 6, 11)
+        st0 = s.stats()
+        ctx.set_tuning(paths, 5, 0)
+        S1, Q1, N1, _ = s.render(size, size, 4, 6, 11)
+        st1 = s.stats()
+    finally:
+        ctx.set_tuning(**old)
+    np.testing.assert_array_equal(S0, S1)
+    np.testing.assert_array_equal(Q0, Q1)
+    np.testing.assert_array_equal(N0, N1)
+    assert (st0["rays_closest"], st0["rays_any"]) == (st1["rays_closest"], st1["rays_any"])
