@@ -1533,7 +1533,8 @@ static bsh_status enqueue_batch(bsh_context* c, const Lane* lanes, int nlanes, c
             else
                 hipLaunchKernelGGL(k_trace_queue<false>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1);
             if (T) BSH_HIP(hipEventRecord(T->next(), st));
-            const bool infer = gr && gr->guide && b < ch.depth && (gr->bounces == 0 || b < gr->bounces);
+            // no inference at the chunk's last bounce: no job of it samples a continuation there
+            const bool infer = gr && gr->guide && b + 1 < ch.depth && (gr->bounces == 0 || b < gr->bounces);
             if (infer)
             {
                 hipLaunchKernelGGL(k_guide_inputs, pg, dim3(kBlock), 0, st, R, b);
