@@ -1426,28 +1426,36 @@ static dim3 grid_for(size_t n) { return dim3((uint32_t)std::max<size_t>(1, (n + 
 
 // Enqueues every kernel of a batch on `st`. With T == nullptr no events are recorded
 // (the form captured into a hipGraph).
-// Orders a second stream after the first (fork) and the first after the second (join).
+// Orders further streams after a first one (fork) and the first after all of them (join).
 struct ForkJoin
 {
-    hipEvent_t f = nullptr, j = nullptr;
-    bsh_status fork(hipStream_t a, hipStream_t b)
+    static constexpr int kMax = bsh_context::kLanes - 1;
+    hipEvent_t f = nullptr, j[kMax] = {};
+    bsh_status fork(hipStream_t a, const hipStream_t* b, int n)
     {
         BSH_HIP(hipEventCreateWithFlags(&f, hipEventDisableTiming));
-        BSH_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
         BSH_HIP(hipEventRecord(f, a));
-        BSH_HIP(hipStreamWaitEvent(b, f, 0));
+        for (int k = 0; k < n; ++k)
+        {
+            BSH_HIP(hipEventCreateWithFlags(&j[k], hipEventDisableTiming));
+            BSH_HIP(hipStreamWaitEvent(b[k], f, 0));
+        }
         return BSH_OK;
     }
-    bsh_status join(hipStream_t a, hipStream_t b)
+    bsh_status join(hipStream_t a, const hipStream_t* b, int n)
     {
-        BSH_HIP(hipEventRecord(j, b));
-        BSH_HIP(hipStreamWaitEvent(a, j, 0));
+        for (int k = 0; k < n; ++k)
+        {
+            BSH_HIP(hipEventRecord(j[k], b[k]));
+            BSH_HIP(hipStreamWaitEvent(a, j[k], 0));
+        }
         return BSH_OK;
     }
     ~ForkJoin()
     {
         if (f) (void)hipEventDestroy(f);
-        if (j) (void)hipEventDestroy(j);
+        for (hipEvent_t e : j)
+            if (e) (void)hipEventDestroy(e);
     }
 };
 
@@ -1731,8 +1739,9 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     const bool record = gr && gr->records && gr->cap > 0;
     // Two lanes when the run has chunks to alternate and nothing ties the chunks to one order
     // (training records take a running offset).
-    const int nlanes = (!record && chunks.size() >= 2) ? 2 : 1;
-    if (nlanes == 2 && !c->lane_stream) BSH_HIP(hipStreamCreateWithFlags(&c->lane_stream, hipStreamNonBlocking));
+    const int nlanes = record ? 1 : (int)std::min<size_t>(bsh_context::kLanes, chunks.size());
+    for (int k = 0; k + 1 < nlanes; ++k)
+        if (!c->lane_stream[k]) BSH_HIP(hipStreamCreateWithFlags(&c->lane_stream[k], hipStreamNonBlocking));
     bool probe_dir = false;
     for (const JobDev& J : hjobs) probe_dir = probe_dir || (J.flags & JOB_PROBE_DIR) != 0;
     // 4 queue counters per launch, then, from the next 256-B boundary (a cache line of their
@@ -1798,15 +1807,15 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         R.rec_count = record ? lw.rec_count.p : nullptr;
         return BSH_OK;
     };
-    Lane lanes[2];
+    Lane lanes[bsh_context::kLanes];
     lanes[0].w = &c->work;
     lanes[0].st = st;
     BSH_TRY(setup_lane(c->work, lanes[0].R));
-    if (nlanes == 2)
+    for (int k = 1; k < nlanes; ++k)
     {
-        lanes[1].w = &c->work2;
-        lanes[1].st = c->lane_stream;
-        BSH_TRY(setup_lane(c->work2, lanes[1].R));
+        lanes[k].w = &c->lane_work[k - 1];
+        lanes[k].st = c->lane_stream[k - 1];
+        BSH_TRY(setup_lane(c->lane_work[k - 1], lanes[k].R));
     }
     if (record) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, w.rec_count.p, 1);
 
@@ -1834,12 +1843,13 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         {
             // the second lane joins the capture through an event recorded on the capture stream
             // and is joined back before it ends, so the graph holds both lanes' chains
-            Lane gl[2] = {lanes[0], lanes[1]};
+            Lane gl[bsh_context::kLanes];
+            for (int k = 0; k < nlanes; ++k) gl[k] = lanes[k];
             gl[0].st = cs;
             ForkJoin fj;
-            if (nlanes == 2) rc = fj.fork(cs, c->lane_stream);
+            if (nlanes > 1) rc = fj.fork(cs, c->lane_stream, nlanes - 1);
             if (rc == BSH_OK) rc = enqueue_batch(c, gl, nlanes, chunks, hjobs, count, tune, nullptr, gr);
-            if (rc == BSH_OK && nlanes == 2) rc = fj.join(cs, c->lane_stream);
+            if (rc == BSH_OK && nlanes > 1) rc = fj.join(cs, c->lane_stream, nlanes - 1);
             hipError_t e2 = hipStreamEndCapture(cs, &graph);
             if (rc == BSH_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
         }
@@ -1883,9 +1893,9 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         // the second lane starts after everything before it on `st` (job table, cleared
         // statistics) and `st` continues after both lanes
         ForkJoin fj;
-        if (nlanes == 2) BSH_TRY(fj.fork(st, c->lane_stream));
+        if (nlanes > 1) BSH_TRY(fj.fork(st, c->lane_stream, nlanes - 1));
         BSH_TRY(enqueue_batch(c, lanes, nlanes, chunks, hjobs, count, tune, &T, gr));
-        if (nlanes == 2) BSH_TRY(fj.join(st, c->lane_stream));
+        if (nlanes > 1) BSH_TRY(fj.join(st, c->lane_stream, nlanes - 1));
         BSH_HIP(hipEventRecord(T.next(), st));
         BSH_HIP(hipEventSynchronize(T.evs[T.ei - 1]));
         // per pass: before trace, after trace, after the guide inputs, after inference, after shade
