@@ -158,7 +158,7 @@ struct bsh_context
             if (s) (void)hipStreamDestroy(s);
     }
     bsh::DevBuf<bsh::JobDev> jobs;
-    bsh_tuning tuning{1 << 23, 5, 0, 0, 0};
+    bsh_tuning tuning{1 << 23, 0, 0, 0, 0};
     bsh_render_stats stats{};
 };
 

@@ -57,6 +57,7 @@ constexpr int kTraceLds = 32;
 constexpr int kRecVerts = 4, kRecStride = 5 * kRecVerts;
 static_assert(kGuideSeg == kBlock, "guide row segments are the blocks of k_guide_inputs");
 constexpr int kMaxTraceBlocksPerCU = 5;
+constexpr int kTraceBlocksPerLane = 3;   // the default when two lanes share more than two chunks
 // The shade kernel is held to 128 VGPRs: 4 waves per SIMD (at 129 it would run 3).
 constexpr int kShadeWavesPerSimd = 4;
 // Statistics buffer: 4 counters, then the guided-row partials, kRowStride apart (128 B).
@@ -1691,7 +1692,6 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     BSH_HIP(hipSetDevice(c->device));
     bsh_tuning tune = tune_in ? *tune_in : c->tuning;
     if (tune.paths_per_chunk <= 0) tune.paths_per_chunk = 1 << 23;
-    if (tune.trace_blocks_per_cu <= 0) tune.trace_blocks_per_cu = kMaxTraceBlocksPerCU;
     int nlmax = 1;
     uint32_t maxunit = 1;
     uint64_t samples = 0;
@@ -1740,6 +1740,12 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     // Two lanes when the run has chunks to alternate and nothing ties the chunks to one order
     // (training records take a running offset).
     const int nlanes = record ? 1 : (int)std::min<size_t>(bsh_context::kLanes, chunks.size());
+    // persistent trace workgroups per CU: all of them on one lane, or when two lanes hold one
+    // chunk each and run side by side from start to end (a 4096-probe bake: 5 per CU 1525
+    // Mrays/s, 3 1477); with more chunks each lane's grid leaves room for the other lane's
+    // kernels in its drain (1080p frame, 16 chunks: 3 per CU 2515, 4 2483, 5 2468, 2 2415)
+    if (tune.trace_blocks_per_cu <= 0)
+        tune.trace_blocks_per_cu = nlanes > 1 && chunks.size() > 2 ? kTraceBlocksPerLane : kMaxTraceBlocksPerCU;
     for (int k = 0; k + 1 < nlanes; ++k)
         if (!c->lane_stream[k]) BSH_HIP(hipStreamCreateWithFlags(&c->lane_stream[k], hipStreamNonBlocking));
     bool probe_dir = false;

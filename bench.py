@@ -147,8 +147,9 @@ def time_to_rmse(ctx, scene, guide, max_passes, seed0) -> dict:
 def make_context(args):
     from banshee_amd import hip
     ctx = hip.Context(0)
-    if args.refill_lanes or args.paths_per_chunk:
-        ctx.set_tuning(paths_per_chunk=args.paths_per_chunk, refill_lanes=args.refill_lanes)
+    if args.refill_lanes or args.paths_per_chunk or args.trace_blocks:
+        ctx.set_tuning(paths_per_chunk=args.paths_per_chunk, trace_blocks_per_cu=args.trace_blocks,
+                       refill_lanes=args.refill_lanes)
     return ctx
 
 
@@ -350,6 +351,8 @@ def main():
     ap.add_argument("--refill-lanes", type=int, default=0,
                     help="idle lanes per wave that trigger a ray refill in the 4-wide trace kernel "
                          "(0 = library default, -1 = whole 64-ray batches per wave)")
+    ap.add_argument("--trace-blocks", type=int, default=0,
+                    help="persistent trace workgroups per CU, 1..5 (0 = library default, 5)")
     ap.add_argument("--guided-only", action="store_true",
                     help="run only the guided-vs-plain comparison and print its JSON (for profiling)")
     ap.add_argument("--workload", default="gi", choices=("gi", "batch", "probes"),
