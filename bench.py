@@ -80,7 +80,7 @@ def cpu_baseline(sc_full_args) -> dict | None:
 G_RES, G_DEPTH, G_TARGET = 1024, 8, 0.02
 G_PASS_GRID, G_CHECK = 2, 1               # 4 spp per pass; RMSE estimated after every pass (outside the clock)
 G_ALPHA, G_WARM, G_LR, G_BATCH = 0.5, 4, 5e-3, 1 << 18
-G_STEPS, G_STEPS_LATE, G_LATE = 8, 0, 10   # Adam steps per pass: G_STEPS, from pass G_LATE on G_STEPS_LATE
+G_STEPS, G_STEPS_LATE, G_LATE = 10, 0, 8   # Adam steps per pass: G_STEPS, from pass G_LATE on G_STEPS_LATE
 # (passes that train no step write no records)
 G_BOUNCES = 0                              # guided path vertices (0 = all)
 # the BSDF fraction falls linearly from 1 to G_ALPHA over the first G_WARM passes, while the
