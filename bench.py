@@ -213,7 +213,7 @@ def batch_workload(args) -> dict:
     setup_s = time.time() - t0
     tris = sum(s.info()["tris"] for s in scenes)
     t1 = time.time()
-    best, best_ms = ctx.autotune(jobs, paths=[1 << 21, 1 << 22, 1 << 23], blocks=[3, 4, 5])
+    best, best_ms = ctx.autotune(jobs, paths=[1 << 21, 1 << 22, 1 << 23], blocks=[2, 3, 4, 5])
     tune_s = time.time() - t1
     tuning = dict(best, use_graph=1)
     for _ in range(args.warmup):
