@@ -22,7 +22,7 @@ LIB = os.path.join(HERE, "libbanshee_hip.so")
 ROOT = os.path.dirname(HERE)
 
 HIP_SOURCES = ["api.hip", "trace.hip", "render.hip", "diag.hip", "sh.hip", "guide.hip"]
-CXX_SOURCES = ["bvh_build.cpp"]
+CXX_SOURCES = ["bvh_build.cpp", "obj_import.cpp"]
 
 ARCH = os.environ.get("BANSHEE_ARCH", "gfx950")
 

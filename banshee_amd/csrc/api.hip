@@ -29,6 +29,23 @@ bsh_status hip_fail(hipError_t e, const char* what)
     return BSH_ERR_HIP;
 }
 
+// Every launch-shape parameter a tuning may carry, checked before any work is enqueued:
+// trace workgroups per CU are bounded by the trace kernel's LDS stack; a claim must cover a
+// whole wave's idle lanes (64), or a wave's reserve would hand out entries past its claim.
+bsh_status validate_tuning(const bsh_tuning& t, const char* who)
+{
+    const std::string w(who);
+    if (t.paths_per_chunk < 0) return fail(BSH_ERR_INVALID, w + ": paths_per_chunk must be >= 0");
+    if (t.trace_blocks_per_cu < 0 || t.trace_blocks_per_cu > 5)
+        return fail(BSH_ERR_INVALID, w + ": trace_blocks_per_cu must be 0..5 (LDS limit)");
+    if (t.refill_lanes < -1 || t.refill_lanes > 64) return fail(BSH_ERR_INVALID, w + ": refill_lanes must be -1..64");
+    if (t.claim_rays != 0 && (t.claim_rays < 64 || t.claim_rays > 4096))
+        return fail(BSH_ERR_INVALID, w + ": claim_rays must be 0 or 64..4096");
+    if (t.use_graph != 0 && t.use_graph != 1) return fail(BSH_ERR_INVALID, w + ": use_graph must be 0 or 1");
+    if (t.lanes < 0 || t.lanes > bsh_context::kLanes) return fail(BSH_ERR_INVALID, w + ": lanes must be 0, 1 or 2");
+    return BSH_OK;
+}
+
 CameraDev make_camera(const bsh_camera& c)
 {
     // PerscpectiveCamera ctor, perspective_camera.cpp:5-23
@@ -306,6 +323,28 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
     if (!s->h_lights.empty())
         BSH_HIP(hipMemcpy(s->d_lights.p, s->h_lights.data(), s->h_lights.size() * sizeof(LightDev),
                           hipMemcpyHostToDevice));
+    // the light table's power CDF (quirk-free light picks): power ~ area x luminance(e), summed in
+    // double; lights of no power are never picked unless every light has none (then uniform)
+    {
+        const size_t nl = s->h_lights.size();
+        std::vector<double> pw(nl);
+        double tot = 0.0;
+        for (size_t l = 0; l < nl; ++l)
+        {
+            const LightDev& d = s->h_lights[l];
+            pw[l] = (double)d.area * std::max(0.0, 0.2126 * d.e.x + 0.7152 * d.e.y + 0.0722 * d.e.z);
+            tot += pw[l];
+        }
+        std::vector<float> cdf(std::max<size_t>(nl, 1), 1.f);
+        double run = 0.0;
+        for (size_t l = 0; l < nl; ++l)
+        {
+            run += tot > 0.0 ? pw[l] : 1.0;
+            cdf[l] = l + 1 == nl ? 1.f : (float)(run / (tot > 0.0 ? tot : (double)nl));
+        }
+        BSH_HIP(s->d_light_cdf.alloc(cdf.size()));
+        BSH_HIP(hipMemcpy(s->d_light_cdf.p, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     for (int k = 0; k < 3; ++k)
     {
         s->box_lo[k] = std::numeric_limits<float>::max();
@@ -483,15 +522,19 @@ bsh_status bsh_context_stats_get(const bsh_context* ctx, bsh_render_stats* out)
     return BSH_OK;
 }
 
+bsh_status bsh_context_launch_stats(const bsh_context* ctx, bsh_launch_stats* out, size_t cap, size_t* n)
+{
+    if (!ctx || !n || (cap && !out)) return fail(BSH_ERR_INVALID, "bsh_context_launch_stats: NULL argument");
+    *n = ctx->launch_stats.size();
+    for (size_t k = 0; k < std::min(cap, ctx->launch_stats.size()); ++k) out[k] = ctx->launch_stats[k];
+    return BSH_OK;
+}
+
 bsh_status bsh_context_set_tuning(bsh_context* ctx, const bsh_tuning* t)
 {
     if (!ctx || !t) return fail(BSH_ERR_INVALID, "bsh_context_set_tuning: NULL argument");
-    if (t->paths_per_chunk < 0 || t->trace_blocks_per_cu < 0 || t->trace_blocks_per_cu > 5)
-        return fail(BSH_ERR_INVALID, "bsh_context_set_tuning: trace_blocks_per_cu must be 0..5 (LDS limit)");
-    if (t->refill_lanes < -1 || t->refill_lanes > 64)
-        return fail(BSH_ERR_INVALID, "bsh_context_set_tuning: refill_lanes must be -1..64");
-    if (t->claim_rays != 0 && (t->claim_rays < 64 || t->claim_rays > 4096))
-        return fail(BSH_ERR_INVALID, "bsh_context_set_tuning: claim_rays must be 0 or 64..4096");
+    bsh_status r = validate_tuning(*t, "bsh_context_set_tuning");
+    if (r != BSH_OK) return r;
     ctx->tuning = *t;
     return BSH_OK;
 }
@@ -542,6 +585,8 @@ bsh_status bsh_batch_run(bsh_context* ctx, const bsh_job* jobs, size_t njobs, co
     bsh_status r = build_jobs(ctx, jobs, njobs, hj, count);
     if (r != BSH_OK) return r;
     const bsh_tuning& t = tuning ? *tuning : ctx->tuning;
+    r = validate_tuning(t, "bsh_batch_run");
+    if (r != BSH_OK) return r;
     return run_jobs(ctx, hj, count, &t, t.use_graph, (hipStream_t)stream);
 }
 
@@ -565,7 +610,10 @@ bsh_status bsh_autotune(bsh_context* ctx, const bsh_job* jobs, size_t njobs, con
         {
             if (blocks_per_cu[b] < 1 || blocks_per_cu[b] > 5 || paths_per_chunk[a] < 1)
                 return fail(BSH_ERR_INVALID, "bsh_autotune: candidate out of range");
-            bsh_tuning t{paths_per_chunk[a], blocks_per_cu[b], 0, ctx->tuning.refill_lanes, ctx->tuning.claim_rays};
+            bsh_tuning t{paths_per_chunk[a], blocks_per_cu[b], 0, ctx->tuning.refill_lanes, ctx->tuning.claim_rays,
+                          ctx->tuning.lanes};
+            r = validate_tuning(t, "bsh_autotune");
+            if (r != BSH_OK) return r;
             std::vector<JobDev> copy = hj;
             r = run_jobs(ctx, copy, false, &t, 0, nullptr);
             if (r != BSH_OK) return r;

@@ -93,6 +93,43 @@ static bsh_status up(DevBuf<T>& d, const T* h, size_t n)
         if (r_ != BSH_OK) return r_;      \
     } while (0)
 
+// Streaming read: the memory ceilings the trace kernel is measured against. Each pass reads the
+// buffer's 16-B words in coalesced runs of kStreamBlock words per workgroup, four independent
+// loads in flight per lane, and XORs them into a per-thread word written at the end (so no load
+// can be dropped). per_xcd = 0: the grid partitions the buffer (every byte read once per pass:
+// the HBM ceiling for buffers far beyond the 256 MB Infinity Cache, that cache's for buffers
+// inside it). per_xcd = 1: workgroups w, w + 8, w + 16, ... (one XCD under round-robin dispatch)
+// partition the buffer among themselves, so each XCD reads all of it: a buffer within one XCD's
+// 4 MB L2 is then read from L2 on every pass after the first.
+constexpr int kStreamBlock = 256;
+__global__ __launch_bounds__(kStreamBlock) void k_stream_read(const float4* p, size_t n16, int reps, int per_xcd,
+                                                              uint32_t* sink)
+{
+    const size_t group = per_xcd ? blockIdx.x >> 3 : blockIdx.x;
+    const size_t groups = per_xcd ? gridDim.x >> 3 : gridDim.x;
+    const size_t stride = groups * kStreamBlock;
+    uint32_t acc = 0;
+    for (int r = 0; r < reps; ++r)
+    {
+        size_t i = group * kStreamBlock + threadIdx.x;
+        for (; i + 3 * stride < n16; i += 4 * stride)
+        {
+            const float4 a = gload(p, i), b = gload(p, i + stride), c = gload(p, i + 2 * stride),
+                         d = gload(p, i + 3 * stride);
+            acc ^= __float_as_uint(a.x) ^ __float_as_uint(a.y) ^ __float_as_uint(a.z) ^ __float_as_uint(a.w);
+            acc ^= __float_as_uint(b.x) ^ __float_as_uint(b.y) ^ __float_as_uint(b.z) ^ __float_as_uint(b.w);
+            acc ^= __float_as_uint(c.x) ^ __float_as_uint(c.y) ^ __float_as_uint(c.z) ^ __float_as_uint(c.w);
+            acc ^= __float_as_uint(d.x) ^ __float_as_uint(d.y) ^ __float_as_uint(d.z) ^ __float_as_uint(d.w);
+        }
+        for (; i < n16; i += stride)
+        {
+            const float4 a = gload(p, i);
+            acc ^= __float_as_uint(a.x) ^ __float_as_uint(a.y) ^ __float_as_uint(a.z) ^ __float_as_uint(a.w);
+        }
+    }
+    sink[(size_t)blockIdx.x * kStreamBlock + threadIdx.x] = acc;
+}
+
 }  // namespace bsh
 
 using namespace bsh;
@@ -170,6 +207,37 @@ bsh_status bsh_diag_keyed(bsh_context* ctx, uint32_t seed, size_t n, const uint3
                        c.p, o.p);
     BSH_HIP(hipGetLastError());
     BSH_HIP(hipMemcpy(out, o.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return BSH_OK;
+}
+
+bsh_status bsh_diag_stream_read(bsh_context* ctx, const void* buf, size_t bytes, int reps, int per_xcd, double* ms)
+{
+    if (!ctx || !buf || !ms) return fail(BSH_ERR_INVALID, "bsh_diag_stream_read: NULL argument");
+    if (bytes < 16 * kStreamBlock || bytes % 16 || reps < 1)
+        return fail(BSH_ERR_INVALID, "bsh_diag_stream_read: bytes must be a multiple of 16 and >= 4 KiB, reps >= 1");
+    BSH_HIP(hipSetDevice(ctx->device));
+    const uint32_t blocks = (uint32_t)ctx->num_cus * 8;   // a multiple of the 8 XCDs
+    DevBuf<uint32_t> sink;
+    BSH_HIP(sink.alloc((size_t)blocks * kStreamBlock));
+    const size_t n16 = bytes / 16;
+    hipEvent_t e0, e1;
+    BSH_HIP(hipEventCreate(&e0));
+    BSH_HIP(hipEventCreate(&e1));
+    // one untimed pass (page mapping, and for cache-resident sizes the first fill), then the timed one
+    hipLaunchKernelGGL(k_stream_read, dim3(blocks), dim3(kStreamBlock), 0, 0, (const float4*)buf, n16, 1, per_xcd,
+                       sink.p);
+    (void)hipEventRecord(e0, 0);
+    hipLaunchKernelGGL(k_stream_read, dim3(blocks), dim3(kStreamBlock), 0, 0, (const float4*)buf, n16, reps, per_xcd,
+                       sink.p);
+    (void)hipEventRecord(e1, 0);
+    hipError_t e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_fail(e, "bsh_diag_stream_read");
+    BSH_HIP(hipGetLastError());
+    *ms = t;
     return BSH_OK;
 }
 

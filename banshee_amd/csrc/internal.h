@@ -19,6 +19,8 @@ namespace bsh {
 void set_error(const std::string& msg);
 bsh_status fail(bsh_status code, const std::string& msg);
 bsh_status hip_fail(hipError_t e, const char* what);
+// BSH_OK, or BSH_ERR_INVALID naming `who` and the field out of range.
+bsh_status validate_tuning(const bsh_tuning& t, const char* who);
 
 #define BSH_HIP(call)                                        \
     do                                                       \
@@ -67,7 +69,7 @@ struct RenderWork
     DevBuf<float4> ray_o, ray_d, thr, thr_nee, rad, out;
     DevBuf<int4> hit;
     DevBuf<uint4> meta;
-    DevBuf<uint32_t> nee_mask;
+    DevBuf<unsigned long long> nee_mask;   // per path: NEE slots with a ray in flight (bit 2j shadow, 2j+1 MIS)
     DevBuf<float4> nee_a, nee_b, nee_wi;
     DevBuf<int4> nee_res;
     DevBuf<float4> qc_o[2], qc_d[2], qa_o[2], qa_d[2];   // closest / any-hit queues
@@ -158,8 +160,9 @@ struct bsh_context
             if (s) (void)hipStreamDestroy(s);
     }
     bsh::DevBuf<bsh::JobDev> jobs;
-    bsh_tuning tuning{1 << 23, 0, 0, 0, 0};
+    bsh_tuning tuning{1 << 23, 0, 0, 0, 0, 0};
     bsh_render_stats stats{};
+    std::vector<bsh_launch_stats> launch_stats;   // per bounce launch, of the last run
 };
 
 struct bsh_scene
@@ -177,6 +180,7 @@ struct bsh_scene
     bsh::DevBuf<float4> d_nodes;   // GpuNode (width 2) or GpuNode4 (width 4) as float4s
     bsh::DevBuf<float4> d_tris, d_stri;
     bsh::DevBuf<bsh::LightDev> d_lights;
+    bsh::DevBuf<float> d_light_cdf;
     std::vector<bsh::LightDev> h_lights;
     float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {0.f, 0.f, 0.f};
     uint64_t nnodes = 0;
@@ -193,6 +197,7 @@ struct bsh_scene
         s.tris = d_tris.p;
         s.stri = d_stri.p;
         s.lights = d_lights.p;
+        s.light_cdf = d_light_cdf.p;
         s.nlights = (int)lights.size();
         s.ntris = (int)tri_v.size() / 3;
         return s;

@@ -48,6 +48,8 @@ namespace bsh {
 
 
 constexpr int kBlock = 256;
+// Lights the bounce-0 NEE loop covers (two ray slots each in a 64-bit mask per path).
+constexpr int kMaxNeeLights = 32;
 // LDS entries of the wavefront trace kernel's traversal stack: all of the binary layout's
 // kMaxStack (32 x 4 B x 256 threads = 32 KB per block, 5 blocks per CU); the 4-wide
 // layout's deeper entries go to per-thread global slots. Measured on the 3M-triangle
@@ -60,8 +62,12 @@ constexpr int kMaxTraceBlocksPerCU = 5;
 constexpr int kTraceBlocksPerLane = 3;   // the default when two lanes share more than two chunks
 // The shade kernel is held to 128 VGPRs: 4 waves per SIMD (at 129 it would run 3).
 constexpr int kShadeWavesPerSimd = 4;
-// Statistics buffer: 4 counters, then the guided-row partials, kRowStride apart (128 B).
+// Statistics buffer: 4 counters, then the guided-row partials, kRowStride apart (128 B), then per
+// bounce launch b: closest rays, any-hit rays, and in counting mode node visits and triangle tests
+// of the closest and of the any-hit rays (kLaunchStat counters each).
 constexpr int kStatRows = 4, kRowParts = 64, kRowStride = 16, kStatTotal = kStatRows + kRowParts * kRowStride;
+constexpr int kMaxLaunches = 256, kLaunchStat = 6, kStatAll = kStatTotal + kMaxLaunches * kLaunchStat;
+static_assert(kMaxLaunches > 250, "a job's depth is at most 250, so a chunk has at most 251 launches");
 // Idle lanes of a wave that trigger a refill in k_trace_refill (when the tuning leaves it 0).
 // Measured on the 3M-triangle terrain at 1080p: 8 lanes 733 Mrays/s, 16 1058, 32 1314,
 // 48 1381, 56 1393, 64 1353; whole 64-ray batches per wave (k_trace_queue) 1065.
@@ -93,7 +99,7 @@ struct RenderDev
     float4 *ray_o, *ray_d, *thr, *thr_nee, *rad, *out;
     int4* hit;
     uint4* meta;       // x: key base, y: flags | bounce << 8 | nee lights << 16 | job << 24, z/w: BSDF sample
-    uint32_t* nee_mask;
+    unsigned long long* nee_mask;   // bit 2j: shadow ray of NEE slot j in flight, 2j + 1: its MIS ray
     float4 *nee_a, *nee_b, *nee_wi;
     int4* nee_res;
     float4 *qc_o[2], *qc_d[2], *qa_o[2], *qa_d[2];
@@ -144,6 +150,8 @@ __global__ void k_chunk_rays(const uint32_t* counters, int launches, unsigned lo
         {
             c += counters[4 * L + 0];
             a += counters[4 * L + 1];
+            atomicAdd(&stats[kStatTotal + kLaunchStat * L + 0], (unsigned long long)counters[4 * L + 0]);
+            atomicAdd(&stats[kStatTotal + kLaunchStat * L + 1], (unsigned long long)counters[4 * L + 1]);
         }
         atomicAdd(&stats[0], c);
         atomicAdd(&stats[1], a);
@@ -372,6 +380,15 @@ __global__ __launch_bounds__(kBlock) void k_trace_queue(RenderDev R, int L, int 
     uint32_t vis_nodes = 0, vis_tris = 0;
     for (int pass = 0; pass < 2; ++pass)
     {
+        if (kCount && pass == 1)
+        {
+            // the closest-hit queue's counts, then the any-hit queue's
+            atomicAdd(&R.stats[kStatTotal + kLaunchStat * L + 2], (unsigned long long)vis_nodes);
+            atomicAdd(&R.stats[kStatTotal + kLaunchStat * L + 3], (unsigned long long)vis_tris);
+            atomicAdd(&R.stats[2], (unsigned long long)vis_nodes);
+            atomicAdd(&R.stats[3], (unsigned long long)vis_tris);
+            vis_nodes = vis_tris = 0;
+        }
         const uint32_t count = R.counters[4 * L + pass];
         uint32_t* work = &R.counters[4 * L + 2 + pass];
         const float4* qo = pass == 0 ? R.qc_o[buf] : R.qa_o[buf];
@@ -421,6 +438,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_queue(RenderDev R, int L, int 
     }
     if (kCount)
     {
+        atomicAdd(&R.stats[kStatTotal + kLaunchStat * L + 4], (unsigned long long)vis_nodes);
+        atomicAdd(&R.stats[kStatTotal + kLaunchStat * L + 5], (unsigned long long)vis_tris);
         atomicAdd(&R.stats[2], (unsigned long long)vis_nodes);
         atomicAdd(&R.stats[3], (unsigned long long)vis_tris);
     }
@@ -446,7 +465,7 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
     const uint32_t total = nc + R.counters[4 * L + 1];
     uint32_t* work = &R.counters[4 * L + 2];
     const float kInf = __builtin_inff();
-    uint32_t vis_nodes = 0, vis_tris = 0;
+    uint32_t vis_nodes = 0, vis_tris = 0, vis_nodes_any = 0, vis_tris_any = 0;
     bool active = false, any = false, exhausted = false;
     // the wave's reserve: queue entries [rb, re) it has claimed and not yet handed to a lane
     uint32_t rb = 0, re = 0;
@@ -468,7 +487,8 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
         if ((!exhausted || rb < re) && (nidle >= (uint32_t)refill || nidle == 64u))
         {
             const uint32_t from_res = min(re - rb, nidle);
-            const uint32_t want = claim_rays ? claim_rays : nidle;   // >= the lanes it must cover
+            // >= the lanes it must cover, whatever claim_rays the host let through
+            const uint32_t want = claim_rays ? max(claim_rays, nidle) : nidle;
             uint32_t nb = 0;
             const bool claim = from_res < nidle && !exhausted;
             if (claim)
@@ -532,7 +552,7 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
         if (__ballot(active) == 0ull) break;
         if (!active) continue;
         // ---- one node visit (traverse4's loop body) ----
-        if (kCount) vis_nodes++;
+        if (kCount) (any ? vis_nodes_any : vis_nodes)++;
         const size_t nb = 8 * (size_t)node;
         const float4 LX = gload(nodes, nb), HX = gload(nodes, nb + 1), LY = gload(nodes, nb + 2),
                      HY = gload(nodes, nb + 3), LZ = gload(nodes, nb + 4), HZ = gload(nodes, nb + 5);
@@ -565,7 +585,7 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
                     const float4 v0 = gload(tris, 3 * (size_t)q + 0);
                     const float4 e1 = gload(tris, 3 * (size_t)q + 1);
                     const float4 e2 = gload(tris, 3 * (size_t)q + 2);
-                    if (kCount) vis_tris++;
+                    if (kCount) (any ? vis_tris_any : vis_tris)++;
                     float t, u, v;
                     if (intersect_face(ld3(v0), ld3(e1), ld3(e2), o, d, tmin, tcur, t, u, v))
                     {
@@ -634,8 +654,13 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
     }
     if (kCount)
     {
-        atomicAdd(&R.stats[2], (unsigned long long)vis_nodes);
-        atomicAdd(&R.stats[3], (unsigned long long)vis_tris);
+        unsigned long long* ls = R.stats + kStatTotal + kLaunchStat * L;
+        atomicAdd(&ls[2], (unsigned long long)vis_nodes);
+        atomicAdd(&ls[3], (unsigned long long)vis_tris);
+        atomicAdd(&ls[4], (unsigned long long)vis_nodes_any);
+        atomicAdd(&ls[5], (unsigned long long)vis_tris_any);
+        atomicAdd(&R.stats[2], (unsigned long long)(vis_nodes + vis_nodes_any));
+        atomicAdd(&R.stats[3], (unsigned long long)(vis_tris + vis_tris_any));
     }
 }
 
@@ -673,7 +698,7 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
     }
     const int K = 1 << R.Kshift;
     bool emit_ext = false, keep = false;
-    uint32_t emit_mask = 0;
+    unsigned long long emit_mask = 0;
     f3 org = mk(0.f, 0.f, 0.f), ext_d = mk(0.f, 0.f, 0.f);
     if (have)
     {
@@ -697,18 +722,18 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
             // ---- 1. resolve the NEE rays issued at bounce pb (DiTracer::GetDi tail) ----
             if (fl & F_NEE)
             {
-                uint32_t mask = R.nee_mask[i];
+                const unsigned long long mask = R.nee_mask[i];
                 f3 T = ld3(R.thr_nee[i]);
                 f3 ps = ld3(R.ray_o[i]);
                 for (int j = 0; j < ncnt; ++j)
                 {
                     f3 local = mk(0.f, 0.f, 0.f);
                     size_t ks = (size_t)i * K + 2 * j;
-                    if (mask & (1u << (2 * j)))
+                    if (mask & (1ull << (2 * j)))
                     {
                         if (R.nee_res[ks].x == 0) local = local + ld3(R.nee_a[ks]);
                     }
-                    if (mask & (1u << (2 * j + 1)))
+                    if (mask & (1ull << (2 * j + 1)))
                     {
                         int4 r = R.nee_res[ks + 1];
                         float4 nb = R.nee_b[ks + 1];
@@ -741,7 +766,9 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                             }
                         }
                     }
-                    if (pb == 0) rad = rad + T * local;
+                    // the reference scales its one uniformly picked light by nl; a quirk-free pick's
+                    // 1 / probability is in the slot's contributions already
+                    if (pb == 0 || nq) rad = rad + T * local;
                     else rad = rad + (T * local) * (float)nl;
                 }
                 fl &= ~F_NEE;
@@ -805,12 +832,21 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                         // ---- next-event estimation (gitracer.cpp:70-84 -> ditracer.cpp:57-239) ----
                         if (nl > 0)
                         {
-                            int jn = b == 0 ? nl : 1;
+                            // every light at bounce 0, then one picked uniformly (gitracer.cpp:72-84);
+                            // quirk-free: every light at bounce 0 while they fit the NEE slots, else one
+                            // light picked in proportion to its power, weighted by 1 / its probability
+                            const bool all = b == 0 && (!nq || nl <= kMaxNeeLights);
+                            const int jn = all ? nl : 1;
                             int Lsel = 0;
-                            if (b != 0) Lsel = (int)(ks_rand(base, rand_pick_counter(b)) % (uint32_t)nl);
+                            float pickw = 1.f;
+                            if (!all)
+                            {
+                                if (nq) Lsel = pick_light(S, ks_rand_float(base, rand_pick_counter(b)), pickw);
+                                else Lsel = (int)(ks_rand(base, rand_pick_counter(b)) % (uint32_t)nl);
+                            }
                             for (int j = 0; j < jn; ++j)
                             {
-                                int Lid = b == 0 ? j : Lsel;
+                                int Lid = all ? j : Lsel;
                                 const LightDev* L = S.lights + Lid;
                                 uint32_t lc = b == 0 ? (uint32_t)(2 * j) : (uint32_t)(2 * nl + 2 * (b - 1));
                                 uint32_t bc = 2u + lc;
@@ -841,9 +877,10 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                     // see), so the quirk-free estimator adds the Jacobian's cos_l
                                     if (nq) le = le * cosl;
                                     f3 c = le * f * fabsf(dot(h.n, wi)) * w * (1.f / lpdf);
+                                    if (nq) c = c * pickw;
                                     R.nee_a[ks] = st3(c, 0.f);
                                     R.nee_wi[ks] = st3(wi, dist - 0.01f);
-                                    emit_mask |= 1u << (2 * j);
+                                    emit_mask |= 1ull << (2 * j);
                                 }
                                 // BSDF sample + MIS closest ray
                                 f3 wis;
@@ -858,9 +895,9 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                     {
                                         float w = power_heuristic(bpdf, lp);
                                         R.nee_a[ks + 1] = st3(fb, fabsf(dot(h.n, wi)));
-                                        R.nee_b[ks + 1] = make_float4(w, 1.f / bpdf, (float)Lid, 0.f);
+                                        R.nee_b[ks + 1] = make_float4(nq ? w * pickw : w, 1.f / bpdf, (float)Lid, 0.f);
                                         R.nee_wi[ks + 1] = st3(wi, 10000000.f);
-                                        emit_mask |= 1u << (2 * j + 1);
+                                        emit_mask |= 1ull << (2 * j + 1);
                                     }
                                 }
                             }
@@ -1031,7 +1068,7 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
     uint32_t ncl = (uint32_t)__popcll(__ballot(emit_ext)), nany = 0;
     for (int k = 0; k < K; ++k)
     {
-        const uint32_t c = (uint32_t)__popcll(__ballot((emit_mask >> k) & 1u));
+        const uint32_t c = (uint32_t)__popcll(__ballot((emit_mask >> k) & 1ull));
         if (k & 1) ncl += c;
         else nany += c;
     }
@@ -1091,7 +1128,7 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
         }
         for (int k = 0; k < K; ++k)
         {
-            const bool mine = ((emit_mask >> k) & 1u) != 0;
+            const bool mine = ((emit_mask >> k) & 1ull) != 0;
             const unsigned long long m = __ballot(mine);
             if (mine)
             {
@@ -1352,7 +1389,7 @@ static bsh_status ensure_work(RenderWork& w, size_t P, int K, int nlmax)
         BSH_HIP(w.qa_d[k].alloc(qa));
         BSH_HIP(w.qa_own[k].alloc(qa));
     }
-    BSH_HIP(w.stats.alloc(kStatTotal));
+    BSH_HIP(w.stats.alloc(kStatAll));
     return BSH_OK;
 }
 
@@ -1360,6 +1397,7 @@ static bsh_status ensure_work(RenderWork& w, size_t P, int K, int nlmax)
 struct LaunchTimer
 {
     std::vector<hipEvent_t> evs;
+    std::vector<int> bounce;   // the bounce (launch index within its chunk) of every pass, in order
     size_t ei = 0;
     bsh_status init(size_t pairs)
     {
@@ -1527,7 +1565,11 @@ static bsh_status enqueue_batch(bsh_context* c, const Lane* lanes, int nlanes, c
         dim3 pg = grid_for(ch.P);
         for (int b = 0; b <= ch.depth; ++b)
         {
-            if (T) BSH_HIP(hipEventRecord(T->next(), st));
+            if (T)
+            {
+                BSH_HIP(hipEventRecord(T->next(), st));
+                T->bounce.push_back(b);
+            }
             if (wide)
             {
                 if (count)
@@ -1595,7 +1637,6 @@ static bsh_status enqueue_batch(bsh_context* c, const Lane* lanes, int nlanes, c
     return BSH_OK;
 }
 
-constexpr int kMaxLights = 16;
 
 // The guide's position input is the hit point mapped to [0,1]^3 over the scene's box.
 static void set_guide_box(const bsh_scene* s, JobDev& J)
@@ -1615,7 +1656,12 @@ bsh_status make_render_job(bsh_scene* s, const bsh_render_params* p, float* accu
     if (!s->has_cam) return fail(BSH_ERR_STATE, "bsh_render: no camera");
     if (p->width <= 0 || p->height <= 0 || p->img_grid <= 0 || p->max_depth <= 0 || p->max_depth > 250)
         return fail(BSH_ERR_INVALID, "bsh_render: bad frame parameters");
-    if (s->lights.size() > (size_t)kMaxLights) return fail(BSH_ERR_UNSUPPORTED, "bsh_render: more than 16 area lights");
+    if (p->chunk_spp != 0)
+        return fail(BSH_ERR_INVALID, "bsh_render: chunk_spp is reserved and must be 0 (bound the paths per "
+                                     "chunk with bsh_tuning.paths_per_chunk)");
+    if (s->lights.size() > (size_t)kMaxNeeLights && !(p->flags & BSH_RENDER_NO_QUIRKS))
+        return fail(BSH_ERR_UNSUPPORTED, "bsh_render: GiTracer's bounce-0 loop over every light supports at most 32 "
+                                         "area lights (BSH_RENDER_NO_QUIRKS picks one light per vertex by power)");
     int cw = p->crop_w > 0 ? p->crop_w : p->width, ch = p->crop_h > 0 ? p->crop_h : p->height;
     int cx = p->crop_w > 0 ? p->crop_x : 0, cy = p->crop_h > 0 ? p->crop_y : 0;
     if (cx < 0 || cy < 0 || cx + cw > p->width || cy + ch > p->height)
@@ -1660,9 +1706,14 @@ bsh_status make_probe_job(bsh_scene* s, const float* probes, size_t nprobes, con
     int g = (int)std::lround(std::sqrt((double)(N > 0 ? N : 0)));
     if (N <= 0 || g * g != N) return fail(BSH_ERR_INVALID, "bsh_bake_sh: rays_per_probe must be a square");
     if (p->max_depth <= 0 || p->max_depth > 250) return fail(BSH_ERR_INVALID, "bsh_bake_sh: bad max_depth");
+    if (p->chunk_probes != 0)
+        return fail(BSH_ERR_INVALID, "bsh_bake_sh: chunk_probes is reserved and must be 0 (bound the paths per "
+                                     "chunk with bsh_tuning.paths_per_chunk)");
     if ((uint64_t)nprobes * (uint64_t)N > 0xffffffffull)
         return fail(BSH_ERR_UNSUPPORTED, "bsh_bake_sh: more than 2^32 probe rays");
-    if (s->lights.size() > (size_t)kMaxLights) return fail(BSH_ERR_UNSUPPORTED, "bsh_bake_sh: more than 16 area lights");
+    if (s->lights.size() > (size_t)kMaxNeeLights && !(p->flags & BSH_RENDER_NO_QUIRKS))
+        return fail(BSH_ERR_UNSUPPORTED, "bsh_bake_sh: GiTracer's bounce-0 loop over every light supports at most 32 "
+                                         "area lights (BSH_RENDER_NO_QUIRKS picks one light per vertex by power)");
     J = JobDev{};
     J.s = s->dev();
     J.type = 1;
@@ -1691,13 +1742,15 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     std::lock_guard<std::mutex> lock(c->mu);
     BSH_HIP(hipSetDevice(c->device));
     bsh_tuning tune = tune_in ? *tune_in : c->tuning;
+    BSH_TRY(validate_tuning(tune, "run"));
     if (tune.paths_per_chunk <= 0) tune.paths_per_chunk = 1 << 23;
     int nlmax = 1;
     uint32_t maxunit = 1;
     uint64_t samples = 0;
     for (const JobDev& J : hjobs)
     {
-        nlmax = std::max(nlmax, J.s.nlights);
+        // NEE slots: two per light of the bounce-0 loop, two for a picked light
+        nlmax = std::max(nlmax, (J.flags & JOB_NO_QUIRKS) && J.s.nlights > kMaxNeeLights ? 1 : J.s.nlights);
         maxunit = std::max<uint32_t>(maxunit, J.type == 0 ? (uint32_t)J.s_count : (uint32_t)J.N);
         samples += J.type == 0 ? (uint64_t)J.cw * J.ch * J.s_count : (uint64_t)J.nprobes * J.N;
     }
@@ -1706,6 +1759,9 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     const int K = 1 << Kshift;
     size_t pmax = std::max<size_t>((size_t)tune.paths_per_chunk, maxunit);
     if (pmax * (size_t)K >= ((size_t)1 << 29)) pmax = (((size_t)1 << 29) - 1) / K;
+    // the NEE slot buffers (64 B per slot) hold pmax x K slots: at most 2^27 (8 GiB) per lane, so a
+    // scene with many lights runs fewer paths per chunk
+    if (pmax * (size_t)K > ((size_t)1 << 27)) pmax = std::max<size_t>(((size_t)1 << 27) / K, maxunit);
     if (pmax < maxunit) return fail(BSH_ERR_UNSUPPORTED, "unit larger than the NEE slot space");
     std::vector<Chunk> chunks = plan_chunks(hjobs, pmax);
     // An unguided run that fits one chunk of at least kSplitPaths paths is planned as two, so
@@ -1739,7 +1795,7 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     const bool record = gr && gr->records && gr->cap > 0;
     // Two lanes when the run has chunks to alternate and nothing ties the chunks to one order
     // (training records take a running offset).
-    const int nlanes = record ? 1 : (int)std::min<size_t>(bsh_context::kLanes, chunks.size());
+    const int nlanes = record || tune.lanes == 1 ? 1 : (int)std::min<size_t>(bsh_context::kLanes, chunks.size());
     // persistent trace workgroups per CU: all of them on one lane, or when two lanes hold one
     // chunk each and run side by side from start to end (a 4096-probe bake: 5 per CU 1525
     // Mrays/s, 3 1477); with more chunks each lane's grid leaves room for the other lane's
@@ -1825,9 +1881,10 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     }
     if (record) hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, w.rec_count.p, 1);
 
-    hipLaunchKernelGGL(k_zero, grid_for(2 * kStatTotal), dim3(kBlock), 0, st, (uint32_t*)w.stats.p, 2 * kStatTotal);
+    hipLaunchKernelGGL(k_zero, grid_for(2 * kStatAll), dim3(kBlock), 0, st, (uint32_t*)w.stats.p, 2 * kStatAll);
     bsh_render_stats& S = c->stats;
     S = bsh_render_stats{};
+    std::vector<double> launch_ms(maxdepth + 1, 0.0);
     if (use_graph)
     {
         // Capture the whole batch (every chunk's raygen, bounces, accumulation) as one graph
@@ -1913,6 +1970,7 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
             BSH_HIP(hipEventElapsedTime(&b, T.evs[k + 3], T.evs[k + 4]));
             S.trace_ms += a;
             S.shade_ms += b;
+            launch_ms[T.bounce[(k - 1) / 5]] += a;
             S.trace_launches++;
             S.shade_launches++;
             if (guided && g > 0.f)
@@ -1925,8 +1983,21 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         BSH_HIP(hipEventElapsedTime(&tot, T.evs[0], T.evs[T.ei - 1]));
         S.total_ms = tot;
     }
-    unsigned long long st2[kStatTotal];
-    BSH_HIP(hipMemcpy(st2, w.stats.p, sizeof(st2), hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> st2(kStatAll);
+    BSH_HIP(hipMemcpy(st2.data(), w.stats.p, kStatAll * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    c->launch_stats.assign(maxdepth + 1, bsh_launch_stats{});
+    for (int b = 0; b <= maxdepth; ++b)
+    {
+        const unsigned long long* ls = &st2[kStatTotal + kLaunchStat * b];
+        bsh_launch_stats& o = c->launch_stats[b];
+        o.rays_closest = ls[0];
+        o.rays_any = ls[1];
+        o.nodes_closest = count ? ls[2] : 0;
+        o.tris_closest = count ? ls[3] : 0;
+        o.nodes_any = count ? ls[4] : 0;
+        o.tris_any = count ? ls[5] : 0;
+        o.trace_ms = launch_ms[b];
+    }
     S.guide_rows = 0;
     for (int k = 0; k < kRowParts; ++k) S.guide_rows += st2[kStatRows + kRowStride * k];
     S.rays_closest = st2[0];

@@ -55,6 +55,7 @@ struct SceneDev
     const float4* stri;       // per prim, kShadeF4 float4: p1 (w: material index), p2 (w: bundle
                               // light), p3, n1, n2, n3, its material (type, c.xyz), (eta, alpha, 0, 0)
     const LightDev* lights;
+    const float* light_cdf;   // per light: cumulative share of the emitted power (area x luminance), last = 1
     int nlights;
     int ntris;
     int width;                // BVH layout: 2 (GpuNode, 4 float4) or 4 (GpuNode4, 8 float4)

@@ -31,6 +31,22 @@ __device__ __forceinline__ int tri_bundle_light(const SceneDev& s, int prim)
     return __float_as_int(s.stri[(size_t)kShadeF4 * prim + 1].w);
 }
 
+// Quirk-free light pick: the first light whose cumulative power share exceeds u in [0, 1)
+// (binary search over the scene's CDF); w receives 1 / its probability.
+__device__ __forceinline__ int pick_light(const SceneDev& s, float u, float& w)
+{
+    int lo = 0, hi = s.nlights - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi) >> 1;
+        if (s.light_cdf[mid] > u) hi = mid;
+        else lo = mid + 1;
+    }
+    const float p = s.light_cdf[lo] - (lo > 0 ? s.light_cdf[lo - 1] : 0.f);
+    w = 1.f / p;
+    return lo;
+}
+
 // AreaLight::GetPdf -> Mesh::GetPdfOnShape (arealight.cpp:55-58, mesh.cpp:354-378)
 __device__ __forceinline__ float light_pdf(const LightDev* Lp, f3 p, f3 w)
 {

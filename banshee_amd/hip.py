@@ -69,7 +69,8 @@ class ProbeParams(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("paths_per_chunk", ctypes.c_int32), ("trace_blocks_per_cu", ctypes.c_int32),
-                ("use_graph", ctypes.c_int32), ("refill_lanes", ctypes.c_int32), ("claim_rays", ctypes.c_int32)]
+                ("use_graph", ctypes.c_int32), ("refill_lanes", ctypes.c_int32), ("claim_rays", ctypes.c_int32),
+                ("lanes", ctypes.c_int32)]
 
 
 class Job(ctypes.Structure):
@@ -93,6 +94,12 @@ class RenderStats(ctypes.Structure):
                 ("guide_rows", ctypes.c_uint64), ("guide_ms", ctypes.c_double)]
 
 
+class LaunchStats(ctypes.Structure):
+    _fields_ = [("rays_closest", ctypes.c_uint64), ("rays_any", ctypes.c_uint64),
+                ("nodes_closest", ctypes.c_uint64), ("tris_closest", ctypes.c_uint64),
+                ("nodes_any", ctypes.c_uint64), ("tris_any", ctypes.c_uint64), ("trace_ms", ctypes.c_double)]
+
+
 # Every symbol include/banshee_hip.h declares.
 EXPORTS = [
     "bsh_context_create", "bsh_context_destroy", "bsh_last_error", "bsh_device_alloc", "bsh_device_free",
@@ -106,7 +113,36 @@ EXPORTS = [
     "bsh_guide_forward", "bsh_guide_gradient", "bsh_guide_train_step", "bsh_render_guided", "bsh_diag_mfma",
     "bsh_diag_guide", "bsh_bvh_build_host", "bsh_guide_set_bsdf_fraction", "bsh_guide_train",
     "bsh_bake_sh_guided", "bsh_guide_forward_fp32", "bsh_guide_gradient_fp32",
+    "bsh_context_launch_stats", "bsh_diag_stream_read", "bsh_import_obj",
 ]
+
+
+class AssetCounts(ctypes.Structure):
+    _fields_ = [(n, c_size_t) for n in ("nverts", "nnormals", "ntris", "nbundles", "nmaterials", "nlights",
+                                         "ignored_maps")]
+
+
+def import_obj(path: str, camera, name: str = ""):
+    """bsh_import_obj (host only): the World arrays the reference's AssimpAssetImporter builds from
+    an OBJ/MTL file, as a banshee_amd.scenes.Scene with the given camera (13 floats: eye, at, up,
+    znear, zfar, fovy, aspect)."""
+    from .scenes import Scene as SceneData
+    c = AssetCounts()
+    p = os.fsencode(path)
+    _check(lib().bsh_import_obj(p, ctypes.byref(c), *([None] * 8)), "bsh_import_obj")
+    v = np.zeros((c.nverts, 3), np.float32)
+    n = np.zeros((c.nnormals, 3), np.float32)
+    tv = np.zeros((c.ntris, 3), np.int32)
+    tn = np.zeros((c.ntris, 3), np.int32)
+    tm = np.zeros(c.ntris, np.int32)
+    b = np.zeros((c.nbundles, 6), np.int32)
+    m = np.zeros((c.nmaterials, 8), np.float32)
+    li = np.zeros((max(c.nlights, 1), 2), np.int32)
+    _check(lib().bsh_import_obj(p, ctypes.byref(c), _p(v), _p(n), _p(tv), _p(tn), _p(tm), _p(b), _p(m), _p(li)),
+           "bsh_import_obj")
+    return SceneData(verts=v, normals=n, tri_v=tv, tri_n=tn, tri_m=tm, bundles=b, materials=m,
+                     lights=np.ascontiguousarray(li[:c.nlights]), camera=np.asarray(camera, np.float32),
+                     name=name or os.path.basename(path), meta={"ignored_maps": int(c.ignored_maps)})
 
 _lib = None
 
@@ -274,8 +310,8 @@ class Context:
         _check(lib().bsh_context_get_tuning(self.h, ctypes.byref(t)), "bsh_context_get_tuning")
         return {k: getattr(t, k) for k, _ in t._fields_ if k != "reserved"}
 
-    def set_tuning(self, paths_per_chunk=0, trace_blocks_per_cu=0, use_graph=0, refill_lanes=0, claim_rays=0):
-        t = Tuning(paths_per_chunk, trace_blocks_per_cu, use_graph, refill_lanes, claim_rays)
+    def set_tuning(self, paths_per_chunk=0, trace_blocks_per_cu=0, use_graph=0, refill_lanes=0, claim_rays=0, lanes=0):
+        t = Tuning(paths_per_chunk, trace_blocks_per_cu, use_graph, refill_lanes, claim_rays, lanes)
         _check(lib().bsh_context_set_tuning(self.h, ctypes.byref(t)), "bsh_context_set_tuning")
 
     def stats(self) -> dict:
@@ -283,13 +319,31 @@ class Context:
         _check(lib().bsh_context_stats_get(self.h, ctypes.byref(st)), "bsh_context_stats_get")
         return {k: getattr(st, k) for k, _ in st._fields_}
 
+    def stream_read_gbs(self, buf: DeviceArray, reps: int, per_xcd: bool = False) -> float:
+        """GB/s of bsh_diag_stream_read over all of `buf` (per_xcd: every XCD reads all of it)."""
+        ms = ctypes.c_double()
+        _check(lib().bsh_diag_stream_read(self.h, buf.ptr, c_size_t(buf.nbytes), c_int(reps), c_int(int(per_xcd)),
+                                          ctypes.byref(ms)), "bsh_diag_stream_read")
+        moved = buf.nbytes * reps * (8 if per_xcd else 1)
+        return moved / (ms.value * 1e-3) / 1e9
+
+    def launch_stats(self) -> list:
+        """Per bounce launch of the last run: dicts of LaunchStats' fields."""
+        n = c_size_t()
+        _check(lib().bsh_context_launch_stats(self.h, None, c_size_t(0), ctypes.byref(n)), "bsh_context_launch_stats")
+        arr = (LaunchStats * max(n.value, 1))()
+        _check(lib().bsh_context_launch_stats(self.h, arr, c_size_t(n.value), ctypes.byref(n)),
+               "bsh_context_launch_stats")
+        return [{k: getattr(arr[i], k) for k, _ in LaunchStats._fields_} for i in range(n.value)]
+
     def batch_run(self, jobs, tuning: dict | None = None, stream=None):
         """jobs: list of BatchJob. Runs them in one wavefront (one hipGraph when use_graph)."""
         arr = (Job * len(jobs))(*[j.struct() for j in jobs])
         t = None
         if tuning is not None:
             t = Tuning(tuning.get("paths_per_chunk", 0), tuning.get("trace_blocks_per_cu", 0),
-                       tuning.get("use_graph", 0), tuning.get("refill_lanes", 0), tuning.get("claim_rays", 0))
+                       tuning.get("use_graph", 0), tuning.get("refill_lanes", 0), tuning.get("claim_rays", 0),
+                       tuning.get("lanes", 0))
         _check(lib().bsh_batch_run(self.h, arr, c_size_t(len(jobs)), ctypes.byref(t) if t else None,
                                    c_void_p(stream)), "bsh_batch_run")
 
@@ -303,7 +357,7 @@ class Context:
                                   _p(bl) if bl.size else None, c_size_t(bl.size), ctypes.byref(best),
                                   ctypes.byref(ms)), "bsh_autotune")
         return {"paths_per_chunk": best.paths_per_chunk, "trace_blocks_per_cu": best.trace_blocks_per_cu,
-                "refill_lanes": best.refill_lanes, "claim_rays": best.claim_rays}, ms.value
+                "refill_lanes": best.refill_lanes, "claim_rays": best.claim_rays, "lanes": best.lanes}, ms.value
 
     def close(self):
         if self.h:
@@ -607,9 +661,11 @@ class Scene:
         return a[..., 0:3], a[..., 4:7], a[..., 3], (samp.numpy() if samp else None)
 
     def bake_sh(self, probes, rays_per_probe, lmax, max_depth, seed, tmin=0.0, chunk_probes=0, radiance=False,
-                count=False, coeffs: DeviceArray | None = None, download=True, quirk_free=False):
+                count=False, coeffs: DeviceArray | None = None, download=True, quirk_free=False,
+                bg=(0.0, 0.0, 0.0)):
         """Returns (coeffs [nprobes, (lmax+1)^2, 3], per-ray radiance [nprobes, N, 3] or None).
-        quirk_free selects the unbiased estimator (BSH_RENDER_NO_QUIRKS) instead of GiTracer's."""
+        quirk_free selects the unbiased estimator (BSH_RENDER_NO_QUIRKS) instead of GiTracer's;
+        bg is the radiance of a probe ray that hits nothing (World::bgcolor_)."""
         probes = np.ascontiguousarray(probes, np.float32).reshape(-1, 3)
         n = probes.shape[0]
         dp = self.ctx.to_device(probes)
@@ -620,6 +676,8 @@ class Scene:
         p = ProbeParams()
         p.rays_per_probe, p.lmax, p.max_depth, p.seed = rays_per_probe, lmax, max_depth, seed
         p.chunk_probes, p.tmin = chunk_probes, tmin
+        for k in range(3):
+            p.bg[k] = bg[k]
         p.flags = (4 if count else 0) | (2 if quirk_free else 0)
         _check(lib().bsh_bake_sh(self.h, dp.ptr, c_size_t(n), ctypes.byref(p), coeffs.ptr,
                                  rad.ptr if rad else c_void_p(None), c_void_p(None)), "bsh_bake_sh")

@@ -366,6 +366,18 @@ def probe_grid(scene: Scene, n=16, margin=0.05, lift=0.5):
     return P.astype(np.float32)
 
 
+def far_speck(distance=1.0e4, size=1.0e-3) -> Scene:
+    """An almost empty world: one Lambert triangle `size` across at `distance` from the origin.
+    From within a few units of the origin it subtends about (size / distance)^2 sr, so rays cast
+    from there all miss, and what a ray returns is the background alone."""
+    B = _Builder()
+    d, s = float(distance), float(size)
+    B.bundle([[d, d, d], [d + s, d, d], [d, d + s, d]], np.tile([0, 0, 1], (3, 1)), [[0, 1, 2]],
+             B.material(LAMBERT, (0.5, 0.5, 0.5)))
+    cam = [0.0, 0.0, 0.0, 0.0, 0.0, -1.0, 0, 1, 0, 0.01, 10000.0, 1.0, 1.0]
+    return B.build(cam, "far_speck")
+
+
 def lit_floor(albedo=0.5, light_radiance=10.0, target=(0.3, 0.0, 0.2), fovy=1e-3, width=64, height=64) -> Scene:
     """A Lambert floor (y = 0, 20 x 20) under one emissive triangle facing down at y = 1.5,
     seen through a very narrow camera aimed at `target`, so every camera sample lands within
@@ -380,6 +392,31 @@ def lit_floor(albedo=0.5, light_radiance=10.0, target=(0.3, 0.0, 0.2), fovy=1e-3
     B.area_lights(lb)
     cam = [0.0, 1.0, 3.0, target[0], target[1], target[2], 0, 1, 0, 0.01, 10000.0, fovy, width / height]
     return B.build(cam, "lit_floor", {"light": lt, "albedo": albedo, "radiance": light_radiance})
+
+
+def lit_floor_many(n_lights=40, seed=5, albedo=0.5, target=(0.3, 0.0, 0.2), fovy=1e-3, width=64, height=64) -> Scene:
+    """The lit_floor set-up under n_lights emissive triangles facing down at heights 1.2-2.0, each its
+    own bundle and material with radiance U[1, 30] per channel, so the lights' powers differ. With
+    depth 1 the image estimates the sum of the triangles' closed-form direct light at the target."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    B = _Builder()
+    B.quad([[-10, 0, -10], [-10, 0, 10], [10, 0, 10], [10, 0, -10]], [0, 1, 0],
+           B.material(LAMBERT, (albedo, albedo, albedo)))
+    tris, rad = [], []
+    for _ in range(n_lights):
+        c = np.array([rng.uniform(-2, 2), rng.uniform(1.2, 2.0), rng.uniform(-2, 2)])
+        s = rng.uniform(0.1, 0.4)
+        a = rng.uniform(0, 2 * np.pi)
+        # corners ordered so that the triangle's normal (p2 - p1) x (p3 - p1) faces down
+        lt = np.array([c + s * np.array([np.cos(a + k * 2.0944), 0, -np.sin(a + k * 2.0944)]) for k in range(3)],
+                      np.float32)[::-1]
+        e = rng.uniform(1, 30, 3)
+        lb = B.bundle(lt, np.tile([0, -1, 0], (3, 1)), [[0, 1, 2]], B.material(EMISSIVE, e))
+        B.area_lights(lb)
+        tris.append(lt)
+        rad.append(np.asarray(e, np.float32))
+    cam = [0.0, 1.0, 3.0, target[0], target[1], target[2], 0, 1, 0, 0.01, 10000.0, fovy, width / height]
+    return B.build(cam, "lit_floor_many", {"lights": np.stack(tris), "radiance": np.stack(rad), "albedo": albedo})
 
 
 def _grid_quad(B, origin, du, dv, nu, nv, normal, mat, keep=None):

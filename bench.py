@@ -338,6 +338,124 @@ def probe_guided_comparison(ctx, scene, probes) -> dict:
             "efficiency": round(var_p * t_p / (var_g * t_g), 3), "dc_mean_z": round(dc_z, 2)}
 
 
+def measured_ceilings(ctx) -> dict:
+    """Streaming-read ceilings of this GPU in GB/s (bsh_diag_stream_read): a 4 GiB buffer read
+    once per pass (HBM: 16x the Infinity Cache), a 128 MiB buffer (inside the 256 MB Infinity
+    Cache) and a 2 MiB buffer read whole by every XCD (inside each XCD's 4 MB L2)."""
+    import numpy as np
+    out = {}
+    for key, nbytes, reps, per_xcd in (("hbm_4GiB", 4 << 30, 2, False), ("infinity_cache_128MiB", 128 << 20, 20, False),
+                                       ("l2_2MiB_per_xcd", 2 << 20, 100, True)):
+        buf = ctx.alloc(nbytes // 4, np.float32)
+        out[key] = round(ctx.stream_read_gbs(buf, reps, per_xcd), 1)
+        buf.free()
+    return out
+
+
+def ray_classes(ls, s_node, s_tri) -> dict:
+    """Per ray class, from the per-launch statistics of a counting frame: camera rays (launch 0),
+    extension and MIS rays (closest hit, launches >= 1) and shadow rays (any hit, launches >= 1),
+    with node visits and triangle tests per ray and B_ray = 32 (ray) + 16 (hit) + S_node V_node +
+    S_tri V_tri."""
+    def cls(rays, nodes, tris):
+        v_n, v_t = nodes / max(rays, 1), tris / max(rays, 1)
+        return {"rays": int(rays), "node_visits_per_ray": round(v_n, 3), "tri_tests_per_ray": round(v_t, 3),
+                "bytes_per_ray": round(48 + s_node * v_n + s_tri * v_t, 1)}
+    sec = ls[1:]
+    return {"camera": cls(ls[0]["rays_closest"], ls[0]["nodes_closest"], ls[0]["tris_closest"]),
+            "extension_mis": cls(sum(l["rays_closest"] for l in sec), sum(l["nodes_closest"] for l in sec),
+                                 sum(l["tris_closest"] for l in sec)),
+            "shadow": cls(sum(l["rays_any"] for l in sec), sum(l["nodes_any"] for l in sec),
+                          sum(l["tris_any"] for l in sec))}
+
+
+def isolated_launches(ctx, scene, classes, seed) -> dict:
+    """One untimed frame of the workload on one wavefront lane at the two-lane default's per-lane
+    grid (3 trace workgroups per CU), so no trace launch shares the GPU with another lane: per
+    launch class, the summed HIP-event durations on the launch stream and the algorithmic bytes
+    of the rays it traced (rays x that class's B_ray)."""
+    old = ctx.tuning()
+    try:
+        ctx.set_tuning(0, 3, 0, 0, 0, 1)
+        scene.render(W, H, GRID, DEPTH, seed, download=False)
+        ctx.sync()
+        ls = ctx.launch_stats()
+        st = scene.stats()
+    finally:
+        ctx.set_tuning(**old)
+    b = {k: classes[k]["bytes_per_ray"] for k in classes}
+    cam_bytes = ls[0]["rays_closest"] * b["camera"]
+    sec_bytes = sum(l["rays_closest"] * b["extension_mis"] + l["rays_any"] * b["shadow"] for l in ls[1:])
+    cam_ms = ls[0]["trace_ms"]
+    sec_ms = sum(l["trace_ms"] for l in ls[1:])
+    return {"launches": st["trace_launches"], "trace_ms": cam_ms + sec_ms, "bytes": cam_bytes + sec_bytes,
+            "camera": {"ms": cam_ms, "bytes": cam_bytes, "rays": ls[0]["rays_closest"]},
+            "secondary": {"ms": sec_ms, "bytes": sec_bytes,
+                          "rays": sum(l["rays_closest"] + l["rays_any"] for l in ls[1:])}}
+
+
+def batch_line(ctx, args) -> dict:
+    """The batched workload (64 seeded terrain scenes, 1k-2M triangles, 128^2-1024^2, 4-64 spp,
+    depth 4-12) with the library's default launch shape captured as one hipGraph: Mrays/s over
+    `steps` runs, against the same jobs rendered one bsh_render call each."""
+    from banshee_amd import hip
+    from banshee_amd.scenes import batch_jobs, terrain_for_tris
+    cfgs = batch_jobs(n=64, seed=100)
+    scenes, jobs = [], []
+    for c in cfgs:
+        s = hip.Scene(ctx, terrain_for_tris(c["tris"], seed=c["seed"], width=c["res"], height=c["res"]),
+                      width=args.bvh_width)
+        scenes.append(s)
+        jobs.append(hip.BatchJob(s, "render", width=c["res"], height=c["res"], img_grid=c["grid"],
+                                 max_depth=c["depth"], seed=c["seed"]))
+    tuning = {"use_graph": 1}
+    ctx.batch_run(jobs, tuning=tuning)
+    ctx.sync()
+    rays = 0
+    t0 = time.perf_counter()
+    steps = max(1, min(args.steps, 5))
+    for _ in range(steps):
+        ctx.batch_run(jobs, tuning=tuning)
+        st = ctx.stats()
+        rays += st["rays_closest"] + st["rays_any"]
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / steps
+    t1 = time.perf_counter()
+    for j, c in zip(jobs, cfgs):
+        j.scene.render(c["res"], c["res"], c["grid"], c["depth"], c["seed"], accum=j.accum, download=False)
+    ctx.sync()
+    solo = time.perf_counter() - t1
+    return {"value": round(rays / steps / dt / 1e6, 1), "unit": "Mrays/s", "s_per_batch": round(dt, 4),
+            "one_by_one_s": round(solo, 4), "batch_speedup": round(solo / dt, 3), "steps": steps,
+            "tris_total": int(sum(s.info()["tris"] for s in scenes)), "tuning": "library default, one hipGraph",
+            "workload": "64 seeded terrain scenes, tris log-uniform 1k-2M, 128^2-1024^2, 4-64 spp, depth 4-12"}
+
+
+def probes_line(ctx, args) -> dict:
+    """SH probe baking: 4096 probes (16^3 grid) in a 2M-triangle terrain scene, 1024 rays per
+    probe, depth 6, order-3 SH, in one wavefront: Mrays/s and probes/s."""
+    from banshee_amd import hip
+    from banshee_amd.scenes import probe_grid, terrain_for_tris
+    sc = terrain_for_tris(2_000_000, seed=7)
+    scene = hip.Scene(ctx, sc, width=args.bvh_width)
+    probes = probe_grid(sc, 16)
+    coeffs = ctx.alloc((len(probes), 9, 3), "float32")
+    scene.bake_sh(probes, 1024, 2, 6, 1, coeffs=coeffs, download=False)
+    ctx.sync()
+    rays = 0
+    steps = max(1, min(args.steps, 10))
+    t0 = time.perf_counter()
+    for k in range(steps):
+        scene.bake_sh(probes, 1024, 2, 6, 2 + k, coeffs=coeffs, download=False)
+        st = scene.stats()
+        rays += st["rays_closest"] + st["rays_any"]
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / steps
+    return {"value": round(rays / steps / dt / 1e6, 1), "unit": "Mrays/s", "ms_per_bake": round(dt * 1e3, 3),
+            "probes_per_s": round(len(probes) / dt, 1), "steps": steps, "tris": int(sc.ntris),
+            "workload": "4096 probes (16^3 grid) in a 2M-triangle terrain, 1024 rays/probe, depth 6, order-3 SH"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -345,6 +463,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain time-to-RMSE comparison")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batch and probe-baking figures")
     ap.add_argument("--bvh-width", type=int, default=4, choices=(2, 4), help="BVH node width")
     ap.add_argument("--paths-per-chunk", type=int, default=0,
                     help="paths in flight per wavefront chunk (0 = library default, 2^23)")
@@ -385,15 +504,19 @@ def main():
     setup_s = time.time() - t0
     accum = ctx.alloc((H, W, 8), "float32")
 
-    # counting frame (untimed): node visits and triangle tests of the same kernel
+    # counting frames (untimed): node visits and triangle tests per ray class, with this layout and,
+    # for the layout-independent figure, with the binary layout of the same triangles
+    s_node = 128 if info["width"] == 4 else 64
     scene.render(W, H, 2, DEPTH, SEED, accum=accum, download=False, count=True)
     ctx.sync()
-    cst = scene.stats()
-    crays = cst["rays_closest"] + cst["rays_any"]
-    v_node = cst["node_visits"] / crays
-    v_tri = cst["tri_tests"] / crays
-    s_node = 128 if info["width"] == 4 else 64
-    b_ray = 32 + 16 + s_node * v_node + 48 * v_tri
+    classes = ray_classes(ctx.launch_stats(), s_node, 48)
+    bin_scene = hip.Scene(ctx, sc, width=2)
+    bin_scene.render(W, H, 2, DEPTH, SEED, accum=accum, download=False, count=True)
+    ctx.sync()
+    classes_bin = ray_classes(ctx.launch_stats(), 64, 36)
+    del bin_scene
+    ceilings = measured_ceilings(ctx)
+    iso = isolated_launches(ctx, scene, classes, SEED + 1000)
 
     for _ in range(args.warmup):
         scene.render(W, H, GRID, DEPTH, SEED, accum=accum, download=False)
@@ -414,9 +537,21 @@ def main():
     dt = time.perf_counter() - t1
 
     value = rays / dt / 1e6
-    avg_launch_s = trace_ms / 1e3 / trace_launches
-    bytes_per_launch = (rays / trace_launches) * b_ray
+    # the dominant kernel's own launch time: the isolated frame's HIP events (no second lane)
+    avg_launch_s = iso["trace_ms"] / 1e3 / iso["launches"]
+    bytes_per_launch = iso["bytes"] / iso["launches"]
     achieved = bytes_per_launch / avg_launch_s / 1e9
+    # the same bytes anchored to the timed steps' wall clock (includes every other kernel of the frame)
+    b_mix = iso["bytes"] / (iso["camera"]["rays"] + iso["secondary"]["rays"])
+    anchored = rays / dt * b_mix / 1e9
+    hbm, l2 = ceilings["hbm_4GiB"], ceilings["l2_2MiB_per_xcd"]
+    cam_gbs = iso["camera"]["bytes"] / (iso["camera"]["ms"] * 1e-3) / 1e9
+    sec_gbs = iso["secondary"]["bytes"] / (iso["secondary"]["ms"] * 1e-3) / 1e9
+    cb = classes_bin
+    b_ref = (iso["camera"]["rays"] * cb["camera"]["bytes_per_ray"] +
+             iso["secondary"]["rays"] * (cb["extension_mis"]["bytes_per_ray"] * cb["extension_mis"]["rays"] +
+                                         cb["shadow"]["bytes_per_ray"] * cb["shadow"]["rays"]) /
+             max(cb["extension_mis"]["rays"] + cb["shadow"]["rays"], 1))
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -441,12 +576,31 @@ def main():
                      "traffic_source": "profiles/k_trace_refill_terrain3M_pmc.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
                      "kernel": "k_trace_refill" if int(info["width"]) == 4 and args.refill_lanes != -1 else "k_trace_queue",
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4),
-                     "bytes_per_ray": round(b_ray, 1), "node_visits_per_ray": round(v_node, 2),
-                     "tri_tests_per_ray": round(v_tri, 2), "trace_share": round(trace_ms / (dt * 1e3), 3)},
+                     "launch_timing": "HIP events on the launch stream, in an untimed frame of the workload on one "
+                                      "lane at the default per-lane grid (3 workgroups/CU), so no launch overlaps "
+                                      "another lane's kernels",
+                     "avg_launch_ms_two_lanes": round(trace_ms / trace_launches, 4),
+                     "bytes_per_ray": round(b_mix, 1), "anchored_to_steps_gbs": round(anchored, 1),
+                     "anchored_frac": round(anchored / HBM_PEAK_GBS, 4),
+                     "measured_ceilings_gbs": ceilings,
+                     "frac_of_measured": {"hbm": round(achieved / hbm, 4), "l2": round(achieved / l2, 4)},
+                     "incoherent_bounces": {"rays": int(iso["secondary"]["rays"]), "achieved": round(sec_gbs, 1),
+                                            "frac_of_measured_hbm": round(sec_gbs / hbm, 4),
+                                            "frac_of_measured_l2": round(sec_gbs / l2, 4)},
+                     "camera_rays": {"rays": int(iso["camera"]["rays"]), "achieved": round(cam_gbs, 1),
+                                     "frac_of_measured_hbm": round(cam_gbs / hbm, 4)},
+                     "ray_classes": classes,
+                     "layout_independent": {"s_node": 64, "s_tri": 36, "ray_classes": classes_bin,
+                                            "achieved": round(b_ref / (iso["trace_ms"] * 1e-3) / 1e9, 1),
+                                            "frac_of_measured_hbm": round(b_ref / (iso["trace_ms"] * 1e-3) / 1e9 / hbm,
+                                                                          4)}},
         "cpu_baseline": None,
     }
     if not args.no_guided:
         out["guided"] = guided_comparison(ctx, args.bvh_width)
+    if not args.no_extra:
+        out["batch"] = batch_line(ctx, args)
+        out["probes"] = probes_line(ctx, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(None)
     print(json.dumps(out), flush=True)

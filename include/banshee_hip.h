@@ -120,7 +120,8 @@ typedef struct
     int32_t spp_begin, spp_count;  /* render samples [spp_begin, spp_begin+spp_count) of each pixel; count 0 = all */
     int32_t max_depth;             /* GiTracer maxdepth_ */
     uint32_t seed;
-    int32_t chunk_spp;             /* samples per pixel per wavefront chunk, 0 = auto */
+    int32_t chunk_spp;             /* reserved, must be 0 (BSH_ERR_INVALID otherwise); the paths in flight per
+                                      wavefront chunk are bsh_tuning.paths_per_chunk */
     float bg[3];                   /* World::bgcolor_ */
     int32_t flags;                 /* BSH_RENDER_* */
 } bsh_render_params;
@@ -168,6 +169,28 @@ bsh_status bsh_scene_create(bsh_context* ctx, const float* verts, size_t nverts,
                             size_t nnormals, const int32_t* tri_vidx, const int32_t* tri_nidx,
                             const int32_t* tri_material, size_t ntris, const bsh_bundle* bundles,
                             size_t nbundles, bsh_scene** out);
+/* Asset import, host only (no device needed): replaces AssimpAssetImporter::Import
+   (import/assimp_assetimporter.cpp:20-146) for Wavefront OBJ/MTL files. What its onmaterial_,
+   onprimitive_ and onlight_ callbacks would receive comes back as the arrays of
+   bsh_scene_create / bsh_scene_set_materials / bsh_scene_set_lights:
+   - materials: Assimp's default material (Lambert 0.6) first, then the MTL's in file order; a
+     material named "light" or with a nonzero Ke is Emissive(0.6 (60, 55, 40)), any other
+     Lambert(Kd) (assimp_assetimporter.cpp:74-85);
+   - one bundle per non-empty run of faces between `o` / `usemtl` statements, in file order
+     (Assimp's OBJ loader), its distinct (position, normal) pairs as its vertices;
+   - triangles: quads split at their concave corner if any, else (a b c), (a c d); larger convex
+     polygons fanned; faces without normals get their triangle's normal;
+   - one AreaLight(shape, bundle) per triangle of every emissive bundle (:137-145).
+   Texture maps are not applied (ignored_maps counts the materials that name one). Call with every
+   array NULL to fill *counts, then again with arrays of at least those sizes (read from *counts). */
+typedef struct
+{
+    size_t nverts, nnormals, ntris, nbundles, nmaterials, nlights;
+    size_t ignored_maps;
+} bsh_asset_counts;
+bsh_status bsh_import_obj(const char* path, bsh_asset_counts* counts, float* verts, float* normals,
+                          int32_t* tri_vidx, int32_t* tri_nidx, int32_t* tri_material, bsh_bundle* bundles,
+                          bsh_material* materials, bsh_area_light* lights);
 /* World::materials_ */
 bsh_status bsh_scene_set_materials(bsh_scene* s, const bsh_material* mats, size_t n);
 /* World::lights_ (AreaLight only) */
@@ -222,7 +245,7 @@ typedef struct
     int32_t lmax;             /* 0..4; the reference's order-3 probes use lmax = 2 (9 coefficients) */
     int32_t max_depth;
     uint32_t seed;
-    int32_t chunk_probes;     /* probes per wavefront chunk, 0 = auto */
+    int32_t chunk_probes;     /* reserved, must be 0 (BSH_ERR_INVALID otherwise); see bsh_tuning.paths_per_chunk */
     float tmin;               /* probe ray tmin */
     float bg[3];
     int32_t flags;            /* BSH_RENDER_COUNT and BSH_RENDER_NO_QUIRKS honoured */
@@ -261,6 +284,8 @@ typedef struct
     int32_t use_graph;
     int32_t refill_lanes;
     int32_t claim_rays;
+    int32_t lanes;     /* wavefront lanes: 0 = automatic (two when the run has chunks to alternate and writes
+                          no training records), 1 = one lane on the caller's stream, 2 = two lanes */
 } bsh_tuning;
 
 enum { BSH_JOB_RENDER = 0, BSH_JOB_BAKE_SH = 1 };
@@ -290,6 +315,19 @@ bsh_status bsh_batch_run(bsh_context* ctx, const bsh_job* jobs, size_t njobs, co
 bsh_status bsh_context_set_tuning(bsh_context* ctx, const bsh_tuning* t);
 bsh_status bsh_context_get_tuning(const bsh_context* ctx, bsh_tuning* t);
 bsh_status bsh_context_stats_get(const bsh_context* ctx, bsh_render_stats* out);
+/* Per bounce launch of the last bsh_render / bsh_bake_sh / bsh_batch_run / guided run on a context
+   (profiling): entry b sums the b-th trace launch of every chunk. Launch 0 traces the camera (or
+   probe) rays; launch b >= 1 the extension and MIS rays (closest hit) and the shadow rays (any
+   hit) that shade pass b-1 emitted. */
+typedef struct
+{
+    uint64_t rays_closest, rays_any;
+    uint64_t nodes_closest, tris_closest;   /* node visits and triangle tests, BSH_RENDER_COUNT only */
+    uint64_t nodes_any, tris_any;
+    double trace_ms;                        /* summed HIP-event durations of these launches (0 in graph mode) */
+} bsh_launch_stats;
+/* Copies min(cap, launches) entries to out and the number of launches of the last run to *n. */
+bsh_status bsh_context_launch_stats(const bsh_context* ctx, bsh_launch_stats* out, size_t cap, size_t* n);
 /* Autotuner: times the batch under every candidate (paths_per_chunk x trace_blocks_per_cu),
    ncand each, and stores the fastest as the context tuning (also returned in best).
    The jobs' outputs are overwritten. NULL candidate arrays use the built-in sweep. */
@@ -392,6 +430,14 @@ bsh_status bsh_diag_camera(bsh_context* ctx, const bsh_camera* cam, size_t n, co
 /* Device keyed sample stream values ks_u32(ks_base(seed, camsample), stream, counter). Host pointers. */
 bsh_status bsh_diag_keyed(bsh_context* ctx, uint32_t seed, size_t n, const uint32_t* camsample,
                           const uint32_t* stream, const uint32_t* counter, uint32_t* out);
+
+/* Memory ceiling for the roofline (no reference counterpart): times one streaming read of `reps`
+   passes over `bytes` of device memory at `buf` (a multiple of 16, at least 4 KiB) after one
+   untimed pass, and returns the duration in *ms. per_xcd = 0: every byte is read once per pass
+   (HBM for buffers beyond the 256 MB Infinity Cache, the Infinity Cache for buffers inside it);
+   per_xcd = 1: each of the 8 XCDs reads the whole buffer every pass (its L2, for buffers that fit
+   one XCD's 4 MB L2), so a pass moves 8 x bytes. */
+bsh_status bsh_diag_stream_read(bsh_context* ctx, const void* buf, size_t bytes, int reps, int per_xcd, double* ms);
 
 #ifdef __cplusplus
 }

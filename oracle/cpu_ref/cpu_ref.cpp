@@ -787,6 +787,55 @@ V3 get_li(const Scene& s, V3 o, V3 d, float tmin, float tmax, Draws& D, int maxd
     return V3{clampf(r.x, 0.f, MAXRADIANCE), clampf(r.y, 0.f, MAXRADIANCE), clampf(r.z, 0.f, MAXRADIANCE)};
 }
 
+/* An estimator of the integral the quirk-free GPU tracer estimates, sharing none of its light
+   sampling, MIS or light pdfs: paths scatter by BSDF sampling alone and collect the emitted
+   radiance GetLe = e max(n.wo, 0) (emissive.h:84-95) wherever they land. Emitters end paths.
+   Vertices 0 .. maxdepth-1 scatter, so the emitters it reaches are those the GPU tracer's
+   next-event estimation at vertices 0 .. maxdepth-1 reaches. The continuation draws a fresh BSDF
+   sample per vertex, weighs it by Evaluate(view, dir) |n.dir| / GetPdf(view, dir), and applies
+   Russian roulette after vertex 3 as GiTracer does (gitracer.cpp:116-125). No reference
+   counterpart: it is the independent check of the quirk-free estimator at depth > 1. */
+V3 get_li_bsdf_only(const Scene& s, V3 o, V3 d, float tmin, float tmax, Draws& D, int maxdepth, V3 bg)
+{
+    Hit hit;
+    if (!world_closest(s, o, d, tmin, tmax, hit, nullptr)) return bg;
+    V3 L = V3{0, 0, 0}, T = V3{1.f, 1.f, 1.f}, rd = d;
+    for (int b = 0;; ++b)
+    {
+        const Mat& mat = s.mats[hit.m];
+        V3 wo = neg(rd);
+        if (mat.type == 2)
+        {
+            L = add(L, mulv(T, le_emissive(mat.c, hit.n, wo)));
+            break;
+        }
+        if (b >= maxdepth) break;
+        float ux, uy;
+        D.sample2d(KS_BRDF, ux, uy);
+        V3 wi;
+        float pdf = 0.f;
+        int type = 0;
+        (void)mat_sample(mat, hit, ux, uy, wo, wi, pdf, type);
+        wi = nrm(wi);
+        V3 f = mat_eval(mat, hit, wo, wi);
+        float p = mat_pdf(mat, hit, wo, wi);
+        if (!(p > 0.f) || sqn(f) == 0.f) break;
+        T = mulv(T, muls(f, std::fabs(dot(hit.n, wi)) / p));
+        if (b > 3)
+        {
+            float rnd = (float)D.rnd() / 2147483647;
+            float lum = 0.2126f * T.x + 0.7152f * T.y + 0.0722f * T.z;
+            float q = std::min(0.5f, lum);
+            if (rnd > q) break;
+            T = muls(T, 1.f / q);
+        }
+        V3 no = hit.p;
+        if (!world_closest(s, no, wi, 0.01f, 1000000.f, hit, nullptr)) break;
+        rd = wi;
+    }
+    return L;
+}
+
 struct Cam
 {
     V3 eye, fwd, right, up;
@@ -1029,6 +1078,53 @@ void cref_render_keyed(void* h, int W, int H, int x0, int y0, int tw, int th, in
     if (rays) { rays[0] = (long long)c[0]; rays[1] = (long long)c[1]; }
 }
 
+/* The BSDF-sampling-only estimator (get_li_bsdf_only) over a tile of the image: spp samples per
+   pixel on the stratified image grid, drawn from the keyed stream of `seed`; Sigma L, Sigma L^2 and
+   the count per pixel (tile-local, not y-flipped). Rows are spread over `threads` workers
+   (0 = hardware concurrency). */
+void cref_render_bsdf_only(void* h, int W, int H, int x0, int y0, int tw, int th, int img_grid, int depth,
+                           unsigned seed, float const* bg, double* sum, double* sumsq, float* cnt, int threads)
+{
+    const Scene& s = *(Scene*)h;
+    Cam k = make_cam(s.cam);
+    const int spp = img_grid * img_grid;
+    const V3 B = bg ? at(bg, 0) : V3{0, 0, 0};
+    int nth = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nth; ++t)
+        pool.emplace_back([&]() {
+            for (;;)
+            {
+                int yy = next.fetch_add(1);
+                if (yy >= th) break;
+                int y = y0 + yy;
+                for (int x = x0; x < x0 + tw; ++x)
+                {
+                    int i = yy * tw + (x - x0);
+                    double S[3] = {0, 0, 0}, Q[3] = {0, 0, 0};
+                    for (int si = 0; si < spp; ++si)
+                    {
+                        Draws D;
+                        D.base = ks_base(seed, (uint32_t)(((uint64_t)y * W + x) * spp + si));
+                        for (int c = 0; c < 8; ++c) D.ctr[c] = 0;
+                        float ux, uy;
+                        D.sample2d(KS_IMG, ux, uy);
+                        float cell = 1.f / img_grid;
+                        float sx = ((si % img_grid) + ux) * cell, sy = ((si / img_grid) + uy) * cell;
+                        float imx = (float)x / W + (1.f / W) * sx, imy = (float)y / H + (1.f / H) * sy;
+                        V3 L = get_li_bsdf_only(s, k.eye, cam_dir(k, imx, imy), k.znear, k.zfar, D, depth, B);
+                        S[0] += L.x; S[1] += L.y; S[2] += L.z;
+                        Q[0] += (double)L.x * L.x; Q[1] += (double)L.y * L.y; Q[2] += (double)L.z * L.z;
+                    }
+                    for (int c = 0; c < 3; ++c) { sum[3 * i + c] = S[c]; sumsq[3 * i + c] = Q[c]; }
+                    cnt[i] = (float)spp;
+                }
+            }
+        });
+    for (auto& p : pool) p.join();
+}
+
 /* Multi-threaded wall-clock render of the full W x H frame (rows handed out dynamically to
    `threads` workers, 0 = hardware concurrency); returns seconds, rays[0..1] = closest, any. */
 double cref_time_render(void* h, int W, int H, int img_grid, int nee_grid, int depth, unsigned seed, int threads,
@@ -1211,10 +1307,12 @@ void cref_sh_irradiance(int lmax, float const* coeffs, float const* normals, int
 /* Probe baking restated (no reference counterpart): for probe p and ray j the keyed
    sample g = p*N + j draws a stratified uniform-sphere direction (stream 4, draws 0/1)
    and GiTracer::GetLi estimates L; coefficients c_i = (4 pi / N) sum_j L Y_i(w_j) are
-   accumulated in double. coeffs: nprobes x (lmax+1)^2 x 3; radiance (optional): nprobes x N x 3. */
+   accumulated in double. bg (3 floats, may be NULL = black) is World::bgcolor_, the radiance of a
+   probe ray that hits nothing. coeffs: nprobes x (lmax+1)^2 x 3; radiance (optional): nprobes x N x 3. */
 void cref_bake_sh(void* h, float const* probes, int nprobes, int N, int lmax, int depth, unsigned seed, float tmin,
-                  double* coeffs, float* radiance, int threads)
+                  float const* bg, double* coeffs, float* radiance, int threads)
 {
+    const V3 B = bg ? at(bg, 0) : V3{0, 0, 0};
     const Scene& s = *(Scene*)h;
     int g = (int)std::lround(std::sqrt((double)N));
     int nt = (lmax + 1) * (lmax + 1);
@@ -1243,7 +1341,7 @@ void cref_bake_sh(void* h, float const* probes, int nprobes, int N, int lmax, in
                     float r = std::sqrt(std::max(0.f, 1.f - z * z));
                     float phi = 2.f * PI * v;
                     V3 d = v3(r * cosf(phi), r * sinf(phi), z);
-                    V3 L = get_li(s, o, d, tmin, 1000000.f, D, depth, V3{0, 0, 0}, nullptr);
+                    V3 L = get_li(s, o, d, tmin, 1000000.f, D, depth, B, nullptr);
                     if (radiance)
                     {
                         float* q = radiance + 3 * ((size_t)p * N + j);

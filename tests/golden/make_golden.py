@@ -2,7 +2,7 @@
 (oracle/_ref/liboracle_ref.so, built by `make -C oracle ref` from the reference's own
 sources). Run in a checkout where the reference is present:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture ...]   # all fixtures, or only the named ones
 
 Every fixture holds inputs and the reference's outputs only, plus the digest of the
 generated scene it was made on, so a drift of the scene generator is detected.
@@ -29,6 +29,13 @@ CORNELL_CROP = (120, 60, 16, 16)
 TERRAIN_TRIS = 100_000
 TERRAIN = dict(W=192, H=108, grid=2, nee=1, depth=8, seed=2)
 TERRAIN_CROP = (80, 40, 32, 24)
+# the reference's Cornell asset with BuildWorldCornellBox's camera (UnitTest/basic_features.h:239)
+CORNELL_OBJ = dict(W=128, H=128, grid=4, nee=1, depth=5, seed=3)
+CORNELL_OBJ_CROP = (40, 30, 24, 24)
+CORNELL_OBJ_CAMERA = [0.0, 1.0, 3.5, 0.0, 1.0, 0.0, 0.0, 1.0, 0.0, 0.01, 10000.0, float(np.float32(np.pi / 4)), 1.0]
+# recorded rays of the terrain: every TERRAIN_RAYS_STRIDE-th BVH query of a render of this crop
+TERRAIN_RAYS_CROP = (16, 10, 160, 88)
+TERRAIN_RAYS_STRIDE = 5
 
 
 def unit(rng, n):
@@ -68,23 +75,27 @@ def bsdf_cases(rng, n):
     return frames, view, u, ea, eb
 
 
-def main():
+def main(only=()):
     o = Oracle("ref")
     rng = np.random.Generator(np.random.PCG64(2024))
-    out = {}
+
+    def want(name):
+        return not only or name in only
 
     # ---------------- samplers / rng ----------------
     first_is_x = int(o.lib.oref_sample2d_first_is_x())
     mc = {s: o.mcrng(s, 64) for s in (1, 7, 12345)}
     strat = o.stratified(4, 3, 48)
-    np.savez_compressed(os.path.join(HERE, "rng.npz"), first_is_x=first_is_x, mc_seeds=np.array([1, 7, 12345]),
+    if want("rng"):
+        np.savez_compressed(os.path.join(HERE, "rng.npz"), first_is_x=first_is_x, mc_seeds=np.array([1, 7, 12345]),
                         mc=np.stack([mc[1], mc[7], mc[12345]]), strat_grid=4, strat_seed=3, strat=strat)
 
     # ---------------- camera ----------------
     cb = cornell_box()
     tr = terrain_for_tris(TERRAIN_TRIS, seed=1, width=TERRAIN["W"], height=TERRAIN["H"])
     samples = rng.uniform(0, 1, (256, 2)).astype(np.float32)
-    np.savez_compressed(os.path.join(HERE, "camera.npz"), cams=np.stack([cb.camera, tr.camera]), samples=samples,
+    if want("camera"):
+        np.savez_compressed(os.path.join(HERE, "camera.npz"), cams=np.stack([cb.camera, tr.camera]), samples=samples,
                         rays=np.stack([o.camera_rays(cb.camera, samples), o.camera_rays(tr.camera, samples)]))
 
     # ---------------- BSDFs ----------------
@@ -98,7 +109,8 @@ def main():
     n = 512
     cases = [bsdf_cases(rng, n) for _ in mats]
     res = [o.bsdf(m, *c) for m, c in zip(mats, cases)]
-    np.savez_compressed(os.path.join(HERE, "bsdf.npz"), mats=mats,
+    if want("bsdf"):
+        np.savez_compressed(os.path.join(HERE, "bsdf.npz"), mats=mats,
                         frames=np.stack([c[0] for c in cases]), view=np.stack([c[1] for c in cases]),
                         u=np.stack([c[2] for c in cases]), ea=np.stack([c[3] for c in cases]),
                         eb=np.stack([c[4] for c in cases]), out=np.stack(res))
@@ -112,7 +124,8 @@ def main():
     w /= np.linalg.norm(w, axis=1, keepdims=True)
     w[::5] = unit(rng, len(w[::5]))
     lres = np.stack([sc.light(li, p, u, w) for li in range(cb.lights.shape[0])])
-    np.savez_compressed(os.path.join(HERE, "light.npz"), digest=cb.digest(), p=p, u=u, w=w, out=lres)
+    if want("light"):
+        np.savez_compressed(os.path.join(HERE, "light.npz"), digest=cb.digest(), p=p, u=u, w=w, out=lres)
 
     # ---------------- spherical harmonics ----------------
     dirs = np.concatenate([unit(rng, 1000), np.array([[0, 0, 1], [0, 0, -1], [1, 0, 0], [-1, 0, 0], [0, 1, 0],
@@ -125,7 +138,8 @@ def main():
     const_proj = o.sh_project(np.ones((64, 128, 3), np.float32), 2)
     nrm = unit(rng, 200)
     irr = o.sh_irradiance(2, cin, nrm)
-    np.savez_compressed(os.path.join(HERE, "sh.npz"), dirs=dirs, ylm4=ylm4, cin=cin, conv=conv, env=env, proj=proj,
+    if want("sh"):
+        np.savez_compressed(os.path.join(HERE, "sh.npz"), dirs=dirs, ylm4=ylm4, cin=cin, conv=conv, env=env, proj=proj,
                         const_proj=const_proj, normals=nrm, irr=irr)
 
     # ---------------- GI on the Cornell box (cfg0) + recorded rays ----------------
@@ -139,10 +153,12 @@ def main():
     x0, y0, cw, ch = CORNELL_CROP
     _, _, _, PS, _ = sc.render_keyed(C["W"], C["H"], x0, y0, cw, ch, C["grid"], C["nee"], C["depth"], C["seed"],
                                      per_sample=True)
-    np.savez_compressed(os.path.join(HERE, "gi_cornell.npz"), digest=cb.digest(), sum=S, sumsq=Q, n=N,
+    if want("gi_cornell"):
+        np.savez_compressed(os.path.join(HERE, "gi_cornell.npz"), digest=cb.digest(), sum=S, sumsq=Q, n=N,
                         ray_counts=rays, crop=np.array(CORNELL_CROP), crop_samples=PS,
                         **{k: v for k, v in C.items()})
-    np.savez_compressed(os.path.join(HERE, "rays_cornell.npz"), digest=cb.digest(), rays=rr, kind=kind, prim=prim,
+    if want("rays_cornell"):
+        np.savez_compressed(os.path.join(HERE, "rays_cornell.npz"), digest=cb.digest(), rays=rr, kind=kind, prim=prim,
                         thit=thit)
 
     # ---------------- GI on a 100k-triangle terrain scene (cfg1 generator) ----------------
@@ -150,20 +166,48 @@ def main():
     st = o.scene(tr)
     t0 = time.time()
     x0, y0, cw, ch = TERRAIN_CROP
-    st.record(max_rays=24000, stride=23)
     S, Q, N, PS, rays = st.render_keyed(T["W"], T["H"], x0, y0, cw, ch, T["grid"], T["nee"], T["depth"], T["seed"],
                                         per_sample=True)
-    rr, kind, prim, thit = st.fetch(24000)
-    print(f"terrain crop {time.time() - t0:.1f}s rays={rays} recorded={len(rr)} tris={tr.ntris}")
-    np.savez_compressed(os.path.join(HERE, "gi_terrain.npz"), digest=tr.digest(), sum=S, sumsq=Q, n=N,
-                        ray_counts=rays, crop=np.array(TERRAIN_CROP), crop_samples=PS, tris=tr.ntris,
-                        **{k: v for k, v in T.items()})
-    np.savez_compressed(os.path.join(HERE, "rays_terrain.npz"), digest=tr.digest(), rays=rr, kind=kind, prim=prim,
-                        thit=thit)
+    print(f"terrain crop {time.time() - t0:.1f}s rays={rays} tris={tr.ntris}")
+    if want("gi_terrain"):
+        np.savez_compressed(os.path.join(HERE, "gi_terrain.npz"), digest=tr.digest(), sum=S, sumsq=Q, n=N,
+                            ray_counts=rays, crop=np.array(TERRAIN_CROP), crop_samples=PS, tris=tr.ntris,
+                            **{k: v for k, v in T.items()})
+    if want("rays_terrain"):
+        # camera, bounce and shadow rays of a wider crop: every TERRAIN_RAYS_STRIDE-th query
+        t0 = time.time()
+        x0, y0, cw, ch = TERRAIN_RAYS_CROP
+        st.record(max_rays=80000, stride=TERRAIN_RAYS_STRIDE)
+        _, _, _, _, rays = st.render_keyed(T["W"], T["H"], x0, y0, cw, ch, T["grid"], T["nee"], T["depth"],
+                                           T["seed"])
+        rr, kind, prim, thit = st.fetch(80000)
+        print(f"terrain rays {time.time() - t0:.1f}s queries={rays} recorded={len(rr)} "
+              f"(closest {(kind == 0).sum()}, any {(kind == 1).sum()})")
+        np.savez_compressed(os.path.join(HERE, "rays_terrain.npz"), digest=tr.digest(), rays=rr, kind=kind, prim=prim,
+                            thit=thit, crop=np.array(TERRAIN_RAYS_CROP), stride=TERRAIN_RAYS_STRIDE)
+    # ---------------- the reference's own Cornell asset (Resources/CornellBox/orig.objm) ----------------
+    # The World BuildWorldCornellBox builds (UnitTest/basic_features.h:235-270), from the arrays
+    # bsh_import_obj returns, rendered by the reference's GiTracer.
+    if want("gi_cornell_obj"):
+        from banshee_amd.hip import import_obj
+        O = CORNELL_OBJ
+        ob = import_obj(os.path.join(HERE, "cornell_obj", "orig.objm"), CORNELL_OBJ_CAMERA)
+        so = o.scene(ob)
+        t0 = time.time()
+        S, Q, N, _, rays = so.render_keyed(O["W"], O["H"], 0, 0, O["W"], O["H"], O["grid"], O["nee"], O["depth"],
+                                           O["seed"])
+        x0, y0, cw, ch = CORNELL_OBJ_CROP
+        _, _, _, PS, _ = so.render_keyed(O["W"], O["H"], x0, y0, cw, ch, O["grid"], O["nee"], O["depth"], O["seed"],
+                                         per_sample=True)
+        print(f"cornell asset {time.time() - t0:.1f}s rays={rays} tris={ob.ntris}")
+        np.savez_compressed(os.path.join(HERE, "gi_cornell_obj.npz"), digest=ob.digest(), sum=S, sumsq=Q, n=N,
+                            ray_counts=rays, crop=np.array(CORNELL_OBJ_CROP), crop_samples=PS,
+                            camera=np.asarray(CORNELL_OBJ_CAMERA, np.float32), **{k: v for k, v in O.items()})
+
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))

@@ -55,7 +55,8 @@ def _load(kind):
         getattr(lib, pre + "sh_project").argtypes = [vp, i, i, i, vp]
         getattr(lib, pre + "sh_irradiance").argtypes = [i, vp, vp, i, vp]
         if kind == "cpu":
-            lib.cref_bake_sh.argtypes = [vp, vp, i, i, i, i, ctypes.c_uint, ctypes.c_float, vp, vp, i]
+            lib.cref_bake_sh.argtypes = [vp, vp, i, i, i, i, ctypes.c_uint, ctypes.c_float, vp, vp, vp, i]
+            lib.cref_render_bsdf_only.argtypes = [vp, i, i, i, i, i, i, i, i, ctypes.c_uint, vp, vp, vp, vp, i]
         if kind == "ref":
             lib.oref_record_begin.argtypes = [vp, i, i]
             lib.oref_record_fetch.argtypes = [vp, vp, vp, vp, vp]
@@ -186,15 +187,28 @@ class OracleScene:
         t = self.o._fn("time_render")(self.h, W, H, img_grid, nee_grid, depth, seed, 1 if mt else 0, _p(rays))
         return t, rays
 
-    def bake_sh(self, probes, N, lmax, depth, seed, tmin=0.0, threads=0):
+    def bake_sh(self, probes, N, lmax, depth, seed, tmin=0.0, threads=0, bg=(0.0, 0.0, 0.0)):
         """CPU restatement only: (coeffs float64 [P, nt, 3], radiance float32 [P, N, 3])."""
         probes = f32(probes).reshape(-1, 3)
         P = probes.shape[0]
         nt = (lmax + 1) ** 2
         c = np.zeros((P, nt, 3), np.float64)
         r = np.zeros((P, N, 3), np.float32)
-        self.o.lib.cref_bake_sh(self.h, _p(probes), P, N, lmax, depth, seed, tmin, _p(c), _p(r), threads)
+        b = f32(bg)
+        self.o.lib.cref_bake_sh(self.h, _p(probes), P, N, lmax, depth, seed, tmin, _p(b), _p(c), _p(r), threads)
         return c, r
+
+    def render_bsdf_only(self, W, H, img_grid, depth, seed, crop=None, bg=(0.0, 0.0, 0.0), threads=0):
+        """CPU restatement only: the BSDF-sampling-only estimator of the quirk-free integral.
+        Returns (sum float64 [h, w, 3], sumsq float64 [h, w, 3], n float32 [h, w])."""
+        x0, y0, tw, th = crop if crop is not None else (0, 0, W, H)
+        s = np.zeros((th, tw, 3), np.float64)
+        q = np.zeros((th, tw, 3), np.float64)
+        n = np.zeros((th, tw), np.float32)
+        b = f32(bg)
+        self.o.lib.cref_render_bsdf_only(self.h, W, H, x0, y0, tw, th, img_grid, depth, seed, _p(b), _p(s), _p(q),
+                                         _p(n), threads)
+        return s, q, n
 
     def light(self, idx, p, u, w):
         p, u, w = f32(p), f32(u), f32(w)
