@@ -16,16 +16,41 @@ def triangle_points(tri, m):
     return np.concatenate([c_up, c_dn]), area
 
 
-def emitter_irradiance(p, n, tri, light_normal, cos_power, m=600):
+def segment_blocked(p, q, occluders):
+    """For each point of q (M x 3): does the open segment p -> q cross any triangle of
+    `occluders` (K x 3 x 3)? Moller-Trumbore in FP64, both faces blocking."""
+    p = np.asarray(p, np.float64)
+    d = np.asarray(q, np.float64) - p
+    out = np.zeros(len(d), bool)
+    for tri in np.asarray(occluders, np.float64):
+        p1, p2, p3 = tri
+        e1, e2 = p2 - p1, p3 - p1
+        s1 = np.cross(d, e2)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            inv = 1.0 / (s1 @ e1)
+            s = p - p1
+            b1 = (s1 @ s) * inv
+            s2 = np.cross(s, e1)
+            b2 = (d @ s2) * inv
+            t = (s2 @ e2) * inv
+        out |= (b1 >= 0) & (b2 >= 0) & (b1 + b2 <= 1) & (t > 1e-9) & (t < 1 - 1e-9)
+    return out
+
+
+def emitter_irradiance(p, n, tri, light_normal, cos_power, m=600, occluders=None):
     """E(p) = integral over the emitter of cos_l^cos_power cos_p / r^2 dA (unit emission),
-    for a one-sided emitter facing `light_normal`; midpoint rule on m^2 sub-triangles."""
+    for a one-sided emitter facing `light_normal`; midpoint rule on m^2 sub-triangles. With
+    `occluders`, sub-triangles whose centroid is hidden from p by one of them count zero."""
     c, dA = triangle_points(tri, m)
     d = c - np.asarray(p, np.float64)
     r2 = np.sum(d * d, axis=1)
     w = d / np.sqrt(r2)[:, None]
     cos_p = np.maximum(w @ np.asarray(n, np.float64), 0)
     cos_l = np.maximum(-(w @ np.asarray(light_normal, np.float64)), 0)
-    return float(np.sum(cos_l ** cos_power * cos_p / r2) * dA)
+    f = cos_l ** cos_power * cos_p / r2
+    if occluders is not None and len(occluders):
+        f = np.where(segment_blocked(p, c, occluders), 0.0, f)
+    return float(np.sum(f) * dA)
 
 
 def lambert_polygon_irradiance(p, n, poly):
