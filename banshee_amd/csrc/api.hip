@@ -88,6 +88,8 @@ bsh_status bsh_context_create(int device, bsh_context** out)
     std::string arch = prop.gcnArchName;
     if (arch.rfind("gfx950", 0) != 0)
         return fail(BSH_ERR_NO_DEVICE, "bsh_context_create: device is " + arch + ", this build targets gfx950");
+    bsh_status sv = env_scratch_init(device);
+    if (sv != BSH_OK) return sv;
     bsh_context* c = new bsh_context();
     c->device = device;
     c->num_cus = prop.multiProcessorCount;

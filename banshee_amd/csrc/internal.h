@@ -22,6 +22,14 @@ bsh_status hip_fail(hipError_t e, const char* what);
 // BSH_OK, or BSH_ERR_INVALID naming `who` and the field out of range.
 bsh_status validate_tuning(const bsh_tuning& t, const char* who);
 
+// Scratch of bsh_sh_project_env_noalloc: 4 MiB per device (tables and block partials).
+constexpr size_t kEnvScratchFloats = (size_t)1 << 20;
+constexpr int kMaxDevices = 64;
+// Looks up the device address of the scratch array on `device` (once per device).
+bsh_status env_scratch_init(int device);
+// That address, or NULL before env_scratch_init succeeded on `device`.
+float* env_scratch_ptr(int device);
+
 #define BSH_HIP(call)                                        \
     do                                                       \
     {                                                        \

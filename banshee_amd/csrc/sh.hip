@@ -3,7 +3,10 @@
 // and the basis diagnostic. Probe baking lives with the wavefront kernels in render.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "internal.h"
@@ -103,6 +106,90 @@ __global__ void k_diag_sh(int lmax, uint32_t n, const float* dirs, float* out)
     for (int i = 0; i < nt; ++i) out[(size_t)t * nt + i] = y[i];
 }
 
+// The sin/cos tables exactly as the reference computes them on the host
+// (shproject.cpp:85-104), laid out as sin(theta)[H], cos(theta)[H], sin(phi)[W], cos(phi)[W];
+// writes entries [first, first + n) of that concatenation.
+static void env_tables(int W, int H, size_t first, size_t n, float* out)
+{
+    float thetastep = kPi / H, phistep = 2.f * kPi / W;
+    float theta0 = kPi / H / 2, phi0 = 2.f * kPi / W / 2;
+    for (size_t k = 0; k < n; ++k)
+    {
+        size_t i = first + k;
+        if (i < (size_t)H) out[k] = std::sin(theta0 + (int)i * thetastep);
+        else if (i < 2 * (size_t)H) out[k] = std::cos(theta0 + (int)(i - H) * thetastep);
+        else if (i < 2 * (size_t)H + W) out[k] = std::sin(phi0 + (int)(i - 2 * H) * phistep);
+        else out[k] = std::cos(phi0 + (int)(i - 2 * H - W) * phistep);
+    }
+}
+
+static size_t env_partial_floats(int W, int H, int lmax)
+{
+    size_t nb = ((size_t)W * H + kShBlock - 1) / kShBlock;
+    return nb * 3 * (lmax + 1) * (lmax + 1);
+}
+
+static bsh_status check_env_args(bsh_context* ctx, const float* env, int width, int height, int lmax,
+                                 const float* coeffs, const char* who)
+{
+    std::string w(who);
+    if (!ctx || !env || !coeffs) return fail(BSH_ERR_INVALID, w + ": NULL argument");
+    if (width <= 0 || height <= 0) return fail(BSH_ERR_INVALID, w + ": bad size");
+    if (lmax < 0 || lmax > kShMaxL) return fail(BSH_ERR_UNSUPPORTED, w + ": lmax must be 0..4");
+    if ((size_t)width * height >= ((size_t)1 << 31)) return fail(BSH_ERR_UNSUPPORTED, w + ": too large");
+    return BSH_OK;
+}
+
+// Both passes over `tab` (the four tables, 2(W+H) floats) and `partial` (one block's
+// 3(lmax+1)^2 sums per kShBlock texels, every entry written by pass 1 before pass 2 reads it).
+static hipError_t launch_env_projection(const float* env, int W, int H, int lmax, const float* tab, float* partial,
+                                        float* coeffs, hipStream_t st)
+{
+    const float *sth = tab, *cth = tab + H, *sph = tab + 2 * H, *cph = tab + 2 * H + W;
+    int nt3 = 3 * (lmax + 1) * (lmax + 1);
+    int nb = (int)(((size_t)W * H + kShBlock - 1) / kShBlock);
+    dim3 g(nb), b(kShBlock);
+    switch (lmax)
+    {
+    case 0: hipLaunchKernelGGL(k_env_partial<0>, g, b, 0, st, env, W, H, sth, cth, sph, cph, partial); break;
+    case 1: hipLaunchKernelGGL(k_env_partial<1>, g, b, 0, st, env, W, H, sth, cth, sph, cph, partial); break;
+    case 2: hipLaunchKernelGGL(k_env_partial<2>, g, b, 0, st, env, W, H, sth, cth, sph, cph, partial); break;
+    case 3: hipLaunchKernelGGL(k_env_partial<3>, g, b, 0, st, env, W, H, sth, cth, sph, cph, partial); break;
+    default: hipLaunchKernelGGL(k_env_partial<4>, g, b, 0, st, env, W, H, sth, cth, sph, cph, partial); break;
+    }
+    hipLaunchKernelGGL(k_env_final, dim3(1), dim3(128), 0, st, partial, nb, nt3, coeffs);
+    return hipGetLastError();
+}
+
+// Scratch of the allocation-free projection: one copy per device, held until the process
+// ends. Calls on one device share it, so calls that overlap on one device race.
+static __device__ __align__(256) float scratch[kEnvScratchFloats];
+
+static std::mutex g_scratch_mu;
+static float* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, looked up once per device with that device
+// current (in host code the array's name is only a stand-in for it).
+bsh_status env_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(BSH_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return BSH_OK;
+    BSH_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    BSH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(BSH_ERR_STATE, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_dev[device] = (float*)p;
+    return BSH_OK;
+}
+
+float* env_scratch_ptr(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_dev[device];
+}
+
 }  // namespace bsh
 
 using namespace bsh;
@@ -112,50 +199,49 @@ extern "C" {
 bsh_status bsh_sh_project_env(bsh_context* ctx, const float* env, int width, int height, int lmax, float* coeffs,
                               void* stream)
 {
-    if (!ctx || !env || !coeffs) return fail(BSH_ERR_INVALID, "bsh_sh_project_env: NULL argument");
-    if (width <= 0 || height <= 0) return fail(BSH_ERR_INVALID, "bsh_sh_project_env: bad size");
-    if (lmax < 0 || lmax > kShMaxL) return fail(BSH_ERR_UNSUPPORTED, "bsh_sh_project_env: lmax must be 0..4");
-    if ((size_t)width * height >= ((size_t)1 << 31)) return fail(BSH_ERR_UNSUPPORTED, "bsh_sh_project_env: too large");
+    bsh_status v = check_env_args(ctx, env, width, height, lmax, coeffs, "bsh_sh_project_env");
+    if (v != BSH_OK) return v;
     BSH_HIP(hipSetDevice(ctx->device));
     hipStream_t st = (hipStream_t)stream;
-    // sin/cos tables exactly as the reference computes them on the host (shproject.cpp:85-104)
-    std::vector<float> sth(height), cth(height), sph(width), cph(width);
-    float thetastep = kPi / height, phistep = 2.f * kPi / width;
-    float theta0 = kPi / height / 2, phi0 = 2.f * kPi / width / 2;
-    for (int i = 0; i < width; ++i)
-    {
-        sph[i] = std::sin(phi0 + i * phistep);
-        cph[i] = std::cos(phi0 + i * phistep);
-    }
-    for (int i = 0; i < height; ++i)
-    {
-        sth[i] = std::sin(theta0 + i * thetastep);
-        cth[i] = std::cos(theta0 + i * thetastep);
-    }
-    DevBuf<float> d_sth, d_cth, d_sph, d_cph, d_part;
-    BSH_HIP(d_sth.alloc(height));
-    BSH_HIP(d_cth.alloc(height));
-    BSH_HIP(d_sph.alloc(width));
-    BSH_HIP(d_cph.alloc(width));
-    BSH_HIP(hipMemcpyAsync(d_sth.p, sth.data(), height * 4, hipMemcpyHostToDevice, st));
-    BSH_HIP(hipMemcpyAsync(d_cth.p, cth.data(), height * 4, hipMemcpyHostToDevice, st));
-    BSH_HIP(hipMemcpyAsync(d_sph.p, sph.data(), width * 4, hipMemcpyHostToDevice, st));
-    BSH_HIP(hipMemcpyAsync(d_cph.p, cph.data(), width * 4, hipMemcpyHostToDevice, st));
-    int nt3 = 3 * (lmax + 1) * (lmax + 1);
-    int nb = (width * height + kShBlock - 1) / kShBlock;
-    BSH_HIP(d_part.alloc((size_t)nb * nt3));
-    dim3 g(nb), b(kShBlock);
-    switch (lmax)
-    {
-    case 0: hipLaunchKernelGGL(k_env_partial<0>, g, b, 0, st, env, width, height, d_sth.p, d_cth.p, d_sph.p, d_cph.p, d_part.p); break;
-    case 1: hipLaunchKernelGGL(k_env_partial<1>, g, b, 0, st, env, width, height, d_sth.p, d_cth.p, d_sph.p, d_cph.p, d_part.p); break;
-    case 2: hipLaunchKernelGGL(k_env_partial<2>, g, b, 0, st, env, width, height, d_sth.p, d_cth.p, d_sph.p, d_cph.p, d_part.p); break;
-    case 3: hipLaunchKernelGGL(k_env_partial<3>, g, b, 0, st, env, width, height, d_sth.p, d_cth.p, d_sph.p, d_cph.p, d_part.p); break;
-    default: hipLaunchKernelGGL(k_env_partial<4>, g, b, 0, st, env, width, height, d_sth.p, d_cth.p, d_sph.p, d_cph.p, d_part.p); break;
-    }
-    hipLaunchKernelGGL(k_env_final, dim3(1), dim3(128), 0, st, d_part.p, nb, nt3, coeffs);
-    BSH_HIP(hipGetLastError());
+    std::vector<float> tab(2 * ((size_t)width + height));
+    env_tables(width, height, 0, tab.size(), tab.data());
+    DevBuf<float> d_tab, d_part;
+    BSH_HIP(d_tab.alloc(tab.size()));
+    BSH_HIP(hipMemcpyAsync(d_tab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st));
+    BSH_HIP(d_part.alloc(env_partial_floats(width, height, lmax)));
+    BSH_HIP(launch_env_projection(env, width, height, lmax, d_tab.p, d_part.p, coeffs, st));
     BSH_HIP(hipStreamSynchronize(st));   // the temporaries are freed on return
+    return BSH_OK;
+}
+
+bsh_status bsh_sh_project_env_noalloc(bsh_context* ctx, const float* env, int width, int height, int lmax,
+                                      float* coeffs, void* stream)
+{
+    bsh_status v = check_env_args(ctx, env, width, height, lmax, coeffs, "bsh_sh_project_env_noalloc");
+    if (v != BSH_OK) return v;
+    size_t ntab = 2 * ((size_t)width + height), npart = env_partial_floats(width, height, lmax);
+    size_t part_off = (ntab + 63) & ~(size_t)63;   // partials start on a 256-B boundary
+    if (part_off + npart > kEnvScratchFloats)
+        return fail(BSH_ERR_INVALID, "bsh_sh_project_env_noalloc: a " + std::to_string(width) + "x" +
+                                         std::to_string(height) + " map at lmax " + std::to_string(lmax) + " needs " +
+                                         std::to_string((part_off + npart) * 4) + " B of scratch, more than its " +
+                                         std::to_string(kEnvScratchFloats * 4) + " B; use bsh_sh_project_env");
+    BSH_HIP(hipSetDevice(ctx->device));
+    float* base = env_scratch_ptr(ctx->device);
+    if (!base) return fail(BSH_ERR_STATE, "bsh_sh_project_env_noalloc: scratch of device " +
+                                              std::to_string(ctx->device) + " was not set up by bsh_context_create");
+    hipStream_t st = (hipStream_t)stream;
+    // the host tables go up through a fixed stack buffer, one piece at a time; each piece is
+    // waited for before the buffer is refilled
+    float piece[2048];
+    for (size_t o = 0; o < ntab; o += 2048)
+    {
+        size_t n = std::min<size_t>(2048, ntab - o);
+        env_tables(width, height, o, n, piece);
+        BSH_HIP(hipMemcpyAsync(base + o, piece, n * 4, hipMemcpyHostToDevice, st));
+        BSH_HIP(hipStreamSynchronize(st));
+    }
+    BSH_HIP(launch_env_projection(env, width, height, lmax, base, base + part_off, coeffs, st));
     return BSH_OK;
 }
 

@@ -113,7 +113,7 @@ EXPORTS = [
     "bsh_guide_forward", "bsh_guide_gradient", "bsh_guide_train_step", "bsh_render_guided", "bsh_diag_mfma",
     "bsh_diag_guide", "bsh_bvh_build_host", "bsh_guide_set_bsdf_fraction", "bsh_guide_train",
     "bsh_bake_sh_guided", "bsh_guide_forward_fp32", "bsh_guide_gradient_fp32",
-    "bsh_context_launch_stats", "bsh_diag_stream_read", "bsh_import_obj",
+    "bsh_context_launch_stats", "bsh_diag_stream_read", "bsh_import_obj", "bsh_sh_project_env_noalloc",
 ]
 
 
@@ -273,6 +273,17 @@ class Context:
         dc = self.alloc(((lmax + 1) ** 2, 3), np.float32)
         _check(lib().bsh_sh_project_env(self.h, de.ptr, c_int(W), c_int(H), c_int(lmax), dc.ptr, c_void_p(None)),
                "bsh_sh_project_env")
+        self.sync()
+        return dc.numpy()
+
+    def sh_project_env_noalloc(self, env, lmax, env_dev=None):
+        """bsh_sh_project_env_noalloc on `env` (H x W x 3; or on `env_dev`, a device copy of it)."""
+        env = np.ascontiguousarray(env, np.float32)
+        H, W = env.shape[:2]
+        de = env_dev if env_dev is not None else self.to_device(env)
+        dc = self.alloc(((lmax + 1) ** 2, 3), np.float32)
+        _check(lib().bsh_sh_project_env_noalloc(self.h, de.ptr, c_int(W), c_int(H), c_int(lmax), dc.ptr,
+                                                c_void_p(None)), "bsh_sh_project_env_noalloc")
         self.sync()
         return dc.numpy()
 

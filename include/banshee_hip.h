@@ -260,6 +260,17 @@ bsh_status bsh_bake_sh(bsh_scene* s, const float* probes, size_t nprobes, const 
    float[(lmax+1)^2*3], overwritten. */
 bsh_status bsh_sh_project_env(bsh_context* ctx, const float* env, int width, int height, int lmax, float* coeffs,
                               void* stream);
+/* bsh_sh_project_env without any allocation. No counterpart in the reference: it is for callers
+   that cannot allocate device memory (code under stream capture, tight serving loops). The sin/cos
+   tables and the block partials live in a 4 MiB device array of the library (one per device,
+   held until the process ends), so 2(W+H) + ceil(W*H/256)*3(lmax+1)^2 floats (tables rounded up
+   to 64 floats) must fit in 1,048,576 (e.g. 2048x1024 at lmax 4, 4096x2048 at lmax 2); larger
+   maps return BSH_ERR_INVALID. The tables are copied up through a stack buffer, with the stream
+   synchronised after each 8 KB piece; the kernels are then queued on `stream` without waiting.
+   Calls on one device share the array: calls that overlap on one device (other streams or
+   threads) race. Same results as bsh_sh_project_env, bit for bit. */
+bsh_status bsh_sh_project_env_noalloc(bsh_context* ctx, const float* env, int width, int height, int lmax,
+                                      float* coeffs, void* stream);
 /* ShTracer::GetE (tracer/shtracer.cpp:20-36): irradiance sum_i Y_i(n) ShConvolveCosTheta(c)_i 0.7/pi
    (math/sh.cpp:160-180). coeffs device float[(lmax+1)^2*3]; normals/out device float[n*3]. */
 bsh_status bsh_sh_irradiance(bsh_context* ctx, int lmax, const float* coeffs, const float* normals, size_t n,
