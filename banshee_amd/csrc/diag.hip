@@ -78,6 +78,24 @@ __global__ void k_diag_keyed(uint32_t seed, uint32_t n, const uint32_t* cs, cons
     out[i] = ks_u32(ks_base(seed, cs[i]), stream[i], ctr[i]);
 }
 
+// Image samples of pixel p, sample s (one-row image of npix pixels) or the k-th light
+// sampler draw of camera sample 0, exactly as k_raygen and k_shade form them.
+__global__ void k_diag_samples(int kind, int arg, int image, uint32_t seed, uint32_t n, int spp, float* out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x, y;
+    if (image)
+    {
+        int s = (int)(i % (uint32_t)spp);
+        image_sample2d(kind, arg, s, ks_base(seed, i), ks_base(seed, i - (uint32_t)s), x, y);
+    }
+    else
+        nee_sample2d(kind, ks_base(seed, 0u), KS_LIGHT, 2u * i, x, y);
+    out[2 * i] = x;
+    out[2 * i + 1] = y;
+}
+
 template <class T>
 static bsh_status up(DevBuf<T>& d, const T* h, size_t n)
 {
@@ -207,6 +225,25 @@ bsh_status bsh_diag_keyed(bsh_context* ctx, uint32_t seed, size_t n, const uint3
                        c.p, o.p);
     BSH_HIP(hipGetLastError());
     BSH_HIP(hipMemcpy(out, o.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return BSH_OK;
+}
+
+bsh_status bsh_diag_samples(bsh_context* ctx, int kind, int arg, int image, uint32_t seed, size_t n, float* out)
+{
+    if (!ctx || (n && !out)) return fail(BSH_ERR_INVALID, "bsh_diag_samples: NULL argument");
+    if (kind < SMP_STRATIFIED || kind > SMP_SOBOL || arg <= 0 || arg > 4096)
+        return fail(BSH_ERR_INVALID, "bsh_diag_samples: bad sampler");
+    if (n == 0) return BSH_OK;
+    if (n > (1u << 30)) return fail(BSH_ERR_INVALID, "bsh_diag_samples: too many samples");
+    int spp = image ? sampler_num_samples(kind, arg) : 1;
+    if (image && n % (size_t)spp) return fail(BSH_ERR_INVALID, "bsh_diag_samples: n is not whole pixels");
+    BSH_HIP(hipSetDevice(ctx->device));
+    DevBuf<float> o;
+    BSH_HIP(o.alloc(2 * n));
+    hipLaunchKernelGGL(k_diag_samples, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, 0, kind, arg, image, seed,
+                       (uint32_t)n, spp, o.p);
+    BSH_HIP(hipGetLastError());
+    BSH_HIP(hipMemcpy(out, o.p, 2 * n * sizeof(float), hipMemcpyDeviceToHost));
     return BSH_OK;
 }
 

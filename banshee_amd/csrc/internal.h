@@ -11,6 +11,7 @@
 #include "../../include/banshee_hip.h"
 #include "bsh_math.h"
 #include "bvh_build.h"
+#include "sampler_dev.h"
 #include "scene_dev.h"
 
 
@@ -104,6 +105,7 @@ struct JobDev
     CameraDev cam;
     int type;
     int W, H, cx, cy, cw, ch, grid, spp, s_begin, s_count;
+    int img_smp, light_smp, brdf_smp;   // SMP_* kinds (sampler_dev.h); grid is the image sampler's argument
     int depth;
     uint32_t seed;
     float bg[3];

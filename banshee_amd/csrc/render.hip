@@ -190,12 +190,10 @@ __global__ __launch_bounds__(kBlock) void k_raygen(RenderDev R, uint32_t job, ui
     int y = J.cy + (int)(q / (uint32_t)J.cw);
     uint32_t g = ((uint32_t)y * (uint32_t)J.W + (uint32_t)x) * (uint32_t)J.spp + (uint32_t)s;
     uint32_t base = ks_base(J.seed, g);
-    float ux, uy;
-    ks_sample2d(base, KS_IMG, 0, ux, uy);
-    int idx = s % (J.grid * J.grid);
-    float cell = 1.f / (float)J.grid;
-    float sx = ((float)(idx % J.grid) + ux) * cell;
-    float sy = ((float)(idx / J.grid) + uy) * cell;
+    // CMJ and Sobol patterns are drawn once per pixel, keyed by the pixel's sample 0
+    uint32_t base0 = J.img_smp == SMP_CMJ || J.img_smp == SMP_SOBOL ? ks_base(J.seed, g - (uint32_t)s) : base;
+    float sx, sy;
+    image_sample2d(J.img_smp, J.grid, s, base, base0, sx, sy);
     float imx = (float)x / (float)J.W + (1.f / (float)J.W) * sx;
     float imy = (float)y / (float)J.H + (1.f / (float)J.H) * sy;
     float hx = imx - 0.5f, hy = imy - 0.5f;
@@ -808,7 +806,7 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                     float ux, uy;
                     if (b == 0)
                     {
-                        ks_sample2d(base, KS_BRDF, 0, ux, uy);   // pre-drawn once per path, gitracer.cpp:36-41
+                        nee_sample2d(J.brdf_smp, base, KS_BRDF, 0, ux, uy);   // pre-drawn once per path, gitracer.cpp:36-41
                         meta.z = __float_as_uint(ux);
                         meta.w = __float_as_uint(uy);
                     }
@@ -851,8 +849,8 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                 uint32_t lc = b == 0 ? (uint32_t)(2 * j) : (uint32_t)(2 * nl + 2 * (b - 1));
                                 uint32_t bc = 2u + lc;
                                 float lx, ly, bx, by;
-                                ks_sample2d(base, KS_LIGHT, lc, lx, ly);
-                                ks_sample2d(base, KS_BRDF, bc, bx, by);
+                                nee_sample2d(J.light_smp, base, KS_LIGHT, lc, lx, ly);
+                                nee_sample2d(J.brdf_smp, base, KS_BRDF, bc, bx, by);
                                 size_t ks = (size_t)i * K + 2 * j;
                                 // light sample + shadow ray
                                 f3 ld;
@@ -1666,7 +1664,20 @@ bsh_status make_render_job(bsh_scene* s, const bsh_render_params* p, float* accu
     int cx = p->crop_w > 0 ? p->crop_x : 0, cy = p->crop_h > 0 ? p->crop_y : 0;
     if (cx < 0 || cy < 0 || cx + cw > p->width || cy + ch > p->height)
         return fail(BSH_ERR_INVALID, "bsh_render: crop outside the image");
-    int spp = p->img_grid * p->img_grid;
+    if (p->img_sampler < SMP_STRATIFIED || p->img_sampler > SMP_SOBOL)
+        return fail(BSH_ERR_INVALID, "bsh_render: unknown image sampler kind");
+    for (int k : {p->light_sampler, p->brdf_sampler})
+    {
+        if (k < SMP_STRATIFIED || k > SMP_SOBOL) return fail(BSH_ERR_INVALID, "bsh_render: unknown light/BSDF sampler kind");
+        // a Sobol sequence runs over every draw a pixel makes, in the order its paths make
+        // them (sobol_sampler.cpp:32-39); wavefront paths have no such order
+        if (k == SMP_SOBOL)
+            return fail(BSH_ERR_UNSUPPORTED, "bsh_render: a Sobol light or BSDF sampler is not supported (its sequence "
+                                             "index runs over all of a pixel's draws in path order)");
+    }
+    if (p->img_grid > ((p->img_sampler == SMP_RANDOM || p->img_sampler == SMP_SOBOL) ? (1 << 24) : 4096))
+        return fail(BSH_ERR_INVALID, "bsh_render: img_grid too large");
+    int spp = sampler_num_samples(p->img_sampler, p->img_grid);
     int sb = p->spp_begin, sc = p->spp_count > 0 ? p->spp_count : spp - p->spp_begin;
     if (sb < 0 || sc <= 0 || sb + sc > spp) return fail(BSH_ERR_INVALID, "bsh_render: bad sample range");
     if ((uint64_t)p->width * p->height * spp > 0xffffffffull)
@@ -1682,6 +1693,9 @@ bsh_status make_render_job(bsh_scene* s, const bsh_render_params* p, float* accu
     J.cw = cw;
     J.ch = ch;
     J.grid = p->img_grid;
+    J.img_smp = p->img_sampler;
+    J.light_smp = p->light_sampler;
+    J.brdf_smp = p->brdf_sampler;
     J.spp = spp;
     J.s_begin = sb;
     J.s_count = sc;

@@ -58,7 +58,29 @@ class RenderParams(ctypes.Structure):
                 ("crop_y", ctypes.c_int32), ("crop_w", ctypes.c_int32), ("crop_h", ctypes.c_int32),
                 ("img_grid", ctypes.c_int32), ("spp_begin", ctypes.c_int32), ("spp_count", ctypes.c_int32),
                 ("max_depth", ctypes.c_int32), ("seed", ctypes.c_uint32), ("chunk_spp", ctypes.c_int32),
-                ("bg", ctypes.c_float * 3), ("flags", ctypes.c_int32)]
+                ("bg", ctypes.c_float * 3), ("flags", ctypes.c_int32), ("img_sampler", ctypes.c_int32),
+                ("light_sampler", ctypes.c_int32), ("brdf_sampler", ctypes.c_int32)]
+
+
+# BSH_SAMPLER_* kinds
+SAMPLERS = {"stratified": 0, "regular": 1, "random": 2, "cmj": 3, "sobol": 4}
+
+
+def num_samples(sampler: str, arg: int) -> int:
+    """Samples per pixel of the image sampler `sampler` built with argument `arg`
+    (Sampler::num_samples)."""
+    if sampler in ("stratified", "regular", "cmj"):
+        return arg * arg
+    if sampler == "random":
+        return arg
+    if sampler == "sobol":
+        return 1 << max(0, int(arg - 1).bit_length())
+    raise ValueError(sampler)
+
+
+def _set_samplers(p, samplers):
+    img, light, brdf = samplers
+    p.img_sampler, p.light_sampler, p.brdf_sampler = SAMPLERS[img], SAMPLERS[light], SAMPLERS[brdf]
 
 
 class ProbeParams(ctypes.Structure):
@@ -114,6 +136,7 @@ EXPORTS = [
     "bsh_diag_guide", "bsh_bvh_build_host", "bsh_guide_set_bsdf_fraction", "bsh_guide_train",
     "bsh_bake_sh_guided", "bsh_guide_forward_fp32", "bsh_guide_gradient_fp32",
     "bsh_context_launch_stats", "bsh_diag_stream_read", "bsh_import_obj", "bsh_sh_project_env_noalloc",
+    "bsh_diag_samples",
 ]
 
 
@@ -260,6 +283,15 @@ class Context:
                "bsh_diag_keyed")
         return out
 
+    def diag_samples(self, sampler, arg, seed, n, image=True):
+        """Device sampler patterns (bsh_diag_samples): image samples of n pixels [n, spp, 2], or
+        the first n light-sampler samples [n, 2]."""
+        spp = num_samples(sampler, arg) if image else 1
+        out = np.zeros((n * spp, 2), np.float32)
+        _check(lib().bsh_diag_samples(self.h, c_int(SAMPLERS[sampler]), c_int(arg), c_int(1 if image else 0),
+                                      c_uint32(seed), c_size_t(n * spp), _p(out)), "bsh_diag_samples")
+        return out.reshape(n, spp, 2) if image else out
+
     def diag_sh_eval(self, dirs, lmax):
         d = np.ascontiguousarray(dirs, np.float32)
         out = np.zeros((d.shape[0], (lmax + 1) ** 2), np.float32)
@@ -377,13 +409,15 @@ class Context:
 
 
 def render_params(width, height, img_grid, max_depth, seed, crop=None, spp_range=None, chunk_spp=0,
-                  bg=(0.0, 0.0, 0.0), count=False, accumulate=False) -> RenderParams:
+                  bg=(0.0, 0.0, 0.0), count=False, accumulate=False,
+                  samplers=("stratified", "stratified", "stratified")) -> RenderParams:
     p = RenderParams()
     p.width, p.height = width, height
     if crop is not None:
         p.crop_x, p.crop_y, p.crop_w, p.crop_h = crop
     p.img_grid = img_grid
-    spp = img_grid * img_grid
+    _set_samplers(p, samplers)
+    spp = num_samples(samplers[0], img_grid)
     sb, sc = (spp_range if spp_range else (0, spp))
     p.spp_begin, p.spp_count = sb, sc
     p.max_depth = max_depth
@@ -641,8 +675,10 @@ class Scene:
 
     def render(self, width, height, img_grid, max_depth, seed, crop=None, spp_range=None, chunk_spp=0,
                bg=(0.0, 0.0, 0.0), per_sample=False, accum: DeviceArray | None = None, stream=None, download=True,
-               count=False):
-        """Returns (sum [h,w,3], sumsq [h,w,3], count [h,w], per_sample [h,w,s,3] or None)."""
+               count=False, samplers=("stratified", "stratified", "stratified")):
+        """Returns (sum [h,w,3], sumsq [h,w,3], count [h,w], per_sample [h,w,s,3] or None).
+        samplers: kinds of the image sampler (built with img_grid), the light and the BSDF sampler
+        (built with 1), from SAMPLERS."""
         p = RenderParams()
         p.width, p.height = width, height
         if crop is not None:
@@ -650,7 +686,8 @@ class Scene:
         cw = crop[2] if crop else width
         ch = crop[3] if crop else height
         p.img_grid = img_grid
-        spp = img_grid * img_grid
+        _set_samplers(p, samplers)
+        spp = num_samples(samplers[0], img_grid)
         sb, sc = (spp_range if spp_range else (0, spp))
         p.spp_begin, p.spp_count = sb, sc
         p.max_depth = max_depth

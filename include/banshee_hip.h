@@ -107,10 +107,12 @@ typedef struct
 } bsh_bvh_params;
 
 /* Frame parameters for bsh_render. The estimator is GiTracer(max_depth)
-   (tracer/gitracer.cpp:13-140) with DiTracer::GetDi NEE (ditracer.cpp:57-239);
-   image sampler StratifiedSampler(img_grid), light and BSDF samplers
-   StratifiedSampler(1) (sampler/stratified_sampler.cpp:7-24) fed by the keyed sample
-   stream. Camera sample g = (y*width + x)*spp + s, spp = img_grid^2. */
+   (tracer/gitracer.cpp:13-140) with DiTracer::GetDi NEE (ditracer.cpp:57-239), driven by
+   ImageRenderer::Render (renderer/imagerenderer.cpp:9-59) with the image sampler
+   <img_sampler>(img_grid) and light and BSDF samplers <light_sampler>(1), <brdf_sampler>(1),
+   all fed by the keyed sample stream. Camera sample g = (y*width + x)*spp + s, where spp is
+   the image sampler's num_samples(): img_grid^2 for the stratified, regular and CMJ samplers,
+   img_grid for RandomSampler, the next power of two >= img_grid for SobolSampler. */
 typedef struct
 {
     int32_t width, height;         /* full image plane (ImagePlane::resolution) */
@@ -124,7 +126,21 @@ typedef struct
                                       wavefront chunk are bsh_tuning.paths_per_chunk */
     float bg[3];                   /* World::bgcolor_ */
     int32_t flags;                 /* BSH_RENDER_* */
+    int32_t img_sampler;           /* BSH_SAMPLER_* of the image sampler, built with argument img_grid */
+    int32_t light_sampler;         /* BSH_SAMPLER_* of the light sampler (argument 1); Sobol is not supported */
+    int32_t brdf_sampler;          /* BSH_SAMPLER_* of the BSDF sampler (argument 1); Sobol is not supported */
 } bsh_render_params;
+
+/* Sampler kinds (the reference's sampler directory). 0 keeps the StratifiedSampler of earlier versions. */
+enum
+{
+    BSH_SAMPLER_STRATIFIED = 0, /* StratifiedSampler(g), sampler/stratified_sampler.cpp:7-24 */
+    BSH_SAMPLER_REGULAR = 1,    /* RegularSampler(g): cell centres, sampler/regular_sampler.h:46-72 */
+    BSH_SAMPLER_RANDOM = 2,     /* RandomSampler(n), sampler/random_sampler.h:48-62 */
+    BSH_SAMPLER_CMJ = 3,        /* CmjSampler(g): Kensler's correlated multi-jitter, sampler/cmj_sampler.cpp:3-73 */
+    BSH_SAMPLER_SOBOL = 4       /* SobolSampler(n): scrambled (0,2)-sequence, reset per pixel,
+                                   sampler/sobol_sampler.cpp:5-48 */
+};
 
 enum
 {
@@ -441,6 +457,12 @@ bsh_status bsh_diag_camera(bsh_context* ctx, const bsh_camera* cam, size_t n, co
 /* Device keyed sample stream values ks_u32(ks_base(seed, camsample), stream, counter). Host pointers. */
 bsh_status bsh_diag_keyed(bsh_context* ctx, uint32_t seed, size_t n, const uint32_t* camsample,
                           const uint32_t* stream, const uint32_t* counter, uint32_t* out);
+/* Device sampler patterns (sampler/stratified_sampler.cpp, regular_sampler.h, random_sampler.h,
+   cmj_sampler.cpp, sobol_sampler.cpp) on the keyed stream, as bsh_render forms them. image != 0:
+   the image samples of a one-row image of n / spp pixels, camera sample g = p*spp + s, for the
+   sampler BSH_SAMPLER_<kind>(arg); image == 0: the first n samples of a light sampler
+   <kind>(1) in camera sample 0. out: host float[n*2]. */
+bsh_status bsh_diag_samples(bsh_context* ctx, int kind, int arg, int image, uint32_t seed, size_t n, float* out);
 
 /* Memory ceiling for the roofline (no reference counterpart): times one streaming read of `reps`
    passes over `bytes` of device memory at `buf` (a multiple of 16, at least 4 KiB) after one

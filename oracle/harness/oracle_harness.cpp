@@ -17,6 +17,10 @@
  *   - KeyedRng: an Rng (Banshee/rng/rng.h:39-53) drawing from the keyed
  *     sample stream, plus link-time wrapping of rand()/srand() so GiTracer's
  *     light pick and Russian roulette (gitracer.cpp:75,119) draw from it too.
+ *   - KeyedCamera: a PerscpectiveCamera whose GenerateRay (a virtual of
+ *     Banshee/camera/camera.h:53) first moves the keyed stream to the next
+ *     camera sample, so the light, BSDF and rand() draws of a camera sample are
+ *     keyed by it whatever the image sampler draws.
  */
 #include <atomic>
 #include <chrono>
@@ -39,6 +43,10 @@
 #include "camera/perspective_camera.h"
 #include "tracer/gitracer.h"
 #include "sampler/stratified_sampler.h"
+#include "sampler/regular_sampler.h"
+#include "sampler/random_sampler.h"
+#include "sampler/cmj_sampler.h"
+#include "sampler/sobol_sampler.h"
 #include "rng/mcrng.h"
 #include "renderer/imagerenderer.h"
 #include "renderer/mt_imagerenderer.h"
@@ -78,8 +86,35 @@ struct KeyCtx
     uint32_t ctr[8] = {0};
     /* tile mapping: sequential camera sample q -> global index */
     int x0 = 0, y0 = 0, tw = 1, W = 1, spp = 1;
+    /* image draws: img_draws == 0 -> two per camera sample (stratified, random), keyed by
+       that sample; img_draws = d > 0 -> d per pixel (CMJ 1, Sobol 2), keyed by the pixel's
+       sample 0. cam_keyed: KeyedCamera, not the image draws, moves the other streams to
+       the next camera sample. stride: counter step per draw of each stream (2 for a
+       CmjSampler(1), whose one draw per sample takes the first of the sample's two counters). */
+    int img_draws = 0;
+    bool cam_keyed = false;
+    uint64_t cam_calls = 0;
+    uint32_t stride[8] = {1, 1, 1, 1, 1, 1, 1, 1};
 };
 KeyCtx g_key;
+
+/* Global camera-sample index of the q-th camera sample in the tile's loop order
+   (y outer, x inner, s innermost). */
+uint32_t key_camsample(uint64_t q)
+{
+    uint64_t pix = q / (uint64_t)g_key.spp;
+    uint64_t s = q % (uint64_t)g_key.spp;
+    uint64_t x = g_key.x0 + pix % (uint64_t)g_key.tw;
+    uint64_t y = g_key.y0 + pix / (uint64_t)g_key.tw;
+    return (uint32_t)((y * (uint64_t)g_key.W + x) * (uint64_t)g_key.spp + s);
+}
+
+void key_enter_sample(uint32_t camsample)
+{
+    g_key.camsample = camsample;
+    g_key.base = ks_base(g_key.seed, camsample);
+    for (int i = 0; i < 8; ++i) g_key.ctr[i] = 0;
+}
 
 class KeyedRng : public Rng
 {
@@ -90,25 +125,36 @@ public:
         if (stream_ == KS_IMG)
         {
             uint64_t k = g_key.img_calls++;
-            if ((k & 1) == 0)
+            if (g_key.img_draws > 0)
             {
-                uint64_t q = k >> 1;
-                uint64_t pix = q / (uint64_t)g_key.spp;
-                uint64_t s = q % (uint64_t)g_key.spp;
-                uint64_t x = g_key.x0 + pix % (uint64_t)g_key.tw;
-                uint64_t y = g_key.y0 + pix / (uint64_t)g_key.tw;
-                g_key.camsample = (uint32_t)((y * (uint64_t)g_key.W + x) * (uint64_t)g_key.spp + s);
-                g_key.base = ks_base(g_key.seed, g_key.camsample);
-                for (int i = 0; i < 8; ++i) g_key.ctr[i] = 0;
+                uint64_t d = (uint64_t)g_key.img_draws;
+                uint32_t cs0 = key_camsample((k / d) * (uint64_t)g_key.spp);
+                return ks_u32(ks_base(g_key.seed, cs0), KS_IMG, (uint32_t)(k % d));
             }
-            return ks_u32(g_key.base, KS_IMG, (uint32_t)(k & 1));
+            uint32_t cs = key_camsample(k >> 1);
+            if ((k & 1) == 0 && !g_key.cam_keyed) key_enter_sample(cs);
+            return ks_u32(ks_base(g_key.seed, cs), KS_IMG, (uint32_t)(k & 1));
         }
-        return ks_u32(g_key.base, stream_, g_key.ctr[stream_]++);
+        uint32_t c = g_key.ctr[stream_];
+        g_key.ctr[stream_] += g_key.stride[stream_];
+        return ks_u32(g_key.base, stream_, c);
     }
     float NextFloat() const { return ks_float(NextUint()); }
     Rng* Clone() const { return new KeyedRng(stream_); }
 private:
     uint32_t stream_;
+};
+
+/* The reference's perspective camera, keying the stream per camera sample when asked. */
+class KeyedCamera : public PerscpectiveCamera
+{
+public:
+    using PerscpectiveCamera::PerscpectiveCamera;
+    void GenerateRay(float2 const& sample, ray& r) const
+    {
+        if (g_key.on && g_key.cam_keyed) key_enter_sample(key_camsample(g_key.cam_calls++));
+        PerscpectiveCamera::GenerateRay(sample, r);
+    }
 };
 
 /* Probe RNG used to detect which Sample2D component takes the first draw. */
@@ -262,6 +308,39 @@ Material* make_material(float const* m)
     return new Emissive(c);
 }
 
+/* ---------------- the reference's other samplers on the keyed stream ---------------- */
+/* kind: 0 StratifiedSampler, 1 RegularSampler, 2 RandomSampler, 3 CmjSampler, 4 SobolSampler */
+Sampler* make_sampler(int kind, int arg, uint32_t stream)
+{
+    switch (kind)
+    {
+    case 1: return new RegularSampler(arg);
+    case 2: return new RandomSampler(arg, new KeyedRng(stream));
+    case 3: return new CmjSampler(arg, new KeyedRng(stream));
+    case 4: return new SobolSampler(arg, new KeyedRng(stream));
+    default: return new StratifiedSampler(arg, new KeyedRng(stream));
+    }
+}
+
+int sampler_spp(int kind, int arg)
+{
+    if (kind == 2) return arg;
+    if (kind == 4) return (int)upper_power_of_two((unsigned long)arg);
+    return arg * arg;
+}
+
+void key_setup(unsigned seed, int x0, int y0, int tw, int W, int spp, int img_kind, int light_kind, int brdf_kind)
+{
+    g_key = KeyCtx();
+    g_key.on = true;
+    g_key.seed = seed;
+    g_key.x0 = x0; g_key.y0 = y0; g_key.tw = tw; g_key.W = W; g_key.spp = spp;
+    g_key.cam_keyed = true;
+    g_key.img_draws = img_kind == 3 ? 1 : img_kind == 4 ? 2 : 0;
+    g_key.stride[KS_LIGHT] = light_kind == 3 ? 2 : 1;
+    g_key.stride[KS_BRDF] = brdf_kind == 3 ? 2 : 1;
+}
+
 } // namespace
 
 extern "C" {
@@ -313,7 +392,7 @@ void* oref_scene_create(float const* verts, int nverts, float const* normals, in
         int mat = m->GetFaces()[shape].m;
         s->world.lights_.push_back(std::unique_ptr<Light>(new AreaLight(shape, *m, *s->world.materials_[mat])));
     }
-    s->world.camera_.reset(new PerscpectiveCamera(float3(cam[0], cam[1], cam[2]), float3(cam[3], cam[4], cam[5]),
+    s->world.camera_.reset(new KeyedCamera(float3(cam[0], cam[1], cam[2]), float3(cam[3], cam[4], cam[5]),
                                                   float3(cam[6], cam[7], cam[8]), float2(cam[9], cam[10]),
                                                   cam[11], cam[12]));
     s->world.Commit();
@@ -428,6 +507,73 @@ void oref_render_keyed(void* h, int W, int H, int x0, int y0, int tw, int th, in
     std::memcpy(sumsq, plane.sq_.data(), plane.sq_.size() * sizeof(float));
     std::memcpy(cnt, plane.n_.data(), plane.n_.size() * sizeof(float));
     if (per_sample) std::memcpy(per_sample, plane.samples_.data(), plane.samples_.size() * sizeof(float));
+}
+
+/* Full-frame GI through ImageRenderer::Render (imagerenderer.cpp:9-59), which resets the
+   three samplers after every pixel: image sampler <img_kind>(img_arg), light and BSDF
+   samplers <light_kind>(1), <brdf_kind>(1), GiTracer(depth), all on the keyed stream.
+   Outputs row-major over the frame, NOT y-flipped; per_sample in (pixel, s) order. */
+void oref_render_keyed_frame(void* h, int W, int H, int img_kind, int img_arg, int light_kind, int brdf_kind,
+                             int depth, unsigned seed, float* sum, float* sumsq, float* cnt, float* per_sample,
+                             long long* rays)
+{
+    Scene* s = (Scene*)h;
+    int spp = sampler_spp(img_kind, img_arg);
+    StatsPlane plane(W, H, 0, 0, W, H, spp, per_sample != nullptr);
+    key_setup(seed, 0, 0, W, W, spp, img_kind, light_kind, brdf_kind);
+    reset_counts();
+    {
+        Sampler* img = make_sampler(img_kind, img_arg, KS_IMG);
+        Sampler* light = make_sampler(light_kind, 1, KS_LIGHT);
+        Sampler* brdf = make_sampler(brdf_kind, 1, KS_BRDF);
+        ImageRenderer r(plane, new GiTracer(depth), img, light, brdf);
+        r.Render(s->world);
+    }
+    g_key.on = false;
+    read_counts(rays);
+    std::memcpy(sum, plane.sum_.data(), plane.sum_.size() * sizeof(float));
+    std::memcpy(sumsq, plane.sq_.data(), plane.sq_.size() * sizeof(float));
+    std::memcpy(cnt, plane.n_.data(), plane.n_.size() * sizeof(float));
+    if (per_sample) std::memcpy(per_sample, plane.samples_.data(), plane.samples_.size() * sizeof(float));
+}
+
+/* The image samples of <kind>(arg) for pixels 0..npix-1 of a one-row image keyed by seed,
+   with Reset() after every pixel as ImageRenderer::Render does: out[(p*spp + s)*2 + {0,1}]. */
+void oref_image_samples(int kind, int arg, unsigned seed, int npix, float* out)
+{
+    int spp = sampler_spp(kind, arg);
+    key_setup(seed, 0, 0, npix, npix, spp, kind, 0, 0);
+    {
+        std::unique_ptr<Sampler> smp(make_sampler(kind, arg, KS_IMG));
+        for (int p = 0; p < npix; ++p)
+        {
+            for (int k = 0; k < spp; ++k)
+            {
+                float2 v = smp->Sample2D();
+                out[2 * ((size_t)p * spp + k)] = v.x;
+                out[2 * ((size_t)p * spp + k) + 1] = v.y;
+            }
+            smp->Reset();
+        }
+    }
+    g_key.on = false;
+}
+
+/* n successive samples of a light sampler <kind>(1) within camera sample 0 keyed by seed. */
+void oref_nee_samples(int kind, unsigned seed, int n, float* out)
+{
+    key_setup(seed, 0, 0, 1, 1, 1, 0, kind, kind);
+    key_enter_sample(0);
+    {
+        std::unique_ptr<Sampler> smp(make_sampler(kind, 1, KS_LIGHT));
+        for (int i = 0; i < n; ++i)
+        {
+            float2 v = smp->Sample2D();
+            out[2 * i] = v.x;
+            out[2 * i + 1] = v.y;
+        }
+    }
+    g_key.on = false;
 }
 
 /* Wall-clock timing of the reference renderer as shipped: McRng samplers seeded by

@@ -34,6 +34,16 @@ CORNELL_OBJ = dict(W=128, H=128, grid=4, nee=1, depth=5, seed=3)
 CORNELL_OBJ_CROP = (40, 30, 24, 24)
 CORNELL_OBJ_CAMERA = [0.0, 1.0, 3.5, 0.0, 1.0, 0.0, 0.0, 1.0, 0.0, 0.01, 10000.0, float(np.float32(np.pi / 4)), 1.0]
 # recorded rays of the terrain: every TERRAIN_RAYS_STRIDE-th BVH query of a render of this crop
+# samplers: image patterns per (kind, argument), light-sampler draws, and whole Cornell frames
+# rendered through ImageRenderer::Render with other sampler sets; the first frame uses the
+# reference's own unit-test samplers (RegularSampler image, light and BSDF samplers,
+# UnitTest/basic_features.h:408-412)
+IMAGE_SAMPLERS = [("stratified", 4), ("regular", 3), ("random", 5), ("cmj", 4), ("cmj", 3), ("sobol", 5),
+                  ("sobol", 16)]
+NEE_SAMPLERS = ["stratified", "regular", "random", "cmj"]
+SAMPLER_SEED, SAMPLER_PIXELS, SAMPLER_W, SAMPLER_DEPTH = 9, 16, 24, 5
+SAMPLER_FRAMES = {"regular": (("regular", "regular", "regular"), 4), "cmj": (("cmj", "cmj", "cmj"), 4),
+                  "sobol": (("sobol", "stratified", "random"), 16), "random": (("random", "random", "stratified"), 8)}
 TERRAIN_RAYS_CROP = (16, 10, 160, 88)
 TERRAIN_RAYS_STRIDE = 5
 
@@ -203,6 +213,24 @@ def main(only=()):
         np.savez_compressed(os.path.join(HERE, "gi_cornell_obj.npz"), digest=ob.digest(), sum=S, sumsq=Q, n=N,
                             ray_counts=rays, crop=np.array(CORNELL_OBJ_CROP), crop_samples=PS,
                             camera=np.asarray(CORNELL_OBJ_CAMERA, np.float32), **{k: v for k, v in O.items()})
+
+    # ---------------- the reference's other samplers (sampler/*), on the keyed stream ----------------
+    if want("samplers"):
+        img = {f"{k}_{a}": o.image_samples(k, a, SAMPLER_SEED, SAMPLER_PIXELS) for k, a in IMAGE_SAMPLERS}
+        nee = {k: o.nee_samples(k, SAMPLER_SEED, 64) for k in NEE_SAMPLERS}
+        cb = cornell_box()
+        sc = o.scene(cb)
+        frames = {}
+        for name, (kinds, arg) in SAMPLER_FRAMES.items():
+            S, Q, N, PS, rays = sc.render_keyed_frame(SAMPLER_W, SAMPLER_W, kinds, arg, SAMPLER_DEPTH, SAMPLER_SEED,
+                                                      per_sample=True)
+            frames[f"frame_{name}"] = PS
+            frames[f"rays_{name}"] = rays
+            print(f"samplers {name}: rays={rays}")
+        np.savez_compressed(os.path.join(HERE, "samplers.npz"), digest=cb.digest(), seed=SAMPLER_SEED,
+                            pixels=SAMPLER_PIXELS, W=SAMPLER_W, depth=SAMPLER_DEPTH,
+                            **{f"img_{k}": v for k, v in img.items()}, **{f"nee_{k}": v for k, v in nee.items()},
+                            **frames)
 
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

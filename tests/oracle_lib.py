@@ -58,6 +58,9 @@ def _load(kind):
             lib.cref_bake_sh.argtypes = [vp, vp, i, i, i, i, ctypes.c_uint, ctypes.c_float, vp, vp, vp, i]
             lib.cref_render_bsdf_only.argtypes = [vp, i, i, i, i, i, i, i, i, ctypes.c_uint, vp, vp, vp, vp, i]
         if kind == "ref":
+            lib.oref_render_keyed_frame.argtypes = [vp, i, i, i, i, i, i, i, ctypes.c_uint, vp, vp, vp, vp, vp]
+            lib.oref_image_samples.argtypes = [i, i, ctypes.c_uint, i, vp]
+            lib.oref_nee_samples.argtypes = [i, ctypes.c_uint, i, vp]
             lib.oref_record_begin.argtypes = [vp, i, i]
             lib.oref_record_fetch.argtypes = [vp, vp, vp, vp, vp]
             lib.oref_record_fetch.restype = i
@@ -109,6 +112,23 @@ class Oracle:
     def stratified(self, grid, seed, n):
         out = np.zeros((n, 2), np.float32)
         self._fn("stratified")(grid, seed, n, _p(out))
+        return out
+
+    # ---- the reference's samplers on the keyed stream (reference oracle only) ----
+    def image_samples(self, sampler, arg, seed, npix):
+        """[npix, spp, 2]: the image samples of <sampler>(arg) over a one-row image, reset
+        after every pixel as ImageRenderer::Render does."""
+        from banshee_amd.hip import SAMPLERS, num_samples
+        spp = num_samples(sampler, arg)
+        out = np.zeros((npix, spp, 2), np.float32)
+        self.lib.oref_image_samples(SAMPLERS[sampler], arg, seed, npix, _p(out))
+        return out
+
+    def nee_samples(self, sampler, seed, n):
+        """[n, 2]: successive samples of a light sampler <sampler>(1) in camera sample 0."""
+        from banshee_amd.hip import SAMPLERS
+        out = np.zeros((n, 2), np.float32)
+        self.lib.oref_nee_samples(SAMPLERS[sampler], seed, n, _p(out))
         return out
 
     def sh_eval(self, dirs, lmax):
@@ -180,6 +200,21 @@ class OracleScene:
         rays = np.zeros(2, np.int64)
         self.o._fn("render_keyed")(self.h, W, H, x0, y0, tw, th, img_grid, nee_grid, depth, seed, _p(s), _p(q),
                                    _p(c), _p(ps), _p(rays))
+        return s, q, c, ps, rays
+
+    def render_keyed_frame(self, W, H, samplers, img_arg, depth, seed, per_sample=False):
+        """The whole frame through ImageRenderer::Render with samplers = (image, light, BSDF)
+        kinds, the image sampler built with img_arg, the others with 1 (reference only)."""
+        from banshee_amd.hip import SAMPLERS, num_samples
+        spp = num_samples(samplers[0], img_arg)
+        s = np.zeros((H, W, 3), np.float32)
+        q = np.zeros((H, W, 3), np.float32)
+        c = np.zeros((H, W), np.float32)
+        ps = np.zeros((H, W, spp, 3), np.float32) if per_sample else None
+        rays = np.zeros(2, np.int64)
+        self.o.lib.oref_render_keyed_frame(self.h, W, H, SAMPLERS[samplers[0]], img_arg, SAMPLERS[samplers[1]],
+                                           SAMPLERS[samplers[2]], depth, seed, _p(s), _p(q), _p(c), _p(ps),
+                                           _p(rays))
         return s, q, c, ps, rays
 
     def time_render(self, W, H, img_grid, nee_grid, depth, seed, mt=True):
