@@ -71,6 +71,47 @@ CameraDev make_camera(const bsh_camera& c)
 
 using namespace bsh;
 
+namespace bsh {
+// EnvironmentLightIs's ctor (environment_light_is.cpp:13-37) over the scene's map: the 512 x 256
+// luminance x sin(theta) image, sampled at uv = (j / 512, i / 256) with the texture filter, then
+// Distribution2D's tables (math/distribution1d.cpp:7-31, distribution2d.cpp:5-27) in the same
+// float operations. out: marginal funcvals (256), cdf (257), conditional funcvals (256 x 512),
+// cdfs (256 x 513), funcints (256); returns the marginal funcint.
+static void dist1d_build(const float* vals, int n, float* val_out, float* cdf, float& funcint)
+{
+    for (int i = 0; i < n; ++i) val_out[i] = vals[i];
+    cdf[0] = 0.f;
+    for (int i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + vals[i - 1] / (float)n;
+    funcint = cdf[n];
+    for (int i = 0; i < n + 1; ++i) cdf[i] /= funcint;
+}
+
+float build_env_distribution(const float* tex, int w, int h, float* out)
+{
+    const int n = kEnvDistW, m = kEnvDistH;
+    std::vector<float> img((size_t)n * m);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < n; ++j)
+        {
+            float u = (float)j / (float)n, v = (float)i / (float)m;
+            float sintheta = sinf(kPi * (float)((float)i + 0.5f) / (float)m);
+            f3 c = env_bilinear(tex, w, h, u, v);
+            float luminance = 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z;
+            img[(size_t)i * n + j] = luminance * sintheta;
+        }
+    float* mval = out;
+    float* mcdf = mval + m;
+    float* cval = mcdf + m + 1;
+    float* ccdf = cval + (size_t)m * n;
+    float* cint = ccdf + (size_t)m * (n + 1);
+    for (int i = 0; i < m; ++i)
+        dist1d_build(img.data() + (size_t)i * n, n, cval + (size_t)i * n, ccdf + (size_t)i * (n + 1), cint[i]);
+    float mint = 0.f;
+    dist1d_build(cint, m, mval, mcdf, mint);
+    return mint;
+}
+}  // namespace bsh
+
 extern "C" {
 
 const char* bsh_last_error(void) { return tl_error.c_str(); }
@@ -204,6 +245,31 @@ bsh_status bsh_scene_set_lights(bsh_scene* s, const bsh_area_light* lights, size
     return BSH_OK;
 }
 
+bsh_status bsh_scene_set_other_lights(bsh_scene* s, const bsh_light* lights, size_t n)
+{
+    if (!s || (n && !lights)) return fail(BSH_ERR_INVALID, "bsh_scene_set_other_lights: NULL argument");
+    for (size_t l = 0; l < n; ++l)
+        if (lights[l].type < BSH_LIGHT_POINT || lights[l].type > BSH_LIGHT_ENVIRONMENT_IS)
+            return fail(BSH_ERR_INVALID, "bsh_scene_set_other_lights: light " + std::to_string(l) + " has an unknown type");
+    s->others.assign(lights, lights + n);
+    s->committed = false;
+    return BSH_OK;
+}
+
+bsh_status bsh_scene_set_env_map(bsh_scene* s, const float* texels, int width, int height)
+{
+    if (!s || !texels) return fail(BSH_ERR_INVALID, "bsh_scene_set_env_map: NULL argument");
+    if (width <= 0 || height <= 0 || (size_t)width * height > ((size_t)1 << 26))
+        return fail(BSH_ERR_INVALID, "bsh_scene_set_env_map: bad size");
+    s->env_tex.assign(texels, texels + 3 * (size_t)width * height);
+    for (float v : s->env_tex)
+        if (!std::isfinite(v)) return fail(BSH_ERR_INVALID, "bsh_scene_set_env_map: non-finite texel");
+    s->env_w = width;
+    s->env_h = height;
+    s->committed = false;
+    return BSH_OK;
+}
+
 bsh_status bsh_scene_set_camera(bsh_scene* s, const bsh_camera* cam)
 {
     if (!s || !cam) return fail(BSH_ERR_INVALID, "bsh_scene_set_camera: NULL argument");
@@ -245,7 +311,7 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
         d.tri = tri;
         d.area = sqrtf(fabsf(sqnorm(cross(p3 - p1, p3 - p2)))) * 0.5f;  // GetShapeSurfaceArea, mesh.cpp:320
         d.pdf = 1.f / d.area;                                              // FillSample, mesh.cpp:121
-        d.pad = 0.f;
+        d.type = LIGHT_AREA;
         d.e = make_float4(m.color[0], m.color[1], m.color[2], 0.f);
         const int32_t* tn = &s->tri_n[3 * tri];
         const float* N = s->normals.data();
@@ -255,6 +321,59 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
             d.n[k] = make_float4(N[3 * tn[k]], N[3 * tn[k] + 1], N[3 * tn[k] + 2], 0.f);
         }
         s->h_lights.push_back(d);
+    }
+    // the other lights follow, as World::lights_ holds them after the area lights
+    bool has_env = false, has_env_is = false;
+    for (size_t l = 0; l < s->others.size(); ++l)
+    {
+        const bsh_light& O = s->others[l];
+        LightDev d{};
+        d.tri = -1;
+        d.type = O.type;
+        if (O.type == BSH_LIGHT_POINT || O.type == BSH_LIGHT_DIRECTIONAL)
+        {
+            f3 v = mk(O.v[0], O.v[1], O.v[2]);
+            if (O.type == BSH_LIGHT_DIRECTIONAL) v = normalize(v);   // DirectionalLight ctor: d_(normalize(d))
+            d.p[0] = make_float4(v.x, v.y, v.z, 0.f);
+            d.e = make_float4(O.e[0], O.e[1], O.e[2], 0.f);
+        }
+        else
+        {
+            if (!(O.gamma != 0.f) || !std::isfinite(O.scale))
+                return fail(BSH_ERR_INVALID, "bsh_scene_commit: environment light " + std::to_string(l) +
+                                                 " needs a finite scale and a nonzero gamma");
+            d.e = make_float4(O.scale, 1.f / O.gamma, 0.f, 0.f);   // invgamma_(1.f / gamma)
+            has_env = true;
+            has_env_is = has_env_is || O.type == BSH_LIGHT_ENVIRONMENT_IS;
+        }
+        s->h_lights.push_back(d);
+    }
+    if (has_env && s->env_tex.empty())
+        return fail(BSH_ERR_STATE, "bsh_scene_commit: environment lights need bsh_scene_set_env_map");
+    s->env = EnvDev{};
+    if (has_env)
+    {
+        const size_t ntex = s->env_tex.size();
+        const size_t nd = has_env_is ? (size_t)kEnvDistH * (2 + kEnvDistW + (kEnvDistW + 1)) + kEnvDistH + 1 : 0;
+        std::vector<float> buf(ntex + nd);
+        std::copy(s->env_tex.begin(), s->env_tex.end(), buf.begin());
+        float mint = 0.f;
+        if (has_env_is) mint = build_env_distribution(s->env_tex.data(), s->env_w, s->env_h, buf.data() + ntex);
+        BSH_HIP(s->d_env.alloc(buf.size()));
+        BSH_HIP(hipMemcpy(s->d_env.p, buf.data(), buf.size() * sizeof(float), hipMemcpyHostToDevice));
+        s->env.tex = s->d_env.p;
+        s->env.w = s->env_w;
+        s->env.h = s->env_h;
+        if (has_env_is)
+        {
+            float* q = s->d_env.p + ntex;
+            s->env.mval = q;
+            s->env.mcdf = q + kEnvDistH;
+            s->env.cval = q + 2 * kEnvDistH + 1;
+            s->env.ccdf = s->env.cval + (size_t)kEnvDistH * kEnvDistW;
+            s->env.cint = s->env.ccdf + (size_t)kEnvDistH * (kEnvDistW + 1);
+            s->env.mint = mint;
+        }
     }
     // BVH
     std::vector<float> tv(9 * nt);
@@ -331,12 +450,22 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
         const size_t nl = s->h_lights.size();
         std::vector<double> pw(nl);
         double tot = 0.0;
+        double area_tot = 0.0;
+        size_t area_pos = 0;
         for (size_t l = 0; l < nl; ++l)
         {
             const LightDev& d = s->h_lights[l];
+            if (d.type != LIGHT_AREA) continue;
             pw[l] = (double)d.area * std::max(0.0, 0.2126 * d.e.x + 0.7152 * d.e.y + 0.0722 * d.e.z);
-            tot += pw[l];
+            area_tot += pw[l];
+            area_pos += pw[l] > 0.0 ? 1 : 0;
         }
+        // point, directional and environment lights have no power comparable to an area light's;
+        // each is picked like an average area light (any positive share keeps the estimator
+        // unbiased, the pick's 1 / probability weighing it)
+        for (size_t l = 0; l < nl; ++l)
+            if (s->h_lights[l].type != LIGHT_AREA) pw[l] = area_pos ? area_tot / (double)area_pos : 1.0;
+        for (size_t l = 0; l < nl; ++l) tot += pw[l];
         std::vector<float> cdf(std::max<size_t>(nl, 1), 1.f);
         double run = 0.0;
         for (size_t l = 0; l < nl; ++l)

@@ -96,6 +96,36 @@ __global__ void k_diag_samples(int kind, int arg, int image, uint32_t seed, uint
     out[2 * i + 1] = y;
 }
 
+// Light::GetSample / GetPdf / GetLe of light `light` (any type) at points p with shading
+// normals nrm, as the shade kernel calls them.
+__global__ void k_diag_light_full(SceneDev s, int light, uint32_t n, const float* p, const float* nrm, const float* u,
+                                  const float* w, float* out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const LightDev* L = s.lights + light;
+    f3 pp = mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]), nn = mk(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+    f3 ww = mk(w[3 * i], w[3 * i + 1], w[3 * i + 2]);
+    f3 d = mk(0.f, 0.f, 0.f), le, ge = mk(0.f, 0.f, 0.f);
+    float pdf = 0.f, gp;
+    if (L->type == LIGHT_AREA)
+    {
+        le = light_sample(L, pp, u[2 * i], u[2 * i + 1], d, pdf);
+        gp = light_pdf(L, pp, ww);
+    }
+    else
+    {
+        le = other_light_sample(s, *L, pp, nn, u[2 * i], u[2 * i + 1], d, pdf);
+        gp = other_light_pdf(s, *L, nn, ww);
+        ge = other_light_le(s, *L, ww);
+    }
+    float* o = out + 16 * i;
+    o[0] = le.x; o[1] = le.y; o[2] = le.z; o[3] = pdf;
+    o[4] = d.x; o[5] = d.y; o[6] = d.z; o[7] = gp;
+    o[8] = ge.x; o[9] = ge.y; o[10] = ge.z;
+    for (int k = 11; k < 16; ++k) o[k] = 0.f;
+}
+
 template <class T>
 static bsh_status up(DevBuf<T>& d, const T* h, size_t n)
 {
@@ -180,7 +210,7 @@ bsh_status bsh_diag_light(bsh_scene* s, int light, size_t n, const float* p, con
 {
     if (!s) return fail(BSH_ERR_INVALID, "bsh_diag_light: NULL scene");
     if (!s->committed) return fail(BSH_ERR_STATE, "bsh_diag_light: scene not committed");
-    if (light < 0 || (size_t)light >= s->lights.size()) return fail(BSH_ERR_INVALID, "bsh_diag_light: bad light");
+    if (light < 0 || (size_t)light >= s->lights.size()) return fail(BSH_ERR_INVALID, "bsh_diag_light: bad area light");
     if (n == 0) return BSH_OK;
     BSH_HIP(hipSetDevice(s->ctx->device));
     DevBuf<float> dp, du, dw, dout;
@@ -225,6 +255,27 @@ bsh_status bsh_diag_keyed(bsh_context* ctx, uint32_t seed, size_t n, const uint3
                        c.p, o.p);
     BSH_HIP(hipGetLastError());
     BSH_HIP(hipMemcpy(out, o.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return BSH_OK;
+}
+
+bsh_status bsh_diag_light_full(bsh_scene* s, int light, size_t n, const float* p, const float* nrm, const float* u,
+                               const float* w, float* out)
+{
+    if (!s || (n && (!p || !nrm || !u || !w || !out))) return fail(BSH_ERR_INVALID, "bsh_diag_light_full: NULL argument");
+    if (!s->committed) return fail(BSH_ERR_STATE, "bsh_diag_light_full: scene not committed");
+    if (light < 0 || (size_t)light >= s->h_lights.size()) return fail(BSH_ERR_INVALID, "bsh_diag_light_full: bad light");
+    if (n == 0) return BSH_OK;
+    BSH_HIP(hipSetDevice(s->ctx->device));
+    DevBuf<float> dp, dn, du, dw, o;
+    BSH_TRY(up(dp, p, 3 * n));
+    BSH_TRY(up(dn, nrm, 3 * n));
+    BSH_TRY(up(du, u, 2 * n));
+    BSH_TRY(up(dw, w, 3 * n));
+    BSH_HIP(o.alloc(16 * n));
+    hipLaunchKernelGGL(k_diag_light_full, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, 0, s->dev(), light,
+                       (uint32_t)n, dp.p, dn.p, du.p, dw.p, o.p);
+    BSH_HIP(hipGetLastError());
+    BSH_HIP(hipMemcpy(out, o.p, 16 * n * sizeof(float), hipMemcpyDeviceToHost));
     return BSH_OK;
 }
 

@@ -183,6 +183,9 @@ struct bsh_scene
     std::vector<bsh_bundle> bundles;
     std::vector<bsh_material> mats;
     std::vector<bsh_area_light> lights;
+    std::vector<bsh_light> others;        // after the area lights in the light list
+    std::vector<float> env_tex;           // env_h rows of env_w RGB texels
+    int env_w = 0, env_h = 0;
     bsh_camera cam{};
     bool has_cam = false;
     bool committed = false;
@@ -191,6 +194,8 @@ struct bsh_scene
     bsh::DevBuf<float4> d_tris, d_stri;
     bsh::DevBuf<bsh::LightDev> d_lights;
     bsh::DevBuf<float> d_light_cdf;
+    bsh::DevBuf<float> d_env;             // map, then the environment distribution's tables
+    bsh::EnvDev env{};
     std::vector<bsh::LightDev> h_lights;
     float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {0.f, 0.f, 0.f};
     uint64_t nnodes = 0;
@@ -208,7 +213,9 @@ struct bsh_scene
         s.stri = d_stri.p;
         s.lights = d_lights.p;
         s.light_cdf = d_light_cdf.p;
-        s.nlights = (int)lights.size();
+        s.env = env;
+        s.nlights = (int)(lights.size() + others.size());
+        s.narea = (int)lights.size();
         s.ntris = (int)tri_v.size() / 3;
         return s;
     }

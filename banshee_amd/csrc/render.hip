@@ -678,8 +678,9 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 // kFull = false compiles only the reference estimator (no quirk-free, guided or record
 // code), keeping the plain path's registers low. kOne: the run has a single job, so its
 // record and scene pointers are read with scalar loads at the start instead of per lane
-// after the path's meta word.
-template <bool kFull, bool kOne>
+// after the path's meta word. kLights (with kFull) adds point, directional and environment
+// lights, kept out of the other variants' registers.
+template <bool kFull, bool kOne, bool kLights = false>
 __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev R, int b, int Lout)
 {
     // Bounce 0 shades every path of the chunk; a later bounce only the paths the previous
@@ -763,6 +764,17 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                 }
                             }
                         }
+                        else if (kLights && Lid >= S.narea)
+                        {
+                            // the ray left the scene: the light's GetLe, nonzero for environment
+                            // lights (ditracer.cpp:218-222)
+                            f3 le = other_light_le(S, S.lights[Lid], ld3(R.nee_wi[ks + 1]));
+                            if (sqnorm(le) > 0.f)
+                            {
+                                float4 na = R.nee_a[ks + 1];
+                                local = local + le * ld3(na) * na.w * nb.x * nb.y;
+                            }
+                        }
                     }
                     // the reference scales its one uniformly picked light by nl; a quirk-free pick's
                     // 1 / probability is in the slot's contributions already
@@ -790,6 +802,10 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                     if (b == 0)
                     {
                         rad = mk(J.bg[0], J.bg[1], J.bg[2]);
+                        // a camera ray that leaves the scene also gets every light's GetLe
+                        // (gitracer.cpp:20-30), which only environment lights make nonzero
+                        if (kLights)
+                            for (int l = S.narea; l < nl; ++l) rad = rad + other_light_le(S, S.lights[l], ld3(R.ray_d[i]));
                         fl |= F_NOCLAMP;
                     }
                     fl |= F_TERM;
@@ -855,25 +871,36 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                 // light sample + shadow ray
                                 f3 ld;
                                 float lpdf = 0.f, cosl = 0.f;
-                                f3 le = light_sample(L, h.p, lx, ly, ld, lpdf, &cosl);
+                                // point, directional and environment lights exist only in kLights runs
+                                const bool area = !kLights || L->type == LIGHT_AREA;
+                                // point and directional lights are singular: light sample only, no
+                                // MIS (ditracer.cpp:113-119,149)
+                                const bool singular = !area && L->type <= LIGHT_DIRECTIONAL;
+                                f3 le = area ? light_sample(L, h.p, lx, ly, ld, lpdf, &cosl)
+                                             : other_light_sample(S, *L, h.p, h.n, lx, ly, ld, lpdf);
                                 if ((nq ? lpdf > 0.f : lpdf > kMinPdf) && sqnorm(le) > 0.f)
                                 {
                                     f3 wi = normalize(ld);
                                     float dist = sqrtf(sqnorm(ld));
                                     f3 f = mat_eval(m, h, wi, wo);
-                                    // the reference asks GetPdf(light dir, view), the density of the
-                                    // view direction; MIS needs the density of sampling the light
-                                    // direction from the view, GetPdf(view, light dir)
-                                    float bp = nq ? mat_pdf(m, h, wo, wi) : mat_pdf(m, h, wi, wo);
-                                    // quirk-free: both MIS weights in solid angle (the reference
-                                    // weighs the area pdf against the BSDF's solid-angle pdf)
-                                    float lw = nq ? lpdf * sqnorm(ld) / cosl : lpdf;
-                                    float w = power_heuristic(lw, bp);
+                                    float w = 1.f;
+                                    if (!singular)
+                                    {
+                                        // the reference asks GetPdf(light dir, view), the density of the
+                                        // view direction; MIS needs the density of sampling the light
+                                        // direction from the view, GetPdf(view, light dir)
+                                        float bp = nq ? mat_pdf(m, h, wo, wi) : mat_pdf(m, h, wi, wo);
+                                        // quirk-free: both MIS weights in solid angle (the reference
+                                        // weighs an area light's area pdf against the BSDF's solid-angle
+                                        // pdf; environment pdfs are in solid angle already)
+                                        float lw = nq && area ? lpdf * sqnorm(ld) / cosl : lpdf;
+                                        w = power_heuristic(lw, bp);
+                                    }
                                     // GetLe / d^2 is the reference's whole area-to-solid-angle factor,
                                     // which treats the emitter as uniform radiance e; the emitter's
                                     // radiance is GetLe = e cos_l (what the camera and the BSDF branch
                                     // see), so the quirk-free estimator adds the Jacobian's cos_l
-                                    if (nq) le = le * cosl;
+                                    if (nq && area) le = le * cosl;
                                     f3 c = le * f * fabsf(dot(h.n, wi)) * w * (1.f / lpdf);
                                     if (nq) c = c * pickw;
                                     R.nee_a[ks] = st3(c, 0.f);
@@ -881,17 +908,20 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                     emit_mask |= 1ull << (2 * j);
                                 }
                                 // BSDF sample + MIS closest ray
-                                f3 wis;
+                                f3 wis = mk(0.f, 0.f, 0.f);
                                 float bpdf = 0.f;
                                 int type = 0;
-                                f3 fb = mat_sample(m, h, bx, by, wo, wis, bpdf, type);
+                                f3 fb = singular ? mk(0.f, 0.f, 0.f) : mat_sample(m, h, bx, by, wo, wis, bpdf, type);
                                 f3 wi = normalize(wis);
-                                if (sqnorm(fb) > 0.f && (nq ? bpdf > 0.f : bpdf > kMinPdf))
+                                if (!singular && sqnorm(fb) > 0.f && (nq ? bpdf > 0.f : bpdf > kMinPdf))
                                 {
-                                    float lp = light_pdf(L, h.p, wi);
-                                    if (nq ? lp > 0.f : !(lp < kMinPdf))
+                                    float lp = area ? light_pdf(L, h.p, wi) : other_light_pdf(S, *L, h.n, wi);
+                                    // quirk-free: where an environment light's density is 0 the BSDF
+                                    // sample carries the whole weight (an area light's density is 0
+                                    // only where the ray cannot reach it)
+                                    if (nq ? lp > 0.f || !area : !(lp < kMinPdf))
                                     {
-                                        float w = power_heuristic(bpdf, lp);
+                                        float w = nq && !(lp > 0.f) ? 1.f : power_heuristic(bpdf, lp);
                                         R.nee_a[ks + 1] = st3(fb, fabsf(dot(h.n, wi)));
                                         R.nee_b[ks + 1] = make_float4(nq ? w * pickw : w, 1.f / bpdf, (float)Lid, 0.f);
                                         R.nee_wi[ks + 1] = st3(wi, 10000000.f);
@@ -1524,10 +1554,12 @@ static bsh_status enqueue_batch(bsh_context* c, const Lane* lanes, int nlanes, c
         hipStream_t st = ln.st;
         const int launches = ch.depth + 1;
         R.P = ch.P;
-        bool full = false, wide = refill > 0;
+        bool full = false, lights = false, wide = refill > 0;
         for (const Segment& sg : ch.segs)
         {
             full = full || hjobs[sg.job].flags != 0;
+            // point, directional and environment lights are compiled into the kLights variants only
+            lights = lights || hjobs[sg.job].s.narea != hjobs[sg.job].s.nlights;
             wide = wide && hjobs[sg.job].s.width == 4;
         }
         hipLaunchKernelGGL(k_zero, grid_for(w.counters.n), dim3(kBlock), 0, st, w.counters.p, (int)w.counters.n);
@@ -1596,7 +1628,9 @@ static bsh_status enqueue_batch(bsh_context* c, const Lane* lanes, int nlanes, c
             }
             if (T) BSH_HIP(hipEventRecord(T->next(), st));
             const int lo = b + 1 <= ch.depth ? b + 1 : b;
-            if (full && one) hipLaunchKernelGGL((k_shade<true, true>), pg, dim3(kBlock), 0, st, R, b, lo);
+            if (lights && one) hipLaunchKernelGGL((k_shade<true, true, true>), pg, dim3(kBlock), 0, st, R, b, lo);
+            else if (lights) hipLaunchKernelGGL((k_shade<true, false, true>), pg, dim3(kBlock), 0, st, R, b, lo);
+            else if (full && one) hipLaunchKernelGGL((k_shade<true, true>), pg, dim3(kBlock), 0, st, R, b, lo);
             else if (full) hipLaunchKernelGGL((k_shade<true, false>), pg, dim3(kBlock), 0, st, R, b, lo);
             else if (one) hipLaunchKernelGGL((k_shade<false, true>), pg, dim3(kBlock), 0, st, R, b, lo);
             else hipLaunchKernelGGL((k_shade<false, false>), pg, dim3(kBlock), 0, st, R, b, lo);
@@ -1657,9 +1691,9 @@ bsh_status make_render_job(bsh_scene* s, const bsh_render_params* p, float* accu
     if (p->chunk_spp != 0)
         return fail(BSH_ERR_INVALID, "bsh_render: chunk_spp is reserved and must be 0 (bound the paths per "
                                      "chunk with bsh_tuning.paths_per_chunk)");
-    if (s->lights.size() > (size_t)kMaxNeeLights && !(p->flags & BSH_RENDER_NO_QUIRKS))
+    if (s->lights.size() + s->others.size() > (size_t)kMaxNeeLights && !(p->flags & BSH_RENDER_NO_QUIRKS))
         return fail(BSH_ERR_UNSUPPORTED, "bsh_render: GiTracer's bounce-0 loop over every light supports at most 32 "
-                                         "area lights (BSH_RENDER_NO_QUIRKS picks one light per vertex by power)");
+                                         "lights (BSH_RENDER_NO_QUIRKS picks one light per vertex by power)");
     int cw = p->crop_w > 0 ? p->crop_w : p->width, ch = p->crop_h > 0 ? p->crop_h : p->height;
     int cx = p->crop_w > 0 ? p->crop_x : 0, cy = p->crop_h > 0 ? p->crop_y : 0;
     if (cx < 0 || cy < 0 || cx + cw > p->width || cy + ch > p->height)
@@ -1725,9 +1759,9 @@ bsh_status make_probe_job(bsh_scene* s, const float* probes, size_t nprobes, con
                                      "chunk with bsh_tuning.paths_per_chunk)");
     if ((uint64_t)nprobes * (uint64_t)N > 0xffffffffull)
         return fail(BSH_ERR_UNSUPPORTED, "bsh_bake_sh: more than 2^32 probe rays");
-    if (s->lights.size() > (size_t)kMaxNeeLights && !(p->flags & BSH_RENDER_NO_QUIRKS))
+    if (s->lights.size() + s->others.size() > (size_t)kMaxNeeLights && !(p->flags & BSH_RENDER_NO_QUIRKS))
         return fail(BSH_ERR_UNSUPPORTED, "bsh_bake_sh: GiTracer's bounce-0 loop over every light supports at most 32 "
-                                         "area lights (BSH_RENDER_NO_QUIRKS picks one light per vertex by power)");
+                                         "lights (BSH_RENDER_NO_QUIRKS picks one light per vertex by power)");
     J = JobDev{};
     J.s = s->dev();
     J.type = 1;

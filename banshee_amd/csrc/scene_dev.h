@@ -5,6 +5,7 @@
 
 #include "bsh_math.h"
 #include "bvh_build.h"
+#include "light_dev.h"
 
 namespace bsh {
 
@@ -34,14 +35,17 @@ typedef __attribute__((address_space(3))) uint32_t lu32;
 // Mesh::GetShapeSurfaceArea / FillSample compute them (mesh.cpp:304-321, 121).
 struct LightDev
 {
-    int tri;
+    int tri;                  // area lights: the emissive triangle; other lights: -1
     float area;
     float pdf;
-    float pad;
+    int type;                 // LIGHT_* (light_dev.h)
+    // area, point and directional lights: emission; environment lights: (scale, 1 / gamma)
     float4 e;
     // the light triangle's vertices and vertex normals, copied from the scene's arrays so a
     // light sample or pdf reads them with one load instead of going through the triangle's
     // index record
+    // area lights: triangle vertices and normals; point light: p[0] = position; directional
+    // light: p[0] = its normalised direction
     float4 p[3];
     float4 n[3];
 };
@@ -56,7 +60,9 @@ struct SceneDev
                               // light), p3, n1, n2, n3, its material (type, c.xyz), (eta, alpha, 0, 0)
     const LightDev* lights;
     const float* light_cdf;   // per light: cumulative share of the emitted power (area x luminance), last = 1
+    EnvDev env;               // environment map of the environment lights (tex NULL when none)
     int nlights;
+    int narea;                // lights [0, narea) are area lights, the rest point/directional/environment
     int ntris;
     int width;                // BVH layout: 2 (GpuNode, 4 float4) or 4 (GpuNode4, 8 float4)
 };

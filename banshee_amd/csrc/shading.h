@@ -47,6 +47,74 @@ __device__ __forceinline__ int pick_light(const SceneDev& s, float u, float& w)
     return lo;
 }
 
+// Light::GetPdf of the non-area lights at a point with shading normal n, direction w:
+// point and directional lights 0 (light.h, pointlight.h, directional_light.h);
+// EnvironmentLight n.normalize(w) / pi (environment_light.cpp:64-70); EnvironmentLightIs the
+// distribution's density in solid angle (environment_light_is.cpp:94-107).
+__device__ __forceinline__ float other_light_pdf(const SceneDev& s, const LightDev& L, f3 n, f3 w)
+{
+    if (L.type == LIGHT_ENV) return dot(n, normalize(w)) * (1.f / kPi);
+    if (L.type == LIGHT_ENV_IS)
+    {
+        float u, v, theta;
+        dir_to_uv(w, u, v, &theta);
+        float sintheta = sinf(theta);
+        return sintheta == 0.f ? 0.f : dist2d_pdf(s.env, u, v) / (2.f * kPi * kPi * sintheta);
+    }
+    return 0.f;
+}
+
+// Light::GetSample of the non-area lights (pointlight.cpp, directional_light.cpp,
+// environment_light.cpp:9-35, environment_light_is.cpp:49-73) at a point p with shading
+// normal n and sample (ux, uy): d is the (long) vector to the light, pdf its density.
+__device__ __forceinline__ f3 other_light_sample(const SceneDev& s, const LightDev& L, f3 p, f3 n, float ux, float uy,
+                                                 f3& d, float& pdf)
+{
+    if (L.type == LIGHT_POINT)
+    {
+        d = ld3(L.p[0]) - p;
+        pdf = 1.f;
+        float d2inv = 1.f / sqnorm(d);
+        return ld3(L.e) * d2inv;
+    }
+    if (L.type == LIGHT_DIRECTIONAL)
+    {
+        d = -ld3(L.p[0]) * 100000000.f;
+        pdf = 1.f;
+        return ld3(L.e);
+    }
+    float u, v;
+    if (L.type == LIGHT_ENV)
+    {
+        d = map_to_hemisphere_cos(n, ux, uy);
+        pdf = dot(n, d) * (1.f / kPi);
+        dir_to_uv(d, u, v);
+        d = d * 10000000.f;
+    }
+    else
+    {
+        float dpdf = 0.f;
+        dist2d_sample(s.env, ux, uy, u, v, dpdf);
+        float theta = v * kPi;
+        float phi = u * 2.f * kPi;
+        float costheta = cosf(theta), sintheta = sinf(theta);
+        float sinphi = sinf(phi), cosphi = cosf(phi);
+        d = 100000000.f * normalize(mk(sintheta * cosphi, costheta, sintheta * sinphi));
+        pdf = sintheta == 0.f ? 0.f : dpdf / (2.f * kPi * kPi * sintheta);
+    }
+    return env_radiance(s.env, L.e.x, L.e.y, u, v);
+}
+
+// Light::GetLe of a ray that left the scene in direction d: environment lights only
+// (environment_light.cpp:37-50); every other light returns 0.
+__device__ __forceinline__ f3 other_light_le(const SceneDev& s, const LightDev& L, f3 d)
+{
+    if (L.type != LIGHT_ENV && L.type != LIGHT_ENV_IS) return mk(0.f, 0.f, 0.f);
+    float u, v;
+    dir_to_uv(d, u, v);
+    return env_radiance(s.env, L.e.x, L.e.y, u, v);
+}
+
 // AreaLight::GetPdf -> Mesh::GetPdfOnShape (arealight.cpp:55-58, mesh.cpp:354-378)
 __device__ __forceinline__ float light_pdf(const LightDev* Lp, f3 p, f3 w)
 {

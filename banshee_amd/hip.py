@@ -83,6 +83,11 @@ def _set_samplers(p, samplers):
     p.img_sampler, p.light_sampler, p.brdf_sampler = SAMPLERS[img], SAMPLERS[light], SAMPLERS[brdf]
 
 
+class Light(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("v", ctypes.c_float * 3), ("e", ctypes.c_float * 3),
+                ("scale", ctypes.c_float), ("gamma", ctypes.c_float)]
+
+
 class ProbeParams(ctypes.Structure):
     _fields_ = [("rays_per_probe", ctypes.c_int32), ("lmax", ctypes.c_int32), ("max_depth", ctypes.c_int32),
                 ("seed", ctypes.c_uint32), ("chunk_probes", ctypes.c_int32), ("tmin", ctypes.c_float),
@@ -136,7 +141,7 @@ EXPORTS = [
     "bsh_diag_guide", "bsh_bvh_build_host", "bsh_guide_set_bsdf_fraction", "bsh_guide_train",
     "bsh_bake_sh_guided", "bsh_guide_forward_fp32", "bsh_guide_gradient_fp32",
     "bsh_context_launch_stats", "bsh_diag_stream_read", "bsh_import_obj", "bsh_sh_project_env_noalloc",
-    "bsh_diag_samples",
+    "bsh_diag_samples", "bsh_scene_set_other_lights", "bsh_scene_set_env_map", "bsh_diag_light_full",
 ]
 
 
@@ -617,6 +622,22 @@ class Scene:
         _check(lib().bsh_scene_set_materials(self.h, _p(mats), c_size_t(mats.shape[0])), "bsh_scene_set_materials")
         lights = np.ascontiguousarray(sc.lights, np.int32).reshape(-1, 2)
         _check(lib().bsh_scene_set_lights(self.h, _p(lights), c_size_t(lights.shape[0])), "bsh_scene_set_lights")
+        env = getattr(sc, "env", None)
+        if env is not None:
+            e = np.ascontiguousarray(env, np.float32)
+            _check(lib().bsh_scene_set_env_map(self.h, _p(e), c_int(e.shape[1]), c_int(e.shape[0])),
+                   "bsh_scene_set_env_map")
+        others = getattr(sc, "other_lights", None)
+        if others is not None and len(others):
+            rows = np.asarray(others, np.float32).reshape(-1, 9)
+            arr = (Light * rows.shape[0])()
+            for k, r in enumerate(rows):
+                arr[k].type = int(r[0])
+                for c in range(3):
+                    arr[k].v[c] = float(r[1 + c])
+                    arr[k].e[c] = float(r[4 + c])
+                arr[k].scale, arr[k].gamma = float(r[7]), float(r[8])
+            _check(lib().bsh_scene_set_other_lights(self.h, arr, c_size_t(rows.shape[0])), "bsh_scene_set_other_lights")
         self.set_camera(sc.camera)
         bp = BvhParams(max_leaf, sah_bins, threads, width)
         _check(lib().bsh_scene_commit(self.h, ctypes.byref(bp)), "bsh_scene_commit")
@@ -632,6 +653,16 @@ class Scene:
                                     ctypes.byref(ms), ctypes.byref(depth), ctypes.byref(width)), "bsh_scene_info")
         return {"nodes": nodes.value, "tris": tris.value, "bytes": nbytes.value, "build_ms": ms.value,
                 "depth": depth.value, "width": width.value}
+
+    def diag_light_full(self, light, p, nrm, u, w):
+        """Device Light::GetSample / GetPdf / GetLe of light `light` (area lights first, then the
+        other lights): n x 16 = le(3), pdf, d(3), GetPdf(w), GetLe(w)(3), 0 x 5."""
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (p, nrm, u, w)]
+        n = arrs[0].shape[0]
+        out = np.zeros((n, 16), np.float32)
+        _check(lib().bsh_diag_light_full(self.h, c_int(light), c_size_t(n), *[_p(a) for a in arrs], _p(out)),
+               "bsh_diag_light_full")
+        return out
 
     # ---- queries on device buffers ----
     def intersect_closest_dev(self, rays: DeviceArray, n: int, hits: DeviceArray, stream=None):

@@ -37,6 +37,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 LAMBERT, MICROFACET, EMISSIVE = 0, 1, 2
+# bsh_light types; a Scene's other_lights rows are (type, v xyz, e xyz, scale, gamma)
+POINT_LIGHT, DIRECTIONAL_LIGHT, ENV_LIGHT, ENV_LIGHT_IS = 1, 2, 3, 4
 
 
 @dataclass
@@ -52,6 +54,10 @@ class Scene:
     camera: np.ndarray
     name: str = ""
     meta: dict = field(default_factory=dict)
+    # lights after the area lights (PointLight, DirectionalLight, EnvironmentLight[Is]) as rows
+    # (type, v xyz, e xyz, scale, gamma), and the environment lights' lat-long map [h, w, 3]
+    other_lights: np.ndarray | None = None
+    env: np.ndarray | None = None
 
     @property
     def ntris(self) -> int:
@@ -62,6 +68,9 @@ class Scene:
         for a in (self.verts, self.normals, self.tri_v, self.tri_n, self.tri_m, self.bundles,
                   self.materials, self.lights, self.camera):
             h.update(np.ascontiguousarray(a).tobytes())
+        for a in (self.other_lights, self.env):
+            if a is not None:
+                h.update(np.ascontiguousarray(a, np.float32).tobytes())
         return h.hexdigest()[:16]
 
     def with_camera(self, camera) -> "Scene":
@@ -109,8 +118,10 @@ class _Builder:
         for s in range(self.b[bundle][1]):
             self.lights.append([bundle, s])
 
-    def build(self, camera, name, meta=None) -> Scene:
+    def build(self, camera, name, meta=None, other_lights=None, env=None) -> Scene:
         return Scene(
+            other_lights=None if other_lights is None else np.asarray(other_lights, np.float32).reshape(-1, 9),
+            env=None if env is None else np.ascontiguousarray(env, np.float32),
             verts=np.ascontiguousarray(np.concatenate(self.v), np.float32),
             normals=np.ascontiguousarray(np.concatenate(self.n), np.float32),
             tri_v=np.ascontiguousarray(np.concatenate(self.tv), np.int32),
@@ -506,3 +517,57 @@ def occluded_room(target_tris=500_000, seed=3, width=1024, height=1024, hole_fra
     cam = [4.0, 1.8, 2.9, 3.6, 0.9, 0.0, 0, 1, 0, 0.01, 10000.0, math.pi / 2.5, width / height]
     return B.build(cam, "occluded_room", {"seed": seed, "openings": holes, "hole_cells": int((~keep).sum()),
                                           "spheres": n})
+
+
+def env_map(width=64, height=32, seed=0, sun=(0.3, 0.25), sun_radiance=40.0) -> np.ndarray:
+    """A seeded lat-long sky [height, width, 3]: a bluish gradient from the zenith (row 0) to the
+    horizon, a darker ground half, a bright sun disc at (u, v) = sun, and mild value noise. Every
+    texel is positive."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    U, V = np.meshgrid(u, v)
+    sky = np.stack([0.35 + 0.4 * V, 0.5 + 0.35 * V, 0.9 - 0.2 * V], -1)
+    ground = np.stack([0.25 + 0 * V, 0.2 + 0 * V, 0.15 + 0 * V], -1)
+    img = np.where((V < 0.5)[..., None], sky, ground)
+    du = np.minimum(np.abs(U - sun[0]), 1 - np.abs(U - sun[0]))
+    d2 = (du * 2) ** 2 + (V - sun[1]) ** 2
+    img = img + sun_radiance * np.exp(-d2 / 0.002)[..., None] * np.array([1.0, 0.9, 0.7])
+    img = img * (1 + 0.1 * rng.uniform(-1, 1, (height, width, 1)))
+    return np.ascontiguousarray(img, np.float32)
+
+
+def open_scene(env_kind=ENV_LIGHT, width=24, height=24, seed=4) -> Scene:
+    """Every light type of the reference in one open scene: a 12 x 12 Lambert floor, a Lambert box
+    and a GGX box, a small emissive quad (2 area lights), a PointLight, a DirectionalLight and an
+    environment light of kind env_kind (EnvironmentLight or EnvironmentLightIs, scale 0.8, gamma
+    2.2) over env_map(); the camera sees floor, boxes and sky."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    B = _Builder()
+    B.quad([[-6, 0, -6], [-6, 0, 6], [6, 0, 6], [6, 0, -6]], [0, 1, 0], B.material(LAMBERT, (0.6, 0.55, 0.5)))
+    for lo, hi, mat in (([-1.5, 0, -1.0], [-0.5, 1.2, 0.0], B.material(LAMBERT, rng.uniform(0.2, 0.8, 3))),
+                        ([0.4, 0, -0.4], [1.4, 0.8, 0.6], B.material(MICROFACET, rng.uniform(0.2, 0.8, 3), 1.5, 0.3))):
+        v, n, t = box_mesh(lo, hi)
+        B.bundle(v, n, t, mat)
+    em = B.material(EMISSIVE, (6.0, 5.5, 5.0))
+    lq = np.asarray([[-0.4, 2.5, 1.2], [0.4, 2.5, 1.2], [0.4, 2.5, 1.8], [-0.4, 2.5, 1.8]], np.float32)
+    lb = B.bundle(lq, np.tile([0, -1, 0], (4, 1)), [[0, 2, 1], [0, 3, 2]], em)
+    B.area_lights(lb)
+    others = [[POINT_LIGHT, 1.5, 1.8, 1.5, 3.0, 2.5, 2.0, 0, 0],
+              [DIRECTIONAL_LIGHT, -0.3, -1.0, -0.4, 0.9, 0.85, 0.8, 0, 0],
+              [env_kind, 0, 0, 0, 0, 0, 0, 0.8, 2.2]]
+    cam = [0.0, 2.2, 6.0, 0.0, 0.6, 0.0, 0, 1, 0, 0.01, 10000.0, 0.9, width / height]
+    return B.build(cam, "open_scene", {"env_kind": env_kind}, other_lights=others, env=env_map())
+
+
+def floor_under(light_row, env=None, albedo=0.5, target=(0.3, 0.0, 0.2), fovy=1e-3, width=32, height=32) -> Scene:
+    """A 40 x 40 Lambert floor lit by one non-area light (a bsh_light row), seen through a very
+    narrow camera aimed at `target`: with depth 1 the image estimates the light reflected there,
+    which has a closed form (point: albedo/pi e cos/d^2; directional: albedo/pi e cos; a constant
+    environment of radiance c: albedo c)."""
+    B = _Builder()
+    B.quad([[-20, 0, -20], [-20, 0, 20], [20, 0, 20], [20, 0, -20]], [0, 1, 0],
+           B.material(LAMBERT, (albedo, albedo, albedo)))
+    cam = [0.0, 1.0, 3.0, target[0], target[1], target[2], 0, 1, 0, 0.01, 10000.0, fovy, width / height]
+    return B.build(cam, "floor_under", {"albedo": albedo, "target": np.asarray(target)},
+                   other_lights=[light_row], env=env)

@@ -211,6 +211,31 @@ bsh_status bsh_import_obj(const char* path, bsh_asset_counts* counts, float* ver
 bsh_status bsh_scene_set_materials(bsh_scene* s, const bsh_material* mats, size_t n);
 /* World::lights_ (AreaLight only) */
 bsh_status bsh_scene_set_lights(bsh_scene* s, const bsh_area_light* lights, size_t n);
+
+/* The reference's other lights (light/pointlight.h, directional_light.h, environment_light.h,
+   environment_light_is.h). They follow the area lights in World::lights_, in array order: the
+   light list GiTracer loops over is the area lights of bsh_scene_set_lights, then these. */
+enum
+{
+    BSH_LIGHT_POINT = 1,          /* PointLight(p = v, e): GetSample d = p - x, pdf 1, e / |d|^2 */
+    BSH_LIGHT_DIRECTIONAL = 2,    /* DirectionalLight(d = v, e): v is normalised as the ctor does */
+    BSH_LIGHT_ENVIRONMENT = 3,    /* EnvironmentLight(texsys, map, scale, gamma): cosine-sampled */
+    BSH_LIGHT_ENVIRONMENT_IS = 4  /* EnvironmentLightIs(texsys, map, scale, gamma): sampled by a
+                                     512 x 256 Distribution2D of the map's luminance x sin(theta) */
+};
+typedef struct
+{
+    int32_t type;       /* BSH_LIGHT_* */
+    float v[3];         /* point: position; directional: direction; environment: unused */
+    float e[3];         /* point, directional: emission; environment: unused */
+    float scale, gamma; /* environment: radiance scale and the map's gamma (reference defaults 1, 2.2) */
+} bsh_light;
+bsh_status bsh_scene_set_other_lights(bsh_scene* s, const bsh_light* lights, size_t n);
+/* The lat-long map the environment lights read (the texture they name): height rows of width
+   RGB floats, host memory, copied. Row 0 is theta = 0 (+y); column 0 is phi = 0 (+x), phi
+   growing toward +z (math/mathutils.h:68-74). It is read with bilinear filtering and periodic
+   wrap, texel centres at half-integer coordinates (TextureSystem defaults). */
+bsh_status bsh_scene_set_env_map(bsh_scene* s, const float* texels, int width, int height);
 /* World::camera_ */
 bsh_status bsh_scene_set_camera(bsh_scene* s, const bsh_camera* cam);
 /* World::Commit (world/world.cpp:5-16): host SAH build, flatten, upload. params may be NULL. */
@@ -463,6 +488,12 @@ bsh_status bsh_diag_keyed(bsh_context* ctx, uint32_t seed, size_t n, const uint3
    sampler BSH_SAMPLER_<kind>(arg); image == 0: the first n samples of a light sampler
    <kind>(1) in camera sample 0. out: host float[n*2]. */
 bsh_status bsh_diag_samples(bsh_context* ctx, int kind, int arg, int image, uint32_t seed, size_t n, float* out);
+/* Device Light::GetSample / GetPdf / GetLe of light `light` of the light list (area lights, then
+   the other lights) at points p (n x 3) with shading normals nrm (n x 3), samples u (n x 2) and
+   pdf/GetLe directions w (n x 3); out: n x 16 = le(3), pdf, d(3), GetPdf(w), GetLe(w)(3), 0 x 5.
+   Host pointers. */
+bsh_status bsh_diag_light_full(bsh_scene* s, int light, size_t n, const float* p, const float* nrm, const float* u,
+                               const float* w, float* out);
 
 /* Memory ceiling for the roofline (no reference counterpart): times one streaming read of `reps`
    passes over `bytes` of device memory at `buf` (a multiple of 16, at least 4 KiB) after one

@@ -17,6 +17,14 @@
  *   - KeyedRng: an Rng (Banshee/rng/rng.h:39-53) drawing from the keyed
  *     sample stream, plus link-time wrapping of rand()/srand() so GiTracer's
  *     light pick and Russian roulette (gitracer.cpp:75,119) draw from it too.
+ *   - MapTextureSystem: in-memory float maps served by name. The reference's
+ *     only TextureSystem reads textures through OpenImageIO 1.6
+ *     (texture/oiio_texturesystem.cpp), which is absent here; this restates
+ *     the two filters and wrap modes it maps to: InterpClosest (kPoint) reads
+ *     texel (floor(u w), floor(v h)); InterpBilinear (kBilinear) blends the
+ *     four texels around (u w - 1/2, v h - 1/2), texel centres at half-integer
+ *     coordinates; WrapPeriodic (kRepeat) wraps texel indices, WrapMirror
+ *     (kMirror) reflects them.
  *   - KeyedCamera: a PerscpectiveCamera whose GenerateRay (a virtual of
  *     Banshee/camera/camera.h:53) first moves the keyed stream to the next
  *     camera sample, so the light, BSDF and rand() draws of a camera sample are
@@ -27,7 +35,10 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <cmath>
+#include <map>
 #include <memory>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -40,6 +51,12 @@
 #include "bsdf/microfacet.h"
 #include "bsdf/fresnel.h"
 #include "light/arealight.h"
+#include "light/pointlight.h"
+#include "light/directional_light.h"
+#include "light/environment_light.h"
+#include "light/environment_light_is.h"
+#include "math/distribution1d.h"
+#include "math/distribution2d.h"
 #include "camera/perspective_camera.h"
 #include "tracer/gitracer.h"
 #include "sampler/stratified_sampler.h"
@@ -74,6 +91,52 @@ public:
 };
 
 NullTextureSystem g_texsys;
+
+class MapTextureSystem : public TextureSystem
+{
+public:
+    struct Map
+    {
+        int w = 0, h = 0;
+        std::vector<float> t;   // h rows of w RGB texels
+    };
+    std::map<std::string, Map> maps;
+
+    static int wrap(int i, int n, bool mirror)
+    {
+        if (!mirror) return ((i % n) + n) % n;
+        int k = ((i % (2 * n)) + 2 * n) % (2 * n);
+        return k < n ? k : 2 * n - 1 - k;
+    }
+    static float3 texel(Map const& m, int x, int y)
+    {
+        float const* p = &m.t[3 * ((size_t)y * m.w + x)];
+        return float3(p[0], p[1], p[2]);
+    }
+    float3 Sample(std::string const& name, float2 const& uv, float2 const&, Options const& o) const
+    {
+        auto it = maps.find(name);
+        if (it == maps.end()) return float3(0, 0, 0);
+        Map const& m = it->second;
+        bool mirror = o.wrapmode == Options::kMirror;
+        if (o.filter == Options::kPoint)
+            return texel(m, wrap((int)std::floor(uv.x * (float)m.w), m.w, mirror),
+                         wrap((int)std::floor(uv.y * (float)m.h), m.h, mirror));
+        float s = uv.x * (float)m.w - 0.5f, t = uv.y * (float)m.h - 0.5f;
+        float sf = std::floor(s), tf = std::floor(t);
+        float fs = s - sf, ft = t - tf;
+        int x0 = wrap((int)sf, m.w, mirror), x1 = wrap((int)sf + 1, m.w, mirror);
+        int y0 = wrap((int)tf, m.h, mirror), y1 = wrap((int)tf + 1, m.h, mirror);
+        float3 a = texel(m, x0, y0), b = texel(m, x1, y0), c = texel(m, x0, y1), d = texel(m, x1, y1);
+        return (1.f - ft) * ((1.f - fs) * a + fs * b) + ft * ((1.f - fs) * c + fs * d);
+    }
+    void GetTextureInfo(std::string const& name, TextureDesc& d) const
+    {
+        auto it = maps.find(name);
+        d.width = it == maps.end() ? 0 : it->second.w;
+        d.height = it == maps.end() ? 0 : it->second.h;
+    }
+};
 
 /* ---------------- keyed stream context (single-threaded use) ---------------- */
 struct KeyCtx
@@ -292,6 +355,7 @@ public:
 
 struct Scene
 {
+    MapTextureSystem texsys;   // the environment lights' map, named "env"
     World world;
     std::vector<RecordingMesh*> meshes;
     RecordingAccel* accel = nullptr;
@@ -645,6 +709,88 @@ void oref_bsdf(float const* mat, int n, float const* frames, float const* view, 
 
 /* Area light probes on scene light `light`. p n x 3 shading points, u n x 2,
    w n x 3 directions for GetPdf. out n x 8: le3, pdf, d3, getpdf. */
+/* The map the scene's environment lights read ("env"): h rows of w RGB floats. */
+void oref_scene_set_env(void* h, float const* tex, int w, int hgt)
+{
+    Scene* s = (Scene*)h;
+    MapTextureSystem::Map m;
+    m.w = w;
+    m.h = hgt;
+    m.t.assign(tex, tex + 3 * (size_t)w * hgt);
+    s->texsys.maps["env"] = m;
+}
+
+/* Appends a light to World::lights_ after the area lights: type 1 PointLight(v, e), 2
+   DirectionalLight(v, e), 3 EnvironmentLight(env, scale, gamma), 4 EnvironmentLightIs(env,
+   scale, gamma) (its ctor reads the map, so set the map first). */
+void oref_scene_add_light(void* h, int type, float const* v, float const* e, float scale, float gamma)
+{
+    Scene* s = (Scene*)h;
+    Light* L = nullptr;
+    if (type == 1) L = new PointLight(float3(v[0], v[1], v[2]), float3(e[0], e[1], e[2]));
+    else if (type == 2) L = new DirectionalLight(float3(v[0], v[1], v[2]), float3(e[0], e[1], e[2]));
+    else if (type == 3) L = new EnvironmentLight(s->texsys, "env", scale, gamma);
+    else L = new EnvironmentLightIs(s->texsys, "env", scale, gamma);
+    s->world.lights_.push_back(std::unique_ptr<Light>(L));
+}
+
+/* Light::GetSample / GetPdf / GetLe of light `light` at points p with shading normals nrm:
+   out[16 i ..] = le(3), pdf, d(3), GetPdf(w), GetLe(ray along w)(3), 0 x 5. */
+void oref_light_full(void* h, int light, int n, float const* p, float const* nrm, float const* u, float const* w,
+                     float* out)
+{
+    Scene* s = (Scene*)h;
+    Light const& L = *s->world.lights_[light];
+    for (int i = 0; i < n; ++i)
+    {
+        ShapeBundle::Hit hit;
+        hit.p = float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+        hit.n = float3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+        float3 d;
+        float pdf = 0.f;
+        float3 le = L.GetSample(hit, float2(u[2 * i], u[2 * i + 1]), d, pdf);
+        float3 ww(w[3 * i], w[3 * i + 1], w[3 * i + 2]);
+        float gp = L.GetPdf(hit, ww);
+        ray r;
+        r.o = hit.p;
+        r.d = ww;
+        float3 ge = L.GetLe(r);
+        float* o = out + 16 * i;
+        o[0] = le.x; o[1] = le.y; o[2] = le.z; o[3] = pdf;
+        o[4] = d.x; o[5] = d.y; o[6] = d.z; o[7] = gp;
+        o[8] = ge.x; o[9] = ge.y; o[10] = ge.z;
+        for (int k = 11; k < 16; ++k) o[k] = 0.f;
+    }
+}
+
+/* The texture system's lookup of the scene's map: out[3 i ..] = Sample("env", uv_i) with the
+   given filter (0 point, 1 bilinear) and wrap (0 repeat, 1 mirror). */
+void oref_texture_sample(void* h, int n, float const* uv, int filter, int wrapmode, float* out)
+{
+    Scene* s = (Scene*)h;
+    TextureSystem::Options o(filter ? TextureSystem::Options::kBilinear : TextureSystem::Options::kPoint,
+                             wrapmode ? TextureSystem::Options::kMirror : TextureSystem::Options::kRepeat);
+    for (int i = 0; i < n; ++i)
+    {
+        float3 v = s->texsys.Sample("env", float2(uv[2 * i], uv[2 * i + 1]), float2(0, 0), o);
+        out[3 * i] = v.x; out[3 * i + 1] = v.y; out[3 * i + 2] = v.z;
+    }
+}
+
+/* Distribution2D(n, m, values) (math/distribution2d.cpp): Sample2D at k sample pairs
+   (out_s: sample uv, pdf) and Pdf at k query points (out_p). */
+void oref_dist2d(int n, int m, float const* values, int k, float const* u, float const* q, float* out_s, float* out_p)
+{
+    Distribution2D d(n, m, values);
+    for (int i = 0; i < k; ++i)
+    {
+        float pdf = 0.f;
+        float2 sm = d.Sample2D(float2(u[2 * i], u[2 * i + 1]), pdf);
+        out_s[3 * i] = sm.x; out_s[3 * i + 1] = sm.y; out_s[3 * i + 2] = pdf;
+        out_p[i] = d.Pdf(float2(q[2 * i], q[2 * i + 1]));
+    }
+}
+
 void oref_light(void* h, int light, int n, float const* p, float const* u, float const* w, float* out)
 {
     Scene* s = (Scene*)h;

@@ -59,6 +59,11 @@ def _load(kind):
             lib.cref_render_bsdf_only.argtypes = [vp, i, i, i, i, i, i, i, i, ctypes.c_uint, vp, vp, vp, vp, i]
         if kind == "ref":
             lib.oref_render_keyed_frame.argtypes = [vp, i, i, i, i, i, i, i, ctypes.c_uint, vp, vp, vp, vp, vp]
+            lib.oref_scene_set_env.argtypes = [vp, vp, i, i]
+            lib.oref_scene_add_light.argtypes = [vp, i, vp, vp, ctypes.c_float, ctypes.c_float]
+            lib.oref_light_full.argtypes = [vp, i, i, vp, vp, vp, vp, vp]
+            lib.oref_texture_sample.argtypes = [vp, i, vp, i, i, vp]
+            lib.oref_dist2d.argtypes = [i, i, vp, i, vp, vp, vp, vp]
             lib.oref_image_samples.argtypes = [i, i, ctypes.c_uint, i, vp]
             lib.oref_nee_samples.argtypes = [i, ctypes.c_uint, i, vp]
             lib.oref_record_begin.argtypes = [vp, i, i]
@@ -131,6 +136,17 @@ class Oracle:
         self.lib.oref_nee_samples(SAMPLERS[sampler], seed, n, _p(out))
         return out
 
+    def dist2d(self, values, u, q):
+        """Reference only: Distribution2D(n, m, values [m, n]): Sample2D at u (k x 2) -> (k x 3:
+        uv, pdf) and Pdf at q (k x 2)."""
+        values = f32(values)
+        m, n = values.shape
+        u, q = f32(u), f32(q)
+        s = np.zeros((u.shape[0], 3), np.float32)
+        p = np.zeros(q.shape[0], np.float32)
+        self.lib.oref_dist2d(n, m, _p(values), u.shape[0], _p(u), _p(q), _p(s), _p(p))
+        return s, p
+
     def sh_eval(self, dirs, lmax):
         dirs = f32(dirs)
         out = np.zeros((dirs.shape[0], (lmax + 1) ** 2), np.float32)
@@ -169,6 +185,18 @@ class OracleScene:
             _p(k[5]), k[5].shape[0], _p(k[6]), k[6].shape[0], _p(k[7]), k[7].shape[0], _p(k[8]))
         if not self.h:
             raise RuntimeError("oracle scene creation failed")
+        others = getattr(sc, "other_lights", None)
+        if others is not None and len(others):
+            if orc.kind != "ref":
+                raise RuntimeError("point, directional and environment lights need the reference oracle")
+            env = getattr(sc, "env", None)
+            if env is not None:
+                e = f32(env)
+                self._keep.append(e)
+                orc.lib.oref_scene_set_env(self.h, _p(e), e.shape[1], e.shape[0])
+            for r in np.asarray(others, np.float32).reshape(-1, 9):
+                v, em = f32(r[1:4]), f32(r[4:7])
+                orc.lib.oref_scene_add_light(self.h, int(r[0]), _p(v), _p(em), float(r[7]), float(r[8]))
 
     def __del__(self):
         try:
@@ -244,6 +272,20 @@ class OracleScene:
         self.o.lib.cref_render_bsdf_only(self.h, W, H, x0, y0, tw, th, img_grid, depth, seed, _p(b), _p(s), _p(q),
                                          _p(n), threads)
         return s, q, n
+
+    def light_full(self, idx, p, nrm, u, w):
+        """Reference only: Light::GetSample / GetPdf / GetLe, n x 16 (bsh_diag_light_full's layout)."""
+        arrs = [f32(a) for a in (p, nrm, u, w)]
+        out = np.zeros((arrs[0].shape[0], 16), np.float32)
+        self.o.lib.oref_light_full(self.h, idx, arrs[0].shape[0], *[_p(a) for a in arrs], _p(out))
+        return out
+
+    def texture_sample(self, uv, filt=1, wrap=0):
+        """Reference only: the texture system's lookup of the scene's environment map."""
+        uv = f32(uv)
+        out = np.zeros((uv.shape[0], 3), np.float32)
+        self.o.lib.oref_texture_sample(self.h, uv.shape[0], _p(uv), filt, wrap, _p(out))
+        return out
 
     def light(self, idx, p, u, w):
         p, u, w = f32(p), f32(u), f32(w)
