@@ -229,7 +229,7 @@ bsh_status bsh_scene_set_materials(bsh_scene* s, const bsh_material* mats, size_
     for (size_t i = 0; i < n; ++i)
     {
         int t = (int)mats[i].type;
-        if (t < 0 || t > 2 || (float)t != mats[i].type)
+        if (t < BSH_LAMBERT || t > BSH_MIXED || (float)t != mats[i].type)
             return fail(BSH_ERR_INVALID, "bsh_scene_set_materials: unknown material type at " + std::to_string(i));
     }
     s->mats.assign(mats, mats + n);
@@ -288,6 +288,48 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
         if (s->tri_m[i] < 0 || (size_t)s->tri_m[i] >= s->mats.size())
             return fail(BSH_ERR_INVALID, "bsh_scene_commit: material index out of range at triangle " +
                                              std::to_string(i));
+    // material rows: types 0-7; a MixedMaterial's components are rows of single BSDFs
+    s->ext_mats = 0;
+    for (size_t k = 0; k < s->mats.size(); ++k)
+    {
+        const bsh_material& m = s->mats[k];
+        const int t = (int)m.type;
+        if (!(m.type == (float)t) || t < BSH_LAMBERT || t > BSH_MIXED)
+            return fail(BSH_ERR_INVALID, "bsh_scene_commit: material " + std::to_string(k) + " has an unknown type");
+        if (t >= BSH_ORENNAYAR) s->ext_mats = 1;
+        if (t == BSH_MIXED)
+        {
+            int nr = 0, ntr = 0;
+            for (float fid : {m.alpha, m.pad[0], m.pad[1]})
+            {
+                const int id = (int)fid;
+                if (id < 0) continue;
+                if ((size_t)id >= s->mats.size())
+                    return fail(BSH_ERR_INVALID, "bsh_scene_commit: mixed material " + std::to_string(k) +
+                                                     " names a component row out of range");
+                const int ct = (int)s->mats[id].type;
+                if (ct != BSH_LAMBERT && ct != BSH_MICROFACET && ct != BSH_ORENNAYAR && ct != BSH_PERFECT_REFLECT &&
+                    ct != BSH_PERFECT_REFRACT)
+                    return fail(BSH_ERR_INVALID, "bsh_scene_commit: mixed material " + std::to_string(k) +
+                                                     ": a component must be a single BSDF (types 0, 1, 3, 4, 5)");
+                (bsdf_flags(ct) & BSDF_REFLECTION ? nr : ntr)++;
+            }
+            if (nr == 0)
+                return fail(BSH_ERR_INVALID, "bsh_scene_commit: mixed material " + std::to_string(k) +
+                                                 " needs a reflecting component (MixedMaterial::Sample asserts one)");
+        }
+    }
+    {
+        std::vector<float4> mt(2 * std::max<size_t>(s->mats.size(), 1), make_float4(0.f, 0.f, 0.f, 0.f));
+        for (size_t k = 0; k < s->mats.size(); ++k)
+        {
+            const bsh_material& m = s->mats[k];
+            mt[2 * k] = make_float4(m.type, m.color[0], m.color[1], m.color[2]);
+            mt[2 * k + 1] = make_float4(m.eta, m.alpha, m.pad[0], m.pad[1]);
+        }
+        BSH_HIP(s->d_mats.alloc(mt.size()));
+        BSH_HIP(hipMemcpy(s->d_mats.p, mt.data(), mt.size() * sizeof(float4), hipMemcpyHostToDevice));
+    }
     // lights: validate exactly as AreaLight's ctor does (arealight.cpp:16-19)
     std::vector<int32_t> bundle_light(s->bundles.size(), -1);
     s->h_lights.clear();
@@ -429,7 +471,7 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
             r[4] = v4(s->normals, s->tri_n[3 * i + 1], 0.f);
             r[5] = v4(s->normals, s->tri_n[3 * i + 2], 0.f);
             r[6] = make_float4(m.type, m.color[0], m.color[1], m.color[2]);
-            r[7] = make_float4(m.eta, m.alpha, 0.f, 0.f);
+            r[7] = make_float4(m.eta, m.alpha, m.pad[0], m.pad[1]);
         }
     }
     const size_t node_f4 = bo.width == 4 ? bo.nodes4.size() * 8 : bo.nodes.size() * 4;

@@ -159,6 +159,7 @@ struct Mat
     int type;
     f3 c;
     float eta, alpha;
+    float x, y;   // material rows 4-7 (material_dev.h): Fresnel flag, MixedMaterial components
 };
 
 enum : int { BSDF_REFLECTION = 1, BSDF_DIFFUSE = 4, BSDF_GLOSSY = 16 };

@@ -20,6 +20,8 @@ __global__ void k_diag_bsdf(float4 m0, float4 m1, uint32_t n, const float* frame
     m.c = mk(m0.y, m0.z, m0.w);
     m.eta = m1.x;
     m.alpha = m1.y;
+    m.x = m1.z;
+    m.y = m1.w;
     const float* f = frames + 12 * i;
     Frame h;
     h.p = mk(0.f, 0.f, 0.f);
@@ -94,6 +96,36 @@ __global__ void k_diag_samples(int kind, int arg, int image, uint32_t seed, uint
         nee_sample2d(kind, ks_base(seed, 0u), KS_LIGHT, 2u * i, x, y);
     out[2 * i] = x;
     out[2 * i + 1] = y;
+}
+
+// Material::Sample / Evaluate / GetPdf of material row `midx` of the scene (every type), the
+// rand() draws of case i keyed by ks_base(seed, i) from counter 0.
+__global__ void k_diag_material(SceneDev s, int midx, uint32_t seed, uint32_t n, const float* frames, const float* view,
+                                const float* u, const float* ea, const float* eb, float* out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Mat m = load_mat(s.mats, midx);
+    const float* f = frames + 12 * i;
+    Frame h;
+    h.p = mk(0.f, 0.f, 0.f);
+    h.n = mk(f[0], f[1], f[2]);
+    h.ng = mk(f[3], f[4], f[5]);
+    h.dpdu = mk(f[6], f[7], f[8]);
+    h.dpdv = mk(f[9], f[10], f[11]);
+    f3 wi = mk(view[3 * i], view[3 * i + 1], view[3 * i + 2]);
+    f3 wo = mk(0.f, 0.f, 0.f);
+    float pdf = 0.f;
+    int type = 0;
+    RandStream rs{ks_base(seed, i), 0u};
+    f3 fs = mat_sample_x(s.mats, m, h, u[2 * i], u[2 * i + 1], wi, wo, pdf, type, rs);
+    f3 a = mk(ea[3 * i], ea[3 * i + 1], ea[3 * i + 2]), b = mk(eb[3 * i], eb[3 * i + 1], eb[3 * i + 2]);
+    f3 fe = mat_eval_x(s.mats, m, h, a, b);
+    float pe = mat_pdf_x(s.mats, m, h, a, b);
+    float* o = out + 12 * i;
+    o[0] = fs.x; o[1] = fs.y; o[2] = fs.z; o[3] = pdf;
+    o[4] = wo.x; o[5] = wo.y; o[6] = wo.z; o[7] = (float)type;
+    o[8] = fe.x; o[9] = fe.y; o[10] = fe.z; o[11] = pe;
 }
 
 // Light::GetSample / GetPdf / GetLe of light `light` (any type) at points p with shading
@@ -276,6 +308,29 @@ bsh_status bsh_diag_light_full(bsh_scene* s, int light, size_t n, const float* p
                        (uint32_t)n, dp.p, dn.p, du.p, dw.p, o.p);
     BSH_HIP(hipGetLastError());
     BSH_HIP(hipMemcpy(out, o.p, 16 * n * sizeof(float), hipMemcpyDeviceToHost));
+    return BSH_OK;
+}
+
+bsh_status bsh_diag_material(bsh_scene* s, int material, uint32_t seed, size_t n, const float* frames,
+                             const float* view, const float* u, const float* ea, const float* eb, float* out)
+{
+    if (!s || (n && (!frames || !view || !u || !ea || !eb || !out)))
+        return fail(BSH_ERR_INVALID, "bsh_diag_material: NULL argument");
+    if (!s->committed) return fail(BSH_ERR_STATE, "bsh_diag_material: scene not committed");
+    if (material < 0 || (size_t)material >= s->mats.size()) return fail(BSH_ERR_INVALID, "bsh_diag_material: bad row");
+    if (n == 0) return BSH_OK;
+    BSH_HIP(hipSetDevice(s->ctx->device));
+    DevBuf<float> df, dv, du, da, db, o;
+    BSH_TRY(up(df, frames, 12 * n));
+    BSH_TRY(up(dv, view, 3 * n));
+    BSH_TRY(up(du, u, 2 * n));
+    BSH_TRY(up(da, ea, 3 * n));
+    BSH_TRY(up(db, eb, 3 * n));
+    BSH_HIP(o.alloc(12 * n));
+    hipLaunchKernelGGL(k_diag_material, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, 0, s->dev(), material, seed,
+                       (uint32_t)n, df.p, dv.p, du.p, da.p, db.p, o.p);
+    BSH_HIP(hipGetLastError());
+    BSH_HIP(hipMemcpy(out, o.p, 12 * n * sizeof(float), hipMemcpyDeviceToHost));
     return BSH_OK;
 }
 

@@ -11,6 +11,7 @@
 #include "../../include/banshee_hip.h"
 #include "bsh_math.h"
 #include "bvh_build.h"
+#include "material_dev.h"
 #include "sampler_dev.h"
 #include "scene_dev.h"
 
@@ -195,6 +196,8 @@ struct bsh_scene
     bsh::DevBuf<bsh::LightDev> d_lights;
     bsh::DevBuf<float> d_light_cdf;
     bsh::DevBuf<float> d_env;             // map, then the environment distribution's tables
+    bsh::DevBuf<float4> d_mats;           // material rows, 2 float4 each
+    int ext_mats = 0;
     bsh::EnvDev env{};
     std::vector<bsh::LightDev> h_lights;
     float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {0.f, 0.f, 0.f};
@@ -216,6 +219,8 @@ struct bsh_scene
         s.env = env;
         s.nlights = (int)(lights.size() + others.size());
         s.narea = (int)lights.size();
+        s.mats = d_mats.p;
+        s.ext_mats = ext_mats;
         s.ntris = (int)tri_v.size() / 3;
         return s;
     }

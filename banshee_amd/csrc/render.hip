@@ -679,7 +679,8 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 // code), keeping the plain path's registers low. kOne: the run has a single job, so its
 // record and scene pointers are read with scalar loads at the start instead of per lane
 // after the path's meta word. kLights (with kFull) adds point, directional and environment
-// lights, kept out of the other variants' registers.
+// lights and the materials beyond Lambert, Microfacet and Emissive (material_dev.h), kept out
+// of the other variants' registers.
 template <bool kFull, bool kOne, bool kLights = false>
 __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev R, int b, int Lout)
 {
@@ -716,7 +717,11 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
         const bool nq = (jflags & JOB_NO_QUIRKS) != 0;
         if (!(fl & F_DONE))
         {
-            f3 rad = ld3(R.rad[i]);
+            const float4 rad4 = R.rad[i];
+            f3 rad = ld3(rad4);
+            // kLights runs keep the path's rand() draws in a running counter (rad.w): Glass and
+            // MixedMaterial draw inside Sample, so the draw count depends on the path
+            RandStream rs{meta.x, kLights ? __float_as_uint(rad4.w) : 0u};
             const bool had_nee = (fl & F_NEE) != 0;
             // ---- 1. resolve the NEE rays issued at bounce pb (DiTracer::GetDi tail) ----
             if (fl & F_NEE)
@@ -855,8 +860,12 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                             float pickw = 1.f;
                             if (!all)
                             {
-                                if (nq) Lsel = pick_light(S, ks_rand_float(base, rand_pick_counter(b)), pickw);
-                                else Lsel = (int)(ks_rand(base, rand_pick_counter(b)) % (uint32_t)nl);
+                                if (nq)
+                                    Lsel = pick_light(S, kLights ? rs.next_float() : ks_rand_float(base, rand_pick_counter(b)),
+                                                      pickw);
+                                else
+                                    Lsel = (int)((kLights ? rs.next_uint() : ks_rand(base, rand_pick_counter(b))) %
+                                                 (uint32_t)nl);
                             }
                             for (int j = 0; j < jn; ++j)
                             {
@@ -882,14 +891,15 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                 {
                                     f3 wi = normalize(ld);
                                     float dist = sqrtf(sqnorm(ld));
-                                    f3 f = mat_eval(m, h, wi, wo);
+                                    f3 f = kLights ? mat_eval_x(S.mats, m, h, wi, wo) : mat_eval(m, h, wi, wo);
                                     float w = 1.f;
                                     if (!singular)
                                     {
                                         // the reference asks GetPdf(light dir, view), the density of the
                                         // view direction; MIS needs the density of sampling the light
                                         // direction from the view, GetPdf(view, light dir)
-                                        float bp = nq ? mat_pdf(m, h, wo, wi) : mat_pdf(m, h, wi, wo);
+                                        float bp = kLights ? (nq ? mat_pdf_x(S.mats, m, h, wo, wi) : mat_pdf_x(S.mats, m, h, wi, wo))
+                                                           : (nq ? mat_pdf(m, h, wo, wi) : mat_pdf(m, h, wi, wo));
                                         // quirk-free: both MIS weights in solid angle (the reference
                                         // weighs an area light's area pdf against the BSDF's solid-angle
                                         // pdf; environment pdfs are in solid angle already)
@@ -911,17 +921,22 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                 f3 wis = mk(0.f, 0.f, 0.f);
                                 float bpdf = 0.f;
                                 int type = 0;
-                                f3 fb = singular ? mk(0.f, 0.f, 0.f) : mat_sample(m, h, bx, by, wo, wis, bpdf, type);
+                                f3 fb = singular ? mk(0.f, 0.f, 0.f)
+                                                 : kLights ? mat_sample_x(S.mats, m, h, bx, by, wo, wis, bpdf, type, rs)
+                                                           : mat_sample(m, h, bx, by, wo, wis, bpdf, type);
                                 f3 wi = normalize(wis);
                                 if (!singular && sqnorm(fb) > 0.f && (nq ? bpdf > 0.f : bpdf > kMinPdf))
                                 {
-                                    float lp = area ? light_pdf(L, h.p, wi) : other_light_pdf(S, *L, h.n, wi);
+                                    // a specular BSDF sample takes weight 1 without asking the light's
+                                    // pdf (ditracer.cpp:167-181)
+                                    const bool spec = kLights && (type & BSDF_SPECULAR) != 0;
+                                    float lp = spec ? 0.f : area ? light_pdf(L, h.p, wi) : other_light_pdf(S, *L, h.n, wi);
                                     // quirk-free: where an environment light's density is 0 the BSDF
                                     // sample carries the whole weight (an area light's density is 0
                                     // only where the ray cannot reach it)
-                                    if (nq ? lp > 0.f || !area : !(lp < kMinPdf))
+                                    if (spec || (nq ? lp > 0.f || !area : !(lp < kMinPdf)))
                                     {
-                                        float w = nq && !(lp > 0.f) ? 1.f : power_heuristic(bpdf, lp);
+                                        float w = spec || (nq && !(lp > 0.f)) ? 1.f : power_heuristic(bpdf, lp);
                                         R.nee_a[ks + 1] = st3(fb, fabsf(dot(h.n, wi)));
                                         R.nee_b[ks + 1] = make_float4(nq ? w * pickw : w, 1.f / bpdf, (float)Lid, 0.f);
                                         R.nee_wi[ks + 1] = st3(wi, 10000000.f);
@@ -963,11 +978,22 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                     // stays unbiased
                                     guided = lobes_finite(G);
                                 }
+                                // materials with delta lobes (perfect reflection and refraction, glass,
+                                // mixtures) continue with their own sample and its weight, unguided
+                                const bool own = kLights && m.type >= MAT_REFLECT;
+                                guided = guided && !own;
                                 float alpha = guided ? J.bsdf_frac : 1.f;
                                 float usel = ks_float(ks_u32(base, KS_GUIDE, c6 + 2));
-                                if (!guided || usel < alpha)
+                                f3 fown = mk(0.f, 0.f, 0.f);
+                                if (own)
                                 {
-                                    (void)mat_sample(m, h, gx, gy, wo, wi, pdf, type);
+                                    fown = mat_sample_x(S.mats, m, h, gx, gy, wo, wi, pdf, type, rs);
+                                    wi = normalize(wi);
+                                }
+                                else if (!guided || usel < alpha)
+                                {
+                                    if (kLights) (void)mat_sample_x(S.mats, m, h, gx, gy, wo, wi, pdf, type, rs);
+                                    else (void)mat_sample(m, h, gx, gy, wo, wi, pdf, type);
                                     wi = normalize(wi);
                                 }
                                 else
@@ -977,14 +1003,20 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                     float u2 = ks_float(ks_u32(base, KS_GUIDE, c6 + 5));
                                     wi = guide_sample(G, R.gout + (size_t)grow * kGUsed, u0, u1, u2);
                                 }
-                                f = mat_eval(m, h, wo, wi);
-                                float pb = mat_pdf(m, h, wo, wi);
-                                pdf = alpha * pb + (guided ? (1.f - alpha) * guide_pdf(G, wi) : 0.f);
+                                if (own)
+                                    f = fown;
+                                else
+                                {
+                                    f = kLights ? mat_eval_x(S.mats, m, h, wo, wi) : mat_eval(m, h, wo, wi);
+                                    float pb = kLights ? mat_pdf_x(S.mats, m, h, wo, wi) : mat_pdf(m, h, wo, wi);
+                                    pdf = alpha * pb + (guided ? (1.f - alpha) * guide_pdf(G, wi) : 0.f);
+                                }
                                 stop = !(pdf > 0.f) || sqnorm(f) == 0.f;
                             }
                             else
                             {
-                                f = mat_sample(m, h, ux, uy, wo, wi, pdf, type);
+                                f = kLights ? mat_sample_x(S.mats, m, h, ux, uy, wo, wi, pdf, type, rs)
+                                            : mat_sample(m, h, ux, uy, wo, wi, pdf, type);
                                 stop = sqnorm(f) == 0.f || pdf < kMinPdf;
                             }
                             if (stop)
@@ -1011,7 +1043,7 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                 bool alive = true;
                                 if (b > 3)
                                 {
-                                    float rnd = ks_rand_float(base, rand_pick_counter(b) + 1u);
+                                    float rnd = kLights ? rs.next_float() : ks_rand_float(base, rand_pick_counter(b) + 1u);
                                     float lum = 0.2126f * T.x + 0.7152f * T.y + 0.0722f * T.z;
                                     float q = (lum < 0.5f) ? lum : 0.5f;
                                     if (rnd > q) alive = false;
@@ -1076,7 +1108,7 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                 R.out[i] = st3(o, 0.f);
                 fl |= F_DONE;
             }
-            R.rad[i] = st3(rad, 0.f);
+            R.rad[i] = st3(rad, kLights ? __uint_as_float(rs.c) : 0.f);
             meta.y = fl | ((uint32_t)b << 8) | ((uint32_t)ncnt << 16) | (job << 24);
             R.meta[i] = meta;
             keep = !(fl & F_DONE);
@@ -1558,8 +1590,9 @@ static bsh_status enqueue_batch(bsh_context* c, const Lane* lanes, int nlanes, c
         for (const Segment& sg : ch.segs)
         {
             full = full || hjobs[sg.job].flags != 0;
-            // point, directional and environment lights are compiled into the kLights variants only
-            lights = lights || hjobs[sg.job].s.narea != hjobs[sg.job].s.nlights;
+            // point, directional and environment lights and the materials of rows 3-7 are compiled
+            // into the kLights variants only
+            lights = lights || hjobs[sg.job].s.narea != hjobs[sg.job].s.nlights || hjobs[sg.job].s.ext_mats;
             wide = wide && hjobs[sg.job].s.width == 4;
         }
         hipLaunchKernelGGL(k_zero, grid_for(w.counters.n), dim3(kBlock), 0, st, w.counters.p, (int)w.counters.n);

@@ -61,8 +61,10 @@ struct SceneDev
     const LightDev* lights;
     const float* light_cdf;   // per light: cumulative share of the emitted power (area x luminance), last = 1
     EnvDev env;               // environment map of the environment lights (tex NULL when none)
+    const float4* mats;       // material rows, 2 float4 each: (type, c), (eta, alpha, x, y)
     int nlights;
     int narea;                // lights [0, narea) are area lights, the rest point/directional/environment
+    int ext_mats;             // 1 when a material row is of type 3-7 (material_dev.h)
     int ntris;
     int width;                // BVH layout: 2 (GpuNode, 4 float4) or 4 (GpuNode4, 8 float4)
 };

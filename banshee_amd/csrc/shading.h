@@ -1,6 +1,7 @@
 // Scene-dependent shading helpers shared by the render and diagnostic kernels.
 #pragma once
 
+#include "material_dev.h"
 #include "scene_dev.h"
 
 namespace bsh {
@@ -23,6 +24,8 @@ __device__ __forceinline__ void tri_verts(const SceneDev& s, int prim, f3& p1, f
     m.c = mk(g.y, g.z, g.w);
     m.eta = h.x;
     m.alpha = h.y;
+    m.x = h.z;
+    m.y = h.w;
 }
 
 // The area light of the primitive's bundle, -1 when it has none.

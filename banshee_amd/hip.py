@@ -142,6 +142,7 @@ EXPORTS = [
     "bsh_bake_sh_guided", "bsh_guide_forward_fp32", "bsh_guide_gradient_fp32",
     "bsh_context_launch_stats", "bsh_diag_stream_read", "bsh_import_obj", "bsh_sh_project_env_noalloc",
     "bsh_diag_samples", "bsh_scene_set_other_lights", "bsh_scene_set_env_map", "bsh_diag_light_full",
+    "bsh_diag_material",
 ]
 
 
@@ -618,7 +619,7 @@ class Scene:
                                       _p(tm), c_size_t(tm.shape[0]), _p(b), c_size_t(b.shape[0]),
                                       ctypes.byref(self.h)), "bsh_scene_create")
         mats = np.zeros((sc.materials.shape[0], 8), np.float32)
-        mats[:, :6] = sc.materials[:, :6]
+        mats[:, :sc.materials.shape[1]] = sc.materials[:, :8]
         _check(lib().bsh_scene_set_materials(self.h, _p(mats), c_size_t(mats.shape[0])), "bsh_scene_set_materials")
         lights = np.ascontiguousarray(sc.lights, np.int32).reshape(-1, 2)
         _check(lib().bsh_scene_set_lights(self.h, _p(lights), c_size_t(lights.shape[0])), "bsh_scene_set_lights")
@@ -653,6 +654,15 @@ class Scene:
                                     ctypes.byref(ms), ctypes.byref(depth), ctypes.byref(width)), "bsh_scene_info")
         return {"nodes": nodes.value, "tris": tris.value, "bytes": nbytes.value, "build_ms": ms.value,
                 "depth": depth.value, "width": width.value}
+
+    def diag_material(self, material, seed, frames, view, u, ea, eb):
+        """Device Material::Sample / Evaluate / GetPdf of row `material` (bsh_diag_material): n x 12."""
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (frames, view, u, ea, eb)]
+        n = arrs[0].shape[0]
+        out = np.zeros((n, 12), np.float32)
+        _check(lib().bsh_diag_material(self.h, c_int(material), c_uint32(seed), c_size_t(n), *[_p(a) for a in arrs],
+                                       _p(out)), "bsh_diag_material")
+        return out
 
     def diag_light_full(self, light, p, nrm, u, w):
         """Device Light::GetSample / GetPdf / GetLe of light `light` (area lights first, then the

@@ -37,6 +37,9 @@ from dataclasses import dataclass, field
 import numpy as np
 
 LAMBERT, MICROFACET, EMISSIVE = 0, 1, 2
+# material rows 3-7: OrenNayar(kd, kr = alpha), PerfectReflect / PerfectRefract(eta, ks, Fresnel when
+# x != 0), Glass(eta, ks), MixedMaterial(eta) over the component rows alpha, x, y (-1: none)
+ORENNAYAR, PERFECT_REFLECT, PERFECT_REFRACT, GLASS, MIXED = 3, 4, 5, 6, 7
 # bsh_light types; a Scene's other_lights rows are (type, v xyz, e xyz, scale, gamma)
 POINT_LIGHT, DIRECTIONAL_LIGHT, ENV_LIGHT, ENV_LIGHT_IS = 1, 2, 3, 4
 
@@ -87,8 +90,8 @@ class _Builder:
         self.mats = []
         self.lights = []
 
-    def material(self, type_, c, eta=0.0, alpha=0.0) -> int:
-        self.mats.append([type_, c[0], c[1], c[2], eta, alpha, 0.0, 0.0])
+    def material(self, type_, c, eta=0.0, alpha=0.0, x=0.0, y=0.0) -> int:
+        self.mats.append([type_, c[0], c[1], c[2], eta, alpha, x, y])
         return len(self.mats) - 1
 
     def bundle(self, verts, normals, tris, mat) -> int:
@@ -571,3 +574,41 @@ def floor_under(light_row, env=None, albedo=0.5, target=(0.3, 0.0, 0.2), fovy=1e
     cam = [0.0, 1.0, 3.0, target[0], target[1], target[2], 0, 1, 0, 0.01, 10000.0, fovy, width / height]
     return B.build(cam, "floor_under", {"albedo": albedo, "target": np.asarray(target)},
                    other_lights=[light_row], env=env)
+
+
+def material_scene(width=24, height=24) -> Scene:
+    """A closed box lit by a ceiling quad, holding every material the reference offers beyond
+    Lambert and Microfacet: Oren-Nayar walls (two roughnesses), a glass box, a mirror panel
+    (PerfectReflect without Fresnel), a refracting panel (PerfectRefract with Fresnel), a
+    MixedMaterial floor (Lambert + Microfacet + PerfectReflect) and a MixedMaterial back wall
+    (Oren-Nayar + PerfectRefract)."""
+    B = _Builder()
+    lam = B.material(LAMBERT, (0.6, 0.6, 0.55))
+    on_a = B.material(ORENNAYAR, (0.7, 0.3, 0.25), alpha=0.4)
+    on_b = B.material(ORENNAYAR, (0.25, 0.6, 0.3), alpha=1.2)
+    ggx = B.material(MICROFACET, (0.7, 0.65, 0.5), 1.5, 0.25)
+    mirror = B.material(PERFECT_REFLECT, (0.9, 0.9, 0.9), 1.5, x=0.0)
+    refl_f = B.material(PERFECT_REFLECT, (0.8, 0.8, 0.8), 1.5, x=1.0)
+    refr = B.material(PERFECT_REFRACT, (0.9, 0.95, 0.9), 1.4, x=1.0)
+    glass = B.material(GLASS, (0.95, 0.95, 0.95), 1.5)
+    mix_floor = B.material(MIXED, (0, 0, 0), 1.6, alpha=lam, x=ggx, y=refl_f)
+    mix_back = B.material(MIXED, (0, 0, 0), 1.3, alpha=on_b, x=refr, y=-1)
+    em = B.material(EMISSIVE, (12.0, 11.0, 9.0))
+    B.quad([[0, 0, 0], [2, 0, 0], [2, 0, 2], [0, 0, 2]], [0, 1, 0], mix_floor)                 # floor
+    B.quad([[0, 2, 0], [0, 2, 2], [2, 2, 2], [2, 2, 0]], [0, -1, 0], lam)                   # ceiling
+    B.quad([[0, 0, 0], [0, 2, 0], [2, 2, 0], [2, 0, 0]], [0, 0, 1], mix_back)               # back
+    B.quad([[0, 0, 0], [0, 0, 2], [0, 2, 2], [0, 2, 0]], [1, 0, 0], on_a)                   # left
+    B.quad([[2, 0, 0], [2, 2, 0], [2, 2, 2], [2, 0, 2]], [-1, 0, 0], on_b)                  # right
+    for lo, hi, mat in (([0.3, 0.0, 0.4], [0.9, 0.7, 1.0], glass),):
+        v, n, t = box_mesh(lo, hi)
+        B.bundle(v, n, t, mat)
+    B.quad([[1.2, 0.05, 0.3], [1.8, 0.05, 0.3], [1.8, 1.0, 0.3], [1.2, 1.0, 0.3]], [0, 0, 1], mirror)
+    B.quad([[1.1, 0.05, 1.2], [1.7, 0.05, 1.2], [1.7, 0.8, 1.2], [1.1, 0.8, 1.2]], [0, 0, 1], refr)
+    lq = np.asarray([[0.75, 1.98, 0.75], [1.25, 1.98, 0.75], [1.25, 1.98, 1.25], [0.75, 1.98, 1.25]], np.float32)
+    lb = B.bundle(lq, np.tile([0, -1, 0], (4, 1)), [[0, 1, 2], [0, 2, 3]], em)
+    B.area_lights(lb)
+    cam = [1.0, 1.0, 3.5, 1.0, 0.9, 0.0, 0, 1, 0, 0.01, 10000.0, 0.785398, width / height]
+    return B.build(cam, "material_scene", {"rows": {"lambert": lam, "orennayar_a": on_a, "orennayar_b": on_b,
+                                                    "microfacet": ggx, "mirror": mirror, "reflect_fresnel": refl_f,
+                                                    "refract": refr, "glass": glass, "mixed_floor": mix_floor,
+                                                    "mixed_back": mix_back}})

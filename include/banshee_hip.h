@@ -72,13 +72,30 @@ typedef struct
 /* Material row. type 0: SimpleMaterial(Lambert(kd=color)) (bsdf/lambert.h:45-135);
    type 1: SimpleMaterial(Microfacet(eta, ks=color, FresnelDielectric, GgxDistribution(alpha)))
    (bsdf/microfacet.h:53-150, microfacet_distribution.h:196-286, fresnel.h:53-91);
-   type 2: Emissive(e=color) (material/emissive.h:45-99). */
+   type 2: Emissive(e=color) (material/emissive.h:45-99);
+   type 3: SimpleMaterial(OrenNayar(kd=color, kr=alpha)) (bsdf/orennayar.h:47-190);
+   type 4: SimpleMaterial(PerfectReflect(eta, ks=color, pad[0] != 0 ? FresnelDielectric : none))
+   (bsdf/perfect_reflect.h:52-130); type 5: the same with PerfectRefract (perfect_refract.h:44-150);
+   type 6: Glass(eta, ks=color) (material/glass.h:45-110);
+   type 7: MixedMaterial(eta) with AddBsdf of the single-BSDF rows alpha, pad[0], pad[1] in that
+   order (-1: none; rows of types 0, 1, 3, 4, 5; at least one reflecting)
+   (material/mixedmaterial.h:45-174). */
 typedef struct
 {
     float type, color[3], eta, alpha, pad[2];
 } bsh_material;
 
-enum { BSH_LAMBERT = 0, BSH_MICROFACET = 1, BSH_EMISSIVE = 2 };
+enum
+{
+    BSH_LAMBERT = 0,
+    BSH_MICROFACET = 1,
+    BSH_EMISSIVE = 2,
+    BSH_ORENNAYAR = 3,
+    BSH_PERFECT_REFLECT = 4,
+    BSH_PERFECT_REFRACT = 5,
+    BSH_GLASS = 6,
+    BSH_MIXED = 7
+};
 
 /* AreaLight(shapeidx, bundle, material) (light/arealight.cpp:11-23). As in the
    reference, the bundle's light pointer is the LAST AreaLight created on it; the
@@ -488,6 +505,12 @@ bsh_status bsh_diag_keyed(bsh_context* ctx, uint32_t seed, size_t n, const uint3
    sampler BSH_SAMPLER_<kind>(arg); image == 0: the first n samples of a light sampler
    <kind>(1) in camera sample 0. out: host float[n*2]. */
 bsh_status bsh_diag_samples(bsh_context* ctx, int kind, int arg, int image, uint32_t seed, size_t n, float* out);
+/* Device Material::Sample / Evaluate / GetPdf of material row `material` of a committed scene
+   (any type; MixedMaterial reads its component rows), the rand() draws of Glass and
+   MixedMaterial for case i keyed by ks_base(seed, i) from counter 0. Host pointers, as
+   bsh_diag_bsdf: out n x 12 = Sample f(3), pdf, wo(3), type, Evaluate(ea, eb)(3), GetPdf(ea, eb). */
+bsh_status bsh_diag_material(bsh_scene* s, int material, uint32_t seed, size_t n, const float* frames,
+                             const float* view, const float* u, const float* ea, const float* eb, float* out);
 /* Device Light::GetSample / GetPdf / GetLe of light `light` of the light list (area lights, then
    the other lights) at points p (n x 3) with shading normals nrm (n x 3), samples u (n x 2) and
    pdf/GetLe directions w (n x 3); out: n x 16 = le(3), pdf, d(3), GetPdf(w), GetLe(w)(3), 0 x 5.

@@ -50,6 +50,11 @@
 #include "bsdf/lambert.h"
 #include "bsdf/microfacet.h"
 #include "bsdf/fresnel.h"
+#include "bsdf/orennayar.h"
+#include "bsdf/perfect_reflect.h"
+#include "bsdf/perfect_refract.h"
+#include "material/glass.h"
+#include "material/mixedmaterial.h"
 #include "light/arealight.h"
 #include "light/pointlight.h"
 #include "light/directional_light.h"
@@ -361,15 +366,35 @@ struct Scene
     RecordingAccel* accel = nullptr;
 };
 
-Material* make_material(float const* m)
+/* The single BSDF of material rows 0, 1, 3, 4 and 5 (also a MixedMaterial's components). */
+Bsdf* make_bsdf(float const* m)
 {
     int type = (int)m[0];
     float3 c(m[1], m[2], m[3]);
-    if (type == 0) return new SimpleMaterial(new Lambert(g_texsys, c));
-    if (type == 1)
-        return new SimpleMaterial(new Microfacet(g_texsys, m[4], c, "", "", new FresnelDielectric(),
-                                                 new GgxDistribution(m[5])));
-    return new Emissive(c);
+    if (type == 0) return new Lambert(g_texsys, c);
+    if (type == 1) return new Microfacet(g_texsys, m[4], c, "", "", new FresnelDielectric(), new GgxDistribution(m[5]));
+    if (type == 3) return new OrenNayar(g_texsys, c, m[5]);
+    if (type == 4) return new PerfectReflect(g_texsys, m[4], c, "", "", m[6] != 0.f ? new FresnelDielectric() : nullptr);
+    return new PerfectRefract(g_texsys, m[4], c, "", "", m[6] != 0.f ? new FresnelDielectric() : nullptr);
+}
+
+/* Material row (bsh_material): 0 Lambert, 1 Microfacet, 2 Emissive, 3 OrenNayar, 4 PerfectReflect,
+   5 PerfectRefract, 6 Glass, 7 MixedMaterial over the rows m[5], m[6], m[7] (-1: none). */
+Material* make_material(float const* rows, int i)
+{
+    float const* m = rows + 8 * i;
+    int type = (int)m[0];
+    float3 c(m[1], m[2], m[3]);
+    if (type == 2) return new Emissive(c);
+    if (type == 6) return new Glass(g_texsys, m[4], c);
+    if (type == 7)
+    {
+        MixedMaterial* mm = new MixedMaterial(m[4]);
+        for (int k = 5; k < 8; ++k)
+            if ((int)m[k] >= 0) mm->AddBsdf(make_bsdf(rows + 8 * (int)m[k]));
+        return mm;
+    }
+    return new SimpleMaterial(make_bsdf(m));
 }
 
 /* ---------------- the reference's other samplers on the keyed stream ---------------- */
@@ -433,7 +458,7 @@ void* oref_scene_create(float const* verts, int nverts, float const* normals, in
     (void)nverts; (void)nnormals; (void)ntris;
     Scene* s = new Scene();
     for (int i = 0; i < nmaterials; ++i)
-        s->world.materials_.push_back(std::unique_ptr<Material>(make_material(materials + 8 * i)));
+        s->world.materials_.push_back(std::unique_ptr<Material>(make_material(materials, i)));
     for (int b = 0; b < nbundles; ++b)
     {
         int tb = bundles[6 * b], tc = bundles[6 * b + 1], vb = bundles[6 * b + 2], vc = bundles[6 * b + 3];
@@ -672,10 +697,54 @@ double oref_time_render(void* h, int W, int H, int img_grid, int nee_grid, int d
 /* BSDF probes. mat: 8 floats. frames n x 12 (n, ng, dpdu, dpdv); view n x 3 (the
    Sample() "wi" argument); u n x 2; ea/eb n x 3: arguments of Evaluate/GetPdf.
    out n x 12: sample f3, pdf, wo3, type, eval f3, getpdf. */
+/* Material::Sample / Evaluate / GetPdf of world.materials_[midx] (any row type, MixedMaterial
+   with its components), rand() of case i drawn from the keyed stream ks_base(seed, i) from
+   counter 0: out n x 12 as oref_bsdf. */
+void oref_material(void* h, int midx, unsigned seed, int n, float const* frames, float const* view, float const* u,
+                   float const* ea, float const* eb, float* out)
+{
+    Scene* s = (Scene*)h;
+    Material const& m = *s->world.materials_[midx];
+    for (int i = 0; i < n; ++i)
+    {
+        float const* f = frames + 12 * i;
+        ShapeBundle::Hit hit;
+        hit.t = 1.f;
+        hit.p = float3(0, 0, 0);
+        hit.n = float3(f[0], f[1], f[2]);
+        hit.ng = float3(f[3], f[4], f[5]);
+        hit.dpdu = float3(f[6], f[7], f[8]);
+        hit.dpdv = float3(f[9], f[10], f[11]);
+        hit.uv = float2(0, 0);
+        hit.m = midx;
+        hit.bundle = nullptr;
+        g_key = KeyCtx();
+        g_key.on = true;
+        g_key.seed = seed;
+        g_key.base = ks_base(seed, (uint32_t)i);
+        float3 wi(view[3 * i], view[3 * i + 1], view[3 * i + 2]);
+        float3 wo;
+        float pdf = 0.f;
+        int type = 0;
+        ShapeBundle::Hit h1 = hit;
+        float3 fs = m.Sample(h1, float2(u[2 * i], u[2 * i + 1]), wi, wo, pdf, type);
+        g_key.on = false;
+        ShapeBundle::Hit h2 = hit;
+        float3 a(ea[3 * i], ea[3 * i + 1], ea[3 * i + 2]), b(eb[3 * i], eb[3 * i + 1], eb[3 * i + 2]);
+        float3 fe = m.Evaluate(h2, a, b);
+        ShapeBundle::Hit h3 = hit;
+        float pe = m.GetPdf(h3, a, b);
+        float* o = out + 12 * i;
+        o[0] = fs.x; o[1] = fs.y; o[2] = fs.z; o[3] = pdf;
+        o[4] = wo.x; o[5] = wo.y; o[6] = wo.z; o[7] = (float)type;
+        o[8] = fe.x; o[9] = fe.y; o[10] = fe.z; o[11] = pe;
+    }
+}
+
 void oref_bsdf(float const* mat, int n, float const* frames, float const* view, float const* u,
                float const* ea, float const* eb, float* out)
 {
-    std::unique_ptr<Material> m(make_material(mat));
+    std::unique_ptr<Material> m(make_material(mat, 0));
     for (int i = 0; i < n; ++i)
     {
         float const* f = frames + 12 * i;

@@ -60,6 +60,7 @@ def _load(kind):
         if kind == "ref":
             lib.oref_render_keyed_frame.argtypes = [vp, i, i, i, i, i, i, i, ctypes.c_uint, vp, vp, vp, vp, vp]
             lib.oref_scene_set_env.argtypes = [vp, vp, i, i]
+            lib.oref_material.argtypes = [vp, i, ctypes.c_uint, i, vp, vp, vp, vp, vp, vp]
             lib.oref_scene_add_light.argtypes = [vp, i, vp, vp, ctypes.c_float, ctypes.c_float]
             lib.oref_light_full.argtypes = [vp, i, i, vp, vp, vp, vp, vp]
             lib.oref_texture_sample.argtypes = [vp, i, vp, i, i, vp]
@@ -278,6 +279,14 @@ class OracleScene:
         arrs = [f32(a) for a in (p, nrm, u, w)]
         out = np.zeros((arrs[0].shape[0], 16), np.float32)
         self.o.lib.oref_light_full(self.h, idx, arrs[0].shape[0], *[_p(a) for a in arrs], _p(out))
+        return out
+
+    def material(self, idx, seed, frames, view, u, ea, eb):
+        """Reference only: Material::Sample / Evaluate / GetPdf of world.materials_[idx], rand()
+        of case i keyed by ks_base(seed, i): n x 12 (bsh_diag_material's layout)."""
+        arrs = [f32(a) for a in (frames, view, u, ea, eb)]
+        out = np.zeros((arrs[0].shape[0], 12), np.float32)
+        self.o.lib.oref_material(self.h, idx, seed, arrs[0].shape[0], *[_p(a) for a in arrs], _p(out))
         return out
 
     def texture_sample(self, uv, filt=1, wrap=0):
