@@ -182,28 +182,33 @@ static bsh_status up(DevBuf<T>& d, const T* h, size_t n)
 // partition the buffer among themselves, so each XCD reads all of it: a buffer within one XCD's
 // 4 MB L2 is then read from L2 on every pass after the first.
 constexpr int kStreamBlock = 256;
+constexpr int kStreamUnroll = 8;
+typedef float sf4 __attribute__((ext_vector_type(4)));
+template <bool kNT>
 __global__ __launch_bounds__(kStreamBlock) void k_stream_read(const float4* p, size_t n16, int reps, int per_xcd,
                                                               uint32_t* sink)
 {
     const size_t group = per_xcd ? blockIdx.x >> 3 : blockIdx.x;
     const size_t groups = per_xcd ? gridDim.x >> 3 : gridDim.x;
     const size_t stride = groups * kStreamBlock;
+    const sf4* q = (const sf4*)p;
     uint32_t acc = 0;
     for (int r = 0; r < reps; ++r)
     {
         size_t i = group * kStreamBlock + threadIdx.x;
-        for (; i + 3 * stride < n16; i += 4 * stride)
+        for (; i + (kStreamUnroll - 1) * stride < n16; i += kStreamUnroll * stride)
         {
-            const float4 a = gload(p, i), b = gload(p, i + stride), c = gload(p, i + 2 * stride),
-                         d = gload(p, i + 3 * stride);
-            acc ^= __float_as_uint(a.x) ^ __float_as_uint(a.y) ^ __float_as_uint(a.z) ^ __float_as_uint(a.w);
-            acc ^= __float_as_uint(b.x) ^ __float_as_uint(b.y) ^ __float_as_uint(b.z) ^ __float_as_uint(b.w);
-            acc ^= __float_as_uint(c.x) ^ __float_as_uint(c.y) ^ __float_as_uint(c.z) ^ __float_as_uint(c.w);
-            acc ^= __float_as_uint(d.x) ^ __float_as_uint(d.y) ^ __float_as_uint(d.z) ^ __float_as_uint(d.w);
+            sf4 v[kStreamUnroll];
+#pragma unroll
+            for (int k = 0; k < kStreamUnroll; ++k)
+                v[k] = kNT ? __builtin_nontemporal_load(q + i + k * stride) : q[i + k * stride];
+#pragma unroll
+            for (int k = 0; k < kStreamUnroll; ++k)
+                acc ^= __float_as_uint(v[k].x) ^ __float_as_uint(v[k].y) ^ __float_as_uint(v[k].z) ^ __float_as_uint(v[k].w);
         }
         for (; i < n16; i += stride)
         {
-            const float4 a = gload(p, i);
+            const sf4 a = q[i];
             acc ^= __float_as_uint(a.x) ^ __float_as_uint(a.y) ^ __float_as_uint(a.z) ^ __float_as_uint(a.w);
         }
     }
@@ -359,7 +364,10 @@ bsh_status bsh_diag_stream_read(bsh_context* ctx, const void* buf, size_t bytes,
     if (bytes < 16 * kStreamBlock || bytes % 16 || reps < 1)
         return fail(BSH_ERR_INVALID, "bsh_diag_stream_read: bytes must be a multiple of 16 and >= 4 KiB, reps >= 1");
     BSH_HIP(hipSetDevice(ctx->device));
-    const uint32_t blocks = (uint32_t)ctx->num_cus * 8;   // a multiple of the 8 XCDs
+    // 32 workgroups per CU (full occupancy), or 8 with flag 4; a multiple of the 8 XCDs either way
+    const uint32_t blocks = (uint32_t)ctx->num_cus * ((per_xcd & 4) ? 8 : 32);
+    const bool nt = (per_xcd & 2) != 0;
+    per_xcd &= 1;
     DevBuf<uint32_t> sink;
     BSH_HIP(sink.alloc((size_t)blocks * kStreamBlock));
     const size_t n16 = bytes / 16;
@@ -367,11 +375,17 @@ bsh_status bsh_diag_stream_read(bsh_context* ctx, const void* buf, size_t bytes,
     BSH_HIP(hipEventCreate(&e0));
     BSH_HIP(hipEventCreate(&e1));
     // one untimed pass (page mapping, and for cache-resident sizes the first fill), then the timed one
-    hipLaunchKernelGGL(k_stream_read, dim3(blocks), dim3(kStreamBlock), 0, 0, (const float4*)buf, n16, 1, per_xcd,
-                       sink.p);
+    auto launch = [&](int r) {
+        if (nt)
+            hipLaunchKernelGGL(k_stream_read<true>, dim3(blocks), dim3(kStreamBlock), 0, 0, (const float4*)buf, n16, r,
+                               per_xcd, sink.p);
+        else
+            hipLaunchKernelGGL(k_stream_read<false>, dim3(blocks), dim3(kStreamBlock), 0, 0, (const float4*)buf, n16, r,
+                               per_xcd, sink.p);
+    };
+    launch(1);
     (void)hipEventRecord(e0, 0);
-    hipLaunchKernelGGL(k_stream_read, dim3(blocks), dim3(kStreamBlock), 0, 0, (const float4*)buf, n16, reps, per_xcd,
-                       sink.p);
+    launch(reps);
     (void)hipEventRecord(e1, 0);
     hipError_t e = hipEventSynchronize(e1);
     float t = 0.f;

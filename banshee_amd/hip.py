@@ -368,10 +368,13 @@ class Context:
         _check(lib().bsh_context_stats_get(self.h, ctypes.byref(st)), "bsh_context_stats_get")
         return {k: getattr(st, k) for k, _ in st._fields_}
 
-    def stream_read_gbs(self, buf: DeviceArray, reps: int, per_xcd: bool = False) -> float:
-        """GB/s of bsh_diag_stream_read over all of `buf` (per_xcd: every XCD reads all of it)."""
+    def stream_read_gbs(self, buf: DeviceArray, reps: int, per_xcd: bool = False, nontemporal: bool = False,
+                        small_grid: bool = False) -> float:
+        """GB/s of bsh_diag_stream_read over all of `buf` (per_xcd: every XCD reads all of it;
+        nontemporal: non-temporal loads; small_grid: 8 workgroups per CU instead of 32)."""
         ms = ctypes.c_double()
-        _check(lib().bsh_diag_stream_read(self.h, buf.ptr, c_size_t(buf.nbytes), c_int(reps), c_int(int(per_xcd)),
+        flags = int(per_xcd) | (2 if nontemporal else 0) | (4 if small_grid else 0)
+        _check(lib().bsh_diag_stream_read(self.h, buf.ptr, c_size_t(buf.nbytes), c_int(reps), c_int(flags),
                                           ctypes.byref(ms)), "bsh_diag_stream_read")
         moved = buf.nbytes * reps * (8 if per_xcd else 1)
         return moved / (ms.value * 1e-3) / 1e9

@@ -523,7 +523,9 @@ bsh_status bsh_diag_light_full(bsh_scene* s, int light, size_t n, const float* p
    untimed pass, and returns the duration in *ms. per_xcd = 0: every byte is read once per pass
    (HBM for buffers beyond the 256 MB Infinity Cache, the Infinity Cache for buffers inside it);
    per_xcd = 1: each of the 8 XCDs reads the whole buffer every pass (its L2, for buffers that fit
-   one XCD's 4 MB L2), so a pass moves 8 x bytes. */
+   one XCD's 4 MB L2), so a pass moves 8 x bytes. per_xcd | 2: non-temporal loads (the fastest
+   form from HBM); per_xcd | 4: 8 workgroups per CU instead of 32 (the faster form from L2).
+   Eight 16-B loads in flight per lane. */
 bsh_status bsh_diag_stream_read(bsh_context* ctx, const void* buf, size_t bytes, int reps, int per_xcd, double* ms);
 
 #ifdef __cplusplus
