@@ -1015,6 +1015,29 @@ bsh_status bsh_guide_set_params(bsh_guide* g, const float* host)
     return upload_params(g);
 }
 
+bsh_status bsh_guide_get_state(const bsh_guide* g, float* params, float* m1, float* m2, int64_t* step)
+{
+    if (!g || !params || !m1 || !m2 || !step) return fail(BSH_ERR_INVALID, "bsh_guide_get_state: NULL argument");
+    BSH_HIP(hipSetDevice(g->ctx->device));
+    BSH_HIP(hipMemcpy(params, g->p.p, kGParams * 4, hipMemcpyDeviceToHost));
+    BSH_HIP(hipMemcpy(m1, g->m.p, kGParams * 4, hipMemcpyDeviceToHost));
+    BSH_HIP(hipMemcpy(m2, g->v.p, kGParams * 4, hipMemcpyDeviceToHost));
+    *step = g->step;
+    return BSH_OK;
+}
+
+bsh_status bsh_guide_set_state(bsh_guide* g, const float* params, const float* m1, const float* m2, int64_t step)
+{
+    if (!g || !params || !m1 || !m2) return fail(BSH_ERR_INVALID, "bsh_guide_set_state: NULL argument");
+    if (step < 0 || step > (int64_t)1 << 30) return fail(BSH_ERR_INVALID, "bsh_guide_set_state: step out of range");
+    BSH_HIP(hipSetDevice(g->ctx->device));
+    g->host_params.assign(params, params + kGParams);
+    BSH_HIP(hipMemcpy(g->m.p, m1, kGParams * 4, hipMemcpyHostToDevice));
+    BSH_HIP(hipMemcpy(g->v.p, m2, kGParams * 4, hipMemcpyHostToDevice));
+    g->step = (int)step;
+    return upload_params(g);
+}
+
 bsh_status bsh_guide_set_bsdf_fraction(bsh_guide* g, float bsdf_fraction)
 {
     if (!g) return fail(BSH_ERR_INVALID, "bsh_guide_set_bsdf_fraction: NULL guide");

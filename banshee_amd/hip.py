@@ -142,7 +142,7 @@ EXPORTS = [
     "bsh_bake_sh_guided", "bsh_guide_forward_fp32", "bsh_guide_gradient_fp32",
     "bsh_context_launch_stats", "bsh_diag_stream_read", "bsh_import_obj", "bsh_sh_project_env_noalloc",
     "bsh_diag_samples", "bsh_scene_set_other_lights", "bsh_scene_set_env_map", "bsh_diag_light_full",
-    "bsh_diag_material",
+    "bsh_diag_material", "bsh_guide_get_state", "bsh_guide_set_state",
 ]
 
 
@@ -520,6 +520,20 @@ class Guide:
         w = np.ascontiguousarray(w, np.float32)
         assert w.size == self.num_params()
         _check(lib().bsh_guide_set_params(self.h, _p(w)), "bsh_guide_set_params")
+
+    def get_state(self) -> dict:
+        """Training state (bsh_guide_get_state): parameters, Adam moments and step count."""
+        n = self.num_params()
+        p, m1, m2 = (np.zeros(n, np.float32) for _ in range(3))
+        step = ctypes.c_int64()
+        _check(lib().bsh_guide_get_state(self.h, _p(p), _p(m1), _p(m2), ctypes.byref(step)), "bsh_guide_get_state")
+        return {"params": p, "m1": m1, "m2": m2, "step": int(step.value)}
+
+    def set_state(self, st: dict):
+        arrs = [np.ascontiguousarray(st[k], np.float32) for k in ("params", "m1", "m2")]
+        assert all(a.size == self.num_params() for a in arrs)
+        _check(lib().bsh_guide_set_state(self.h, *[_p(a) for a in arrs], ctypes.c_int64(int(st["step"]))),
+               "bsh_guide_set_state")
 
     def forward(self, inputs, fp32=False):
         """inputs: n x 8 raw rows (position in [0,1]^3, roughness, normal, unused) -> n x 48 outputs.

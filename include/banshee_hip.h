@@ -427,6 +427,13 @@ bsh_status bsh_guide_destroy(bsh_guide* g);
    W3, b3, W4, b4, W5[64][48], b5); set resets the Adam state. Host pointers. */
 bsh_status bsh_guide_get_params(const bsh_guide* g, float* host);
 bsh_status bsh_guide_set_params(bsh_guide* g, const float* host);
+/* Training state for checkpoint and resume (no reference counterpart: the reference keeps no
+   state beyond its image plane): the FP32 master parameters, Adam's first and second moments
+   (bsh_guide_num_params floats each) and its step count. set_state restores all of them (and
+   the FP16 copies derived from the parameters), so training resumed from a saved state
+   continues bit for bit as it would have without the interruption. Host pointers. */
+bsh_status bsh_guide_get_state(const bsh_guide* g, float* params, float* m1, float* m2, int64_t* step);
+bsh_status bsh_guide_set_state(bsh_guide* g, const float* params, const float* m1, const float* m2, int64_t step);
 /* The one-sample MIS BSDF fraction used by later guided render and bake calls, in (0, 1]. */
 bsh_status bsh_guide_set_bsdf_fraction(bsh_guide* g, float bsdf_fraction);
 /* Network outputs for n rows of raw inputs (8 floats per row: position in [0,1]^3,

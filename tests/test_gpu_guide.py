@@ -256,3 +256,34 @@ def test_multi_step_train_equals_single_steps(ctx):
         g2.train_step(rec[off::stride][:1024])
     assert np.array_equal(g1.get_params(), g2.get_params())
     assert np.array_equal(g1.get_params(), g3.get_params())
+
+
+def test_checkpoint_resume_is_bit_exact(ctx):
+    """Training interrupted after a 5-step bsh_guide_train call and resumed on a new guide from
+    bsh_guide_get_state ends, after a 7-step call, exactly where the same two calls on one guide
+    end (each call picks its batches by its own step index): parameters, both moments, step count
+    and the FP16 network's outputs. Resuming from bsh_guide_set_params instead (Adam's moments
+    reset) does not, which is what the state adds."""
+    from banshee_amd import hip
+    rec = records(np.random.default_rng(21), 4096)
+    d = ctx.to_device(rec)
+    full = hip.Guide(ctx, seed=9)
+    full.train(d, 4096, 5, 1024)
+    full.train(d, 4096, 7, 1024)
+    first = hip.Guide(ctx, seed=9)
+    first.train(d, 4096, 5, 1024)
+    st = first.get_state()
+    assert st["step"] == 5
+    resumed = hip.Guide(ctx, seed=3)          # other initial weights: the state overrides them
+    resumed.set_state(st)
+    resumed.train(d, 4096, 7, 1024)
+    a, b = full.get_state(), resumed.get_state()
+    assert a["step"] == b["step"] == 12
+    for k in ("params", "m1", "m2"):
+        assert np.array_equal(a[k], b[k]), k
+    x = raw_inputs(np.random.default_rng(2), 512)
+    assert np.array_equal(full.forward(x), resumed.forward(x))
+    naive = hip.Guide(ctx, seed=9)
+    naive.set_params(st["params"])
+    naive.train(d, 4096, 7, 1024)
+    assert not np.array_equal(naive.get_params(), a["params"])
