@@ -245,6 +245,25 @@ bsh_status bsh_scene_set_lights(bsh_scene* s, const bsh_area_light* lights, size
     return BSH_OK;
 }
 
+bsh_status bsh_scene_set_transforms(bsh_scene* s, const float* m_minv, size_t n)
+{
+    if (!s) return fail(BSH_ERR_INVALID, "bsh_scene_set_transforms: NULL scene");
+    if (!m_minv)
+    {
+        s->xforms.clear();
+        s->committed = false;
+        return BSH_OK;
+    }
+    if (n != s->bundles.size())
+        return fail(BSH_ERR_INVALID, "bsh_scene_set_transforms: one transform pair per bundle (" +
+                                         std::to_string(s->bundles.size()) + ")");
+    for (size_t i = 0; i < 32 * n; ++i)
+        if (!std::isfinite(m_minv[i])) return fail(BSH_ERR_INVALID, "bsh_scene_set_transforms: non-finite entry");
+    s->xforms.assign(m_minv, m_minv + 32 * n);
+    s->committed = false;
+    return BSH_OK;
+}
+
 bsh_status bsh_scene_set_other_lights(bsh_scene* s, const bsh_light* lights, size_t n)
 {
     if (!s || (n && !lights)) return fail(BSH_ERR_INVALID, "bsh_scene_set_other_lights: NULL argument");
@@ -283,6 +302,44 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
     if (!s) return fail(BSH_ERR_INVALID, "bsh_scene_commit: NULL scene");
     BSH_HIP(hipSetDevice(s->ctx->device));
     const size_t nt = s->tri_m.size();
+    // bundle transforms, baked into world-space copies: p' = transform_point(p, m), n' =
+    // transform_normal(n, minv) = minv^T n, unnormalised (math/mathutils.h:129-151; FillHit
+    // interpolates and then normalises, mesh.cpp:67-69)
+    std::vector<float> wverts, wnormals;
+    const bool xf = !s->xforms.empty();
+    if (xf)
+    {
+        wverts = s->verts;
+        wnormals = s->normals;
+        for (size_t b = 0; b < s->bundles.size(); ++b)
+        {
+            const float* M = &s->xforms[32 * b];
+            const float* Mi = M + 16;
+            const bsh_bundle& B = s->bundles[b];
+            for (int32_t k = B.vert_begin; k < B.vert_begin + B.vert_count; ++k)
+            {
+                const float* p = &s->verts[3 * (size_t)k];
+                for (int r = 0; r < 3; ++r)
+                {
+                    float acc = 0.f;
+                    for (int c = 0; c < 3; ++c) acc += M[4 * r + c] * p[c];
+                    wverts[3 * (size_t)k + r] = acc + M[4 * r + 3];
+                }
+            }
+            for (int32_t k = B.norm_begin; k < B.norm_begin + B.norm_count; ++k)
+            {
+                const float* n = &s->normals[3 * (size_t)k];
+                for (int r = 0; r < 3; ++r)
+                {
+                    float acc = 0.f;
+                    for (int c = 0; c < 3; ++c) acc += Mi[4 * c + r] * n[c];
+                    wnormals[3 * (size_t)k + r] = acc;
+                }
+            }
+        }
+    }
+    const std::vector<float>& VV = xf ? wverts : s->verts;
+    const std::vector<float>& NN = xf ? wnormals : s->normals;
     // materials referenced by triangles must exist
     for (size_t i = 0; i < nt; ++i)
         if (s->tri_m[i] < 0 || (size_t)s->tri_m[i] >= s->mats.size())
@@ -344,19 +401,25 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
         if ((int)m.type != BSH_EMISSIVE)
             return fail(BSH_ERR_INVALID, "AreaLight: The material passed is not an emissive one");
         bundle_light[L.bundle] = (int32_t)l;  // bundle_.arealight_ = this (last one wins)
-        const float* V = s->verts.data();
+        const float* V = VV.data();
+        const float* V0 = s->verts.data();   // object space
         const int32_t* tv = &s->tri_v[3 * tri];
         f3 p1 = mk(V[3 * tv[0]], V[3 * tv[0] + 1], V[3 * tv[0] + 2]);
         f3 p2 = mk(V[3 * tv[1]], V[3 * tv[1] + 1], V[3 * tv[1] + 2]);
         f3 p3 = mk(V[3 * tv[2]], V[3 * tv[2] + 1], V[3 * tv[2] + 2]);
+        f3 o1 = mk(V0[3 * tv[0]], V0[3 * tv[0] + 1], V0[3 * tv[0] + 2]);
+        f3 o2 = mk(V0[3 * tv[1]], V0[3 * tv[1] + 1], V0[3 * tv[1] + 2]);
+        f3 o3 = mk(V0[3 * tv[2]], V0[3 * tv[2] + 1], V0[3 * tv[2] + 2]);
         LightDev d;
         d.tri = tri;
-        d.area = sqrtf(fabsf(sqnorm(cross(p3 - p1, p3 - p2)))) * 0.5f;  // GetShapeSurfaceArea, mesh.cpp:320
-        d.pdf = 1.f / d.area;                                              // FillSample, mesh.cpp:121
+        // GetShapeSurfaceArea takes the transformed vertices (mesh.cpp:304-321); FillSample's area
+        // pdf the untransformed ones (mesh.cpp:121): they differ under a scaling transform
+        d.area = sqrtf(fabsf(sqnorm(cross(p3 - p1, p3 - p2)))) * 0.5f;
+        d.pdf = 1.f / (sqrtf(fabsf(sqnorm(cross(o3 - o1, o3 - o2)))) * 0.5f);
         d.type = LIGHT_AREA;
         d.e = make_float4(m.color[0], m.color[1], m.color[2], 0.f);
         const int32_t* tn = &s->tri_n[3 * tri];
-        const float* N = s->normals.data();
+        const float* N = NN.data();
         for (int k = 0; k < 3; ++k)
         {
             d.p[k] = make_float4(V[3 * tv[k]], V[3 * tv[k] + 1], V[3 * tv[k] + 2], 0.f);
@@ -421,7 +484,7 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
     std::vector<float> tv(9 * nt);
     for (size_t i = 0; i < nt; ++i)
         for (int k = 0; k < 3; ++k)
-            for (int c = 0; c < 3; ++c) tv[9 * i + 3 * k + c] = s->verts[3 * s->tri_v[3 * i + k] + c];
+            for (int c = 0; c < 3; ++c) tv[9 * i + 3 * k + c] = VV[3 * s->tri_v[3 * i + k] + c];
     BuildParams bp;
     if (params)
     {
@@ -464,12 +527,12 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
             float mw, bw;
             std::memcpy(&mw, &mi, 4);
             std::memcpy(&bw, &bl, 4);
-            r[0] = v4(s->verts, s->tri_v[3 * i], mw);
-            r[1] = v4(s->verts, s->tri_v[3 * i + 1], bw);
-            r[2] = v4(s->verts, s->tri_v[3 * i + 2], 0.f);
-            r[3] = v4(s->normals, s->tri_n[3 * i], 0.f);
-            r[4] = v4(s->normals, s->tri_n[3 * i + 1], 0.f);
-            r[5] = v4(s->normals, s->tri_n[3 * i + 2], 0.f);
+            r[0] = v4(VV, s->tri_v[3 * i], mw);
+            r[1] = v4(VV, s->tri_v[3 * i + 1], bw);
+            r[2] = v4(VV, s->tri_v[3 * i + 2], 0.f);
+            r[3] = v4(NN, s->tri_n[3 * i], 0.f);
+            r[4] = v4(NN, s->tri_n[3 * i + 1], 0.f);
+            r[5] = v4(NN, s->tri_n[3 * i + 2], 0.f);
             r[6] = make_float4(m.type, m.color[0], m.color[1], m.color[2]);
             r[7] = make_float4(m.eta, m.alpha, m.pad[0], m.pad[1]);
         }
@@ -523,11 +586,11 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
         s->box_lo[k] = std::numeric_limits<float>::max();
         s->box_hi[k] = -std::numeric_limits<float>::max();
     }
-    for (size_t i = 0; i < s->verts.size() / 3; ++i)
+    for (size_t i = 0; i < VV.size() / 3; ++i)
         for (int k = 0; k < 3; ++k)
         {
-            s->box_lo[k] = std::min(s->box_lo[k], s->verts[3 * i + k]);
-            s->box_hi[k] = std::max(s->box_hi[k], s->verts[3 * i + k]);
+            s->box_lo[k] = std::min(s->box_lo[k], VV[3 * i + k]);
+            s->box_hi[k] = std::max(s->box_hi[k], VV[3 * i + k]);
         }
     s->nnodes = bo.width == 4 ? bo.nodes4.size() : bo.nodes.size();
     s->width = bo.width;

@@ -185,6 +185,7 @@ struct bsh_scene
     std::vector<bsh_material> mats;
     std::vector<bsh_area_light> lights;
     std::vector<bsh_light> others;        // after the area lights in the light list
+    std::vector<float> xforms;            // per bundle: m then minv, row-major 4x4 (empty: identities)
     std::vector<float> env_tex;           // env_h rows of env_w RGB texels
     int env_w = 0, env_h = 0;
     bsh_camera cam{};

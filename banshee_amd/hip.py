@@ -142,7 +142,7 @@ EXPORTS = [
     "bsh_bake_sh_guided", "bsh_guide_forward_fp32", "bsh_guide_gradient_fp32",
     "bsh_context_launch_stats", "bsh_diag_stream_read", "bsh_import_obj", "bsh_sh_project_env_noalloc",
     "bsh_diag_samples", "bsh_scene_set_other_lights", "bsh_scene_set_env_map", "bsh_diag_light_full",
-    "bsh_diag_material", "bsh_guide_get_state", "bsh_guide_set_state",
+    "bsh_diag_material", "bsh_guide_get_state", "bsh_guide_set_state", "bsh_scene_set_transforms",
 ]
 
 
@@ -640,6 +640,10 @@ class Scene:
         _check(lib().bsh_scene_set_materials(self.h, _p(mats), c_size_t(mats.shape[0])), "bsh_scene_set_materials")
         lights = np.ascontiguousarray(sc.lights, np.int32).reshape(-1, 2)
         _check(lib().bsh_scene_set_lights(self.h, _p(lights), c_size_t(lights.shape[0])), "bsh_scene_set_lights")
+        xf = getattr(sc, "transforms", None)
+        if xf is not None:
+            x = np.ascontiguousarray(xf, np.float32).reshape(-1, 32)
+            _check(lib().bsh_scene_set_transforms(self.h, _p(x), c_size_t(x.shape[0])), "bsh_scene_set_transforms")
         env = getattr(sc, "env", None)
         if env is not None:
             e = np.ascontiguousarray(env, np.float32)

@@ -61,6 +61,8 @@ class Scene:
     # (type, v xyz, e xyz, scale, gamma), and the environment lights' lat-long map [h, w, 3]
     other_lights: np.ndarray | None = None
     env: np.ndarray | None = None
+    # per bundle (ShapeBundle::SetTransform): row-major m then minv, [nbundles, 32]; None: identities
+    transforms: np.ndarray | None = None
 
     @property
     def ntris(self) -> int:
@@ -71,7 +73,7 @@ class Scene:
         for a in (self.verts, self.normals, self.tri_v, self.tri_n, self.tri_m, self.bundles,
                   self.materials, self.lights, self.camera):
             h.update(np.ascontiguousarray(a).tobytes())
-        for a in (self.other_lights, self.env):
+        for a in (self.other_lights, self.env, self.transforms):
             if a is not None:
                 h.update(np.ascontiguousarray(a, np.float32).tobytes())
         return h.hexdigest()[:16]
@@ -612,3 +614,35 @@ def material_scene(width=24, height=24) -> Scene:
                                                     "microfacet": ggx, "mirror": mirror, "reflect_fresnel": refl_f,
                                                     "refract": refr, "glass": glass, "mixed_floor": mix_floor,
                                                     "mixed_back": mix_back}})
+
+
+def transformed_cornell(width=64, height=64) -> Scene:
+    """The Cornell box with its two inner boxes and its light moved by per-bundle transforms
+    (ShapeBundle::SetTransform): a rotated, non-uniformly scaled and translated tall box, a
+    mirrored short box (negative determinant) and a light quad scaled by 1.5 in x (its area pdf
+    then differs between FillSample and GetShapeSurfaceArea)."""
+    sc = cornell_box(width, height)
+    nb = sc.bundles.shape[0]
+    xf = np.tile(np.concatenate([np.eye(4).ravel(), np.eye(4).ravel()]), (nb, 1))
+
+    def pair(m):
+        return np.concatenate([m.ravel(), np.linalg.inv(m).ravel()])
+
+    def rot_y(a):
+        c, s_ = np.cos(a), np.sin(a)
+        return np.array([[c, 0, s_, 0], [0, 1, 0, 0], [-s_, 0, c, 0], [0, 0, 0, 1]])
+
+    def about(m, cx, cz):
+        t = np.eye(4); t[0, 3], t[2, 3] = cx, cz
+        ti = np.eye(4); ti[0, 3], ti[2, 3] = -cx, -cz
+        return t @ m @ ti
+
+    short, tall, light_bundle = 5, 6, int(sc.lights[0, 0])   # cornell_box's bundle order
+    xf[tall] = pair(about(rot_y(0.35) @ np.diag([0.9, 1.1, 1.2, 1.0]), 1.4, 0.6) + np.array(
+        [[0, 0, 0, -0.05], [0, 0, 0, 0], [0, 0, 0, 0.05], [0, 0, 0, 0]]))
+    xf[short] = pair(about(np.diag([-1.0, 1.0, 1.0, 1.0]) @ rot_y(-0.2), 0.6, 1.2))
+    xf[light_bundle] = pair(about(np.diag([1.5, 1.0, 1.0, 1.0]), 1.0, 1.0))
+    out = Scene(**{k: getattr(sc, k) for k in sc.__dataclass_fields__})
+    out.transforms = xf.astype(np.float32)
+    out.name = "transformed_cornell"
+    return out

@@ -226,6 +226,15 @@ bsh_status bsh_import_obj(const char* path, bsh_asset_counts* counts, float* ver
                           bsh_material* materials, bsh_area_light* lights);
 /* World::materials_ */
 bsh_status bsh_scene_set_materials(bsh_scene* s, const bsh_material* mats, size_t n);
+/* ShapeBundle::SetTransform(m, minv) of every bundle (primitive/shapebundle.h:139,261-265):
+   n x 32 floats, row-major m (4x4) then row-major minv, n = the scene's bundle count; NULL
+   restores identities. bsh_scene_commit bakes them into world space as the reference's meshes
+   transform at use: points by transform_point(p, m), normals by transform_normal(n, minv) =
+   minv^T n (math/mathutils.h:129-151); an area light's sample pdf keeps the untransformed
+   triangle's area (mesh.cpp:121) and its GetPdf the transformed one's (mesh.cpp:304-321). The
+   reference intersects in object space (mesh.cpp:216-261); the baked world-space test agrees
+   with it up to rounding. */
+bsh_status bsh_scene_set_transforms(bsh_scene* s, const float* m_minv, size_t n);
 /* World::lights_ (AreaLight only) */
 bsh_status bsh_scene_set_lights(bsh_scene* s, const bsh_area_light* lights, size_t n);
 

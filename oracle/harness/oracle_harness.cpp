@@ -778,6 +778,27 @@ void oref_bsdf(float const* mat, int n, float const* frames, float const* view, 
 
 /* Area light probes on scene light `light`. p n x 3 shading points, u n x 2,
    w n x 3 directions for GetPdf. out n x 8: le3, pdf, d3, getpdf. */
+/* ShapeBundle::SetTransform(m, minv) of every bundle (n x 32 floats: row-major m, then row-major
+   minv), then World::Commit again (the BVH over the transformed bounds) under the recording
+   accelerator. */
+void oref_scene_set_transforms(void* h, float const* xf, int n)
+{
+    Scene* s = (Scene*)h;
+    for (int b = 0; b < n && b < (int)s->meshes.size(); ++b)
+    {
+        float const* m = xf + 32 * b;
+        float const* mi = m + 16;
+        s->meshes[b]->SetTransform(matrix(m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], m[9], m[10], m[11],
+                                          m[12], m[13], m[14], m[15]),
+                                   matrix(mi[0], mi[1], mi[2], mi[3], mi[4], mi[5], mi[6], mi[7], mi[8], mi[9], mi[10],
+                                          mi[11], mi[12], mi[13], mi[14], mi[15]));
+    }
+    s->world.accel_.reset();
+    s->world.Commit();
+    s->accel = new RecordingAccel(std::move(s->world.accel_));
+    s->world.accel_.reset(s->accel);
+}
+
 /* The map the scene's environment lights read ("env"): h rows of w RGB floats. */
 void oref_scene_set_env(void* h, float const* tex, int w, int hgt)
 {

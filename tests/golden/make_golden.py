@@ -47,6 +47,8 @@ SAMPLER_FRAMES = {"regular": (("regular", "regular", "regular"), 4), "cmj": (("c
 # the box with every material beyond Lambert/Microfacet (banshee_amd.scenes.material_scene)
 MATERIAL_SEED = 21
 MATERIAL_FRAME = dict(W=24, grid=4, depth=5, seed=3)
+# the Cornell box with transformed bundles (banshee_amd.scenes.transformed_cornell), whole frame
+TRANSFORMED = dict(W=32, grid=4, depth=5, seed=6)
 # the open scene with every light type (banshee_amd.scenes.open_scene), whole frames
 OPEN_SCENE = dict(W=24, grid=4, depth=4, seed=4)
 TERRAIN_RAYS_CROP = (16, 10, 160, 88)
@@ -283,6 +285,20 @@ def main(only=()):
         np.savez_compressed(os.path.join(HERE, "materials.npz"), digest=sc.digest(), frames=frames, view=view, u=u,
                             ea=ea, eb=eb, probe_seed=MATERIAL_SEED, probes=probes, frame=PS, rays=rays,
                             **{k: v for k, v in M.items()})
+
+    # ---------------- per-bundle transforms (ShapeBundle::SetTransform) ----------------
+    if want("transforms"):
+        from banshee_amd.scenes import transformed_cornell
+        X = TRANSFORMED
+        xs = transformed_cornell(X["W"], X["W"])
+        so = o.scene(xs)
+        so.record(max_rays=30000, stride=7)
+        _, _, _, PS, rays = so.render_keyed_frame(X["W"], X["W"], ("stratified", "stratified", "stratified"),
+                                                  X["grid"], X["depth"], X["seed"], per_sample=True)
+        rr, kind, prim, thit = so.fetch(30000)
+        print(f"transforms: rays={rays} recorded={len(rr)} (closest {(kind == 0).sum()})")
+        np.savez_compressed(os.path.join(HERE, "transforms.npz"), digest=xs.digest(), frame=PS, ray_counts=rays,
+                            rays=rr, kind=kind, prim=prim, thit=thit, **{k: v for k, v in X.items()})
 
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

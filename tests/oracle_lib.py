@@ -61,6 +61,7 @@ def _load(kind):
             lib.oref_render_keyed_frame.argtypes = [vp, i, i, i, i, i, i, i, ctypes.c_uint, vp, vp, vp, vp, vp]
             lib.oref_scene_set_env.argtypes = [vp, vp, i, i]
             lib.oref_material.argtypes = [vp, i, ctypes.c_uint, i, vp, vp, vp, vp, vp, vp]
+            lib.oref_scene_set_transforms.argtypes = [vp, vp, i]
             lib.oref_scene_add_light.argtypes = [vp, i, vp, vp, ctypes.c_float, ctypes.c_float]
             lib.oref_light_full.argtypes = [vp, i, i, vp, vp, vp, vp, vp]
             lib.oref_texture_sample.argtypes = [vp, i, vp, i, i, vp]
@@ -186,6 +187,13 @@ class OracleScene:
             _p(k[5]), k[5].shape[0], _p(k[6]), k[6].shape[0], _p(k[7]), k[7].shape[0], _p(k[8]))
         if not self.h:
             raise RuntimeError("oracle scene creation failed")
+        xf = getattr(sc, "transforms", None)
+        if xf is not None:
+            if orc.kind != "ref":
+                raise RuntimeError("bundle transforms need the reference oracle")
+            x = f32(xf).reshape(-1, 32)
+            self._keep.append(x)
+            orc.lib.oref_scene_set_transforms(self.h, _p(x), x.shape[0])
         others = getattr(sc, "other_lights", None)
         if others is not None and len(others):
             if orc.kind != "ref":
