@@ -245,6 +245,59 @@ bsh_status bsh_scene_set_lights(bsh_scene* s, const bsh_area_light* lights, size
     return BSH_OK;
 }
 
+bsh_status bsh_scene_set_uvs(bsh_scene* s, const float* uvs, size_t nuvs, const int32_t* tri_uv)
+{
+    if (!s) return fail(BSH_ERR_INVALID, "bsh_scene_set_uvs: NULL scene");
+    if (!uvs || !tri_uv)
+    {
+        s->uvs.clear();
+        s->tri_uv.clear();
+        s->committed = false;
+        return BSH_OK;
+    }
+    const size_t nt = s->tri_m.size();
+    for (size_t i = 0; i < 3 * nt; ++i)
+        if (tri_uv[i] < 0 || (size_t)tri_uv[i] >= nuvs)
+            return fail(BSH_ERR_INVALID, "bsh_scene_set_uvs: uv index out of range at triangle " + std::to_string(i / 3));
+    for (size_t i = 0; i < 2 * nuvs; ++i)
+        if (!std::isfinite(uvs[i])) return fail(BSH_ERR_INVALID, "bsh_scene_set_uvs: non-finite coordinate");
+    s->uvs.assign(uvs, uvs + 2 * nuvs);
+    s->tri_uv.assign(tri_uv, tri_uv + 3 * nt);
+    s->committed = false;
+    return BSH_OK;
+}
+
+bsh_status bsh_scene_set_textures(bsh_scene* s, const float* texels, size_t nfloats, const bsh_texture* maps, size_t n)
+{
+    if (!s || (n && (!texels || !maps))) return fail(BSH_ERR_INVALID, "bsh_scene_set_textures: NULL argument");
+    for (size_t k = 0; k < n; ++k)
+    {
+        const bsh_texture& d = maps[k];
+        if (d.width <= 0 || d.height <= 0 || d.offset + 3 * (uint64_t)d.width * (uint64_t)d.height > nfloats)
+            return fail(BSH_ERR_INVALID, "bsh_scene_set_textures: map " + std::to_string(k) + " lies outside the texels");
+    }
+    for (size_t i = 0; i < nfloats; ++i)
+        if (!std::isfinite(texels[i])) return fail(BSH_ERR_INVALID, "bsh_scene_set_textures: non-finite texel");
+    s->tex_texels.assign(texels, texels + nfloats);
+    s->tex_desc.assign(maps, maps + n);
+    s->committed = false;
+    return BSH_OK;
+}
+
+bsh_status bsh_scene_set_material_maps(bsh_scene* s, const int32_t* maps, size_t nrows)
+{
+    if (!s) return fail(BSH_ERR_INVALID, "bsh_scene_set_material_maps: NULL scene");
+    if (!maps)
+    {
+        s->mat_maps.clear();
+        s->committed = false;
+        return BSH_OK;
+    }
+    s->mat_maps.assign(maps, maps + 2 * nrows);
+    s->committed = false;
+    return BSH_OK;
+}
+
 bsh_status bsh_scene_set_transforms(bsh_scene* s, const float* m_minv, size_t n)
 {
     if (!s) return fail(BSH_ERR_INVALID, "bsh_scene_set_transforms: NULL scene");
@@ -386,6 +439,53 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
         }
         BSH_HIP(s->d_mats.alloc(mt.size()));
         BSH_HIP(hipMemcpy(s->d_mats.p, mt.data(), mt.size() * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    // texture coordinates, maps and the rows that read them
+    if (!s->tri_uv.empty() && s->tri_uv.size() != 3 * nt)
+        return fail(BSH_ERR_STATE, "bsh_scene_commit: texture coordinates were set for another triangle count");
+    if (!s->mat_maps.empty())
+    {
+        if (s->mat_maps.size() != 2 * s->mats.size())
+            return fail(BSH_ERR_INVALID, "bsh_scene_commit: material maps need one pair per material row");
+        for (size_t k = 0; k < s->mat_maps.size(); ++k)
+        {
+            const int32_t id = s->mat_maps[k];
+            if (id < -1 || (id >= 0 && (size_t)id >= s->tex_desc.size()))
+                return fail(BSH_ERR_INVALID, "bsh_scene_commit: material " + std::to_string(k / 2) +
+                                                 " names a texture map out of range");
+            const int t = (int)s->mats[k / 2].type;
+            if (id >= 0 && (t == BSH_EMISSIVE || t == BSH_MIXED || (k % 2 == 1 && t != BSH_ORENNAYAR)))
+                return fail(BSH_ERR_INVALID, "bsh_scene_commit: material " + std::to_string(k / 2) +
+                                                 " takes no such map (colour maps: rows 0, 1, 3-6; kr maps: OrenNayar)");
+        }
+    }
+    if (!s->tri_uv.empty() || !s->mat_maps.empty()) s->ext_mats = 1;   // read in the kLights shade variants
+    {
+        std::vector<float4> uv(2 * std::max<size_t>(s->tri_uv.empty() ? 0 : nt, 1), make_float4(0.f, 0.f, 0.f, 0.f));
+        if (!s->tri_uv.empty())
+            for (size_t i = 0; i < nt; ++i)
+            {
+                const float* a = &s->uvs[2 * (size_t)s->tri_uv[3 * i]];
+                const float* b = &s->uvs[2 * (size_t)s->tri_uv[3 * i + 1]];
+                const float* c = &s->uvs[2 * (size_t)s->tri_uv[3 * i + 2]];
+                uv[2 * i] = make_float4(a[0], a[1], b[0], b[1]);
+                uv[2 * i + 1] = make_float4(c[0], c[1], 0.f, 0.f);
+            }
+        BSH_HIP(s->d_tri_uv.alloc(uv.size()));
+        BSH_HIP(hipMemcpy(s->d_tri_uv.p, uv.data(), uv.size() * sizeof(float4), hipMemcpyHostToDevice));
+        BSH_HIP(s->d_tex.alloc(std::max<size_t>(s->tex_texels.size(), 1)));
+        if (!s->tex_texels.empty())
+            BSH_HIP(hipMemcpy(s->d_tex.p, s->tex_texels.data(), s->tex_texels.size() * 4, hipMemcpyHostToDevice));
+        std::vector<int4> td(std::max<size_t>(s->tex_desc.size(), 1), make_int4(0, 0, 0, 0));
+        for (size_t k = 0; k < s->tex_desc.size(); ++k)
+            td[k] = make_int4(s->tex_desc[k].width, s->tex_desc[k].height, (int)(uint32_t)(s->tex_desc[k].offset & 0xffffffffu),
+                              (int)(uint32_t)(s->tex_desc[k].offset >> 32));
+        BSH_HIP(s->d_tex_desc.alloc(td.size()));
+        BSH_HIP(hipMemcpy(s->d_tex_desc.p, td.data(), td.size() * sizeof(int4), hipMemcpyHostToDevice));
+        std::vector<int2> mm(std::max<size_t>(s->mats.size(), 1), make_int2(-1, -1));
+        for (size_t k = 0; k < s->mat_maps.size() / 2; ++k) mm[k] = make_int2(s->mat_maps[2 * k], s->mat_maps[2 * k + 1]);
+        BSH_HIP(s->d_mat_maps.alloc(mm.size()));
+        BSH_HIP(hipMemcpy(s->d_mat_maps.p, mm.data(), mm.size() * sizeof(int2), hipMemcpyHostToDevice));
     }
     // lights: validate exactly as AreaLight's ctor does (arealight.cpp:16-19)
     std::vector<int32_t> bundle_light(s->bundles.size(), -1);

@@ -149,6 +149,28 @@ BSH_HD Frame fill_hit(f3 p1, f3 p2, f3 p3, f3 n1, f3 n2, f3 n3, float a, float b
     return h;
 }
 
+// Mesh::FillHit with texture coordinates t1..t3 (mesh.cpp:72-94, identity transform): the
+// frame from the UV derivatives when their determinant is nonzero, else fill_hit's; writes the
+// interpolated (u, v).
+BSH_HD Frame fill_hit_uv(f3 p1, f3 p2, f3 p3, f3 n1, f3 n2, f3 n3, float a, float b, float u1, float v1, float u2,
+                         float v2, float u3, float v3, float& u, float& v)
+{
+    Frame h = fill_hit(p1, p2, p3, n1, n2, n3, a, b);
+    float du1 = u1 - u3, du2 = u2 - u3, dv1 = v1 - v3, dv2 = v2 - v3;
+    f3 dp1 = p1 - p3, dp2 = p2 - p3;
+    float det = du1 * dv2 - dv1 * du2;
+    if (det != 0.f)
+    {
+        float invdet = 1.f / det;
+        h.dpdu = normalize((dv2 * dp1 - dv1 * dp2) * invdet);
+        h.dpdv = normalize((-du2 * dp1 + du1 * dp2) * invdet);
+    }
+    float w = 1.f - a - b;
+    u = w * u1 + a * u2 + b * u3;
+    v = w * v1 + a * v2 + b * v3;
+    return h;
+}
+
 // ---------------------------------------------------------------------------------
 // Materials. type 0 Lambert, 1 Microfacet+GGX+FresnelDielectric, 2 Emissive.
 // Argument names follow the reference: Sample(hit, u, wi, out wo) samples wo given wi;

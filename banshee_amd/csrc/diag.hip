@@ -106,6 +106,8 @@ __global__ void k_diag_material(SceneDev s, int midx, uint32_t seed, uint32_t n,
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Mat m = load_mat(s.mats, midx);
+    TexCtx tx{s.tex, s.tex_desc, s.mat_maps, 0.f, 0.f};   // the probes' hit.uv is (0, 0)
+    apply_maps(tx, midx, m);
     const float* f = frames + 12 * i;
     Frame h;
     h.p = mk(0.f, 0.f, 0.f);
@@ -118,10 +120,10 @@ __global__ void k_diag_material(SceneDev s, int midx, uint32_t seed, uint32_t n,
     float pdf = 0.f;
     int type = 0;
     RandStream rs{ks_base(seed, i), 0u};
-    f3 fs = mat_sample_x(s.mats, m, h, u[2 * i], u[2 * i + 1], wi, wo, pdf, type, rs);
+    f3 fs = mat_sample_x(s.mats, tx, m, h, u[2 * i], u[2 * i + 1], wi, wo, pdf, type, rs);
     f3 a = mk(ea[3 * i], ea[3 * i + 1], ea[3 * i + 2]), b = mk(eb[3 * i], eb[3 * i + 1], eb[3 * i + 2]);
-    f3 fe = mat_eval_x(s.mats, m, h, a, b);
-    float pe = mat_pdf_x(s.mats, m, h, a, b);
+    f3 fe = mat_eval_x(s.mats, tx, m, h, a, b);
+    float pe = mat_pdf_x(s.mats, tx, m, h, a, b);
     float* o = out + 12 * i;
     o[0] = fs.x; o[1] = fs.y; o[2] = fs.z; o[3] = pdf;
     o[4] = wo.x; o[5] = wo.y; o[6] = wo.z; o[7] = (float)type;

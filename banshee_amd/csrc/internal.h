@@ -186,6 +186,11 @@ struct bsh_scene
     std::vector<bsh_area_light> lights;
     std::vector<bsh_light> others;        // after the area lights in the light list
     std::vector<float> xforms;            // per bundle: m then minv, row-major 4x4 (empty: identities)
+    std::vector<float> uvs;               // texture coordinates, (u, v) pairs
+    std::vector<int32_t> tri_uv;          // per triangle: three indices into uvs (empty: no UVs)
+    std::vector<float> tex_texels;        // texture maps' texels
+    std::vector<bsh_texture> tex_desc;
+    std::vector<int32_t> mat_maps;        // per material row: colour map, kr map (empty: none)
     std::vector<float> env_tex;           // env_h rows of env_w RGB texels
     int env_w = 0, env_h = 0;
     bsh_camera cam{};
@@ -198,6 +203,10 @@ struct bsh_scene
     bsh::DevBuf<float> d_light_cdf;
     bsh::DevBuf<float> d_env;             // map, then the environment distribution's tables
     bsh::DevBuf<float4> d_mats;           // material rows, 2 float4 each
+    bsh::DevBuf<float4> d_tri_uv;
+    bsh::DevBuf<float> d_tex;
+    bsh::DevBuf<int4> d_tex_desc;
+    bsh::DevBuf<int2> d_mat_maps;
     int ext_mats = 0;
     bsh::EnvDev env{};
     std::vector<bsh::LightDev> h_lights;
@@ -222,6 +231,10 @@ struct bsh_scene
         s.narea = (int)lights.size();
         s.mats = d_mats.p;
         s.ext_mats = ext_mats;
+        s.tri_uv = tri_uv.empty() ? nullptr : d_tri_uv.p;
+        s.tex = d_tex.p;
+        s.tex_desc = d_tex_desc.p;
+        s.mat_maps = mat_maps.empty() ? nullptr : d_mat_maps.p;
         s.ntris = (int)tri_v.size() / 3;
         return s;
     }

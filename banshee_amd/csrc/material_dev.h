@@ -7,6 +7,7 @@
 #pragma once
 
 #include "bsh_math.h"
+#include "light_dev.h"
 
 namespace bsh {
 
@@ -22,6 +23,33 @@ enum : int
     MAT_MIXED = 7        // MixedMaterial(eta) over the component rows alpha, x, y (-1: none)
 };
 enum : int { BSDF_TRANSMISSION = 2, BSDF_SPECULAR = 8 };
+
+// Texture maps of material rows (GET_VALUE, bsdf/bsdf.h:42): a row's colour map replaces its kd
+// (Lambert, OrenNayar) or ks (Microfacet, PerfectReflect, PerfectRefract, Glass), its kr map
+// OrenNayar's kr (the map's first channel); each read at the hit's (u, v) with the texture
+// system's default filter, bilinear with periodic wrap (texture/texturesystem.h:35).
+struct TexCtx
+{
+    const float* tex;   // all maps' RGB texels
+    const int4* desc;   // per map: width, height, first float (low and high 32 bits)
+    const int2* maps;   // per material row: colour map, kr map (-1: none); NULL when no row has one
+    float u, v;
+};
+
+BSH_HD f3 tex_sample(const TexCtx& t, int id)
+{
+    const int4 d = t.desc[id];
+    const size_t off = (size_t)(uint32_t)d.z | ((size_t)(uint32_t)d.w << 32);
+    return env_bilinear(t.tex + off, d.x, d.y, t.u, t.v);
+}
+
+BSH_HD void apply_maps(const TexCtx& t, int row, Mat& m)
+{
+    if (!t.maps) return;
+    const int2 mp = t.maps[row];
+    if (mp.x >= 0) m.c = tex_sample(t, mp.x);
+    if (mp.y >= 0) m.alpha = tex_sample(t, mp.y).x;
+}
 
 // The path's keyed rand() stream (glibc rand() and rand_float, math/mathutils.h:59-65).
 struct RandStream
@@ -200,7 +228,7 @@ BSH_HD void mixed_counts(const float4* mats, const Mat& m, int& nr, int& nt)
 }
 
 // The idx-th reflecting (or transmitting) component.
-BSH_HD Mat mixed_pick(const float4* mats, const Mat& m, bool reflecting, int idx)
+BSH_HD Mat mixed_pick(const float4* mats, const TexCtx& tx, const Mat& m, bool reflecting, int idx)
 {
     const float ids[3] = {m.alpha, m.x, m.y};
     int sel = -1, seen = 0;
@@ -216,11 +244,13 @@ BSH_HD Mat mixed_pick(const float4* mats, const Mat& m, bool reflecting, int idx
             ++seen;
         }
     }
-    return load_mat(mats, sel < 0 ? 0 : sel);
+    Mat c = load_mat(mats, sel < 0 ? 0 : sel);
+    apply_maps(tx, sel < 0 ? 0 : sel, c);
+    return c;
 }
 
 // Material::Sample of every material type; rs advances for Glass and MixedMaterial.
-static __device__ __noinline__ f3 mat_sample_x(const float4* mats, const Mat& m, const Frame& h, float ux, float uy, f3 wi, f3& wo, float& pdf,
+static __device__ __noinline__ f3 mat_sample_x(const float4* mats, const TexCtx& tx, const Mat& m, const Frame& h, float ux, float uy, f3 wi, f3& wo, float& pdf,
                        int& type, RandStream& rs)
 {
     if (m.type <= MAT_EMISSIVE) return mat_sample(m, h, ux, uy, wi, wo, pdf, type);
@@ -255,7 +285,7 @@ static __device__ __noinline__ f3 mat_sample_x(const float4* mats, const Mat& m,
         const bool refl = nt == 0 || rnd < reflectance;
         const int cnt = refl ? nr : nt;
         const int idx = (int)(rs.next_uint() % (uint32_t)cnt);
-        const Mat c = mixed_pick(mats, m, refl, idx);
+        const Mat c = mixed_pick(mats, tx, m, refl, idx);
         f3 f = bsdf_sample(c, h, ux, uy, wi, wo, pdf);
         type = bsdf_flags(c.type);
         pdf *= (1.f / (float)cnt);
@@ -266,7 +296,7 @@ static __device__ __noinline__ f3 mat_sample_x(const float4* mats, const Mat& m,
 }
 
 // Material::Evaluate of every material type (Glass: 0, glass.h:87-90).
-static __device__ __noinline__ f3 mat_eval_x(const float4* mats, const Mat& m, const Frame& h, f3 wi, f3 wo)
+static __device__ __noinline__ f3 mat_eval_x(const float4* mats, const TexCtx& tx, const Mat& m, const Frame& h, f3 wi, f3 wo)
 {
     if (m.type <= MAT_EMISSIVE) return mat_eval(m, h, wi, wo);
     if (m.type == MAT_GLASS) return mk(0.f, 0.f, 0.f);
@@ -282,6 +312,7 @@ static __device__ __noinline__ f3 mat_eval_x(const float4* mats, const Mat& m, c
             int id = (int)ids[k];
             if (id < 0) continue;
             Mat c = load_mat(mats, id);
+            apply_maps(tx, id, c);
             if (((bsdf_flags(c.type) & BSDF_REFLECTION) != 0) == refl) f = f + bsdf_eval(c, h, wi, wo);
         }
         return f;
@@ -290,7 +321,7 @@ static __device__ __noinline__ f3 mat_eval_x(const float4* mats, const Mat& m, c
 }
 
 // Material::GetPdf of every material type (Glass: 0, glass.h:82-85).
-static __device__ __noinline__ float mat_pdf_x(const float4* mats, const Mat& m, const Frame& h, f3 wi, f3 wo)
+static __device__ __noinline__ float mat_pdf_x(const float4* mats, const TexCtx& tx, const Mat& m, const Frame& h, f3 wi, f3 wo)
 {
     if (m.type <= MAT_EMISSIVE) return mat_pdf(m, h, wi, wo);
     if (m.type == MAT_GLASS) return 0.f;
@@ -307,6 +338,7 @@ static __device__ __noinline__ float mat_pdf_x(const float4* mats, const Mat& m,
             int id = (int)ids[k];
             if (id < 0) continue;
             Mat c = load_mat(mats, id);
+            apply_maps(tx, id, c);
             if (((bsdf_flags(c.type) & BSDF_REFLECTION) != 0) == refl)
             {
                 pdf += bsdf_pdf(c, h, wi, wo);

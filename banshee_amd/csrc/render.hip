@@ -820,7 +820,18 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                     f3 p1, p2, p3, n1, n2, n3;
                     Mat m;
                     tri_verts(S, hr.x, p1, p2, p3, n1, n2, n3, m);
-                    Frame h = fill_hit(p1, p2, p3, n1, n2, n3, __int_as_float(hr.z), __int_as_float(hr.w));
+                    Frame h;
+                    TexCtx tx{S.tex, S.tex_desc, S.mat_maps, 0.f, 0.f};
+                    if (kLights && S.tri_uv)
+                    {
+                        // texture coordinates: FillHit's UV frame and (u, v) (mesh.cpp:72-94)
+                        const float4 ua = S.tri_uv[2 * (size_t)hr.x], ub = S.tri_uv[2 * (size_t)hr.x + 1];
+                        h = fill_hit_uv(p1, p2, p3, n1, n2, n3, __int_as_float(hr.z), __int_as_float(hr.w), ua.x, ua.y,
+                                        ua.z, ua.w, ub.x, ub.y, tx.u, tx.v);
+                    }
+                    else
+                        h = fill_hit(p1, p2, p3, n1, n2, n3, __int_as_float(hr.z), __int_as_float(hr.w));
+                    if (kLights && S.mat_maps) apply_maps(tx, __float_as_int(S.stri[(size_t)kShadeF4 * hr.x].w), m);
                     f3 wo = -ld3(R.ray_d[i]);
                     uint32_t base = meta.x;
                     f3 T = ld3(R.thr[i]);
@@ -891,14 +902,14 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                 {
                                     f3 wi = normalize(ld);
                                     float dist = sqrtf(sqnorm(ld));
-                                    f3 f = kLights ? mat_eval_x(S.mats, m, h, wi, wo) : mat_eval(m, h, wi, wo);
+                                    f3 f = kLights ? mat_eval_x(S.mats, tx, m, h, wi, wo) : mat_eval(m, h, wi, wo);
                                     float w = 1.f;
                                     if (!singular)
                                     {
                                         // the reference asks GetPdf(light dir, view), the density of the
                                         // view direction; MIS needs the density of sampling the light
                                         // direction from the view, GetPdf(view, light dir)
-                                        float bp = kLights ? (nq ? mat_pdf_x(S.mats, m, h, wo, wi) : mat_pdf_x(S.mats, m, h, wi, wo))
+                                        float bp = kLights ? (nq ? mat_pdf_x(S.mats, tx, m, h, wo, wi) : mat_pdf_x(S.mats, tx, m, h, wi, wo))
                                                            : (nq ? mat_pdf(m, h, wo, wi) : mat_pdf(m, h, wi, wo));
                                         // quirk-free: both MIS weights in solid angle (the reference
                                         // weighs an area light's area pdf against the BSDF's solid-angle
@@ -922,7 +933,7 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                 float bpdf = 0.f;
                                 int type = 0;
                                 f3 fb = singular ? mk(0.f, 0.f, 0.f)
-                                                 : kLights ? mat_sample_x(S.mats, m, h, bx, by, wo, wis, bpdf, type, rs)
+                                                 : kLights ? mat_sample_x(S.mats, tx, m, h, bx, by, wo, wis, bpdf, type, rs)
                                                            : mat_sample(m, h, bx, by, wo, wis, bpdf, type);
                                 f3 wi = normalize(wis);
                                 if (!singular && sqnorm(fb) > 0.f && (nq ? bpdf > 0.f : bpdf > kMinPdf))
@@ -987,12 +998,12 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                 f3 fown = mk(0.f, 0.f, 0.f);
                                 if (own)
                                 {
-                                    fown = mat_sample_x(S.mats, m, h, gx, gy, wo, wi, pdf, type, rs);
+                                    fown = mat_sample_x(S.mats, tx, m, h, gx, gy, wo, wi, pdf, type, rs);
                                     wi = normalize(wi);
                                 }
                                 else if (!guided || usel < alpha)
                                 {
-                                    if (kLights) (void)mat_sample_x(S.mats, m, h, gx, gy, wo, wi, pdf, type, rs);
+                                    if (kLights) (void)mat_sample_x(S.mats, tx, m, h, gx, gy, wo, wi, pdf, type, rs);
                                     else (void)mat_sample(m, h, gx, gy, wo, wi, pdf, type);
                                     wi = normalize(wi);
                                 }
@@ -1007,15 +1018,15 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                     f = fown;
                                 else
                                 {
-                                    f = kLights ? mat_eval_x(S.mats, m, h, wo, wi) : mat_eval(m, h, wo, wi);
-                                    float pb = kLights ? mat_pdf_x(S.mats, m, h, wo, wi) : mat_pdf(m, h, wo, wi);
+                                    f = kLights ? mat_eval_x(S.mats, tx, m, h, wo, wi) : mat_eval(m, h, wo, wi);
+                                    float pb = kLights ? mat_pdf_x(S.mats, tx, m, h, wo, wi) : mat_pdf(m, h, wo, wi);
                                     pdf = alpha * pb + (guided ? (1.f - alpha) * guide_pdf(G, wi) : 0.f);
                                 }
                                 stop = !(pdf > 0.f) || sqnorm(f) == 0.f;
                             }
                             else
                             {
-                                f = kLights ? mat_sample_x(S.mats, m, h, ux, uy, wo, wi, pdf, type, rs)
+                                f = kLights ? mat_sample_x(S.mats, tx, m, h, ux, uy, wo, wi, pdf, type, rs)
                                             : mat_sample(m, h, ux, uy, wo, wi, pdf, type);
                                 stop = sqnorm(f) == 0.f || pdf < kMinPdf;
                             }

@@ -62,6 +62,10 @@ struct SceneDev
     const float* light_cdf;   // per light: cumulative share of the emitted power (area x luminance), last = 1
     EnvDev env;               // environment map of the environment lights (tex NULL when none)
     const float4* mats;       // material rows, 2 float4 each: (type, c), (eta, alpha, x, y)
+    const float4* tri_uv;     // per primitive: (u1, v1, u2, v2), (u3, v3, 0, 0); NULL without UVs
+    const float* tex;         // texture maps' RGB texels
+    const int4* tex_desc;     // per map: width, height, first float (low, high 32 bits)
+    const int2* mat_maps;     // per material row: colour map, kr map (-1: none); NULL when none
     int nlights;
     int narea;                // lights [0, narea) are area lights, the rest point/directional/environment
     int ext_mats;             // 1 when a material row is of type 3-7 (material_dev.h)

@@ -83,6 +83,10 @@ def _set_samplers(p, samplers):
     p.img_sampler, p.light_sampler, p.brdf_sampler = SAMPLERS[img], SAMPLERS[light], SAMPLERS[brdf]
 
 
+class Texture(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("offset", ctypes.c_uint64)]
+
+
 class Light(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32), ("v", ctypes.c_float * 3), ("e", ctypes.c_float * 3),
                 ("scale", ctypes.c_float), ("gamma", ctypes.c_float)]
@@ -143,6 +147,7 @@ EXPORTS = [
     "bsh_context_launch_stats", "bsh_diag_stream_read", "bsh_import_obj", "bsh_sh_project_env_noalloc",
     "bsh_diag_samples", "bsh_scene_set_other_lights", "bsh_scene_set_env_map", "bsh_diag_light_full",
     "bsh_diag_material", "bsh_guide_get_state", "bsh_guide_set_state", "bsh_scene_set_transforms",
+    "bsh_scene_set_uvs", "bsh_scene_set_textures", "bsh_scene_set_material_maps",
 ]
 
 
@@ -640,6 +645,27 @@ class Scene:
         _check(lib().bsh_scene_set_materials(self.h, _p(mats), c_size_t(mats.shape[0])), "bsh_scene_set_materials")
         lights = np.ascontiguousarray(sc.lights, np.int32).reshape(-1, 2)
         _check(lib().bsh_scene_set_lights(self.h, _p(lights), c_size_t(lights.shape[0])), "bsh_scene_set_lights")
+        uv = getattr(sc, "uvs", None)
+        if uv is not None:
+            u = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+            tu = np.ascontiguousarray(sc.tri_uv, np.int32).reshape(-1, 3)
+            _check(lib().bsh_scene_set_uvs(self.h, _p(u), c_size_t(u.shape[0]), _p(tu)), "bsh_scene_set_uvs")
+        texs = getattr(sc, "textures", None)
+        if texs is not None and len(texs):
+            flat = np.concatenate([np.ascontiguousarray(t, np.float32).ravel() for t in texs])
+            descs = (Texture * len(texs))()
+            off = 0
+            for k, t in enumerate(texs):
+                descs[k].height, descs[k].width = int(t.shape[0]), int(t.shape[1])
+                descs[k].offset = off
+                off += t.size
+            _check(lib().bsh_scene_set_textures(self.h, _p(flat), c_size_t(flat.size), descs, c_size_t(len(texs))),
+                   "bsh_scene_set_textures")
+        mm = getattr(sc, "mat_maps", None)
+        if mm is not None:
+            m2 = np.ascontiguousarray(mm, np.int32).reshape(-1, 2)
+            _check(lib().bsh_scene_set_material_maps(self.h, _p(m2), c_size_t(m2.shape[0])),
+                   "bsh_scene_set_material_maps")
         xf = getattr(sc, "transforms", None)
         if xf is not None:
             x = np.ascontiguousarray(xf, np.float32).reshape(-1, 32)

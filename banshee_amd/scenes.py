@@ -63,6 +63,12 @@ class Scene:
     env: np.ndarray | None = None
     # per bundle (ShapeBundle::SetTransform): row-major m then minv, [nbundles, 32]; None: identities
     transforms: np.ndarray | None = None
+    # texture coordinates: (u, v) pairs and three indices per triangle; texture maps [h, w, 3];
+    # per material row (colour map, kr map), -1 for none
+    uvs: np.ndarray | None = None
+    tri_uv: np.ndarray | None = None
+    textures: list | None = None
+    mat_maps: np.ndarray | None = None
 
     @property
     def ntris(self) -> int:
@@ -73,9 +79,11 @@ class Scene:
         for a in (self.verts, self.normals, self.tri_v, self.tri_n, self.tri_m, self.bundles,
                   self.materials, self.lights, self.camera):
             h.update(np.ascontiguousarray(a).tobytes())
-        for a in (self.other_lights, self.env, self.transforms):
+        for a in (self.other_lights, self.env, self.transforms, self.uvs, self.tri_uv, self.mat_maps):
             if a is not None:
                 h.update(np.ascontiguousarray(a, np.float32).tobytes())
+        for t in self.textures or ():
+            h.update(np.ascontiguousarray(t, np.float32).tobytes())
         return h.hexdigest()[:16]
 
     def with_camera(self, camera) -> "Scene":
@@ -645,4 +653,41 @@ def transformed_cornell(width=64, height=64) -> Scene:
     out = Scene(**{k: getattr(sc, k) for k in sc.__dataclass_fields__})
     out.transforms = xf.astype(np.float32)
     out.name = "transformed_cornell"
+    return out
+
+
+def textured_cornell(width=32, height=32, seed=7) -> Scene:
+    """The Cornell box with texture coordinates on every face (each quad (0,0)-(2,2), so maps
+    wrap) and texture maps: a checkerboard kd on the floor (Lambert), a ks gradient on the back
+    wall (Microfacet), kd and kr maps on the left wall (OrenNayar), and a ks map on a glass short
+    box. UVs also give the hit frames FillHit's UV derivatives."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sc = cornell_box(width, height)
+    nv = sc.verts.shape[0]
+    quad_uv = np.array([[0, 0], [2, 0], [2, 2], [0, 2]], np.float32)
+    uvs = np.tile(quad_uv, (nv // 4, 1))
+    assert uvs.shape[0] == nv                    # every bundle is a run of 4-vertex quads
+    checker = np.where(((np.arange(16)[:, None] // 4 + np.arange(16)[None, :] // 4) % 2 == 0)[..., None],
+                       np.array([0.8, 0.75, 0.7]), np.array([0.15, 0.2, 0.3])).astype(np.float32)
+    gy, gx = np.meshgrid(np.linspace(0, 1, 4), np.linspace(0, 1, 8), indexing="ij")
+    gradient = np.stack([0.2 + 0.6 * gx, 0.7 - 0.4 * gy, 0.3 + 0.2 * gx * gy], -1).astype(np.float32)
+    noise = rng.uniform(0.2, 0.9, (12, 12, 3)).astype(np.float32)
+    rough = rng.uniform(0.1, 1.4, (6, 6, 3)).astype(np.float32)
+    out = Scene(**{k: getattr(sc, k) for k in sc.__dataclass_fields__})
+    mats = sc.materials.copy()
+    # cornell_box's rows: walls 0-4 (floor, ceiling, back, left, right), boxes 5-6, light 7
+    mats[2] = [MICROFACET, 0.6, 0.6, 0.6, 1.5, 0.3, 0, 0]
+    mats[3] = [ORENNAYAR, 0.5, 0.5, 0.5, 0, 0.7, 0, 0]
+    mats[5] = [GLASS, 0.9, 0.9, 0.9, 1.5, 0, 0, 0]
+    maps = -np.ones((mats.shape[0], 2), np.int32)
+    maps[0, 0] = 0          # floor kd: checkerboard
+    maps[2, 0] = 1          # back wall ks: gradient
+    maps[3] = (2, 3)        # left wall kd and kr
+    maps[5, 0] = 2          # glass ks
+    out.materials = mats
+    out.uvs = uvs
+    out.tri_uv = sc.tri_v.copy()
+    out.textures = [checker, gradient, noise, rough]
+    out.mat_maps = maps
+    out.name = "textured_cornell"
     return out

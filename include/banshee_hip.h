@@ -235,6 +235,28 @@ bsh_status bsh_scene_set_materials(bsh_scene* s, const bsh_material* mats, size_
    reference intersects in object space (mesh.cpp:216-261); the baked world-space test agrees
    with it up to rounding. */
 bsh_status bsh_scene_set_transforms(bsh_scene* s, const float* m_minv, size_t n);
+/* Texture coordinates (Mesh's uvs_ and Face::ti0..ti2, primitive/mesh.h:57-63): nuvs (u, v) pairs
+   and three indices into them per triangle, in triangle order. A hit then gets FillHit's UV frame
+   and coordinates (mesh.cpp:72-94); without them its uv is (0, 0) and its frame the orthovector
+   one. With bundle transforms, the UV frame is taken from the world-space triangle (the
+   reference maps its object-space tangents by minv^T, the same only for rotations). */
+bsh_status bsh_scene_set_uvs(bsh_scene* s, const float* uvs, size_t nuvs, const int32_t* tri_uv);
+/* Texture maps (what the BSDF constructors name as kdmap / ksmap / krmap): n maps of width x
+   height RGB texels, row 0 at v = 0, each starting at float `offset` of `texels` (nfloats
+   floats); read at a hit's (u, v) with bilinear filtering and periodic wrap (the texture system's
+   defaults, texture/texturesystem.h:35). Host pointers, copied. */
+typedef struct
+{
+    int32_t width, height;
+    uint64_t offset;
+} bsh_texture;
+bsh_status bsh_scene_set_textures(bsh_scene* s, const float* texels, size_t nfloats, const bsh_texture* maps, size_t n);
+/* Per material row: {colour map, kr map}, map indices or -1. The colour map replaces the row's
+   kd (Lambert, OrenNayar) or ks (Microfacet, PerfectReflect, PerfectRefract, Glass; GET_VALUE,
+   bsdf/bsdf.h:42); the kr map OrenNayar's kr (its first channel, orennayar.h:113). Emissive and
+   MixedMaterial rows take none (a mixture's components carry their own). Normal maps
+   (bsdf/normal_mapping.h) are not supported. */
+bsh_status bsh_scene_set_material_maps(bsh_scene* s, const int32_t* maps, size_t nrows);
 /* World::lights_ (AreaLight only) */
 bsh_status bsh_scene_set_lights(bsh_scene* s, const bsh_area_light* lights, size_t n);
 

@@ -244,8 +244,9 @@ class RecordingMesh : public Mesh
 {
 public:
     RecordingMesh(float const* v, int nv, float const* n, int nn, int const* vi, int const* ni,
-                  int const* mi, int nf, int global_start)
-        : Mesh(v, nv, 0, n, nn, 0, nullptr, 0, 0, vi, 0, ni, 0, nullptr, 0, mi, (int)sizeof(int), nf)
+                  int const* mi, int nf, int global_start, float const* uv = nullptr, int nuv = 0,
+                  int const* ui = nullptr)
+        : Mesh(v, nv, 0, n, nn, 0, uv, nuv, 0, vi, 0, ni, 0, ui, 0, mi, (int)sizeof(int), nf)
         , start_(global_start)
     {
     }
@@ -366,35 +367,52 @@ struct Scene
     RecordingAccel* accel = nullptr;
 };
 
-/* The single BSDF of material rows 0, 1, 3, 4 and 5 (also a MixedMaterial's components). */
-Bsdf* make_bsdf(float const* m)
+/* The texture map names of material row i: maps[2 i] (colour) and maps[2 i + 1] (kr) name
+   "map<k>" of the scene's texture system, or nothing. */
+struct RowMaps
 {
+    std::string color, kr;
+};
+RowMaps row_maps(int const* maps, int i)
+{
+    RowMaps r;
+    if (maps && maps[2 * i] >= 0) r.color = "map" + std::to_string(maps[2 * i]);
+    if (maps && maps[2 * i + 1] >= 0) r.kr = "map" + std::to_string(maps[2 * i + 1]);
+    return r;
+}
+
+/* The single BSDF of material rows 0, 1, 3, 4 and 5 (also a MixedMaterial's components), with
+   its colour (kd / ks) and kr maps. */
+Bsdf* make_bsdf(TextureSystem const& ts, float const* rows, int i, int const* maps)
+{
+    float const* m = rows + 8 * i;
     int type = (int)m[0];
     float3 c(m[1], m[2], m[3]);
-    if (type == 0) return new Lambert(g_texsys, c);
-    if (type == 1) return new Microfacet(g_texsys, m[4], c, "", "", new FresnelDielectric(), new GgxDistribution(m[5]));
-    if (type == 3) return new OrenNayar(g_texsys, c, m[5]);
-    if (type == 4) return new PerfectReflect(g_texsys, m[4], c, "", "", m[6] != 0.f ? new FresnelDielectric() : nullptr);
-    return new PerfectRefract(g_texsys, m[4], c, "", "", m[6] != 0.f ? new FresnelDielectric() : nullptr);
+    RowMaps r = row_maps(maps, i);
+    if (type == 0) return new Lambert(ts, c, r.color);
+    if (type == 1) return new Microfacet(ts, m[4], c, r.color, "", new FresnelDielectric(), new GgxDistribution(m[5]));
+    if (type == 3) return new OrenNayar(ts, c, m[5], r.color, r.kr);
+    if (type == 4) return new PerfectReflect(ts, m[4], c, r.color, "", m[6] != 0.f ? new FresnelDielectric() : nullptr);
+    return new PerfectRefract(ts, m[4], c, r.color, "", m[6] != 0.f ? new FresnelDielectric() : nullptr);
 }
 
 /* Material row (bsh_material): 0 Lambert, 1 Microfacet, 2 Emissive, 3 OrenNayar, 4 PerfectReflect,
    5 PerfectRefract, 6 Glass, 7 MixedMaterial over the rows m[5], m[6], m[7] (-1: none). */
-Material* make_material(float const* rows, int i)
+Material* make_material(float const* rows, int i, TextureSystem const& ts = g_texsys, int const* maps = nullptr)
 {
     float const* m = rows + 8 * i;
     int type = (int)m[0];
     float3 c(m[1], m[2], m[3]);
     if (type == 2) return new Emissive(c);
-    if (type == 6) return new Glass(g_texsys, m[4], c);
+    if (type == 6) return new Glass(ts, m[4], c, row_maps(maps, i).color);
     if (type == 7)
     {
         MixedMaterial* mm = new MixedMaterial(m[4]);
         for (int k = 5; k < 8; ++k)
-            if ((int)m[k] >= 0) mm->AddBsdf(make_bsdf(rows + 8 * (int)m[k]));
+            if ((int)m[k] >= 0) mm->AddBsdf(make_bsdf(ts, rows, (int)m[k], maps));
         return mm;
     }
-    return new SimpleMaterial(make_bsdf(m));
+    return new SimpleMaterial(make_bsdf(ts, rows, i, maps));
 }
 
 /* ---------------- the reference's other samplers on the keyed stream ---------------- */
@@ -450,15 +468,28 @@ int oref_sample2d_first_is_x(void)
     return v.x == 0.25f ? 1 : 0;
 }
 
-void* oref_scene_create(float const* verts, int nverts, float const* normals, int nnormals,
-                        int const* tri_v, int const* tri_n, int const* tri_m, int ntris,
-                        int const* bundles, int nbundles, float const* materials, int nmaterials,
-                        int const* lights, int nlights, float const* cam)
+/* oref_scene_create with texture coordinates (nuvs (u, v) pairs, three indices per triangle),
+   texture maps (ntex maps: tex_desc = width, height, first float of texels, each) named "map<k>"
+   in the scene's texture system, and per material row its colour and kr map (-1: none). */
+void* oref_scene_create_textured(float const* verts, int nverts, float const* normals, int nnormals,
+                                 int const* tri_v, int const* tri_n, int const* tri_m, int ntris,
+                                 int const* bundles, int nbundles, float const* materials, int nmaterials,
+                                 int const* lights, int nlights, float const* cam, float const* uvs, int nuvs,
+                                 int const* tri_uv, float const* texels, int const* tex_desc, int ntex,
+                                 int const* mat_maps)
 {
     (void)nverts; (void)nnormals; (void)ntris;
     Scene* s = new Scene();
+    for (int k = 0; k < ntex; ++k)
+    {
+        MapTextureSystem::Map mp;
+        mp.w = tex_desc[3 * k];
+        mp.h = tex_desc[3 * k + 1];
+        mp.t.assign(texels + tex_desc[3 * k + 2], texels + tex_desc[3 * k + 2] + 3 * (size_t)mp.w * mp.h);
+        s->texsys.maps["map" + std::to_string(k)] = mp;
+    }
     for (int i = 0; i < nmaterials; ++i)
-        s->world.materials_.push_back(std::unique_ptr<Material>(make_material(materials, i)));
+        s->world.materials_.push_back(std::unique_ptr<Material>(make_material(materials, i, s->texsys, mat_maps)));
     for (int b = 0; b < nbundles; ++b)
     {
         int tb = bundles[6 * b], tc = bundles[6 * b + 1], vb = bundles[6 * b + 2], vc = bundles[6 * b + 3];
@@ -470,7 +501,7 @@ void* oref_scene_create(float const* verts, int nverts, float const* normals, in
             ni[k] = tri_n[3 * tb + k] - nb;
         }
         RecordingMesh* m = new RecordingMesh(verts + 3 * vb, vc, normals + 3 * nb, nc, vi.data(), ni.data(),
-                                             tri_m + tb, tc, tb);
+                                             tri_m + tb, tc, tb, uvs, uvs ? nuvs : 0, tri_uv ? tri_uv + 3 * tb : nullptr);
         s->meshes.push_back(m);
         s->world.shapebundles_.push_back(std::unique_ptr<ShapeBundle>(m));
     }
@@ -488,6 +519,16 @@ void* oref_scene_create(float const* verts, int nverts, float const* normals, in
     s->accel = new RecordingAccel(std::move(s->world.accel_));
     s->world.accel_.reset(s->accel);
     return s;
+}
+
+void* oref_scene_create(float const* verts, int nverts, float const* normals, int nnormals,
+                        int const* tri_v, int const* tri_n, int const* tri_m, int ntris,
+                        int const* bundles, int nbundles, float const* materials, int nmaterials,
+                        int const* lights, int nlights, float const* cam)
+{
+    return oref_scene_create_textured(verts, nverts, normals, nnormals, tri_v, tri_n, tri_m, ntris, bundles, nbundles,
+                                      materials, nmaterials, lights, nlights, cam, nullptr, 0, nullptr, nullptr, nullptr,
+                                      0, nullptr);
 }
 
 void oref_scene_destroy(void* h) { delete (Scene*)h; }

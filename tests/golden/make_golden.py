@@ -47,6 +47,8 @@ SAMPLER_FRAMES = {"regular": (("regular", "regular", "regular"), 4), "cmj": (("c
 # the box with every material beyond Lambert/Microfacet (banshee_amd.scenes.material_scene)
 MATERIAL_SEED = 21
 MATERIAL_FRAME = dict(W=24, grid=4, depth=5, seed=3)
+# the Cornell box with texture coordinates and maps (banshee_amd.scenes.textured_cornell)
+TEXTURED = dict(W=32, grid=4, depth=5, seed=8)
 # the Cornell box with transformed bundles (banshee_amd.scenes.transformed_cornell), whole frame
 TRANSFORMED = dict(W=32, grid=4, depth=5, seed=6)
 # the open scene with every light type (banshee_amd.scenes.open_scene), whole frames
@@ -299,6 +301,18 @@ def main(only=()):
         print(f"transforms: rays={rays} recorded={len(rr)} (closest {(kind == 0).sum()})")
         np.savez_compressed(os.path.join(HERE, "transforms.npz"), digest=xs.digest(), frame=PS, ray_counts=rays,
                             rays=rr, kind=kind, prim=prim, thit=thit, **{k: v for k, v in X.items()})
+
+    # ---------------- texture coordinates and maps (GET_VALUE over the texture system) ----------------
+    if want("textures"):
+        from banshee_amd.scenes import textured_cornell
+        X = TEXTURED
+        xs = textured_cornell(X["W"], X["W"])
+        so = o.scene(xs)
+        _, _, _, PS, rays = so.render_keyed_frame(X["W"], X["W"], ("stratified", "stratified", "stratified"),
+                                                  X["grid"], X["depth"], X["seed"], per_sample=True)
+        print(f"textures: rays={rays}")
+        np.savez_compressed(os.path.join(HERE, "textures.npz"), digest=xs.digest(), frame=PS, ray_counts=rays,
+                            **{k: v for k, v in X.items()})
 
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):

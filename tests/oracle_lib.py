@@ -59,6 +59,9 @@ def _load(kind):
             lib.cref_render_bsdf_only.argtypes = [vp, i, i, i, i, i, i, i, i, ctypes.c_uint, vp, vp, vp, vp, i]
         if kind == "ref":
             lib.oref_render_keyed_frame.argtypes = [vp, i, i, i, i, i, i, i, ctypes.c_uint, vp, vp, vp, vp, vp]
+            lib.oref_scene_create_textured.restype = vp
+            lib.oref_scene_create_textured.argtypes = [vp, i, vp, i, vp, vp, vp, i, vp, i, vp, i, vp, i, vp,
+                                                       vp, i, vp, vp, vp, i, vp]
             lib.oref_scene_set_env.argtypes = [vp, vp, i, i]
             lib.oref_material.argtypes = [vp, i, ctypes.c_uint, i, vp, vp, vp, vp, vp, vp]
             lib.oref_scene_set_transforms.argtypes = [vp, vp, i]
@@ -182,9 +185,29 @@ class OracleScene:
         self._keep = [f32(sc.verts), f32(sc.normals), i32(sc.tri_v), i32(sc.tri_n), i32(sc.tri_m),
                       i32(sc.bundles), f32(sc.materials), i32(sc.lights), f32(sc.camera)]
         k = self._keep
-        self.h = orc._fn("scene_create")(
-            _p(k[0]), k[0].shape[0], _p(k[1]), k[1].shape[0], _p(k[2]), _p(k[3]), _p(k[4]), k[2].shape[0],
-            _p(k[5]), k[5].shape[0], _p(k[6]), k[6].shape[0], _p(k[7]), k[7].shape[0], _p(k[8]))
+        textured = any(getattr(sc, a, None) is not None for a in ("uvs", "textures", "mat_maps"))
+        if textured:
+            if orc.kind != "ref":
+                raise RuntimeError("texture coordinates and maps need the reference oracle")
+            uv = f32(sc.uvs).reshape(-1, 2) if sc.uvs is not None else None
+            tu = i32(sc.tri_uv).reshape(-1, 3) if sc.tri_uv is not None else None
+            texs = sc.textures or []
+            flat = f32(np.concatenate([np.asarray(t, np.float32).ravel() for t in texs])) if texs else None
+            desc, off = [], 0
+            for t in texs:
+                desc += [t.shape[1], t.shape[0], off]
+                off += t.size
+            desc = i32(desc) if texs else None
+            mm = i32(sc.mat_maps).reshape(-1, 2) if sc.mat_maps is not None else None
+            self._keep += [a for a in (uv, tu, flat, desc, mm) if a is not None]
+            self.h = orc.lib.oref_scene_create_textured(
+                _p(k[0]), k[0].shape[0], _p(k[1]), k[1].shape[0], _p(k[2]), _p(k[3]), _p(k[4]), k[2].shape[0],
+                _p(k[5]), k[5].shape[0], _p(k[6]), k[6].shape[0], _p(k[7]), k[7].shape[0], _p(k[8]),
+                _p(uv), 0 if uv is None else uv.shape[0], _p(tu), _p(flat), _p(desc), len(texs), _p(mm))
+        else:
+            self.h = orc._fn("scene_create")(
+                _p(k[0]), k[0].shape[0], _p(k[1]), k[1].shape[0], _p(k[2]), _p(k[3]), _p(k[4]), k[2].shape[0],
+                _p(k[5]), k[5].shape[0], _p(k[6]), k[6].shape[0], _p(k[7]), k[7].shape[0], _p(k[8]))
         if not self.h:
             raise RuntimeError("oracle scene creation failed")
         xf = getattr(sc, "transforms", None)
