@@ -37,7 +37,7 @@ constexpr int kGW5 = kGB4 + kGHid, kGB5 = kGW5 + kGHid * kGOut;
 constexpr int kGParams = kGB5 + kGOut;
 
 // The five always-zero encoded features (raw slots past v6).
-BSH_HD inline bool guide_pad_feature(int j)
+BSH_HD bool guide_pad_feature(int j)
 {
     int b = j / 16, p = j % 16;
     return p >= 14 && 2 * b + (p - 14) >= kGRaw;

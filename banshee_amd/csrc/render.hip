@@ -573,33 +573,48 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
             hit[c] = (tn[c] <= tf) && (cn[c] >= 0);
         }
         bool done = false;
+        // The hit leaf children's triangles, in child order, as one run of ranges [q, qe),
+        // then [r1, e1) ... (empty ranges are (0, 0)): the wave loops as often as its lane with
+        // the most triangles, instead of once per child slot for the slot's longest leaf.
+        int q = 0, qe = 0, r1 = 0, e1 = 0, r2 = 0, e2 = 0, r3 = 0, e3 = 0;
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
+        for (int c = 3; c >= 0; --c)
         {
-            if (!done && hit[c] && cn[c] > 0)
+            if (hit[c] && cn[c] > 0)
             {
-                for (int q = ref[c]; q < ref[c] + cn[c]; ++q)
-                {
-                    const float4 v0 = gload(tris, 3 * (size_t)q + 0);
-                    const float4 e1 = gload(tris, 3 * (size_t)q + 1);
-                    const float4 e2 = gload(tris, 3 * (size_t)q + 2);
-                    if (kCount) (any ? vis_tris_any : vis_tris)++;
-                    float t, u, v;
-                    if (intersect_face(ld3(v0), ld3(e1), ld3(e2), o, d, tmin, tcur, t, u, v))
-                    {
-                        prim = __builtin_bit_cast(int, v0.w);
-                        tcur = t;
-                        hb1 = u;
-                        hb2 = v;
-                        if (any)
-                        {
-                            done = true;
-                            break;
-                        }
-                    }
-                }
+                r3 = r2; e3 = e2;
+                r2 = r1; e2 = e1;
+                r1 = q; e1 = qe;
+                q = ref[c]; qe = ref[c] + cn[c];
             }
             if (cn[c] > 0) hit[c] = false;
+        }
+        while (q < qe)
+        {
+            const float4 v0 = gload(tris, 3 * (size_t)q + 0);
+            const float4 a1 = gload(tris, 3 * (size_t)q + 1);
+            const float4 a2 = gload(tris, 3 * (size_t)q + 2);
+            if (kCount) (any ? vis_tris_any : vis_tris)++;
+            float t, u, v;
+            if (intersect_face(ld3(v0), ld3(a1), ld3(a2), o, d, tmin, tcur, t, u, v))
+            {
+                prim = __builtin_bit_cast(int, v0.w);
+                tcur = t;
+                hb1 = u;
+                hb2 = v;
+                if (any)
+                {
+                    done = true;
+                    break;
+                }
+            }
+            if (++q == qe)
+            {
+                q = r1; qe = e1;
+                r1 = r2; e1 = e2;
+                r2 = r3; e2 = e3;
+                r3 = 0; e3 = 0;
+            }
         }
         if (!done)
         {
