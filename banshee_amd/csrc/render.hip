@@ -471,6 +471,12 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
     float ix = 0.f, iy = 0.f, iz = 0.f, ox = 0.f, oy = 0.f, oz = 0.f, tmin = 0.f, tcur = 0.f, hb1 = 0.f, hb2 = 0.f;
     int prim = -1, sp = 0;
     uint32_t node = 0, own = 0;
+    // pending leaf triangles [tb, te), [r1, e1), [r2, e2), [r3, e3) and the last visit's internal
+    // children (key = entry distance, kInf if missed), not yet culled and pushed while !sorted
+    int tb = 0, te = 0, r1 = 0, e1 = 0, r2 = 0, e2 = 0, r3 = 0, e3 = 0;
+    float key[4] = {0.f, 0.f, 0.f, 0.f};
+    uint32_t id[4] = {0u, 0u, 0u, 0u};
+    bool sorted = true;
     const float4* nodes = nullptr;
     const float4* tris = nullptr;
     for (;;)
@@ -534,6 +540,8 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
                     hb1 = hb2 = 0.f;
                     node = 0;
                     sp = 0;
+                    tb = te = r1 = e1 = r2 = e2 = r3 = e3 = 0;
+                    sorted = true;
                     active = true;
                 }
             }
@@ -549,51 +557,17 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
         }
         if (__ballot(active) == 0ull) break;
         if (!active) continue;
-        // ---- one node visit (traverse4's loop body) ----
-        if (kCount) (any ? vis_nodes_any : vis_nodes)++;
-        const size_t nb = 8 * (size_t)node;
-        const float4 LX = gload(nodes, nb), HX = gload(nodes, nb + 1), LY = gload(nodes, nb + 2),
-                     HY = gload(nodes, nb + 3), LZ = gload(nodes, nb + 4), HZ = gload(nodes, nb + 5);
-        const int4 REF = gload_i(nodes, nb + 6);
-        const int4 CNT = gload_i(nodes, nb + 7);
-        const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, hx[4] = {HX.x, HX.y, HX.z, HX.w};
-        const float ly[4] = {LY.x, LY.y, LY.z, LY.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w};
-        const float lz[4] = {LZ.x, LZ.y, LZ.z, LZ.w}, hz[4] = {HZ.x, HZ.y, HZ.z, HZ.w};
-        const int ref[4] = {REF.x, REF.y, REF.z, REF.w}, cn[4] = {CNT.x, CNT.y, CNT.z, CNT.w};
-        float tn[4];
-        bool hit[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-        {
-            float x0 = __builtin_fmaf(lx[c], ix, -ox), x1 = __builtin_fmaf(hx[c], ix, -ox);
-            float y0 = __builtin_fmaf(ly[c], iy, -oy), y1 = __builtin_fmaf(hy[c], iy, -oy);
-            float z0 = __builtin_fmaf(lz[c], iz, -oz), z1 = __builtin_fmaf(hz[c], iz, -oz);
-            tn[c] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tcur));
-            hit[c] = (tn[c] <= tf) && (cn[c] >= 0);
-        }
+        // ---- one step: a triangle of the pending leaves, or a node visit ----
+        // A visit's leaf triangles are tested one per iteration, before its internal children
+        // are culled against the updated tcur, sorted and pushed (the order traverse4 tests and
+        // culls in), so a wave does either kind of work per lane per iteration instead of
+        // looping over its busiest lane's triangles while the other lanes wait.
         bool done = false;
-        // The hit leaf children's triangles, in child order, as one run of ranges [q, qe),
-        // then [r1, e1) ... (empty ranges are (0, 0)): the wave loops as often as its lane with
-        // the most triangles, instead of once per child slot for the slot's longest leaf.
-        int q = 0, qe = 0, r1 = 0, e1 = 0, r2 = 0, e2 = 0, r3 = 0, e3 = 0;
-#pragma unroll
-        for (int c = 3; c >= 0; --c)
+        if (tb < te)
         {
-            if (hit[c] && cn[c] > 0)
-            {
-                r3 = r2; e3 = e2;
-                r2 = r1; e2 = e1;
-                r1 = q; e1 = qe;
-                q = ref[c]; qe = ref[c] + cn[c];
-            }
-            if (cn[c] > 0) hit[c] = false;
-        }
-        while (q < qe)
-        {
-            const float4 v0 = gload(tris, 3 * (size_t)q + 0);
-            const float4 a1 = gload(tris, 3 * (size_t)q + 1);
-            const float4 a2 = gload(tris, 3 * (size_t)q + 2);
+            const float4 v0 = gload(tris, 3 * (size_t)tb + 0);
+            const float4 a1 = gload(tris, 3 * (size_t)tb + 1);
+            const float4 a2 = gload(tris, 3 * (size_t)tb + 2);
             if (kCount) (any ? vis_tris_any : vis_tris)++;
             float t, u, v;
             if (intersect_face(ld3(v0), ld3(a1), ld3(a2), o, d, tmin, tcur, t, u, v))
@@ -602,30 +576,62 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
                 tcur = t;
                 hb1 = u;
                 hb2 = v;
-                if (any)
-                {
-                    done = true;
-                    break;
-                }
+                if (any) done = true;
             }
-            if (++q == qe)
+            if (++tb == te)
             {
-                q = r1; qe = e1;
+                tb = r1; te = e1;
                 r1 = r2; e1 = e2;
                 r2 = r3; e2 = e3;
                 r3 = 0; e3 = 0;
             }
         }
-        if (!done)
+        else
         {
-            float key[4];
-            uint32_t id[4];
+            if (kCount) (any ? vis_nodes_any : vis_nodes)++;
+            const size_t nb = 8 * (size_t)node;
+            const float4 LX = gload(nodes, nb), HX = gload(nodes, nb + 1), LY = gload(nodes, nb + 2),
+                         HY = gload(nodes, nb + 3), LZ = gload(nodes, nb + 4), HZ = gload(nodes, nb + 5);
+            const int4 REF = gload_i(nodes, nb + 6);
+            const int4 CNT = gload_i(nodes, nb + 7);
+            const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, hx[4] = {HX.x, HX.y, HX.z, HX.w};
+            const float ly[4] = {LY.x, LY.y, LY.z, LY.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w};
+            const float lz[4] = {LZ.x, LZ.y, LZ.z, LZ.w}, hz[4] = {HZ.x, HZ.y, HZ.z, HZ.w};
+            const int ref[4] = {REF.x, REF.y, REF.z, REF.w}, cn[4] = {CNT.x, CNT.y, CNT.z, CNT.w};
+            bool hitc[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c)
             {
-                key[c] = (hit[c] && tn[c] <= tcur) ? tn[c] : kInf;
+                float x0 = __builtin_fmaf(lx[c], ix, -ox), x1 = __builtin_fmaf(hx[c], ix, -ox);
+                float y0 = __builtin_fmaf(ly[c], iy, -oy), y1 = __builtin_fmaf(hy[c], iy, -oy);
+                float z0 = __builtin_fmaf(lz[c], iz, -oz), z1 = __builtin_fmaf(hz[c], iz, -oz);
+                const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+                const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tcur));
+                hitc[c] = (tn <= tf) && (cn[c] >= 0);
+                key[c] = (hitc[c] && cn[c] == 0) ? tn : kInf;
                 id[c] = (uint32_t)ref[c];
             }
+            // the hit leaf children's triangles, in child order, as the ranges [tb, te), [r1, e1) ...
+#pragma unroll
+            for (int c = 3; c >= 0; --c)
+            {
+                if (hitc[c] && cn[c] > 0)
+                {
+                    r3 = r2; e3 = e2;
+                    r2 = r1; e2 = e1;
+                    r1 = tb; e1 = te;
+                    tb = ref[c]; te = ref[c] + cn[c];
+                }
+            }
+            sorted = false;
+        }
+        if (!done && !sorted && tb >= te)
+        {
+            // internal children still in front of the (possibly updated) closest hit, nearest
+            // first: visit it, push the others
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (!(key[c] <= tcur)) key[c] = kInf;
 #define BSH_CSWAP(a, b)                                       \
     if (key[b] < key[a])                                      \
     {                                                         \
@@ -651,6 +657,7 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
                     if (k < n) stack_push<kTraceLds, kMaxStack4>(my, kBlock, ovf, ostride, sp, id[k]);
                 node = id[0];
             }
+            sorted = true;
         }
         if (done)
         {
