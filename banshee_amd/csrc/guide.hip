@@ -182,28 +182,17 @@ __device__ __forceinline__ void dense_t(const h4 (&x)[kT][K / 16], const _Float1
     }
 }
 
-template <int N>
-__device__ __forceinline__ void store_rows(_Float16* dst, uint32_t row, bool live, const h4 (&y)[N / 16], int q)
-{
-    if (!live) return;
-#pragma unroll
-    for (int mt = 0; mt < N / 16; ++mt) *reinterpret_cast<h4*>(&dst[(size_t)row * N + mt * 16 + 4 * q]) = y[mt];
-}
-
 constexpr int kFwdWaves = 8;
 constexpr int kFwdTiles = 2;   // 16-row tiles per wave per step (weight reads shared; 4 costs occupancy)
 
 // Fused forward: rows of raw inputs (`stride` float4 per row, the first two holding
 // v0..v6) -> the network's outputs. Weights and biases are staged in LDS once per block; each
-// wave runs kFwdTiles 16-row tiles at a time with all activations in registers. With kTrain
-// the encoded input and the four hidden activations are also written (FP16, row-major) for
-// backward. kHalf: write only the kGUsed outputs the mixture uses, in FP16 (the wavefront's
+// wave runs kFwdTiles 16-row tiles at a time with all activations in registers. kHalf: write only the kGUsed outputs the mixture uses, in FP16 (the wavefront's
 // form), else all kGOut in FP32. seg (when set): rows come in segments of kGuideSeg of which
 // the first seg[s] are live; tile groups past a segment's rows are skipped.
-template <bool kTrain, bool kHalf>
+template <bool kHalf>
 __global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet g, const float4* in, int stride,
-                                                                 uint32_t nrows, const uint32_t* seg, void* outp,
-                                                                 _Float16* x0, _Float16* hs)
+                                                                 uint32_t nrows, const uint32_t* seg, void* outp)
 {
     constexpr int kGroup = 16 * kFwdTiles;
     static_assert(kGuideSeg % kGroup == 0, "tile groups must not straddle segments");
@@ -243,27 +232,12 @@ __global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet g, co
                 c = in[(size_t)stride * row[t] + 1];
             }
             encode_row(a, c, q, live[t], x[t]);
-            if (kTrain) store_rows<kGIn>(x0, row[t], live[t], x[t], q);
         }
         h4 h1[kFwdTiles][kGHid / 16], h2[kFwdTiles][kGHid / 16];
         dense_t<kGIn, kGHid, kFwdTiles>(x, W + kLF1, kL1S, lb, h1, r, q);
-        if (kTrain)
-#pragma unroll
-            for (int t = 0; t < kFwdTiles; ++t) store_rows<kGHid>(hs, row[t], live[t], h1[t], q);
         dense_t<kGHid, kGHid, kFwdTiles>(h1, W + kLF2, kLHS, lb + kGHid, h2, r, q);
-        if (kTrain)
-#pragma unroll
-            for (int t = 0; t < kFwdTiles; ++t) store_rows<kGHid>(hs + (size_t)nrows * kGHid, row[t], live[t], h2[t], q);
         dense_t<kGHid, kGHid, kFwdTiles>(h2, W + kLF3, kLHS, lb + 2 * kGHid, h1, r, q);
-        if (kTrain)
-#pragma unroll
-            for (int t = 0; t < kFwdTiles; ++t)
-                store_rows<kGHid>(hs + (size_t)2 * nrows * kGHid, row[t], live[t], h1[t], q);
         dense_t<kGHid, kGHid, kFwdTiles>(h1, W + kLF4, kLHS, lb + 3 * kGHid, h2, r, q);
-        if (kTrain)
-#pragma unroll
-            for (int t = 0; t < kFwdTiles; ++t)
-                store_rows<kGHid>(hs + (size_t)3 * nrows * kGHid, row[t], live[t], h2[t], q);
         // output layer, FP32 straight from the accumulators: lane holds outputs 16 mt + 4q + i of its row
 #pragma unroll
         for (int mt = 0; mt < kGOut / 16; ++mt)
@@ -397,153 +371,387 @@ __global__ __launch_bounds__(kWsumBlock) void k_guide_wsum_final(const float* pa
     if (threadIdx.x == 0) wsum[0] = t;
 }
 
-// dX[rows x NIN] = (dY[rows x KOUT] W^T) * (H > 0), one wave per 16 rows; W (backward
-// layout, rows n_in of k_out) staged in LDS.
-template <int KOUT, int NIN>
-__global__ __launch_bounds__(256) void k_guide_bwd(const _Float16* wb, const _Float16* dy, const _Float16* h,
-                                                   uint32_t nrows, _Float16* dx)
+// ---- fused training step ----
+// One kernel computes a batch's loss gradient for every parameter: blocks of 4 waves take
+// rounds of 64 rows (16 per wave). In a round each wave runs the forward pass of its rows with
+// the activations in registers, the loss and its gradient with respect to the 48 outputs (4 lanes
+// per row, two lobes each), and the backward pass as the transposed products
+// dH^T = W dOut^T masked by H > 0, so every activation and gradient stays in the forward's
+// register layout (lane r + 16 q: row r, features 16 j + 4 q + i). The weight gradients
+// dW_l = In_l^T G_l reduce over rows, so each wave stages its rows' layer inputs In_l and
+// output gradients G_l in LDS feature-major (one LDS row per feature, one column per row of the
+// round), and after a block barrier every wave adds the round's 64 rows to the dW tiles it owns
+// with 16x32 products whose operands are 8 consecutive rows of one feature, i.e. one 16-byte
+// read. The tiles stay in registers over all the block's rounds (bias gradients as a product
+// with a ones operand); the blocks' partial sums are added in a fixed order by
+// k_guide_reduce_part / k_guide_reduce_train, so a step's result does not depend on scheduling.
+constexpr int kTrRows = 64;                                    // rows per round
+constexpr int kRp = kTrRows + kPad;                            // staging pitch (halves) per feature
+constexpr int kTX = 0;                                         // x^T: 96 features
+constexpr int kTH = kTX + kGIn * kRp;                          // h1^T..h4^T: 4 x 64 features
+constexpr int kTY = kTH + 4 * kGHid * kRp;                     // output gradient^T: 48
+constexpr int kTD = kTY + kGOut * kRp;                         // dH1^T..dH4^T: 4 x 64
+constexpr int kTTotal = kTD + 4 * kGHid * kRp;
+constexpr int kPartStride = kGParams + 4;                      // a block's partial sums, then its loss
+static_assert((kRp * 2) % 16 == 0, "16-byte aligned feature rows");
+
+// A[m = in][k = out] fragment of W^T from the forward LDS image (out-major rows, perm32
+// columns): four rows apart.
+// dense_t for one tile with the k-blocks outer: each k-block's weight fragments for all output
+// blocks are read together and feed independent accumulators (the training kernel runs one wave
+// per SIMD, so nothing else hides the LDS latency of a dependent chain).
+template <int K, int N>
+__device__ __forceinline__ void dense1(const h4 (&x)[K / 16], const _Float16* Wt, int ws, const float* b,
+                                       h4 (&y)[N / 16], int r, int q)
 {
-    constexpr int S = KOUT + kPad;
-    __shared__ _Float16 lw[NIN * S];
-    for (int t = threadIdx.x; t < NIN * KOUT; t += 256) lw[(t / KOUT) * S + (t % KOUT)] = wb[t];
-    __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-    const uint32_t ntiles = (nrows + 15) / 16;
-    for (uint32_t tile = blockIdx.x * 4 + wave; tile < ntiles; tile += gridDim.x * 4)
+    f4v acc[N / 16];
+#pragma unroll
+    for (int mt = 0; mt < N / 16; ++mt)
     {
-        uint32_t row0 = tile * 16;
-        uint32_t ra = row0 + r;
+        const float4 bb = *reinterpret_cast<const float4*>(&b[mt * 16 + 4 * q]);
+        acc[mt] = f4v{bb.x, bb.y, bb.z, bb.w};
+    }
+#pragma unroll
+    for (int kb = 0; kb < K / 32; ++kb)
+    {
+        h8 w[N / 16];
+#pragma unroll
+        for (int mt = 0; mt < N / 16; ++mt) w[mt] = *reinterpret_cast<const h8*>(&Wt[(mt * 16 + r) * ws + kb * 32 + 8 * q]);
+#pragma unroll
+        for (int mt = 0; mt < N / 16; ++mt) acc[mt] = mfma32(w[mt], cat8(x[2 * kb], x[2 * kb + 1]), acc[mt]);
+    }
+#pragma unroll
+    for (int mt = 0; mt < N / 16; ++mt) y[mt] = relu_h4(acc[mt]);
+}
+
+// ReLU derivative bits of a layer's four fragments: bit 4 j + i = (h[j][i] > 0).
+__device__ __forceinline__ uint32_t relu_bits(const h4 (&h)[kGHid / 16])
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < kGHid / 16; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) m |= ((float)h[j][i] > 0.f ? 1u : 0u) << (4 * j + i);
+    return m;
+}
+
+template <int NOUT, int NIN>
+__device__ __forceinline__ void bwd_t(const _Float16* Wt, int ws, const h4 (&dout)[NOUT / 16], uint32_t mask,
+                                      h4 (&dh)[NIN / 16], int r, int q)
+{
+    // out-blocks outer, four independent accumulators: each block's four A fragments are read
+    // together and feed four products
+    f4v acc[NIN / 16];
+#pragma unroll
+    for (int nt = 0; nt < NIN / 16; ++nt) acc[nt] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < NOUT / 16; ++ks)
+    {
+        h4 a[NIN / 16];
 #pragma unroll
         for (int nt = 0; nt < NIN / 16; ++nt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[nt][i] = Wt[(16 * ks + 4 * q + i) * ws + perm32(16 * nt + r)];
+#pragma unroll
+        for (int nt = 0; nt < NIN / 16; ++nt) acc[nt] = mfma16(a[nt], dout[ks], acc[nt]);
+    }
+#pragma unroll
+    for (int nt = 0; nt < NIN / 16; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dh[nt][i] = (mask >> (4 * nt + i)) & 1u ? (_Float16)acc[nt][i] : (_Float16)0.f;
+}
+
+// Feature-major staging of the lane's fragments: T[feature][lr].
+template <int N>
+__device__ __forceinline__ void stage_t(_Float16* T, int lr, const h4 (&v)[N / 16], int q)
+{
+#pragma unroll
+    for (int j = 0; j < N / 16; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) T[(16 * j + 4 * q + i) * kRp + lr] = v[j][i];
+}
+
+// One wave's 16 rows of a round (raw inputs a, c and record (dir, weight) d already loaded):
+// forward, loss gradient, backward; stages In_l and G_l for the weight gradients. O: the wave's
+// 16 x 48 output exchange area. Returns the wave's summed loss (every lane the same value).
+__device__ __forceinline__ float train_rows(const _Float16* lw, const float* lb, _Float16* st, float* O, float4 a,
+                                            float4 c, float4 d, bool live, int wave, float scale, int r, int q)
+{
+    h4 x[1][kGIn / 16], h1[1][kGHid / 16], h2[1][kGHid / 16], h3[1][kGHid / 16], hh[1][kGHid / 16];
+    encode_row(a, c, q, live, x[0]);
+    dense1<kGIn, kGHid>(x[0], lw + kLF1, kL1S, lb, h1[0], r, q);
+    dense1<kGHid, kGHid>(h1[0], lw + kLF2, kLHS, lb + kGHid, h2[0], r, q);
+    dense1<kGHid, kGHid>(h2[0], lw + kLF3, kLHS, lb + 2 * kGHid, h3[0], r, q);
+    dense1<kGHid, kGHid>(h3[0], lw + kLF4, kLHS, lb + 3 * kGHid, hh[0], r, q);
+    {
+        f4v acc[kGOut / 16];
+#pragma unroll
+        for (int mt = 0; mt < kGOut / 16; ++mt)
         {
-            f4v acc = {0.f, 0.f, 0.f, 0.f};
+            const float4 bb = *reinterpret_cast<const float4*>(&lb[4 * kGHid + mt * 16 + 4 * q]);
+            acc[mt] = f4v{bb.x, bb.y, bb.z, bb.w};
+        }
 #pragma unroll
-            for (int ks = 0; ks < KOUT / 16; ++ks)
-            {
-                h4 a = {0, 0, 0, 0};
-                if (ra < nrows) a = *reinterpret_cast<const h4*>(&dy[(size_t)ra * KOUT + ks * 16 + 4 * q]);
-                h4 w = *reinterpret_cast<const h4*>(&lw[(nt * 16 + r) * S + ks * 16 + 4 * q]);
-                acc = mfma16(a, w, acc);
-            }
+        for (int kb = 0; kb < kGHid / 32; ++kb)
+        {
+            h8 w[kGOut / 16];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int mt = 0; mt < kGOut / 16; ++mt)
+                w[mt] = *reinterpret_cast<const h8*>(&lw[kLF5 + (mt * 16 + r) * kLHS + kb * 32 + 8 * q]);
+#pragma unroll
+            for (int mt = 0; mt < kGOut / 16; ++mt) acc[mt] = mfma32(w[mt], cat8(hh[0][2 * kb], hh[0][2 * kb + 1]), acc[mt]);
+        }
+#pragma unroll
+        for (int mt = 0; mt < kGOut / 16; ++mt)
+            *reinterpret_cast<float4*>(&O[r * kGOut + mt * 16 + 4 * q]) =
+                make_float4(acc[mt][0], acc[mt][1], acc[mt][2], acc[mt][3]);
+    }
+    // stage the layer inputs now; the backward keeps only the ReLU derivative bits
+    const int lr = 16 * wave + r;
+    stage_t<kGIn>(st + kTX, lr, x[0], q);
+    stage_t<kGHid>(st + kTH + 0 * kGHid * kRp, lr, h1[0], q);
+    stage_t<kGHid>(st + kTH + 1 * kGHid * kRp, lr, h2[0], q);
+    stage_t<kGHid>(st + kTH + 2 * kGHid * kRp, lr, h3[0], q);
+    stage_t<kGHid>(st + kTH + 3 * kGHid * kRp, lr, hh[0], q);
+    const uint32_t m12 = relu_bits(h1[0]) | (relu_bits(h2[0]) << 16), m34 = relu_bits(h3[0]) | (relu_bits(hh[0]) << 16);
+    __builtin_amdgcn_wave_barrier();   // the wave's LDS accesses complete in order
+    // loss -s log q(w) and its gradient (k_guide_loss's formulas); lane q takes lobes 2q, 2q+1
+    const float* o = O + r * kGOut;
+    float mx = o[0];
+#pragma unroll
+    for (int k = 1; k < kGLobes; ++k) mx = fmaxf(mx, o[k]);
+    float se = 0.f;
+#pragma unroll
+    for (int k = 0; k < kGLobes; ++k) se += gexp(o[k] - mx);
+    const float is = grcp(se);
+    const f3 wd = mk(d.x, d.y, d.z);
+    const float sw = fminf(d.w * scale, kMaxRecordWeight);
+    float wk[2], vk[2], kap[2];
+    f3 mu[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+    {
+        const int k = 2 * q + j;
+        wk[j] = gexp(o[k] - mx) * is;
+        mu[j] = lobe_mu(o, k);
+        kap[j] = gexp(clampf(o[32 + k], kLogKappaLo, kLogKappaHi));
+        vk[j] = vmf_pdf(mu[j], kap[j], wd);
+    }
+    float qm = wk[0] * vk[0] + wk[1] * vk[1];
+    qm += __shfl_xor(qm, 16);
+    qm += __shfl_xor(qm, 32);
+    const bool ok = live && qm > 0.f && sw > 0.f && isfinite(qm) && isfinite(sw);
+    float gl[2] = {0.f, 0.f}, gk[2] = {0.f, 0.f};
+    f3 gmu[2] = {mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
+    if (ok)
+    {
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+        {
+            const int k = 2 * q + j;
+            const float gam = wk[j] * vk[j] / qm;
+            gl[j] = sw * (wk[j] - gam);
+            const f3 rr = mk(o[8 + 3 * k], o[9 + 3 * k], o[10 + 3 * k]);
+            const float rn = sqrtf(sqnorm(rr));
+            if (rn > 0.f)
             {
-                uint32_t row = row0 + 4 * q + i;
-                if (row < nrows)
-                {
-                    size_t o = (size_t)row * NIN + nt * 16 + r;
-                    dx[o] = (float)h[o] > 0.f ? (_Float16)acc[i] : (_Float16)0.f;
-                }
+                const f3 gm = wd * (-sw * gam * kap[j]);
+                gmu[j] = (gm - mu[j] * dot(mu[j], gm)) * (1.f / rn);
             }
+            const float lk = o[32 + k];
+            if (lk > kLogKappaLo && lk < kLogKappaHi)
+            {
+                const float e = expf(-2.f * kap[j]);
+                const float dlogv = 1.f / kap[j] - 2.f * e / (1.f - e) + (dot(mu[j], wd) - 1.f);
+                gk[j] = -sw * gam * kap[j] * dlogv;
+            }
+        }
+    }
+    float loss = (ok && q == 0) ? -sw * logf(qm) : 0.f;
+    // stage the output gradient (this lane's 12 of the row's 48)
+    _Float16* Yt = st + kTY + lr;
+    Yt[(2 * q) * kRp] = (_Float16)gl[0];
+    Yt[(2 * q + 1) * kRp] = (_Float16)gl[1];
+    Yt[(8 + 6 * q + 0) * kRp] = (_Float16)gmu[0].x;
+    Yt[(8 + 6 * q + 1) * kRp] = (_Float16)gmu[0].y;
+    Yt[(8 + 6 * q + 2) * kRp] = (_Float16)gmu[0].z;
+    Yt[(8 + 6 * q + 3) * kRp] = (_Float16)gmu[1].x;
+    Yt[(8 + 6 * q + 4) * kRp] = (_Float16)gmu[1].y;
+    Yt[(8 + 6 * q + 5) * kRp] = (_Float16)gmu[1].z;
+    Yt[(32 + 2 * q) * kRp] = (_Float16)gk[0];
+    Yt[(33 + 2 * q) * kRp] = (_Float16)gk[1];
+    Yt[(40 + 2 * q) * kRp] = (_Float16)0.f;
+    Yt[(41 + 2 * q) * kRp] = (_Float16)0.f;
+    __builtin_amdgcn_wave_barrier();
+    h4 dy[kGOut / 16];
+#pragma unroll
+    for (int ks = 0; ks < kGOut / 16; ++ks)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dy[ks][i] = Yt[(16 * ks + 4 * q + i) * kRp];
+    h4 g4[kGHid / 16], g3[kGHid / 16];
+    bwd_t<kGOut, kGHid>(lw + kLF5, kLHS, dy, m34 >> 16, g4, r, q);
+    stage_t<kGHid>(st + kTD + 3 * kGHid * kRp, lr, g4, q);
+    bwd_t<kGHid, kGHid>(lw + kLF4, kLHS, g4, m34 & 0xffffu, g3, r, q);
+    stage_t<kGHid>(st + kTD + 2 * kGHid * kRp, lr, g3, q);
+    bwd_t<kGHid, kGHid>(lw + kLF3, kLHS, g3, m12 >> 16, g4, r, q);
+    stage_t<kGHid>(st + kTD + 1 * kGHid * kRp, lr, g4, q);
+    bwd_t<kGHid, kGHid>(lw + kLF2, kLHS, g4, m12 & 0xffffu, g3, r, q);
+    stage_t<kGHid>(st + kTD + 0 * kGHid * kRp, lr, g3, q);
+    // the wave's loss, summed over its lanes in a fixed butterfly
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) loss += __shfl_xor(loss, m);
+    return loss;
+}
+
+// acc[A0 + NJ m + nj] += sum over the round's 64 rows of In[row][16 m + in] G[row][16 nj + out], In^T
+// and G^T pointing at the tile block's first feature (A[m = in][k = row] and B[k = row][n = out]:
+// 8 consecutive rows of one feature per lane); with kBias also acc[B0 + nj] += sum_rows
+// G[row][16 nj + out] in every row (a product with a ones operand).
+constexpr int kAccTiles = 28;   // dW and bias tiles per wave
+template <int NMI, int NJ, int A0, bool kBias, int B0>
+__device__ __forceinline__ void wgrad_round(const _Float16* In, const _Float16* G, f4v (&acc)[kAccTiles], int r, int q)
+{
+    const h8 ones = {(_Float16)1.f, (_Float16)1.f, (_Float16)1.f, (_Float16)1.f,
+                     (_Float16)1.f, (_Float16)1.f, (_Float16)1.f, (_Float16)1.f};
+#pragma unroll
+    for (int g = 0; g < kTrRows / 32; ++g)
+    {
+        h8 b[NJ];
+#pragma unroll
+        for (int nj = 0; nj < NJ; ++nj) b[nj] = *reinterpret_cast<const h8*>(&G[(16 * nj + r) * kRp + 32 * g + 8 * q]);
+        if (kBias)
+#pragma unroll
+            for (int nj = 0; nj < NJ; ++nj) acc[B0 + nj] = mfma32(ones, b[nj], acc[B0 + nj]);
+#pragma unroll
+        for (int m = 0; m < NMI; ++m)
+        {
+            const h8 a = *reinterpret_cast<const h8*>(&In[(16 * m + r) * kRp + 32 * g + 8 * q]);
+#pragma unroll
+            for (int nj = 0; nj < NJ; ++nj) acc[A0 + NJ * m + nj] = mfma32(a, b[nj], acc[A0 + NJ * m + nj]);
         }
     }
 }
 
-// Partial weight gradients over a chunk of kWgRows rows: dW[k][n] = sum_rows X[row][k] dY[row][n]
-// (MFMA with the rows as the reduction dimension) and db[n] = sum_rows dY[row][n]. Four waves
-// take a quarter of the chunk each, staging 16-row tiles in wave-private LDS; their sums are
-// added in wave order, so the result does not depend on scheduling.
-// part: [gridDim.x][KIN*NOUT + NOUT]
-constexpr int kWgRows = 512;
-template <int KIN, int NOUT>
-__global__ __launch_bounds__(256) void k_guide_wgrad(const _Float16* x, const _Float16* dy, uint32_t nrows, float* part)
+template <int NMI, int NJ, int A0>
+__device__ __forceinline__ void wgrad_store(float* P, int nout, const f4v (&acc)[kAccTiles], int r, int q)
 {
-    constexpr int XS = KIN + kPad, YS = NOUT + kPad;
-    __shared__ __align__(16) _Float16 xs[4][16 * XS];
-    __shared__ __align__(16) _Float16 ys[4][16 * YS];
-    __shared__ float red[KIN * NOUT + NOUT];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-    f4v acc[KIN / 16][NOUT / 16];
+    // P points at the block's first weight: row = in feature, nout columns
 #pragma unroll
-    for (int a = 0; a < KIN / 16; ++a)
+    for (int m = 0; m < NMI; ++m)
 #pragma unroll
-        for (int b = 0; b < NOUT / 16; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
-    float bsum = 0.f;
-    _Float16* X = xs[wave];
-    _Float16* Y = ys[wave];
-    const uint32_t r0 = blockIdx.x * kWgRows + wave * (kWgRows / 4);
-    const uint32_t r1 = min(r0 + kWgRows / 4, nrows);
-    for (uint32_t s0 = r0; s0 < r1; s0 += 16)
+        for (int nj = 0; nj < NJ; ++nj)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) P[(16 * m + 4 * q + i) * nout + 16 * nj + r] = acc[A0 + NJ * m + nj][i];
+}
+
+// The dW tiles of wave w (the same shapes for every wave, so one code path):
+//   acc[0..15]  a 64 x 64 block: layer 1 in-features 0..63 (w = 0) or layer w + 1, with the
+//               layer's bias in acc[16..19];
+//   acc[20..22] layer 5, in-features 16 w .. 16 w + 15, all 48 outputs (wave 0 also the bias in
+//               acc[25..27]);
+//   acc[23..24] layer 1, in-features 64 + 16 (w / 2) .., outputs 32 (w % 2) .. + 31.
+__device__ __forceinline__ void wgrad_wave(const _Float16* st, int wave, f4v (&acc)[kAccTiles], int r, int q)
+{
+    const _Float16* in1 = wave == 0 ? st + kTX : st + kTH + (wave - 1) * kGHid * kRp;
+    wgrad_round<4, 4, 0, true, 16>(in1, st + kTD + wave * kGHid * kRp, acc, r, q);
+    const _Float16* h4t = st + kTH + 3 * kGHid * kRp + 16 * wave * kRp;
+    if (wave == 0) wgrad_round<1, 3, 20, true, 25>(h4t, st + kTY, acc, r, q);
+    else wgrad_round<1, 3, 20, false, 25>(h4t, st + kTY, acc, r, q);
+    wgrad_round<1, 2, 23, false, 0>(st + kTX + (64 + 16 * (wave >> 1)) * kRp, st + kTD + 32 * (wave & 1) * kRp, acc,
+                                    r, q);
+}
+
+__device__ __forceinline__ void wgrad_wave_store(float* P, int wave, const f4v (&acc)[kAccTiles], int r, int q)
+{
+    const int woff = wave == 0 ? kGW1 : wave == 1 ? kGW2 : wave == 2 ? kGW3 : kGW4;
+    const int boff = wave == 0 ? kGB1 : wave == 1 ? kGB2 : wave == 2 ? kGB3 : kGB4;
+    wgrad_store<4, 4, 0>(P + woff, kGHid, acc, r, q);
+    wgrad_store<1, 3, 20>(P + kGW5 + 16 * wave * kGOut, kGOut, acc, r, q);
+    wgrad_store<1, 2, 23>(P + kGW1 + (64 + 16 * (wave >> 1)) * kGHid + 32 * (wave & 1), kGHid, acc, r, q);
+    if (q == 0)
     {
-        for (int t = lane; t < 16 * (KIN / 4); t += 64)
-        {
-            int rr = t / (KIN / 4), c = (t % (KIN / 4)) * 4;
-            uint32_t row = s0 + rr;
-            h4 v = {0, 0, 0, 0};
-            if (row < nrows) v = *reinterpret_cast<const h4*>(&x[(size_t)row * KIN + c]);
-            *reinterpret_cast<h4*>(&X[rr * XS + c]) = v;
-        }
-        for (int t = lane; t < 16 * (NOUT / 4); t += 64)
-        {
-            int rr = t / (NOUT / 4), c = (t % (NOUT / 4)) * 4;
-            uint32_t row = s0 + rr;
-            h4 v = {0, 0, 0, 0};
-            if (row < nrows) v = *reinterpret_cast<const h4*>(&dy[(size_t)row * NOUT + c]);
-            *reinterpret_cast<h4*>(&Y[rr * YS + c]) = v;
-        }
-        __builtin_amdgcn_wave_barrier();   // wave-private tiles: LDS ops of one wave complete in order
 #pragma unroll
-        for (int mt = 0; mt < KIN / 16; ++mt)
-        {
-            h4 a;   // A[m = k_in][kk = row]: X[row 4q+i][16 mt + r]
+        for (int nj = 0; nj < 4; ++nj) P[boff + 16 * nj + r] = acc[16 + nj][0];
+        if (wave == 0)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = X[(4 * q + i) * XS + mt * 16 + r];
-#pragma unroll
-            for (int nt = 0; nt < NOUT / 16; ++nt)
-            {
-                h4 b;   // B[kk = row][n]: dY[row 4q+i][16 nt + r]
-#pragma unroll
-                for (int i = 0; i < 4; ++i) b[i] = Y[(4 * q + i) * YS + nt * 16 + r];
-                acc[mt][nt] = mfma16(a, b, acc[mt][nt]);
-            }
-        }
-        if (lane < NOUT)
-            for (int i = 0; i < 16; ++i) bsum += (float)Y[i * YS + lane];
-        __builtin_amdgcn_wave_barrier();
+            for (int nj = 0; nj < 3; ++nj) P[kGB5 + 16 * nj + r] = acc[25 + nj][0];
     }
-    for (int w = 0; w < 4; ++w)
+}
+
+__global__ __launch_bounds__(256, 1) void k_guide_train_fused(GuideNet g, const float4* rec, int srow, uint32_t nrows,
+                                                              const float* wsum, float* part)
+{
+    __shared__ __align__(16) _Float16 lw[kLFTotal];
+    __shared__ __align__(16) float lb[kBiasTotal];
+    __shared__ __align__(16) _Float16 st[kTTotal];
+    __shared__ __align__(16) float ox[4 * 16 * kGOut];
+    __shared__ float red[4];
+    load_fwd_weights(g, lw, lb);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+    const float scale = (float)nrows / fmaxf(wsum[0], 1e-30f);
+    f4v acc[kAccTiles];
+#pragma unroll
+    for (int t = 0; t < kAccTiles; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+    float loss = 0.f;
+    const uint32_t rounds = (nrows + kTrRows - 1) / kTrRows;
+    // the lane's record of the next round, loaded one round ahead
+    float4 na = make_float4(0.f, 0.f, 0.f, 0.f), nc = na, nd = na;
+    uint32_t nrow = blockIdx.x * kTrRows + 16 * wave + r;
+    if (blockIdx.x < rounds && nrow < nrows)
     {
-        if (wave == w)
+        na = rec[(size_t)srow * nrow];
+        nc = rec[(size_t)srow * nrow + 1];
+        nd = rec[(size_t)srow * nrow + 2];
+    }
+    for (uint32_t rd = blockIdx.x; rd < rounds; rd += gridDim.x)
+    {
+        const float4 a = na, c = nc, d = nd;
+        const bool live = nrow < nrows;
+        nrow += gridDim.x * kTrRows;
+        if (rd + gridDim.x < rounds && nrow < nrows)
         {
-#pragma unroll
-            for (int mt = 0; mt < KIN / 16; ++mt)
-#pragma unroll
-                for (int nt = 0; nt < NOUT / 16; ++nt)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)   // D[row m = 16 mt + 4q + i][col n = 16 nt + r]
-                    {
-                        int o = (mt * 16 + 4 * q + i) * NOUT + nt * 16 + r;
-                        red[o] = (w == 0 ? 0.f : red[o]) + acc[mt][nt][i];
-                    }
-            if (lane < NOUT) red[KIN * NOUT + lane] = (w == 0 ? 0.f : red[KIN * NOUT + lane]) + bsum;
+            na = rec[(size_t)srow * nrow];
+            nc = rec[(size_t)srow * nrow + 1];
+            nd = rec[(size_t)srow * nrow + 2];
         }
+        // an offset the compiler cannot see through keeps the weight reads inside the loop
+        int zoff = 0;
+        asm volatile("" : "+v"(zoff));
+        loss += train_rows(lw + zoff, lb, st, ox + wave * 16 * kGOut, a, c, d, live, wave, scale, r, q);
+        __syncthreads();
+        wgrad_wave(st, wave, acc, r, q);
         __syncthreads();
     }
-    float* P = part + (size_t)blockIdx.x * (KIN * NOUT + NOUT);
-    for (int t = threadIdx.x; t < KIN * NOUT + NOUT; t += 256) P[t] = red[t];
+    float* P = part + (size_t)blockIdx.x * kPartStride;
+    wgrad_wave_store(P, wave, acc, r, q);
+    if (lane == 0) red[wave] = loss;
+    __syncthreads();
+    if (threadIdx.x == 0) P[kGParams] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-// Sums the partial gradients of one layer into grad[W..., b...] in a fixed order and undoes
-// the loss scaling, in two passes so that no thread walks all chunks: pass 1 sums groups of
-// kRedGroup chunks (blockIdx.y = group), pass 2 sums the groups.
+// Sums the fused kernel's block partials in a fixed order, in two passes so that no thread walks
+// every block: pass 1 (blockIdx.y = group) adds groups of kRedGroup blocks, pass 2 the groups,
+// giving grad = sum / rows and loss = the summed per-row losses.
 constexpr int kRedGroup = 16;
-__global__ void k_guide_reduce_groups(const float* part, int nchunks, int n, float* out)
+__global__ void k_guide_reduce_part(const float* part, int nblocks, float* out)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int c0 = blockIdx.y * kRedGroup, c1 = min(c0 + kRedGroup, nchunks);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > kGParams) return;
+    const int b0 = blockIdx.y * kRedGroup, b1 = min(b0 + kRedGroup, nblocks);
     float s = 0.f;
-    for (int c = c0; c < c1; ++c) s += part[(size_t)c * n + t];
-    out[(size_t)blockIdx.y * n + t] = s;
+    for (int b = b0; b < b1; ++b) s += part[(size_t)b * kPartStride + t];
+    out[(size_t)blockIdx.y * kPartStride + t] = s;
 }
 
-__global__ void k_guide_reduce(const float* part, int nchunks, int n, float inv_rows, float* grad)
+__global__ void k_guide_reduce_train(const float* part, int ngroups, float inv_rows, float* grad, float* loss)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > kGParams) return;
     float s = 0.f;
-    for (int c = 0; c < nchunks; ++c) s += part[(size_t)c * n + t];
-    grad[t] = s * inv_rows;
+    for (int b = 0; b < ngroups; ++b) s += part[(size_t)b * kPartStride + t];
+    if (t < kGParams) grad[t] = s * inv_rows;
+    else loss[0] = s;
 }
 
 __global__ void k_guide_zero(float* p, int n)
@@ -704,10 +912,9 @@ struct bsh_guide
     bsh_context* ctx = nullptr;
     bsh_guide_params prm{};
     std::vector<float> host_params;
-    DevBuf<float> p, m, v, grad, bias, part, part2, out, loss, wsum;
-    DevBuf<_Float16> wf, wb, x0, hs, d5, d4, d3, d2, d1;
+    DevBuf<float> p, m, v, grad, bias, part, part2, loss, wsum;
+    DevBuf<_Float16> wf, wb;
     int step = 0;
-    size_t cap = 0;
     GuideNet net() const { return GuideNet{wf.p, wb.p, bias.p}; }
 };
 
@@ -758,24 +965,6 @@ static bsh_status upload_params(bsh_guide* g)
     return BSH_OK;
 }
 
-static bsh_status ensure_train(bsh_guide* g, size_t rows)
-{
-    if (rows <= g->cap) return BSH_OK;
-    BSH_HIP(g->x0.alloc(rows * kGIn));
-    BSH_HIP(g->hs.alloc(rows * kGHid * 4));
-    BSH_HIP(g->out.alloc(rows * kGOut));
-    BSH_HIP(g->d5.alloc(rows * kGOut));
-    BSH_HIP(g->d4.alloc(rows * kGHid));
-    BSH_HIP(g->d3.alloc(rows * kGHid));
-    BSH_HIP(g->d2.alloc(rows * kGHid));
-    BSH_HIP(g->d1.alloc(rows * kGHid));
-    size_t nchunks = (rows + kWgRows - 1) / kWgRows;
-    BSH_HIP(g->part.alloc(nchunks * (size_t)(kGIn * kGHid + kGHid)));
-    BSH_HIP(g->part2.alloc(((nchunks + kRedGroup - 1) / kRedGroup) * (size_t)(kGIn * kGHid + kGHid)));
-    g->cap = rows;
-    return BSH_OK;
-}
-
 float guide_bsdf_fraction(const bsh_guide* g) { return g->prm.bsdf_fraction; }
 int guide_bounces(const bsh_guide* g) { return g->prm.guided_bounces; }
 
@@ -784,8 +973,8 @@ bsh_status guide_forward(bsh_guide* g, const float4* in, int stride, uint32_t n,
     if (n == 0) return BSH_OK;
     uint32_t groups = (n + 16 * kFwdTiles - 1) / (16 * kFwdTiles);
     uint32_t blocks = std::min<uint32_t>((groups + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
-    hipLaunchKernelGGL((k_guide_forward<false, false>), dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), in,
-                       stride, n, (const uint32_t*)nullptr, (void*)out, (_Float16*)nullptr, (_Float16*)nullptr);
+    hipLaunchKernelGGL((k_guide_forward<false>), dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), in,
+                       stride, n, (const uint32_t*)nullptr, (void*)out);
     BSH_HIP(hipGetLastError());
     return BSH_OK;
 }
@@ -796,64 +985,29 @@ bsh_status guide_forward_rows(bsh_guide* g, const float4* in, const uint32_t* se
     if (rows == 0) return BSH_OK;
     uint32_t groups = (rows + 16 * kFwdTiles - 1) / (16 * kFwdTiles);
     uint32_t blocks = std::min<uint32_t>((groups + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
-    hipLaunchKernelGGL((k_guide_forward<false, true>), dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), in, 2,
-                       rows, seg_counts, (void*)out, (_Float16*)nullptr, (_Float16*)nullptr);
+    hipLaunchKernelGGL((k_guide_forward<true>), dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), in, 2,
+                       rows, seg_counts, (void*)out);
     BSH_HIP(hipGetLastError());
     return BSH_OK;
 }
 
-template <int K, int N>
-static void wgrad_layer(bsh_guide* g, const _Float16* x, const _Float16* dy, uint32_t n, int param_off, hipStream_t st)
-{
-    int nchunks = (int)((n + kWgRows - 1) / kWgRows);
-    hipLaunchKernelGGL((k_guide_wgrad<K, N>), dim3(nchunks), dim3(256), 0, st, x, dy, n, g->part.p);
-    int cnt = K * N + N;
-    if (nchunks > kRedGroup)
-    {
-        int groups = (nchunks + kRedGroup - 1) / kRedGroup;
-        hipLaunchKernelGGL(k_guide_reduce_groups, dim3((cnt + 255) / 256, groups), dim3(256), 0, st, g->part.p,
-                           nchunks, cnt, g->part2.p);
-        hipLaunchKernelGGL(k_guide_reduce, dim3((cnt + 255) / 256), dim3(256), 0, st, g->part2.p, groups, cnt,
-                           1.f / (float)n, g->grad.p + param_off);
-    }
-    else
-    {
-        hipLaunchKernelGGL(k_guide_reduce, dim3((cnt + 255) / 256), dim3(256), 0, st, g->part.p, nchunks, cnt,
-                           1.f / (float)n, g->grad.p + param_off);
-    }
-}
-
 // Forward + backward on `rec` (3 float4 per row); leaves the batch gradient in g->grad
-// and the weighted mean loss (times rows) in g->loss.
+// and the summed loss (times rows) in g->loss.
 static bsh_status guide_gradients(bsh_guide* g, const float4* rec, int srow, uint32_t n, hipStream_t st)
 {
-    bsh_status r = ensure_train(g, n);
-    if (r != BSH_OK) return r;
-    hipLaunchKernelGGL(k_guide_zero, dim3(1), dim3(64), 0, st, g->wsum.p, 1);
-    hipLaunchKernelGGL(k_guide_zero, dim3(1), dim3(64), 0, st, g->loss.p, 1);
+    const int ncu = g->ctx->num_cus;
+    const uint32_t nb = std::min<uint32_t>((n + kTrRows - 1) / kTrRows, (uint32_t)ncu);
+    const uint32_t ngroups = (nb + kRedGroup - 1) / kRedGroup;
+    if (!g->part.p) BSH_HIP(g->part.alloc((size_t)ncu * kPartStride));
+    if (!g->part2.p) BSH_HIP(g->part2.alloc((size_t)((ncu + kRedGroup - 1) / kRedGroup) * kPartStride));
     hipLaunchKernelGGL(k_guide_wsum, dim3(kWsumParts), dim3(kWsumBlock), 0, st, rec, srow, n, g->wsum.p + 1);
     hipLaunchKernelGGL(k_guide_wsum_final, dim3(1), dim3(kWsumBlock), 0, st, g->wsum.p + 1, g->wsum.p);
-    uint32_t tiles = (n + 15) / 16;
-    uint32_t groups = (n + 16 * kFwdTiles - 1) / (16 * kFwdTiles);
-    uint32_t blocks = std::min<uint32_t>((groups + kFwdWaves - 1) / kFwdWaves, (uint32_t)g->ctx->num_cus * 3);
-    hipLaunchKernelGGL((k_guide_forward<true, false>), dim3(blocks), dim3(64 * kFwdWaves), 0, st, g->net(), rec, srow, n,
-                       (const uint32_t*)nullptr, (void*)g->out.p, g->x0.p, g->hs.p);
-    hipLaunchKernelGGL(k_guide_loss<_Float16>, dim3((n + 255) / 256), dim3(256), 0, st, g->out.p, rec, srow, n,
-                       g->wsum.p, g->d5.p, g->loss.p);
-    const _Float16* H1 = g->hs.p;
-    const _Float16* H2 = g->hs.p + (size_t)n * kGHid;
-    const _Float16* H3 = g->hs.p + (size_t)2 * n * kGHid;
-    const _Float16* H4 = g->hs.p + (size_t)3 * n * kGHid;
-    uint32_t bb = std::min<uint32_t>((tiles + 3) / 4, (uint32_t)g->ctx->num_cus * 8);
-    hipLaunchKernelGGL((k_guide_bwd<kGOut, kGHid>), dim3(bb), dim3(256), 0, st, g->wb.p + kB5, g->d5.p, H4, n, g->d4.p);
-    hipLaunchKernelGGL((k_guide_bwd<kGHid, kGHid>), dim3(bb), dim3(256), 0, st, g->wb.p + kB4, g->d4.p, H3, n, g->d3.p);
-    hipLaunchKernelGGL((k_guide_bwd<kGHid, kGHid>), dim3(bb), dim3(256), 0, st, g->wb.p + kB3, g->d3.p, H2, n, g->d2.p);
-    hipLaunchKernelGGL((k_guide_bwd<kGHid, kGHid>), dim3(bb), dim3(256), 0, st, g->wb.p + kB2, g->d2.p, H1, n, g->d1.p);
-    wgrad_layer<kGHid, kGOut>(g, H4, g->d5.p, n, kGW5, st);
-    wgrad_layer<kGHid, kGHid>(g, H3, g->d4.p, n, kGW4, st);
-    wgrad_layer<kGHid, kGHid>(g, H2, g->d3.p, n, kGW3, st);
-    wgrad_layer<kGHid, kGHid>(g, H1, g->d2.p, n, kGW2, st);
-    wgrad_layer<kGIn, kGHid>(g, g->x0.p, g->d1.p, n, kGW1, st);
+    hipLaunchKernelGGL(k_guide_train_fused, dim3(nb), dim3(256), 0, st, g->net(), rec, srow, n,
+                       (const float*)g->wsum.p, g->part.p);
+    const dim3 rg((kGParams + 1 + 255) / 256, ngroups);
+    hipLaunchKernelGGL(k_guide_reduce_part, rg, dim3(256), 0, st, (const float*)g->part.p, (int)nb, g->part2.p);
+    hipLaunchKernelGGL(k_guide_reduce_train, dim3(rg.x), dim3(256), 0, st, (const float*)g->part2.p, (int)ngroups,
+                       1.f / (float)n, g->grad.p, g->loss.p);
     BSH_HIP(hipGetLastError());
     return BSH_OK;
 }
