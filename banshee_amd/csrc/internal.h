@@ -165,8 +165,23 @@ struct bsh_context
     static constexpr int kLanes = 2;
     bsh::RenderWork lane_work[kLanes - 1];
     hipStream_t lane_stream[kLanes - 1] = {};
+    // the last batch graph (bsh_batch_run with use_graph), replayed while its key matches
+    hipStream_t graph_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint8_t> graph_key;
+    void drop_graph()
+    {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_key.clear();
+    }
     ~bsh_context()
     {
+        drop_graph();
+        if (graph_stream) (void)hipStreamDestroy(graph_stream);
         for (hipStream_t s : lane_stream)
             if (s) (void)hipStreamDestroy(s);
     }

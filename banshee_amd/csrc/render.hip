@@ -34,6 +34,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -1984,6 +1985,7 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         return BSH_OK;
     };
     Lane lanes[bsh_context::kLanes];
+    std::memset(lanes, 0, sizeof lanes);   // zero padding: the device views are compared bytewise below
     lanes[0].w = &c->work;
     lanes[0].st = st;
     BSH_TRY(setup_lane(c->work, lanes[0].R));
@@ -2002,47 +2004,72 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     if (use_graph)
     {
         // Capture the whole batch (every chunk's raygen, bounces, accumulation) as one graph
-        // and launch it once. Counters are cleared by kernels, never by memset nodes.
+        // and launch it; counters are cleared by kernels, never by memset nodes. The launches
+        // bake in the lanes' device views (buffer pointers), the chunk plan and the kernel
+        // variants, which follow from the key below; the job table itself is uploaded before
+        // every launch. So an unguided run whose key matches the context's last graph replays
+        // it without capture or instantiation.
+        std::vector<uint8_t> key;
+        auto put = [&key](const void* p, size_t n) {
+            const uint8_t* b = static_cast<const uint8_t*>(p);
+            key.insert(key.end(), b, b + n);
+        };
+        const int head[3] = {count ? 1 : 0, nlanes, (int)hjobs.size()};
+        put(head, sizeof head);
+        put(&tune, sizeof tune);
+        for (const JobDev& J : hjobs)
+        {
+            const int f[12] = {J.type, J.s_count, J.N, J.cw, J.ch, J.nprobes, J.depth, J.flags,
+                               J.s.narea, J.s.nlights, J.s.ext_mats, J.s.width};
+            put(f, sizeof f);
+        }
+        for (int k = 0; k < nlanes; ++k) put(&lanes[k].R, sizeof(RenderDev));
+        const bool cacheable = !gr;
+        bool replay = cacheable && c->graph_exec && c->graph_key == key;
+        if (!c->graph_stream) BSH_HIP(hipStreamCreateWithFlags(&c->graph_stream, hipStreamNonBlocking));
+        hipStream_t cs = c->graph_stream;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         BSH_HIP(hipEventCreate(&e0));
         BSH_HIP(hipEventCreate(&e1));
-        hipStream_t cs = nullptr;
-        BSH_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
         hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
+        hipGraphExec_t exec = replay ? c->graph_exec : nullptr;
         bsh_status rc = BSH_OK;
-        // order the capture stream after prior work on `st` (job table upload, stats clear)
+        // order the graph stream after prior work on `st` (job table upload, stats clear)
         (void)hipEventRecord(e0, st);
         (void)hipStreamWaitEvent(cs, e0, 0);
-        (void)hipStreamSynchronize(cs);
-        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess)
+        if (!replay)
         {
-            // the second lane joins the capture through an event recorded on the capture stream
-            // and is joined back before it ends, so the graph holds both lanes' chains
-            Lane gl[bsh_context::kLanes];
-            for (int k = 0; k < nlanes; ++k) gl[k] = lanes[k];
-            gl[0].st = cs;
-            ForkJoin fj;
-            if (nlanes > 1) rc = fj.fork(cs, c->lane_stream, nlanes - 1);
-            if (rc == BSH_OK) rc = enqueue_batch(c, gl, nlanes, chunks, hjobs, count, tune, nullptr, gr);
-            if (rc == BSH_OK && nlanes > 1) rc = fj.join(cs, c->lane_stream, nlanes - 1);
-            hipError_t e2 = hipStreamEndCapture(cs, &graph);
-            if (rc == BSH_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
-        }
-        else
-        {
-            rc = hip_fail(e, "hipStreamBeginCapture");
-        }
-        if (rc == BSH_OK)
-        {
-            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+            c->drop_graph();
+            (void)hipStreamSynchronize(cs);
+            hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess)
+            {
+                // the second lane joins the capture through an event recorded on the capture
+                // stream and is joined back before it ends, so the graph holds both lanes' chains
+                Lane gl[bsh_context::kLanes];
+                for (int k = 0; k < nlanes; ++k) gl[k] = lanes[k];
+                gl[0].st = cs;
+                ForkJoin fj;
+                if (nlanes > 1) rc = fj.fork(cs, c->lane_stream, nlanes - 1);
+                if (rc == BSH_OK) rc = enqueue_batch(c, gl, nlanes, chunks, hjobs, count, tune, nullptr, gr);
+                if (rc == BSH_OK && nlanes > 1) rc = fj.join(cs, c->lane_stream, nlanes - 1);
+                hipError_t e2 = hipStreamEndCapture(cs, &graph);
+                if (rc == BSH_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
+            }
+            else
+            {
+                rc = hip_fail(e, "hipStreamBeginCapture");
+            }
+            if (rc == BSH_OK)
+            {
+                e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+            }
         }
         if (rc == BSH_OK)
         {
             (void)hipEventRecord(e0, cs);
-            e = hipGraphLaunch(exec, cs);
+            hipError_t e = hipGraphLaunch(exec, cs);
             if (e != hipSuccess) rc = hip_fail(e, "hipGraphLaunch");
             (void)hipEventRecord(e1, cs);
             if (rc == BSH_OK)
@@ -2054,11 +2081,23 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
             (void)hipEventElapsedTime(&ms, e0, e1);
             S.total_ms = ms;
         }
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        (void)hipStreamDestroy(cs);
+        if (!replay)
+        {
+            if (rc == BSH_OK && cacheable)
+            {
+                c->graph_exec = exec;
+                c->graph = graph;
+                c->graph_key.swap(key);
+            }
+            else
+            {
+                if (exec) (void)hipGraphExecDestroy(exec);
+                if (graph) (void)hipGraphDestroy(graph);
+            }
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        S.graph_replayed = replay ? 1u : 0u;
         if (rc != BSH_OK) return rc;
         S.trace_launches = S.shade_launches = (uint32_t)pairs;
     }

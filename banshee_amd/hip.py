@@ -122,7 +122,8 @@ class RenderStats(ctypes.Structure):
                 ("tri_tests", ctypes.c_uint64), ("trace_ms", ctypes.c_double), ("shade_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("trace_launches", ctypes.c_uint32),
                 ("shade_launches", ctypes.c_uint32), ("guide_launches", ctypes.c_uint32),
-                ("guide_rows", ctypes.c_uint64), ("guide_ms", ctypes.c_double)]
+                ("guide_rows", ctypes.c_uint64), ("guide_ms", ctypes.c_double),
+                ("graph_replayed", ctypes.c_uint32)]
 
 
 class LaunchStats(ctypes.Structure):

@@ -31,6 +31,8 @@ METRIC = "GI Mrays/s; time-to-target-RMSE guided vs plain PT, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
 MFMA_FP16_PEAK_TFLOPS = 2500.0
 MLP_FLOP_PER_ROW = 2 * (96 * 64 + 3 * 64 * 64 + 64 * 48)   # one forward row of the guide network
+# one training row: the forward, the backward products of layers 5..2 and every weight gradient
+TRAIN_FLOP_PER_ROW = MLP_FLOP_PER_ROW + 2 * (48 * 64 + 3 * 64 * 64) + 2 * (96 * 64 + 3 * 64 * 64 + 64 * 48)
 W, H, GRID, DEPTH, SEED = 1920, 1080, 8, 8, 1
 
 
@@ -153,6 +155,36 @@ def make_context(args):
     return ctx
 
 
+def training_rate(ctx, rows=1 << 18, steps=30) -> dict:
+    """Adam steps of the guide on `rows` synthetic records (uniform positions and roughness,
+    isotropic directions, weights in [0.1, 2)), launched back to back and timed by the wall
+    clock around them (the launches run ahead of the device, so the clock sees device time)."""
+    import numpy as np
+    from banshee_amd import hip
+    rng = np.random.default_rng(5)
+    rec = np.zeros((rows, 12), np.float32)
+    rec[:, 0:7] = rng.uniform(0.0, 1.0, (rows, 7))
+    d = rng.normal(size=(rows, 3))
+    rec[:, 8:11] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rec[:, 11] = rng.uniform(0.1, 2.0, rows)
+    dr = ctx.to_device(rec)
+    guide = hip.Guide(ctx, seed=3)
+    for _ in range(3):
+        guide.train_step(dr, want_loss=False)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        guide.train_step(dr, want_loss=False)
+    ctx.sync()
+    s = (time.perf_counter() - t) / steps
+    tflops = rows * TRAIN_FLOP_PER_ROW / s / 1e12
+    return {"bound": "mfma", "kernel": "k_guide_train_fused (one Adam step: forward, loss, backward, weight "
+                                       "gradients; FP16 MFMA, FP32 sums)",
+            "achieved": round(tflops, 1), "peak": MFMA_FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / MFMA_FP16_PEAK_TFLOPS, 4), "rows": rows, "flop_per_row": TRAIN_FLOP_PER_ROW,
+            "ms_per_step": round(s * 1e3, 4)}
+
+
 def guided_comparison(ctx, bvh_width) -> dict:
     from banshee_amd import hip
     from banshee_amd.scenes import occluded_room
@@ -177,7 +209,8 @@ def guided_comparison(ctx, bvh_width) -> dict:
             "mlp_roofline": {"bound": "mfma", "kernel": "k_guide_forward (inference, FP16 MFMA)",
                              "achieved": round(tflops, 1), "peak": MFMA_FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
                              "frac": round(tflops / MFMA_FP16_PEAK_TFLOPS, 4), "rows": int(rows),
-                             "flop_per_row": MLP_FLOP_PER_ROW, "inference_ms": round(infer_ms, 2)},
+                             "flop_per_row": MLP_FLOP_PER_ROW, "inference_ms": round(infer_ms, 2),
+                             "training": training_rate(ctx)},
             "guide": {"bsdf_fraction": G_ALPHA, "warmup_passes": G_WARM, "lr": G_LR, "guided_bounces": G_BOUNCES,
                       "adam_steps_per_pass": [G_STEPS, G_STEPS_LATE, G_LATE], "batch": G_BATCH,
                       "spp_per_pass": G_PASS_GRID ** 2}}

@@ -180,6 +180,9 @@ typedef struct
     uint32_t guide_launches;           /* guide network inference launches (guided render or bake) */
     uint64_t guide_rows;               /* path vertices run through the guide network */
     double guide_ms;                   /* summed duration of the inference launches */
+    uint32_t graph_replayed;           /* bsh_batch_run with use_graph: 1 when the run replayed the context's
+                                          previous graph (same jobs' shapes, tuning and buffers), 0 when it
+                                          captured one */
 } bsh_render_stats;
 
 /* ---------------- context ---------------- */

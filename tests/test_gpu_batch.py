@@ -41,10 +41,17 @@ def test_batch_equals_solo(ctx, use_graph):
         ctx.batch_run([j])
         solo.append(j.result())
     # a small chunk size forces chunks that mix jobs and split jobs across chunks; repeated
-    # runs (graph mode: one captured graph per run) with eager work in between
-    for rep in range(3):
+    # runs with eager work in between (graph mode: the first run captures the graph, the later
+    # ones replay it with new job tables and accumulation buffers), then a run with another
+    # chunk size, which must capture a new graph
+    for rep, chunk in enumerate((20000, 20000, 20000, 30000)):
         batch = [_make(spec) for spec in specs]
-        ctx.batch_run(batch, tuning={"paths_per_chunk": 20000, "trace_blocks_per_cu": 3, "use_graph": use_graph})
+        ctx.batch_run(batch, tuning={"paths_per_chunk": chunk, "trace_blocks_per_cu": 3, "use_graph": use_graph})
+        replayed = ctx.stats()["graph_replayed"]
+        if use_graph and rep in (1, 2):
+            assert replayed == 1, rep
+        if not use_graph or rep == 3:
+            assert replayed == 0, rep
         for k, (a, j) in enumerate(zip(solo, batch)):
             b = j.result()
             for x, y in zip(a, b):
@@ -52,6 +59,7 @@ def test_batch_equals_solo(ctx, use_graph):
                     np.testing.assert_array_equal(x, y, err_msg=f"rep {rep} job {k}")
         eager = _make(specs[-1])
         ctx.batch_run([eager])
+        assert ctx.stats()["graph_replayed"] == 0
     st = ctx.stats()
     assert st["rays_closest"] > 0
 
