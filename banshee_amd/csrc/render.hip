@@ -444,6 +444,34 @@ __global__ __launch_bounds__(kBlock) void k_trace_queue(RenderDev R, int L, int 
     }
 }
 
+// The trace stack with its top entry in a register: entries 0 .. sp-2 in LDS (past kTraceLds in
+// the global overflow), entry sp-1 in `top`. A pop returns the register at once and refills it
+// from LDS, so the next node's fetch does not wait for an LDS read.
+__device__ __forceinline__ void push_top(lu32* stk, gu32* ovf, int ostride, int& sp, uint32_t& top, uint32_t v)
+{
+    if (sp >= kMaxStack4) return;   // the builder bounds the depth, so this never drops an entry
+    if (sp > 0)
+    {
+        const int p = sp - 1;
+        if (p < kTraceLds) stk[p * kBlock] = top;
+        else ovf[(p - kTraceLds) * ostride] = top;
+    }
+    top = v;
+    ++sp;
+}
+
+__device__ __forceinline__ uint32_t pop_top(lu32* stk, gu32* ovf, int ostride, int& sp, uint32_t& top)
+{
+    const uint32_t v = top;
+    --sp;
+    if (sp > 0)
+    {
+        const int p = sp - 1;
+        top = p < kTraceLds ? stk[p * kBlock] : ovf[(p - kTraceLds) * ostride];
+    }
+    return v;
+}
+
 // 4-wide traversal with per-lane refill. The closest-hit queue and the any-hit queue form one
 // index range [0, nc + na) drawn by one work counter. Every lane holds its own ray state and
 // advances it by one node visit per loop iteration; a finished lane idles until `refill` or
@@ -471,7 +499,7 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
     f3 o = mk(0.f, 0.f, 0.f), d = mk(0.f, 0.f, 0.f);
     float ix = 0.f, iy = 0.f, iz = 0.f, ox = 0.f, oy = 0.f, oz = 0.f, tmin = 0.f, tcur = 0.f, hb1 = 0.f, hb2 = 0.f;
     int prim = -1, sp = 0;
-    uint32_t node = 0, own = 0;
+    uint32_t node = 0, own = 0, top = 0;
     // pending leaf triangles [tb, te), [r1, e1), [r2, e2), [r3, e3) and the last visit's internal
     // children (key = entry distance, kInf if missed), not yet culled and pushed while !sorted
     int tb = 0, te = 0, r1 = 0, e1 = 0, r2 = 0, e2 = 0, r3 = 0, e3 = 0;
@@ -649,13 +677,13 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
             if (n == 0)
             {
                 if (sp == 0) done = true;
-                else node = stack_pop<kTraceLds, kMaxStack4>(my, kBlock, ovf, ostride, sp);
+                else node = pop_top(my, ovf, ostride, sp, top);
             }
             else
             {
 #pragma unroll
                 for (int k = 3; k >= 1; --k)
-                    if (k < n) stack_push<kTraceLds, kMaxStack4>(my, kBlock, ovf, ostride, sp, id[k]);
+                    if (k < n) push_top(my, ovf, ostride, sp, top, id[k]);
                 node = id[0];
             }
             sorted = true;
