@@ -500,9 +500,11 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
     float ix = 0.f, iy = 0.f, iz = 0.f, ox = 0.f, oy = 0.f, oz = 0.f, tmin = 0.f, tcur = 0.f, hb1 = 0.f, hb2 = 0.f;
     int prim = -1, sp = 0;
     uint32_t node = 0, own = 0, top = 0;
-    // pending leaf triangles [tb, te), [r1, e1), [r2, e2), [r3, e3) and the last visit's internal
+    // the last visit's hit leaf children still to test (bit c: child c, tested in child order,
+    // triangle loff of the lowest one next) with that node's refs and counts, and its internal
     // children (key = entry distance, kInf if missed), not yet culled and pushed while !sorted
-    int tb = 0, te = 0, r1 = 0, e1 = 0, r2 = 0, e2 = 0, r3 = 0, e3 = 0;
+    uint32_t lmask = 0u;
+    int loff = 0, lr0 = 0, lr1 = 0, lr2 = 0, lr3 = 0, lc0 = 0, lc1 = 0, lc2 = 0, lc3 = 0;
     float key[4] = {0.f, 0.f, 0.f, 0.f};
     uint32_t id[4] = {0u, 0u, 0u, 0u};
     bool sorted = true;
@@ -569,7 +571,7 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
                     hb1 = hb2 = 0.f;
                     node = 0;
                     sp = 0;
-                    tb = te = r1 = e1 = r2 = e2 = r3 = e3 = 0;
+                    lmask = 0u;
                     sorted = true;
                     active = true;
                 }
@@ -592,11 +594,15 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
         // culls in), so a wave does either kind of work per lane per iteration instead of
         // looping over its busiest lane's triangles while the other lanes wait.
         bool done = false;
-        if (tb < te)
+        if (lmask)
         {
-            const float4 v0 = gload(tris, 3 * (size_t)tb + 0);
-            const float4 a1 = gload(tris, 3 * (size_t)tb + 1);
-            const float4 a2 = gload(tris, 3 * (size_t)tb + 2);
+            const int c = __builtin_ctz(lmask);
+            const int lref = c == 0 ? lr0 : c == 1 ? lr1 : c == 2 ? lr2 : lr3;
+            const int lcnt = c == 0 ? lc0 : c == 1 ? lc1 : c == 2 ? lc2 : lc3;
+            const int tq = lref + loff;
+            const float4 v0 = gload(tris, 3 * (size_t)tq + 0);
+            const float4 a1 = gload(tris, 3 * (size_t)tq + 1);
+            const float4 a2 = gload(tris, 3 * (size_t)tq + 2);
             if (kCount) (any ? vis_tris_any : vis_tris)++;
             float t, u, v;
             if (intersect_face(ld3(v0), ld3(a1), ld3(a2), o, d, tmin, tcur, t, u, v))
@@ -607,12 +613,10 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
                 hb2 = v;
                 if (any) done = true;
             }
-            if (++tb == te)
+            if (++loff == lcnt)
             {
-                tb = r1; te = e1;
-                r1 = r2; e1 = e2;
-                r2 = r3; e2 = e3;
-                r3 = 0; e3 = 0;
+                lmask &= lmask - 1u;
+                loff = 0;
             }
         }
         else
@@ -640,21 +644,16 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
                 key[c] = (hitc[c] && cn[c] == 0) ? tn : kInf;
                 id[c] = (uint32_t)ref[c];
             }
-            // the hit leaf children's triangles, in child order, as the ranges [tb, te), [r1, e1) ...
+            // the hit leaf children, tested from the next iteration on
+            lmask = 0u;
 #pragma unroll
-            for (int c = 3; c >= 0; --c)
-            {
-                if (hitc[c] && cn[c] > 0)
-                {
-                    r3 = r2; e3 = e2;
-                    r2 = r1; e2 = e1;
-                    r1 = tb; e1 = te;
-                    tb = ref[c]; te = ref[c] + cn[c];
-                }
-            }
+            for (int c = 0; c < 4; ++c) lmask |= (hitc[c] && cn[c] > 0) ? 1u << c : 0u;
+            loff = 0;
+            lr0 = ref[0]; lr1 = ref[1]; lr2 = ref[2]; lr3 = ref[3];
+            lc0 = cn[0]; lc1 = cn[1]; lc2 = cn[2]; lc3 = cn[3];
             sorted = false;
         }
-        if (!done && !sorted && tb >= te)
+        if (!done && !sorted && lmask == 0u)
         {
             // internal children still in front of the (possibly updated) closest hit, nearest
             // first: visit it, push the others
