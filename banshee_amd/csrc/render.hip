@@ -51,11 +51,11 @@ namespace bsh {
 constexpr int kBlock = 256;
 // Lights the bounce-0 NEE loop covers (two ray slots each in a 64-bit mask per path).
 constexpr int kMaxNeeLights = 32;
-// LDS entries of the wavefront trace kernel's traversal stack: all of the binary layout's
-// kMaxStack (32 x 4 B x 256 threads = 32 KB per block, 5 blocks per CU); the 4-wide
-// layout's deeper entries go to per-thread global slots. Measured on the 3M-triangle
-// terrain, more waves per SIMD with a smaller LDS part did not trace faster.
-constexpr int kTraceLds = 32;
+// LDS entries of the wavefront trace kernel's traversal stack (16 x 4 B x 256 threads = 16 KB
+// per block); deeper entries go to per-thread global slots. Two lanes' trace grids of 3 blocks
+// per CU then fit on a CU together with the shade kernels: measured on the 3M-triangle terrain
+// at 1080p, 32 entries 2855 Mrays/s, 24 3021, 20 3042, 16 3064, 12 3057, 8 3035.
+constexpr int kTraceLds = 16;
 // Training records are kept for the first kRecVerts vertices of a path, 5 float4 each.
 constexpr int kRecVerts = 4, kRecStride = 5 * kRecVerts;
 static_assert(kGuideSeg == kBlock, "guide row segments are the blocks of k_guide_inputs");
