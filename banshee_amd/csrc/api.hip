@@ -298,6 +298,15 @@ bsh_status bsh_scene_set_material_maps(bsh_scene* s, const int32_t* maps, size_t
     return BSH_OK;
 }
 
+bsh_status bsh_scene_set_normal_maps(bsh_scene* s, const int32_t* maps, size_t nrows)
+{
+    if (!s) return fail(BSH_ERR_INVALID, "bsh_scene_set_normal_maps: NULL scene");
+    if (!maps) s->nmaps.clear();
+    else s->nmaps.assign(maps, maps + nrows);
+    s->committed = false;
+    return BSH_OK;
+}
+
 bsh_status bsh_scene_set_transforms(bsh_scene* s, const float* m_minv, size_t n)
 {
     if (!s) return fail(BSH_ERR_INVALID, "bsh_scene_set_transforms: NULL scene");
@@ -459,7 +468,25 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
                                                  " takes no such map (colour maps: rows 0, 1, 3-6; kr maps: OrenNayar)");
         }
     }
-    if (!s->tri_uv.empty() || !s->mat_maps.empty()) s->ext_mats = 1;   // read in the kLights shade variants
+    if (!s->nmaps.empty())
+    {
+        // NormalMapping wraps one BSDF (bsdf/normal_mapping.h:40-56): rows 0, 1, 3, 4, 5
+        if (s->nmaps.size() != s->mats.size())
+            return fail(BSH_ERR_INVALID, "bsh_scene_commit: normal maps need one entry per material row");
+        for (size_t k = 0; k < s->nmaps.size(); ++k)
+        {
+            const int32_t id = s->nmaps[k];
+            if (id < -1 || (id >= 0 && (size_t)id >= s->tex_desc.size()))
+                return fail(BSH_ERR_INVALID, "bsh_scene_commit: material " + std::to_string(k) +
+                                                 " names a normal map out of range");
+            const int t = (int)s->mats[k].type;
+            if (id >= 0 && (t == BSH_EMISSIVE || t == BSH_GLASS || t == BSH_MIXED))
+                return fail(BSH_ERR_INVALID, "bsh_scene_commit: material " + std::to_string(k) +
+                                                 " takes no normal map (rows 0, 1, 3, 4, 5: one BSDF)");
+        }
+    }
+    if (!s->tri_uv.empty() || !s->mat_maps.empty() || !s->nmaps.empty())
+        s->ext_mats = 1;   // read in the kLights shade variants
     {
         std::vector<float4> uv(2 * std::max<size_t>(s->tri_uv.empty() ? 0 : nt, 1), make_float4(0.f, 0.f, 0.f, 0.f));
         if (!s->tri_uv.empty())
@@ -486,6 +513,10 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
         for (size_t k = 0; k < s->mat_maps.size() / 2; ++k) mm[k] = make_int2(s->mat_maps[2 * k], s->mat_maps[2 * k + 1]);
         BSH_HIP(s->d_mat_maps.alloc(mm.size()));
         BSH_HIP(hipMemcpy(s->d_mat_maps.p, mm.data(), mm.size() * sizeof(int2), hipMemcpyHostToDevice));
+        std::vector<int> nm(std::max<size_t>(s->mats.size(), 1), -1);
+        for (size_t k = 0; k < s->nmaps.size(); ++k) nm[k] = s->nmaps[k];
+        BSH_HIP(s->d_nmaps.alloc(nm.size()));
+        BSH_HIP(hipMemcpy(s->d_nmaps.p, nm.data(), nm.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     // lights: validate exactly as AreaLight's ctor does (arealight.cpp:16-19)
     std::vector<int32_t> bundle_light(s->bundles.size(), -1);

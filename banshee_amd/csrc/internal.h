@@ -206,6 +206,7 @@ struct bsh_scene
     std::vector<float> tex_texels;        // texture maps' texels
     std::vector<bsh_texture> tex_desc;
     std::vector<int32_t> mat_maps;        // per material row: colour map, kr map (empty: none)
+    std::vector<int32_t> nmaps;           // per material row: normal map, -1: none (empty: none)
     std::vector<float> env_tex;           // env_h rows of env_w RGB texels
     int env_w = 0, env_h = 0;
     bsh_camera cam{};
@@ -222,6 +223,7 @@ struct bsh_scene
     bsh::DevBuf<float> d_tex;
     bsh::DevBuf<int4> d_tex_desc;
     bsh::DevBuf<int2> d_mat_maps;
+    bsh::DevBuf<int> d_nmaps;
     int ext_mats = 0;
     bsh::EnvDev env{};
     std::vector<bsh::LightDev> h_lights;
@@ -250,6 +252,7 @@ struct bsh_scene
         s.tex = d_tex.p;
         s.tex_desc = d_tex_desc.p;
         s.mat_maps = mat_maps.empty() ? nullptr : d_mat_maps.p;
+        s.nmaps = nmaps.empty() ? nullptr : d_nmaps.p;
         s.ntris = (int)tri_v.size() / 3;
         return s;
     }

@@ -43,6 +43,30 @@ BSH_HD f3 tex_sample(const TexCtx& t, int id)
     return env_bilinear(t.tex + off, d.x, d.y, t.u, t.v);
 }
 
+// The closest texel with periodic wrap (the texture system's kPoint filter): texel
+// (floor(u w), floor(v h)).
+BSH_HD f3 tex_point(const TexCtx& t, int id)
+{
+    const int4 d = t.desc[id];
+    const size_t off = (size_t)(uint32_t)d.z | ((size_t)(uint32_t)d.w << 32);
+    const int x = wrap_index((int)floorf(t.u * (float)d.x), d.x), y = wrap_index((int)floorf(t.v * (float)d.y), d.y);
+    const float* p = t.tex + off + 3 * ((size_t)y * d.x + x);
+    return mk(p[0], p[1], p[2]);
+}
+
+// Bsdf::MapNormal (bsdf/bsdf.h:97-113) applied to normal n: the map's closest texel as the
+// tangent-space normal 2 texel - 1 in the frame (du, dv, n), du and dv Gram-Schmidt
+// orthogonalised from the hit's dpdu and dpdv against n, in the reference's operation order.
+BSH_HD f3 map_normal(const TexCtx& t, int id, const Frame& h, f3 n)
+{
+    const float ndotdu = dot(n, h.dpdu);
+    const f3 du = normalize(h.dpdu - ndotdu * n);
+    const float dudotdv = dot(du, h.dpdv);
+    const f3 dv = normalize(h.dpdv - ndotdu * n - dudotdv * du);
+    const f3 tn = normalize(2.f * tex_point(t, id) - mk(1.f, 1.f, 1.f));
+    return normalize(n * tn.z + du * tn.x - dv * tn.y);
+}
+
 BSH_HD void apply_maps(const TexCtx& t, int row, Mat& m)
 {
     if (!t.maps) return;

@@ -87,7 +87,8 @@ enum : uint32_t
     F_NEE = 2,      // NEE rays in flight
     F_TERM = 4,     // path ended; finalise once NEE results are in
     F_DONE = 8,     // final radiance written
-    F_NOCLAMP = 16  // primary miss: GetLi returns bg + sum Le unclamped (gitracer.cpp:22-33)
+    F_NOCLAMP = 16, // primary miss: GetLi returns bg + sum Le unclamped (gitracer.cpp:22-33)
+    F_NMAP = 32     // the NEE rays in flight come from a normal-mapped vertex (reference estimator)
 };
 
 enum : uint32_t { Q_EXT = 0, Q_NEE_CLOSEST = 1, Q_NEE_ANY = 2 };
@@ -787,10 +788,20 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                     {
                         if (R.nee_res[ks].x == 0) local = local + ld3(R.nee_a[ks]);
                     }
-                    if (mask & (1ull << (2 * j + 1)))
+                    // a normal-mapped vertex traced two BSDF rays for this light: the one that saw
+                    // the normal mapped twice (its light sample was unoccluded, so Evaluate ran
+                    // first) in slot 2j + 1, the other in slot 2 ncnt + 2j + 1
+                    int bbit = 2 * j + 1;
+                    size_t bks = ks + 1;
+                    if (kLights && (fl & F_NMAP) && (mask & (1ull << (2 * j))) && R.nee_res[ks].x != 0)
                     {
-                        int4 r = R.nee_res[ks + 1];
-                        float4 nb = R.nee_b[ks + 1];
+                        bbit = 2 * ncnt + 2 * j + 1;
+                        bks = (size_t)i * K + bbit;
+                    }
+                    if (mask & (1ull << bbit))
+                    {
+                        int4 r = R.nee_res[bks];
+                        float4 nb = R.nee_b[bks];
                         int Lid = (int)nb.z;
                         if (r.x >= 0)
                         {
@@ -803,7 +814,7 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                 Mat lm;
                                 tri_verts(S, r.x, p1, p2, p3, n1, n2, n3, lm);
                                 Frame hh = fill_hit(p1, p2, p3, n1, n2, n3, __int_as_float(r.z), __int_as_float(r.w));
-                                f3 wi = ld3(R.nee_wi[ks + 1]);
+                                f3 wi = ld3(R.nee_wi[bks]);
                                 f3 d = hh.p - ps;
                                 f3 le = mk(0.f, 0.f, 0.f);
                                 if (dot(hh.n, -wi) > 0.f)
@@ -814,7 +825,7 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                 }
                                 if (sqnorm(le) > 0.f)
                                 {
-                                    float4 na = R.nee_a[ks + 1];
+                                    float4 na = R.nee_a[bks];
                                     local = local + le * ld3(na) * na.w * nb.x * nb.y;
                                 }
                             }
@@ -823,10 +834,10 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                         {
                             // the ray left the scene: the light's GetLe, nonzero for environment
                             // lights (ditracer.cpp:218-222)
-                            f3 le = other_light_le(S, S.lights[Lid], ld3(R.nee_wi[ks + 1]));
+                            f3 le = other_light_le(S, S.lights[Lid], ld3(R.nee_wi[bks]));
                             if (sqnorm(le) > 0.f)
                             {
-                                float4 na = R.nee_a[ks + 1];
+                                float4 na = R.nee_a[bks];
                                 local = local + le * ld3(na) * na.w * nb.x * nb.y;
                             }
                         }
@@ -836,7 +847,7 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                     if (pb == 0 || nq) rad = rad + T * local;
                     else rad = rad + (T * local) * (float)nl;
                 }
-                fl &= ~F_NEE;
+                fl &= ~(F_NEE | F_NMAP);
             }
             // record slot of path vertex b: b, or b + 1 on guided probe paths, whose slot 0 is
             // the probe itself
@@ -881,7 +892,21 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                     }
                     else
                         h = fill_hit(p1, p2, p3, n1, n2, n3, __int_as_float(hr.z), __int_as_float(hr.w));
-                    if (kLights && S.mat_maps) apply_maps(tx, __float_as_int(S.stri[(size_t)kShadeF4 * hr.x].w), m);
+                    const int mrow = kLights ? __float_as_int(S.stri[(size_t)kShadeF4 * hr.x].w) : 0;
+                    if (kLights && S.mat_maps) apply_maps(tx, mrow, m);
+                    // NormalMapping (bsdf/normal_mapping.h:57-80): nA, nB = the hit normal mapped
+                    // once and twice. The reference estimator keeps the unmapped normal for light
+                    // sampling and uses nA / nB as Sample and Evaluate leave them (see the NEE loop);
+                    // the quirk-free one maps it once for every use.
+                    const int nmap = kLights && S.nmaps ? S.nmaps[mrow] : -1;
+                    f3 nA = h.n, nB = h.n;
+                    if (kLights && nmap >= 0)
+                    {
+                        nA = map_normal(tx, nmap, h, h.n);
+                        if (nq) h.n = nA;
+                        else nB = map_normal(tx, nmap, h, nA);
+                    }
+                    const bool nmref = kLights && nmap >= 0 && !nq;
                     f3 wo = -ld3(R.ray_d[i]);
                     uint32_t base = meta.x;
                     f3 T = ld3(R.thr[i]);
@@ -952,14 +977,20 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                 {
                                     f3 wi = normalize(ld);
                                     float dist = sqrtf(sqnorm(ld));
-                                    f3 f = kLights ? mat_eval_x(S.mats, tx, m, h, wi, wo) : mat_eval(m, h, wi, wo);
+                                    // a normal-mapped vertex: Evaluate maps the normal once (nA, also
+                                    // for the cosine) and GetPdf maps that copy again (nB)
+                                    Frame hE = h;
+                                    if (nmref) hE.n = nA;
+                                    f3 f = kLights ? mat_eval_x(S.mats, tx, m, hE, wi, wo) : mat_eval(m, h, wi, wo);
                                     float w = 1.f;
                                     if (!singular)
                                     {
                                         // the reference asks GetPdf(light dir, view), the density of the
                                         // view direction; MIS needs the density of sampling the light
                                         // direction from the view, GetPdf(view, light dir)
-                                        float bp = kLights ? (nq ? mat_pdf_x(S.mats, tx, m, h, wo, wi) : mat_pdf_x(S.mats, tx, m, h, wi, wo))
+                                        Frame hP = h;
+                                        if (nmref) hP.n = nB;
+                                        float bp = kLights ? (nq ? mat_pdf_x(S.mats, tx, m, h, wo, wi) : mat_pdf_x(S.mats, tx, m, hP, wi, wo))
                                                            : (nq ? mat_pdf(m, h, wo, wi) : mat_pdf(m, h, wi, wo));
                                         // quirk-free: both MIS weights in solid angle (the reference
                                         // weighs an area light's area pdf against the BSDF's solid-angle
@@ -972,45 +1003,73 @@ __global__ __launch_bounds__(kBlock, kShadeWavesPerSimd) void k_shade(RenderDev 
                                     // radiance is GetLe = e cos_l (what the camera and the BSDF branch
                                     // see), so the quirk-free estimator adds the Jacobian's cos_l
                                     if (nq && area) le = le * cosl;
-                                    f3 c = le * f * fabsf(dot(h.n, wi)) * w * (1.f / lpdf);
+                                    f3 c = le * f * fabsf(dot(hE.n, wi)) * w * (1.f / lpdf);
                                     if (nq) c = c * pickw;
                                     R.nee_a[ks] = st3(c, 0.f);
                                     R.nee_wi[ks] = st3(wi, dist - 0.01f);
                                     emit_mask |= 1ull << (2 * j);
                                 }
-                                // BSDF sample + MIS closest ray
-                                f3 wis = mk(0.f, 0.f, 0.f);
-                                float bpdf = 0.f;
-                                int type = 0;
-                                f3 fb = singular ? mk(0.f, 0.f, 0.f)
-                                                 : kLights ? mat_sample_x(S.mats, tx, m, h, bx, by, wo, wis, bpdf, type, rs)
-                                                           : mat_sample(m, h, bx, by, wo, wis, bpdf, type);
-                                f3 wi = normalize(wis);
-                                if (!singular && sqnorm(fb) > 0.f && (nq ? bpdf > 0.f : bpdf > kMinPdf))
-                                {
-                                    // a specular BSDF sample takes weight 1 without asking the light's
-                                    // pdf (ditracer.cpp:167-181)
-                                    const bool spec = kLights && (type & BSDF_SPECULAR) != 0;
-                                    float lp = spec ? 0.f : area ? light_pdf(L, h.p, wi) : other_light_pdf(S, *L, h.n, wi);
-                                    // quirk-free: where an environment light's density is 0 the BSDF
-                                    // sample carries the whole weight (an area light's density is 0
-                                    // only where the ray cannot reach it)
-                                    if (spec || (nq ? lp > 0.f || !area : !(lp < kMinPdf)))
+                                // BSDF sample + MIS closest ray, sampled at frame hs into NEE slot `bit`
+                                auto bsdf_nee = [&](const Frame& hs, int bit) {
+                                    f3 wis = mk(0.f, 0.f, 0.f);
+                                    float bpdf = 0.f;
+                                    int type = 0;
+                                    f3 fb = singular ? mk(0.f, 0.f, 0.f)
+                                                     : kLights ? mat_sample_x(S.mats, tx, m, hs, bx, by, wo, wis, bpdf, type, rs)
+                                                               : mat_sample(m, hs, bx, by, wo, wis, bpdf, type);
+                                    f3 wi = normalize(wis);
+                                    if (!singular && sqnorm(fb) > 0.f && (nq ? bpdf > 0.f : bpdf > kMinPdf))
                                     {
-                                        float w = spec || (nq && !(lp > 0.f)) ? 1.f : power_heuristic(bpdf, lp);
-                                        R.nee_a[ks + 1] = st3(fb, fabsf(dot(h.n, wi)));
-                                        R.nee_b[ks + 1] = make_float4(nq ? w * pickw : w, 1.f / bpdf, (float)Lid, 0.f);
-                                        R.nee_wi[ks + 1] = st3(wi, 10000000.f);
-                                        emit_mask |= 1ull << (2 * j + 1);
+                                        // a specular BSDF sample takes weight 1 without asking the light's
+                                        // pdf (ditracer.cpp:167-181)
+                                        const bool spec = kLights && (type & BSDF_SPECULAR) != 0;
+                                        float lp = spec ? 0.f : area ? light_pdf(L, hs.p, wi) : other_light_pdf(S, *L, hs.n, wi);
+                                        // quirk-free: where an environment light's density is 0 the BSDF
+                                        // sample carries the whole weight (an area light's density is 0
+                                        // only where the ray cannot reach it)
+                                        if (spec || (nq ? lp > 0.f || !area : !(lp < kMinPdf)))
+                                        {
+                                            float w = spec || (nq && !(lp > 0.f)) ? 1.f : power_heuristic(bpdf, lp);
+                                            const size_t kb = (size_t)i * K + bit;
+                                            R.nee_a[kb] = st3(fb, fabsf(dot(hs.n, wi)));
+                                            R.nee_b[kb] = make_float4(nq ? w * pickw : w, 1.f / bpdf, (float)Lid, 0.f);
+                                            R.nee_wi[kb] = st3(wi, 10000000.f);
+                                            emit_mask |= 1ull << bit;
+                                        }
+                                    }
+                                };
+                                if (nmref)
+                                {
+                                    // Sample maps the normal as the light branch left it: twice where
+                                    // the light sample was unoccluded (Evaluate ran), else once. The
+                                    // shadow ray decides later, so both rays are traced.
+                                    Frame hs = h;
+                                    if ((emit_mask >> (2 * j)) & 1ull)
+                                    {
+                                        hs.n = nB;
+                                        bsdf_nee(hs, 2 * j + 1);
+                                        hs.n = nA;
+                                        bsdf_nee(hs, 2 * jn + 2 * j + 1);
+                                    }
+                                    else
+                                    {
+                                        hs.n = nA;
+                                        bsdf_nee(hs, 2 * j + 1);
                                     }
                                 }
+                                else
+                                    bsdf_nee(h, 2 * j + 1);
                             }
                             R.thr_nee[i] = st3(T, 0.f);
                             R.ray_o[i] = st3(h.p, 0.f);
                             R.nee_mask[i] = emit_mask;
                             if (emit_mask) fl |= F_NEE;
+                            if (emit_mask && nmref) fl |= F_NMAP;
                             ncnt = jn;
                         }
+                        // GiTracer's own Sample maps the hit's normal once (gitracer.cpp:97), and
+                        // the throughput's cosine reads it
+                        if (nmref) h.n = nA;
                         // ---- continuation: BSDF sample, throughput, Russian roulette ----
                         if (b + 1 < J.depth)
                         {
@@ -1785,6 +1844,11 @@ bsh_status make_render_job(bsh_scene* s, const bsh_render_params* p, float* accu
     if (p->chunk_spp != 0)
         return fail(BSH_ERR_INVALID, "bsh_render: chunk_spp is reserved and must be 0 (bound the paths per "
                                      "chunk with bsh_tuning.paths_per_chunk)");
+    if (!s->nmaps.empty() && s->lights.size() + s->others.size() > (size_t)kMaxNeeLights / 2 &&
+        !(p->flags & BSH_RENDER_NO_QUIRKS))
+        return fail(BSH_ERR_UNSUPPORTED, "bsh_render: normal maps with more than " + std::to_string(kMaxNeeLights / 2) +
+                                             " lights need BSH_RENDER_NO_QUIRKS (the bounce-0 loop traces two BSDF "
+                                             "rays per light)");
     if (s->lights.size() + s->others.size() > (size_t)kMaxNeeLights && !(p->flags & BSH_RENDER_NO_QUIRKS))
         return fail(BSH_ERR_UNSUPPORTED, "bsh_render: GiTracer's bounce-0 loop over every light supports at most 32 "
                                          "lights (BSH_RENDER_NO_QUIRKS picks one light per vertex by power)");
@@ -1853,6 +1917,10 @@ bsh_status make_probe_job(bsh_scene* s, const float* probes, size_t nprobes, con
                                      "chunk with bsh_tuning.paths_per_chunk)");
     if ((uint64_t)nprobes * (uint64_t)N > 0xffffffffull)
         return fail(BSH_ERR_UNSUPPORTED, "bsh_bake_sh: more than 2^32 probe rays");
+    if (!s->nmaps.empty() && s->lights.size() + s->others.size() > (size_t)kMaxNeeLights / 2 &&
+        !(p->flags & BSH_RENDER_NO_QUIRKS))
+        return fail(BSH_ERR_UNSUPPORTED, "bsh_bake_sh: normal maps with more than " + std::to_string(kMaxNeeLights / 2) +
+                                             " lights need BSH_RENDER_NO_QUIRKS");
     if (s->lights.size() + s->others.size() > (size_t)kMaxNeeLights && !(p->flags & BSH_RENDER_NO_QUIRKS))
         return fail(BSH_ERR_UNSUPPORTED, "bsh_bake_sh: GiTracer's bounce-0 loop over every light supports at most 32 "
                                          "lights (BSH_RENDER_NO_QUIRKS picks one light per vertex by power)");
@@ -1889,15 +1957,19 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     int nlmax = 1;
     uint32_t maxunit = 1;
     uint64_t samples = 0;
+    bool nmaps = false;
     for (const JobDev& J : hjobs)
     {
-        // NEE slots: two per light of the bounce-0 loop, two for a picked light
+        // NEE slots: two per light of the bounce-0 loop, two for a picked light (four with
+        // normal maps: the second BSDF ray of a normal-mapped vertex)
         nlmax = std::max(nlmax, (J.flags & JOB_NO_QUIRKS) && J.s.nlights > kMaxNeeLights ? 1 : J.s.nlights);
+        nmaps = nmaps || (J.s.nmaps && !(J.flags & JOB_NO_QUIRKS));
         maxunit = std::max<uint32_t>(maxunit, J.type == 0 ? (uint32_t)J.s_count : (uint32_t)J.N);
         samples += J.type == 0 ? (uint64_t)J.cw * J.ch * J.s_count : (uint64_t)J.nprobes * J.N;
     }
     int Kshift = 1;
-    while ((1 << Kshift) < 2 * nlmax) ++Kshift;
+    while ((1 << Kshift) < (nmaps ? 4 : 2) * nlmax) ++Kshift;
+    if (Kshift > 6) return fail(BSH_ERR_UNSUPPORTED, "more NEE slots per path than the 64-bit slot mask holds");
     const int K = 1 << Kshift;
     size_t pmax = std::max<size_t>((size_t)tune.paths_per_chunk, maxunit);
     if (pmax * (size_t)K >= ((size_t)1 << 29)) pmax = (((size_t)1 << 29) - 1) / K;

@@ -66,6 +66,7 @@ struct SceneDev
     const float* tex;         // texture maps' RGB texels
     const int4* tex_desc;     // per map: width, height, first float (low, high 32 bits)
     const int2* mat_maps;     // per material row: colour map, kr map (-1: none); NULL when none
+    const int* nmaps;         // per material row: normal map (-1: none); NULL when no row has one
     int nlights;
     int narea;                // lights [0, narea) are area lights, the rest point/directional/environment
     int ext_mats;             // 1 when a material row is of type 3-7 (material_dev.h)

@@ -148,7 +148,7 @@ EXPORTS = [
     "bsh_context_launch_stats", "bsh_diag_stream_read", "bsh_import_obj", "bsh_sh_project_env_noalloc",
     "bsh_diag_samples", "bsh_scene_set_other_lights", "bsh_scene_set_env_map", "bsh_diag_light_full",
     "bsh_diag_material", "bsh_guide_get_state", "bsh_guide_set_state", "bsh_scene_set_transforms",
-    "bsh_scene_set_uvs", "bsh_scene_set_textures", "bsh_scene_set_material_maps",
+    "bsh_scene_set_uvs", "bsh_scene_set_textures", "bsh_scene_set_material_maps", "bsh_scene_set_normal_maps",
 ]
 
 
@@ -667,6 +667,10 @@ class Scene:
             m2 = np.ascontiguousarray(mm, np.int32).reshape(-1, 2)
             _check(lib().bsh_scene_set_material_maps(self.h, _p(m2), c_size_t(m2.shape[0])),
                    "bsh_scene_set_material_maps")
+        nm = getattr(sc, "normal_maps", None)
+        if nm is not None:
+            n1 = np.ascontiguousarray(nm, np.int32).ravel()
+            _check(lib().bsh_scene_set_normal_maps(self.h, _p(n1), c_size_t(n1.size)), "bsh_scene_set_normal_maps")
         xf = getattr(sc, "transforms", None)
         if xf is not None:
             x = np.ascontiguousarray(xf, np.float32).reshape(-1, 32)

@@ -69,6 +69,8 @@ class Scene:
     tri_uv: np.ndarray | None = None
     textures: list | None = None
     mat_maps: np.ndarray | None = None
+    # per material row the texture map used as its normal map (NormalMapping), -1 for none
+    normal_maps: np.ndarray | None = None
 
     @property
     def ntris(self) -> int:
@@ -79,7 +81,8 @@ class Scene:
         for a in (self.verts, self.normals, self.tri_v, self.tri_n, self.tri_m, self.bundles,
                   self.materials, self.lights, self.camera):
             h.update(np.ascontiguousarray(a).tobytes())
-        for a in (self.other_lights, self.env, self.transforms, self.uvs, self.tri_uv, self.mat_maps):
+        for a in (self.other_lights, self.env, self.transforms, self.uvs, self.tri_uv, self.mat_maps,
+                  self.normal_maps):
             if a is not None:
                 h.update(np.ascontiguousarray(a, np.float32).tobytes())
         for t in self.textures or ():
@@ -691,3 +694,19 @@ def textured_cornell(width=32, height=32, seed=7) -> Scene:
     out.mat_maps = maps
     out.name = "textured_cornell"
     return out
+
+
+def normal_mapped_cornell(width=32, height=32, seed=7) -> Scene:
+    """textured_cornell with a normal map (a 16 x 16 bump pattern, tangent-space normals tilted
+    up to 50 degrees) on the floor (Lambert), the back wall (Microfacet), the left wall
+    (OrenNayar) and the second box (Lambert), read as NormalMapping reads them."""
+    out = textured_cornell(width, height, seed)
+    y, x = np.meshgrid(np.arange(16), np.arange(16), indexing="ij")
+    bump = np.stack([0.5 + 0.35 * np.sin(2 * np.pi * x / 8), 0.5 + 0.35 * np.cos(2 * np.pi * y / 5),
+                     np.full(x.shape, 0.8)], -1).astype(np.float32)
+    out.textures = list(out.textures) + [bump]
+    nm = -np.ones(out.materials.shape[0], np.int32)
+    nm[[0, 2, 3, 6]] = len(out.textures) - 1
+    out.normal_maps = nm
+    return out
+

@@ -260,6 +260,15 @@ bsh_status bsh_scene_set_textures(bsh_scene* s, const float* texels, size_t nflo
    MixedMaterial rows take none (a mixture's components carry their own). Normal maps
    (bsdf/normal_mapping.h) are not supported. */
 bsh_status bsh_scene_set_material_maps(bsh_scene* s, const int32_t* maps, size_t nrows);
+/* NormalMapping(bsdf, nmap) (bsdf/normal_mapping.h:40-90) around the single BSDF of material
+   rows 0, 1, 3, 4, 5: per row a map index or -1. Bsdf::MapNormal (bsdf/bsdf.h:97-113) reads the
+   map's closest texel at the hit's (u, v) as a tangent-space normal 2 texel - 1 in the frame of
+   dpdu, dpdv and n. As in the reference, Sample and Evaluate rewrite the hit's normal in place
+   and GetPdf maps a copy, so within one light's GetDi the BSDF sample sees the normal mapped
+   twice when the light sample was unoccluded, once otherwise; the wavefront traces both BSDF
+   rays and keeps the one the shadow ray selects (at most 16 lights at bounce 0). The quirk-free
+   estimator maps the normal once for every use. NULL clears them. */
+bsh_status bsh_scene_set_normal_maps(bsh_scene* s, const int32_t* maps, size_t nrows);
 /* World::lights_ (AreaLight only) */
 bsh_status bsh_scene_set_lights(bsh_scene* s, const bsh_area_light* lights, size_t n);
 

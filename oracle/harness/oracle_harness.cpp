@@ -53,6 +53,7 @@
 #include "bsdf/orennayar.h"
 #include "bsdf/perfect_reflect.h"
 #include "bsdf/perfect_refract.h"
+#include "bsdf/normal_mapping.h"
 #include "material/glass.h"
 #include "material/mixedmaterial.h"
 #include "light/arealight.h"
@@ -398,12 +399,19 @@ Bsdf* make_bsdf(TextureSystem const& ts, float const* rows, int i, int const* ma
 
 /* Material row (bsh_material): 0 Lambert, 1 Microfacet, 2 Emissive, 3 OrenNayar, 4 PerfectReflect,
    5 PerfectRefract, 6 Glass, 7 MixedMaterial over the rows m[5], m[6], m[7] (-1: none). */
+/* Per material row a normal map ("map<k>") that wraps the row's BSDF in NormalMapping
+   (bsdf/normal_mapping.h:40-90), or -1; set only while oref_scene_create_normal_mapped builds
+   the materials. */
+int const* g_row_nmaps = nullptr;
+
 Material* make_material(float const* rows, int i, TextureSystem const& ts = g_texsys, int const* maps = nullptr)
 {
     float const* m = rows + 8 * i;
     int type = (int)m[0];
     float3 c(m[1], m[2], m[3]);
     if (type == 2) return new Emissive(c);
+    if (g_row_nmaps && g_row_nmaps[i] >= 0 && type != 6 && type != 7)
+        return new SimpleMaterial(new NormalMapping(make_bsdf(ts, rows, i, maps), "map" + std::to_string(g_row_nmaps[i])));
     if (type == 6) return new Glass(ts, m[4], c, row_maps(maps, i).color);
     if (type == 7)
     {
@@ -518,6 +526,23 @@ void* oref_scene_create_textured(float const* verts, int nverts, float const* no
     s->world.Commit();
     s->accel = new RecordingAccel(std::move(s->world.accel_));
     s->world.accel_.reset(s->accel);
+    return s;
+}
+
+/* oref_scene_create_textured with a normal map per material row (-1: none) around the rows'
+   single BSDFs. */
+void* oref_scene_create_normal_mapped(float const* verts, int nverts, float const* normals, int nnormals,
+                                      int const* tri_v, int const* tri_n, int const* tri_m, int ntris,
+                                      int const* bundles, int nbundles, float const* materials, int nmaterials,
+                                      int const* lights, int nlights, float const* cam, float const* uvs, int nuvs,
+                                      int const* tri_uv, float const* texels, int const* tex_desc, int ntex,
+                                      int const* mat_maps, int const* nmaps)
+{
+    g_row_nmaps = nmaps;
+    void* s = oref_scene_create_textured(verts, nverts, normals, nnormals, tri_v, tri_n, tri_m, ntris, bundles,
+                                         nbundles, materials, nmaterials, lights, nlights, cam, uvs, nuvs, tri_uv,
+                                         texels, tex_desc, ntex, mat_maps);
+    g_row_nmaps = nullptr;
     return s;
 }
 

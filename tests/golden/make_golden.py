@@ -49,6 +49,7 @@ MATERIAL_SEED = 21
 MATERIAL_FRAME = dict(W=24, grid=4, depth=5, seed=3)
 # the Cornell box with texture coordinates and maps (banshee_amd.scenes.textured_cornell)
 TEXTURED = dict(W=32, grid=4, depth=5, seed=8)
+NORMAL_MAPPED = dict(W=32, grid=4, depth=5, seed=9)
 # the Cornell box with transformed bundles (banshee_amd.scenes.transformed_cornell), whole frame
 TRANSFORMED = dict(W=32, grid=4, depth=5, seed=6)
 # the open scene with every light type (banshee_amd.scenes.open_scene), whole frames
@@ -312,6 +313,18 @@ def main(only=()):
                                                   X["grid"], X["depth"], X["seed"], per_sample=True)
         print(f"textures: rays={rays}")
         np.savez_compressed(os.path.join(HERE, "textures.npz"), digest=xs.digest(), frame=PS, ray_counts=rays,
+                            **{k: v for k, v in X.items()})
+
+    # ---------------- normal maps (NormalMapping around single-BSDF rows) ----------------
+    if want("normalmaps"):
+        from banshee_amd.scenes import normal_mapped_cornell
+        X = NORMAL_MAPPED
+        xs = normal_mapped_cornell(X["W"], X["W"])
+        so = o.scene(xs)
+        _, _, _, PS, rays = so.render_keyed_frame(X["W"], X["W"], ("stratified", "stratified", "stratified"),
+                                                  X["grid"], X["depth"], X["seed"], per_sample=True)
+        print(f"normalmaps: rays={rays}")
+        np.savez_compressed(os.path.join(HERE, "normalmaps.npz"), digest=xs.digest(), frame=PS, ray_counts=rays,
                             **{k: v for k, v in X.items()})
 
     for f in sorted(os.listdir(HERE)):

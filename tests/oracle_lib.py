@@ -62,6 +62,8 @@ def _load(kind):
             lib.oref_scene_create_textured.restype = vp
             lib.oref_scene_create_textured.argtypes = [vp, i, vp, i, vp, vp, vp, i, vp, i, vp, i, vp, i, vp,
                                                        vp, i, vp, vp, vp, i, vp]
+            lib.oref_scene_create_normal_mapped.restype = vp
+            lib.oref_scene_create_normal_mapped.argtypes = lib.oref_scene_create_textured.argtypes + [vp]
             lib.oref_scene_set_env.argtypes = [vp, vp, i, i]
             lib.oref_material.argtypes = [vp, i, ctypes.c_uint, i, vp, vp, vp, vp, vp, vp]
             lib.oref_scene_set_transforms.argtypes = [vp, vp, i]
@@ -185,7 +187,7 @@ class OracleScene:
         self._keep = [f32(sc.verts), f32(sc.normals), i32(sc.tri_v), i32(sc.tri_n), i32(sc.tri_m),
                       i32(sc.bundles), f32(sc.materials), i32(sc.lights), f32(sc.camera)]
         k = self._keep
-        textured = any(getattr(sc, a, None) is not None for a in ("uvs", "textures", "mat_maps"))
+        textured = any(getattr(sc, a, None) is not None for a in ("uvs", "textures", "mat_maps", "normal_maps"))
         if textured:
             if orc.kind != "ref":
                 raise RuntimeError("texture coordinates and maps need the reference oracle")
@@ -199,11 +201,15 @@ class OracleScene:
                 off += t.size
             desc = i32(desc) if texs else None
             mm = i32(sc.mat_maps).reshape(-1, 2) if sc.mat_maps is not None else None
-            self._keep += [a for a in (uv, tu, flat, desc, mm) if a is not None]
-            self.h = orc.lib.oref_scene_create_textured(
-                _p(k[0]), k[0].shape[0], _p(k[1]), k[1].shape[0], _p(k[2]), _p(k[3]), _p(k[4]), k[2].shape[0],
-                _p(k[5]), k[5].shape[0], _p(k[6]), k[6].shape[0], _p(k[7]), k[7].shape[0], _p(k[8]),
-                _p(uv), 0 if uv is None else uv.shape[0], _p(tu), _p(flat), _p(desc), len(texs), _p(mm))
+            nm = i32(sc.normal_maps).ravel() if getattr(sc, "normal_maps", None) is not None else None
+            self._keep += [a for a in (uv, tu, flat, desc, mm, nm) if a is not None]
+            args = (_p(k[0]), k[0].shape[0], _p(k[1]), k[1].shape[0], _p(k[2]), _p(k[3]), _p(k[4]), k[2].shape[0],
+                    _p(k[5]), k[5].shape[0], _p(k[6]), k[6].shape[0], _p(k[7]), k[7].shape[0], _p(k[8]),
+                    _p(uv), 0 if uv is None else uv.shape[0], _p(tu), _p(flat), _p(desc), len(texs), _p(mm))
+            if nm is not None:
+                self.h = orc.lib.oref_scene_create_normal_mapped(*args, _p(nm))
+            else:
+                self.h = orc.lib.oref_scene_create_textured(*args)
         else:
             self.h = orc._fn("scene_create")(
                 _p(k[0]), k[0].shape[0], _p(k[1]), k[1].shape[0], _p(k[2]), _p(k[3]), _p(k[4]), k[2].shape[0],
