@@ -99,9 +99,9 @@ def rmse_estimate(a) -> float:
     return float(np.sqrt(np.mean(var)))
 
 
-def time_to_rmse(ctx, scene, guide, max_passes, seed0) -> dict:
+def time_to_rmse(ctx, scene, guide, max_passes, seed0, target=G_TARGET, keep=False) -> dict:
     """Renders passes of G_PASS_GRID^2 spp into one accumulation buffer until the RMSE
-    estimate reaches G_TARGET. With a guide, every pass also writes training records and
+    estimate reaches `target` (keep: also return the accumulation buffer as "image"). With a guide, every pass also writes training records and
     runs G_STEPS Adam steps on G_BATCH of them; every pass's samples count in the image.
     Returns the seconds of render + training, spp, rmse, training seconds, guided network rows,
     and the summed launch times (ms) of trace, shade and inference with the BVH queries. The
@@ -137,10 +137,11 @@ def time_to_rmse(ctx, scene, guide, max_passes, seed0) -> dict:
         k += 1
         if k % G_CHECK == 0:
             e = rmse_estimate(acc.numpy())
-            if e <= G_TARGET:
+            if e <= target:
                 break
     spp = k * G_PASS_GRID ** 2
     return {"s": clock, "spp": spp, "rmse": e, "train_s": train, "rows": tot["guide_rows"],
+            "image": acc.numpy() if keep else None,
             "infer_ms": tot["guide_ms"], "ms_per_spp": clock * 1e3 / spp,
             "trace_ms_per_spp": tot["trace_ms"] / spp, "shade_ms_per_spp": tot["shade_ms"] / spp,
             "rays_per_sample": tot["rays"] / (spp * G_RES * G_RES)}
@@ -214,6 +215,45 @@ def guided_comparison(ctx, bvh_width) -> dict:
             "guide": {"bsdf_fraction": G_ALPHA, "warmup_passes": G_WARM, "lr": G_LR, "guided_bounces": G_BOUNCES,
                       "adam_steps_per_pass": [G_STEPS, G_STEPS_LATE, G_LATE], "batch": G_BATCH,
                       "spp_per_pass": G_PASS_GRID ** 2}}
+
+
+def rmse_check(ctx, bvh_width, ref_spp=4096) -> dict:
+    """The guided-vs-plain comparison at the headline RMSE target and at half of it, each run's
+    final image also scored against a converged plain render of ref_spp samples per pixel:
+    RMSE_true = sqrt(mean (m - m_ref)^2 - mean var(m_ref)), the reference's own variance of the
+    mean taken out. Not part of the default line (about a minute of GPU time)."""
+    import numpy as np
+    from banshee_amd import hip
+    from banshee_amd.scenes import occluded_room
+    sc = occluded_room(width=G_RES, height=G_RES)
+    scene = hip.Scene(ctx, sc, width=bvh_width)
+    ref = ctx.alloc((G_RES, G_RES, 8), np.float32)
+    for k in range(ref_spp // 64):
+        scene.render_guided(G_RES, G_RES, 8, G_DEPTH, 900000 + k, accum=ref, accumulate=k > 0, download=False)
+    ctx.sync()
+    r = ref.numpy().astype(np.float64)
+    nr = r[..., 3:4]
+    mr = r[..., 0:3] / nr
+    vr = np.maximum(r[..., 4:7] / nr - mr * mr, 0.0) / np.maximum(nr - 1.0, 1.0)
+
+    def score(res):
+        a = res.pop("image").astype(np.float64)
+        n = a[..., 3:4]
+        m = a[..., 0:3] / n
+        res["rmse_true"] = float(np.sqrt(max(np.mean((m - mr) ** 2) - np.mean(vr), 0.0)))
+        return {k: (round(v, 5) if isinstance(v, float) else v) for k, v in res.items()
+                if k in ("s", "spp", "rmse", "rmse_true", "train_s")}
+
+    out = {"workload": f"occluded room ({sc.ntris} tris), {G_RES}x{G_RES}, depth {G_DEPTH}, quirk-free estimator",
+           "reference_spp": ref_spp, "reference_rmse_estimate": round(float(np.sqrt(np.mean(vr))), 6), "targets": {}}
+    for i, target in enumerate((G_TARGET, G_TARGET / 2)):
+        p = score(time_to_rmse(ctx, scene, None, 4096, 100000 + 10000 * i, target, keep=True))
+        guide = hip.Guide(ctx, seed=1, bsdf_fraction=1.0, lr=G_LR, beta2=0.99, guided_bounces=G_BOUNCES)
+        g = score(time_to_rmse(ctx, scene, guide, 4096, 200000 + 10000 * i, target, keep=True))
+        guide.close()
+        out["targets"][str(target)] = {"plain": p, "guided": g, "time_ratio": round(g["s"] / p["s"], 3),
+                                       "rmse_true_ratio_at_end": round(g["rmse_true"] / p["rmse_true"], 3)}
+    return out
 
 
 def measured_traffic(width: int):
@@ -508,6 +548,9 @@ def main():
                          "(0 = library default, -1 = whole 64-ray batches per wave)")
     ap.add_argument("--trace-blocks", type=int, default=0,
                     help="persistent trace workgroups per CU, 1..5 (0 = library default, 5)")
+    ap.add_argument("--rmse-check", action="store_true",
+                    help="run the guided comparison at the RMSE target and at half of it, scoring each final "
+                         "image against a converged render, and print its JSON")
     ap.add_argument("--guided-only", action="store_true",
                     help="run only the guided-vs-plain comparison and print its JSON (for profiling)")
     ap.add_argument("--workload", default="gi", choices=("gi", "batch", "probes"),
@@ -522,6 +565,9 @@ def main():
     from banshee_amd import hip
     from banshee_amd.scenes import terrain_scene
 
+    if args.rmse_check:
+        print(json.dumps(rmse_check(make_context(args), args.bvh_width)), flush=True)
+        return
     if args.guided_only:
         print(json.dumps(guided_comparison(make_context(args), args.bvh_width)), flush=True)
         return
