@@ -372,34 +372,43 @@ __global__ __launch_bounds__(kWsumBlock) void k_guide_wsum_final(const float* pa
 }
 
 // ---- fused training step ----
-// One kernel computes a batch's loss gradient for every parameter: blocks of 4 waves take
-// rounds of 64 rows (16 per wave). In a round each wave runs the forward pass of its rows with
-// the activations in registers, the loss and its gradient with respect to the 48 outputs (4 lanes
-// per row, two lobes each), and the backward pass as the transposed products
-// dH^T = W dOut^T masked by H > 0, so every activation and gradient stays in the forward's
-// register layout (lane r + 16 q: row r, features 16 j + 4 q + i). The weight gradients
-// dW_l = In_l^T G_l reduce over rows, so each wave stages its rows' layer inputs In_l and
-// output gradients G_l in LDS feature-major (one LDS row per feature, one column per row of the
-// round), and after a block barrier every wave adds the round's 64 rows to the dW tiles it owns
-// with 16x32 products whose operands are 8 consecutive rows of one feature, i.e. one 16-byte
-// read. The tiles stay in registers over all the block's rounds (bias gradients as a product
-// with a ones operand); the blocks' partial sums are added in a fixed order by
-// k_guide_reduce_part / k_guide_reduce_train, so a step's result does not depend on scheduling.
-constexpr int kTrRows = 64;                                    // rows per round
+// One kernel computes a batch's loss gradient for every parameter: blocks of 8 waves (two per
+// SIMD) take rounds of 128 rows, 16 per wave. In a round each wave runs the forward pass of its
+// rows with the activations in registers, the loss and its gradient with respect to the 48
+// outputs (4 lanes per row, two lobes each), then the backward pass layer by layer as the
+// transposed products dH^T = W dOut^T masked by H > 0, so every activation and gradient stays in
+// the forward's register layout (lane r + 16 q: row r, features 16 j + 4 q + i). For each layer
+// the waves stage its inputs In_l and output gradients G_l feature-major in LDS (one LDS row per
+// feature, one column per row of the round), compute the next layer's gradient, and after a
+// block barrier add the round's 128 rows to the weight-gradient tiles they own with 16x32
+// products whose operands are 8 rows of one feature, one 16-byte LDS read. Only one layer is
+// staged at a time, which keeps LDS at 115 KB. The tiles stay in registers over all the block's
+// rounds (bias gradients as a product with a ones operand); the blocks' partial sums are added
+// in a fixed order by k_guide_reduce_part / k_guide_reduce_train, so a step's result does not
+// depend on scheduling.
+constexpr int kTrWaves = 8;                                    // waves per block, 16 rows each
+constexpr int kTrRows = 16 * kTrWaves;                         // rows per round
 constexpr int kRp = kTrRows + kPad;                            // staging pitch (halves) per feature
-constexpr int kTX = 0;                                         // x^T: 96 features
-constexpr int kTH = kTX + kGIn * kRp;                          // h1^T..h4^T: 4 x 64 features
-constexpr int kTY = kTH + 4 * kGHid * kRp;                     // output gradient^T: 48
-constexpr int kTD = kTY + kGOut * kRp;                         // dH1^T..dH4^T: 4 x 64
-constexpr int kTTotal = kTD + 4 * kGHid * kRp;
+constexpr int kStIn = 0;                                       // a layer's inputs^T: up to 96 features
+constexpr int kStG = kStIn + kGIn * kRp;                       // its output gradients^T: up to 64
+constexpr int kStTotal = kStG + kGHid * kRp;
 constexpr int kPartStride = kGParams + 4;                      // a block's partial sums, then its loss
 static_assert((kRp * 2) % 16 == 0, "16-byte aligned feature rows");
 
-// A[m = in][k = out] fragment of W^T from the forward LDS image (out-major rows, perm32
-// columns): four rows apart.
+// ReLU derivative bits of a layer's four fragments: bit 4 j + i = (h[j][i] > 0).
+__device__ __forceinline__ uint32_t relu_bits(const h4 (&h)[kGHid / 16])
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < kGHid / 16; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) m |= ((float)h[j][i] > 0.f ? 1u : 0u) << (4 * j + i);
+    return m;
+}
+
 // dense_t for one tile with the k-blocks outer: each k-block's weight fragments for all output
-// blocks are read together and feed independent accumulators (the training kernel runs one wave
-// per SIMD, so nothing else hides the LDS latency of a dependent chain).
+// blocks are read together and feed independent accumulators (the training kernel runs two
+// waves per SIMD, so little else hides the LDS latency of a dependent chain).
 template <int K, int N>
 __device__ __forceinline__ void dense1(const h4 (&x)[K / 16], const _Float16* Wt, int ws, const float* b,
                                        h4 (&y)[N / 16], int r, int q)
@@ -424,23 +433,13 @@ __device__ __forceinline__ void dense1(const h4 (&x)[K / 16], const _Float16* Wt
     for (int mt = 0; mt < N / 16; ++mt) y[mt] = relu_h4(acc[mt]);
 }
 
-// ReLU derivative bits of a layer's four fragments: bit 4 j + i = (h[j][i] > 0).
-__device__ __forceinline__ uint32_t relu_bits(const h4 (&h)[kGHid / 16])
-{
-    uint32_t m = 0;
-#pragma unroll
-    for (int j = 0; j < kGHid / 16; ++j)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) m |= ((float)h[j][i] > 0.f ? 1u : 0u) << (4 * j + i);
-    return m;
-}
-
+// dH^T = W dOut^T masked by the ReLU bits, with A[m = in][k = out] fragments of W^T read from the
+// forward LDS image (out-major rows, perm32 columns): four rows apart. Out-blocks outer, four
+// independent accumulators.
 template <int NOUT, int NIN>
 __device__ __forceinline__ void bwd_t(const _Float16* Wt, int ws, const h4 (&dout)[NOUT / 16], uint32_t mask,
                                       h4 (&dh)[NIN / 16], int r, int q)
 {
-    // out-blocks outer, four independent accumulators: each block's four A fragments are read
-    // together and feed four products
     f4v acc[NIN / 16];
 #pragma unroll
     for (int nt = 0; nt < NIN / 16; ++nt) acc[nt] = f4v{0.f, 0.f, 0.f, 0.f};
@@ -471,18 +470,20 @@ __device__ __forceinline__ void stage_t(_Float16* T, int lr, const h4 (&v)[N / 1
         for (int i = 0; i < 4; ++i) T[(16 * j + 4 * q + i) * kRp + lr] = v[j][i];
 }
 
-// One wave's 16 rows of a round (raw inputs a, c and record (dir, weight) d already loaded):
-// forward, loss gradient, backward; stages In_l and G_l for the weight gradients. O: the wave's
-// 16 x 48 output exchange area. Returns the wave's summed loss (every lane the same value).
-__device__ __forceinline__ float train_rows(const _Float16* lw, const float* lb, _Float16* st, float* O, float4 a,
-                                            float4 c, float4 d, bool live, int wave, float scale, int r, int q)
+// One wave's 16 rows (raw inputs a, c and record (dir, weight) d already loaded): the forward
+// pass into x and h1..h4, the loss gradient with respect to the outputs into dy (the backward's
+// B fragments). O: the wave's 16 x 48 exchange area. Returns the wave's summed loss (every lane
+// the same value).
+__device__ __forceinline__ float train_forward(const _Float16* lw, const float* lb, float* O, float4 a, float4 c,
+                                               float4 d, bool live, float scale, int r, int q, h4 (&x)[kGIn / 16],
+                                               h4 (&h1)[kGHid / 16], h4 (&h2)[kGHid / 16], h4 (&h3)[kGHid / 16],
+                                               h4 (&hh)[kGHid / 16], h4 (&dy)[kGOut / 16])
 {
-    h4 x[1][kGIn / 16], h1[1][kGHid / 16], h2[1][kGHid / 16], h3[1][kGHid / 16], hh[1][kGHid / 16];
-    encode_row(a, c, q, live, x[0]);
-    dense1<kGIn, kGHid>(x[0], lw + kLF1, kL1S, lb, h1[0], r, q);
-    dense1<kGHid, kGHid>(h1[0], lw + kLF2, kLHS, lb + kGHid, h2[0], r, q);
-    dense1<kGHid, kGHid>(h2[0], lw + kLF3, kLHS, lb + 2 * kGHid, h3[0], r, q);
-    dense1<kGHid, kGHid>(h3[0], lw + kLF4, kLHS, lb + 3 * kGHid, hh[0], r, q);
+    encode_row(a, c, q, live, x);
+    dense1<kGIn, kGHid>(x, lw + kLF1, kL1S, lb, h1, r, q);
+    dense1<kGHid, kGHid>(h1, lw + kLF2, kLHS, lb + kGHid, h2, r, q);
+    dense1<kGHid, kGHid>(h2, lw + kLF3, kLHS, lb + 2 * kGHid, h3, r, q);
+    dense1<kGHid, kGHid>(h3, lw + kLF4, kLHS, lb + 3 * kGHid, hh, r, q);
     {
         f4v acc[kGOut / 16];
 #pragma unroll
@@ -499,21 +500,13 @@ __device__ __forceinline__ float train_rows(const _Float16* lw, const float* lb,
             for (int mt = 0; mt < kGOut / 16; ++mt)
                 w[mt] = *reinterpret_cast<const h8*>(&lw[kLF5 + (mt * 16 + r) * kLHS + kb * 32 + 8 * q]);
 #pragma unroll
-            for (int mt = 0; mt < kGOut / 16; ++mt) acc[mt] = mfma32(w[mt], cat8(hh[0][2 * kb], hh[0][2 * kb + 1]), acc[mt]);
+            for (int mt = 0; mt < kGOut / 16; ++mt) acc[mt] = mfma32(w[mt], cat8(hh[2 * kb], hh[2 * kb + 1]), acc[mt]);
         }
 #pragma unroll
         for (int mt = 0; mt < kGOut / 16; ++mt)
             *reinterpret_cast<float4*>(&O[r * kGOut + mt * 16 + 4 * q]) =
                 make_float4(acc[mt][0], acc[mt][1], acc[mt][2], acc[mt][3]);
     }
-    // stage the layer inputs now; the backward keeps only the ReLU derivative bits
-    const int lr = 16 * wave + r;
-    stage_t<kGIn>(st + kTX, lr, x[0], q);
-    stage_t<kGHid>(st + kTH + 0 * kGHid * kRp, lr, h1[0], q);
-    stage_t<kGHid>(st + kTH + 1 * kGHid * kRp, lr, h2[0], q);
-    stage_t<kGHid>(st + kTH + 2 * kGHid * kRp, lr, h3[0], q);
-    stage_t<kGHid>(st + kTH + 3 * kGHid * kRp, lr, hh[0], q);
-    const uint32_t m12 = relu_bits(h1[0]) | (relu_bits(h2[0]) << 16), m34 = relu_bits(h3[0]) | (relu_bits(hh[0]) << 16);
     __builtin_amdgcn_wave_barrier();   // the wave's LDS accesses complete in order
     // loss -s log q(w) and its gradient (k_guide_loss's formulas); lane q takes lobes 2q, 2q+1
     const float* o = O + r * kGOut;
@@ -568,137 +561,104 @@ __device__ __forceinline__ float train_rows(const _Float16* lw, const float* lb,
         }
     }
     float loss = (ok && q == 0) ? -sw * logf(qm) : 0.f;
-    // stage the output gradient (this lane's 12 of the row's 48)
-    _Float16* Yt = st + kTY + lr;
-    Yt[(2 * q) * kRp] = (_Float16)gl[0];
-    Yt[(2 * q + 1) * kRp] = (_Float16)gl[1];
-    Yt[(8 + 6 * q + 0) * kRp] = (_Float16)gmu[0].x;
-    Yt[(8 + 6 * q + 1) * kRp] = (_Float16)gmu[0].y;
-    Yt[(8 + 6 * q + 2) * kRp] = (_Float16)gmu[0].z;
-    Yt[(8 + 6 * q + 3) * kRp] = (_Float16)gmu[1].x;
-    Yt[(8 + 6 * q + 4) * kRp] = (_Float16)gmu[1].y;
-    Yt[(8 + 6 * q + 5) * kRp] = (_Float16)gmu[1].z;
-    Yt[(32 + 2 * q) * kRp] = (_Float16)gk[0];
-    Yt[(33 + 2 * q) * kRp] = (_Float16)gk[1];
-    Yt[(40 + 2 * q) * kRp] = (_Float16)0.f;
-    Yt[(41 + 2 * q) * kRp] = (_Float16)0.f;
+    __builtin_amdgcn_wave_barrier();   // every output read before the area holds the gradient
+    // the row's 48 output gradients as halves over the exchange area, this lane's 12 of them
+    _Float16* Y = reinterpret_cast<_Float16*>(O) + r * kGOut;
+    Y[2 * q] = (_Float16)gl[0];
+    Y[2 * q + 1] = (_Float16)gl[1];
+    Y[8 + 6 * q + 0] = (_Float16)gmu[0].x;
+    Y[8 + 6 * q + 1] = (_Float16)gmu[0].y;
+    Y[8 + 6 * q + 2] = (_Float16)gmu[0].z;
+    Y[8 + 6 * q + 3] = (_Float16)gmu[1].x;
+    Y[8 + 6 * q + 4] = (_Float16)gmu[1].y;
+    Y[8 + 6 * q + 5] = (_Float16)gmu[1].z;
+    Y[32 + 2 * q] = (_Float16)gk[0];
+    Y[33 + 2 * q] = (_Float16)gk[1];
+    Y[40 + 2 * q] = (_Float16)0.f;
+    Y[41 + 2 * q] = (_Float16)0.f;
     __builtin_amdgcn_wave_barrier();
-    h4 dy[kGOut / 16];
 #pragma unroll
-    for (int ks = 0; ks < kGOut / 16; ++ks)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) dy[ks][i] = Yt[(16 * ks + 4 * q + i) * kRp];
-    h4 g4[kGHid / 16], g3[kGHid / 16];
-    bwd_t<kGOut, kGHid>(lw + kLF5, kLHS, dy, m34 >> 16, g4, r, q);
-    stage_t<kGHid>(st + kTD + 3 * kGHid * kRp, lr, g4, q);
-    bwd_t<kGHid, kGHid>(lw + kLF4, kLHS, g4, m34 & 0xffffu, g3, r, q);
-    stage_t<kGHid>(st + kTD + 2 * kGHid * kRp, lr, g3, q);
-    bwd_t<kGHid, kGHid>(lw + kLF3, kLHS, g3, m12 >> 16, g4, r, q);
-    stage_t<kGHid>(st + kTD + 1 * kGHid * kRp, lr, g4, q);
-    bwd_t<kGHid, kGHid>(lw + kLF2, kLHS, g4, m12 & 0xffffu, g3, r, q);
-    stage_t<kGHid>(st + kTD + 0 * kGHid * kRp, lr, g3, q);
+    for (int ks = 0; ks < kGOut / 16; ++ks) dy[ks] = *reinterpret_cast<const h4*>(&Y[16 * ks + 4 * q]);
     // the wave's loss, summed over its lanes in a fixed butterfly
 #pragma unroll
     for (int m = 1; m < 64; m <<= 1) loss += __shfl_xor(loss, m);
     return loss;
 }
 
-// acc[A0 + NJ m + nj] += sum over the round's 64 rows of In[row][16 m + in] G[row][16 nj + out], In^T
-// and G^T pointing at the tile block's first feature (A[m = in][k = row] and B[k = row][n = out]:
-// 8 consecutive rows of one feature per lane); with kBias also acc[B0 + nj] += sum_rows
-// G[row][16 nj + out] in every row (a product with a ones operand).
-constexpr int kAccTiles = 28;   // dW and bias tiles per wave
-template <int NMI, int NJ, int A0, bool kBias, int B0>
-__device__ __forceinline__ void wgrad_round(const _Float16* In, const _Float16* G, f4v (&acc)[kAccTiles], int r, int q)
+// One layer's weight-gradient tiles of a wave over the round's staged rows: acc[A0 + t] +=
+// In[row][16 (ib0 + t ibstep) + in]^T G[row][16 ob + out] for t < NT (when `tiles`), and with
+// `bias` acc[B0] += sum_rows G[row][16 ob + out] in every row (a product with a ones operand).
+// A[m = in][k = row] and B[k = row][n = out] are 8 consecutive rows of one feature per lane.
+constexpr int kAccTiles = 16;
+template <int NT, int A0, int B0>
+__device__ __forceinline__ void wgrad_layer(const _Float16* In, const _Float16* G, int ib0, int ibstep, int ob,
+                                            bool tiles, bool bias, f4v (&acc)[kAccTiles], int r, int q)
 {
     const h8 ones = {(_Float16)1.f, (_Float16)1.f, (_Float16)1.f, (_Float16)1.f,
                      (_Float16)1.f, (_Float16)1.f, (_Float16)1.f, (_Float16)1.f};
 #pragma unroll
     for (int g = 0; g < kTrRows / 32; ++g)
     {
-        h8 b[NJ];
-#pragma unroll
-        for (int nj = 0; nj < NJ; ++nj) b[nj] = *reinterpret_cast<const h8*>(&G[(16 * nj + r) * kRp + 32 * g + 8 * q]);
-        if (kBias)
-#pragma unroll
-            for (int nj = 0; nj < NJ; ++nj) acc[B0 + nj] = mfma32(ones, b[nj], acc[B0 + nj]);
-#pragma unroll
-        for (int m = 0; m < NMI; ++m)
+        const h8 b = *reinterpret_cast<const h8*>(&G[(16 * ob + r) * kRp + 32 * g + 8 * q]);
+        if (bias) acc[B0] = mfma32(ones, b, acc[B0]);
+        if (tiles)
         {
-            const h8 a = *reinterpret_cast<const h8*>(&In[(16 * m + r) * kRp + 32 * g + 8 * q]);
+            h8 a[NT];
 #pragma unroll
-            for (int nj = 0; nj < NJ; ++nj) acc[A0 + NJ * m + nj] = mfma32(a, b[nj], acc[A0 + NJ * m + nj]);
+            for (int t = 0; t < NT; ++t)
+                a[t] = *reinterpret_cast<const h8*>(&In[(16 * (ib0 + t * ibstep) + r) * kRp + 32 * g + 8 * q]);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[A0 + t] = mfma32(a[t], b, acc[A0 + t]);
         }
     }
 }
 
-template <int NMI, int NJ, int A0>
-__device__ __forceinline__ void wgrad_store(float* P, int nout, const f4v (&acc)[kAccTiles], int r, int q)
+template <int NT, int A0, int B0>
+__device__ __forceinline__ void wgrad_store(float* P, int woff, int boff, int nout, int ib0, int ibstep, int ob,
+                                            bool tiles, bool bias, const f4v (&acc)[kAccTiles], int r, int q)
 {
-    // P points at the block's first weight: row = in feature, nout columns
+    if (tiles)
 #pragma unroll
-    for (int m = 0; m < NMI; ++m)
+        for (int t = 0; t < NT; ++t)
 #pragma unroll
-        for (int nj = 0; nj < NJ; ++nj)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) P[(16 * m + 4 * q + i) * nout + 16 * nj + r] = acc[A0 + NJ * m + nj][i];
+            for (int i = 0; i < 4; ++i)
+                P[woff + (16 * (ib0 + t * ibstep) + 4 * q + i) * nout + 16 * ob + r] = acc[A0 + t][i];
+    if (bias && q == 0) P[boff + 16 * ob + r] = acc[B0][0];
 }
 
-// The dW tiles of wave w (the same shapes for every wave, so one code path):
-//   acc[0..15]  a 64 x 64 block: layer 1 in-features 0..63 (w = 0) or layer w + 1, with the
-//               layer's bias in acc[16..19];
-//   acc[20..22] layer 5, in-features 16 w .. 16 w + 15, all 48 outputs (wave 0 also the bias in
-//               acc[25..27]);
-//   acc[23..24] layer 1, in-features 64 + 16 (w / 2) .., outputs 32 (w % 2) .. + 31.
-__device__ __forceinline__ void wgrad_wave(const _Float16* st, int wave, f4v (&acc)[kAccTiles], int r, int q)
-{
-    const _Float16* in1 = wave == 0 ? st + kTX : st + kTH + (wave - 1) * kGHid * kRp;
-    wgrad_round<4, 4, 0, true, 16>(in1, st + kTD + wave * kGHid * kRp, acc, r, q);
-    const _Float16* h4t = st + kTH + 3 * kGHid * kRp + 16 * wave * kRp;
-    if (wave == 0) wgrad_round<1, 3, 20, true, 25>(h4t, st + kTY, acc, r, q);
-    else wgrad_round<1, 3, 20, false, 25>(h4t, st + kTY, acc, r, q);
-    wgrad_round<1, 2, 23, false, 0>(st + kTX + (64 + 16 * (wave >> 1)) * kRp, st + kTD + 32 * (wave & 1) * kRp, acc,
-                                    r, q);
-}
-
-__device__ __forceinline__ void wgrad_wave_store(float* P, int wave, const f4v (&acc)[kAccTiles], int r, int q)
-{
-    const int woff = wave == 0 ? kGW1 : wave == 1 ? kGW2 : wave == 2 ? kGW3 : kGW4;
-    const int boff = wave == 0 ? kGB1 : wave == 1 ? kGB2 : wave == 2 ? kGB3 : kGB4;
-    wgrad_store<4, 4, 0>(P + woff, kGHid, acc, r, q);
-    wgrad_store<1, 3, 20>(P + kGW5 + 16 * wave * kGOut, kGOut, acc, r, q);
-    wgrad_store<1, 2, 23>(P + kGW1 + (64 + 16 * (wave >> 1)) * kGHid + 32 * (wave & 1), kGHid, acc, r, q);
-    if (q == 0)
-    {
-#pragma unroll
-        for (int nj = 0; nj < 4; ++nj) P[boff + 16 * nj + r] = acc[16 + nj][0];
-        if (wave == 0)
-#pragma unroll
-            for (int nj = 0; nj < 3; ++nj) P[kGB5 + 16 * nj + r] = acc[25 + nj][0];
-    }
-}
-
-__global__ __launch_bounds__(256, 1) void k_guide_train_fused(GuideNet g, const float4* rec, int srow, uint32_t nrows,
-                                                              const float* wsum, float* part)
+// Tiles of wave w (the same shapes for every wave, so one code path): out-block w % 4 of layers
+// 1-4, in-blocks w / 4 + 2 t (layer 1: three, layers 2-4: two), and waves 0-5 out-block w % 3
+// of layer 5, in-blocks w / 3 + 2 t (two); the bias tiles go to waves 0-3 (layer 5: 0-2).
+// acc: layer 1 [0, 3) + bias 3, layer 2 [4, 6) + 6, layer 3 [7, 9) + 9, layer 4 [10, 12) + 12,
+// layer 5 [13, 15) + 15.
+__global__ __launch_bounds__(64 * kTrWaves, 1) void k_guide_train_fused(GuideNet g, const float4* rec, int srow,
+                                                                        uint32_t nrows, const float* wsum,
+                                                                        float* part)
 {
     __shared__ __align__(16) _Float16 lw[kLFTotal];
     __shared__ __align__(16) float lb[kBiasTotal];
-    __shared__ __align__(16) _Float16 st[kTTotal];
-    __shared__ __align__(16) float ox[4 * 16 * kGOut];
-    __shared__ float red[4];
+    __shared__ __align__(16) _Float16 st[kStTotal];
+    __shared__ __align__(16) float ox[kTrWaves * 16 * kGOut];
+    __shared__ float red[kTrWaves];
     load_fwd_weights(g, lw, lb);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
     const float scale = (float)nrows / fmaxf(wsum[0], 1e-30f);
+    const int ob = wave & 3, ib = wave >> 2, ob5 = wave % 3, ib5 = wave / 3;
+    const bool bias4 = wave < 4, tiles5 = wave < 6, bias5 = wave < 3;
     f4v acc[kAccTiles];
 #pragma unroll
     for (int t = 0; t < kAccTiles; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
     float loss = 0.f;
     const uint32_t rounds = (nrows + kTrRows - 1) / kTrRows;
+    const int lr = 16 * wave + r;
+    _Float16* sIn = st + kStIn;
+    _Float16* sG = st + kStG;
+    float* O = ox + wave * 16 * kGOut;
     // the lane's record of the next round, loaded one round ahead
     float4 na = make_float4(0.f, 0.f, 0.f, 0.f), nc = na, nd = na;
-    uint32_t nrow = blockIdx.x * kTrRows + 16 * wave + r;
-    if (blockIdx.x < rounds && nrow < nrows)
+    uint32_t nrow = blockIdx.x * kTrRows + lr;
+    if (nrow < nrows)
     {
         na = rec[(size_t)srow * nrow];
         nc = rec[(size_t)srow * nrow + 1];
@@ -718,16 +678,60 @@ __global__ __launch_bounds__(256, 1) void k_guide_train_fused(GuideNet g, const 
         // an offset the compiler cannot see through keeps the weight reads inside the loop
         int zoff = 0;
         asm volatile("" : "+v"(zoff));
-        loss += train_rows(lw + zoff, lb, st, ox + wave * 16 * kGOut, a, c, d, live, wave, scale, r, q);
+        const _Float16* W = lw + zoff;
+        h4 x[kGIn / 16], h1[kGHid / 16], h2[kGHid / 16], h3[kGHid / 16], hh[kGHid / 16], dy[kGOut / 16];
+        loss += train_forward(W, lb, O, a, c, d, live, scale, r, q, x, h1, h2, h3, hh, dy);
+        const uint32_t m12 = relu_bits(h1) | (relu_bits(h2) << 16), m34 = relu_bits(h3) | (relu_bits(hh) << 16);
+        h4 ga[kGHid / 16], gb[kGHid / 16];
+        // layer 5: inputs h4, gradient dy; then dH4
+        stage_t<kGHid>(sIn, lr, hh, q);
+        stage_t<kGOut>(sG, lr, dy, q);
+        bwd_t<kGOut, kGHid>(W + kLF5, kLHS, dy, m34 >> 16, ga, r, q);
         __syncthreads();
-        wgrad_wave(st, wave, acc, r, q);
+        wgrad_layer<2, 13, 15>(sIn, sG, ib5, 2, ob5, tiles5, bias5, acc, r, q);
+        __syncthreads();
+        // layer 4: inputs h3, gradient dH4; then dH3
+        stage_t<kGHid>(sIn, lr, h3, q);
+        stage_t<kGHid>(sG, lr, ga, q);
+        bwd_t<kGHid, kGHid>(W + kLF4, kLHS, ga, m34 & 0xffffu, gb, r, q);
+        __syncthreads();
+        wgrad_layer<2, 10, 12>(sIn, sG, ib, 2, ob, true, bias4, acc, r, q);
+        __syncthreads();
+        // layer 3: inputs h2, gradient dH3; then dH2
+        stage_t<kGHid>(sIn, lr, h2, q);
+        stage_t<kGHid>(sG, lr, gb, q);
+        bwd_t<kGHid, kGHid>(W + kLF3, kLHS, gb, m12 >> 16, ga, r, q);
+        __syncthreads();
+        wgrad_layer<2, 7, 9>(sIn, sG, ib, 2, ob, true, bias4, acc, r, q);
+        __syncthreads();
+        // layer 2: inputs h1, gradient dH2; then dH1
+        stage_t<kGHid>(sIn, lr, h1, q);
+        stage_t<kGHid>(sG, lr, ga, q);
+        bwd_t<kGHid, kGHid>(W + kLF2, kLHS, ga, m12 & 0xffffu, gb, r, q);
+        __syncthreads();
+        wgrad_layer<2, 4, 6>(sIn, sG, ib, 2, ob, true, bias4, acc, r, q);
+        __syncthreads();
+        // layer 1: inputs x, gradient dH1
+        stage_t<kGIn>(sIn, lr, x, q);
+        stage_t<kGHid>(sG, lr, gb, q);
+        __syncthreads();
+        wgrad_layer<3, 0, 3>(sIn, sG, ib, 2, ob, true, bias4, acc, r, q);
         __syncthreads();
     }
     float* P = part + (size_t)blockIdx.x * kPartStride;
-    wgrad_wave_store(P, wave, acc, r, q);
+    wgrad_store<3, 0, 3>(P, kGW1, kGB1, kGHid, ib, 2, ob, true, bias4, acc, r, q);
+    wgrad_store<2, 4, 6>(P, kGW2, kGB2, kGHid, ib, 2, ob, true, bias4, acc, r, q);
+    wgrad_store<2, 7, 9>(P, kGW3, kGB3, kGHid, ib, 2, ob, true, bias4, acc, r, q);
+    wgrad_store<2, 10, 12>(P, kGW4, kGB4, kGHid, ib, 2, ob, true, bias4, acc, r, q);
+    wgrad_store<2, 13, 15>(P, kGW5, kGB5, kGOut, ib5, 2, ob5, tiles5, bias5, acc, r, q);
     if (lane == 0) red[wave] = loss;
     __syncthreads();
-    if (threadIdx.x == 0) P[kGParams] = ((red[0] + red[1]) + red[2]) + red[3];
+    if (threadIdx.x == 0)
+    {
+        float s = 0.f;
+        for (int w = 0; w < kTrWaves; ++w) s += red[w];
+        P[kGParams] = s;
+    }
 }
 
 // Sums the fused kernel's block partials in a fixed order, in two passes so that no thread walks
@@ -1002,7 +1006,7 @@ static bsh_status guide_gradients(bsh_guide* g, const float4* rec, int srow, uin
     if (!g->part2.p) BSH_HIP(g->part2.alloc((size_t)((ncu + kRedGroup - 1) / kRedGroup) * kPartStride));
     hipLaunchKernelGGL(k_guide_wsum, dim3(kWsumParts), dim3(kWsumBlock), 0, st, rec, srow, n, g->wsum.p + 1);
     hipLaunchKernelGGL(k_guide_wsum_final, dim3(1), dim3(kWsumBlock), 0, st, g->wsum.p + 1, g->wsum.p);
-    hipLaunchKernelGGL(k_guide_train_fused, dim3(nb), dim3(256), 0, st, g->net(), rec, srow, n,
+    hipLaunchKernelGGL(k_guide_train_fused, dim3(nb), dim3(64 * kTrWaves), 0, st, g->net(), rec, srow, n,
                        (const float*)g->wsum.p, g->part.p);
     const dim3 rg((kGParams + 1 + 255) / 256, ngroups);
     hipLaunchKernelGGL(k_guide_reduce_part, rg, dim3(256), 0, st, (const float*)g->part.p, (int)nb, g->part2.p);
