@@ -381,8 +381,11 @@ __global__ __launch_bounds__(kWsumBlock) void k_guide_wsum_final(const float* pa
 // the waves stage its inputs In_l and output gradients G_l feature-major in LDS (one LDS row per
 // feature, one column per row of the round), compute the next layer's gradient, and after a
 // block barrier add the round's 128 rows to the weight-gradient tiles they own with 16x32
-// products whose operands are 8 rows of one feature, one 16-byte LDS read. Only one layer is
-// staged at a time, which keeps LDS at 115 KB. The tiles stay in registers over all the block's
+// products whose operands are 8 rows of one feature, one 16-byte LDS read. Two layers are
+// staged at a time (five layers, six barriers per round; LDS 157 KB). The backward products read
+// only the weight image, so they fill the waits at the barriers: dH3 and dH1 are computed after
+// a staging, dH2 after a pair's weight gradients, and each is computed as late as its staging
+// allows, which keeps the kernel inside 256 VGPRs without spills. The tiles stay in registers over all the block's
 // rounds (bias gradients as a product with a ones operand); the blocks' partial sums are added
 // in a fixed order by k_guide_reduce_part / k_guide_reduce_train, so a step's result does not
 // depend on scheduling.
@@ -391,7 +394,8 @@ constexpr int kTrRows = 16 * kTrWaves;                         // rows per round
 constexpr int kRp = kTrRows + kPad;                            // staging pitch (halves) per feature
 constexpr int kStIn = 0;                                       // a layer's inputs^T: up to 96 features
 constexpr int kStG = kStIn + kGIn * kRp;                       // its output gradients^T: up to 64
-constexpr int kStTotal = kStG + kGHid * kRp;
+constexpr int kStLayer = kStG + kGHid * kRp;                   // one staged layer
+constexpr int kStTotal = 2 * kStLayer;                         // two layers at a time
 constexpr int kPartStride = kGParams + 4;                      // a block's partial sums, then its loss
 static_assert((kRp * 2) % 16 == 0, "16-byte aligned feature rows");
 
@@ -652,10 +656,10 @@ __global__ __launch_bounds__(64 * kTrWaves, 1) void k_guide_train_fused(GuideNet
     float loss = 0.f;
     const uint32_t rounds = (nrows + kTrRows - 1) / kTrRows;
     const int lr = 16 * wave + r;
-    _Float16* sIn = st + kStIn;
-    _Float16* sG = st + kStG;
+    _Float16* sA = st;
+    _Float16* sB = st + kStLayer;
     float* O = ox + wave * 16 * kGOut;
-    // the lane's record of the next round, loaded one round ahead
+    // the lane's record of the next round, loaded during the current one
     float4 na = make_float4(0.f, 0.f, 0.f, 0.f), nc = na, nd = na;
     uint32_t nrow = blockIdx.x * kTrRows + lr;
     if (nrow < nrows)
@@ -669,54 +673,53 @@ __global__ __launch_bounds__(64 * kTrWaves, 1) void k_guide_train_fused(GuideNet
         const float4 a = na, c = nc, d = nd;
         const bool live = nrow < nrows;
         nrow += gridDim.x * kTrRows;
-        if (rd + gridDim.x < rounds && nrow < nrows)
-        {
-            na = rec[(size_t)srow * nrow];
-            nc = rec[(size_t)srow * nrow + 1];
-            nd = rec[(size_t)srow * nrow + 2];
-        }
         // an offset the compiler cannot see through keeps the weight reads inside the loop
         int zoff = 0;
         asm volatile("" : "+v"(zoff));
         const _Float16* W = lw + zoff;
         h4 x[kGIn / 16], h1[kGHid / 16], h2[kGHid / 16], h3[kGHid / 16], hh[kGHid / 16], dy[kGOut / 16];
         loss += train_forward(W, lb, O, a, c, d, live, scale, r, q, x, h1, h2, h3, hh, dy);
-        const uint32_t m12 = relu_bits(h1) | (relu_bits(h2) << 16), m34 = relu_bits(h3) | (relu_bits(hh) << 16);
-        h4 ga[kGHid / 16], gb[kGHid / 16];
-        // layer 5: inputs h4, gradient dy; then dH4
-        stage_t<kGHid>(sIn, lr, hh, q);
-        stage_t<kGOut>(sG, lr, dy, q);
-        bwd_t<kGOut, kGHid>(W + kLF5, kLHS, dy, m34 >> 16, ga, r, q);
+        // dH4 before the first barrier (its weight reads are not in the staging area)
+        h4 g4[kGHid / 16], g3[kGHid / 16], g2[kGHid / 16], g1[kGHid / 16];
+        bwd_t<kGOut, kGHid>(W + kLF5, kLHS, dy, relu_bits(hh), g4, r, q);
+        const uint32_t m3 = relu_bits(h3);
+        // the previous round's last weight gradients have read the staging area
         __syncthreads();
-        wgrad_layer<2, 13, 15>(sIn, sG, ib5, 2, ob5, tiles5, bias5, acc, r, q);
+        // layers 5 and 4
+        stage_t<kGHid>(sA + kStIn, lr, hh, q);
+        stage_t<kGOut>(sA + kStG, lr, dy, q);
+        stage_t<kGHid>(sB + kStIn, lr, h3, q);
+        stage_t<kGHid>(sB + kStG, lr, g4, q);
+        bwd_t<kGHid, kGHid>(W + kLF4, kLHS, g4, m3, g3, r, q);
+        // the next round's record, issued here where fewer activations are live
+        if (rd + gridDim.x < rounds && nrow < nrows)
+        {
+            na = rec[(size_t)srow * nrow];
+            nc = rec[(size_t)srow * nrow + 1];
+            nd = rec[(size_t)srow * nrow + 2];
+        }
         __syncthreads();
-        // layer 4: inputs h3, gradient dH4; then dH3
-        stage_t<kGHid>(sIn, lr, h3, q);
-        stage_t<kGHid>(sG, lr, ga, q);
-        bwd_t<kGHid, kGHid>(W + kLF4, kLHS, ga, m34 & 0xffffu, gb, r, q);
+        wgrad_layer<2, 13, 15>(sA + kStIn, sA + kStG, ib5, 2, ob5, tiles5, bias5, acc, r, q);
+        wgrad_layer<2, 10, 12>(sB + kStIn, sB + kStG, ib, 2, ob, true, bias4, acc, r, q);
+        // dH2 while the other waves finish reading layers 5 and 4
+        bwd_t<kGHid, kGHid>(W + kLF3, kLHS, g3, relu_bits(h2), g2, r, q);
+        const uint32_t m1 = relu_bits(h1);
         __syncthreads();
-        wgrad_layer<2, 10, 12>(sIn, sG, ib, 2, ob, true, bias4, acc, r, q);
+        // layers 3 and 2
+        stage_t<kGHid>(sA + kStIn, lr, h2, q);
+        stage_t<kGHid>(sA + kStG, lr, g3, q);
+        stage_t<kGHid>(sB + kStIn, lr, h1, q);
+        stage_t<kGHid>(sB + kStG, lr, g2, q);
+        bwd_t<kGHid, kGHid>(W + kLF2, kLHS, g2, m1, g1, r, q);
         __syncthreads();
-        // layer 3: inputs h2, gradient dH3; then dH2
-        stage_t<kGHid>(sIn, lr, h2, q);
-        stage_t<kGHid>(sG, lr, gb, q);
-        bwd_t<kGHid, kGHid>(W + kLF3, kLHS, gb, m12 >> 16, ga, r, q);
+        wgrad_layer<2, 7, 9>(sA + kStIn, sA + kStG, ib, 2, ob, true, bias4, acc, r, q);
+        wgrad_layer<2, 4, 6>(sB + kStIn, sB + kStG, ib, 2, ob, true, bias4, acc, r, q);
         __syncthreads();
-        wgrad_layer<2, 7, 9>(sIn, sG, ib, 2, ob, true, bias4, acc, r, q);
+        // layer 1
+        stage_t<kGIn>(sA + kStIn, lr, x, q);
+        stage_t<kGHid>(sA + kStG, lr, g1, q);
         __syncthreads();
-        // layer 2: inputs h1, gradient dH2; then dH1
-        stage_t<kGHid>(sIn, lr, h1, q);
-        stage_t<kGHid>(sG, lr, ga, q);
-        bwd_t<kGHid, kGHid>(W + kLF2, kLHS, ga, m12 & 0xffffu, gb, r, q);
-        __syncthreads();
-        wgrad_layer<2, 4, 6>(sIn, sG, ib, 2, ob, true, bias4, acc, r, q);
-        __syncthreads();
-        // layer 1: inputs x, gradient dH1
-        stage_t<kGIn>(sIn, lr, x, q);
-        stage_t<kGHid>(sG, lr, gb, q);
-        __syncthreads();
-        wgrad_layer<3, 0, 3>(sIn, sG, ib, 2, ob, true, bias4, acc, r, q);
-        __syncthreads();
+        wgrad_layer<3, 0, 3>(sA + kStIn, sA + kStG, ib, 2, ob, true, bias4, acc, r, q);
     }
     float* P = part + (size_t)blockIdx.x * kPartStride;
     wgrad_store<3, 0, 3>(P, kGW1, kGB1, kGHid, ib, 2, ob, true, bias4, acc, r, q);
