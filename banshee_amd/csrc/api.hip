@@ -625,6 +625,9 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
         if (params->width != 0 && params->width != 2 && params->width != 4)
             return fail(BSH_ERR_INVALID, "bsh_scene_commit: BVH width must be 0, 2 or 4");
         if (params->width != 0) bp.width = params->width;
+        if (params->quantized < -1 || params->quantized > 1)
+            return fail(BSH_ERR_INVALID, "bsh_scene_commit: quantized must be -1, 0 or 1");
+        bp.quantized = params->quantized;
     }
     BuildOutput bo;
     build_bvh(tv.data(), nt, bp, bo);
@@ -668,10 +671,11 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
             r[7] = make_float4(m.eta, m.alpha, m.pad[0], m.pad[1]);
         }
     }
-    const size_t node_f4 = bo.width == 4 ? bo.nodes4.size() * 8 : bo.nodes.size() * 4;
+    const size_t node_f4 = bo.width != 4 ? bo.nodes.size() * 4 : bo.quantized ? bo.nodesq.size() * 4 : bo.nodes4.size() * 8;
+    const void* node_src = bo.width != 4 ? (const void*)bo.nodes.data()
+                           : bo.quantized ? (const void*)bo.nodesq.data() : (const void*)bo.nodes4.data();
     BSH_HIP(s->d_nodes.alloc(node_f4));
-    BSH_HIP(hipMemcpy(s->d_nodes.p, bo.width == 4 ? (const void*)bo.nodes4.data() : (const void*)bo.nodes.data(),
-                      node_f4 * sizeof(float4), hipMemcpyHostToDevice));
+    BSH_HIP(hipMemcpy(s->d_nodes.p, node_src, node_f4 * sizeof(float4), hipMemcpyHostToDevice));
     BSH_HIP(s->d_tris.alloc(tris.size()));
     BSH_HIP(hipMemcpy(s->d_tris.p, tris.data(), tris.size() * sizeof(float4), hipMemcpyHostToDevice));
     BSH_HIP(s->d_stri.alloc(stri.size()));
@@ -725,6 +729,7 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params)
         }
     s->nnodes = bo.width == 4 ? bo.nodes4.size() : bo.nodes.size();
     s->width = bo.width;
+    s->quantized = bo.width == 4 && bo.quantized;
     s->max_depth = bo.max_depth;
     s->build_ms = bo.build_ms;
     s->committed = true;
@@ -742,7 +747,8 @@ bsh_status bsh_scene_destroy(bsh_scene* s)
 }
 
 bsh_status bsh_bvh_build_host(const float* tri_verts, size_t ntris, const bsh_bvh_params* params, void* nodes,
-                              size_t nodes_cap, int32_t* tri_order, size_t* nnodes, int32_t* width, int32_t* depth)
+                              size_t nodes_cap, int32_t* tri_order, size_t* nnodes, int32_t* width, int32_t* depth,
+                              int32_t* node_bytes)
 {
     if (!tri_verts || ntris == 0 || !nnodes) return fail(BSH_ERR_INVALID, "bsh_bvh_build_host: bad argument");
     BuildParams bp;
@@ -754,35 +760,44 @@ bsh_status bsh_bvh_build_host(const float* tri_verts, size_t ntris, const bsh_bv
         if (params->width != 0 && params->width != 2 && params->width != 4)
             return fail(BSH_ERR_INVALID, "bsh_bvh_build_host: BVH width must be 0, 2 or 4");
         if (params->width != 0) bp.width = params->width;
+        if (params->quantized < -1 || params->quantized > 1)
+            return fail(BSH_ERR_INVALID, "bsh_bvh_build_host: quantized must be -1, 0 or 1");
+        bp.quantized = params->quantized;
     }
     BuildOutput bo;
     build_bvh(tri_verts, ntris, bp, bo);
+    const bool q = bo.width == 4 && bo.quantized;
     const size_t n = bo.width == 4 ? bo.nodes4.size() : bo.nodes.size();
-    const size_t nb = bo.width == 4 ? sizeof(GpuNode4) : sizeof(GpuNode);
+    const size_t nb = bo.width != 4 ? sizeof(GpuNode) : q ? sizeof(GpuNodeQ) : sizeof(GpuNode4);
     *nnodes = n;
     if (width) *width = bo.width;
+    if (node_bytes) *node_bytes = (int32_t)nb;
     if (depth) *depth = bo.max_depth;
     if (nodes)
     {
         if (nodes_cap < n) return fail(BSH_ERR_INVALID, "bsh_bvh_build_host: node buffer too small");
-        std::memcpy(nodes, bo.width == 4 ? (const void*)bo.nodes4.data() : (const void*)bo.nodes.data(), n * nb);
+        std::memcpy(nodes, bo.width != 4 ? (const void*)bo.nodes.data()
+                           : q ? (const void*)bo.nodesq.data() : (const void*)bo.nodes4.data(), n * nb);
     }
     if (tri_order) std::memcpy(tri_order, bo.tri_order.data(), ntris * sizeof(int32_t));
     return BSH_OK;
 }
 
 bsh_status bsh_scene_info(const bsh_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* bytes, double* build_ms,
-                          int32_t* depth, int32_t* width)
+                          int32_t* depth, int32_t* width, int32_t* node_bytes)
 {
     if (!s) return fail(BSH_ERR_INVALID, "bsh_scene_info: NULL scene");
     if (nodes) *nodes = s->nnodes;
     if (tris) *tris = s->tri_m.size();
     if (bytes)
-        *bytes = s->nnodes * (s->width == 4 ? sizeof(GpuNode4) : sizeof(GpuNode)) + s->tri_m.size() * (48 + 32) +
+        *bytes = s->nnodes * (s->width != 4 ? sizeof(GpuNode) : s->quantized ? sizeof(GpuNodeQ) : sizeof(GpuNode4)) +
+                 s->tri_m.size() * (48 + 32) +
                  (s->verts.size() + s->normals.size()) / 3 * 16;
     if (build_ms) *build_ms = s->build_ms;
     if (depth) *depth = s->max_depth;
     if (width) *width = s->width;
+    if (node_bytes) *node_bytes = s->width != 4 ? (int32_t)sizeof(GpuNode) : s->quantized ? (int32_t)sizeof(GpuNodeQ)
+                                                                                       : (int32_t)sizeof(GpuNode4);
     return BSH_OK;
 }
 

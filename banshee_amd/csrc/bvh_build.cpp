@@ -13,6 +13,83 @@
 namespace bsh {
 namespace {
 
+// GpuNode4 -> GpuNodeQ, node for node (refs and counts unchanged). Per node and axis the grid
+// step 2^e is the smallest power of two with 254 steps covering the children's extent and at
+// least 2^-17 of the scene's largest coordinate (the root node's bounds), so a plane moved
+// outward by half a step keeps a margin far above the slab test's rounding error at any
+// coordinate in the scene. Lower planes round down to floor(x - 1/2) (clamped at the node's
+// origin, which is a child's own padded bound), upper planes up to ceil(x + 1/2) <= 255.
+// Returns false (no quantised layout) when an exponent leaves [-100, 60], where 2^e times an
+// inverse direction could leave the normal float range.
+bool quantize4(const std::vector<GpuNode4>& in, std::vector<GpuNodeQ>& out)
+{
+    out.assign(in.size(), GpuNodeQ{});
+    if (in.empty()) return false;
+    double mag = 0.0;
+    {
+        const GpuNode4& r = in[0];
+        for (int c = 0; c < 4; ++c)
+            if (r.cnt[c] >= 0)
+                for (float v : {r.lox[c], r.hix[c], r.loy[c], r.hiy[c], r.loz[c], r.hiz[c]})
+                    mag = std::max(mag, std::fabs((double)v));
+    }
+    const double floor_step = std::max(mag * std::ldexp(1.0, -17), std::ldexp(1.0, -100));
+    for (size_t i = 0; i < in.size(); ++i)
+    {
+        const GpuNode4& g = in[i];
+        GpuNodeQ& q = out[i];
+        const float* lo[3] = {g.lox, g.loy, g.loz};
+        const float* hi[3] = {g.hix, g.hiy, g.hiz};
+        uint32_t* qlo[3] = {&q.qlox, &q.qloy, &q.qloz};
+        uint32_t* qhi[3] = {&q.qhix, &q.qhiy, &q.qhiz};
+        q.exps = 0;
+        q.cnts = 0;
+        q.pad = 0;
+        for (int c = 0; c < 4; ++c)
+        {
+            q.ref[c] = g.ref[c];
+            q.cnts |= (uint32_t)(uint8_t)(int8_t)g.cnt[c] << (8 * c);
+        }
+        for (int a = 0; a < 3; ++a)
+        {
+            float o = std::numeric_limits<float>::max(), h = -std::numeric_limits<float>::max();
+            for (int c = 0; c < 4; ++c)
+                if (g.cnt[c] >= 0)
+                {
+                    o = std::min(o, lo[a][c]);
+                    h = std::max(h, hi[a][c]);
+                }
+            if (!(o <= h)) return false;   // a node has at least two children
+            const double ext = (double)h - (double)o;
+            int e;
+            std::frexp(std::max(ext / 254.0, floor_step), &e);   // 2^(e-1) <= x < 2^e
+            if (std::ldexp(1.0, e - 1) >= std::max(ext / 254.0, floor_step)) --e;
+            if (e < -100 || e > 60) return false;
+            const double step = std::ldexp(1.0, e);
+            q.origin[a] = o;
+            q.exps |= (uint32_t)(e + 127) << (8 * a);
+            uint32_t wl = 0, wh = 0;
+            for (int c = 0; c < 4; ++c)
+            {
+                uint32_t l = 255, u = 0;   // empty child: an inverted box (its count is -1 too)
+                if (g.cnt[c] >= 0)
+                {
+                    const double xl = ((double)lo[a][c] - (double)o) / step;
+                    const double xh = ((double)hi[a][c] - (double)o) / step;
+                    l = (uint32_t)std::max(0.0, std::floor(xl - 0.5));
+                    u = (uint32_t)std::ceil(xh + 0.5);
+                    if (u > 255u) return false;   // ext <= 254 steps makes this unreachable
+                }
+                wl |= l << (8 * c);
+                wh |= u << (8 * c);
+            }
+            *qlo[a] = wl;
+            *qhi[a] = wh;
+        }
+    }
+    return true;
+}
+
 struct Aabb
 {
     float lo[3], hi[3];
@@ -106,6 +183,9 @@ public:
             if (3 * out.max_depth + 1 <= kMaxStack4)
             {
                 out.width = 4;
+                const bool want = p_.quantized > 0 ||
+                                  (p_.quantized == 0 && out.nodes4.size() * sizeof(GpuNode4) > kAutoQuantizeBytes);
+                out.quantized = want && quantize4(out.nodes4, out.nodesq);
                 return;
             }
             out.nodes4.clear();

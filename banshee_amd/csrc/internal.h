@@ -213,7 +213,7 @@ struct bsh_scene
     bool has_cam = false;
     bool committed = false;
 
-    bsh::DevBuf<float4> d_nodes;   // GpuNode (width 2) or GpuNode4 (width 4) as float4s
+    bsh::DevBuf<float4> d_nodes;   // GpuNode (width 2), GpuNode4 or GpuNodeQ (width 4) as float4s
     bsh::DevBuf<float4> d_tris, d_stri;
     bsh::DevBuf<bsh::LightDev> d_lights;
     bsh::DevBuf<float> d_light_cdf;
@@ -230,6 +230,7 @@ struct bsh_scene
     float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {0.f, 0.f, 0.f};
     uint64_t nnodes = 0;
     int width = 2;
+    bool quantized = false;        // width 4 in GpuNodeQ records
     int max_depth = 0;
     double build_ms = 0.0;
 
@@ -239,6 +240,7 @@ struct bsh_scene
         bsh::SceneDev s;
         s.nodes = d_nodes.p;
         s.width = width;
+        s.qnodes = quantized ? 1 : 0;
         s.tris = d_tris.p;
         s.stri = d_stri.p;
         s.lights = d_lights.p;

@@ -61,6 +61,10 @@ constexpr int kRecVerts = 4, kRecStride = 5 * kRecVerts;
 static_assert(kGuideSeg == kBlock, "guide row segments are the blocks of k_guide_inputs");
 constexpr int kMaxTraceBlocksPerCU = 5;
 constexpr int kTraceBlocksPerLane = 3;   // the default when two lanes share more than two chunks
+// k_trace_refill's launch bound: registers for 6 workgroups per CU (6 waves per SIMD, at most
+// 80 VGPRs), so both lanes' default grids (3 per CU each) are resident at once whatever the
+// node layout (the quantised layout's decode would otherwise take 89).
+constexpr int kTraceResidentBlocks = 2 * kTraceBlocksPerLane;
 // The shade kernel is held to 128 VGPRs: 4 waves per SIMD (at 129 it would run 3).
 constexpr int kShadeWavesPerSimd = 4;
 // Statistics buffer: 4 counters, then the guided-row partials, kRowStride apart (128 B), then per
@@ -479,9 +483,10 @@ __device__ __forceinline__ uint32_t pop_top(lu32* stk, gu32* ovf, int ostride, i
 // more lanes of its wave are idle (or the whole wave is), when the wave claims exactly that
 // many new rays with one atomic and hands them to the idle lanes in lane order. Lanes of one
 // wave may hold closest and any-hit rays of different scenes at once. Node visits and
-// accept rules are those of traverse4, so every ray's result is the same.
-template <bool kCount>
-__global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(RenderDev R, int L, int buf, int refill,
+// accept rules are those of traverse4, so every ray's result is the same. kQ: every scene of
+// the launch has GpuNodeQ nodes (else every one has GpuNode4 nodes).
+template <bool kCount, bool kQ>
+__global__ __launch_bounds__(kBlock, kTraceResidentBlocks) void k_trace_refill(RenderDev R, int L, int buf, int refill,
                                                                                uint32_t claim_rays)
 {
     __shared__ uint32_t stk[kTraceLds * kBlock];
@@ -623,26 +628,14 @@ __global__ __launch_bounds__(kBlock, kMaxTraceBlocksPerCU) void k_trace_refill(R
         else
         {
             if (kCount) (any ? vis_nodes_any : vis_nodes)++;
-            const size_t nb = 8 * (size_t)node;
-            const float4 LX = gload(nodes, nb), HX = gload(nodes, nb + 1), LY = gload(nodes, nb + 2),
-                         HY = gload(nodes, nb + 3), LZ = gload(nodes, nb + 4), HZ = gload(nodes, nb + 5);
-            const int4 REF = gload_i(nodes, nb + 6);
-            const int4 CNT = gload_i(nodes, nb + 7);
-            const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, hx[4] = {HX.x, HX.y, HX.z, HX.w};
-            const float ly[4] = {LY.x, LY.y, LY.z, LY.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w};
-            const float lz[4] = {LZ.x, LZ.y, LZ.z, LZ.w}, hz[4] = {HZ.x, HZ.y, HZ.z, HZ.w};
-            const int ref[4] = {REF.x, REF.y, REF.z, REF.w}, cn[4] = {CNT.x, CNT.y, CNT.z, CNT.w};
+            int ref[4], cn[4];
+            float tn[4];
             bool hitc[4];
+            node4_test<kQ>(nodes, node, ix, iy, iz, ox, oy, oz, tmin, tcur, tn, hitc, ref, cn);
 #pragma unroll
             for (int c = 0; c < 4; ++c)
             {
-                float x0 = __builtin_fmaf(lx[c], ix, -ox), x1 = __builtin_fmaf(hx[c], ix, -ox);
-                float y0 = __builtin_fmaf(ly[c], iy, -oy), y1 = __builtin_fmaf(hy[c], iy, -oy);
-                float z0 = __builtin_fmaf(lz[c], iz, -oz), z1 = __builtin_fmaf(hz[c], iz, -oz);
-                const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-                const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tcur));
-                hitc[c] = (tn <= tf) && (cn[c] >= 0);
-                key[c] = (hitc[c] && cn[c] == 0) ? tn : kInf;
+                key[c] = (hitc[c] && cn[c] == 0) ? tn[c] : kInf;
                 id[c] = (uint32_t)ref[c];
             }
             // the hit leaf children, tested from the next iteration on
@@ -1706,15 +1699,20 @@ static bsh_status enqueue_batch(bsh_context* c, const Lane* lanes, int nlanes, c
         hipStream_t st = ln.st;
         const int launches = ch.depth + 1;
         R.P = ch.P;
-        bool full = false, lights = false, wide = refill > 0;
+        bool full = false, lights = false, wide = refill > 0, qall = true, qany = false;
         for (const Segment& sg : ch.segs)
         {
+            qall = qall && hjobs[sg.job].s.qnodes;
+            qany = qany || hjobs[sg.job].s.qnodes;
             full = full || hjobs[sg.job].flags != 0;
             // point, directional and environment lights and the materials of rows 3-7 are compiled
             // into the kLights variants only
             lights = lights || hjobs[sg.job].s.narea != hjobs[sg.job].s.nlights || hjobs[sg.job].s.ext_mats;
             wide = wide && hjobs[sg.job].s.width == 4;
         }
+        // the refill kernel is compiled for one 4-wide node layout; a chunk that mixes the float
+        // and quantised layouts is traced by the queue kernel, which reads the layout per ray
+        wide = wide && qall == qany;
         hipLaunchKernelGGL(k_zero, grid_for(w.counters.n), dim3(kBlock), 0, st, w.counters.p, (int)w.counters.n);
         // guided probe jobs: the network rows of their probes (vertex -1 of every probe path)
         // come first, one row per probe, so the probe rays can be drawn from the mixture
@@ -1755,12 +1753,18 @@ static bsh_status enqueue_batch(bsh_context* c, const Lane* lanes, int nlanes, c
             }
             if (wide)
             {
-                if (count)
-                    hipLaunchKernelGGL(k_trace_refill<true>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1, refill,
-                                       claim);
+                if (count && qall)
+                    hipLaunchKernelGGL((k_trace_refill<true, true>), dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1,
+                                       refill, claim);
+                else if (count)
+                    hipLaunchKernelGGL((k_trace_refill<true, false>), dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1,
+                                       refill, claim);
+                else if (qall)
+                    hipLaunchKernelGGL((k_trace_refill<false, true>), dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1,
+                                       refill, claim);
                 else
-                    hipLaunchKernelGGL(k_trace_refill<false>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1, refill,
-                                       claim);
+                    hipLaunchKernelGGL((k_trace_refill<false, false>), dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1,
+                                       refill, claim);
             }
             else if (count)
                 hipLaunchKernelGGL(k_trace_queue<true>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1);
@@ -2118,8 +2122,8 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
         put(&tune, sizeof tune);
         for (const JobDev& J : hjobs)
         {
-            const int f[12] = {J.type, J.s_count, J.N, J.cw, J.ch, J.nprobes, J.depth, J.flags,
-                               J.s.narea, J.s.nlights, J.s.ext_mats, J.s.width};
+            const int f[13] = {J.type, J.s_count, J.N, J.cw, J.ch, J.nprobes, J.depth, J.flags,
+                               J.s.narea, J.s.nlights, J.s.ext_mats, J.s.width, J.s.qnodes};
             put(f, sizeof f);
         }
         for (int k = 0; k < nlanes; ++k) put(&lanes[k].R, sizeof(RenderDev));

@@ -72,6 +72,7 @@ struct SceneDev
     int ext_mats;             // 1 when a material row is of type 3-7 (material_dev.h)
     int ntris;
     int width;                // BVH layout: 2 (GpuNode, 4 float4) or 4 (GpuNode4, 8 float4)
+    int qnodes;               // width 4: 1 when the nodes are GpuNodeQ (4 float4)
 };
 
 struct TraceResult
@@ -223,7 +224,73 @@ __device__ __forceinline__ uint32_t stack_pop(SP stk, int stride, OP ovf, int os
     return (kLds >= kCap || sp < kLds) ? stk[sp * stride] : ovf[(sp - kLds) * ostride];
 }
 
-// 4-wide traversal over GpuNode4: one visit loads the node's 128 B, tests four boxes,
+// One 4-wide node's slab tests: entry distance tn[c], hit[c] (the box overlaps [tmin, tcur]
+// and the child is not empty), ref[c] and count cn[c], from a GpuNode4 (kQ false: 8 float4
+// per node) or a GpuNodeQ (kQ true: 4 float4; a plane origin + q 2^e is met at
+// q (2^e / d) + (origin / d - o / d), one conversion and one fma, 2^e built from its
+// exponent byte, so 2^e / d is exact).
+template <bool kQ>
+__device__ __forceinline__ void node4_test(const float4* nodes, uint32_t node, float ix, float iy, float iz, float ox,
+                                           float oy, float oz, float tmin, float tcur, float (&tn)[4], bool (&hit)[4],
+                                           int (&ref)[4], int (&cn)[4])
+{
+    if (kQ)
+    {
+        const size_t nb = 4 * (size_t)node;
+        const float4 W0 = gload(nodes, nb);
+        const int4 I1 = gload_i(nodes, nb + 1), I2 = gload_i(nodes, nb + 2);
+        const uint4 W1 = make_uint4((uint32_t)I1.x, (uint32_t)I1.y, (uint32_t)I1.z, (uint32_t)I1.w);
+        const uint4 W2 = make_uint4((uint32_t)I2.x, (uint32_t)I2.y, (uint32_t)I2.z, (uint32_t)I2.w);
+        const int4 REF = gload_i(nodes, nb + 3);
+        const uint32_t ex = __float_as_uint(W0.w);
+        const float sx = __uint_as_float((ex & 0xffu) << 23) * ix;
+        const float sy = __uint_as_float(((ex >> 8) & 0xffu) << 23) * iy;
+        const float sz = __uint_as_float(((ex >> 16) & 0xffu) << 23) * iz;
+        const float bx = __builtin_fmaf(W0.x, ix, -ox), by = __builtin_fmaf(W0.y, iy, -oy),
+                    bz = __builtin_fmaf(W0.z, iz, -oz);
+        ref[0] = REF.x; ref[1] = REF.y; ref[2] = REF.z; ref[3] = REF.w;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+            const int sh = 8 * c;
+            const float x0 = __builtin_fmaf((float)((W1.x >> sh) & 0xffu), sx, bx);
+            const float x1 = __builtin_fmaf((float)((W1.y >> sh) & 0xffu), sx, bx);
+            const float y0 = __builtin_fmaf((float)((W1.z >> sh) & 0xffu), sy, by);
+            const float y1 = __builtin_fmaf((float)((W1.w >> sh) & 0xffu), sy, by);
+            const float z0 = __builtin_fmaf((float)((W2.x >> sh) & 0xffu), sz, bz);
+            const float z1 = __builtin_fmaf((float)((W2.y >> sh) & 0xffu), sz, bz);
+            cn[c] = (int)(int8_t)(uint8_t)((W2.z >> sh) & 0xffu);
+            tn[c] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+            const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tcur));
+            hit[c] = (tn[c] <= tf) && (cn[c] >= 0);
+        }
+    }
+    else
+    {
+        const size_t nb = 8 * (size_t)node;
+        const float4 LX = gload(nodes, nb), HX = gload(nodes, nb + 1), LY = gload(nodes, nb + 2),
+                     HY = gload(nodes, nb + 3), LZ = gload(nodes, nb + 4), HZ = gload(nodes, nb + 5);
+        const int4 REF = gload_i(nodes, nb + 6);
+        const int4 CNT = gload_i(nodes, nb + 7);
+        const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, hx[4] = {HX.x, HX.y, HX.z, HX.w};
+        const float ly[4] = {LY.x, LY.y, LY.z, LY.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w};
+        const float lz[4] = {LZ.x, LZ.y, LZ.z, LZ.w}, hz[4] = {HZ.x, HZ.y, HZ.z, HZ.w};
+        ref[0] = REF.x; ref[1] = REF.y; ref[2] = REF.z; ref[3] = REF.w;
+        cn[0] = CNT.x; cn[1] = CNT.y; cn[2] = CNT.z; cn[3] = CNT.w;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+            const float x0 = __builtin_fmaf(lx[c], ix, -ox), x1 = __builtin_fmaf(hx[c], ix, -ox);
+            const float y0 = __builtin_fmaf(ly[c], iy, -oy), y1 = __builtin_fmaf(hy[c], iy, -oy);
+            const float z0 = __builtin_fmaf(lz[c], iz, -oz), z1 = __builtin_fmaf(hz[c], iz, -oz);
+            tn[c] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+            const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tcur));
+            hit[c] = (tn[c] <= tf) && (cn[c] >= 0);
+        }
+    }
+}
+
+// 4-wide traversal over GpuNode4 or GpuNodeQ: one visit loads the node (128 or 64 B), tests four boxes,
 // intersects leaf children at once, then visits the hit internal children nearest first
 // (a 5-comparator sorting network on their entry distances; the far ones are pushed,
 // farthest first). Same accept rules as traverse(). The builder bounds the stack depth
@@ -246,28 +313,12 @@ __device__ __forceinline__ bool traverse4(const SceneDev& s, f3 o, f3 d, float t
     int sp = 0;
     for (;;)
     {
-        const size_t nb = 8 * (size_t)node;
-        const float4 LX = gload(s.nodes, nb), HX = gload(s.nodes, nb + 1), LY = gload(s.nodes, nb + 2),
-                     HY = gload(s.nodes, nb + 3), LZ = gload(s.nodes, nb + 4), HZ = gload(s.nodes, nb + 5);
-        const int4 REF = gload_i(s.nodes, nb + 6);
-        const int4 CNT = gload_i(s.nodes, nb + 7);
         if (kCount) cnt.nodes++;
-        const float lx[4] = {LX.x, LX.y, LX.z, LX.w}, hx[4] = {HX.x, HX.y, HX.z, HX.w};
-        const float ly[4] = {LY.x, LY.y, LY.z, LY.w}, hy[4] = {HY.x, HY.y, HY.z, HY.w};
-        const float lz[4] = {LZ.x, LZ.y, LZ.z, LZ.w}, hz[4] = {HZ.x, HZ.y, HZ.z, HZ.w};
-        const int ref[4] = {REF.x, REF.y, REF.z, REF.w}, cn[4] = {CNT.x, CNT.y, CNT.z, CNT.w};
+        int ref[4], cn[4];
         float tn[4];
         bool hit[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-        {
-            float x0 = __builtin_fmaf(lx[c], ix, -ox), x1 = __builtin_fmaf(hx[c], ix, -ox);
-            float y0 = __builtin_fmaf(ly[c], iy, -oy), y1 = __builtin_fmaf(hy[c], iy, -oy);
-            float z0 = __builtin_fmaf(lz[c], iz, -oz), z1 = __builtin_fmaf(hz[c], iz, -oz);
-            tn[c] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-            float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tcur));
-            hit[c] = (tn[c] <= tf) && (cn[c] >= 0);
-        }
+        if (s.qnodes) node4_test<true>(s.nodes, node, ix, iy, iz, ox, oy, oz, tmin, tcur, tn, hit, ref, cn);
+        else node4_test<false>(s.nodes, node, ix, iy, iz, ox, oy, oz, tmin, tcur, tn, hit, ref, cn);
 #pragma unroll
         for (int c = 0; c < 4; ++c)
         {

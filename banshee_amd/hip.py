@@ -50,7 +50,7 @@ class Camera(ctypes.Structure):
 
 class BvhParams(ctypes.Structure):
     _fields_ = [("max_leaf", ctypes.c_int32), ("sah_bins", ctypes.c_int32), ("threads", ctypes.c_int32),
-                ("width", ctypes.c_int32)]
+                ("width", ctypes.c_int32), ("quantized", ctypes.c_int32)]
 
 
 class RenderParams(ctypes.Structure):
@@ -598,20 +598,27 @@ class Guide:
             pass
 
 
-def bvh_build_host(tri_verts, max_leaf=4, sah_bins=16, threads=0, width=0):
+def _quant_flag(quantized):
+    """bsh_bvh_params.quantized: None -> 0 (automatic: quantised past 64 MiB of float nodes),
+    True -> 1, False -> -1."""
+    return 0 if quantized is None else 1 if quantized else -1
+
+
+def bvh_build_host(tri_verts, max_leaf=4, sah_bins=16, threads=0, width=0, quantized=None):
     """Host-only BVH build (no device): returns (nodes float32 [n, 16 or 32], tri_order, width, depth).
-    Node rows are the raw 64-B (width 2) or 128-B (width 4) records; the int fields are
-    returned by viewing the row as int32."""
+    Node rows are the raw 64-B (binary, or quantised 4-wide) or 128-B (float 4-wide) records; the
+    int fields are returned by viewing the row as int32."""
     tv = np.ascontiguousarray(tri_verts, np.float32).reshape(-1, 9)
-    bp = BvhParams(max_leaf, sah_bins, threads, width)
-    n, w, d = c_size_t(), ctypes.c_int32(), ctypes.c_int32()
+    bp = BvhParams(max_leaf, sah_bins, threads, width, _quant_flag(quantized))
+    n, w, d, nb = c_size_t(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
     _check(lib().bsh_bvh_build_host(_p(tv), c_size_t(tv.shape[0]), ctypes.byref(bp), None, c_size_t(0), None,
-                                    ctypes.byref(n), ctypes.byref(w), ctypes.byref(d)), "bsh_bvh_build_host")
-    cols = 32 if w.value == 4 else 16
+                                    ctypes.byref(n), ctypes.byref(w), ctypes.byref(d), ctypes.byref(nb)),
+           "bsh_bvh_build_host")
+    cols = nb.value // 4
     nodes = np.zeros((n.value, cols), np.float32)
     order = np.zeros(tv.shape[0], np.int32)
     _check(lib().bsh_bvh_build_host(_p(tv), c_size_t(tv.shape[0]), ctypes.byref(bp), _p(nodes), c_size_t(n.value),
-                                    _p(order), ctypes.byref(n), ctypes.byref(w), ctypes.byref(d)),
+                                    _p(order), ctypes.byref(n), ctypes.byref(w), ctypes.byref(d), None),
            "bsh_bvh_build_host")
     return nodes, order, w.value, d.value
 
@@ -628,7 +635,9 @@ def camera_struct(cam13) -> Camera:
 class Scene:
     """A committed device scene (World after Commit) built from a banshee_amd.scenes.Scene."""
 
-    def __init__(self, ctx: Context, sc, max_leaf=4, sah_bins=16, threads=0, width=0):
+    def __init__(self, ctx: Context, sc, max_leaf=4, sah_bins=16, threads=0, width=0, quantized=None):
+        """quantized: None lets the builder choose (quantised 4-wide nodes past 64 MiB of float
+        nodes), True / False force the quantised / float 4-wide layout."""
         self.ctx = ctx
         self.sc = sc
         self.h = c_void_p()
@@ -692,7 +701,7 @@ class Scene:
                 arr[k].scale, arr[k].gamma = float(r[7]), float(r[8])
             _check(lib().bsh_scene_set_other_lights(self.h, arr, c_size_t(rows.shape[0])), "bsh_scene_set_other_lights")
         self.set_camera(sc.camera)
-        bp = BvhParams(max_leaf, sah_bins, threads, width)
+        bp = BvhParams(max_leaf, sah_bins, threads, width, _quant_flag(quantized))
         _check(lib().bsh_scene_commit(self.h, ctypes.byref(bp)), "bsh_scene_commit")
 
     def set_camera(self, cam13):
@@ -701,11 +710,13 @@ class Scene:
 
     def info(self):
         nodes, tris, nbytes = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
-        ms, depth, width = ctypes.c_double(), ctypes.c_int32(), ctypes.c_int32()
+        ms, depth, width, nb = ctypes.c_double(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         _check(lib().bsh_scene_info(self.h, ctypes.byref(nodes), ctypes.byref(tris), ctypes.byref(nbytes),
-                                    ctypes.byref(ms), ctypes.byref(depth), ctypes.byref(width)), "bsh_scene_info")
+                                    ctypes.byref(ms), ctypes.byref(depth), ctypes.byref(width), ctypes.byref(nb)),
+               "bsh_scene_info")
         return {"nodes": nodes.value, "tris": tris.value, "bytes": nbytes.value, "build_ms": ms.value,
-                "depth": depth.value, "width": width.value}
+                "depth": depth.value, "width": width.value, "quantized": width.value == 4 and nb.value == 64,
+                "node_bytes": nb.value}
 
     def diag_material(self, material, seed, frames, view, u, ea, eb):
         """Device Material::Sample / Evaluate / GetPdf of row `material` (bsh_diag_material): n x 12."""

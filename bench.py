@@ -186,11 +186,16 @@ def training_rate(ctx, rows=1 << 18, steps=30) -> dict:
             "ms_per_step": round(s * 1e3, 4)}
 
 
-def guided_comparison(ctx, bvh_width) -> dict:
+def qflag(v: int):
+    """--bvh-quantized -> hip.Scene(quantized=...): 0 automatic (None), 1 True, -1 False."""
+    return None if v == 0 else v > 0
+
+
+def guided_comparison(ctx, bvh_width, quantized=None) -> dict:
     from banshee_amd import hip
     from banshee_amd.scenes import occluded_room
     sc = occluded_room(width=G_RES, height=G_RES)
-    scene = hip.Scene(ctx, sc, width=bvh_width)
+    scene = hip.Scene(ctx, sc, width=bvh_width, quantized=quantized)
     scene.render_guided(G_RES, G_RES, 1, G_DEPTH, 1, download=False)   # warm the wavefront buffers
     ctx.sync()
     p = time_to_rmse(ctx, scene, None, 4096, 100000)
@@ -217,7 +222,7 @@ def guided_comparison(ctx, bvh_width) -> dict:
                       "spp_per_pass": G_PASS_GRID ** 2}}
 
 
-def rmse_check(ctx, bvh_width, ref_spp=4096) -> dict:
+def rmse_check(ctx, bvh_width, ref_spp=4096, quantized=None) -> dict:
     """The guided-vs-plain comparison at the headline RMSE target and at half of it, each run's
     final image also scored against a converged plain render of ref_spp samples per pixel:
     RMSE_true = sqrt(mean (m - m_ref)^2 - mean var(m_ref)), the reference's own variance of the
@@ -226,7 +231,7 @@ def rmse_check(ctx, bvh_width, ref_spp=4096) -> dict:
     from banshee_amd import hip
     from banshee_amd.scenes import occluded_room
     sc = occluded_room(width=G_RES, height=G_RES)
-    scene = hip.Scene(ctx, sc, width=bvh_width)
+    scene = hip.Scene(ctx, sc, width=bvh_width, quantized=quantized)
     ref = ctx.alloc((G_RES, G_RES, 8), np.float32)
     for k in range(ref_spp // 64):
         scene.render_guided(G_RES, G_RES, 8, G_DEPTH, 900000 + k, accum=ref, accumulate=k > 0, download=False)
@@ -256,14 +261,14 @@ def rmse_check(ctx, bvh_width, ref_spp=4096) -> dict:
     return out
 
 
-def measured_traffic(width: int):
+def measured_traffic(layout: str):
     """HBM bytes per trace launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
     of this workload (calibrated against a copy kernel), or None."""
     import re
     path = os.path.join(ROOT, "profiles", "k_trace_refill_terrain3M_pmc.md")
     if not os.path.exists(path):
         return None
-    m = re.search(rf"traffic_bytes_per_launch\[width={width}\]: (\d+)", open(path).read())
+    m = re.search(rf"traffic_bytes_per_launch\[width={layout}\]: (\d+)", open(path).read())
     return int(m.group(1)) if m else None
 
 
@@ -279,7 +284,7 @@ def batch_workload(args) -> dict:
     scenes, jobs = [], []
     for c in cfgs:
         sc = terrain_for_tris(c["tris"], seed=c["seed"], width=c["res"], height=c["res"])
-        s = hip.Scene(ctx, sc, width=args.bvh_width)
+        s = hip.Scene(ctx, sc, width=args.bvh_width, quantized=qflag(args.bvh_quantized))
         scenes.append(s)
         jobs.append(hip.BatchJob(s, "render", width=c["res"], height=c["res"], img_grid=c["grid"],
                                  max_depth=c["depth"], seed=c["seed"]))
@@ -329,7 +334,7 @@ def probe_workload(args) -> dict:
     from banshee_amd.scenes import probe_grid, terrain_for_tris
     ctx = make_context(args)
     sc = terrain_for_tris(2_000_000, seed=7)
-    scene = hip.Scene(ctx, sc, width=args.bvh_width)
+    scene = hip.Scene(ctx, sc, width=args.bvh_width, quantized=qflag(args.bvh_quantized))
     probes = probe_grid(sc, 16)
     coeffs = ctx.alloc((len(probes), 9, 3), "float32")
     for _ in range(args.warmup):
@@ -480,7 +485,7 @@ def batch_line(ctx, args) -> dict:
     scenes, jobs = [], []
     for c in cfgs:
         s = hip.Scene(ctx, terrain_for_tris(c["tris"], seed=c["seed"], width=c["res"], height=c["res"]),
-                      width=args.bvh_width)
+                      width=args.bvh_width, quantized=qflag(args.bvh_quantized))
         scenes.append(s)
         jobs.append(hip.BatchJob(s, "render", width=c["res"], height=c["res"], img_grid=c["grid"],
                                  max_depth=c["depth"], seed=c["seed"]))
@@ -513,7 +518,7 @@ def probes_line(ctx, args) -> dict:
     from banshee_amd import hip
     from banshee_amd.scenes import probe_grid, terrain_for_tris
     sc = terrain_for_tris(2_000_000, seed=7)
-    scene = hip.Scene(ctx, sc, width=args.bvh_width)
+    scene = hip.Scene(ctx, sc, width=args.bvh_width, quantized=qflag(args.bvh_quantized))
     probes = probe_grid(sc, 16)
     coeffs = ctx.alloc((len(probes), 9, 3), "float32")
     scene.bake_sh(probes, 1024, 2, 6, 1, coeffs=coeffs, download=False)
@@ -541,6 +546,9 @@ def main():
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain time-to-RMSE comparison")
     ap.add_argument("--no-extra", action="store_true", help="skip the batch and probe-baking figures")
     ap.add_argument("--bvh-width", type=int, default=4, choices=(2, 4), help="BVH node width")
+    ap.add_argument("--bvh-quantized", type=int, default=0, choices=(-1, 0, 1),
+                    help="width 4: 1 64-B nodes with 8-bit child boxes, -1 128-B float nodes, 0 the "
+                         "builder's choice (quantised past 64 MiB of float nodes)")
     ap.add_argument("--paths-per-chunk", type=int, default=0,
                     help="paths in flight per wavefront chunk (0 = library default, 2^23)")
     ap.add_argument("--refill-lanes", type=int, default=0,
@@ -566,10 +574,10 @@ def main():
     from banshee_amd.scenes import terrain_scene
 
     if args.rmse_check:
-        print(json.dumps(rmse_check(make_context(args), args.bvh_width)), flush=True)
+        print(json.dumps(rmse_check(make_context(args), args.bvh_width, quantized=qflag(args.bvh_quantized))), flush=True)
         return
     if args.guided_only:
-        print(json.dumps(guided_comparison(make_context(args), args.bvh_width)), flush=True)
+        print(json.dumps(guided_comparison(make_context(args), args.bvh_width, qflag(args.bvh_quantized))), flush=True)
         return
     if args.workload == "batch":
         print(json.dumps(batch_workload(args)), flush=True)
@@ -581,14 +589,15 @@ def main():
     t0 = time.time()
     sc = terrain_scene(width=W, height=H)
     ctx = make_context(args)
-    scene = hip.Scene(ctx, sc, width=args.bvh_width)
+    scene = hip.Scene(ctx, sc, width=args.bvh_width, quantized=qflag(args.bvh_quantized))
     info = scene.info()
     setup_s = time.time() - t0
     accum = ctx.alloc((H, W, 8), "float32")
 
     # counting frames (untimed): node visits and triangle tests per ray class, with this layout and,
     # for the layout-independent figure, with the binary layout of the same triangles
-    s_node = 128 if info["width"] == 4 else 64
+    s_node = int(info["node_bytes"])
+    layout = f"{info['width']}q" if info["quantized"] else str(info["width"])
     scene.render(W, H, 2, DEPTH, SEED, accum=accum, download=False, count=True)
     ctx.sync()
     classes = ray_classes(ctx.launch_stats(), s_node, 48)
@@ -650,11 +659,11 @@ def main():
         "config": {"workload": "plain wavefront GPU path tracing, seeded terrain scene (heightfield 512x512 + 2000 "
                                "icospheres subdiv 3, GGX/Lambert 50/50, 2 quad area lights), 1920x1080, 64 spp, depth 8",
                    "tris": int(info["tris"]), "bvh_nodes": int(info["nodes"]), "bvh_depth": int(info["depth"]),
-                   "bvh_width": int(info["width"]),
+                   "bvh_width": int(info["width"]), "bvh_node_bytes": int(info["node_bytes"]),
                    "samples_per_s": round(samples / dt, 1), "rays_per_step": rays // args.steps,
                    "setup_s": round(setup_s, 2), "bvh_build_ms": round(info["build_ms"], 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(int(info["width"])),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(layout),
                      "traffic_source": "profiles/k_trace_refill_terrain3M_pmc.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
                      "kernel": "k_trace_refill" if int(info["width"]) == 4 and args.refill_lanes != -1 else "k_trace_queue",
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4),
@@ -679,7 +688,7 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_guided:
-        out["guided"] = guided_comparison(ctx, args.bvh_width)
+        out["guided"] = guided_comparison(ctx, args.bvh_width, qflag(args.bvh_quantized))
     if not args.no_extra:
         out["batch"] = batch_line(ctx, args)
         out["probes"] = probes_line(ctx, args)

@@ -121,6 +121,11 @@ typedef struct
     int32_t threads;    /* host build threads, 0 = hardware concurrency */
     int32_t width;      /* node width: 2 (64-B binary nodes) or 4 (128-B 4-wide nodes collapsed from the
                            binary SAH tree); 0 = 4. A tree too deep for the 4-wide stack stays binary. */
+    int32_t quantized;  /* width 4 only: 1 stores the same tree in 64-B nodes with 8-bit child boxes
+                           (conservative, so closest hits are unchanged), -1 in 128-B float nodes,
+                           0 = quantised when the float nodes would take more than 64 MiB. The
+                           layout built is reported as node bytes by bsh_scene_info and
+                           bsh_bvh_build_host (64 for binary or quantised, 128 for float 4-wide). */
 } bsh_bvh_params;
 
 /* Frame parameters for bsh_render. The estimator is GiTracer(max_depth)
@@ -303,13 +308,16 @@ bsh_status bsh_scene_commit(bsh_scene* s, const bsh_bvh_params* params);
 bsh_status bsh_scene_destroy(bsh_scene* s);
 /* The host half of bsh_scene_commit on its own (no device needed): builds the BVH of ntris
    triangles (tri_verts: 9 floats each) and, when nodes is not NULL, copies the flattened nodes
-   (GpuNode 64 B or GpuNode4 128 B, per *width; nodes_cap in nodes) and the leaf order
-   (tri_order: ntris ints). Call with nodes = NULL first to learn *nnodes. */
+   (GpuNode 64 B at *width 2; at *width 4 GpuNode4 128 B or GpuNodeQ 64 B, per *node_bytes;
+   nodes_cap in nodes) and the leaf order (tri_order: ntris ints). Call with nodes = NULL first
+   to learn *nnodes. */
 bsh_status bsh_bvh_build_host(const float* tri_verts, size_t ntris, const bsh_bvh_params* params, void* nodes,
-                              size_t nodes_cap, int32_t* tri_order, size_t* nnodes, int32_t* width, int32_t* depth);
-/* Node count, triangle count, device bytes, build milliseconds, max depth, BVH width built. */
+                              size_t nodes_cap, int32_t* tri_order, size_t* nnodes, int32_t* width, int32_t* depth,
+                              int32_t* node_bytes);
+/* Node count, triangle count, device bytes, build milliseconds, max depth, BVH width built and
+   bytes per node (64: binary or quantised 4-wide, 128: float 4-wide). Any output may be NULL. */
 bsh_status bsh_scene_info(const bsh_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* bytes, double* build_ms,
-                          int32_t* depth, int32_t* width);
+                          int32_t* depth, int32_t* width, int32_t* node_bytes);
 
 /* ---------------- ray queries (Intersectable, accelerator/intersectable.h:41-49) ---------------- */
 /* Intersectable::Intersect(ray const&, Hit&) -> Bvh::Intersect closest hit (bvh.cpp:499-573).

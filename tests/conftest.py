@@ -14,6 +14,17 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) and the built HIP library")
 
 
+def scene_of(ctx, sc, layout):
+    """A committed device scene in BVH layout 2, 4 (float 128-B nodes) or "4q" (quantised 64-B
+    nodes), checked to have got that layout."""
+    from banshee_amd import hip
+    s = hip.Scene(ctx, sc, width=4 if layout == "4q" else layout, quantized=layout == "4q")
+    info = s.info()
+    assert info["width"] == (4 if layout == "4q" else layout) and info["quantized"] == (layout == "4q")
+    assert info["node_bytes"] == (128 if layout == 4 else 64)
+    return s
+
+
 def golden(name):
     return np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
 

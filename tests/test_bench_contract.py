@@ -2,7 +2,7 @@
 
 * More than one GPU is refused with a message and a nonzero status, before any device work.
 * The `traffic` figure comes from the committed counter summary: a positive byte count per
-  trace launch for both node layouts, and the summary names the calibration factors used.
+  trace launch for every node layout, and the summary names the calibration factors used.
 * The guide network's FLOP count per row matches its layer shapes (96 -> 64 -> 64 -> 64 -> 64 -> 48).
 """
 import os
@@ -22,9 +22,9 @@ def test_multi_gpu_is_refused_with_a_message():
 
 def test_traffic_comes_from_the_committed_summary():
     import bench
-    for width in (2, 4):
-        t = bench.measured_traffic(width)
-        assert isinstance(t, int) and t > 0, (width, t)
+    for layout in ("2", "4", "4q"):
+        t = bench.measured_traffic(layout)
+        assert isinstance(t, int) and t > 0, (layout, t)
     text = open(os.path.join(ROOT, "profiles", "k_trace_refill_terrain3M_pmc.md")).read()
     assert "FETCH_SIZE" in text and "WRITE_SIZE" in text and "scale factor" in text
 

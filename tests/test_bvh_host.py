@@ -1,10 +1,13 @@
-"""The host BVH builder and both flattened node layouts (CPU only, through the C ABI).
+"""The host BVH builder and the flattened node layouts (CPU only, through the C ABI).
 
 Invariants: every primitive is in exactly one leaf; every child box contains its subtree's
 triangles (boxes are padded outward); inner references are in range; the depth respects the
 traversal stack bound. A numpy traversal of the flattened nodes (visiting every child whose
 box the ray enters) finds the same closest hits as FP64 brute force, so a wrong box, ref,
-count or collapse step in either layout fails here before it reaches a GPU.
+count or collapse step in any layout fails here before it reaches a GPU. The quantised 4-wide
+layout ("4q") must hold the float 4-wide tree node for node, with every child box containing
+the float box it replaces by at least half a grid step (outward rounding), on grids no finer
+than 2^-17 of the scene's largest coordinate.
 """
 import numpy as np
 import pytest
@@ -19,8 +22,32 @@ def tri_verts(sc):
     return sc.verts[sc.tri_v].reshape(-1, 9).astype(np.float32)
 
 
+def decode_q(nodes):
+    """GpuNodeQ rows (16 words): lo [4,3], hi [4,3], ref [4], cnt [4], origin [3], step [3], in FP64."""
+    u = nodes.view(np.uint32)
+    origin = nodes[:, 0:3].astype(np.float64)
+    step = np.stack([2.0 ** (((u[:, 3] >> (8 * a)) & 255).astype(np.int64) - 127) for a in range(3)], -1)
+    def byte(w, c):
+        return ((w >> (8 * c)) & 255).astype(np.float64)
+    lo = np.stack([np.stack([origin[:, a] + byte(u[:, 4 + 2 * a], c) * step[:, a] for a in range(3)], -1)
+                   for c in range(4)], 1)
+    hi = np.stack([np.stack([origin[:, a] + byte(u[:, 5 + 2 * a], c) * step[:, a] for a in range(3)], -1)
+                   for c in range(4)], 1)
+    cnt = np.stack([((u[:, 10] >> (8 * c)) & 255).astype(np.uint8).view(np.int8).astype(np.int32) for c in range(4)], -1)
+    return lo, hi, nodes.view(np.int32)[:, 12:16], cnt, origin, step
+
+
+def build(tv, layout):
+    """(nodes, order, w, depth) for layout 2, 4 or "4q"."""
+    if layout == "4q":
+        return hip.bvh_build_host(tv, width=4, quantized=True)
+    return hip.bvh_build_host(tv, width=layout, quantized=False)
+
+
 def decode(nodes, width):
     """Per node: lo [k,3], hi [k,3], ref [k], cnt [k] for k = width children."""
+    if width == "4q":
+        return decode_q(nodes)[:4]
     if width == 2:
         f = nodes
         i = nodes.view(np.int32)
@@ -39,13 +66,13 @@ def scene():
     return terrain_for_tris(4000, seed=9, width=64, height=64)
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, "4q"])
 def test_layout_invariants(scene, width):
     tv = tri_verts(scene)
-    nodes, order, w, depth = hip.bvh_build_host(tv, width=width)
-    assert w == width
+    nodes, order, w, depth = build(tv, width)
+    assert w == (4 if width == "4q" else width)
     assert sorted(order.tolist()) == list(range(len(tv)))
-    lo, hi, ref, cnt = decode(nodes, w)
+    lo, hi, ref, cnt = decode(nodes, width)
     n = nodes.shape[0]
     seen = np.zeros(len(tv), np.int32)
     tris = tv.reshape(-1, 3, 3)
@@ -60,14 +87,18 @@ def test_layout_invariants(scene, width):
                 assert 0 < ref[k, c] < n
                 child = ref[k, c]
                 occupied = cnt[child] >= 0
-                assert np.all(lo[child][occupied] >= lo[k, c]) and np.all(hi[child][occupied] <= hi[k, c])
+                # quantised boxes are not nested (each node rounds on its own grid); that they
+                # contain the float boxes, which are, is test_quantized_boxes_contain_float_boxes
+                if width != "4q":
+                    assert np.all(lo[child][occupied] >= lo[k, c]) and np.all(hi[child][occupied] <= hi[k, c])
     assert np.all(seen == 1)
     bound = 32 if w == 2 else 48
     assert (depth + 2 if w == 2 else 3 * depth + 1) <= bound
 
 
-def traverse_np(nodes, w, order, tris, o, d, tmin, tmax):
-    lo, hi, ref, cnt = decode(nodes, w)
+def traverse_np(nodes, layout, order, tris, o, d, tmin, tmax):
+    lo, hi, ref, cnt = decode(nodes, layout)
+    w = lo.shape[1]
     inv = 1.0 / np.where(np.abs(d) < 1e-20, np.copysign(1e-20, d), d)
     best, bt = -1, tmax
     stack = [0]
@@ -102,12 +133,12 @@ def traverse_np(nodes, w, order, tris, o, d, tmin, tmax):
     return best, bt
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, "4q"])
 def test_layout_traversal_matches_brute_force(scene, width):
     tv = tri_verts(scene)
-    nodes, order, w, _ = hip.bvh_build_host(tv, width=width)
+    nodes, order, w, _ = build(tv, width)
     tris = tv.reshape(-1, 3, 3)
-    rng = np.random.default_rng(width)
+    rng = np.random.default_rng(7 if width == "4q" else width)
     lo, hi = scene.verts.min(0), scene.verts.max(0)
     from test_oracle_pins import brute_force
     n = 150
@@ -119,7 +150,7 @@ def test_layout_traversal_matches_brute_force(scene, width):
     r64 = rays.astype(np.float64)
     hits = 0
     for i in range(n):
-        p, t = traverse_np(nodes, w, order, tris, r64[i, 0:3], r64[i, 4:7], r64[i, 3], r64[i, 7])
+        p, t = traverse_np(nodes, width, order, tris, r64[i, 0:3], r64[i, 4:7], r64[i, 3], r64[i, 7])
         assert (p >= 0) == (bp[i] >= 0), i
         if p >= 0:
             hits += 1
@@ -134,3 +165,43 @@ def test_four_wide_collapses_levels(scene):
     assert w == 4
     assert len(n4) < 0.6 * len(n2)
     assert d4 < d2
+
+
+@pytest.mark.parametrize("offset", [0.0, 3.0e4])
+def test_quantized_boxes_contain_float_boxes(scene, offset):
+    """The quantised tree is the float 4-wide tree with each child box grown outward: same refs
+    and counts, every plane at least half a grid step outside the float plane (or, for a lower
+    plane, on the node's origin, which is the lowest child's own float plane), and grid steps no
+    finer than 2^-17 of the scene's largest coordinate (the offset scene puts the terrain far
+    from the origin, where that floor decides the grid of the small nodes)."""
+    tv = tri_verts(scene) + np.float32(offset)
+    n4, o4, w4, _ = hip.bvh_build_host(tv, width=4, quantized=False)
+    nq, oq, wq, _ = hip.bvh_build_host(tv, width=4, quantized=True)
+    assert w4 == wq == 4 and nq.shape == (n4.shape[0], 16)
+    assert np.array_equal(o4, oq)
+    lo, hi, ref, cnt = decode(n4, 4)
+    qlo, qhi, qref, qcnt, origin, step = decode_q(nq)
+    assert np.array_equal(ref, qref) and np.array_equal(cnt, qcnt)
+    live = cnt >= 0
+    lo64, hi64 = lo.astype(np.float64), hi.astype(np.float64)
+    half = 0.5 * step[:, None, :]
+    on_origin = qlo == origin[:, None, :]
+    assert np.all(((lo64 - qlo >= half) | (on_origin & (qlo <= lo64)))[live])
+    assert np.all((qhi - hi64 >= half)[live])
+    # the origin is the lowest live child's float plane; 254 steps cover the node
+    olo = np.where(live[..., None], lo64, np.inf).min(1)
+    ohi = np.where(live[..., None], hi64, -np.inf).max(1)
+    assert np.array_equal(origin, olo)
+    assert np.all(ohi - olo <= 254 * step)
+    mag = np.abs(np.concatenate([lo64[0][live[0]], hi64[0][live[0]]])).max()
+    assert np.all(step >= mag * 2.0 ** -17)
+    # the grid is the finest the two rules allow: half the step would break one of them
+    assert np.all((ohi - olo > 127 * step) | (step / 2 < mag * 2.0 ** -17))
+
+
+def test_automatic_layout_threshold(scene):
+    """quantized=None (the default) keeps float nodes for a tree under 64 MiB of them."""
+    tv = tri_verts(scene)
+    na, _, wa, _ = hip.bvh_build_host(tv, width=4)
+    nf, _, wf, _ = hip.bvh_build_host(tv, width=4, quantized=False)
+    assert wa == wf == 4 and na.shape == nf.shape and np.array_equal(na.view(np.int32), nf.view(np.int32))

@@ -14,7 +14,10 @@ def _jobs(ctx):
     scenes, jobs = [], []
     for k, c in enumerate(cfgs):
         sc = terrain_for_tris(c["tris"], seed=c["seed"], width=c["res"], height=c["res"])
-        s = hip.Scene(ctx, sc, width=4 if k % 2 else 2)   # one chunk mixes both node layouts
+        # chunks mix the node layouts: the float and quantised 4-wide ones (traced together by
+        # the queue kernel) and the binary one
+        layout = ["4q", 4, "4q", 2, 4][k]
+        s = hip.Scene(ctx, sc, width=4 if layout == "4q" else layout, quantized=layout == "4q")
         scenes.append(s)
         jobs.append(("render", s, dict(width=c["res"], height=c["res"], img_grid=c["grid"], max_depth=c["depth"],
                                        seed=c["seed"])))

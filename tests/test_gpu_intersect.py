@@ -3,19 +3,16 @@ import numpy as np
 import pytest
 
 from adjudicate import classify
-from conftest import golden
+from conftest import golden, scene_of
 
 pytestmark = pytest.mark.gpu
 
 
 def _scene(ctx, sc, width=2):
-    from banshee_amd import hip
-    s = hip.Scene(ctx, sc, width=width)
-    assert s.info()["width"] == width
-    return s
+    return scene_of(ctx, sc, width)
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, "4q"])
 @pytest.mark.parametrize("name", ["cornell", "terrain"])
 def test_recorded_rays_match_reference(ctx, cornell, terrain100k, name, width):
     fx = golden(f"rays_{name}")
@@ -80,7 +77,7 @@ def _brute_force_fp32(sc, rays):
     return best, tb
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, "4q"])
 def test_brute_force_cornell(ctx, cornell, width):
     rng = np.random.default_rng(5)
     n = 20000
@@ -100,7 +97,7 @@ def test_brute_force_cornell(ctx, cornell, width):
     assert np.array_equal(occ == 1, bp >= 0) or (occ != (bp >= 0)).sum() <= len(bad)
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, "4q"])
 def test_edge_cases(ctx, cornell, width):
     s = _scene(ctx, cornell, width)
     # tmax == t accepted, tmin == t rejected (mesh.cpp:33); floor at y = 0

@@ -4,7 +4,7 @@ Cornell frame."""
 import numpy as np
 import pytest
 
-from conftest import golden
+from conftest import golden, scene_of
 
 pytestmark = pytest.mark.gpu
 
@@ -14,14 +14,13 @@ def _per_sample_agreement(got, ref, rtol=1e-3, atol=1e-4):
     return close.mean(), close
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, "4q"])
 @pytest.mark.parametrize("name", ["cornell", "terrain"])
 def test_crop_per_sample_matches_reference(ctx, cornell, terrain100k, name, width):
-    from banshee_amd import hip
     fx = golden(f"gi_{name}")
     sc = cornell if name == "cornell" else terrain100k
     assert str(fx["digest"]) == sc.digest()
-    s = hip.Scene(ctx, sc, width=width)
+    s = scene_of(ctx, sc, width)
     x0, y0, cw, ch = (int(v) for v in fx["crop"])
     W, H, grid, depth, seed = (int(fx[k]) for k in ("W", "H", "grid", "depth", "seed"))
     S, Q, N, PS = s.render(W, H, grid, depth, seed, crop=(x0, y0, cw, ch), per_sample=True)
