@@ -183,9 +183,7 @@ public:
             if (3 * out.max_depth + 1 <= kMaxStack4)
             {
                 out.width = 4;
-                const bool want = p_.quantized > 0 ||
-                                  (p_.quantized == 0 && out.nodes4.size() * sizeof(GpuNode4) > kAutoQuantizeBytes);
-                out.quantized = want && quantize4(out.nodes4, out.nodesq);
+                out.quantized = p_.quantized >= 0 && quantize4(out.nodes4, out.nodesq);
                 return;
             }
             out.nodes4.clear();

@@ -75,14 +75,11 @@ struct BuildParams
     int bins = 16;
     int threads = 0;
     int width = 4;      // 2: GpuNode, 4: GpuNode4 (the faster layout on the 3M-triangle terrain)
-    // width 4 only: 1 GpuNodeQ, -1 GpuNode4, 0 GpuNodeQ when the GpuNode4 array would take more
-    // than kAutoQuantizeBytes. Quantised nodes halve the node bytes at about 30 more instructions
-    // per visit: on the 3.08M-triangle terrain (112 MB of float nodes, beyond the caches) the
-    // frame traced 0.9 % faster, on the 64-scene batch (mostly small, cache-resident trees)
-    // 2.2 % slower.
+    // width 4 only: 0 (the default) or 1 GpuNodeQ, -1 GpuNode4. Quantised nodes halve the node
+    // bytes at about 20 more instructions per visit; they traced faster on every workload
+    // measured (3.08M-triangle frame +5 %, 64-scene batch +3.6 %, 2M-triangle probe bake +9.6 %).
     int quantized = 0;
 };
-constexpr size_t kAutoQuantizeBytes = size_t(64) << 20;
 
 struct BuildOutput
 {

@@ -228,7 +228,11 @@ __device__ __forceinline__ uint32_t stack_pop(SP stk, int stride, OP ovf, int os
 // and the child is not empty), ref[c] and count cn[c], from a GpuNode4 (kQ false: 8 float4
 // per node) or a GpuNodeQ (kQ true: 4 float4; a plane origin + q 2^e is met at
 // q (2^e / d) + (origin / d - o / d), one conversion and one fma, 2^e built from its
-// exponent byte, so 2^e / d is exact).
+// exponent byte, so 2^e / d is exact). For the quantised node the plane met first on each
+// axis, the lower one when the direction is positive (1 / d keeps the sign of d, which is
+// never 0 here), is chosen per plane word, four children at once; its distance equals the min
+// of the two planes' distances for a box with lo <= hi, since the rounding of fma is monotone.
+// (For the float node choosing the rows costs as many address instructions as it saves.)
 template <bool kQ>
 __device__ __forceinline__ void node4_test(const float4* nodes, uint32_t node, float ix, float iy, float iz, float ox,
                                            float oy, float oz, float tmin, float tcur, float (&tn)[4], bool (&hit)[4],
@@ -249,19 +253,26 @@ __device__ __forceinline__ void node4_test(const float4* nodes, uint32_t node, f
         const float bx = __builtin_fmaf(W0.x, ix, -ox), by = __builtin_fmaf(W0.y, iy, -oy),
                     bz = __builtin_fmaf(W0.z, iz, -oz);
         ref[0] = REF.x; ref[1] = REF.y; ref[2] = REF.z; ref[3] = REF.w;
+        // per axis the entry plane is the lower one when the direction is positive: choosing
+        // the plane words once per node replaces a min and a max per axis and child (for a
+        // box with lo <= hi the same values; an empty child's inverted box is rejected by
+        // its count either way)
+        const uint32_t nx = ix >= 0.f ? W1.x : W1.y, fx = ix >= 0.f ? W1.y : W1.x;
+        const uint32_t ny = iy >= 0.f ? W1.z : W1.w, fy = iy >= 0.f ? W1.w : W1.z;
+        const uint32_t nz = iz >= 0.f ? W2.x : W2.y, fz = iz >= 0.f ? W2.y : W2.x;
 #pragma unroll
         for (int c = 0; c < 4; ++c)
         {
             const int sh = 8 * c;
-            const float x0 = __builtin_fmaf((float)((W1.x >> sh) & 0xffu), sx, bx);
-            const float x1 = __builtin_fmaf((float)((W1.y >> sh) & 0xffu), sx, bx);
-            const float y0 = __builtin_fmaf((float)((W1.z >> sh) & 0xffu), sy, by);
-            const float y1 = __builtin_fmaf((float)((W1.w >> sh) & 0xffu), sy, by);
-            const float z0 = __builtin_fmaf((float)((W2.x >> sh) & 0xffu), sz, bz);
-            const float z1 = __builtin_fmaf((float)((W2.y >> sh) & 0xffu), sz, bz);
+            const float x0 = __builtin_fmaf((float)((nx >> sh) & 0xffu), sx, bx);
+            const float x1 = __builtin_fmaf((float)((fx >> sh) & 0xffu), sx, bx);
+            const float y0 = __builtin_fmaf((float)((ny >> sh) & 0xffu), sy, by);
+            const float y1 = __builtin_fmaf((float)((fy >> sh) & 0xffu), sy, by);
+            const float z0 = __builtin_fmaf((float)((nz >> sh) & 0xffu), sz, bz);
+            const float z1 = __builtin_fmaf((float)((fz >> sh) & 0xffu), sz, bz);
             cn[c] = (int)(int8_t)(uint8_t)((W2.z >> sh) & 0xffu);
-            tn[c] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-            const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tcur));
+            tn[c] = fmaxf(fmaxf(x0, y0), fmaxf(z0, tmin));
+            const float tf = fminf(fminf(x1, y1), fminf(z1, tcur));
             hit[c] = (tn[c] <= tf) && (cn[c] >= 0);
         }
     }

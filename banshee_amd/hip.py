@@ -599,8 +599,8 @@ class Guide:
 
 
 def _quant_flag(quantized):
-    """bsh_bvh_params.quantized: None -> 0 (automatic: quantised past 64 MiB of float nodes),
-    True -> 1, False -> -1."""
+    """bsh_bvh_params.quantized: None -> 0 (the default: quantised 4-wide nodes), True -> 1,
+    False -> -1 (float 4-wide nodes)."""
     return 0 if quantized is None else 1 if quantized else -1
 
 
@@ -636,8 +636,8 @@ class Scene:
     """A committed device scene (World after Commit) built from a banshee_amd.scenes.Scene."""
 
     def __init__(self, ctx: Context, sc, max_leaf=4, sah_bins=16, threads=0, width=0, quantized=None):
-        """quantized: None lets the builder choose (quantised 4-wide nodes past 64 MiB of float
-        nodes), True / False force the quantised / float 4-wide layout."""
+        """quantized: None (the default) or True build quantised 64-B 4-wide nodes, False float
+        128-B 4-wide nodes."""
         self.ctx = ctx
         self.sc = sc
         self.h = c_void_p()

@@ -187,7 +187,7 @@ def training_rate(ctx, rows=1 << 18, steps=30) -> dict:
 
 
 def qflag(v: int):
-    """--bvh-quantized -> hip.Scene(quantized=...): 0 automatic (None), 1 True, -1 False."""
+    """--bvh-quantized -> hip.Scene(quantized=...): 0 the default (None), 1 True, -1 False."""
     return None if v == 0 else v > 0
 
 
@@ -547,8 +547,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the batch and probe-baking figures")
     ap.add_argument("--bvh-width", type=int, default=4, choices=(2, 4), help="BVH node width")
     ap.add_argument("--bvh-quantized", type=int, default=0, choices=(-1, 0, 1),
-                    help="width 4: 1 64-B nodes with 8-bit child boxes, -1 128-B float nodes, 0 the "
-                         "builder's choice (quantised past 64 MiB of float nodes)")
+                    help="width 4: 0 (default) or 1 64-B nodes with 8-bit child boxes, -1 128-B float "
+                         "nodes")
     ap.add_argument("--paths-per-chunk", type=int, default=0,
                     help="paths in flight per wavefront chunk (0 = library default, 2^23)")
     ap.add_argument("--refill-lanes", type=int, default=0,

@@ -121,11 +121,12 @@ typedef struct
     int32_t threads;    /* host build threads, 0 = hardware concurrency */
     int32_t width;      /* node width: 2 (64-B binary nodes) or 4 (128-B 4-wide nodes collapsed from the
                            binary SAH tree); 0 = 4. A tree too deep for the 4-wide stack stays binary. */
-    int32_t quantized;  /* width 4 only: 1 stores the same tree in 64-B nodes with 8-bit child boxes
-                           (conservative, so closest hits are unchanged), -1 in 128-B float nodes,
-                           0 = quantised when the float nodes would take more than 64 MiB. The
-                           layout built is reported as node bytes by bsh_scene_info and
-                           bsh_bvh_build_host (64 for binary or quantised, 128 for float 4-wide). */
+    int32_t quantized;  /* width 4 only: 0 (default) or 1 stores the tree in 64-B nodes with 8-bit
+                           child boxes (conservative, so closest hits are unchanged), -1 in 128-B
+                           float nodes. A scene whose coordinates leave the exponent range the
+                           quantised boxes allow keeps float nodes. The layout built is reported as
+                           node bytes by bsh_scene_info and bsh_bvh_build_host (64 for binary or
+                           quantised, 128 for float 4-wide). */
 } bsh_bvh_params;
 
 /* Frame parameters for bsh_render. The estimator is GiTracer(max_depth)

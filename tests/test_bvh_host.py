@@ -199,9 +199,9 @@ def test_quantized_boxes_contain_float_boxes(scene, offset):
     assert np.all((ohi - olo > 127 * step) | (step / 2 < mag * 2.0 ** -17))
 
 
-def test_automatic_layout_threshold(scene):
-    """quantized=None (the default) keeps float nodes for a tree under 64 MiB of them."""
+def test_default_layout_is_quantized(scene):
+    """quantized=None (the default) builds the quantised 4-wide nodes."""
     tv = tri_verts(scene)
     na, _, wa, _ = hip.bvh_build_host(tv, width=4)
-    nf, _, wf, _ = hip.bvh_build_host(tv, width=4, quantized=False)
-    assert wa == wf == 4 and na.shape == nf.shape and np.array_equal(na.view(np.int32), nf.view(np.int32))
+    nq, _, wq, _ = hip.bvh_build_host(tv, width=4, quantized=True)
+    assert wa == wq == 4 and na.shape == nq.shape and np.array_equal(na.view(np.int32), nq.view(np.int32))

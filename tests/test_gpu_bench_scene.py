@@ -8,8 +8,8 @@ GiTracer/DiTracer, gitracer.cpp:13-140, ditracer.cpp:57-239).
   as closest-hit queries, plus the shadow rays as any-hit queries (262,144 queries). Primitive
   agreement >= 99.99 %, t and barycentrics within 1e-5 relative where they agree, and every
   disagreement adjudicated in FP64 as an edge, grazing or tie case.
-The scene gets the default node layout, which for its 112 MB of float 4-wide nodes is the
-quantised 4-wide one (64-B nodes, 8-bit child boxes).
+The scene gets the default node layout, the quantised 4-wide one (64-B nodes, 8-bit child
+boxes).
 
 * GI: a band of 1920 x 192 pixels at 64 spp, depth 8, rendered with the default launch shape
   (2^23 paths per chunk: three chunks dealt over the two wavefront lanes, 3 trace workgroups per
