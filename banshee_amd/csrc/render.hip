@@ -484,8 +484,9 @@ __device__ __forceinline__ uint32_t pop_top(lu32* stk, gu32* ovf, int ostride, i
 // many new rays with one atomic and hands them to the idle lanes in lane order. Lanes of one
 // wave may hold closest and any-hit rays of different scenes at once. Node visits and
 // accept rules are those of traverse4, so every ray's result is the same. kQ: every scene of
-// the launch has GpuNodeQ nodes (else every one has GpuNode4 nodes).
-template <bool kCount, bool kQ>
+// the launch has GpuNodeQ nodes (else every one has GpuNode4 nodes). kOne: every job traces
+// one scene (R.nodes1), whose arrays then stay in scalar registers.
+template <bool kCount, bool kQ, bool kOne>
 __global__ __launch_bounds__(kBlock, kTraceResidentBlocks) void k_trace_refill(RenderDev R, int L, int buf, int refill,
                                                                                uint32_t claim_rays)
 {
@@ -514,8 +515,8 @@ __global__ __launch_bounds__(kBlock, kTraceResidentBlocks) void k_trace_refill(R
     float key[4] = {0.f, 0.f, 0.f, 0.f};
     uint32_t id[4] = {0u, 0u, 0u, 0u};
     bool sorted = true;
-    const float4* nodes = nullptr;
-    const float4* tris = nullptr;
+    const float4* nodes = kOne ? R.nodes1 : nullptr;
+    const float4* tris = kOne ? R.tris1 : nullptr;
     for (;;)
     {
         // ---- refill the idle lanes (wave-uniform decision) ----
@@ -549,12 +550,7 @@ __global__ __launch_bounds__(kBlock, kTraceResidentBlocks) void k_trace_refill(R
                     const float4 a = gload(any ? R.qa_o[buf] : R.qc_o[buf], k);
                     const float4 b = gload(any ? R.qa_d[buf] : R.qc_d[buf], k);
                     own = (uint32_t)(any ? R.qa_own[buf][k] : R.qc_own[buf][k]);
-                    if (R.nodes1)
-                    {
-                        nodes = R.nodes1;
-                        tris = R.tris1;
-                    }
-                    else
+                    if (!kOne)
                     {
                         const uint32_t job = R.meta[path_of(R, own >> 29, own & ((1u << 29) - 1u))].y >> 24;
                         nodes = R.jobs[job].s.nodes;
@@ -1636,6 +1632,18 @@ static std::vector<Chunk> plan_chunks(const std::vector<JobDev>& jobs, size_t pm
 
 static dim3 grid_for(size_t n) { return dim3((uint32_t)std::max<size_t>(1, (n + kBlock - 1) / kBlock)); }
 
+// k_trace_refill for one node layout, with the single-scene variant when one scene is traced
+template <bool kCount, bool kQ>
+static void launch_refill(bool one, int blocks, hipStream_t st, const RenderDev& R, int b, int refill, uint32_t claim)
+{
+    if (one)
+        hipLaunchKernelGGL((k_trace_refill<kCount, kQ, true>), dim3(blocks), dim3(kBlock), 0, st, R, b, b & 1, refill,
+                           claim);
+    else
+        hipLaunchKernelGGL((k_trace_refill<kCount, kQ, false>), dim3(blocks), dim3(kBlock), 0, st, R, b, b & 1, refill,
+                           claim);
+}
+
 // Enqueues every kernel of a batch on `st`. With T == nullptr no events are recorded
 // (the form captured into a hipGraph).
 // Orders further streams after a first one (fork) and the first after all of them (join).
@@ -1753,18 +1761,16 @@ static bsh_status enqueue_batch(bsh_context* c, const Lane* lanes, int nlanes, c
             }
             if (wide)
             {
-                if (count && qall)
-                    hipLaunchKernelGGL((k_trace_refill<true, true>), dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1,
-                                       refill, claim);
-                else if (count)
-                    hipLaunchKernelGGL((k_trace_refill<true, false>), dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1,
-                                       refill, claim);
+                const bool one = R.nodes1 != nullptr;
+                if (count)
+                {
+                    if (qall) launch_refill<true, true>(one, trace_blocks, st, R, b, refill, claim);
+                    else launch_refill<true, false>(one, trace_blocks, st, R, b, refill, claim);
+                }
                 else if (qall)
-                    hipLaunchKernelGGL((k_trace_refill<false, true>), dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1,
-                                       refill, claim);
+                    launch_refill<false, true>(one, trace_blocks, st, R, b, refill, claim);
                 else
-                    hipLaunchKernelGGL((k_trace_refill<false, false>), dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1,
-                                       refill, claim);
+                    launch_refill<false, false>(one, trace_blocks, st, R, b, refill, claim);
             }
             else if (count)
                 hipLaunchKernelGGL(k_trace_queue<true>, dim3(trace_blocks), dim3(kBlock), 0, st, R, b, b & 1);
