@@ -205,3 +205,24 @@ def test_default_layout_is_quantized(scene):
     na, _, wa, _ = hip.bvh_build_host(tv, width=4)
     nq, _, wq, _ = hip.bvh_build_host(tv, width=4, quantized=True)
     assert wa == wq == 4 and na.shape == nq.shape and np.array_equal(na.view(np.int32), nq.view(np.int32))
+
+
+def test_quantized_layout_falls_back_outside_its_exponent_range(scene):
+    """Coordinates near 1e30 need grid steps past 2^60, where 2^e / d could leave the float
+    range: the builder keeps float nodes (128 B) for them. Coordinates near 1e-18 (grid steps
+    near 2^-77) still quantise, with boxes that contain every float box. (Below about 1e-24
+    the SAH areas underflow and the tree stays binary, so the grid's 2^-100 floor is not
+    reached through a 4-wide build.)"""
+    tv = tri_verts(scene)
+    big = (tv.astype(np.float64) * 1e28).astype(np.float32)
+    nodes, _, w, _ = hip.bvh_build_host(big, width=4)
+    assert w == 4 and nodes.shape[1] == 32
+    tiny = (tv.astype(np.float64) * 1e-20).astype(np.float32)
+    nq, _, wq, _ = hip.bvh_build_host(tiny, width=4)
+    nf, _, _, _ = hip.bvh_build_host(tiny, width=4, quantized=False)
+    assert wq == 4 and nq.shape[1] == 16
+    lo, hi, _, cnt = decode(nf, 4)
+    qlo, qhi, _, qcnt, _, _ = decode_q(nq)
+    live = cnt >= 0
+    assert np.array_equal(cnt, qcnt)
+    assert np.all((qlo <= lo.astype(np.float64))[live]) and np.all((qhi >= hi.astype(np.float64))[live])
