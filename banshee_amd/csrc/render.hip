@@ -1990,7 +1990,8 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     std::vector<Chunk> chunks = plan_chunks(hjobs, pmax);
     // An unguided run that fits one chunk of at least kSplitPaths paths is planned as two, so
     // the two lanes overlap (4096 probes x 1024 rays, depth 6: 1428 -> 1534 Mrays/s; a
-    // 1024^2 4-spp frame unchanged; guided passes lost 2 %, so they keep one chunk).
+    // 1024^2 4-spp frame unchanged; guided passes lost 2 %, and 3 % when re-measured on the
+    // quantised trace kernel, so they keep one chunk).
     if (chunks.size() == 1 && chunks[0].P >= kSplitPaths && !(gr && gr->guide))
     {
         // whole units per chunk: a half that splits a unit would leave a third chunk, so try
