@@ -284,7 +284,7 @@ def batch_workload(args) -> dict:
     scenes, jobs = [], []
     for c in cfgs:
         sc = terrain_for_tris(c["tris"], seed=c["seed"], width=c["res"], height=c["res"])
-        s = hip.Scene(ctx, sc, width=args.bvh_width, quantized=qflag(args.bvh_quantized))
+        s = hip.Scene(ctx, sc, width=args.bvh_width, quantized=qflag(args.bvh_quantized), max_leaf=args.max_leaf)
         scenes.append(s)
         jobs.append(hip.BatchJob(s, "render", width=c["res"], height=c["res"], img_grid=c["grid"],
                                  max_depth=c["depth"], seed=c["seed"]))
@@ -334,7 +334,7 @@ def probe_workload(args) -> dict:
     from banshee_amd.scenes import probe_grid, terrain_for_tris
     ctx = make_context(args)
     sc = terrain_for_tris(2_000_000, seed=7)
-    scene = hip.Scene(ctx, sc, width=args.bvh_width, quantized=qflag(args.bvh_quantized))
+    scene = hip.Scene(ctx, sc, width=args.bvh_width, quantized=qflag(args.bvh_quantized), max_leaf=args.max_leaf)
     probes = probe_grid(sc, 16)
     coeffs = ctx.alloc((len(probes), 9, 3), "float32")
     for _ in range(args.warmup):
@@ -518,7 +518,7 @@ def probes_line(ctx, args) -> dict:
     from banshee_amd import hip
     from banshee_amd.scenes import probe_grid, terrain_for_tris
     sc = terrain_for_tris(2_000_000, seed=7)
-    scene = hip.Scene(ctx, sc, width=args.bvh_width, quantized=qflag(args.bvh_quantized))
+    scene = hip.Scene(ctx, sc, width=args.bvh_width, quantized=qflag(args.bvh_quantized), max_leaf=args.max_leaf)
     probes = probe_grid(sc, 16)
     coeffs = ctx.alloc((len(probes), 9, 3), "float32")
     scene.bake_sh(probes, 1024, 2, 6, 1, coeffs=coeffs, download=False)
@@ -546,6 +546,7 @@ def main():
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain time-to-RMSE comparison")
     ap.add_argument("--no-extra", action="store_true", help="skip the batch and probe-baking figures")
     ap.add_argument("--bvh-width", type=int, default=4, choices=(2, 4), help="BVH node width")
+    ap.add_argument("--max-leaf", type=int, default=4, help="triangles per BVH leaf, 1..8")
     ap.add_argument("--bvh-quantized", type=int, default=0, choices=(-1, 0, 1),
                     help="width 4: 0 (default) or 1 64-B nodes with 8-bit child boxes, -1 128-B float "
                          "nodes")
@@ -589,7 +590,7 @@ def main():
     t0 = time.time()
     sc = terrain_scene(width=W, height=H)
     ctx = make_context(args)
-    scene = hip.Scene(ctx, sc, width=args.bvh_width, quantized=qflag(args.bvh_quantized))
+    scene = hip.Scene(ctx, sc, width=args.bvh_width, quantized=qflag(args.bvh_quantized), max_leaf=args.max_leaf)
     info = scene.info()
     setup_s = time.time() - t0
     accum = ctx.alloc((H, W, 8), "float32")
