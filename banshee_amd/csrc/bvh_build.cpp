@@ -67,7 +67,7 @@ bool quantize4(const std::vector<GpuNode4>& in, std::vector<GpuNodeQ>& out)
             if (e < -100 || e > 60) return false;
             const double step = std::ldexp(1.0, e);
             q.origin[a] = o;
-            q.exps |= (uint32_t)(e + 127) << (8 * a);
+            q.exps |= (uint32_t)(uint8_t)(int8_t)e << (8 * a);
             uint32_t wl = 0, wh = 0;
             for (int c = 0; c < 4; ++c)
             {

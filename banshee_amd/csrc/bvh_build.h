@@ -42,7 +42,7 @@ static_assert(sizeof(GpuNode4) == 128, "node is 128 B");
 // Quantised 4-wide node, 64 B = 4 x 16 B, the same tree as GpuNode4 with each child box stored
 // as 8-bit planes on a per-node, per-axis power-of-two grid (after Ylitie, Karras and Laine,
 // "Efficient incoherent ray traversal on GPUs through compressed wide BVHs", HPG 2017):
-//   w0 = (origin.x, origin.y, origin.z, exponent bytes: e_x + 127 | e_y + 127 << 8 | e_z + 127 << 16)
+//   w0 = (origin.x, origin.y, origin.z, exponents as signed bytes: e_x | e_y << 8 | e_z << 16)
 //   w1 = (qlo.x, qhi.x, qlo.y, qhi.y), each 4 bytes, byte c for child c
 //   w2 = (qlo.z, qhi.z, counts as 4 signed bytes, 0)
 //   w3 = refs (as in GpuNode4)
@@ -50,6 +50,7 @@ static_assert(sizeof(GpuNode4) == 128, "node is 128 B");
 // outward by at least half a grid step (a lower plane of 0 is the node's own padded bound), and
 // the grid step is at least 2^-17 of the scene's largest coordinate, so the traversal's rounding
 // of these planes stays inside the margin and the boxes contain the float boxes they replace.
+// e stays within [-100, 60].
 struct GpuNodeQ
 {
     float origin[3];

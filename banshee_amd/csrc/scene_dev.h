@@ -227,8 +227,8 @@ __device__ __forceinline__ uint32_t stack_pop(SP stk, int stride, OP ovf, int os
 // One 4-wide node's slab tests: entry distance tn[c], hit[c] (the box overlaps [tmin, tcur]
 // and the child is not empty), ref[c] and count cn[c], from a GpuNode4 (kQ false: 8 float4
 // per node) or a GpuNodeQ (kQ true: 4 float4; a plane origin + q 2^e is met at
-// q (2^e / d) + (origin / d - o / d), one conversion and one fma, 2^e built from its
-// exponent byte, so 2^e / d is exact). For the quantised node the plane met first on each
+// q (2^e / d) + (origin / d - o / d), one conversion and one fma; 2^e / d is ldexp(1 / d, e),
+// exact unless it leaves the normal range). For the quantised node the plane met first on each
 // axis, the lower one when the direction is positive (1 / d keeps the sign of d, which is
 // never 0 here), is chosen per plane word, four children at once; its distance equals the min
 // of the two planes' distances for a box with lo <= hi, since the rounding of fma is monotone.
@@ -246,10 +246,12 @@ __device__ __forceinline__ void node4_test(const float4* nodes, uint32_t node, f
         const uint4 W1 = make_uint4((uint32_t)I1.x, (uint32_t)I1.y, (uint32_t)I1.z, (uint32_t)I1.w);
         const uint4 W2 = make_uint4((uint32_t)I2.x, (uint32_t)I2.y, (uint32_t)I2.z, (uint32_t)I2.w);
         const int4 REF = gload_i(nodes, nb + 3);
+        // 2^e / d = ldexp(1 / d, e) from the signed exponent bytes (one extract and one
+        // v_ldexp per axis; equal to the product with 2^e in every case)
         const uint32_t ex = __float_as_uint(W0.w);
-        const float sx = __uint_as_float((ex & 0xffu) << 23) * ix;
-        const float sy = __uint_as_float(((ex >> 8) & 0xffu) << 23) * iy;
-        const float sz = __uint_as_float(((ex >> 16) & 0xffu) << 23) * iz;
+        const float sx = __builtin_amdgcn_ldexpf(ix, (int)(int8_t)(ex & 0xffu));
+        const float sy = __builtin_amdgcn_ldexpf(iy, (int)(int8_t)((ex >> 8) & 0xffu));
+        const float sz = __builtin_amdgcn_ldexpf(iz, (int)(int8_t)((ex >> 16) & 0xffu));
         const float bx = __builtin_fmaf(W0.x, ix, -ox), by = __builtin_fmaf(W0.y, iy, -oy),
                     bz = __builtin_fmaf(W0.z, iz, -oz);
         ref[0] = REF.x; ref[1] = REF.y; ref[2] = REF.z; ref[3] = REF.w;

@@ -26,7 +26,8 @@ def decode_q(nodes):
     """GpuNodeQ rows (16 words): lo [4,3], hi [4,3], ref [4], cnt [4], origin [3], step [3], in FP64."""
     u = nodes.view(np.uint32)
     origin = nodes[:, 0:3].astype(np.float64)
-    step = np.stack([2.0 ** (((u[:, 3] >> (8 * a)) & 255).astype(np.int64) - 127) for a in range(3)], -1)
+    step = np.stack([2.0 ** ((u[:, 3] >> (8 * a)) & 255).astype(np.uint8).view(np.int8).astype(np.int64)
+                     for a in range(3)], -1)
     def byte(w, c):
         return ((w >> (8 * c)) & 255).astype(np.float64)
     lo = np.stack([np.stack([origin[:, a] + byte(u[:, 4 + 2 * a], c) * step[:, a] for a in range(3)], -1)
