@@ -183,7 +183,10 @@ __device__ __forceinline__ void dense_t(const h4 (&x)[kT][K / 16], const _Float1
 }
 
 constexpr int kFwdWaves = 8;
-constexpr int kFwdTiles = 2;   // 16-row tiles per wave per step (weight reads shared; 4 costs occupancy)
+// 16-row tiles per wave per step, sharing each weight read: 4 halves the LDS reads per FLOP
+// but needs 124 VGPRs (4 waves per SIMD instead of 6), and the guided run's inference took
+// 75.0 ms against 72.1 with 2
+constexpr int kFwdTiles = 2;
 
 // Fused forward: rows of raw inputs (`stride` float4 per row, the first two holding
 // v0..v6) -> the network's outputs. Weights and biases are staged in LDS once per block; each
