@@ -227,3 +227,17 @@ def test_quantized_layout_falls_back_outside_its_exponent_range(scene):
     live = cnt >= 0
     assert np.array_equal(cnt, qcnt)
     assert np.all((qlo <= lo.astype(np.float64))[live]) and np.all((qhi >= hi.astype(np.float64))[live])
+
+
+def test_invalid_quantized_value_is_rejected(scene):
+    """bsh_bvh_params.quantized takes -1, 0 or 1; anything else is BSH_ERR_INVALID with a message."""
+    import ctypes
+    tv = np.ascontiguousarray(tri_verts(scene))
+    n, w, d, nb = ctypes.c_size_t(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    for bad in (2, -2):
+        bp = hip.BvhParams(4, 16, 0, 4, bad)
+        st = hip.lib().bsh_bvh_build_host(tv.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(tv.shape[0]),
+                                          ctypes.byref(bp), None, ctypes.c_size_t(0), None, ctypes.byref(n),
+                                          ctypes.byref(w), ctypes.byref(d), ctypes.byref(nb))
+        assert st != 0
+        assert b"quantized must be -1, 0 or 1" in hip.lib().bsh_last_error()
