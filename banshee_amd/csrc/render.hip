@@ -590,11 +590,14 @@ __global__ __launch_bounds__(kBlock, kTraceResidentBlocks) void k_trace_refill(R
         }
         if (__ballot(active) == 0ull) break;
         if (!active) continue;
-        // ---- one step: a triangle of the pending leaves, or a node visit ----
+        // ---- one step: a triangle of the pending leaves, then, once a lane's leaves are
+        // done, its node's internal children and the next node's visit ----
         // A visit's leaf triangles are tested one per iteration, before its internal children
         // are culled against the updated tcur, sorted and pushed (the order traverse4 tests and
-        // culls in), so a wave does either kind of work per lane per iteration instead of
-        // looping over its busiest lane's triangles while the other lanes wait.
+        // culls in); a lane whose last leaf triangle, or whose stack pop, settles its next
+        // node visits that node in the same iteration. A wave so runs a triangle test, a sort
+        // and a visit per iteration over whichever of its lanes need them, instead of looping
+        // over its busiest lane's triangles while the other lanes wait.
         bool done = false;
         if (lmask)
         {
@@ -620,28 +623,6 @@ __global__ __launch_bounds__(kBlock, kTraceResidentBlocks) void k_trace_refill(R
                 lmask &= lmask - 1u;
                 loff = 0;
             }
-        }
-        else
-        {
-            if (kCount) (any ? vis_nodes_any : vis_nodes)++;
-            int ref[4], cn[4];
-            float tn[4];
-            bool hitc[4];
-            node4_test<kQ>(nodes, node, ix, iy, iz, ox, oy, oz, tmin, tcur, tn, hitc, ref, cn);
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-            {
-                key[c] = (hitc[c] && cn[c] == 0) ? tn[c] : kInf;
-                id[c] = (uint32_t)ref[c];
-            }
-            // the hit leaf children, tested from the next iteration on
-            lmask = 0u;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) lmask |= (hitc[c] && cn[c] > 0) ? 1u << c : 0u;
-            loff = 0;
-            lr0 = ref[0]; lr1 = ref[1]; lr2 = ref[2]; lr3 = ref[3];
-            lc0 = cn[0]; lc1 = cn[1]; lc2 = cn[2]; lc3 = cn[3];
-            sorted = false;
         }
         if (!done && !sorted && lmask == 0u)
         {
@@ -676,6 +657,30 @@ __global__ __launch_bounds__(kBlock, kTraceResidentBlocks) void k_trace_refill(R
                 node = id[0];
             }
             sorted = true;
+        }
+        if (!done && sorted && lmask == 0u)
+        {
+            // visit the next node; its internal children are sorted in the next iteration,
+            // after its leaf triangles
+            if (kCount) (any ? vis_nodes_any : vis_nodes)++;
+            int ref[4], cn[4];
+            float tn[4];
+            bool hitc[4];
+            node4_test<kQ>(nodes, node, ix, iy, iz, ox, oy, oz, tmin, tcur, tn, hitc, ref, cn);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+            {
+                key[c] = (hitc[c] && cn[c] == 0) ? tn[c] : kInf;
+                id[c] = (uint32_t)ref[c];
+            }
+            // the hit leaf children, tested from the next iteration on
+            lmask = 0u;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) lmask |= (hitc[c] && cn[c] > 0) ? 1u << c : 0u;
+            loff = 0;
+            lr0 = ref[0]; lr1 = ref[1]; lr2 = ref[2]; lr3 = ref[3];
+            lc0 = cn[0]; lc1 = cn[1]; lc2 = cn[2]; lc3 = cn[3];
+            sorted = false;
         }
         if (done)
         {
