@@ -479,7 +479,8 @@ __device__ __forceinline__ uint32_t pop_top(lu32* stk, gu32* ovf, int ostride, i
 
 // 4-wide traversal with per-lane refill. The closest-hit queue and the any-hit queue form one
 // index range [0, nc + na) drawn by one work counter. Every lane holds its own ray state and
-// advances it by one node visit per loop iteration; a finished lane idles until `refill` or
+// advances it by one step per loop iteration (a leaf triangle, the sort of a visit's children
+// and the next visit, as far as each applies); a finished lane idles until `refill` or
 // more lanes of its wave are idle (or the whole wave is), when the wave claims exactly that
 // many new rays with one atomic and hands them to the idle lanes in lane order. Lanes of one
 // wave may hold closest and any-hit rays of different scenes at once. Node visits and
