@@ -65,7 +65,9 @@ constexpr int kTraceBlocksPerLane = 3;   // the default when two lanes share mor
 // 80 VGPRs), so both lanes' default grids (3 per CU each) are resident at once whatever the
 // node layout (the quantised layout's decode would otherwise take 89).
 constexpr int kTraceResidentBlocks = 2 * kTraceBlocksPerLane;
-// The shade kernel is held to 128 VGPRs: 4 waves per SIMD (at 129 it would run 3).
+// The shade kernel is held to 128 VGPRs: 4 waves per SIMD (at 129 it would run 3). Holding the
+// plain variant to 5 or 6 waves (96 or 80 VGPRs, 24 or 50 of them spilled) made the 1080p
+// terrain frame 1.1 and 1.5 % slower.
 constexpr int kShadeWavesPerSimd = 4;
 // Statistics buffer: 4 counters, then the guided-row partials, kRowStride apart (128 B), then per
 // bounce launch b: closest rays, any-hit rays, and in counting mode node visits and triangle tests
