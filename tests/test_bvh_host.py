@@ -168,14 +168,20 @@ def test_four_wide_collapses_levels(scene):
     assert d4 < d2
 
 
-@pytest.mark.parametrize("offset", [0.0, 3.0e4])
-def test_quantized_boxes_contain_float_boxes(scene, offset):
+@pytest.mark.parametrize("flat,offset", [(False, 0.0), (False, 3.0e4), (True, 0.0)])
+def test_quantized_boxes_contain_float_boxes(scene, flat, offset):
     """The quantised tree is the float 4-wide tree with each child box grown outward: same refs
     and counts, every plane at least half a grid step outside the float plane (or, for a lower
     plane, on the node's origin, which is the lowest child's own float plane), and grid steps no
     finer than 2^-17 of the scene's largest coordinate (the offset scene puts the terrain far
-    from the origin, where that floor decides the grid of the small nodes)."""
+    from the origin, where that floor decides the grid of the small nodes). The flat case lays
+    the terrain on a plane (its thinnest axis set to 1.5): along that axis every float box is
+    then only its builder padding thick (a relative 2^-20 each way), far below the floor, which
+    then sets the grid there."""
     tv = tri_verts(scene) + np.float32(offset)
+    if flat:
+        v = tv.reshape(-1, 3)
+        v[:, np.argmin(np.ptp(v, axis=0))] = 1.5
     n4, o4, w4, _ = hip.bvh_build_host(tv, width=4, quantized=False)
     nq, oq, wq, _ = hip.bvh_build_host(tv, width=4, quantized=True)
     assert w4 == wq == 4 and nq.shape == (n4.shape[0], 16)
