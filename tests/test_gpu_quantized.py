@@ -72,3 +72,46 @@ def test_quantized_and_float_nodes_find_the_same_hits(layouts, kind):
     np.testing.assert_array_equal(oq, of)
     # an occluded ray is one with a closest hit in its interval, and the other way round
     assert np.array_equal(oq == 1, pq >= 0)
+
+
+def test_flat_scene_layouts_agree(ctx):
+    """A terrain laid on a plane (its thinnest axis set to 1.5): every float box is only its
+    builder padding thick along that axis and the quantised grid there is the floor step.
+    Rays crossing the plane must find the same hits in the quantised, the float 4-wide and the
+    binary tree (disagreements adjudicated in FP64, as above); rays lying in the plane have
+    det = 0 in Moller-Trumbore, so every comparison sees NaN and no layout reports a hit."""
+    from banshee_amd.scenes import terrain_for_tris
+    sc = terrain_for_tris(4000, seed=9, width=64, height=64)
+    v = np.array(sc.verts, np.float32, copy=True)
+    ax = int(np.argmin(np.ptp(v, axis=0)))
+    v[:, ax] = 1.5
+    sc.verts = v
+    layouts = {L: scene_of(ctx, sc, L) for L in ("4q", 4, 2)}
+    rng = np.random.Generator(np.random.PCG64(31))
+    n, m = 1 << 18, 1 << 16
+    lo, hi = v.min(0), v.max(0)
+    o = rng.uniform(lo, hi, (n, 3))
+    o[:, ax] = 1.5 + rng.choice([-1.0, 1.0], n) * rng.uniform(0.5, 3.0, n)
+    d = rng.normal(size=(n, 3))
+    o[:m, ax] = 1.5
+    d[:m, ax] = 0.0
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, np.full((n, 1), 1e-3), d, np.full((n, 1), 1e4)], 1).astype(np.float32)
+    assert np.all(rays[:m, 4 + ax] == 0.0) and np.all(rays[:m, ax] == 1.5)
+    res = {L: s.intersect(rays) for L, s in layouts.items()}
+    pq, tq = res["4q"]
+    for L in ("4q", 4, 2):
+        assert np.all(res[L][0][:m] < 0), L
+    assert (pq[m:] >= 0).mean() > 0.2
+    for L in (4, 2):
+        p, t = res[L]
+        same = pq == p
+        print(f"4q vs {L}: {(~same).sum()} of {n} disagree")
+        np.testing.assert_array_equal(tq[same & (p >= 0)], t[same & (p >= 0)])
+        bad = np.nonzero(~same)[0]
+        assert len(bad) <= 1e-3 * n
+        verdicts = [classify(sc, rays[i], int(pq[i]), int(p[i])) for i in bad]
+        assert "fail" not in verdicts, list(zip(bad, verdicts))
+        oq, _ = layouts["4q"].intersect(rays, any_hit=True)
+        of, _ = layouts[L].intersect(rays, any_hit=True)
+        np.testing.assert_array_equal(oq, of)
