@@ -1,6 +1,7 @@
 """The quantised 4-wide BVH against the float 4-wide BVH of the same tree, on the benchmarked
 3.08M-triangle terrain scene (GPU): 2M incoherent rays (origins uniform in the scene's box,
-directions uniform on the sphere, tmin 0.01, tmax 1e4) and 1M camera rays.
+directions uniform on the sphere, tmin 0.01, tmax 1e4) and 1M camera rays; and all three
+layouts on a terrain laid flat on a plane.
 
 Both layouts run the same triangle arithmetic, and every quantised child box contains the
 float box it replaces, so a closest hit can differ only where the two traversal orders meet two
