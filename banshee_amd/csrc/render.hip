@@ -60,7 +60,7 @@ constexpr int kTraceLds = 16;
 constexpr int kRecVerts = 4, kRecStride = 5 * kRecVerts;
 static_assert(kGuideSeg == kBlock, "guide row segments are the blocks of k_guide_inputs");
 constexpr int kMaxTraceBlocksPerCU = 5;
-constexpr int kTraceBlocksPerLane = 3;   // the default when two lanes share more than two chunks
+constexpr int kTraceBlocksPerLane = 3;   // the default when two lanes run
 // k_trace_refill's launch bound: registers for 6 workgroups per CU (6 waves per SIMD, at most
 // 80 VGPRs), so both lanes' default grids (3 per CU each) are resident at once whatever the
 // node layout (the quantised layout's decode would otherwise take 89).
@@ -1999,7 +1999,8 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     // An unguided run that fits one chunk of at least kSplitPaths paths is planned as two, so
     // the two lanes overlap (4096 probes x 1024 rays, depth 6: 1428 -> 1534 Mrays/s; a
     // 1024^2 4-spp frame unchanged; guided passes lost 2 %, and 3 % when re-measured on the
-    // quantised trace kernel, so they keep one chunk).
+    // quantised trace kernel, and were unchanged with 3 trace workgroups per lane, so they keep
+    // one chunk).
     if (chunks.size() == 1 && chunks[0].P >= kSplitPaths && !(gr && gr->guide))
     {
         // whole units per chunk: a half that splits a unit would leave a third chunk, so try
@@ -2029,12 +2030,12 @@ bsh_status run_jobs(bsh_context* c, std::vector<JobDev>& hjobs, bool count, cons
     // Two lanes when the run has chunks to alternate and nothing ties the chunks to one order
     // (training records take a running offset).
     const int nlanes = record || tune.lanes == 1 ? 1 : (int)std::min<size_t>(bsh_context::kLanes, chunks.size());
-    // persistent trace workgroups per CU: all of them on one lane, or when two lanes hold one
-    // chunk each and run side by side from start to end (a 4096-probe bake: 5 per CU 1525
-    // Mrays/s, 3 1477); with more chunks each lane's grid leaves room for the other lane's
-    // kernels in its drain (1080p frame, 16 chunks: 3 per CU 2515, 4 2483, 5 2468, 2 2415)
-    if (tune.trace_blocks_per_cu <= 0)
-        tune.trace_blocks_per_cu = nlanes > 1 && chunks.size() > 2 ? kTraceBlocksPerLane : kMaxTraceBlocksPerCU;
+    // persistent trace workgroups per CU: all of them on one lane; on two lanes each lane's
+    // grid leaves room for the other lane's, which the trace kernel's register bound keeps
+    // resident beside it (1080p frame, 16 chunks: 3 per CU 2515 Mrays/s, 4 2483, 5 2468, 2 2415;
+    // 4096-probe bake, two chunks side by side: 3 per CU 2302-2337, 4 2244-2250, 5 2232, 2
+    // 2084-2091)
+    if (tune.trace_blocks_per_cu <= 0) tune.trace_blocks_per_cu = nlanes > 1 ? kTraceBlocksPerLane : kMaxTraceBlocksPerCU;
     for (int k = 0; k + 1 < nlanes; ++k)
         if (!c->lane_stream[k]) BSH_HIP(hipStreamCreateWithFlags(&c->lane_stream[k], hipStreamNonBlocking));
     bool probe_dir = false;

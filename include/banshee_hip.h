@@ -389,7 +389,7 @@ bsh_status bsh_sh_irradiance(bsh_context* ctx, int lmax, const float* coeffs, co
 /* ---------------- batching (no reference counterpart) ---------------- */
 /* Launch-shape parameters of the wavefront: paths in flight per chunk (queue capacity)
    and persistent trace workgroups per CU (1..5; 0 = automatic: 3 when the run's chunks are
-   dealt over two wavefront lanes and there are more than two, else 5); use_graph captures
+   dealt over two wavefront lanes, else 5); use_graph captures
    the whole batch as one hipGraph and launches it once. refill_lanes: on 4-wide BVHs each trace lane holds its own
    ray and a wave claims new rays once this many of its 64 lanes are idle (1..64; 0 = the
    default, 56); -1 traces in whole 64-ray batches per wave instead (also the form used when
