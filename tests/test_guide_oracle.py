@@ -20,6 +20,13 @@ def sphere_grid(n_theta=800, n_phi=1600):
     return w, dA
 
 
+def grid_pdf(o_row, w, chunk=1 << 15):
+    """The mixture pdf of one output row at every grid direction, evaluated one slice of the
+    grid at a time (the row broadcast, not repeated): the whole grid at once builds
+    temporaries of several GB."""
+    return np.concatenate([G.mixture_pdf(o_row[None], wc[None])[0] for wc in np.split(w, range(chunk, len(w), chunk))])
+
+
 def random_outputs(rng, n, lk_range=(-1.5, 5.0)):
     o = np.zeros((n, 48))
     o[:, :8] = rng.normal(0, 1.0, (n, 8))
@@ -42,7 +49,7 @@ def test_mixture_normalises():
     w, dA = sphere_grid()
     o = random_outputs(rng, 4, (-1.5, 3.5))
     for r in range(4):
-        q = G.mixture_pdf(np.repeat(o[r:r + 1], w.shape[0], 0), w)
+        q = grid_pdf(o[r], w)
         assert abs(np.sum(q * dA) - 1) < 1e-4
 
 
@@ -62,7 +69,7 @@ def test_sampler_matches_density():
 
     counts = np.bincount(bin_of(d), minlength=48)
     gw, dA = sphere_grid()
-    q = G.mixture_pdf(np.repeat(o[None], gw.shape[0], 0), gw)
+    q = grid_pdf(o, gw)
     expect = np.bincount(bin_of(gw), weights=q * dA, minlength=48) * n
     chi2 = np.sum((counts - expect) ** 2 / expect)
     assert chi2 < 90   # 47 dof: p(chi2 > 90) < 1e-4
